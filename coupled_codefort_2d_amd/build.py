@@ -1,0 +1,56 @@
+"""Build recipe of libfugu2d_hip.so (gfx950) -- explicit hipcc, in-tree output.
+
+FP64 parity needs -ffp-contract=off (a*b+c stays two IEEE roundings, as in the
+reference arithmetic).  Packed FP32 instructions are disabled for all HIP code.
+"""
+from __future__ import annotations
+
+import os
+import shutil
+import subprocess
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+CSRC = os.path.join(HERE, "csrc")
+ROOT = os.path.dirname(HERE)
+INCLUDE = os.path.join(ROOT, "include")
+LIB = os.path.join(HERE, "libfugu2d_hip.so")
+SOURCES = [os.path.join(CSRC, "fugu2d_api.hip")]
+HEADERS = [os.path.join(CSRC, f) for f in ("fugu2d_common.hpp", "fugu2d_kernels.hpp")] + \
+          [os.path.join(INCLUDE, "fugu2d.h")]
+ARCH = os.environ.get("FUGU_OFFLOAD_ARCH", "gfx950")
+
+
+def hipcc() -> str:
+    for cand in (os.environ.get("HIPCC"), "/opt/rocm/bin/hipcc", shutil.which("hipcc")):
+        if cand and os.path.exists(cand):
+            return cand
+    raise RuntimeError("hipcc not found")
+
+
+def command(out: str = LIB) -> list[str]:
+    return [hipcc(), f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-fPIC", "-shared",
+            "-ffp-contract=off", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+            "-Wall", "-Wno-unused-function", "-Wno-unused-variable",
+            "-I", INCLUDE, "-I", CSRC, *SOURCES, "-o", out]
+
+
+def build(verbose: bool = True) -> str:
+    tmp = LIB + ".tmp"
+    cmd = command(tmp)
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    subprocess.run(cmd, check=True)
+    os.replace(tmp, LIB)
+    return LIB
+
+
+def needs_build() -> bool:
+    if not os.path.exists(LIB):
+        return True
+    t = os.path.getmtime(LIB)
+    return any(os.path.getmtime(f) > t for f in SOURCES + HEADERS)
+
+
+if __name__ == "__main__":
+    build(verbose="-q" not in sys.argv)

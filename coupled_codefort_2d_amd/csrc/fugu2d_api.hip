@@ -1,0 +1,1214 @@
+// fugu2d_api.hip -- context, C ABI and host-side orchestration of the FUGU-2D hot path.
+//
+// One context = one device-resident domain on one GPU with one HIP stream.  Solver
+// scalars live in device memory; the host reads back only the Krylov stop flags (one
+// 12-byte copy per Krylov iteration) and the scalars a caller asks for.
+#include "fugu2d.h"
+#include "fugu2d_kernels.hpp"
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <stdexcept>
+#include <string>
+#include <utility>
+#include <vector>
+
+using namespace fugu;
+
+namespace {
+
+thread_local std::string g_err;
+
+struct Error : std::runtime_error {
+  explicit Error(const std::string& m) : std::runtime_error(m) {}
+};
+
+#define HIPCHK(x)                                                                      \
+  do {                                                                                 \
+    hipError_t e_ = (x);                                                               \
+    if (e_ != hipSuccess) throw Error(std::string(#x) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+#define LAUNCH(kern, grid, block, stream, ...)                  \
+  do {                                                          \
+    hipLaunchKernelGGL(kern, grid, block, 0, stream, __VA_ARGS__); \
+    HIPCHK(hipGetLastError());                                  \
+  } while (0)
+
+// Per-precision solver workspace.  Triplets are contiguous [u | v | p].
+template <typename T>
+struct Work {
+  T *X = nullptr, *B = nullptr, *bh = nullptr, *r = nullptr, *rt = nullptr, *uu = nullptr;
+  T *vv = nullptr, *ww = nullptr, *ss = nullptr, *tt = nullptr, *z = nullptr, *q = nullptr;
+  // properties (aliases of the context's FP64 fields for T = double)
+  T *rho_c = nullptr, *rho_u = nullptr, *rho_v = nullptr, *kap = nullptr, *zeta = nullptr, *eta = nullptr;
+  T *uadv = nullptr, *vadv = nullptr, *penu = nullptr, *penv = nullptr, *mdu = nullptr, *mdv = nullptr;
+  // preconditioner
+  T *phi = nullptr, *pr = nullptr, *pz = nullptr, *pp0 = nullptr, *pp1 = nullptr, *Ap = nullptr, *dinv = nullptr;
+  T *cx = nullptr, *cy = nullptr, *zuw = nullptr, *ju0 = nullptr, *ju1 = nullptr, *zvw = nullptr, *jv0 = nullptr, *jv1 = nullptr;
+  bool ready = false;
+};
+
+}  // namespace
+
+struct fugu_ctx {
+  Grid g;
+  fugu_params prm;
+  hipStream_t s = nullptr;
+  hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+  std::vector<void*> allocs;
+  double* sc = nullptr;
+  int* fl = nullptr;
+  double* partials = nullptr;
+  unsigned* ticket = nullptr;
+  double* h_sc = nullptr;
+  int* h_fl = nullptr;
+  size_t ntrip = 0;
+  // resident state: (u, v, p) is one contiguous triplet
+  double *X = nullptr, *u = nullptr, *v = nullptr, *p = nullptr;
+  double *c = nullptr, *u_prev = nullptr, *v_prev = nullptr, *unm1 = nullptr, *vnm1 = nullptr;
+  double *rho_c = nullptr, *rho_u = nullptr, *rho_v = nullptr, *kap = nullptr, *zeta = nullptr, *eta = nullptr;
+  double *kcurv = nullptr, *fu = nullptr, *fv = nullptr, *phis_u = nullptr, *phis_v = nullptr;
+  bool has_solid = false;
+  // step temporaries
+  double *uadv = nullptr, *vadv = nullptr, *cnew = nullptr, *csA = nullptr, *csB = nullptr, *cs = nullptr;
+  double *nxu = nullptr, *nyv = nullptr, *fx = nullptr, *fy = nullptr, *mdu = nullptr, *mdv = nullptr;
+  double *penu = nullptr, *penv = nullptr, *Bst = nullptr;
+  // fine-API staging
+  double *su[4] = {}, *sv[4] = {}, *sp[4] = {};
+  Work<double> wd;
+  Work<float> wf;
+  float *f_penu = nullptr, *f_penv = nullptr, *f_mdu = nullptr, *f_mdv = nullptr;
+  int pcg_calls = 0;
+};
+
+namespace {
+
+template <typename T>
+T* dalloc(fugu_ctx* c, size_t n) {
+  void* p = nullptr;
+  HIPCHK(hipMalloc(&p, n * sizeof(T) + 256));
+  HIPCHK(hipMemsetAsync(p, 0, n * sizeof(T) + 256, c->s));
+  c->allocs.push_back(p);
+  return (T*)p;
+}
+
+inline unsigned nblk(size_t n) {
+  size_t b = (n + 255) / 256;
+  if (b > 8192) b = 8192;
+  if (b < 1) b = 1;
+  return (unsigned)b;
+}
+
+void set_slot(fugu_ctx* c, int slot, double v) {
+  HIPCHK(hipMemcpyAsync(c->sc + slot, &v, sizeof(double), hipMemcpyHostToDevice, c->s));
+  HIPCHK(hipStreamSynchronize(c->s));  // v lives on the host stack
+}
+void reset_flags(fugu_ctx* c) { HIPCHK(hipMemsetAsync(c->fl, 0, F_COUNT * sizeof(int), c->s)); }
+void read_flags(fugu_ctx* c) {
+  HIPCHK(hipMemcpyAsync(c->h_fl, c->fl, F_COUNT * sizeof(int), hipMemcpyDeviceToHost, c->s));
+  HIPCHK(hipStreamSynchronize(c->s));
+}
+void read_slots(fugu_ctx* c) {
+  HIPCHK(hipMemcpyAsync(c->h_sc, c->sc, S_COUNT * sizeof(double), hipMemcpyDeviceToHost, c->s));
+  HIPCHK(hipStreamSynchronize(c->s));
+}
+
+// host/device <-> device staging
+void in(fugu_ctx* c, double* dst, const double* src, size_t n) {
+  if (src) HIPCHK(hipMemcpyAsync(dst, src, n * sizeof(double), hipMemcpyDefault, c->s));
+}
+void out(fugu_ctx* c, double* dst, const double* src, size_t n) {
+  if (dst) HIPCHK(hipMemcpyAsync(dst, src, n * sizeof(double), hipMemcpyDefault, c->s));
+}
+void sync(fugu_ctx* c) { HIPCHK(hipStreamSynchronize(c->s)); }
+
+// ---------------------------------------------------------------------------------
+template <typename T>
+void halo(fugu_ctx* c, T* a, int kind, const int* skip = nullptr) {
+  if (!c->g.per_x && !c->g.per_y) return;
+  LAUNCH((k_halo<T>), grid_union(c->g), dim3(TX, TY), c->s, c->g, a, kind, skip);
+}
+template <typename T>
+void halo_trip(fugu_ctx* c, T* X, const int* skip = nullptr) {
+  halo(c, X, 0, skip);
+  halo(c, X + c->g.nu, 1, skip);
+  halo(c, X + c->g.nu + c->g.nv, 2, skip);
+}
+template <typename T>
+void bcs(fugu_ctx* c, T* u, T* v, double lid, const int* skip = nullptr) {
+  if (c->g.per_x && c->g.per_y) return;
+  LAUNCH((k_velocity_bcs<T>), grid_union(c->g), dim3(TX, TY), c->s, c->g, u, v, lid, skip);
+}
+template <typename T>
+void bcs_trip(fugu_ctx* c, T* X, double lid = 0.0, const int* skip = nullptr) {
+  bcs(c, X, X + c->g.nu, lid, skip);
+}
+
+template <typename T>
+void matvec(fugu_ctx* c, Work<T>& w, T* X, T* Y, T dt, int flags, const int* skip) {
+  const Grid& g = c->g;
+  halo_trip(c, X, skip);
+  MatvecArgs<T> a;
+  a.Xu = X; a.Xv = X + g.nu; a.Xp = X + g.nu + g.nv;
+  a.Yu = Y; a.Yv = Y + g.nu; a.Yp = Y + g.nu + g.nv;
+  a.uadv = w.uadv; a.vadv = w.vadv; a.rho_c = w.rho_c; a.rho_u = w.rho_u; a.rho_v = w.rho_v;
+  a.kap = w.kap; a.zeta = w.zeta; a.eta = w.eta;
+  a.penu = w.penu; a.penv = w.penv; a.mdu = w.mdu; a.mdv = w.mdv;
+  a.dt = dt; a.flags = flags;
+  LAUNCH((k_matvec<T>), grid_union(g), dim3(TX, TY), c->s, g, a, skip);
+}
+
+template <typename T>
+void cdot(fugu_ctx* c, int np, const T* const* a, const T* const* b, int phase, int k, int max_iter) {
+  DotArgs<T> d;
+  for (int q = 0; q < 5; ++q) { d.a[q] = q < np ? a[q] : nullptr; d.b[q] = q < np ? b[q] : nullptr; }
+  d.np = np; d.phase = phase; d.k = k; d.max_iter = max_iter;
+  const dim3 gr = grid_union(c->g), bl(TX, TY);
+  switch (np) {
+    case 1: LAUNCH((k_cdot<T, 1>), gr, bl, c->s, c->g, d, c->partials, c->ticket, c->sc, c->fl); break;
+    case 2: LAUNCH((k_cdot<T, 2>), gr, bl, c->s, c->g, d, c->partials, c->ticket, c->sc, c->fl); break;
+    case 5: LAUNCH((k_cdot<T, 5>), gr, bl, c->s, c->g, d, c->partials, c->ticket, c->sc, c->fl); break;
+    default: throw Error("cdot: unsupported pair count");
+  }
+}
+template <typename T>
+void cdot1(fugu_ctx* c, const T* a, const T* b, int phase, int k = 0, int mi = 0) {
+  const T* A[1] = {a};
+  const T* Bp[1] = {b};
+  cdot<T>(c, 1, A, Bp, phase, k, mi);
+}
+
+template <typename T>
+void lin(fugu_ctx* c, int op, T* o, const T* x0, const T* x1 = nullptr, const T* x2 = nullptr,
+         const T* x3 = nullptr, int c0 = -1, int c1 = -1, int c2 = -1, double s0 = 1.0, bool skip = true) {
+  LinArgs<T> a;
+  a.out = o; a.x0 = x0; a.x1 = x1; a.x2 = x2; a.x3 = x3; a.c0 = c0; a.c1 = c1; a.c2 = c2;
+  a.s0 = s0; a.op = op; a.n = c->ntrip; a.skip_on = skip ? 1 : 0;
+  LAUNCH((k_lin<T, T>), dim3(nblk(a.n)), dim3(256), c->s, a, c->sc, c->fl);
+}
+
+// ---------------------------------------------------------------------------------
+// Workspaces
+template <typename T>
+void work_alloc(fugu_ctx* c, Work<T>& w, bool own_props) {
+  if (w.ready) return;
+  const Grid& g = c->g;
+  const size_t n3 = c->ntrip;
+  T** trips[] = {&w.X, &w.B, &w.bh, &w.r, &w.rt, &w.uu, &w.vv, &w.ww, &w.ss, &w.tt, &w.z, &w.q};
+  for (T** t : trips) *t = dalloc<T>(c, n3);
+  if (own_props) {
+    w.rho_c = dalloc<T>(c, g.np); w.rho_u = dalloc<T>(c, g.nu); w.rho_v = dalloc<T>(c, g.nv);
+    w.kap = dalloc<T>(c, g.np); w.zeta = dalloc<T>(c, g.nw); w.eta = dalloc<T>(c, g.nw);
+    w.uadv = dalloc<T>(c, g.nu); w.vadv = dalloc<T>(c, g.nv);
+  }
+  w.phi = dalloc<T>(c, g.np); w.pr = dalloc<T>(c, g.np); w.pz = dalloc<T>(c, g.np);
+  w.pp0 = dalloc<T>(c, g.np); w.pp1 = dalloc<T>(c, g.np); w.Ap = dalloc<T>(c, g.np); w.dinv = dalloc<T>(c, g.np);
+  w.cx = dalloc<T>(c, g.nu); w.cy = dalloc<T>(c, g.nv);
+  w.zuw = dalloc<T>(c, g.nu); w.ju0 = dalloc<T>(c, g.nu); w.ju1 = dalloc<T>(c, g.nu);
+  w.zvw = dalloc<T>(c, g.nv); w.jv0 = dalloc<T>(c, g.nv); w.jv1 = dalloc<T>(c, g.nv);
+  w.ready = true;
+}
+
+// FP64 workspace views the context's resident property fields.
+void bind_props_d(fugu_ctx* c, bool pen, bool cons) {
+  Work<double>& w = c->wd;
+  w.rho_c = c->rho_c; w.rho_u = c->rho_u; w.rho_v = c->rho_v;
+  w.kap = c->kap; w.zeta = c->zeta; w.eta = c->eta;
+  w.uadv = c->uadv; w.vadv = c->vadv;
+  w.penu = pen ? c->penu : nullptr; w.penv = pen ? c->penv : nullptr;
+  w.mdu = cons ? c->mdu : nullptr; w.mdv = cons ? c->mdv : nullptr;
+}
+
+// FP32 copies of the properties for the mixed inner solve.
+void bind_props_f(fugu_ctx* c, bool pen, bool cons) {
+  Work<float>& w = c->wf;
+  const Grid& g = c->g;
+  auto cv = [&](float* o, const double* i, size_t n) {
+    LAUNCH((k_convert<float, double>), dim3(nblk(n)), dim3(256), c->s, o, i, n, c->sc, -1, 1.0, 0);
+  };
+  cv(w.rho_c, c->rho_c, g.np); cv(w.rho_u, c->rho_u, g.nu); cv(w.rho_v, c->rho_v, g.nv);
+  cv(w.kap, c->kap, g.np); cv(w.zeta, c->zeta, g.nw); cv(w.eta, c->eta, g.nw);
+  cv(w.uadv, c->uadv, g.nu); cv(w.vadv, c->vadv, g.nv);
+  w.penu = w.penv = nullptr;
+  w.mdu = w.mdv = nullptr;
+  if (pen) {
+    if (!c->f_penu) { c->f_penu = dalloc<float>(c, g.nu); c->f_penv = dalloc<float>(c, g.nv); }
+    w.penu = c->f_penu; w.penv = c->f_penv;
+    cv(w.penu, c->penu, g.nu); cv(w.penv, c->penv, g.nv);
+  }
+  if (cons) {
+    if (!c->f_mdu) { c->f_mdu = dalloc<float>(c, g.nu); c->f_mdv = dalloc<float>(c, g.nv); }
+    w.mdu = c->f_mdu; w.mdv = c->f_mdv;
+    cv(w.mdu, c->mdu, g.nu); cv(w.mdv, c->mdv, g.nv);
+  }
+}
+
+// ---------------------------------------------------------------------------------
+// H7: PCG pressure solve.
+template <typename T>
+void pcg_setup(fugu_ctx* c, Work<T>& w) {
+  LAUNCH((k_pcg_coef<T>), grid_union(c->g), dim3(TX, TY), c->s, c->g, w.rho_u, w.rho_v, w.cx, w.cy, w.dinv);
+}
+
+template <typename T>
+void pcg(fugu_ctx* c, Work<T>& w, T* phi, const T* rhs, bool phi_zero, double tol, int max_iter) {
+  const Grid& g = c->g;
+  const int walls_both = (!g.per_x && !g.per_y) ? 1 : 0;
+  const dim3 gc = grid_cells(g), bl(TX, TY);
+  c->pcg_calls++;
+  if (walls_both)
+    LAUNCH((k_pcg_sum<T>), gc, bl, c->s, g, rhs, c->partials, c->ticket, c->sc, (int)S_P_MEAN, 1, c->fl);
+  LAUNCH((k_pcg_init<T>), gc, bl, c->s, g, rhs, phi, w.cx, w.cy, w.dinv, w.pr, w.pz, w.pp0, walls_both,
+         phi_zero ? 1 : 0, c->partials, c->ticket, c->sc, c->fl);
+  T* P[2] = {w.pp0, w.pp1};
+  for (int it = 1; it <= max_iter; ++it) {
+    LAUNCH((k_pcg_iter1<T>), gc, bl, c->s, g, it, w.pz, P[(it - 1) & 1], P[it & 1], w.Ap, w.cx, w.cy,
+           c->partials, c->ticket, c->sc, c->fl);
+    LAUNCH((k_pcg_iter2<T>), gc, bl, c->s, g, it, max_iter, phi, P[it & 1], w.pr, w.Ap, w.pz, w.dinv,
+           c->partials, c->ticket, c->sc, c->fl);
+  }
+  (void)tol;
+}
+
+// H6: block PCD preconditioner R = P^-1 Z (triplets).
+template <typename T>
+void pcd(fugu_ctx* c, Work<T>& w, const T* Z, T* R, T dt) {
+  const Grid& g = c->g;
+  const int walls_both = (!g.per_x && !g.per_y) ? 1 : 0;
+  const T *Zu = Z, *Zv = Z + g.nu, *Zp = Z + g.nu + g.nv;
+  T *Ru = R, *Rv = R + g.nu, *Rp = R + g.nu + g.nv;
+  const dim3 gu = grid_union(g), gc = grid_cells(g), bl(TX, TY);
+  pcg<T>(c, w, w.phi, Zp, true, 1.0e-5, 150);
+  if (walls_both)   // mean of phi, unless the zero-rhs early exit fired (skip when F_P_DONE >= 2)
+    LAUNCH((k_pcg_sum<T>), gc, bl, c->s, g, (const T*)w.phi, c->partials, c->ticket, c->sc, (int)S_P_MEAN,
+           2, c->fl);
+  LAUNCH((k_pcd_rp<T>), gu, bl, c->s, g, w.phi, w.uadv, w.vadv, dt, walls_both, Rp, c->sc, c->fl);
+  halo(c, Rp, 2);
+  LAUNCH((k_pcd_zwork<T>), gu, bl, c->s, g, Zu, Zv, (const T*)Rp, w.zuw, w.zvw);
+  LAUNCH((k_jacobi_v<T>), gu, bl, c->s, g, (const T*)nullptr, w.jv0, w.zvw, w.rho_v, w.kap, w.zeta, w.eta, dt, w.penv);
+  LAUNCH((k_jacobi_v<T>), gu, bl, c->s, g, (const T*)w.jv0, w.jv1, w.zvw, w.rho_v, w.kap, w.zeta, w.eta, dt, w.penv);
+  LAUNCH((k_jacobi_v<T>), gu, bl, c->s, g, (const T*)w.jv1, Rv, w.zvw, w.rho_v, w.kap, w.zeta, w.eta, dt, w.penv);
+  halo(c, Rv, 1);
+  LAUNCH((k_pcd_cross<T>), gc, bl, c->s, g, (const T*)Rv, w.eta, w.zuw);
+  LAUNCH((k_jacobi_u<T>), gu, bl, c->s, g, (const T*)nullptr, w.ju0, w.zuw, w.rho_u, w.kap, w.zeta, w.eta, dt, w.penu);
+  LAUNCH((k_jacobi_u<T>), gu, bl, c->s, g, (const T*)w.ju0, w.ju1, w.zuw, w.rho_u, w.kap, w.zeta, w.eta, dt, w.penu);
+  LAUNCH((k_jacobi_u<T>), gu, bl, c->s, g, (const T*)w.ju1, Ru, w.zuw, w.rho_u, w.kap, w.zeta, w.eta, dt, w.penu);
+  halo(c, Ru, 0);
+}
+
+// ---------------------------------------------------------------------------------
+// H5: BiCGStab (src/mod_bicgstab2.f90:77-264), exact reference recurrences.
+template <typename T>
+void bicgstab1(fugu_ctx* c, Work<T>& w, T dt, double tol, int max_iter, int mvflags, bool has_lid,
+               double lid, int* iters, double* res) {
+  const int* brk = c->fl + F_BRK;
+  T *X = w.X, *r = w.r, *r0 = w.rt, *p = w.uu, *v = w.vv, *s = w.ss, *t = w.tt, *z = w.z;
+  reset_flags(c);
+  set_slot(c, S_TOL, tol);
+  set_slot(c, S_P_TOL, 1.0e-5);
+  pcg_setup(c, w);
+  bcs_trip(c, X);
+  pcd(c, w, w.B, w.bh, dt);
+  cdot1<T>(c, w.bh, w.bh, PH_NORM_B);
+  matvec(c, w, X, z, dt, mvflags, nullptr);
+  pcd(c, w, z, v, dt);
+  lin<T>(c, L_DIFF, r, w.bh, v);
+  lin<T>(c, L_COPY, r0, r);
+  lin<T>(c, L_COPY, p, r);
+  {
+    const T* A[2] = {r0, r};
+    const T* Bq[2] = {r, r};
+    cdot<T>(c, 2, A, Bq, PH_B1_INIT, 0, max_iter);
+  }
+  read_flags(c);
+  int k = 1;
+  if (!c->h_fl[F_DONE]) {
+    for (k = 1; k <= max_iter; ++k) {
+      bcs_trip(c, p, 0.0, brk);
+      matvec(c, w, p, z, dt, mvflags, brk);
+      pcd(c, w, z, v, dt);
+      bcs_trip(c, v, 0.0, brk);
+      cdot1<T>(c, r0, v, PH_B1_RV, k, max_iter);
+      lin<T>(c, L_SUB1, s, r, v, nullptr, nullptr, S_ALPHA);
+      bcs_trip(c, s, 0.0, brk);
+      matvec(c, w, s, z, dt, mvflags, brk);
+      pcd(c, w, z, t, dt);
+      bcs_trip(c, t, 0.0, brk);
+      {
+        const T* A[2] = {t, t};
+        const T* Bq[2] = {t, s};
+        cdot<T>(c, 2, A, Bq, PH_B1_TT_TS, k, max_iter);
+      }
+      lin<T>(c, L_ADD2, X, X, p, s, nullptr, S_ALPHA, S_OMEGA);
+      lin<T>(c, L_SUB1, r, s, t, nullptr, nullptr, S_OMEGA);
+      {
+        const T* A[2] = {r, r0};
+        const T* Bq[2] = {r, r};
+        cdot<T>(c, 2, A, Bq, PH_B1_RES, k, max_iter);
+      }
+      lin<T>(c, L_PUPD, p, r, p, v, nullptr, S_BETA, S_OMEGA);
+      bcs_trip(c, X, 0.0, brk);
+      read_flags(c);
+      if (c->h_fl[F_DONE]) break;
+    }
+  }
+  if (has_lid) bcs_trip(c, X, lid);
+  read_flags(c);
+  read_slots(c);
+  *iters = c->h_fl[F_DONE] ? c->h_fl[F_ITER] : max_iter;
+  *res = c->h_sc[S_REL_RES];
+}
+
+// H4: BiCGStab(2) (src/mod_bicgstab2.f90:270-570).  consistent = BCs on r and s before
+// their matvecs (not in the reference; see fugu2d.h FUGU_B2_CONSISTENT_BCS).
+template <typename T>
+void bicgstab2(fugu_ctx* c, Work<T>& w, T dt, double tol, int max_iter, int mvflags, bool has_lid,
+               double lid, bool consistent, int* iters, double* res) {
+  const int* brk = c->fl + F_BRK;
+  T *X = w.X, *r = w.r, *rt = w.rt, *uu = w.uu, *vv = w.vv, *ww = w.ww, *ss = w.ss, *tt = w.tt, *z = w.z;
+  reset_flags(c);
+  set_slot(c, S_TOL, tol);
+  set_slot(c, S_P_TOL, 1.0e-5);
+  pcg_setup(c, w);
+  bcs_trip(c, X);
+  pcd(c, w, w.B, w.bh, dt);
+  cdot1<T>(c, w.bh, w.bh, PH_NORM_B);
+  matvec(c, w, X, z, dt, mvflags, nullptr);
+  pcd(c, w, z, w.q, dt);
+  lin<T>(c, L_DIFF, r, w.bh, w.q);
+  lin<T>(c, L_COPY, rt, r);
+  HIPCHK(hipMemsetAsync(uu, 0, c->ntrip * sizeof(T), c->s));
+  HIPCHK(hipMemsetAsync(vv, 0, c->ntrip * sizeof(T), c->s));
+  cdot1<T>(c, r, r, PH_B2_INIT, 0, max_iter);
+  read_flags(c);
+  if (!c->h_fl[F_DONE]) {
+    for (int k = 1; k <= max_iter; ++k) {
+      // even step
+      cdot1<T>(c, rt, r, PH_B2_XI_EVEN, k, max_iter);
+      lin<T>(c, L_SUB1, uu, r, uu, nullptr, nullptr, S_BETA);
+      bcs_trip(c, uu, 0.0, brk);
+      matvec(c, w, uu, z, dt, mvflags, brk);
+      pcd(c, w, z, vv, dt);
+      bcs_trip(c, vv, 0.0, brk);
+      cdot1<T>(c, rt, vv, PH_B2_GAMMA, k, max_iter);
+      lin<T>(c, L_SUB1, r, r, vv, nullptr, nullptr, S_ALPHA);
+      if (consistent) bcs_trip(c, r, 0.0, brk);
+      matvec(c, w, r, z, dt, mvflags, brk);
+      pcd(c, w, z, ss, dt);
+      bcs_trip(c, ss, 0.0, brk);
+      lin<T>(c, L_ADD1, X, X, uu, nullptr, nullptr, S_ALPHA);
+      // odd step
+      cdot1<T>(c, rt, ss, PH_B2_XI_ODD, k, max_iter);
+      lin<T>(c, L_SUB1, vv, ss, vv, nullptr, nullptr, S_BETA);
+      bcs_trip(c, vv, 0.0, brk);
+      matvec(c, w, vv, z, dt, mvflags, brk);
+      pcd(c, w, z, ww, dt);
+      bcs_trip(c, ww, 0.0, brk);
+      cdot1<T>(c, rt, ww, PH_B2_GAMMA, k, max_iter);
+      lin<T>(c, L_SUB1, uu, r, uu, nullptr, nullptr, S_BETA);
+      lin<T>(c, L_SUB1, r, r, vv, nullptr, nullptr, S_ALPHA);
+      lin<T>(c, L_SUB1, ss, ss, ww, nullptr, nullptr, S_ALPHA);
+      if (consistent) bcs_trip(c, ss, 0.0, brk);
+      matvec(c, w, ss, z, dt, mvflags, brk);
+      pcd(c, w, z, tt, dt);
+      bcs_trip(c, tt, 0.0, brk);
+      {
+        const T* A[5] = {r, ss, ss, tt, r};
+        const T* Bq[5] = {ss, ss, tt, tt, tt};
+        cdot<T>(c, 5, A, Bq, PH_B2_GCR, k, max_iter);
+      }
+      lin<T>(c, L_ADD3, X, X, r, ss, uu, S_OM1, S_OM2, S_ALPHA);
+      lin<T>(c, L_SUB2, r, r, ss, tt, nullptr, S_OM1, S_OM2);
+      lin<T>(c, L_SUB2, uu, uu, vv, ww, nullptr, S_OM1, S_OM2);
+      cdot1<T>(c, r, r, PH_B2_RES, k, max_iter);
+      bcs_trip(c, X, 0.0, brk);
+      read_flags(c);
+      if (c->h_fl[F_DONE]) break;
+    }
+  }
+  if (has_lid) bcs_trip(c, X, lid);
+  read_flags(c);
+  read_slots(c);
+  *iters = c->h_fl[F_DONE] ? c->h_fl[F_ITER] : max_iter;
+  *res = c->h_sc[S_REL_RES];
+}
+
+// ---------------------------------------------------------------------------------
+// True relative residual ||B - A X|| / ||B|| (coupled_dot norm, FP64) of the resident
+// X triplet against the FP64 workspace B.  Uses wd.z (A X) and wd.q (residual).
+double true_residual(fugu_ctx* c, const double* X, const double* B, int mvflags, double dt) {
+  Work<double>& w = c->wd;
+  HIPCHK(hipMemcpyAsync(w.X, X, c->ntrip * sizeof(double), hipMemcpyDeviceToDevice, c->s));
+  bcs_trip(c, w.X);
+  matvec(c, w, w.X, w.z, dt, mvflags, nullptr);
+  lin<double>(c, L_DIFF, w.q, B, w.z, nullptr, nullptr, -1, -1, -1, 1.0, false);
+  {
+    const double* A[2] = {w.q, B};
+    const double* Bq[2] = {w.q, B};
+    cdot<double>(c, 2, A, Bq, PH_STORE, 0, 0);
+  }
+  read_slots(c);
+  const double nb = c->h_sc[S_DOT1];
+  return nb > 0 ? sqrt(c->h_sc[S_DOT0] / nb) : sqrt(c->h_sc[S_DOT0]);
+}
+
+// M1: FP64 iterative refinement on the true residual; inner BiCGStab(2)+PCD (consistent
+// BCs) on the normalised residual in precision T.  X (FP64 resident triplet) in place,
+// B = wd.B.  Returns total inner iterations, outer count, final true residual.
+template <typename T>
+void solve_refined(fugu_ctx* c, Work<T>& w, double* X, double dt, double tol_true, double inner_tol,
+                   int max_outer, int max_inner, int mvflags, bool has_lid, double lid,
+                   int* iters, int* outer, double* res) {
+  Work<double>& wd = c->wd;
+  const size_t n3 = c->ntrip;
+  int total = 0, o = 0;
+  double rr = 0.0;
+  bcs_trip(c, X);
+  for (o = 0; o <= max_outer; ++o) {
+    // R = B - A X in FP64 (wd.q), ||R||^2 and ||B||^2
+    HIPCHK(hipMemcpyAsync(wd.X, X, n3 * sizeof(double), hipMemcpyDeviceToDevice, c->s));
+    matvec(c, wd, wd.X, wd.z, dt, mvflags, nullptr);
+    lin<double>(c, L_DIFF, wd.q, wd.B, wd.z, nullptr, nullptr, -1, -1, -1, 1.0, false);
+    {
+      const double* A[2] = {wd.q, wd.B};
+      const double* Bq[2] = {wd.q, wd.B};
+      cdot<double>(c, 2, A, Bq, PH_STORE, 0, 0);
+    }
+    read_slots(c);
+    const double rn2 = c->h_sc[S_DOT0], bn2 = c->h_sc[S_DOT1];
+    rr = bn2 > 0 ? sqrt(rn2 / bn2) : sqrt(rn2);
+    if (rr <= tol_true || o == max_outer || rn2 == 0.0) break;
+    // inner system A D = R / ||R||
+    const double rn = sqrt(rn2);
+    set_slot(c, S_SCALE, rn);
+    LAUNCH((k_convert<T, double>), dim3(nblk(n3)), dim3(256), c->s, w.B, wd.q, n3, c->sc, (int)S_SCALE, 1.0, 1);
+    HIPCHK(hipMemsetAsync(w.X, 0, n3 * sizeof(T), c->s));
+    int it = 0;
+    double ires = 0.0;
+    bicgstab2<T>(c, w, (T)dt, inner_tol, max_inner, mvflags, false, 0.0, true, &it, &ires);
+    total += it;
+    LAUNCH((k_refine_add<T>), dim3(nblk(n3)), dim3(256), c->s, X, (const T*)w.X, n3, c->sc, (int)S_SCALE);
+    bcs_trip(c, X);
+  }
+  if (has_lid) bcs_trip(c, X, lid);
+  *iters = total;
+  *outer = o;
+  *res = rr;
+}
+
+// ---------------------------------------------------------------------------------
+// H10 / H11 / H12 / H9 helpers on resident buffers
+void svof(fugu_ctx* c, const double* craw, double* cs_out, double delta, int max_iter, double tol, int* it_out) {
+  const Grid& g = c->g;
+  const double cx = (delta / g.dx) * (delta / g.dx);
+  const double cy = (delta / g.dy) * (delta / g.dy);
+  const double diag = 1.0 + 2.0 * (cx + cy);
+  HIPCHK(hipMemcpyAsync(c->csA, craw, g.np * sizeof(double), hipMemcpyDeviceToDevice, c->s));
+  HIPCHK(hipMemsetAsync(c->fl + F_S_DONE, 0, 2 * sizeof(int), c->s));
+  double* B[2] = {c->csA, c->csB};
+  for (int it = 1; it <= max_iter; ++it)
+    LAUNCH(k_svof_sweep, grid_cells(g), dim3(TX, TY), c->s, g, it, craw, (const double*)B[(it - 1) & 1],
+           B[it & 1], cx, cy, diag, tol, c->partials, c->ticket, c->sc, c->fl);
+  // the last iterate is in B[iter & 1]
+  read_flags(c);
+  const int itf = c->h_fl[F_S_ITER];
+  if (it_out) *it_out = itf;
+  const double* last = (max_iter >= 1) ? B[itf & 1] : c->csA;
+  // ghosts: C_smooth starts as a copy of C_raw (ghosts included), then exchange + mirror
+  HIPCHK(hipMemcpyAsync(cs_out, craw, g.np * sizeof(double), hipMemcpyDeviceToDevice, c->s));
+  {
+    // interior of the last iterate into cs_out
+    LinArgs<double> a;  // not used: copy interior via 2D memcpy below
+    (void)a;
+    const size_t rowb = (size_t)g.nx * sizeof(double);
+    HIPCHK(hipMemcpy2DAsync(cs_out + IP(g, 1, 1), g.ldp * sizeof(double), last + IP(g, 1, 1),
+                            g.ldp * sizeof(double), rowb, g.ny, hipMemcpyDeviceToDevice, c->s));
+  }
+  halo(c, cs_out, 2);
+  if (!g.per_x || !g.per_y) LAUNCH(k_mirror_p<double>, grid_union(g), dim3(TX, TY), c->s, g, cs_out);
+}
+
+void curvature(fugu_ctx* c, const double* cs, double* kc) {
+  const Grid& g = c->g;
+  LAUNCH(k_normals, grid_union(g), dim3(TX, TY), c->s, g, cs, c->nxu, c->nyv);
+  LAUNCH(k_curv, grid_union(g), dim3(TX, TY), c->s, g, (const double*)c->nxu, (const double*)c->nyv, kc);
+  halo(c, kc, 2);
+}
+
+void csf(fugu_ctx* c, const double* craw, const double* kc, double* fu, double* fv) {
+  LAUNCH(k_csf, grid_union(c->g), dim3(TX, TY), c->s, c->g, craw, kc, c->prm.sigma, fu, fv);
+}
+
+void mixture(fugu_ctx* c, const double* cc) {
+  LAUNCH(k_mixture, grid_union(c->g), dim3(TX, TY), c->s, c->g, cc, c->prm.rho1, c->prm.rho2, c->prm.mu1,
+         c->prm.mu2, c->rho_c, c->rho_u, c->rho_v, c->kap, c->zeta, c->eta);
+}
+
+void vof(fugu_ctx* c, const double* cin, double* cout, const double* u, const double* v, double dt,
+         double* fx, double* fy) {
+  LAUNCH(k_vof, grid_union(c->g), dim3(TX, TY), c->s, c->g, cin, cout, u, v, dt, fx, fy);
+  if (fx) halo(c, fx, 0);
+  if (fy) halo(c, fy, 1);
+  halo(c, cout, 2);
+}
+
+// ---------------------------------------------------------------------------------
+// H13: one advance_coupled_step on the resident state.
+int step_impl(fugu_ctx* c, double dt, int step_idx, const fugu_step_opts* o, fugu_step_stats* st) {
+  const Grid& g = c->g;
+  const fugu_params& P = c->prm;
+  fugu_step_opts def = {1, 0, 0.0, 1.0e-6, 500, 0, 0.0, 0.0, 0u};
+  if (!o) o = &def;
+  const bool bdf2 = (P.time_order == 2 && step_idx >= 2);
+  const double dt_eff = bdf2 ? dt / 1.5 : dt;
+  const double lid = o->has_lid ? o->u_lid : 0.0;
+  const bool pen = c->has_solid;
+  const bool amp = o->use_amp != 0;
+  HIPCHK(hipEventRecord(c->ev[0], c->s));
+  // 1. u_adv, u_prev (:363-375): unm1 <- u_prev, u_prev <- u, u_adv <- 2u - unm1
+  HIPCHK(hipMemcpyAsync(c->unm1, c->u_prev, g.nu * sizeof(double), hipMemcpyDeviceToDevice, c->s));
+  HIPCHK(hipMemcpyAsync(c->vnm1, c->v_prev, g.nv * sizeof(double), hipMemcpyDeviceToDevice, c->s));
+  LAUNCH(k_ab2, dim3(nblk(g.nu)), dim3(256), c->s, c->uadv, c->u_prev, (const double*)c->u, g.nu, step_idx <= 1 ? 1 : 0);
+  LAUNCH(k_ab2, dim3(nblk(g.nv)), dim3(256), c->s, c->vadv, c->v_prev, (const double*)c->v, g.nv, step_idx <= 1 ? 1 : 0);
+  // 2. VOF (:377-388) with u^n
+  vof(c, c->c, c->cnew, c->u, c->v, dt, amp ? c->fx : nullptr, amp ? c->fy : nullptr);
+  if (amp) {
+    LAUNCH(k_amp, dim3(nblk(g.nu)), dim3(256), c->s, c->mdu, (const double*)c->fx, (const double*)c->uadv, g.nu,
+           P.rho1 - P.rho2, P.rho2);
+    LAUNCH(k_amp, dim3(nblk(g.nv)), dim3(256), c->s, c->mdv, (const double*)c->fy, (const double*)c->vadv, g.nv,
+           P.rho1 - P.rho2, P.rho2);
+  }
+  std::swap(c->c, c->cnew);
+  // 3. mixture (:391-392)
+  mixture(c, c->c);
+  // 4. SVOF + curvature + CSF (:395-405)
+  if (P.sigma > 1.0e-14) {
+    const double delta = 1.5 * (g.dx < g.dy ? g.dx : g.dy);
+    svof(c, c->c, c->cs, delta, 150, 1.0e-7, nullptr);
+    curvature(c, c->cs, c->kcurv);
+    csf(c, c->c, c->kcurv, c->fu, c->fv);
+  } else {
+    HIPCHK(hipMemsetAsync(c->kcurv, 0, g.np * sizeof(double), c->s));
+    HIPCHK(hipMemsetAsync(c->fu, 0, g.nu * sizeof(double), c->s));
+    HIPCHK(hipMemsetAsync(c->fv, 0, g.nv * sizeof(double), c->s));
+  }
+  // 5. penalty (:408-412)
+  if (pen) {
+    LAUNCH(k_scale, dim3(nblk(g.nu)), dim3(256), c->s, c->penu, (const double*)c->phis_u, g.nu, P.alpha_penalty);
+    LAUNCH(k_scale, dim3(nblk(g.nv)), dim3(256), c->s, c->penv, (const double*)c->phis_v, g.nv, P.alpha_penalty);
+  }
+  // 6. RHS (:415-456) into wd.B
+  Work<double>& w = c->wd;
+  {
+    RhsArgs a;
+    a.un = c->u; a.unm1 = c->unm1; a.vn = c->v; a.vnm1 = c->vnm1; a.rho_u = c->rho_u; a.rho_v = c->rho_v;
+    a.fu = c->fu; a.fv = c->fv; a.penu = pen ? c->penu : nullptr; a.penv = pen ? c->penv : nullptr;
+    a.zeta = c->zeta; a.eta = c->eta;
+    a.Bu = w.B; a.Bv = w.B + g.nu; a.Bp = w.B + g.nu + g.nv;
+    a.dt = dt; a.gx = P.gx; a.gy = P.gy; a.u_solid = o->u_solid; a.v_solid = o->v_solid; a.u_lid = lid;
+    a.bdf2 = bdf2 ? 1 : 0; a.has_lid = o->has_lid ? 1 : 0;
+    LAUNCH(k_rhs, grid_union(g), dim3(TX, TY), c->s, g, a);
+  }
+  // 7. solve (:459-519)
+  bind_props_d(c, pen, amp);
+  const int mvflags = MV_ADV | (pen ? MV_PEN : 0) | (amp ? MV_CONS : 0);
+  HIPCHK(hipEventRecord(c->ev[1], c->s));
+  int iters = 0, outer = 0;
+  double res = 0.0;
+  c->pcg_calls = 0;
+  if (o->solver_flags & FUGU_SOLVE_REFINED) {
+    if (o->solver_flags & FUGU_SOLVE_MIXED) {
+      work_alloc<float>(c, c->wf, true);
+      bind_props_f(c, pen, amp);
+      solve_refined<float>(c, c->wf, c->X, dt_eff, o->tol, 1.0e-2, 200, 25, mvflags, true, lid, &iters, &outer, &res);
+    } else {
+      solve_refined<double>(c, c->wd, c->X, dt_eff, o->tol, 1.0e-2, 200, 25, mvflags, true, lid, &iters, &outer, &res);
+    }
+  } else {
+    HIPCHK(hipMemcpyAsync(w.X, c->X, c->ntrip * sizeof(double), hipMemcpyDeviceToDevice, c->s));
+    if (o->use_bicgstab2)
+      bicgstab2<double>(c, w, dt_eff, o->tol, o->max_iter, mvflags, true, lid,
+                        (o->solver_flags & FUGU_B2_CONSISTENT_BCS) != 0, &iters, &res);
+    else
+      bicgstab1<double>(c, w, dt_eff, o->tol, o->max_iter, mvflags, true, lid, &iters, &res);
+    HIPCHK(hipMemcpyAsync(c->X, w.X, c->ntrip * sizeof(double), hipMemcpyDeviceToDevice, c->s));
+  }
+  HIPCHK(hipEventRecord(c->ev[2], c->s));
+  // 8. final halos + BCs (:522-525)
+  halo(c, c->u, 0);
+  halo(c, c->v, 1);
+  halo(c, c->p, 2);
+  bcs(c, c->u, c->v, lid);
+  HIPCHK(hipEventRecord(c->ev[3], c->s));
+  sync(c);
+  if (st) {
+    float ms_solve = 0.f, ms_step = 0.f;
+    HIPCHK(hipEventElapsedTime(&ms_solve, c->ev[1], c->ev[2]));
+    HIPCHK(hipEventElapsedTime(&ms_step, c->ev[0], c->ev[3]));
+    st->iters = iters;
+    st->outer = outer;
+    st->res = res;
+    st->true_res = -1.0;
+    st->solve_seconds = ms_solve * 1e-3;
+    st->step_seconds = ms_step * 1e-3;
+    st->pcg_calls = c->pcg_calls;
+  }
+  int rc = 0;
+  if (!(res == res)) rc |= FUGU_W_NONFINITE;
+  if (!(o->solver_flags & FUGU_SOLVE_REFINED) && iters >= o->max_iter) rc |= FUGU_W_MAX_ITER;
+  return rc;
+}
+
+template <typename F>
+int guarded(F&& f) {
+  try {
+    return f();
+  } catch (const std::exception& e) {
+    g_err = e.what();
+    return -1;
+  }
+}
+
+#define NEED(ctx) \
+  if (!(ctx)) { g_err = "null context"; return -2; }
+
+}  // namespace
+
+// =====================================================================================
+// C ABI
+extern "C" {
+
+const char* fugu_last_error(void) { return g_err.c_str(); }
+
+long long fugu_field_size(const fugu_ctx* c, int kind) {
+  if (!c) return -1;
+  switch (kind) {
+    case FUGU_KIND_U: return (long long)c->g.nu;
+    case FUGU_KIND_V: return (long long)c->g.nv;
+    case FUGU_KIND_P: return (long long)c->g.np;
+    case FUGU_KIND_W: return (long long)c->g.nw;
+    default: return -1;
+  }
+}
+
+int fugu_ctx_create(const fugu_grid* grid, const fugu_params* prm, fugu_ctx** out) {
+  if (!grid || !prm || !out) { g_err = "null argument"; return -2; }
+  if (grid->nh != 2) { g_err = "nh must be 2 (src/main.f90:22)"; return -2; }
+  if (grid->nx < 4 || grid->ny < 4) { g_err = "grid too small (nx, ny >= 4)"; return -2; }
+  fugu_ctx* c = new fugu_ctx();
+  int rc = guarded([&]() {
+    int ndev = 0;
+    HIPCHK(hipGetDeviceCount(&ndev));
+    if (ndev < 1) throw Error("no HIP device");
+    c->g = make_grid(grid->nx, grid->ny, grid->nh, grid->lx / grid->nx, grid->ly / grid->ny,
+                     grid->periodic_x ? 1 : 0, grid->periodic_y ? 1 : 0);
+    c->prm = *prm;
+    HIPCHK(hipStreamCreateWithFlags(&c->s, hipStreamNonBlocking));
+    for (auto& e : c->ev) HIPCHK(hipEventCreate(&e));
+    const Grid& g = c->g;
+    c->ntrip = g.nu + g.nv + g.np;
+    const dim3 gu = grid_union(g);
+    const size_t maxb = (size_t)gu.x * gu.y + 64;
+    c->partials = dalloc<double>(c, maxb * 10);
+    c->ticket = (unsigned*)dalloc<double>(c, 1);
+    c->sc = dalloc<double>(c, S_COUNT);
+    c->fl = (int*)dalloc<double>(c, F_COUNT);
+    HIPCHK(hipHostMalloc((void**)&c->h_sc, S_COUNT * sizeof(double), hipHostMallocDefault));
+    HIPCHK(hipHostMalloc((void**)&c->h_fl, F_COUNT * sizeof(int), hipHostMallocDefault));
+    c->X = dalloc<double>(c, c->ntrip);
+    c->u = c->X; c->v = c->X + g.nu; c->p = c->X + g.nu + g.nv;
+    c->c = dalloc<double>(c, g.np); c->cnew = dalloc<double>(c, g.np);
+    c->u_prev = dalloc<double>(c, g.nu); c->v_prev = dalloc<double>(c, g.nv);
+    c->unm1 = dalloc<double>(c, g.nu); c->vnm1 = dalloc<double>(c, g.nv);
+    c->rho_c = dalloc<double>(c, g.np); c->rho_u = dalloc<double>(c, g.nu); c->rho_v = dalloc<double>(c, g.nv);
+    c->kap = dalloc<double>(c, g.np); c->zeta = dalloc<double>(c, g.nw); c->eta = dalloc<double>(c, g.nw);
+    c->kcurv = dalloc<double>(c, g.np); c->fu = dalloc<double>(c, g.nu); c->fv = dalloc<double>(c, g.nv);
+    c->phis_u = dalloc<double>(c, g.nu); c->phis_v = dalloc<double>(c, g.nv);
+    c->uadv = dalloc<double>(c, g.nu); c->vadv = dalloc<double>(c, g.nv);
+    c->csA = dalloc<double>(c, g.np); c->csB = dalloc<double>(c, g.np); c->cs = dalloc<double>(c, g.np);
+    c->nxu = dalloc<double>(c, g.nu); c->nyv = dalloc<double>(c, g.nv);
+    c->fx = dalloc<double>(c, g.nu); c->fy = dalloc<double>(c, g.nv);
+    c->mdu = dalloc<double>(c, g.nu); c->mdv = dalloc<double>(c, g.nv);
+    c->penu = dalloc<double>(c, g.nu); c->penv = dalloc<double>(c, g.nv);
+    for (int k = 0; k < 4; ++k) {
+      c->su[k] = dalloc<double>(c, g.nu); c->sv[k] = dalloc<double>(c, g.nv); c->sp[k] = dalloc<double>(c, g.np);
+    }
+    work_alloc<double>(c, c->wd, false);
+    bind_props_d(c, false, false);
+    sync(c);
+    return 0;
+  });
+  if (rc != 0) {
+    fugu_ctx_destroy(c);
+    return rc;
+  }
+  *out = c;
+  return 0;
+}
+
+int fugu_ctx_destroy(fugu_ctx* c) {
+  if (!c) return 0;
+  if (c->s) (void)hipStreamSynchronize(c->s);
+  for (void* p : c->allocs) (void)hipFree(p);
+  if (c->h_sc) (void)hipHostFree(c->h_sc);
+  if (c->h_fl) (void)hipHostFree(c->h_fl);
+  for (auto& e : c->ev)
+    if (e) (void)hipEventDestroy(e);
+  if (c->s) (void)hipStreamDestroy(c->s);
+  delete c;
+  return 0;
+}
+
+int fugu_set_params(fugu_ctx* c, const fugu_params* p) {
+  NEED(c);
+  if (!p) return -2;
+  c->prm = *p;
+  return 0;
+}
+
+int fugu_exchange_halos(fugu_ctx* c, double* a, int kind) {
+  NEED(c);
+  return guarded([&]() {
+    const Grid& g = c->g;
+    if (kind < 0 || kind > 2) throw Error("kind must be FUGU_KIND_U, _V or _P");
+    const size_t n = kind == 0 ? g.nu : (kind == 1 ? g.nv : g.np);
+    double* d = kind == 0 ? c->su[0] : (kind == 1 ? c->sv[0] : c->sp[0]);
+    in(c, d, a, n);
+    halo(c, d, kind);
+    out(c, a, d, n);
+    sync(c);
+    return 0;
+  });
+}
+
+int fugu_op_divergence(fugu_ctx* c, const double* u, const double* v, double* div) {
+  NEED(c);
+  return guarded([&]() {
+    const Grid& g = c->g;
+    Work<double>& w = c->wd;
+    // divergence is the p-row of the matvec: run the matvec with zero properties' effect
+    in(c, w.X, u, g.nu); in(c, w.X + g.nu, v, g.nv);
+    HIPCHK(hipMemsetAsync(w.X + g.nu + g.nv, 0, g.np * sizeof(double), c->s));
+    MatvecArgs<double> a{};
+    a.Xu = w.X; a.Xv = w.X + g.nu; a.Xp = w.X + g.nu + g.nv;
+    a.Yu = w.z; a.Yv = w.z + g.nu; a.Yp = w.z + g.nu + g.nv;
+    a.uadv = c->uadv; a.vadv = c->vadv; a.rho_c = c->rho_c; a.rho_u = c->rho_u; a.rho_v = c->rho_v;
+    a.kap = c->kap; a.zeta = c->zeta; a.eta = c->eta; a.dt = 1.0; a.flags = 0;
+    LAUNCH((k_matvec<double>), grid_union(g), dim3(TX, TY), c->s, g, a, (const int*)nullptr);
+    out(c, div, w.z + g.nu + g.nv, g.np);
+    sync(c);
+    return 0;
+  });
+}
+
+int fugu_apply_velocity_bcs(fugu_ctx* c, double* u, double* v, const double* lid) {
+  NEED(c);
+  return guarded([&]() {
+    const Grid& g = c->g;
+    in(c, c->su[0], u, g.nu); in(c, c->sv[0], v, g.nv);
+    bcs(c, c->su[0], c->sv[0], lid ? *lid : 0.0);
+    out(c, u, c->su[0], g.nu); out(c, v, c->sv[0], g.nv);
+    sync(c);
+    return 0;
+  });
+}
+
+static void upload_props(fugu_ctx* c, const double* uadv, const double* vadv, const double* rho_c,
+                         const double* rho_u, const double* rho_v, const double* kap, const double* zeta,
+                         const double* eta, const double* pu, const double* pv, const double* mu,
+                         const double* mv) {
+  const Grid& g = c->g;
+  in(c, c->uadv, uadv, g.nu); in(c, c->vadv, vadv, g.nv);
+  in(c, c->rho_c, rho_c, g.np); in(c, c->rho_u, rho_u, g.nu); in(c, c->rho_v, rho_v, g.nv);
+  in(c, c->kap, kap, g.np); in(c, c->zeta, zeta, g.nw); in(c, c->eta, eta, g.nw);
+  in(c, c->penu, pu, g.nu); in(c, c->penv, pv, g.nv);
+  in(c, c->mdu, mu, g.nu); in(c, c->mdv, mv, g.nv);
+  bind_props_d(c, pu && pv, mu && mv);
+  if (!pu) c->wd.penu = nullptr;
+  if (!pv) c->wd.penv = nullptr;
+}
+
+int fugu_matvec_coupled(fugu_ctx* c, double* Xu, double* Xv, double* Xp, double* Yu, double* Yv, double* Yp,
+                        const double* uadv, const double* vadv, const double* rho_c, const double* rho_u,
+                        const double* rho_v, const double* kap, const double* zeta, const double* eta,
+                        double dt, int include_adv, const double* pu, const double* pv, const double* mu,
+                        const double* mv) {
+  NEED(c);
+  return guarded([&]() {
+    const Grid& g = c->g;
+    Work<double>& w = c->wd;
+    upload_props(c, uadv, vadv, rho_c, rho_u, rho_v, kap, zeta, eta, pu, pv, mu, mv);
+    in(c, w.X, Xu, g.nu); in(c, w.X + g.nu, Xv, g.nv); in(c, w.X + g.nu + g.nv, Xp, g.np);
+    const int flags = (include_adv ? MV_ADV : 0) | ((pu && pv) ? MV_PEN : 0) | ((mu && mv) ? MV_CONS : 0);
+    matvec<double>(c, w, w.X, w.z, dt, flags, nullptr);
+    out(c, Xu, w.X, g.nu); out(c, Xv, w.X + g.nu, g.nv); out(c, Xp, w.X + g.nu + g.nv, g.np);
+    out(c, Yu, w.z, g.nu); out(c, Yv, w.z + g.nu, g.nv); out(c, Yp, w.z + g.nu + g.nv, g.np);
+    sync(c);
+    return 0;
+  });
+}
+
+int fugu_coupled_dot(fugu_ctx* c, const double* u1, const double* v1, const double* p1, const double* u2,
+                     const double* v2, const double* p2, double* dot_out) {
+  NEED(c);
+  return guarded([&]() {
+    const Grid& g = c->g;
+    Work<double>& w = c->wd;
+    in(c, w.z, u1, g.nu); in(c, w.z + g.nu, v1, g.nv); in(c, w.z + g.nu + g.nv, p1, g.np);
+    in(c, w.q, u2, g.nu); in(c, w.q + g.nu, v2, g.nv); in(c, w.q + g.nu + g.nv, p2, g.np);
+    reset_flags(c);
+    cdot1<double>(c, w.z, w.q, PH_STORE);
+    read_slots(c);
+    if (dot_out) *dot_out = c->h_sc[S_DOT0];
+    return 0;
+  });
+}
+
+int fugu_solve_poisson_pressure(fugu_ctx* c, double* phi, const double* rhs, const double* rho_u,
+                                const double* rho_v, double tol, int max_iter, int* iter_out, double* res_out) {
+  NEED(c);
+  return guarded([&]() {
+    const Grid& g = c->g;
+    Work<double>& w = c->wd;
+    if (tol <= 0) tol = 1.0e-5;
+    if (max_iter <= 0) max_iter = 150;
+    in(c, c->rho_u, rho_u, g.nu); in(c, c->rho_v, rho_v, g.nv);
+    bind_props_d(c, false, false);
+    double* dphi = c->sp[0];
+    double* drhs = c->sp[1];
+    in(c, dphi, phi, g.np); in(c, drhs, rhs, g.np);
+    reset_flags(c);
+    set_slot(c, S_P_TOL, tol);
+    pcg_setup(c, w);
+    // phi ghosts: halo + Neumann mirror of the initial guess (:123-124)
+    halo(c, dphi, 2);
+    if (!g.per_x || !g.per_y) LAUNCH(k_mirror_p<double>, grid_union(g), dim3(TX, TY), c->s, g, dphi);
+    pcg<double>(c, w, dphi, drhs, false, tol, max_iter);
+    if (!g.per_x && !g.per_y) {
+      LAUNCH((k_pcg_sum<double>), grid_cells(g), dim3(TX, TY), c->s, g, (const double*)dphi, c->partials,
+             c->ticket, c->sc, (int)S_P_MEAN, 2, c->fl);
+      LAUNCH((k_pcg_submean<double>), grid_cells(g), dim3(TX, TY), c->s, g, dphi, (const double*)c->sc,
+             (const int*)c->fl);
+    }
+    LAUNCH((k_zero_if<double>), dim3(nblk(g.np)), dim3(256), c->s, dphi, g.np, (const int*)c->fl, (int)F_P_DONE, 2);
+    out(c, phi, dphi, g.np);
+    read_flags(c);
+    read_slots(c);
+    if (iter_out) *iter_out = c->h_fl[F_P_ITER];
+    if (res_out) *res_out = (c->h_fl[F_P_DONE] == 2) ? 0.0 : c->h_sc[S_P_RES] / c->h_sc[S_P_RHSN];
+    return 0;
+  });
+}
+
+int fugu_solve_helmholtz_u(fugu_ctx* c, double* u_sol, const double* rhs, const double* rho_u, const double* kap,
+                           const double* zeta, const double* eta, double dt, int n_sweeps, const double* pen) {
+  NEED(c);
+  return guarded([&]() {
+    const Grid& g = c->g;
+    if (n_sweeps <= 0) n_sweeps = 5;
+    in(c, c->rho_u, rho_u, g.nu); in(c, c->kap, kap, g.np); in(c, c->zeta, zeta, g.nw); in(c, c->eta, eta, g.nw);
+    in(c, c->penu, pen, g.nu);
+    double *a = c->su[0], *b = c->su[1], *r = c->su[2];
+    in(c, a, u_sol, g.nu); in(c, r, rhs, g.nu);
+    for (int k = 0; k < n_sweeps; ++k) {
+      halo(c, a, 0);   // exchange at the top of each sweep (:281) -- ghosts persist
+      LAUNCH((k_jacobi_u<double>), grid_union(g), dim3(TX, TY), c->s, g, (const double*)a, b, (const double*)r,
+             (const double*)c->rho_u, (const double*)c->kap, (const double*)c->zeta, (const double*)c->eta, dt,
+             pen ? (const double*)c->penu : nullptr);
+      std::swap(a, b);
+    }
+    out(c, u_sol, a, g.nu);
+    sync(c);
+    return 0;
+  });
+}
+
+int fugu_solve_helmholtz_v(fugu_ctx* c, double* v_sol, const double* rhs, const double* rho_v, const double* kap,
+                           const double* zeta, const double* eta, double dt, int n_sweeps, const double* pen) {
+  NEED(c);
+  return guarded([&]() {
+    const Grid& g = c->g;
+    if (n_sweeps <= 0) n_sweeps = 5;
+    in(c, c->rho_v, rho_v, g.nv); in(c, c->kap, kap, g.np); in(c, c->zeta, zeta, g.nw); in(c, c->eta, eta, g.nw);
+    in(c, c->penv, pen, g.nv);
+    double *a = c->sv[0], *b = c->sv[1], *r = c->sv[2];
+    in(c, a, v_sol, g.nv); in(c, r, rhs, g.nv);
+    for (int k = 0; k < n_sweeps; ++k) {
+      halo(c, a, 1);
+      LAUNCH((k_jacobi_v<double>), grid_union(g), dim3(TX, TY), c->s, g, (const double*)a, b, (const double*)r,
+             (const double*)c->rho_v, (const double*)c->kap, (const double*)c->zeta, (const double*)c->eta, dt,
+             pen ? (const double*)c->penv : nullptr);
+      std::swap(a, b);
+    }
+    out(c, v_sol, a, g.nv);
+    sync(c);
+    return 0;
+  });
+}
+
+int fugu_apply_pcd_precond(fugu_ctx* c, const double* zu, const double* zv, const double* zp, double* ru,
+                           double* rv, double* rp, const double* uadv, const double* vadv, const double* rho_u,
+                           const double* rho_v, const double* kap, const double* zeta, const double* eta,
+                           double dt, const double* pu, const double* pv) {
+  NEED(c);
+  return guarded([&]() {
+    const Grid& g = c->g;
+    Work<double>& w = c->wd;
+    upload_props(c, uadv, vadv, nullptr, rho_u, rho_v, kap, zeta, eta, pu, pv, nullptr, nullptr);
+    in(c, w.z, zu, g.nu); in(c, w.z + g.nu, zv, g.nv); in(c, w.z + g.nu + g.nv, zp, g.np);
+    reset_flags(c);
+    set_slot(c, S_P_TOL, 1.0e-5);
+    pcg_setup(c, w);
+    pcd<double>(c, w, w.z, w.q, dt);
+    out(c, ru, w.q, g.nu); out(c, rv, w.q + g.nu, g.nv); out(c, rp, w.q + g.nu + g.nv, g.np);
+    sync(c);
+    return 0;
+  });
+}
+
+static int krylov_entry(fugu_ctx* c, int order, double* Xu, double* Xv, double* Xp, const double* Bu,
+                        const double* Bv, const double* Bp, const double* uadv, const double* vadv,
+                        const double* rho_c, const double* rho_u, const double* rho_v, const double* kap,
+                        const double* zeta, const double* eta, double dt, double tol, int max_iter,
+                        int* iter_out, double* res_out, const double* lid, const double* pu, const double* pv,
+                        const double* mu, const double* mv, unsigned flags) {
+  NEED(c);
+  return guarded([&]() {
+    const Grid& g = c->g;
+    Work<double>& w = c->wd;
+    if (tol <= 0) tol = 1.0e-5;
+    if (max_iter <= 0) max_iter = 100;
+    upload_props(c, uadv, vadv, rho_c, rho_u, rho_v, kap, zeta, eta, pu, pv, mu, mv);
+    in(c, w.X, Xu, g.nu); in(c, w.X + g.nu, Xv, g.nv); in(c, w.X + g.nu + g.nv, Xp, g.np);
+    in(c, w.B, Bu, g.nu); in(c, w.B + g.nu, Bv, g.nv); in(c, w.B + g.nu + g.nv, Bp, g.np);
+    set_slot(c, S_P_TOL, 1.0e-5);
+    const int mvflags = MV_ADV | ((pu && pv) ? MV_PEN : 0) | ((mu && mv) ? MV_CONS : 0);
+    int it = 0;
+    double res = 0.0;
+    if (order == 2)
+      bicgstab2<double>(c, w, dt, tol, max_iter, mvflags, lid != nullptr, lid ? *lid : 0.0,
+                        (flags & FUGU_B2_CONSISTENT_BCS) != 0, &it, &res);
+    else
+      bicgstab1<double>(c, w, dt, tol, max_iter, mvflags, lid != nullptr, lid ? *lid : 0.0, &it, &res);
+    out(c, Xu, w.X, g.nu); out(c, Xv, w.X + g.nu, g.nv); out(c, Xp, w.X + g.nu + g.nv, g.np);
+    sync(c);
+    if (iter_out) *iter_out = it;
+    if (res_out) *res_out = res;
+    return (res == res) ? 0 : FUGU_W_NONFINITE;
+  });
+}
+
+int fugu_bicgstab_solve(fugu_ctx* c, double* Xu, double* Xv, double* Xp, const double* Bu, const double* Bv,
+                        const double* Bp, const double* uadv, const double* vadv, const double* rho_c,
+                        const double* rho_u, const double* rho_v, const double* kap, const double* zeta,
+                        const double* eta, double dt, double tol, int max_iter, int* iter_out, double* res_out,
+                        const double* lid, const double* pu, const double* pv, const double* mu, const double* mv) {
+  return krylov_entry(c, 1, Xu, Xv, Xp, Bu, Bv, Bp, uadv, vadv, rho_c, rho_u, rho_v, kap, zeta, eta, dt, tol,
+                      max_iter, iter_out, res_out, lid, pu, pv, mu, mv, 0u);
+}
+
+int fugu_bicgstab2_solve(fugu_ctx* c, double* Xu, double* Xv, double* Xp, const double* Bu, const double* Bv,
+                         const double* Bp, const double* uadv, const double* vadv, const double* rho_c,
+                         const double* rho_u, const double* rho_v, const double* kap, const double* zeta,
+                         const double* eta, double dt, double tol, int max_iter, int* iter_out, double* res_out,
+                         const double* lid, const double* pu, const double* pv, const double* mu, const double* mv,
+                         unsigned flags) {
+  return krylov_entry(c, 2, Xu, Xv, Xp, Bu, Bv, Bp, uadv, vadv, rho_c, rho_u, rho_v, kap, zeta, eta, dt, tol,
+                      max_iter, iter_out, res_out, lid, pu, pv, mu, mv, flags);
+}
+
+int fugu_vof_advection_tvd(fugu_ctx* c, const double* cin, double* cout, const double* u, const double* v,
+                           double dt, double* fx, double* fy) {
+  NEED(c);
+  return guarded([&]() {
+    const Grid& g = c->g;
+    in(c, c->sp[0], cin, g.np); in(c, c->su[0], u, g.nu); in(c, c->sv[0], v, g.nv);
+    vof(c, c->sp[0], c->sp[1], c->su[0], c->sv[0], dt, fx ? c->su[1] : nullptr, fy ? c->sv[1] : nullptr);
+    out(c, cout, c->sp[1], g.np);
+    if (fx) out(c, fx, c->su[1], g.nu);
+    if (fy) out(c, fy, c->sv[1], g.nv);
+    sync(c);
+    return 0;
+  });
+}
+
+int fugu_svof_helmholtz_filter(fugu_ctx* c, const double* craw, double* cs, double delta, int max_iter,
+                               double tol, int* iter_out) {
+  NEED(c);
+  return guarded([&]() {
+    const Grid& g = c->g;
+    if (max_iter <= 0) max_iter = 100;
+    if (tol <= 0) tol = 1.0e-6;
+    in(c, c->sp[0], craw, g.np);
+    svof(c, c->sp[0], c->sp[1], delta, max_iter, tol, iter_out);
+    out(c, cs, c->sp[1], g.np);
+    sync(c);
+    return 0;
+  });
+}
+
+int fugu_compute_interface_curvature(fugu_ctx* c, const double* cs, double* kc) {
+  NEED(c);
+  return guarded([&]() {
+    const Grid& g = c->g;
+    in(c, c->sp[0], cs, g.np);
+    curvature(c, c->sp[0], c->sp[1]);
+    out(c, kc, c->sp[1], g.np);
+    sync(c);
+    return 0;
+  });
+}
+
+int fugu_compute_csf_force(fugu_ctx* c, const double* craw, const double* kc, double* fu, double* fv) {
+  NEED(c);
+  return guarded([&]() {
+    const Grid& g = c->g;
+    in(c, c->sp[0], craw, g.np); in(c, c->sp[1], kc, g.np);
+    csf(c, c->sp[0], c->sp[1], c->su[0], c->sv[0]);
+    out(c, fu, c->su[0], g.nu); out(c, fv, c->sv[0], g.nv);
+    sync(c);
+    return 0;
+  });
+}
+
+int fugu_compute_mixture_densities(fugu_ctx* c, const double* cc, double* rho_c, double* rho_u, double* rho_v) {
+  NEED(c);
+  return guarded([&]() {
+    const Grid& g = c->g;
+    in(c, c->sp[0], cc, g.np);
+    mixture(c, c->sp[0]);
+    out(c, rho_c, c->rho_c, g.np); out(c, rho_u, c->rho_u, g.nu); out(c, rho_v, c->rho_v, g.nv);
+    sync(c);
+    return 0;
+  });
+}
+
+int fugu_compute_mixture_viscosities(fugu_ctx* c, const double* cc, double* kap, double* zeta, double* eta) {
+  NEED(c);
+  return guarded([&]() {
+    const Grid& g = c->g;
+    in(c, c->sp[0], cc, g.np);
+    mixture(c, c->sp[0]);
+    out(c, kap, c->kap, g.np); out(c, zeta, c->zeta, g.nw); out(c, eta, c->eta, g.nw);
+    sync(c);
+    return 0;
+  });
+}
+
+int fugu_compute_adaptive_dt(fugu_ctx* c, const double* u, const double* v, double dt_max, double* dt_out) {
+  NEED(c);
+  return guarded([&]() {
+    const Grid& g = c->g;
+    const fugu_params& P = c->prm;
+    in(c, c->su[0], u, g.nu); in(c, c->sv[0], v, g.nv);
+    LAUNCH(k_absmax, grid_union(g), dim3(TX, TY), c->s, g, (const double*)c->su[0], (const double*)c->sv[0],
+           c->partials, c->ticket, c->sc);
+    read_slots(c);
+    const double um = c->h_sc[S_DOT0], vm = c->h_sc[S_DOT1];
+    const double pi = 3.141592653589793238462643383279502884197;
+    const double h = g.dx < g.dy ? g.dx : g.dy;
+    double dtc = (um / g.dx + vm / g.dy > 1.0e-12) ? P.cfl_target / (um / g.dx + vm / g.dy) : dt_max;
+    double dtl = dt_max < dtc ? dt_max : dtc;
+    if (P.sigma > 1.0e-14) {
+      const double cap = 0.5 * sqrt((P.rho1 + P.rho2) * (h * h * h) / (4.0 * pi * P.sigma));
+      dtl = cap < dtl ? cap : dtl;
+    }
+    const double gm = sqrt(P.gx * P.gx + P.gy * P.gy);
+    if (gm > 1.0e-14) {
+      const double dg = 0.5 * sqrt(h / gm);
+      dtl = dg < dtl ? dg : dtl;
+    }
+    if (dt_out) *dt_out = dtl;
+    return 0;
+  });
+}
+
+int fugu_state_upload(fugu_ctx* c, const double* u, const double* v, const double* u_prev, const double* v_prev,
+                      const double* p, const double* cc, const double* psu, const double* psv) {
+  NEED(c);
+  return guarded([&]() {
+    const Grid& g = c->g;
+    in(c, c->u, u, g.nu); in(c, c->v, v, g.nv); in(c, c->u_prev, u_prev, g.nu); in(c, c->v_prev, v_prev, g.nv);
+    in(c, c->p, p, g.np); in(c, c->c, cc, g.np);
+    c->has_solid = (psu && psv);
+    if (c->has_solid) { in(c, c->phis_u, psu, g.nu); in(c, c->phis_v, psv, g.nv); }
+    sync(c);
+    return 0;
+  });
+}
+
+int fugu_state_download(fugu_ctx* c, double* u, double* v, double* u_prev, double* v_prev, double* p, double* cc) {
+  NEED(c);
+  return guarded([&]() {
+    const Grid& g = c->g;
+    out(c, u, c->u, g.nu); out(c, v, c->v, g.nv); out(c, u_prev, c->u_prev, g.nu); out(c, v_prev, c->v_prev, g.nv);
+    out(c, p, c->p, g.np); out(c, cc, c->c, g.np);
+    sync(c);
+    return 0;
+  });
+}
+
+int fugu_field_download(fugu_ctx* c, const char* name, double* o) {
+  NEED(c);
+  return guarded([&]() {
+    const Grid& g = c->g;
+    struct { const char* n; double* p; size_t sz; } tab[] = {
+        {"u", c->u, g.nu}, {"v", c->v, g.nv}, {"p", c->p, g.np}, {"c", c->c, g.np},
+        {"u_prev", c->u_prev, g.nu}, {"v_prev", c->v_prev, g.nv}, {"rho_c", c->rho_c, g.np},
+        {"rho_u", c->rho_u, g.nu}, {"rho_v", c->rho_v, g.nv}, {"kappa", c->kap, g.np},
+        {"zeta", c->zeta, g.nw}, {"eta", c->eta, g.nw}, {"kappa_curv", c->kcurv, g.np},
+        {"f_sigma_u", c->fu, g.nu}, {"f_sigma_v", c->fv, g.nv}, {"u_adv", c->uadv, g.nu},
+        {"v_adv", c->vadv, g.nv}, {"c_smooth", c->cs, g.np}};
+    for (auto& t : tab)
+      if (std::strcmp(t.n, name) == 0) {
+        out(c, o, t.p, t.sz);
+        sync(c);
+        return 0;
+      }
+    throw Error(std::string("unknown field ") + name);
+  });
+}
+
+int fugu_state_init_properties(fugu_ctx* c) {
+  NEED(c);
+  return guarded([&]() {
+    mixture(c, c->c);
+    sync(c);
+    return 0;
+  });
+}
+
+int fugu_step(fugu_ctx* c, double dt, int step_idx, const fugu_step_opts* o, fugu_step_stats* st) {
+  NEED(c);
+  return guarded([&]() { return step_impl(c, dt, step_idx, o, st); });
+}
+
+int fugu_advance_coupled_step(fugu_ctx* c, double* u, double* v, double* p, double* cc, double* u_prev,
+                              double* v_prev, double* rho_c, double* rho_u, double* rho_v, double* kap,
+                              double* zeta, double* eta, double* kcurv, double* fu, double* fv, double dt,
+                              int step_idx, const fugu_step_opts* o, const double* psu, const double* psv,
+                              int* iter_out, double* res_out) {
+  NEED(c);
+  return guarded([&]() {
+    const Grid& g = c->g;
+    in(c, c->u, u, g.nu); in(c, c->v, v, g.nv); in(c, c->p, p, g.np); in(c, c->c, cc, g.np);
+    in(c, c->u_prev, u_prev, g.nu); in(c, c->v_prev, v_prev, g.nv);
+    c->has_solid = (psu && psv);
+    if (c->has_solid) { in(c, c->phis_u, psu, g.nu); in(c, c->phis_v, psv, g.nv); }
+    fugu_step_stats st{};
+    const int rc = step_impl(c, dt, step_idx, o, &st);
+    out(c, u, c->u, g.nu); out(c, v, c->v, g.nv); out(c, p, c->p, g.np); out(c, cc, c->c, g.np);
+    out(c, u_prev, c->u_prev, g.nu); out(c, v_prev, c->v_prev, g.nv);
+    out(c, rho_c, c->rho_c, g.np); out(c, rho_u, c->rho_u, g.nu); out(c, rho_v, c->rho_v, g.nv);
+    out(c, kap, c->kap, g.np); out(c, zeta, c->zeta, g.nw); out(c, eta, c->eta, g.nw);
+    out(c, kcurv, c->kcurv, g.np); out(c, fu, c->fu, g.nu); out(c, fv, c->fv, g.nv);
+    sync(c);
+    if (iter_out) *iter_out = st.iters;
+    if (res_out) *res_out = st.res;
+    return rc;
+  });
+}
+
+}  // extern "C"

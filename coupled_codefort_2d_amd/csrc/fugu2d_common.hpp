@@ -1,0 +1,316 @@
+// fugu2d_common.hpp -- device-side grid geometry, padded-array indexing, launch and
+// deterministic-reduction helpers shared by every FUGU-2D HIP kernel (gfx950).
+//
+// Layout: the reference's Fortran padded column-major arrays are kept byte for byte in
+// HBM (i fastest), so a host array crosses the C-ABI with one copy and no transpose:
+//   u-kind  (0-nh:nx+nh, 1-nh:ny+nh)   ld = nx+1+2nh   (x-face velocities, rho_u, ...)
+//   v-kind  (1-nh:nx+nh, 0-nh:ny+nh)   ld = nx+2nh     (y-face velocities, rho_v, ...)
+//   p-kind  (1-nh:nx+nh, 1-nh:ny+nh)   ld = nx+2nh     (cell centres: p, C, rho_c, kappa)
+//   w-kind  (0-nh:nx+nh, 0-nh:ny+nh)   ld = nx+1+2nh   (vertices: zeta, eta)
+// A Krylov "triplet" (u, v, p) is one contiguous allocation [u | v | p] so that the
+// reference's whole-array updates are single streaming passes.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cstddef>
+#include <cstdint>
+
+namespace fugu {
+
+struct Grid {
+  int nx, ny, nh;
+  int per_x, per_y;
+  int ldu, ldv, ldp, ldw;     // leading dimensions
+  int ru, rv, rp, rw;         // number of rows of each kind
+  double dx, dy, inv_dx, inv_dy, inv_dx2, inv_dy2, vol;
+  size_t nu, nv, np, nw;      // element counts
+};
+
+inline Grid make_grid(int nx, int ny, int nh, double dx, double dy, int per_x, int per_y) {
+  Grid g;
+  g.nx = nx; g.ny = ny; g.nh = nh; g.per_x = per_x; g.per_y = per_y;
+  g.ldu = nx + 1 + 2 * nh; g.ldv = nx + 2 * nh; g.ldp = nx + 2 * nh; g.ldw = nx + 1 + 2 * nh;
+  g.ru = ny + 2 * nh; g.rv = ny + 1 + 2 * nh; g.rp = ny + 2 * nh; g.rw = ny + 1 + 2 * nh;
+  g.dx = dx; g.dy = dy;
+  g.inv_dx = 1.0 / dx; g.inv_dy = 1.0 / dy;                 // src/mod_operators.f90:43-44
+  g.inv_dx2 = 1.0 / (dx * dx); g.inv_dy2 = 1.0 / (dy * dy); // src/mod_subsolvers.f90:58-59
+  g.vol = dx * dy;                                          // src/mod_bicgstab2.f90:41
+  g.nu = (size_t)g.ldu * g.ru; g.nv = (size_t)g.ldv * g.rv;
+  g.np = (size_t)g.ldp * g.rp; g.nw = (size_t)g.ldw * g.rw;
+  return g;
+}
+
+// Fortran (i, j) -> linear offset, one per field kind.
+__host__ __device__ inline size_t IU(const Grid& g, int i, int j) { return (size_t)(i + g.nh) + (size_t)(j - 1 + g.nh) * g.ldu; }
+__host__ __device__ inline size_t IV(const Grid& g, int i, int j) { return (size_t)(i - 1 + g.nh) + (size_t)(j + g.nh) * g.ldv; }
+__host__ __device__ inline size_t IP(const Grid& g, int i, int j) { return (size_t)(i - 1 + g.nh) + (size_t)(j - 1 + g.nh) * g.ldp; }
+__host__ __device__ inline size_t IW(const Grid& g, int i, int j) { return (size_t)(i + g.nh) + (size_t)(j + g.nh) * g.ldw; }
+
+// Thread tile: 64 consecutive i (one wavefront, coalesced) x 4 rows.
+constexpr int TX = 64;
+constexpr int TY = 4;
+constexpr int NT = TX * TY;
+
+// Launch over the union of all padded kinds: i in [-nh, nx+nh], j in [-nh, ny+nh].
+inline dim3 grid_union(const Grid& g) {
+  return dim3((unsigned)((g.ldw + TX - 1) / TX), (unsigned)((g.rw + TY - 1) / TY));
+}
+// Launch over interior cells i in [1, nx], j in [1, ny] (plus an optional margin m).
+inline dim3 grid_cells(const Grid& g, int m = 0) {
+  return dim3((unsigned)((g.nx + 2 * m + TX - 1) / TX), (unsigned)((g.ny + 2 * m + TY - 1) / TY));
+}
+
+// XCD-aware tile order: consecutive workgroup ids are dealt round-robin to the 8 XCDs;
+// remap them so each XCD walks a contiguous band of rows and reuses its own L2.
+__device__ inline void tile_coords(int& bx, int& by) {
+  const int gx = gridDim.x, gy = gridDim.y;
+  const int nb = gx * gy;
+  const int bid = blockIdx.x + blockIdx.y * gx;
+  int t = bid;
+  if (nb >= 64 && (nb % 8) == 0) {
+    const int per = nb / 8;
+    t = (bid % 8) * per + bid / 8;
+  }
+  bx = t % gx;
+  by = t / gx;
+}
+
+// Union-grid point of this thread as Fortran (i, j), with i in [-nh..], j in [-nh..].
+__device__ inline void union_ij(const Grid& g, int& i, int& j) {
+  int bx, by;
+  tile_coords(bx, by);
+  i = bx * TX + (int)threadIdx.x - g.nh;
+  j = by * TY + (int)threadIdx.y - g.nh;
+}
+// Interior-cell point (i, j) in [1-m, nx+m] x [1-m, ny+m].
+__device__ inline void cell_ij(int m, int& i, int& j) {
+  int bx, by;
+  tile_coords(bx, by);
+  i = bx * TX + (int)threadIdx.x + 1 - m;
+  j = by * TY + (int)threadIdx.y + 1 - m;
+}
+
+// ---------------------------------------------------------------------------------
+// Deterministic grid reductions: wave shuffle -> LDS -> one partial per workgroup ->
+// the last workgroup to finish (ticket counter) folds the partials in a fixed order.
+// Result is bitwise reproducible for a given launch shape.
+
+__device__ inline double wave_sum(double v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+__device__ inline double wave_max(double v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
+  return v;
+}
+
+// Sum NV values over the workgroup; thread 0 gets the totals.
+template <int NV>
+__device__ inline void block_sum(double (&v)[NV], double (*lds)[NV]) {
+  const int tid = threadIdx.x + threadIdx.y * blockDim.x;
+  const int lane = tid & 63, wid = tid >> 6;
+  const int nw = (blockDim.x * blockDim.y + 63) >> 6;
+#pragma unroll
+  for (int k = 0; k < NV; ++k) v[k] = wave_sum(v[k]);
+  if (lane == 0) {
+#pragma unroll
+    for (int k = 0; k < NV; ++k) lds[wid][k] = v[k];
+  }
+  __syncthreads();
+  if (tid == 0) {
+#pragma unroll
+    for (int k = 0; k < NV; ++k) {
+      double s = lds[0][k];
+      for (int w = 1; w < nw; ++w) s += lds[w][k];
+      v[k] = s;
+    }
+  }
+  __syncthreads();
+}
+
+template <int NV>
+__device__ inline void block_max(double (&v)[NV], double (*lds)[NV]) {
+  const int tid = threadIdx.x + threadIdx.y * blockDim.x;
+  const int lane = tid & 63, wid = tid >> 6;
+  const int nw = (blockDim.x * blockDim.y + 63) >> 6;
+#pragma unroll
+  for (int k = 0; k < NV; ++k) v[k] = wave_max(v[k]);
+  if (lane == 0) {
+#pragma unroll
+    for (int k = 0; k < NV; ++k) lds[wid][k] = v[k];
+  }
+  __syncthreads();
+  if (tid == 0) {
+#pragma unroll
+    for (int k = 0; k < NV; ++k) {
+      double s = lds[0][k];
+      for (int w = 1; w < nw; ++w) s = fmax(s, lds[w][k]);
+      v[k] = s;
+    }
+  }
+  __syncthreads();
+}
+
+// Publish this workgroup's NV partials; returns true in exactly one workgroup (the
+// last to arrive), in which all threads then hold the folded totals in `tot`.
+// `partials` holds nblocks*NV doubles, `ticket` one counter that is reset here.
+template <int NV, bool MAX = false>
+__device__ inline bool grid_fold(double (&v)[NV], double* partials, unsigned* ticket,
+                                 double (&tot)[NV]) {
+  __shared__ double lds[16][NV];
+  __shared__ bool last;
+  const int tid = threadIdx.x + threadIdx.y * blockDim.x;
+  const int nthr = blockDim.x * blockDim.y;
+  const int nb = gridDim.x * gridDim.y;
+  const int bid = blockIdx.x + blockIdx.y * gridDim.x;
+  if (MAX) block_max<NV>(v, lds); else block_sum<NV>(v, lds);
+  if (tid == 0) {
+#pragma unroll
+    for (int k = 0; k < NV; ++k) partials[(size_t)bid * NV + k] = v[k];
+    __threadfence();
+    const unsigned t = atomicAdd(ticket, 1u);
+    last = (t == (unsigned)(nb - 1));
+  }
+  __syncthreads();
+  if (!last) return false;
+  __threadfence();
+  double acc[NV];
+#pragma unroll
+  for (int k = 0; k < NV; ++k) acc[k] = MAX ? 0.0 : 0.0;
+  for (int b = tid; b < nb; b += nthr) {
+#pragma unroll
+    for (int k = 0; k < NV; ++k) {
+      const double x = __builtin_nontemporal_load(&partials[(size_t)b * NV + k]);
+      acc[k] = MAX ? fmax(acc[k], x) : acc[k] + x;
+    }
+  }
+  if (MAX) block_max<NV>(acc, lds); else block_sum<NV>(acc, lds);
+  __shared__ double out[NV];
+  if (tid == 0) {
+#pragma unroll
+    for (int k = 0; k < NV; ++k) out[k] = acc[k];
+    *ticket = 0u;
+  }
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < NV; ++k) tot[k] = out[k];
+  return true;
+}
+
+// ---------------------------------------------------------------------------------
+// Compensated (double-double) reductions.  Every FP64 sum of the solver is accumulated
+// as an unevaluated pair (hi, lo) with TwoSum, so the final fl(hi + lo) is the correctly
+// rounded sum of the (bitwise reproducible) terms whatever the summation order: the
+// GPU tree and the oracle's sequential loop return the same double.  Needs
+// -ffp-contract=off (no FMA inside TwoSum).
+struct DD { double hi, lo; };
+
+__host__ __device__ inline DD dd_add(DD a, DD b) {
+  const double s = a.hi + b.hi;
+  const double v = s - a.hi;
+  double e = (a.hi - (s - v)) + (b.hi - v);
+  e = e + (a.lo + b.lo);
+  DD r;
+  r.hi = s + e;
+  r.lo = e - (r.hi - s);
+  return r;
+}
+__host__ __device__ inline DD dd_from(double x) { DD r; r.hi = x; r.lo = 0.0; return r; }
+__host__ __device__ inline double dd_val(DD a) { return a.hi + a.lo; }
+
+__device__ inline DD wave_dd(DD v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    DD w;
+    w.hi = __shfl_xor(v.hi, o, 64);
+    w.lo = __shfl_xor(v.lo, o, 64);
+    // combine in lane order so both lanes of a pair hold the same value
+    const int lane = (int)((threadIdx.x + threadIdx.y * blockDim.x) & 63);
+    v = ((lane & o) == 0) ? dd_add(v, w) : dd_add(w, v);
+  }
+  return v;
+}
+
+template <int NV>
+__device__ inline void block_dd(DD (&v)[NV], DD (*lds)[NV]) {
+  const int tid = threadIdx.x + threadIdx.y * blockDim.x;
+  const int lane = tid & 63, wid = tid >> 6;
+  const int nw = (blockDim.x * blockDim.y + 63) >> 6;
+#pragma unroll
+  for (int k = 0; k < NV; ++k) v[k] = wave_dd(v[k]);
+  if (lane == 0) {
+#pragma unroll
+    for (int k = 0; k < NV; ++k) lds[wid][k] = v[k];
+  }
+  __syncthreads();
+  if (tid == 0) {
+#pragma unroll
+    for (int k = 0; k < NV; ++k) {
+      DD s = lds[0][k];
+      for (int w = 1; w < nw; ++w) s = dd_add(s, lds[w][k]);
+      v[k] = s;
+    }
+  }
+  __syncthreads();
+}
+
+// As grid_fold, with compensated sums of per-lane double-double values; `tot` receives
+// fl(hi + lo) of each total.  `partials` holds nblocks*NV*2 doubles.
+template <int NV>
+__device__ inline bool grid_fold_dd2(DD (&v)[NV], double* partials, unsigned* ticket, double (&tot)[NV]) {
+  __shared__ DD lds[16][NV];
+  __shared__ bool last;
+  const int tid = threadIdx.x + threadIdx.y * blockDim.x;
+  const int nthr = blockDim.x * blockDim.y;
+  const int nb = gridDim.x * gridDim.y;
+  const int bid = blockIdx.x + blockIdx.y * gridDim.x;
+  block_dd<NV>(v, lds);
+  if (tid == 0) {
+#pragma unroll
+    for (int k = 0; k < NV; ++k) {
+      partials[((size_t)bid * NV + k) * 2] = v[k].hi;
+      partials[((size_t)bid * NV + k) * 2 + 1] = v[k].lo;
+    }
+    __threadfence();
+    const unsigned t = atomicAdd(ticket, 1u);
+    last = (t == (unsigned)(nb - 1));
+  }
+  __syncthreads();
+  if (!last) return false;
+  __threadfence();
+  DD acc[NV];
+#pragma unroll
+  for (int k = 0; k < NV; ++k) acc[k] = dd_from(0.0);
+  for (int b = tid; b < nb; b += nthr) {
+#pragma unroll
+    for (int k = 0; k < NV; ++k) {
+      DD p;
+      p.hi = __builtin_nontemporal_load(&partials[((size_t)b * NV + k) * 2]);
+      p.lo = __builtin_nontemporal_load(&partials[((size_t)b * NV + k) * 2 + 1]);
+      acc[k] = dd_add(acc[k], p);
+    }
+  }
+  block_dd<NV>(acc, lds);
+  __shared__ double out[NV];
+  if (tid == 0) {
+#pragma unroll
+    for (int k = 0; k < NV; ++k) out[k] = dd_val(acc[k]);
+    *ticket = 0u;
+  }
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < NV; ++k) tot[k] = out[k];
+  return true;
+}
+
+template <int NV>
+__device__ inline bool grid_fold_dd(const double (&x)[NV], double* partials, unsigned* ticket,
+                                    double (&tot)[NV]) {
+  DD v[NV];
+#pragma unroll
+  for (int k = 0; k < NV; ++k) v[k] = dd_from(x[k]);
+  return grid_fold_dd2<NV>(v, partials, ticket, tot);
+}
+
+}  // namespace fugu

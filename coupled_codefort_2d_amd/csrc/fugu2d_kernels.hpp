@@ -1,0 +1,1173 @@
+// fugu2d_kernels.hpp -- HIP kernels of the FUGU-2D per-step hot path (gfx950).
+//
+// Every kernel is a memory-bound stencil or stream: no MFMA (nothing here is a dense
+// contraction).  FP64 kernels are compiled with -ffp-contract=off and keep the
+// reference's operation order, so element-wise results are bit-identical to the
+// reference arithmetic; reductions are deterministic (fixed-shape tree).  The same
+// templates instantiated with T = float are the inner kernels of the mixed solve
+// (accumulation in FP64).
+#pragma once
+#include "fugu2d_common.hpp"
+
+namespace fugu {
+
+// ---------------------------------------------------------------------------------
+// Device-resident solver scalars (one array per context).  Slots are written only by
+// the last workgroup of a reduction kernel (thread 0) or by single-thread kernels.
+enum Slot : int {
+  S_NORM_B = 0, S_NORM_R, S_REL_RES, S_RHO_OLD, S_ALPHA, S_BETA, S_OMEGA,
+  S_XI0, S_GAMMA, S_OM1, S_OM2, S_TOL, S_SCALE,
+  // PCG (inner pressure solve)
+  S_P_RZ, S_P_PAP, S_P_ALPHA, S_P_BETA, S_P_RHSN, S_P_RES, S_P_MEAN, S_P_TOL,
+  // generic reduction outputs
+  S_DOT0, S_DOT1, S_DOT2, S_DOT3, S_DOT4, S_MAX0,
+  S_COUNT
+};
+enum Flag : int {
+  F_BRK = 0,     // exit in the middle of a Krylov iteration: skip everything after
+  F_DONE,        // no further Krylov iteration
+  F_ITER,        // iteration number reported by the Krylov solver
+  F_P_DONE,      // PCG: 1 = stopped (tolerance / breakdown), 2 = zero-rhs early exit
+  F_P_ITER,      // PCG iteration count
+  F_S_DONE,      // SVOF converged
+  F_S_ITER,
+  F_COUNT
+};
+
+// Reduction phases: scalar logic run by the last workgroup of a dot kernel.  Each one
+// restates a few lines of src/mod_bicgstab2.f90 (cited at the case).
+enum Phase : int {
+  PH_NONE = 0,
+  PH_NORM_B,      // :147-148 / :345-346
+  PH_B1_INIT,     // :173-175   tot = {(r0,r), (r,r)}
+  PH_B1_RV,       // :196-198
+  PH_B1_TT_TS,    // :217-223   tot = {(t,t), (t,s)}
+  PH_B1_RES,      // :233-242   tot = {(r,r), (r0,r)}
+  PH_B2_INIT,     // :380-385   tot = {(r,r)}
+  PH_B2_XI_EVEN,  // :387-401   tot = {(rt,r)}
+  PH_B2_GAMMA,    // :421-423 / :472-474
+  PH_B2_XI_ODD,   // :449-452   tot = {(rt,s)}
+  PH_B2_GCR,      // :504-530   tot = {(r,s),(s,s),(s,t),(t,t),(r,t)}
+  PH_B2_RES,      // :546-547 (+ the loop-top test :385 of the next iteration)
+  PH_STORE        // plain: sc[S_DOT0..] = tot
+};
+
+__device__ inline void krylov_phase(int phase, int k, int max_iter, double* sc, int* fl,
+                                    const double* tot) {
+  switch (phase) {
+    case PH_NORM_B: {
+      double nb = sqrt(tot[0]);
+      if (nb < 1.0e-14) nb = 1.0;
+      sc[S_NORM_B] = nb;
+    } break;
+    case PH_B1_INIT: {
+      sc[S_RHO_OLD] = tot[0];
+      sc[S_NORM_R] = sqrt(tot[1]);
+      sc[S_REL_RES] = sc[S_NORM_R] / sc[S_NORM_B];
+      sc[S_ALPHA] = 0.0;
+      sc[S_OMEGA] = 1.0;
+      if (sc[S_REL_RES] < sc[S_TOL]) { fl[F_DONE] = 1; fl[F_ITER] = 1; }
+    } break;
+    case PH_B1_RV: {
+      const double rv = tot[0];
+      if (fabs(rv) < 1.0e-30) { fl[F_BRK] = 1; fl[F_DONE] = 1; fl[F_ITER] = k; break; }
+      sc[S_ALPHA] = sc[S_RHO_OLD] / rv;
+    } break;
+    case PH_B1_TT_TS: {
+      const double tt = tot[0], ts = tot[1];
+      sc[S_OMEGA] = (tt > 1.0e-30) ? ts / tt : 0.0;
+    } break;
+    case PH_B1_RES: {
+      sc[S_NORM_R] = sqrt(tot[0]);
+      sc[S_REL_RES] = sc[S_NORM_R] / sc[S_NORM_B];
+      if (sc[S_REL_RES] < sc[S_TOL] || fabs(sc[S_OMEGA]) < 1.0e-30) {
+        fl[F_BRK] = 1; fl[F_DONE] = 1; fl[F_ITER] = k; break;
+      }
+      const double rho_new = tot[1];
+      if (fabs(sc[S_RHO_OLD]) < 1.0e-30) { fl[F_BRK] = 1; fl[F_DONE] = 1; fl[F_ITER] = k; break; }
+      sc[S_BETA] = (rho_new / sc[S_RHO_OLD]) * (sc[S_ALPHA] / sc[S_OMEGA]);
+      sc[S_RHO_OLD] = rho_new;
+      if (k == max_iter) { fl[F_DONE] = 1; fl[F_ITER] = max_iter; }
+    } break;
+    case PH_B2_INIT: {
+      sc[S_XI0] = 1.0; sc[S_OM1] = 0.0; sc[S_OM2] = 1.0; sc[S_ALPHA] = 0.0;
+      sc[S_NORM_R] = sqrt(tot[0]);
+      sc[S_REL_RES] = sc[S_NORM_R] / sc[S_NORM_B];
+      if (sc[S_REL_RES] < sc[S_TOL]) { fl[F_DONE] = 1; fl[F_ITER] = 1; }
+    } break;
+    case PH_B2_XI_EVEN: {
+      double xi0 = sc[S_XI0];
+      if (fabs(sc[S_OM2]) > 1.0e-14) xi0 = -sc[S_OM2] * xi0;
+      else if (fabs(sc[S_OM1]) > 1.0e-14) xi0 = -sc[S_OM1] * xi0;
+      else xi0 = -xi0;
+      const double xi1 = tot[0];
+      if (fabs(xi0) < 1.0e-30) { sc[S_XI0] = xi0; fl[F_BRK] = 1; fl[F_DONE] = 1; fl[F_ITER] = k; break; }
+      sc[S_BETA] = sc[S_ALPHA] * (xi1 / xi0);
+      sc[S_XI0] = xi1;
+    } break;
+    case PH_B2_GAMMA: {
+      const double gam = tot[0];
+      if (fabs(gam) < 1.0e-30) { fl[F_BRK] = 1; fl[F_DONE] = 1; fl[F_ITER] = k; break; }
+      sc[S_ALPHA] = sc[S_XI0] / gam;
+    } break;
+    case PH_B2_XI_ODD: {
+      const double xi1 = tot[0];
+      if (fabs(sc[S_XI0]) < 1.0e-30) { fl[F_BRK] = 1; fl[F_DONE] = 1; fl[F_ITER] = k; break; }
+      sc[S_BETA] = sc[S_ALPHA] * (xi1 / sc[S_XI0]);
+      sc[S_XI0] = xi1;
+    } break;
+    case PH_B2_GCR: {
+      double om1 = tot[0];
+      const double lam = tot[1], nu = tot[2], tau = tot[3];
+      double om2 = tot[4];
+      if (lam * tau > 1.0e-30) {
+        const double det = lam * tau - nu * nu;
+        if (det > 1.0e-14 * (lam * tau)) {
+          const double c1 = om1, c2 = om2;
+          om1 = (c1 * tau - c2 * nu) / det;
+          om2 = (c2 * lam - c1 * nu) / det;
+        } else if (lam > 1.0e-30) {
+          om1 = om1 / lam; om2 = 0.0;
+        } else {
+          om1 = 0.0; om2 = 0.0;
+        }
+      } else if (lam > 1.0e-30) {
+        om1 = om1 / lam; om2 = 0.0;
+      } else {
+        om1 = 0.0; om2 = 0.0;
+      }
+      sc[S_OM1] = om1; sc[S_OM2] = om2;
+    } break;
+    case PH_B2_RES: {
+      sc[S_NORM_R] = sqrt(tot[0]);
+      sc[S_REL_RES] = sc[S_NORM_R] / sc[S_NORM_B];
+      // the exit test sits at the top of the next iteration (:385): X still gets its BCs
+      if (sc[S_REL_RES] < sc[S_TOL]) { fl[F_DONE] = 1; fl[F_ITER] = (k + 1 < max_iter) ? k + 1 : max_iter; }
+      else if (k == max_iter) { fl[F_DONE] = 1; fl[F_ITER] = max_iter; }
+    } break;
+    case PH_STORE:
+    default:
+      for (int q = 0; q < 5; ++q) sc[S_DOT0 + q] = tot[q];
+      break;
+  }
+}
+
+// ---------------------------------------------------------------------------------
+// Ghost fills.  Periodic halo exchange at one rank (src/mod_mpi.f90:113-245): x pass
+// over full columns then y pass over full rows, composed so every target reads only
+// non-target points (race free in one launch).  kind: 0 = u, 1 = v, 2 = p.
+
+template <typename T>
+__global__ void k_halo(Grid g, T* __restrict__ a, int kind, const int* __restrict__ skip) {
+  if (skip && *skip) return;
+  int i, j;
+  union_ij(g, i, j);
+  const int nx = g.nx, ny = g.ny, nh = g.nh;
+  // index ranges of the kind
+  const int i_lo = (kind == 0) ? -nh : 1 - nh, i_hi = nx + nh;
+  const int j_lo = (kind == 1) ? -nh : 1 - nh, j_hi = ny + nh;
+  if (i < i_lo || i > i_hi || j < j_lo || j > j_hi) return;
+  auto at = [&](int ii, int jj) -> size_t {
+    return kind == 0 ? IU(g, ii, jj) : (kind == 1 ? IV(g, ii, jj) : IP(g, ii, jj));
+  };
+  // x pass targets: columns 1-nh..0 and nx+1..nx+nh (u: -1..0, so u(-nh) untouched)
+  auto xsrc = [&](int ii) -> int {
+    if (!g.per_x) return ii;
+    if (ii >= 1 - nh && ii <= 0) return ii + nx;
+    if (ii >= nx + 1 && ii <= nx + nh) return ii - nx;
+    return ii;
+  };
+  auto ysrc = [&](int jj) -> int {
+    if (!g.per_y) return jj;
+    if (jj >= 1 - nh && jj <= 0) return jj + ny;
+    if (jj >= ny + 1 && jj <= ny + nh) return jj - ny;
+    return jj;
+  };
+  const int si = xsrc(i), sj = ysrc(j);
+  if (si == i && sj == j) return;
+  a[at(i, j)] = a[at(si, sj)];
+}
+
+// Homogeneous / lid velocity BCs (src/mod_operators.f90:254-333).  x walls first, then
+// y walls; the y-pass value of a corner is formed from the x-pass value of its source.
+template <typename T>
+__device__ inline T u_after_x(const Grid& g, const T* u, int i, int j) {
+  if (g.per_x) return u[IU(g, i, j)];
+  const int nx = g.nx, nh = g.nh;
+  if (i == 0 || i == nx) return T(0);
+  if (i < 0 && i >= -nh) return -u[IU(g, -i, j)];
+  if (i > nx && i <= nx + nh) return -u[IU(g, 2 * nx - i, j)];
+  return u[IU(g, i, j)];
+}
+template <typename T>
+__device__ inline T v_after_x(const Grid& g, const T* v, int i, int j) {
+  if (g.per_x) return v[IV(g, i, j)];
+  const int nx = g.nx, nh = g.nh;
+  if (i <= 0 && i >= 1 - nh) return -v[IV(g, 1 - i, j)];
+  if (i > nx && i <= nx + nh) return -v[IV(g, 2 * nx + 1 - i, j)];
+  return v[IV(g, i, j)];
+}
+
+template <typename T>
+__global__ void k_velocity_bcs(Grid g, T* __restrict__ u, T* __restrict__ v, double u_lid,
+                               const int* __restrict__ skip) {
+  if (skip && *skip) return;
+  int i, j;
+  union_ij(g, i, j);
+  const int nx = g.nx, ny = g.ny, nh = g.nh;
+  // u target
+  if (i >= -nh && i <= nx + nh && j >= 1 - nh && j <= ny + nh) {
+    const bool xt = !g.per_x && (i <= 0 || i >= nx);
+    const bool yt = !g.per_y && (j <= 0 || j >= ny + 1);
+    if (yt) {
+      T val;
+      if (j <= 0) val = -u_after_x(g, u, i, 1 - j);
+      else val = (T)(2.0 * u_lid) - u_after_x(g, u, i, 2 * ny + 1 - j);
+      u[IU(g, i, j)] = val;
+    } else if (xt) {
+      u[IU(g, i, j)] = u_after_x(g, u, i, j);
+    }
+  }
+  // v target
+  if (i >= 1 - nh && i <= nx + nh && j >= -nh && j <= ny + nh) {
+    const bool xt = !g.per_x && (i <= 0 || i >= nx + 1);
+    const bool yt = !g.per_y && (j <= 0 || j >= ny);
+    if (yt) {
+      T val;
+      if (j == 0 || j == ny) val = T(0);
+      else if (j < 0) val = -v_after_x(g, v, i, -j);
+      else val = -v_after_x(g, v, i, 2 * ny - j);
+      v[IV(g, i, j)] = val;
+    } else if (xt) {
+      v[IV(g, i, j)] = v_after_x(g, v, i, j);
+    }
+  }
+}
+
+// Neumann mirror of a cell field (src/mod_subsolvers.f90:378-416, src/mod_vof.f90:199-237):
+// x walls then y walls, composed.
+template <typename T>
+__global__ void k_mirror_p(Grid g, T* __restrict__ a) {
+  int i, j;
+  union_ij(g, i, j);
+  const int nx = g.nx, ny = g.ny, nh = g.nh;
+  if (i < 1 - nh || i > nx + nh || j < 1 - nh || j > ny + nh) return;
+  int si = i, sj = j;
+  if (!g.per_x) {
+    if (i <= 0) si = 1 - i;
+    else if (i > nx) si = 2 * nx + 1 - i;
+  }
+  if (!g.per_y) {
+    if (j <= 0) sj = 1 - j;
+    else if (j > ny) sj = 2 * ny + 1 - j;
+  }
+  if (si == i && sj == j) return;
+  a[IP(g, i, j)] = a[IP(g, si, sj)];
+}
+
+// ---------------------------------------------------------------------------------
+// H1: coupled matvec Y = A X (src/mod_operators.f90:341-443 with :99-162, :60-88,
+// :168-249, :34-53).  One pass over the union grid writes Y_u, Y_v, Y_p including
+// zeroed ghosts; the stress tensors are recomputed per use (same arithmetic as the
+// reference's temporaries).  The periodic halo of X is done before (k_halo).
+enum MvFlags : int { MV_ADV = 1, MV_PEN = 2, MV_CONS = 4 };
+
+template <typename T>
+struct MatvecArgs {
+  const T *Xu, *Xv, *Xp;
+  T *Yu, *Yv, *Yp;
+  const T *uadv, *vadv, *rho_c, *rho_u, *rho_v, *kap, *zeta, *eta;
+  const T *penu, *penv, *mdu, *mdv;
+  T dt;
+  int flags;
+};
+
+template <typename T>
+__global__ void __launch_bounds__(NT) k_matvec(Grid g, MatvecArgs<T> a, const int* __restrict__ skip) {
+  if (skip && *skip) return;
+  int i, j;
+  union_ij(g, i, j);
+  const int nx = g.nx, ny = g.ny, nh = g.nh;
+  const T inv_dx = (T)g.inv_dx, inv_dy = (T)g.inv_dy, half = T(0.5);
+  const int i_u_max = g.per_x ? nx : nx - 1;
+  const int j_v_max = g.per_y ? ny : ny - 1;
+  const bool with_adv = (a.flags & MV_ADV) != 0;
+  const bool cons = (a.flags & MV_CONS) != 0;
+  const T* Xu = a.Xu;
+  const T* Xv = a.Xv;
+
+  auto Txx = [&](int ii, int jj) -> T { return a.kap[IP(g, ii, jj)] * (Xu[IU(g, ii, jj)] - Xu[IU(g, ii - 1, jj)]) * inv_dx; };
+  auto Tyy = [&](int ii, int jj) -> T { return a.kap[IP(g, ii, jj)] * (Xv[IV(g, ii, jj)] - Xv[IV(g, ii, jj - 1)]) * inv_dy; };
+  auto dudy = [&](int ii, int jj) -> T { return (Xu[IU(g, ii, jj + 1)] - Xu[IU(g, ii, jj)]) * inv_dy; };
+  auto dvdx = [&](int ii, int jj) -> T { return (Xv[IV(g, ii + 1, jj)] - Xv[IV(g, ii, jj)]) * inv_dx; };
+  auto Txy = [&](int ii, int jj) -> T {
+    const T ze = a.zeta[IW(g, ii, jj)], et = a.eta[IW(g, ii, jj)];
+    return (ze - et) * dudy(ii, jj) + et * dvdx(ii, jj);
+  };
+  auto Tyx = [&](int ii, int jj) -> T {
+    const T ze = a.zeta[IW(g, ii, jj)], et = a.eta[IW(g, ii, jj)];
+    return et * dudy(ii, jj) + (ze - et) * dvdx(ii, jj);
+  };
+
+  // ---- u rows
+  if (i >= -nh && i <= nx + nh && j >= 1 - nh && j <= ny + nh) {
+    T y = T(0);
+    if (j >= 1 && j <= ny && i >= 1 && i <= i_u_max) {
+      const T visc = -((Txx(i + 1, j) - Txx(i, j)) * inv_dx + (Txy(i, j) - Txy(i, j - 1)) * inv_dy);
+      const T grad = (a.Xp[IP(g, i + 1, j)] - a.Xp[IP(g, i, j)]) * inv_dx;
+      T adv = T(0);
+      if (with_adv) {
+        T mE, mW, mN, mS;
+        if (cons) {
+          mE = half * (a.mdu[IU(g, i, j)] + a.mdu[IU(g, i + 1, j)]);
+          mW = half * (a.mdu[IU(g, i - 1, j)] + a.mdu[IU(g, i, j)]);
+          mN = half * (a.mdv[IV(g, i, j)] + a.mdv[IV(g, i + 1, j)]);
+          mS = half * (a.mdv[IV(g, i, j - 1)] + a.mdv[IV(g, i + 1, j - 1)]);
+        } else {
+          mE = a.rho_c[IP(g, i + 1, j)] * half * (a.uadv[IU(g, i, j)] + a.uadv[IU(g, i + 1, j)]);
+          mW = a.rho_c[IP(g, i, j)] * half * (a.uadv[IU(g, i - 1, j)] + a.uadv[IU(g, i, j)]);
+          mN = half * (a.rho_v[IV(g, i, j)] + a.rho_v[IV(g, i + 1, j)]) * half *
+               (a.vadv[IV(g, i, j)] + a.vadv[IV(g, i + 1, j)]);
+          mS = half * (a.rho_v[IV(g, i, j - 1)] + a.rho_v[IV(g, i + 1, j - 1)]) * half *
+               (a.vadv[IV(g, i, j - 1)] + a.vadv[IV(g, i + 1, j - 1)]);
+        }
+        const T uc = Xu[IU(g, i, j)];
+        const T uE = half * (uc + Xu[IU(g, i + 1, j)]);
+        const T uW = half * (Xu[IU(g, i - 1, j)] + uc);
+        const T uN = half * (uc + Xu[IU(g, i, j + 1)]);
+        const T uS = half * (Xu[IU(g, i, j - 1)] + uc);
+        adv = (mE * uE - mW * uW) * inv_dx + (mN * uN - mS * uS) * inv_dy;
+      }
+      const T x = Xu[IU(g, i, j)];
+      y = (a.rho_u[IU(g, i, j)] / a.dt) * x + adv + visc + grad;
+      if (a.flags & MV_PEN) y = y + a.penu[IU(g, i, j)] * x;
+    } else if (!g.per_x && j >= 1 && j <= ny && (i == 0 || i == nx)) {
+      y = Xu[IU(g, i, j)];
+    }
+    a.Yu[IU(g, i, j)] = y;
+  }
+  // ---- v rows
+  if (i >= 1 - nh && i <= nx + nh && j >= -nh && j <= ny + nh) {
+    T y = T(0);
+    if (j >= 1 && j <= j_v_max && i >= 1 && i <= nx) {
+      const T visc = -((Tyx(i, j) - Tyx(i - 1, j)) * inv_dx + (Tyy(i, j + 1) - Tyy(i, j)) * inv_dy);
+      const T grad = (a.Xp[IP(g, i, j + 1)] - a.Xp[IP(g, i, j)]) * inv_dy;
+      T adv = T(0);
+      if (with_adv) {
+        T mE, mW, mN, mS;
+        if (cons) {
+          mE = half * (a.mdu[IU(g, i, j)] + a.mdu[IU(g, i, j + 1)]);
+          mW = half * (a.mdu[IU(g, i - 1, j)] + a.mdu[IU(g, i - 1, j + 1)]);
+          mN = half * (a.mdv[IV(g, i, j)] + a.mdv[IV(g, i, j + 1)]);
+          mS = half * (a.mdv[IV(g, i, j - 1)] + a.mdv[IV(g, i, j)]);
+        } else {
+          mE = half * (a.rho_u[IU(g, i, j)] + a.rho_u[IU(g, i, j + 1)]) * half *
+               (a.uadv[IU(g, i, j)] + a.uadv[IU(g, i, j + 1)]);
+          mW = half * (a.rho_u[IU(g, i - 1, j)] + a.rho_u[IU(g, i - 1, j + 1)]) * half *
+               (a.uadv[IU(g, i - 1, j)] + a.uadv[IU(g, i - 1, j + 1)]);
+          mN = a.rho_c[IP(g, i, j + 1)] * half * (a.vadv[IV(g, i, j)] + a.vadv[IV(g, i, j + 1)]);
+          mS = a.rho_c[IP(g, i, j)] * half * (a.vadv[IV(g, i, j - 1)] + a.vadv[IV(g, i, j)]);
+        }
+        const T vc = Xv[IV(g, i, j)];
+        const T vE = half * (vc + Xv[IV(g, i + 1, j)]);
+        const T vW = half * (Xv[IV(g, i - 1, j)] + vc);
+        const T vN = half * (vc + Xv[IV(g, i, j + 1)]);
+        const T vS = half * (Xv[IV(g, i, j - 1)] + vc);
+        adv = (mE * vE - mW * vW) * inv_dx + (mN * vN - mS * vS) * inv_dy;
+      }
+      const T x = Xv[IV(g, i, j)];
+      y = (a.rho_v[IV(g, i, j)] / a.dt) * x + adv + visc + grad;
+      if (a.flags & MV_PEN) y = y + a.penv[IV(g, i, j)] * x;
+    } else if (!g.per_y && i >= 1 && i <= nx && (j == 0 || j == ny)) {
+      y = Xv[IV(g, i, j)];
+    }
+    a.Yv[IV(g, i, j)] = y;
+  }
+  // ---- p rows: divergence
+  if (i >= 1 - nh && i <= nx + nh && j >= 1 - nh && j <= ny + nh) {
+    T y = T(0);
+    if (i >= 1 && i <= nx && j >= 1 && j <= ny)
+      y = (Xu[IU(g, i, j)] - Xu[IU(g, i - 1, j)]) * inv_dx + (Xv[IV(g, i, j)] - Xv[IV(g, i, j - 1)]) * inv_dy;
+    a.Yp[IP(g, i, j)] = y;
+  }
+}
+
+// ---------------------------------------------------------------------------------
+// H3: coupled dots (src/mod_bicgstab2.f90:32-72), up to 5 pairs in one pass over the
+// owned sets (u: j=1..ny, i=1..i_u_max; v: j=1..j_v_max, i=1..nx; p: interior), each
+// term (a*b)*vol, FP64 accumulation; the last workgroup runs the Krylov phase.
+template <typename T>
+struct DotArgs {
+  const T* a[5];
+  const T* b[5];
+  int np;
+  int phase, k, max_iter;
+};
+
+template <typename T, int NP>
+__global__ void __launch_bounds__(NT) k_cdot(Grid g, DotArgs<T> d, double* __restrict__ partials,
+                                             unsigned* __restrict__ ticket, double* __restrict__ sc,
+                                             int* __restrict__ fl) {
+  if (fl[F_BRK]) return;
+  int i, j;
+  union_ij(g, i, j);
+  const int nx = g.nx, ny = g.ny;
+  const int i_u_max = g.per_x ? nx : nx - 1;
+  const int j_v_max = g.per_y ? ny : ny - 1;
+  const double vol = g.vol;
+  // terms (a*b)*vol as in the reference; each workgroup lane holds up to three of them,
+  // folded into one double-double per pair so the total is the correctly rounded sum.
+  DD v[NP];
+#pragma unroll
+  for (int q = 0; q < NP; ++q) v[q] = dd_from(0.0);
+  const bool own_u = (j >= 1 && j <= ny && i >= 1 && i <= i_u_max);
+  const bool own_v = (j >= 1 && j <= j_v_max && i >= 1 && i <= nx);
+  const bool own_p = (j >= 1 && j <= ny && i >= 1 && i <= nx);
+  if (own_u) {
+    const size_t o = IU(g, i, j);
+#pragma unroll
+    for (int q = 0; q < NP; ++q) v[q] = dd_add(v[q], dd_from((double)d.a[q][o] * (double)d.b[q][o] * vol));
+  }
+  if (own_v) {
+    const size_t o = g.nu + IV(g, i, j);
+#pragma unroll
+    for (int q = 0; q < NP; ++q) v[q] = dd_add(v[q], dd_from((double)d.a[q][o] * (double)d.b[q][o] * vol));
+  }
+  if (own_p) {
+    const size_t o = g.nu + g.nv + IP(g, i, j);
+#pragma unroll
+    for (int q = 0; q < NP; ++q) v[q] = dd_add(v[q], dd_from((double)d.a[q][o] * (double)d.b[q][o] * vol));
+  }
+  double tot[NP];
+  if (grid_fold_dd2<NP>(v, partials, ticket, tot)) {
+    if (threadIdx.x == 0 && threadIdx.y == 0) {
+      double t5[5] = {0, 0, 0, 0, 0};
+#pragma unroll
+      for (int q = 0; q < NP; ++q) t5[q] = tot[q];
+      krylov_phase(d.phase, d.k, d.max_iter, sc, fl, t5);
+    }
+  }
+}
+
+// Whole-array triplet updates (Fortran whole-array expressions, ghosts included), each
+// with the reference's association order.  Coefficients are read from device slots;
+// `sgn` flips a coefficient (for x - c*y written as x + (-c)*y we keep x - c*y).
+enum LinOp : int {
+  L_SUB1 = 0,   // o = x0 - c0*x1
+  L_ADD1,       // o = x0 + c0*x1
+  L_ADD3,       // o = ((x0 + c0*x1) + c1*x2) + c2*x3
+  L_SUB2,       // o = (x0 - c0*x1) - c1*x2
+  L_ADD2,       // o = (x0 + c0*x1) + c1*x2
+  L_PUPD,       // o = x0 + c0*(x1 - c1*x2)
+  L_DIFF,       // o = x0 - x1
+  L_COPY,       // o = x0
+  L_SCALE,      // o = c0 * x0          (c0 = sc[i0] * s0)
+  L_ADDS        // o = x0 + s0 * x1     (s0 literal)
+};
+
+template <typename T>
+struct LinArgs {
+  T* out;
+  const T *x0, *x1, *x2, *x3;
+  int c0, c1, c2;        // slot indices (or -1)
+  double s0;             // literal factor
+  int op;
+  size_t n;
+  int skip_on;           // 1: skip when F_BRK set
+};
+
+template <typename To, typename T>
+__global__ void __launch_bounds__(256) k_lin(LinArgs<T> a, const double* __restrict__ sc,
+                                             const int* __restrict__ fl) {
+  if (a.skip_on && fl[F_BRK]) return;
+  const T c0 = a.c0 >= 0 ? (T)sc[a.c0] : T(0);
+  const T c1 = a.c1 >= 0 ? (T)sc[a.c1] : T(0);
+  const T c2 = a.c2 >= 0 ? (T)sc[a.c2] : T(0);
+  const size_t stride = (size_t)gridDim.x * blockDim.x;
+  for (size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x; k < a.n; k += stride) {
+    T o;
+    switch (a.op) {
+      case L_SUB1: o = a.x0[k] - c0 * a.x1[k]; break;
+      case L_ADD1: o = a.x0[k] + c0 * a.x1[k]; break;
+      case L_ADD3: o = ((a.x0[k] + c0 * a.x1[k]) + c1 * a.x2[k]) + c2 * a.x3[k]; break;
+      case L_SUB2: o = (a.x0[k] - c0 * a.x1[k]) - c1 * a.x2[k]; break;
+      case L_ADD2: o = (a.x0[k] + c0 * a.x1[k]) + c1 * a.x2[k]; break;
+      case L_PUPD: o = a.x0[k] + c0 * (a.x1[k] - c1 * a.x2[k]); break;
+      case L_DIFF: o = a.x0[k] - a.x1[k]; break;
+      case L_COPY: o = a.x0[k]; break;
+      case L_SCALE: o = (T)(sc[a.c0] * a.s0) * a.x0[k]; break;
+      default: o = a.x0[k] + (T)a.s0 * a.x1[k]; break;
+    }
+    a.out[k] = o;
+  }
+}
+
+// Precision conversion / scaled copy: out[k] = (To)(s * in[k]), s = sc[slot] * lit.
+template <typename To, typename Ti>
+__global__ void __launch_bounds__(256) k_convert(To* __restrict__ out, const Ti* __restrict__ in,
+                                                 size_t n, const double* __restrict__ sc, int slot,
+                                                 double lit, int invert) {
+  double s = lit;
+  if (slot >= 0) s *= invert ? 1.0 / sc[slot] : sc[slot];
+  const size_t stride = (size_t)gridDim.x * blockDim.x;
+  for (size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x; k < n; k += stride)
+    out[k] = (To)(s == 1.0 ? (double)in[k] : s * (double)in[k]);
+}
+
+// X += s * D (FP64 refinement update), s = sc[slot].
+template <typename Td>
+__global__ void __launch_bounds__(256) k_refine_add(double* __restrict__ X, const Td* __restrict__ D,
+                                                    size_t n, const double* __restrict__ sc, int slot) {
+  const double s = sc[slot];
+  const size_t stride = (size_t)gridDim.x * blockDim.x;
+  for (size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x; k < n; k += stride)
+    X[k] = X[k] + s * (double)D[k];
+}
+
+// ---------------------------------------------------------------------------------
+// H7: Jacobi-PCG pressure solve (src/mod_subsolvers.f90:32-246), two fused passes per
+// iteration with device-resident scalars and predication on F_P_DONE.
+
+// Face coefficients c = inv_h2 / rho_face with zeroed wall faces (:65-78), and D^-1 (:61-87).
+template <typename T>
+__global__ void k_pcg_coef(Grid g, const T* __restrict__ rho_u, const T* __restrict__ rho_v,
+                           T* __restrict__ cx, T* __restrict__ cy, T* __restrict__ dinv) {
+  int i, j;
+  union_ij(g, i, j);
+  const int nx = g.nx, ny = g.ny, nh = g.nh;
+  auto cxv = [&](int ii, int jj) -> double {
+    if (!g.per_x && (ii == 0 || ii == nx)) return 0.0;
+    return g.inv_dx2 / (double)rho_u[IU(g, ii, jj)];
+  };
+  auto cyv = [&](int ii, int jj) -> double {
+    if (!g.per_y && (jj == 0 || jj == ny)) return 0.0;
+    return g.inv_dy2 / (double)rho_v[IV(g, ii, jj)];
+  };
+  if (i >= -nh && i <= nx + nh && j >= 1 - nh && j <= ny + nh)
+    cx[IU(g, i, j)] = (j >= 1 && j <= ny && i >= 0 && i <= nx) ? (T)cxv(i, j) : T(0);
+  if (i >= 1 - nh && i <= nx + nh && j >= -nh && j <= ny + nh)
+    cy[IV(g, i, j)] = (i >= 1 && i <= nx && j >= 0 && j <= ny) ? (T)cyv(i, j) : T(0);
+  if (i >= 1 - nh && i <= nx + nh && j >= 1 - nh && j <= ny + nh) {
+    double dv = 1.0;
+    if (i >= 1 && i <= nx && j >= 1 && j <= ny) {
+      const double d = cxv(i, j) + cxv(i - 1, j) + cyv(i, j) + cyv(i, j - 1);
+      dv = (d > 1.0e-14) ? 1.0 / d : 1.0;
+    }
+    dinv[IP(g, i, j)] = (T)dv;
+  }
+}
+
+// Neighbour offsets of an interior cell with periodic wrap; at walls the neighbour is
+// the cell itself (the mirrored ghost), always multiplied by a zero coefficient.
+struct Nbr { size_t c, e, w, n, s; };
+__device__ inline Nbr cell_nbrs(const Grid& g, int i, int j) {
+  const int nx = g.nx, ny = g.ny;
+  int ie = i + 1, iw = i - 1, jn = j + 1, js = j - 1;
+  if (ie > nx) ie = g.per_x ? 1 : nx;
+  if (iw < 1) iw = g.per_x ? nx : 1;
+  if (jn > ny) jn = g.per_y ? 1 : ny;
+  if (js < 1) js = g.per_y ? ny : 1;
+  Nbr r;
+  r.c = IP(g, i, j); r.e = IP(g, ie, j); r.w = IP(g, iw, j); r.n = IP(g, i, jn); r.s = IP(g, i, js);
+  return r;
+}
+
+// Sum of rhs over the interior (for the zero-mean projection, :91-99) -> S_P_MEAN.
+// skip_mode: 0 never, 1 when a Krylov exit is pending (F_BRK), 2 when the PCG did not
+// iterate (zero-rhs early exit or skipped: F_P_DONE >= 2).
+template <typename T>
+__global__ void __launch_bounds__(NT) k_pcg_sum(Grid g, const T* __restrict__ a, double* partials,
+                                                unsigned* ticket, double* sc, int slot, int skip_mode,
+                                                const int* __restrict__ fl) {
+  if (skip_mode == 1 && fl[F_BRK]) return;
+  if (skip_mode == 2 && fl[F_P_DONE] >= 2) return;
+  int i, j;
+  cell_ij(0, i, j);
+  double v[1] = {0.0};
+  if (i <= g.nx && j <= g.ny) v[0] = (double)a[IP(g, i, j)];
+  double tot[1];
+  if (grid_fold_dd<1>(v, partials, ticket, tot))
+    if (threadIdx.x == 0 && threadIdx.y == 0) sc[slot] = tot[0] / (double)(g.nx * g.ny);
+}
+
+// PCG start: rhs_work = rhs - mean (walls) (:90-105), ||rhs_work|| with early exit
+// (:107-120), r = rhs_work - A phi, z = D^-1 r, p = z, (r,z) (:126-157).
+// phi_zero: the caller guarantees phi == 0 (the preconditioner's call), so A phi = 0
+// and phi's interior is written as 0 here.
+template <typename T>
+__global__ void __launch_bounds__(NT) k_pcg_init(Grid g, const T* __restrict__ rhs, T* __restrict__ phi,
+                                                 const T* __restrict__ cx, const T* __restrict__ cy,
+                                                 const T* __restrict__ dinv, T* __restrict__ r,
+                                                 T* __restrict__ z, T* __restrict__ p, int walls_both,
+                                                 int phi_zero, double* partials, unsigned* ticket,
+                                                 double* sc, int* fl) {
+  if (fl[F_BRK]) {   // pending Krylov exit: mark the solve as skipped
+    if (blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x == 0 && threadIdx.y == 0) fl[F_P_DONE] = 3;
+    return;
+  }
+  int i, j;
+  cell_ij(0, i, j);
+  double v[2] = {0.0, 0.0};
+  if (i <= g.nx && j <= g.ny) {
+    const Nbr nb = cell_nbrs(g, i, j);
+    const T mean = walls_both ? (T)sc[S_P_MEAN] : T(0);
+    const T rw = walls_both ? rhs[nb.c] - mean : rhs[nb.c];
+    T ap = T(0);
+    if (!phi_zero) {
+      const T cE = cx[IU(g, i, j)], cW = cx[IU(g, i - 1, j)];
+      const T cN = cy[IV(g, i, j)], cS = cy[IV(g, i, j - 1)];
+      ap = (cE + cW + cN + cS) * phi[nb.c] -
+           (cE * phi[nb.e] + cW * phi[nb.w] + cN * phi[nb.n] + cS * phi[nb.s]);
+    } else {
+      phi[nb.c] = T(0);
+    }
+    const T rr = rw - ap;
+    const T zz = dinv[nb.c] * rr;
+    r[nb.c] = rr; z[nb.c] = zz; p[nb.c] = zz;
+    v[0] = (double)rw * (double)rw;
+    v[1] = (double)rr * (double)zz;
+  }
+  double tot[2];
+  if (grid_fold_dd<2>(v, partials, ticket, tot)) {
+    if (threadIdx.x == 0 && threadIdx.y == 0) {
+      const double rn = sqrt(tot[0]);
+      sc[S_P_RHSN] = rn;
+      sc[S_P_RES] = 0.0;
+      fl[F_P_ITER] = 0;
+      if (rn < 1.0e-14) fl[F_P_DONE] = 2;
+      else { fl[F_P_DONE] = 0; sc[S_P_RZ] = tot[1]; }
+    }
+  }
+}
+
+// Iteration pass 1: p = z + beta p_old (iteration > 1, computed at the 5 stencil points),
+// Ap = A p, (p, Ap) -> alpha  (:161-192, :220-224).
+template <typename T>
+__global__ void __launch_bounds__(NT) k_pcg_iter1(Grid g, int it, const T* __restrict__ z,
+                                                  const T* __restrict__ p_old, T* __restrict__ p_new,
+                                                  T* __restrict__ Ap, const T* __restrict__ cx,
+                                                  const T* __restrict__ cy, double* partials,
+                                                  unsigned* ticket, double* sc, int* fl) {
+  if (fl[F_P_DONE]) return;
+  int i, j;
+  cell_ij(0, i, j);
+  double v[1] = {0.0};
+  if (i <= g.nx && j <= g.ny) {
+    const Nbr nb = cell_nbrs(g, i, j);
+    T pc, pe, pw, pn, ps;
+    if (it == 1) {
+      pc = p_old[nb.c]; pe = p_old[nb.e]; pw = p_old[nb.w]; pn = p_old[nb.n]; ps = p_old[nb.s];
+    } else {
+      const T beta = (T)sc[S_P_BETA];
+      pc = z[nb.c] + beta * p_old[nb.c];
+      pe = z[nb.e] + beta * p_old[nb.e];
+      pw = z[nb.w] + beta * p_old[nb.w];
+      pn = z[nb.n] + beta * p_old[nb.n];
+      ps = z[nb.s] + beta * p_old[nb.s];
+    }
+    const T cE = cx[IU(g, i, j)], cW = cx[IU(g, i - 1, j)];
+    const T cN = cy[IV(g, i, j)], cS = cy[IV(g, i, j - 1)];
+    const T ap = (cE + cW + cN + cS) * pc - (cE * pe + cW * pw + cN * pn + cS * ps);
+    p_new[nb.c] = pc;
+    Ap[nb.c] = ap;
+    v[0] = (double)pc * (double)ap;
+  }
+  double tot[1];
+  if (grid_fold_dd<1>(v, partials, ticket, tot)) {
+    if (threadIdx.x == 0 && threadIdx.y == 0) {
+      const double pAp = tot[0];
+      sc[S_P_PAP] = pAp;
+      if (fabs(pAp) < 1.0e-30) { fl[F_P_DONE] = 1; fl[F_P_ITER] = it; }
+      else sc[S_P_ALPHA] = sc[S_P_RZ] / pAp;
+    }
+  }
+}
+
+// Iteration pass 2: phi += alpha p; r -= alpha Ap; z = D^-1 r; ||r||, (r, z) (:194-218).
+template <typename T>
+__global__ void __launch_bounds__(NT) k_pcg_iter2(Grid g, int it, int max_iter, T* __restrict__ phi,
+                                                  const T* __restrict__ p, T* __restrict__ r,
+                                                  const T* __restrict__ Ap, T* __restrict__ z,
+                                                  const T* __restrict__ dinv, double* partials,
+                                                  unsigned* ticket, double* sc, int* fl) {
+  if (fl[F_P_DONE]) return;
+  int i, j;
+  cell_ij(0, i, j);
+  double v[2] = {0.0, 0.0};
+  if (i <= g.nx && j <= g.ny) {
+    const size_t c = IP(g, i, j);
+    const T alpha = (T)sc[S_P_ALPHA];
+    phi[c] = phi[c] + alpha * p[c];
+    const T rr = r[c] - alpha * Ap[c];
+    const T zz = dinv[c] * rr;
+    r[c] = rr;
+    z[c] = zz;
+    v[0] = (double)rr * (double)rr;
+    v[1] = (double)rr * (double)zz;
+  }
+  double tot[2];
+  if (grid_fold_dd<2>(v, partials, ticket, tot)) {
+    if (threadIdx.x == 0 && threadIdx.y == 0) {
+      const double res = sqrt(tot[0]);
+      sc[S_P_RES] = res;
+      if (res / sc[S_P_RHSN] < sc[S_P_TOL]) { fl[F_P_DONE] = 1; fl[F_P_ITER] = it; }
+      else {
+        const double rz_new = tot[1];
+        sc[S_P_BETA] = rz_new / sc[S_P_RZ];
+        sc[S_P_RZ] = rz_new;
+        if (it == max_iter) fl[F_P_ITER] = max_iter;
+      }
+    }
+  }
+}
+
+// Final zero-mean removal of phi (:227-241) -- fine-API path; the preconditioner fuses it.
+template <typename T>
+__global__ void k_pcg_submean(Grid g, T* __restrict__ phi, const double* __restrict__ sc,
+                              const int* __restrict__ fl) {
+  if (fl[F_P_DONE] == 2) return;
+  int i, j;
+  cell_ij(0, i, j);
+  if (i <= g.nx && j <= g.ny) {
+    const size_t c = IP(g, i, j);
+    phi[c] = phi[c] - (T)sc[S_P_MEAN];
+  }
+}
+
+template <typename T>
+__global__ void k_zero_if(T* __restrict__ a, size_t n, const int* __restrict__ fl, int flag, int value) {
+  if (fl[flag] != value) return;
+  const size_t stride = (size_t)gridDim.x * blockDim.x;
+  for (size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x; k < n; k += stride) a[k] = T(0);
+}
+
+// ---------------------------------------------------------------------------------
+// H6: PCD preconditioner pieces (src/mod_pcd_precond.f90).
+
+// r_p = -((1/dt) phi + u.grad phi) on cells, zero ghosts (:54-62).  phi is the PCG
+// iterate before mean removal: the mean is subtracted here on the interior; wall
+// ghosts of phi are 0; periodic ghosts wrap.  A zero-rhs early exit means phi = 0.
+template <typename T>
+__global__ void __launch_bounds__(NT) k_pcd_rp(Grid g, const T* __restrict__ phi, const T* __restrict__ uadv,
+                                               const T* __restrict__ vadv, T dt, int walls_both,
+                                               T* __restrict__ rp, const double* __restrict__ sc,
+                                               const int* __restrict__ fl) {
+  int i, j;
+  union_ij(g, i, j);
+  const int nx = g.nx, ny = g.ny, nh = g.nh;
+  if (i < 1 - nh || i > nx + nh || j < 1 - nh || j > ny + nh) return;
+  T out = T(0);
+  if (i >= 1 && i <= nx && j >= 1 && j <= ny) {
+    const bool early = (fl[F_P_DONE] == 2);
+    const T mean = walls_both ? (T)sc[S_P_MEAN] : T(0);
+    auto ph = [&](int ii, int jj) -> T {
+      if (early) return T(0);
+      if (ii < 1 || ii > nx) { if (!g.per_x) return T(0); ii = (ii < 1) ? ii + nx : ii - nx; }
+      if (jj < 1 || jj > ny) { if (!g.per_y) return T(0); jj = (jj < 1) ? jj + ny : jj - ny; }
+      const T x = phi[IP(g, ii, jj)];
+      return walls_both ? x - mean : x;
+    };
+    const T half = T(0.5);
+    const T u_avg = half * (uadv[IU(g, i, j)] + uadv[IU(g, i - 1, j)]);
+    const T v_avg = half * (vadv[IV(g, i, j)] + vadv[IV(g, i, j - 1)]);
+    const T conv = u_avg * (ph(i + 1, j) - ph(i - 1, j)) * (half * (T)g.inv_dx) +
+                   v_avg * (ph(i, j + 1) - ph(i, j - 1)) * (half * (T)g.inv_dy);
+    out = -((T(1) / dt) * ph(i, j) + conv);
+  }
+  rp[IP(g, i, j)] = out;
+}
+
+// z_u - d r_p/dx on faces i=1..nx-1 (copies at i=0, nx), z_v - d r_p/dy on j=1..ny-1
+// (copies at j=0, ny), zero elsewhere (:66-84, single-rank loop bounds).
+template <typename T>
+__global__ void __launch_bounds__(NT) k_pcd_zwork(Grid g, const T* __restrict__ zu, const T* __restrict__ zv,
+                                                  const T* __restrict__ rp, T* __restrict__ zuw,
+                                                  T* __restrict__ zvw) {
+  int i, j;
+  union_ij(g, i, j);
+  const int nx = g.nx, ny = g.ny, nh = g.nh;
+  if (i >= -nh && i <= nx + nh && j >= 1 - nh && j <= ny + nh) {
+    T o = T(0);
+    if (j >= 1 && j <= ny) {
+      if (i >= 1 && i <= nx - 1) o = zu[IU(g, i, j)] - (rp[IP(g, i + 1, j)] - rp[IP(g, i, j)]) * (T)g.inv_dx;
+      else if (i == 0 || i == nx) o = zu[IU(g, i, j)];
+    }
+    zuw[IU(g, i, j)] = o;
+  }
+  if (i >= 1 - nh && i <= nx + nh && j >= -nh && j <= ny + nh) {
+    T o = T(0);
+    if (i >= 1 && i <= nx) {
+      if (j >= 1 && j <= ny - 1) o = zv[IV(g, i, j)] - (rp[IP(g, i, j + 1)] - rp[IP(g, i, j)]) * (T)g.inv_dy;
+      else if (j == 0 || j == ny) o = zv[IV(g, i, j)];
+    }
+    zvw[IV(g, i, j)] = o;
+  }
+}
+
+// One damped-Jacobi sweep (omega = 0.8) of the v block (src/mod_subsolvers.f90:342-370).
+// v_old == nullptr means the zero start.  Periodic neighbours wrap (the exchange at the
+// top of each sweep); wall rows carry rhs; all other points copy v_old.
+template <typename T>
+__global__ void __launch_bounds__(NT) k_jacobi_v(Grid g, const T* __restrict__ v_old, T* __restrict__ v_new,
+                                                 const T* __restrict__ rhs, const T* __restrict__ rho_v,
+                                                 const T* __restrict__ kap, const T* __restrict__ zeta,
+                                                 const T* __restrict__ eta, T dt, const T* __restrict__ pen) {
+  int i, j;
+  union_ij(g, i, j);
+  const int nx = g.nx, ny = g.ny, nh = g.nh;
+  if (i < 1 - nh || i > nx + nh || j < -nh || j > ny + nh) return;
+  const int j_v_max = g.per_y ? ny : ny - 1;
+  auto old = [&](int ii, int jj) -> T {
+    if (!v_old) return T(0);
+    if (g.per_x && (ii < 1 || ii > nx)) ii = (ii < 1) ? ii + nx : ii - nx;
+    if (g.per_y && (jj < 1 || jj > ny)) jj = (jj < 1) ? jj + ny : jj - ny;
+    return v_old[IV(g, ii, jj)];
+  };
+  T out;
+  if (j >= 1 && j <= j_v_max && i >= 1 && i <= nx) {
+    const T inv_dx2 = (T)g.inv_dx2, inv_dy2 = (T)g.inv_dy2;
+    const T cE = (zeta[IW(g, i, j)] - eta[IW(g, i, j)]) * inv_dx2;
+    const T cW = (zeta[IW(g, i - 1, j)] - eta[IW(g, i - 1, j)]) * inv_dx2;
+    const T cN = kap[IP(g, i, j + 1)] * inv_dy2;
+    const T cS = kap[IP(g, i, j)] * inv_dy2;
+    T diag = (rho_v[IV(g, i, j)] / dt) + cE + cW + cN + cS;
+    if (pen) diag = diag + pen[IV(g, i, j)];
+    const T off = cE * old(i + 1, j) + cW * old(i - 1, j) + cN * old(i, j + 1) + cS * old(i, j - 1);
+    const T vo = old(i, j);
+    if (diag > (T)1.0e-15) out = (T)(1.0 - 0.8) * vo + (T)0.8 * ((rhs[IV(g, i, j)] + off) / diag);
+    else out = vo;
+  } else if (!g.per_y && i >= 1 && i <= nx && (j == 0 || j == ny)) {
+    out = rhs[IV(g, i, j)];
+  } else {
+    out = v_old ? v_old[IV(g, i, j)] : T(0);
+  }
+  v_new[IV(g, i, j)] = out;
+}
+
+// u block sweep (src/mod_subsolvers.f90:280-307).
+template <typename T>
+__global__ void __launch_bounds__(NT) k_jacobi_u(Grid g, const T* __restrict__ u_old, T* __restrict__ u_new,
+                                                 const T* __restrict__ rhs, const T* __restrict__ rho_u,
+                                                 const T* __restrict__ kap, const T* __restrict__ zeta,
+                                                 const T* __restrict__ eta, T dt, const T* __restrict__ pen) {
+  int i, j;
+  union_ij(g, i, j);
+  const int nx = g.nx, ny = g.ny, nh = g.nh;
+  if (i < -nh || i > nx + nh || j < 1 - nh || j > ny + nh) return;
+  const int i_u_max = g.per_x ? nx : nx - 1;
+  auto old = [&](int ii, int jj) -> T {
+    if (!u_old) return T(0);
+    if (g.per_x && (ii < 1 || ii > nx)) ii = (ii < 1) ? ii + nx : ii - nx;
+    if (g.per_y && (jj < 1 || jj > ny)) jj = (jj < 1) ? jj + ny : jj - ny;
+    return u_old[IU(g, ii, jj)];
+  };
+  T out;
+  if (j >= 1 && j <= ny && i >= 1 && i <= i_u_max) {
+    const T inv_dx2 = (T)g.inv_dx2, inv_dy2 = (T)g.inv_dy2;
+    const T cE = kap[IP(g, i + 1, j)] * inv_dx2;
+    const T cW = kap[IP(g, i, j)] * inv_dx2;
+    const T cN = (zeta[IW(g, i, j)] - eta[IW(g, i, j)]) * inv_dy2;
+    const T cS = (zeta[IW(g, i, j - 1)] - eta[IW(g, i, j - 1)]) * inv_dy2;
+    T diag = (rho_u[IU(g, i, j)] / dt) + cE + cW + cN + cS;
+    if (pen) diag = diag + pen[IU(g, i, j)];
+    const T off = cE * old(i + 1, j) + cW * old(i - 1, j) + cN * old(i, j + 1) + cS * old(i, j - 1);
+    const T uo = old(i, j);
+    if (diag > (T)1.0e-15) out = (T)(1.0 - 0.8) * uo + (T)0.8 * ((rhs[IU(g, i, j)] + off) / diag);
+    else out = uo;
+  } else if (!g.per_x && j >= 1 && j <= ny && (i == 0 || i == nx)) {
+    out = rhs[IU(g, i, j)];
+  } else {
+    out = u_old ? u_old[IU(g, i, j)] : T(0);
+  }
+  u_new[IU(g, i, j)] = out;
+}
+
+// u gets the eta cross term of the solved v block, i=1..nx-1, j=1..ny (:91-98).
+template <typename T>
+__global__ void k_pcd_cross(Grid g, const T* __restrict__ rv, const T* __restrict__ eta, T* __restrict__ zuw) {
+  int i, j;
+  cell_ij(0, i, j);
+  if (i > g.nx - 1 || j > g.ny) return;
+  const T inv_dx = (T)g.inv_dx, inv_dy = (T)g.inv_dy;
+  const T cross = -inv_dy * (eta[IW(g, i, j)] * (rv[IV(g, i + 1, j)] - rv[IV(g, i, j)]) * inv_dx -
+                             eta[IW(g, i, j - 1)] * (rv[IV(g, i + 1, j - 1)] - rv[IV(g, i, j - 1)]) * inv_dx);
+  const size_t o = IU(g, i, j);
+  zuw[o] = zuw[o] - cross;
+}
+
+// ---------------------------------------------------------------------------------
+// H12: mixture properties over the full padded arrays (src/mod_viscosity.f90:28-151).
+__device__ inline double clamp01(double x) {
+  const double m = (1.0 < x) ? 1.0 : x;     // Fortran min(one, x)
+  return (m > 0.0) ? m : 0.0;               // Fortran max(zero, .)
+}
+__device__ inline double harmonic_mu(double c, double mu1, double mu2) {
+  const double cc = clamp01(c);
+  if (fabs(mu1 - mu2) < 1.0e-14) return mu1;
+  const double den = mu2 * cc + mu1 * (1.0 - cc);
+  if (den > 1.0e-15) return (mu1 * mu2) / den;
+  return (mu2 < mu1) ? mu2 : mu1;
+}
+
+__global__ void k_mixture(Grid g, const double* __restrict__ c, double rho1, double rho2, double mu1,
+                          double mu2, double* __restrict__ rho_c, double* __restrict__ rho_u,
+                          double* __restrict__ rho_v, double* __restrict__ kap, double* __restrict__ zeta,
+                          double* __restrict__ eta) {
+  int i, j;
+  union_ij(g, i, j);
+  const int nx = g.nx, ny = g.ny, nh = g.nh;
+  auto rc = [&](int ii, int jj) -> double {
+    const double cv = clamp01(c[IP(g, ii, jj)]);
+    return rho1 * cv + rho2 * (1.0 - cv);
+  };
+  if (i >= 1 - nh && i <= nx + nh && j >= 1 - nh && j <= ny + nh) {
+    rho_c[IP(g, i, j)] = rc(i, j);
+    kap[IP(g, i, j)] = 2.0 * harmonic_mu(c[IP(g, i, j)], mu1, mu2);
+  }
+  if (i >= -nh && i <= nx + nh && j >= 1 - nh && j <= ny + nh) {
+    int ii = i;
+    if (ii < 1 - nh) ii = 1 - nh;
+    if (ii > nx + nh - 1) ii = nx + nh - 1;
+    rho_u[IU(g, i, j)] = 0.5 * (rc(ii, j) + rc(ii + 1, j));
+  }
+  if (i >= 1 - nh && i <= nx + nh && j >= -nh && j <= ny + nh) {
+    int jj = j;
+    if (jj < 1 - nh) jj = 1 - nh;
+    if (jj > ny + nh - 1) jj = ny + nh - 1;
+    rho_v[IV(g, i, j)] = 0.5 * (rc(i, jj) + rc(i, jj + 1));
+  }
+  if (i >= -nh && i <= nx + nh && j >= -nh && j <= ny + nh) {
+    int ii = i, jj = j;
+    if (ii < 1 - nh) ii = 1 - nh;
+    if (ii > nx + nh - 1) ii = nx + nh - 1;
+    if (jj < 1 - nh) jj = 1 - nh;
+    if (jj > ny + nh - 1) jj = ny + nh - 1;
+    const double cv = 0.25 * (c[IP(g, ii, jj)] + c[IP(g, ii + 1, jj)] + c[IP(g, ii, jj + 1)] + c[IP(g, ii + 1, jj + 1)]);
+    const double m = harmonic_mu(cv, mu1, mu2);
+    zeta[IW(g, i, j)] = 2.0 * m;
+    eta[IW(g, i, j)] = m;
+  }
+}
+
+// ---------------------------------------------------------------------------------
+// H9: VOF-TVD advection (src/mod_vof.f90:25-140).  Face fluxes are recomputed per use
+// (each face by its two cells) with the same arithmetic as the reference's Flux arrays.
+__device__ inline double van_leer(double r) { return (r <= 0.0) ? 0.0 : (2.0 * r) / (1.0 + r); }
+
+__device__ inline double vof_flux_x(const Grid& g, const double* c, const double* u, double dt, int i, int j) {
+  const double uf = u[IU(g, i, j)];
+  double num, den, rr, cfl, cf;
+  if (uf >= 0.0) {
+    num = c[IP(g, i, j)] - c[IP(g, i - 1, j)];
+    den = c[IP(g, i + 1, j)] - c[IP(g, i, j)];
+    rr = (fabs(den) > 1.0e-12) ? num / den : 0.0;
+    cfl = fabs(uf) * dt * g.inv_dx;
+    cf = c[IP(g, i, j)] + 0.5 * (1.0 - cfl) * van_leer(rr) * den;
+  } else {
+    num = c[IP(g, i + 2, j)] - c[IP(g, i + 1, j)];
+    den = c[IP(g, i + 1, j)] - c[IP(g, i, j)];
+    rr = (fabs(den) > 1.0e-12) ? num / den : 0.0;
+    cfl = fabs(uf) * dt * g.inv_dx;
+    cf = c[IP(g, i + 1, j)] - 0.5 * (1.0 - cfl) * van_leer(rr) * den;
+  }
+  return uf * clamp01(cf);
+}
+__device__ inline double vof_flux_y(const Grid& g, const double* c, const double* v, double dt, int i, int j) {
+  const double vf = v[IV(g, i, j)];
+  double num, den, rr, cfl, cf;
+  if (vf >= 0.0) {
+    num = c[IP(g, i, j)] - c[IP(g, i, j - 1)];
+    den = c[IP(g, i, j + 1)] - c[IP(g, i, j)];
+    rr = (fabs(den) > 1.0e-12) ? num / den : 0.0;
+    cfl = fabs(vf) * dt * g.inv_dy;
+    cf = c[IP(g, i, j)] + 0.5 * (1.0 - cfl) * van_leer(rr) * den;
+  } else {
+    num = c[IP(g, i, j + 2)] - c[IP(g, i, j + 1)];
+    den = c[IP(g, i, j + 1)] - c[IP(g, i, j)];
+    rr = (fabs(den) > 1.0e-12) ? num / den : 0.0;
+    cfl = fabs(vf) * dt * g.inv_dy;
+    cf = c[IP(g, i, j + 1)] - 0.5 * (1.0 - cfl) * van_leer(rr) * den;
+  }
+  return vf * clamp01(cf);
+}
+
+__global__ void __launch_bounds__(NT) k_vof(Grid g, const double* __restrict__ cin, double* __restrict__ cout,
+                                            const double* __restrict__ u, const double* __restrict__ v,
+                                            double dt, double* __restrict__ fx, double* __restrict__ fy) {
+  int i, j;
+  union_ij(g, i, j);
+  const int nx = g.nx, ny = g.ny, nh = g.nh;
+  if (i >= 1 - nh && i <= nx + nh && j >= 1 - nh && j <= ny + nh) {
+    double o = cin[IP(g, i, j)];
+    if (i >= 1 && i <= nx && j >= 1 && j <= ny) {
+      const double fe = vof_flux_x(g, cin, u, dt, i, j), fw = vof_flux_x(g, cin, u, dt, i - 1, j);
+      const double fn = vof_flux_y(g, cin, v, dt, i, j), fs = vof_flux_y(g, cin, v, dt, i, j - 1);
+      o = cin[IP(g, i, j)] - dt * ((fe - fw) * g.inv_dx + (fn - fs) * g.inv_dy);
+      o = clamp01(o);
+    }
+    cout[IP(g, i, j)] = o;
+  }
+  if (fx && i >= -nh && i <= nx + nh && j >= 1 - nh && j <= ny + nh)
+    fx[IU(g, i, j)] = (j >= 1 && j <= ny && i >= 0 && i <= nx) ? vof_flux_x(g, cin, u, dt, i, j) : 0.0;
+  if (fy && i >= 1 - nh && i <= nx + nh && j >= -nh && j <= ny + nh)
+    fy[IV(g, i, j)] = (i >= 1 && i <= nx && j >= 0 && j <= ny) ? vof_flux_y(g, cin, v, dt, i, j) : 0.0;
+}
+
+// H10: one SVOF Jacobi sweep (src/mod_vof.f90:168-190) with mirror/wrap neighbour reads
+// of the previous iterate and a max-reduction of the update; predicated on F_S_DONE.
+__global__ void __launch_bounds__(NT) k_svof_sweep(Grid g, int it, const double* __restrict__ craw,
+                                                   const double* __restrict__ cold, double* __restrict__ cnew,
+                                                   double cx, double cy, double diag, double tol,
+                                                   double* partials, unsigned* ticket, double* sc, int* fl) {
+  if (fl[F_S_DONE]) return;
+  int i, j;
+  cell_ij(0, i, j);
+  double v[1] = {0.0};
+  if (i <= g.nx && j <= g.ny) {
+    const Nbr nb = cell_nbrs(g, i, j);
+    // at walls the mirrored ghost equals the cell itself: cell_nbrs already maps to it
+    const double o = (craw[nb.c] + cx * (cold[nb.e] + cold[nb.w]) + cy * (cold[nb.n] + cold[nb.s])) / diag;
+    cnew[nb.c] = o;
+    v[0] = fabs(o - cold[nb.c]);
+  }
+  double tot[1];
+  if (grid_fold<1, true>(v, partials, ticket, tot)) {
+    if (threadIdx.x == 0 && threadIdx.y == 0) {
+      sc[S_MAX0] = tot[0];
+      fl[F_S_ITER] = it;
+      if (tot[0] < tol) fl[F_S_DONE] = 1;
+    }
+  }
+}
+
+// H11: face normals from the smoothed C (src/mod_surface_tension.f90:41-60).
+__global__ void k_normals(Grid g, const double* __restrict__ cs, double* __restrict__ nxu, double* __restrict__ nyv) {
+  int i, j;
+  union_ij(g, i, j);
+  const int nx = g.nx, ny = g.ny;
+  const double eps = 1.0e-12;
+  if (j >= 1 && j <= ny && i >= 0 && i <= nx) {
+    const double gx = (cs[IP(g, i + 1, j)] - cs[IP(g, i, j)]) * g.inv_dx;
+    const double gy = 0.25 * (cs[IP(g, i, j + 1)] + cs[IP(g, i + 1, j + 1)] - cs[IP(g, i, j - 1)] - cs[IP(g, i + 1, j - 1)]) * g.inv_dy;
+    nxu[IU(g, i, j)] = gx / sqrt(gx * gx + gy * gy + eps);
+  }
+  if (i >= 1 && i <= nx && j >= 0 && j <= ny) {
+    const double gx = 0.25 * (cs[IP(g, i + 1, j)] + cs[IP(g, i + 1, j + 1)] - cs[IP(g, i - 1, j)] - cs[IP(g, i - 1, j + 1)]) * g.inv_dx;
+    const double gy = (cs[IP(g, i, j + 1)] - cs[IP(g, i, j)]) * g.inv_dy;
+    nyv[IV(g, i, j)] = gy / sqrt(gx * gx + gy * gy + eps);
+  }
+}
+
+// kappa = -div n at centres, zero ghosts (:63-69).
+__global__ void k_curv(Grid g, const double* __restrict__ nxu, const double* __restrict__ nyv, double* __restrict__ kc) {
+  int i, j;
+  union_ij(g, i, j);
+  const int nx = g.nx, ny = g.ny, nh = g.nh;
+  if (i < 1 - nh || i > nx + nh || j < 1 - nh || j > ny + nh) return;
+  double o = 0.0;
+  if (i >= 1 && i <= nx && j >= 1 && j <= ny)
+    o = -((nxu[IU(g, i, j)] - nxu[IU(g, i - 1, j)]) * g.inv_dx + (nyv[IV(g, i, j)] - nyv[IV(g, i, j - 1)]) * g.inv_dy);
+  kc[IP(g, i, j)] = o;
+}
+
+// CSF face force sigma * kappa_face * grad C (src/mod_surface_tension.f90:78-109).
+__global__ void k_csf(Grid g, const double* __restrict__ craw, const double* __restrict__ kc, double sigma,
+                      double* __restrict__ fu, double* __restrict__ fv) {
+  int i, j;
+  union_ij(g, i, j);
+  const int nx = g.nx, ny = g.ny, nh = g.nh;
+  if (i >= -nh && i <= nx + nh && j >= 1 - nh && j <= ny + nh) {
+    double o = 0.0;
+    if (j >= 1 && j <= ny && i >= 0 && i <= nx) {
+      const double kf = 0.5 * (kc[IP(g, i, j)] + kc[IP(g, i + 1, j)]);
+      o = sigma * kf * (craw[IP(g, i + 1, j)] - craw[IP(g, i, j)]) * g.inv_dx;
+    }
+    fu[IU(g, i, j)] = o;
+  }
+  if (i >= 1 - nh && i <= nx + nh && j >= -nh && j <= ny + nh) {
+    double o = 0.0;
+    if (i >= 1 && i <= nx && j >= 0 && j <= ny) {
+      const double kf = 0.5 * (kc[IP(g, i, j)] + kc[IP(g, i, j + 1)]);
+      o = sigma * kf * (craw[IP(g, i, j + 1)] - craw[IP(g, i, j)]) * g.inv_dy;
+    }
+    fv[IV(g, i, j)] = o;
+  }
+}
+
+// ---------------------------------------------------------------------------------
+// H13: step preparation (src/mod_solver_coupled.f90:362-456).
+// u_adv = 2 u^n - u^(n-1) (or u^n at step 1) and u_prev = u^n, over the full arrays.
+__global__ void k_ab2(double* __restrict__ uadv, double* __restrict__ uprev, const double* __restrict__ un,
+                      size_t n, int first) {
+  const size_t stride = (size_t)gridDim.x * blockDim.x;
+  for (size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x; k < n; k += stride) {
+    const double a = un[k];
+    uadv[k] = first ? a : 2.0 * a - uprev[k];
+    uprev[k] = a;
+  }
+}
+// mdot = (rho1 - rho2) flux + rho2 u_adv (:383-384)
+__global__ void k_amp(double* __restrict__ md, const double* __restrict__ flux, const double* __restrict__ uadv,
+                      size_t n, double drho, double rho2) {
+  const size_t stride = (size_t)gridDim.x * blockDim.x;
+  for (size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x; k < n; k += stride)
+    md[k] = drho * flux[k] + rho2 * uadv[k];
+}
+// pen = alpha * phi_s (:410-411)
+__global__ void k_scale(double* __restrict__ o, const double* __restrict__ a, size_t n, double s) {
+  const size_t stride = (size_t)gridDim.x * blockDim.x;
+  for (size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x; k < n; k += stride) o[k] = s * a[k];
+}
+
+struct RhsArgs {
+  const double *un, *unm1, *vn, *vnm1, *rho_u, *rho_v, *fu, *fv, *penu, *penv, *zeta, *eta;
+  double *Bu, *Bv, *Bp;
+  double dt, gx, gy, u_solid, v_solid, u_lid;
+  int bdf2, has_lid;
+};
+// B_u, B_v, B_p (:415-456) over the full padded arrays (zero outside owned rows).
+__global__ void __launch_bounds__(NT) k_rhs(Grid g, RhsArgs a) {
+  int i, j;
+  union_ij(g, i, j);
+  const int nx = g.nx, ny = g.ny, nh = g.nh;
+  const int i_u_max = g.per_x ? nx : nx - 1;
+  const int j_v_max = g.per_y ? ny : ny - 1;
+  if (i >= -nh && i <= nx + nh && j >= 1 - nh && j <= ny + nh) {
+    double b = 0.0;
+    if (j >= 1 && j <= ny && i >= 1 && i <= i_u_max) {
+      const size_t o = IU(g, i, j);
+      if (a.bdf2) b = (a.rho_u[o] / a.dt) * (2.0 * a.un[o] - 0.5 * a.unm1[o]) + a.rho_u[o] * a.gx + a.fu[o];
+      else b = (a.rho_u[o] / a.dt) * a.un[o] + a.rho_u[o] * a.gx + a.fu[o];
+      if (a.penu) b = b + a.penu[o] * a.u_solid;
+      if (a.has_lid && !g.per_y && j == ny)
+        b = b + (2.0 * (a.zeta[IW(g, i, ny)] - a.eta[IW(g, i, ny)]) / (g.dy * g.dy)) * a.u_lid;
+    }
+    a.Bu[IU(g, i, j)] = b;
+  }
+  if (i >= 1 - nh && i <= nx + nh && j >= -nh && j <= ny + nh) {
+    double b = 0.0;
+    if (j >= 1 && j <= j_v_max && i >= 1 && i <= nx) {
+      const size_t o = IV(g, i, j);
+      if (a.bdf2) b = (a.rho_v[o] / a.dt) * (2.0 * a.vn[o] - 0.5 * a.vnm1[o]) + a.rho_v[o] * a.gy + a.fv[o];
+      else b = (a.rho_v[o] / a.dt) * a.vn[o] + a.rho_v[o] * a.gy + a.fv[o];
+      if (a.penv) b = b + a.penv[o] * a.v_solid;
+    }
+    a.Bv[IV(g, i, j)] = b;
+  }
+  if (i >= 1 - nh && i <= nx + nh && j >= 1 - nh && j <= ny + nh) a.Bp[IP(g, i, j)] = 0.0;
+}
+
+// max |u|, |v| over faces for the adaptive dt (src/mod_solver_coupled.f90:207-219).
+__global__ void __launch_bounds__(NT) k_absmax(Grid g, const double* __restrict__ u, const double* __restrict__ v,
+                                               double* partials, unsigned* ticket, double* sc) {
+  int i, j;
+  union_ij(g, i, j);
+  double m[2] = {0.0, 0.0};
+  if (j >= 1 && j <= g.ny && i >= 0 && i <= g.nx) m[0] = fabs(u[IU(g, i, j)]);
+  if (i >= 1 && i <= g.nx && j >= 0 && j <= g.ny) m[1] = fabs(v[IV(g, i, j)]);
+  double tot[2];
+  if (grid_fold<2, true>(m, partials, ticket, tot))
+    if (threadIdx.x == 0 && threadIdx.y == 0) { sc[S_DOT0] = tot[0]; sc[S_DOT1] = tot[1]; }
+}
+
+}  // namespace fugu

@@ -1,0 +1,192 @@
+/*
+ * fugu2d.h -- C ABI of the MI355X (gfx950) FUGU-2D per-step hot path (libfugu2d_hip.so).
+ *
+ * Drop-in boundary for the reference's Fortran modules (mod_operators, mod_bicgstab2,
+ * mod_pcd_precond, mod_subsolvers, mod_vof, mod_surface_tension, mod_viscosity,
+ * mod_solver_coupled).  Plain pointers and sizes only.
+ *
+ * Arrays: every field argument is the reference's padded Fortran array, passed as a pointer
+ * to its first element (lowest bounds), column-major with i fastest:
+ *   u-kind (0-nh:nx+nh, 1-nh:ny+nh)   v-kind (1-nh:nx+nh, 0-nh:ny+nh)
+ *   p-kind (1-nh:nx+nh, 1-nh:ny+nh)   w-kind (0-nh:nx+nh, 0-nh:ny+nh) (vertices)
+ * Pointers may be host memory or device memory of the context's GPU (hipMemcpyDefault
+ * staging): the "fine" entry points copy in, run on the device, copy out and synchronise.
+ * The "device-resident" entry points (fugu_state_*, fugu_step, fugu_solve_*) keep all
+ * fields in HBM between calls.
+ *
+ * Optional Fortran arguments map to NULL pointers; logical flags to int.
+ * Return value: 0 ok; < 0 error (see fugu_last_error()); > 0 warning bits (FUGU_W_*).
+ * A context is not shared between host threads.
+ */
+#ifndef FUGU2D_H
+#define FUGU2D_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct fugu_ctx fugu_ctx;
+
+/* Grid of one device-resident domain.  Replaces the module globals of src/mod_mpi.f90:10-26
+ * (dx_grid, dy_grid, periods, coords, dims) at one rank; nh must be 2 (src/main.f90:22). */
+typedef struct {
+  int nx, ny, nh;
+  double lx, ly;          /* dx = lx/nx, dy = ly/ny  (src/mod_mpi.f90:104-105) */
+  int periodic_x, periodic_y;
+} fugu_grid;
+
+/* Physical parameters: the mutable globals of src/mod_params.f90:19-50. */
+typedef struct {
+  double rho1, rho2, mu1, mu2, sigma, gx, gy;
+  int time_order;           /* 1 = BDF1, 2 = BDF2 from step 2 on */
+  double alpha_penalty;
+  double cfl_target;
+} fugu_params;
+
+/* Options of advance_coupled_step (src/mod_solver_coupled.f90:290-297). */
+typedef struct {
+  int use_bicgstab2;        /* use_bicgstab2 */
+  int has_lid;              /* present(u_top_lid) */
+  double u_lid;
+  double tol;               /* tol_in (reference default tol_bicgstab2 = 1e-6) */
+  int max_iter;             /* max_iter_in (reference default 500) */
+  int use_amp;              /* consistent_amp */
+  double u_solid, v_solid;  /* u_solid_in, v_solid_in */
+  unsigned solver_flags;    /* FUGU_SOLVE_* (0 = the reference solver exactly) */
+} fugu_step_opts;
+
+enum {
+  /* BiCGStab(2) with velocity BCs applied to r and s before their matvecs (the reference
+   * omits them, src/mod_bicgstab2.f90:430,489, which stalls it on walled domains). */
+  FUGU_B2_CONSISTENT_BCS = 1u << 0,
+  /* Step solve by FP64 iterative refinement on the true residual, inner BiCGStab(2)+PCD. */
+  FUGU_SOLVE_REFINED = 1u << 1,
+  /* With FUGU_SOLVE_REFINED: inner Krylov + preconditioner in FP32 (mixed precision). */
+  FUGU_SOLVE_MIXED = 1u << 2
+};
+
+enum { FUGU_W_MAX_ITER = 1, FUGU_W_NONFINITE = 2 };
+enum { FUGU_KIND_U = 0, FUGU_KIND_V = 1, FUGU_KIND_P = 2, FUGU_KIND_W = 3 };
+
+/* ---- context -------------------------------------------------------------------- */
+int fugu_ctx_create(const fugu_grid* grid, const fugu_params* params, fugu_ctx** out);
+int fugu_ctx_destroy(fugu_ctx* ctx);
+int fugu_set_params(fugu_ctx* ctx, const fugu_params* params);
+const char* fugu_last_error(void);
+/* Element count of a padded array of the given kind. */
+long long fugu_field_size(const fugu_ctx* ctx, int kind);
+
+/* ---- fine API: one entry per reference public procedure ----------------------------- */
+/* src/mod_mpi.f90:113-245 at one rank (periodic self wrap-copy; walls untouched). */
+int fugu_exchange_halos(fugu_ctx* ctx, double* a, int kind);
+/* src/mod_operators.f90:34-53 */
+int fugu_op_divergence(fugu_ctx* ctx, const double* u, const double* v, double* div);
+/* src/mod_operators.f90:254-333; u_top_lid NULL = absent */
+int fugu_apply_velocity_bcs(fugu_ctx* ctx, double* u, double* v, const double* u_top_lid);
+/* src/mod_operators.f90:341-443.  X is in/out (periodic halo exchange writes its ghosts). */
+int fugu_matvec_coupled(fugu_ctx* ctx, double* X_u, double* X_v, double* X_p,
+                        double* Y_u, double* Y_v, double* Y_p,
+                        const double* u_adv, const double* v_adv,
+                        const double* rho_c, const double* rho_u, const double* rho_v,
+                        const double* kappa, const double* zeta, const double* eta,
+                        double dt_step, int include_advection,
+                        const double* penalty_u, const double* penalty_v,
+                        const double* mdot_u_cons, const double* mdot_v_cons);
+/* src/mod_bicgstab2.f90:32-72 */
+int fugu_coupled_dot(fugu_ctx* ctx, const double* u1, const double* v1, const double* p1,
+                     const double* u2, const double* v2, const double* p2, double* dot_out);
+/* src/mod_subsolvers.f90:32-246 (tol <= 0 / max_iter <= 0 select the defaults 1e-5 / 150) */
+int fugu_solve_poisson_pressure(fugu_ctx* ctx, double* phi, const double* rhs,
+                                const double* rho_u, const double* rho_v,
+                                double tol, int max_iter, int* iter_out, double* res_out);
+/* src/mod_subsolvers.f90:252-309 and 314-372 (n_sweeps <= 0 selects the default 5) */
+int fugu_solve_helmholtz_u(fugu_ctx* ctx, double* u_sol, const double* rhs_u, const double* rho_u,
+                           const double* kappa, const double* zeta, const double* eta,
+                           double dt_step, int n_sweeps, const double* penalty_u);
+int fugu_solve_helmholtz_v(fugu_ctx* ctx, double* v_sol, const double* rhs_v, const double* rho_v,
+                           const double* kappa, const double* zeta, const double* eta,
+                           double dt_step, int n_sweeps, const double* penalty_v);
+/* src/mod_pcd_precond.f90:14-104 */
+int fugu_apply_pcd_precond(fugu_ctx* ctx, const double* z_u, const double* z_v, const double* z_p,
+                           double* r_u, double* r_v, double* r_p,
+                           const double* u_adv, const double* v_adv,
+                           const double* rho_u, const double* rho_v,
+                           const double* kappa, const double* zeta, const double* eta,
+                           double dt_step, const double* penalty_u, const double* penalty_v);
+/* src/mod_bicgstab2.f90:77-264 (BiCGStab) and 270-570 (BiCGStab(2)).  tol <= 0 / max_iter
+ * <= 0 select the defaults 1e-5 / 100.  flags: FUGU_B2_CONSISTENT_BCS (BiCGStab(2) only). */
+int fugu_bicgstab_solve(fugu_ctx* ctx, double* X_u, double* X_v, double* X_p,
+                        const double* B_u, const double* B_v, const double* B_p,
+                        const double* u_adv, const double* v_adv,
+                        const double* rho_c, const double* rho_u, const double* rho_v,
+                        const double* kappa, const double* zeta, const double* eta,
+                        double dt_step, double tol, int max_iter, int* iter_out, double* res_out,
+                        const double* u_top_lid, const double* penalty_u, const double* penalty_v,
+                        const double* mdot_u_cons, const double* mdot_v_cons);
+int fugu_bicgstab2_solve(fugu_ctx* ctx, double* X_u, double* X_v, double* X_p,
+                         const double* B_u, const double* B_v, const double* B_p,
+                         const double* u_adv, const double* v_adv,
+                         const double* rho_c, const double* rho_u, const double* rho_v,
+                         const double* kappa, const double* zeta, const double* eta,
+                         double dt_step, double tol, int max_iter, int* iter_out, double* res_out,
+                         const double* u_top_lid, const double* penalty_u, const double* penalty_v,
+                         const double* mdot_u_cons, const double* mdot_v_cons, unsigned flags);
+/* src/mod_vof.f90:38-140 (flux outputs NULL = absent) */
+int fugu_vof_advection_tvd(fugu_ctx* ctx, const double* c_in, double* c_out,
+                           const double* u_vel, const double* v_vel, double dt_step,
+                           double* flux_x_out, double* flux_y_out);
+/* src/mod_vof.f90:145-194 (max_iter <= 0 / tol <= 0 select the defaults 100 / 1e-6) */
+int fugu_svof_helmholtz_filter(fugu_ctx* ctx, const double* c_raw, double* c_smooth,
+                               double delta_filter, int max_iter, double tol, int* iter_out);
+/* src/mod_surface_tension.f90:25-72 and 78-109 (sigma from the context parameters) */
+int fugu_compute_interface_curvature(fugu_ctx* ctx, const double* c_smooth, double* kappa_curv);
+int fugu_compute_csf_force(fugu_ctx* ctx, const double* c_raw, const double* kappa_curv,
+                           double* f_sigma_u, double* f_sigma_v);
+/* src/mod_viscosity.f90:54-151 */
+int fugu_compute_mixture_densities(fugu_ctx* ctx, const double* c, double* rho_c, double* rho_u,
+                                   double* rho_v);
+int fugu_compute_mixture_viscosities(fugu_ctx* ctx, const double* c, double* kappa, double* zeta,
+                                     double* eta);
+/* src/mod_solver_coupled.f90:194-244 */
+int fugu_compute_adaptive_dt(fugu_ctx* ctx, const double* u, const double* v, double dt_max,
+                             double* dt_out);
+/* src/mod_solver_coupled.f90:290-533: every state array in/out (host or device). */
+int fugu_advance_coupled_step(fugu_ctx* ctx, double* u, double* v, double* p, double* c,
+                              double* u_prev, double* v_prev,
+                              double* rho_c, double* rho_u, double* rho_v,
+                              double* kappa, double* zeta, double* eta,
+                              double* kappa_curv, double* f_sigma_u, double* f_sigma_v,
+                              double dt_step, int step_idx, const fugu_step_opts* opts,
+                              const double* phi_s_u, const double* phi_s_v,
+                              int* iter_out, double* res_out);
+
+/* ---- device-resident API ------------------------------------------------------------- */
+/* The six restart fields of src/mod_io.f90:169-187 plus the solid mask (NULL = none). */
+int fugu_state_upload(fugu_ctx* ctx, const double* u, const double* v, const double* u_prev,
+                      const double* v_prev, const double* p, const double* c,
+                      const double* phi_s_u, const double* phi_s_v);
+int fugu_state_download(fugu_ctx* ctx, double* u, double* v, double* u_prev, double* v_prev,
+                        double* p, double* c);
+/* Download any device field by name ("rho_c", "kappa_curv", "f_sigma_u", ...). */
+int fugu_field_download(fugu_ctx* ctx, const char* name, double* out);
+/* Mixture properties from the resident C (src/main.f90:185-186). */
+int fugu_state_init_properties(fugu_ctx* ctx);
+
+typedef struct {
+  int iters;              /* Krylov iterations (inner ones summed for the refined solve) */
+  int outer;              /* refinement steps (0 for the reference solver) */
+  double res;             /* reference: recurrence residual; refined: true relative residual */
+  double true_res;        /* ||B - A X|| / ||B|| in the coupled_dot norm, FP64 */
+  double solve_seconds;   /* device time of the coupled solve (HIP events) */
+  double step_seconds;    /* device time of the whole step */
+  int pcg_calls;          /* preconditioner applications */
+} fugu_step_stats;
+
+/* One advance_coupled_step on the resident state.  stats may be NULL. */
+int fugu_step(fugu_ctx* ctx, double dt_step, int step_idx, const fugu_step_opts* opts,
+              fugu_step_stats* stats);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,244 @@
+"""GPU parity of every fine C-ABI entry point against the CPU oracle (same seeded inputs).
+
+Element-wise kernels are compared bit for bit (FP64, no contraction, same operation order);
+routines with global reductions to the tolerances stated at each test.  Every call goes
+through the C ABI of libfugu2d_hip.so."""
+import numpy as np
+import pytest
+
+import pyoracle as O
+from cases import interior, rand, two_phase_case, droplet_c
+
+pytestmark = pytest.mark.gpu
+
+abi = pytest.importorskip("coupled_codefort_2d_amd.abi")
+
+GRIDS = [
+    dict(nx=24, ny=20, lx=1.2, ly=1.0, per_x=False, per_y=False),
+    dict(nx=16, ny=24, lx=1.0, ly=1.5, per_x=True, per_y=False),
+    dict(nx=20, ny=16, lx=1.0, ly=1.0, per_x=True, per_y=True),
+]
+IDS = ["walls", "periodic_x", "periodic_xy"]
+
+
+def ctx_for(g: O.Grid, phys=None):
+    return abi.Context(g.nx, g.ny, g.lx, g.ly, g.per_x, g.per_y,
+                       params=abi.FuguParams(*(phys or (1, 1, 1, 1, 0, 0, 0, 1, 1e12, 0.5))))
+
+
+def same(a, b, what=""):
+    if not np.array_equal(a, b):
+        d = np.abs(a - b)
+        k = np.unravel_index(np.argmax(d), d.shape)
+        raise AssertionError(f"{what}: not bit-identical, max |diff| {d.max():.3e} at {k} "
+                             f"({a[k]!r} vs {b[k]!r})")
+
+
+def close(a, b, rtol, what=""):
+    scale = max(np.abs(b).max(), 1e-300)
+    err = np.abs(a - b).max() / scale
+    assert err <= rtol, f"{what}: max rel diff {err:.3e} > {rtol:.1e}"
+
+
+def grid(kw):
+    return O.Grid(kw["nx"], kw["ny"], kw["lx"], kw["ly"], per_x=kw["per_x"], per_y=kw["per_y"])
+
+
+@pytest.mark.parametrize("kw", GRIDS, ids=IDS)
+def test_halo_and_bcs_bitwise(kw):
+    g = grid(kw)
+    ctx = ctx_for(g)
+    rng = np.random.default_rng(100)
+    for kind in "uvp":
+        a = rand(g, kind, rng)
+        b = a.copy()
+        O.halo(g, a, kind)
+        ctx.halo(b, kind)
+        same(b, a, f"halo {kind}")
+    u, v = rand(g, "u", rng), rand(g, "v", rng)
+    u2, v2 = u.copy(), v.copy()
+    O.velocity_bcs(g, u, v, 1.25)
+    ctx.velocity_bcs(u2, v2, 1.25)
+    same(u2, u, "bcs u"); same(v2, v, "bcs v")
+    O.velocity_bcs(g, u, v, 0.0)
+    ctx.velocity_bcs(u2, v2, None)
+    same(u2, u, "bcs u hom"); same(v2, v, "bcs v hom")
+
+
+@pytest.mark.parametrize("kw", GRIDS, ids=IDS)
+@pytest.mark.parametrize("variant", ["plain", "noadv", "penalty", "amp"])
+def test_matvec_bitwise(kw, variant):
+    g = grid(kw)
+    ctx = ctx_for(g)
+    rng = np.random.default_rng(101)
+    c, pr, ua, va = two_phase_case(g, rng)
+    X = [rand(g, k, rng) for k in "uvp"]
+    pen = (rng.uniform(0, 1e3, g.shape("u")), rng.uniform(0, 1e3, g.shape("v"))) if variant == "penalty" else None
+    mdot = (rand(g, "u", rng), rand(g, "v", rng)) if variant == "amp" else None
+    adv = variant != "noadv"
+    Xo = [a.copy() for a in X]
+    Yo = O.matvec(g, Xo, pr, ua, va, 3.7e-3, include_advection=adv, pen=pen, mdot=mdot)
+    Xg = [a.copy() for a in X]
+    Yg = [g.zeros(k) for k in "uvp"]
+    ctx.matvec(Xg, Yg, pr, ua, va, 3.7e-3, include_advection=adv, pen=pen, mdot=mdot)
+    for k in range(3):
+        same(Yg[k], Yo[k], f"Y[{k}]")
+        same(Xg[k], Xo[k], f"X[{k}] (halo in place)")
+
+
+@pytest.mark.parametrize("kw", GRIDS, ids=IDS)
+def test_coupled_dot(kw):
+    g = grid(kw)
+    ctx = ctx_for(g)
+    rng = np.random.default_rng(102)
+    a = [rand(g, k, rng) for k in "uvp"]
+    b = [rand(g, k, rng) for k in "uvp"]
+    ref = O.coupled_dot(g, a, b)
+    got = ctx.coupled_dot(a, b)
+    assert abs(got - ref) <= 1e-13 * (abs(ref) + np.sqrt(O.coupled_dot(g, a, a) * O.coupled_dot(g, b, b)))
+
+
+@pytest.mark.parametrize("kw", GRIDS, ids=IDS)
+def test_mixture_bitwise(kw):
+    g = grid(kw)
+    ctx = ctx_for(g, (1000.0, 1.0, 1.0, 0.01, 0, 0, 0, 1, 1e12, 0.5))
+    rng = np.random.default_rng(103)
+    c = rng.uniform(-0.2, 1.2, g.shape("p"))
+    ref = O.mixture(g, c, 1000.0, 1.0, 1.0, 0.01)
+    out = {k: np.zeros_like(a) for k, a in ref.items()}
+    ctx.mixture(c, out["rho_c"], out["rho_u"], out["rho_v"], out["kappa"], out["zeta"], out["eta"])
+    for k in ref:
+        same(out[k], ref[k], k)
+
+
+@pytest.mark.parametrize("kw", GRIDS, ids=IDS)
+def test_jacobi_sweeps_bitwise(kw):
+    g = grid(kw)
+    ctx = ctx_for(g)
+    rng = np.random.default_rng(104)
+    c, pr, _, _ = two_phase_case(g, rng)
+    for n_sweeps in (1, 3, 5):
+        ru, rv = rand(g, "u", rng), rand(g, "v", rng)
+        x0u, x0v = rand(g, "u", rng), rand(g, "v", rng)
+        pen_u = rng.uniform(0, 10, g.shape("u"))
+        a, b = x0u.copy(), x0u.copy()
+        O.helmholtz_u(g, a, ru, pr, 2e-3, n_sweeps, pen_u)
+        ctx.helmholtz_u(b, ru, pr, 2e-3, n_sweeps, pen_u)
+        same(b, a, f"helmholtz_u x{n_sweeps}")
+        a, b = x0v.copy(), x0v.copy()
+        O.helmholtz_v(g, a, rv, pr, 2e-3, n_sweeps)
+        ctx.helmholtz_v(b, rv, pr, 2e-3, n_sweeps)
+        same(b, a, f"helmholtz_v x{n_sweeps}")
+
+
+@pytest.mark.parametrize("kw", GRIDS, ids=IDS)
+def test_poisson_pcg(kw):
+    """Truncated 150-iteration PCG: same iteration count, iterate within 1e-10 relative
+    (reductions are summed in a different order than the reference's sequential loop)."""
+    g = grid(kw)
+    ctx = ctx_for(g)
+    rng = np.random.default_rng(105)
+    c, pr, _, _ = two_phase_case(g, rng, rho=(100.0, 1.0))
+    if g.per_x or g.per_y:
+        O.halo(g, pr["rho_u"], "u"); O.halo(g, pr["rho_v"], "v")
+    rhs = rand(g, "p", rng)
+    if g.per_x or g.per_y:
+        rhs[interior(g, "p")] -= rhs[interior(g, "p")].mean()
+    for tol, phi0 in ((1e-5, None), (1e-9, "rand")):
+        phi_o = g.zeros("p") if phi0 is None else rand(g, "p", rng)
+        phi_g = phi_o.copy()
+        it_o, res_o = O.poisson_pcg(g, phi_o, rhs, pr["rho_u"], pr["rho_v"], tol=tol, max_iter=150)
+        it_g, res_g = ctx.poisson_pcg(phi_g, rhs, pr["rho_u"], pr["rho_v"], tol=tol, max_iter=150)
+        assert it_g == it_o
+        assert abs(res_g - res_o) <= 1e-12 * max(res_o, 1.0)
+        close(phi_g, phi_o, 1e-10, "phi")
+    # zero rhs early exit
+    phi = np.ones(g.shape("p"))
+    it, res = ctx.poisson_pcg(phi, g.zeros("p"), pr["rho_u"], pr["rho_v"])
+    assert (it, res) == (0, 0.0) and np.all(phi == 0)
+
+
+@pytest.mark.parametrize("kw", GRIDS, ids=IDS)
+def test_pcd_precond(kw):
+    g = grid(kw)
+    ctx = ctx_for(g)
+    rng = np.random.default_rng(106)
+    c, pr, ua, va = two_phase_case(g, rng)
+    if g.per_x or g.per_y:
+        for k, kind in (("rho_u", "u"), ("rho_v", "v"), ("rho_c", "p")):
+            O.halo(g, pr[k], kind)
+    Z = [rand(g, k, rng) for k in "uvp"]
+    if g.per_x or g.per_y:
+        Z[2][interior(g, "p")] -= Z[2][interior(g, "p")].mean()
+    Ro = O.pcd(g, Z, pr, ua, va, 1e-3)
+    Rg = [g.zeros(k) for k in "uvp"]
+    ctx.pcd(Z, Rg, pr, ua, va, 1e-3)
+    for k in range(3):
+        close(Rg[k], Ro[k], 1e-10, f"R[{k}]")
+
+
+@pytest.mark.parametrize("kw", GRIDS[:2], ids=IDS[:2])
+@pytest.mark.parametrize("order,consistent", [(1, False), (2, False), (2, True)],
+                         ids=["bicgstab", "bicgstab2_ref", "bicgstab2_bc"])
+def test_krylov_solve(kw, order, consistent):
+    """bicgstab_solve / bicgstab2_solve: same iteration count and the solution within 1e-10
+    relative (u, v, p) of the oracle's (both reduce with correctly rounded sums)."""
+    g = grid(kw)
+    ctx = ctx_for(g)
+    rng = np.random.default_rng(107)
+    c, pr, ua, va = two_phase_case(g, rng, rho=(10.0, 1.0), mu=(1.0, 0.1))
+    B = [rand(g, "u", rng), rand(g, "v", rng), g.zeros("p")]
+    X0 = [0.01 * rand(g, k, rng) for k in "uvp"]
+    for tol, lid in ((1e-6, None), (1e-9, 1.0)):
+        Xo = [a.copy() for a in X0]
+        Xg = [a.copy() for a in X0]
+        it_o, res_o = O.krylov(g, Xo, B, pr, ua, va, 5e-3, tol=tol, max_iter=40, order=order, u_lid=lid,
+                               consistent=consistent)
+        it_g, res_g = ctx.krylov(Xg, B, pr, ua, va, 5e-3, tol=tol, max_iter=40, order=order, u_lid=lid,
+                                 flags=abi.B2_CONSISTENT_BCS if consistent else 0)
+        assert it_g == it_o, (it_g, it_o)
+        for k in range(3):
+            close(Xg[k], Xo[k], 1e-10, f"X[{k}]")
+
+
+@pytest.mark.parametrize("kw", GRIDS, ids=IDS)
+def test_vof_bitwise(kw):
+    g = grid(kw)
+    ctx = ctx_for(g)
+    rng = np.random.default_rng(108)
+    c = np.clip(droplet_c(g, 0.3) + 0.2 * rng.standard_normal(g.shape("p")), -0.05, 1.05)
+    u, v = rand(g, "u", rng), rand(g, "v", rng)
+    co, fxo, fyo = O.vof_advection(g, c, u, v, 0.01, want_flux=True)
+    cg, fxg, fyg = g.zeros("p"), g.zeros("u"), g.zeros("v")
+    ctx.vof_advection(c, cg, u, v, 0.01, fxg, fyg)
+    same(cg, co, "C_out"); same(fxg, fxo, "flux_x"); same(fyg, fyo, "flux_y")
+
+
+@pytest.mark.parametrize("kw", GRIDS, ids=IDS)
+def test_svof_curvature_csf_bitwise(kw):
+    g = grid(kw)
+    ctx = ctx_for(g, (1, 1, 1, 1, 0.7, 0, 0, 1, 1e12, 0.5))
+    c = droplet_c(g, 0.3)
+    for it_max, tol in ((150, 1e-7), (7, 1e-12)):
+        cs_o, it_o = O.svof(g, c, 1.5 * g.dx, it_max, tol)
+        cs_g = g.zeros("p")
+        it_g = ctx.svof(c, cs_g, 1.5 * g.dx, it_max, tol)
+        assert it_g == it_o
+        same(cs_g, cs_o, "C_smooth")
+    kc_o = O.curvature(g, cs_o)
+    kc_g = g.zeros("p")
+    ctx.curvature(cs_o, kc_g)
+    same(kc_g, kc_o, "kappa")
+    fu_o, fv_o = O.csf(g, c, kc_o, 0.7)
+    fu_g, fv_g = g.zeros("u"), g.zeros("v")
+    ctx.csf(c, kc_o, fu_g, fv_g)
+    same(fu_g, fu_o, "f_u"); same(fv_g, fv_o, "f_v")
+
+
+def test_adaptive_dt():
+    g = O.Grid(32, 24, 1.0, 0.75)
+    ph = O.OgPhys(1000.0, 1.0, 1.0, 0.01, 0.5, 0.0, -9.81, 1, 1e12, 0.25)
+    ctx = ctx_for(g, (1000.0, 1.0, 1.0, 0.01, 0.5, 0.0, -9.81, 1, 1e12, 0.25))
+    rng = np.random.default_rng(109)
+    u, v = rand(g, "u", rng, 3.0), rand(g, "v", rng, 2.0)
+    assert ctx.adaptive_dt(u, v, 1.0) == O.adaptive_dt(g, ph, u, v, 1.0)
