@@ -32,7 +32,9 @@ class FuguParams(C.Structure):
 class FuguStepOpts(C.Structure):
     _fields_ = [("use_bicgstab2", C.c_int), ("has_lid", C.c_int), ("u_lid", C.c_double),
                 ("tol", C.c_double), ("max_iter", C.c_int), ("use_amp", C.c_int),
-                ("u_solid", C.c_double), ("v_solid", C.c_double), ("solver_flags", C.c_uint)]
+                ("u_solid", C.c_double), ("v_solid", C.c_double), ("solver_flags", C.c_uint),
+                ("inner_tol", C.c_double), ("inner_max_iter", C.c_int), ("mg_cycles", C.c_int),
+                ("restart", C.c_int)]
 
 
 class FuguStepStats(C.Structure):
@@ -43,6 +45,8 @@ class FuguStepStats(C.Structure):
 B2_CONSISTENT_BCS = 1
 SOLVE_REFINED = 2
 SOLVE_MIXED = 4
+PRECOND_MG = 8
+PRECOND_VISC = 16
 
 EXPORTS = [
     "fugu_ctx_create", "fugu_ctx_destroy", "fugu_set_params", "fugu_last_error", "fugu_field_size",
@@ -52,7 +56,8 @@ EXPORTS = [
     "fugu_svof_helmholtz_filter", "fugu_compute_interface_curvature", "fugu_compute_csf_force",
     "fugu_compute_mixture_densities", "fugu_compute_mixture_viscosities", "fugu_compute_adaptive_dt",
     "fugu_advance_coupled_step", "fugu_state_upload", "fugu_state_download", "fugu_field_download",
-    "fugu_state_init_properties", "fugu_step",
+    "fugu_state_init_properties", "fugu_step", "fugu_kernel_timing", "fugu_kernel_time",
+    "fugu_solve_pressure_mg",
 ]
 
 _lib = None
@@ -107,6 +112,9 @@ def _declare(L):
     s("fugu_field_download", [P, C.c_char_p, _D])
     s("fugu_state_init_properties", [P])
     s("fugu_step", [P, C.c_double, C.c_int, C.POINTER(FuguStepOpts), C.POINTER(FuguStepStats)])
+    s("fugu_kernel_timing", [P, C.c_int])
+    s("fugu_solve_pressure_mg", [P, _D, _D, _D, _D, C.c_int])
+    s("fugu_kernel_time", [P, C.c_char_p, _D, C.POINTER(C.c_longlong)])
 
 
 def ptr(a):
@@ -186,6 +194,10 @@ class Context:
         check(load().fugu_solve_poisson_pressure(self.h, ptr(phi), ptr(rhs), ptr(rho_u), ptr(rho_v), tol,
                                                  max_iter, C.byref(it), C.byref(res)), "poisson")
         return it.value, res.value
+
+    def pressure_mg(self, phi, rhs, rho_u, rho_v, n_cycles=1):
+        check(load().fugu_solve_pressure_mg(self.h, ptr(phi), ptr(rhs), ptr(rho_u), ptr(rho_v), n_cycles),
+              "pressure_mg")
 
     def helmholtz_u(self, u_sol, rhs, props, dt, n_sweeps=5, pen_u=None):
         check(load().fugu_solve_helmholtz_u(self.h, ptr(u_sol), ptr(rhs), ptr(props["rho_u"]), ptr(props["kappa"]),
@@ -269,6 +281,15 @@ class Context:
     def init_properties(self):
         check(load().fugu_state_init_properties(self.h), "init_properties")
 
+    def kernel_timing(self, enable=True):
+        check(load().fugu_kernel_timing(self.h, int(enable)), "kernel_timing")
+
+    def kernel_time(self, name):
+        """(total_ms, launches) of an instrumented kernel since timing was enabled."""
+        t, n = C.c_double(0), C.c_longlong(0)
+        check(load().fugu_kernel_time(self.h, name.encode(), C.byref(t), C.byref(n)), "kernel_time")
+        return t.value, n.value
+
     def step(self, dt, step_idx, opts: FuguStepOpts):
         st = FuguStepStats()
         rc = check(load().fugu_step(self.h, dt, step_idx, C.byref(opts), C.byref(st)), "step")
@@ -276,6 +297,7 @@ class Context:
 
 
 def step_opts(use_bicgstab2=True, u_lid=None, tol=1e-6, max_iter=500, use_amp=False, u_solid=0.0, v_solid=0.0,
-              solver_flags=0):
+              solver_flags=0, inner_tol=0.0, inner_max_iter=0, mg_cycles=0, restart=0):
     return FuguStepOpts(int(use_bicgstab2), int(u_lid is not None), float(u_lid or 0.0), tol, max_iter,
-                        int(use_amp), u_solid, v_solid, solver_flags)
+                        int(use_amp), u_solid, v_solid, solver_flags, inner_tol, inner_max_iter, mg_cycles,
+                        restart)

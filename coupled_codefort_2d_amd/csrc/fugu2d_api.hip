@@ -5,10 +5,14 @@
 // 12-byte copy per Krylov iteration) and the scalars a caller asks for.
 #include "fugu2d.h"
 #include "fugu2d_kernels.hpp"
+#include "fugu2d_mg.hpp"
 
 #include <cmath>
+#include <cstdlib>
 #include <cstdio>
+#include <algorithm>
 #include <cstring>
+#include <map>
 #include <stdexcept>
 #include <string>
 #include <utility>
@@ -36,9 +40,25 @@ struct Error : std::runtime_error {
     HIPCHK(hipGetLastError());                                  \
   } while (0)
 
+// LAUNCH bracketed by timing events when the context's kernel timing is on.
+#define TLAUNCH(ctx, name, kern, grid, block, stream, ...)                       \
+  do {                                                                           \
+    KTimer* kt_ = (ctx)->kt_on ? kt_slot(ctx, name) : nullptr;                   \
+    if (kt_) HIPCHK(hipEventRecord(kt_->ev[2 * kt_->used], stream));             \
+    LAUNCH(kern, grid, block, stream, __VA_ARGS__);                              \
+    if (kt_) { HIPCHK(hipEventRecord(kt_->ev[2 * kt_->used + 1], stream)); kt_->used++; } \
+  } while (0)
+
+template <typename T>
+struct MgLevel {
+  MgDims d;
+  T *cx = nullptr, *cy = nullptr, *dinv = nullptr, *b = nullptr, *phi = nullptr;
+};
+
 // Per-precision solver workspace.  Triplets are contiguous [u | v | p].
 template <typename T>
 struct Work {
+  std::vector<MgLevel<T>> mg;   // pressure multigrid hierarchy (chosen fast forms)
   T *X = nullptr, *B = nullptr, *bh = nullptr, *r = nullptr, *rt = nullptr, *uu = nullptr;
   T *vv = nullptr, *ww = nullptr, *ss = nullptr, *tt = nullptr, *z = nullptr, *q = nullptr;
   // properties (aliases of the context's FP64 fields for T = double)
@@ -52,6 +72,16 @@ struct Work {
 
 }  // namespace
 
+// Optional per-kernel device timing (HIP events around selected launches on the
+// context stream); resolved after each synchronising call.
+struct KTimer {
+  std::vector<hipEvent_t> ev;   // start/stop pairs
+  int used = 0;
+  double total_ms = 0.0;
+  long long count = 0;
+};
+constexpr int KT_POOL = 1024;
+
 struct fugu_ctx {
   Grid g;
   fugu_params prm;
@@ -61,7 +91,6 @@ struct fugu_ctx {
   double* sc = nullptr;
   int* fl = nullptr;
   double* partials = nullptr;
-  unsigned* ticket = nullptr;
   double* h_sc = nullptr;
   int* h_fl = nullptr;
   size_t ntrip = 0;
@@ -80,10 +109,40 @@ struct fugu_ctx {
   Work<double> wd;
   Work<float> wf;
   float *f_penu = nullptr, *f_penv = nullptr, *f_mdu = nullptr, *f_mdv = nullptr;
+  std::vector<double*> kv, kz;   // outer FGMRES basis and preconditioned directions (FP64 triplets)
+  double* fg = nullptr;          // outer FGMRES Hessenberg / Givens state (device)
+  hipEvent_t fg_ev[2] = {nullptr, nullptr};
   int pcg_calls = 0;
+  bool use_mg = false;   // pressure block by multigrid V-cycles instead of the reference PCG
+  int mg_cycles = 1;
+  double pcd_visc = 0.0; // viscous term of the Schur approximation (0 = the reference PCD)
+  bool kt_on = false;
+  std::map<std::string, KTimer> kt;
+  bool debug = std::getenv("FUGU_DEBUG") != nullptr;
 };
 
 namespace {
+
+KTimer* kt_slot(fugu_ctx* c, const char* name) {
+  KTimer& t = c->kt[name];
+  if (t.ev.empty()) {
+    t.ev.resize(2 * KT_POOL);
+    for (auto& e : t.ev) HIPCHK(hipEventCreate(&e));
+  }
+  return t.used < KT_POOL ? &t : nullptr;
+}
+void kt_resolve(fugu_ctx* c) {
+  for (auto& kv : c->kt) {
+    KTimer& t = kv.second;
+    for (int k = 0; k < t.used; ++k) {
+      float ms = 0.f;
+      HIPCHK(hipEventElapsedTime(&ms, t.ev[2 * k], t.ev[2 * k + 1]));
+      t.total_ms += ms;
+      t.count++;
+    }
+    t.used = 0;
+  }
+}
 
 template <typename T>
 T* dalloc(fugu_ctx* c, size_t n) {
@@ -105,7 +164,15 @@ void set_slot(fugu_ctx* c, int slot, double v) {
   HIPCHK(hipMemcpyAsync(c->sc + slot, &v, sizeof(double), hipMemcpyHostToDevice, c->s));
   HIPCHK(hipStreamSynchronize(c->s));  // v lives on the host stack
 }
-void reset_flags(fugu_ctx* c) { HIPCHK(hipMemsetAsync(c->fl, 0, F_COUNT * sizeof(int), c->s)); }
+void reset_flags(fugu_ctx* c, bool relative_guards = false) {
+  HIPCHK(hipMemsetAsync(c->fl, 0, F_COUNT * sizeof(int), c->s));
+  if (relative_guards) {
+    static const int one = 1;
+    HIPCHK(hipMemcpyAsync(c->fl + F_RELG, &one, sizeof(int), hipMemcpyHostToDevice, c->s));
+  }
+  static const double unit = 1.0;
+  HIPCHK(hipMemcpyAsync(c->sc + S_GSCALE, &unit, sizeof(double), hipMemcpyHostToDevice, c->s));
+}
 void read_flags(fugu_ctx* c) {
   HIPCHK(hipMemcpyAsync(c->h_fl, c->fl, F_COUNT * sizeof(int), hipMemcpyDeviceToHost, c->s));
   HIPCHK(hipStreamSynchronize(c->s));
@@ -147,7 +214,7 @@ void bcs_trip(fugu_ctx* c, T* X, double lid = 0.0, const int* skip = nullptr) {
 }
 
 template <typename T>
-void matvec(fugu_ctx* c, Work<T>& w, T* X, T* Y, T dt, int flags, const int* skip) {
+void matvec(fugu_ctx* c, Work<T>& w, T* X, T* Y, double dt, int flags, const int* skip) {
   const Grid& g = c->g;
   halo_trip(c, X, skip);
   MatvecArgs<T> a;
@@ -157,7 +224,8 @@ void matvec(fugu_ctx* c, Work<T>& w, T* X, T* Y, T dt, int flags, const int* ski
   a.kap = w.kap; a.zeta = w.zeta; a.eta = w.eta;
   a.penu = w.penu; a.penv = w.penv; a.mdu = w.mdu; a.mdv = w.mdv;
   a.dt = dt; a.flags = flags;
-  LAUNCH((k_matvec<T>), grid_union(g), dim3(TX, TY), c->s, g, a, skip);
+  TLAUNCH(c, sizeof(T) == 8 ? "matvec_f64" : "matvec_f32", (k_matvec<T>), grid_union(g), dim3(TX, TY), c->s,
+          g, a, skip);
 }
 
 template <typename T>
@@ -165,13 +233,34 @@ void cdot(fugu_ctx* c, int np, const T* const* a, const T* const* b, int phase, 
   DotArgs<T> d;
   for (int q = 0; q < 5; ++q) { d.a[q] = q < np ? a[q] : nullptr; d.b[q] = q < np ? b[q] : nullptr; }
   d.np = np; d.phase = phase; d.k = k; d.max_iter = max_iter;
-  const dim3 gr = grid_union(c->g), bl(TX, TY);
+  const dim3 gr = grid_union_red(c->g), bl(TX, TY);
+  const int nb = (int)nblocks(gr);
+  const int* fl = c->fl;
+  const double* pa = c->partials;
   switch (np) {
-    case 1: LAUNCH((k_cdot<T, 1>), gr, bl, c->s, c->g, d, c->partials, c->ticket, c->sc, c->fl); break;
-    case 2: LAUNCH((k_cdot<T, 2>), gr, bl, c->s, c->g, d, c->partials, c->ticket, c->sc, c->fl); break;
-    case 5: LAUNCH((k_cdot<T, 5>), gr, bl, c->s, c->g, d, c->partials, c->ticket, c->sc, c->fl); break;
+    case 1:
+      LAUNCH((k_cdot<T, 1>), gr, bl, c->s, c->g, d, c->partials, fl);
+      LAUNCH((k_fold_krylov<1>), dim3(1), dim3(NT), c->s, nb, phase, k, max_iter, pa, c->sc, c->fl);
+      break;
+    case 2:
+      LAUNCH((k_cdot<T, 2>), gr, bl, c->s, c->g, d, c->partials, fl);
+      LAUNCH((k_fold_krylov<2>), dim3(1), dim3(NT), c->s, nb, phase, k, max_iter, pa, c->sc, c->fl);
+      break;
+    case 5:
+      LAUNCH((k_cdot<T, 5>), gr, bl, c->s, c->g, d, c->partials, fl);
+      LAUNCH((k_fold_krylov<5>), dim3(1), dim3(NT), c->s, nb, phase, k, max_iter, pa, c->sc, c->fl);
+      break;
     default: throw Error("cdot: unsupported pair count");
   }
+}
+
+// sc[slot] = interior sum of a cell field / (nx ny) (two passes); skip_mode as k_pcg_sum.
+template <typename T>
+void cell_mean(fugu_ctx* c, const T* a, int slot, int skip_mode) {
+  const dim3 gr = grid_cells_red(c->g);
+  LAUNCH((k_pcg_sum<T>), gr, dim3(TX, TY), c->s, c->g, a, c->partials, skip_mode, (const int*)c->fl);
+  LAUNCH(k_fold_sum, dim3(1), dim3(NT), c->s, c->g, (int)nblocks(gr), slot, skip_mode, (const double*)c->partials,
+         c->sc, (const int*)c->fl);
 }
 template <typename T>
 void cdot1(fugu_ctx* c, const T* a, const T* b, int phase, int k = 0, int mi = 0) {
@@ -208,6 +297,22 @@ void work_alloc(fugu_ctx* c, Work<T>& w, bool own_props) {
   w.cx = dalloc<T>(c, g.nu); w.cy = dalloc<T>(c, g.nv);
   w.zuw = dalloc<T>(c, g.nu); w.ju0 = dalloc<T>(c, g.nu); w.ju1 = dalloc<T>(c, g.nu);
   w.zvw = dalloc<T>(c, g.nv); w.jv0 = dalloc<T>(c, g.nv); w.jv1 = dalloc<T>(c, g.nv);
+  // multigrid levels: factor-2 coarsening while both extents are even and >= 8
+  MgDims d{g.nx, g.ny, g.per_x, g.per_y};
+  for (;;) {
+    MgLevel<T> L;
+    L.d = d;
+    const size_t n = (size_t)d.nx * d.ny;
+    L.cx = dalloc<T>(c, (size_t)(d.nx + 1) * d.ny);
+    L.cy = dalloc<T>(c, (size_t)d.nx * (d.ny + 1));
+    L.dinv = dalloc<T>(c, n);
+    L.b = dalloc<T>(c, n);
+    L.phi = dalloc<T>(c, n);
+    w.mg.push_back(L);
+    if (d.nx % 2 || d.ny % 2 || d.nx < 8 || d.ny < 8) break;
+    d.nx /= 2;
+    d.ny /= 2;
+  }
   w.ready = true;
 }
 
@@ -256,20 +361,101 @@ template <typename T>
 void pcg(fugu_ctx* c, Work<T>& w, T* phi, const T* rhs, bool phi_zero, double tol, int max_iter) {
   const Grid& g = c->g;
   const int walls_both = (!g.per_x && !g.per_y) ? 1 : 0;
-  const dim3 gc = grid_cells(g), bl(TX, TY);
+  const dim3 gc = grid_cells_red(g), bl(TX, TY), b1(NT);
+  const int nb = (int)nblocks(gc);
+  const double* pa = c->partials;
+  const double* sc = c->sc;
+  const int* fl = c->fl;
   c->pcg_calls++;
-  if (walls_both)
-    LAUNCH((k_pcg_sum<T>), gc, bl, c->s, g, rhs, c->partials, c->ticket, c->sc, (int)S_P_MEAN, 1, c->fl);
+  if (walls_both) cell_mean<T>(c, rhs, (int)S_P_MEAN, 1);
   LAUNCH((k_pcg_init<T>), gc, bl, c->s, g, rhs, phi, w.cx, w.cy, w.dinv, w.pr, w.pz, w.pp0, walls_both,
-         phi_zero ? 1 : 0, c->partials, c->ticket, c->sc, c->fl);
+         phi_zero ? 1 : 0, c->partials, sc, fl);
+  LAUNCH(k_fold_pcg_init, dim3(1), b1, c->s, nb, pa, c->sc, c->fl);
   T* P[2] = {w.pp0, w.pp1};
   for (int it = 1; it <= max_iter; ++it) {
-    LAUNCH((k_pcg_iter1<T>), gc, bl, c->s, g, it, w.pz, P[(it - 1) & 1], P[it & 1], w.Ap, w.cx, w.cy,
-           c->partials, c->ticket, c->sc, c->fl);
-    LAUNCH((k_pcg_iter2<T>), gc, bl, c->s, g, it, max_iter, phi, P[it & 1], w.pr, w.Ap, w.pz, w.dinv,
-           c->partials, c->ticket, c->sc, c->fl);
+    TLAUNCH(c, sizeof(T) == 8 ? "pcg_iter1_f64" : "pcg_iter1_f32", (k_pcg_iter1<T>), gc, bl, c->s, g, it,
+            w.pz, P[(it - 1) & 1], P[it & 1], w.Ap, w.cx, w.cy, c->partials, sc, fl);
+    LAUNCH(k_fold_pcg_iter1, dim3(1), b1, c->s, nb, it, pa, c->sc, c->fl);
+    TLAUNCH(c, sizeof(T) == 8 ? "pcg_iter2_f64" : "pcg_iter2_f32", (k_pcg_iter2<T>), gc, bl, c->s, g, phi,
+            P[it & 1], w.pr, w.Ap, w.pz, w.dinv, c->partials, sc, fl);
+    LAUNCH(k_fold_pcg_iter2, dim3(1), b1, c->s, nb, it, max_iter, pa, c->sc, c->fl);
   }
   (void)tol;
+}
+
+// ---------------------------------------------------------------------------------
+// Pressure multigrid (chosen fast forms).
+inline dim3 g2(int nx, int ny, int bx = 64, int by = 4) {
+  return dim3((unsigned)((nx + bx - 1) / bx), (unsigned)((ny + by - 1) / by));
+}
+
+template <typename T>
+void mg_setup(fugu_ctx* c, Work<T>& w) {
+  const Grid& g = c->g;
+  const dim3 bl(64, 4);
+  LAUNCH((k_mg_coef0<T>), g2(g.nx + 1, g.ny + 1), bl, c->s, g, w.rho_u, w.rho_v, w.mg[0].cx, w.mg[0].cy);
+  for (size_t l = 0; l + 1 < w.mg.size(); ++l) {
+    const MgDims& f = w.mg[l].d;
+    LAUNCH((k_mg_coarsen<T>), g2(f.nx / 2 + 1, f.ny / 2 + 1), bl, c->s, f, (const T*)w.mg[l].cx,
+           (const T*)w.mg[l].cy, w.mg[l + 1].cx, w.mg[l + 1].cy);
+  }
+  for (auto& L : w.mg)
+    LAUNCH((k_mg_dinv<T>), g2(L.d.nx, L.d.ny), bl, c->s, L.d, (const T*)L.cx, (const T*)L.cy, L.dinv);
+}
+
+template <typename T>
+void mg_smooth(fugu_ctx* c, MgLevel<T>& L, int colour, int zero_other) {
+  const int half = (L.d.nx + 1) / 2;
+  TLAUNCH(c, sizeof(T) == 8 ? "mg_smooth_f64" : "mg_smooth_f32", (k_mg_rbgs<T>), g2(half, L.d.ny), dim3(64, 4),
+          c->s, L.d, colour, (const T*)L.cx, (const T*)L.cy, (const T*)L.dinv, (const T*)L.b, L.phi, zero_other);
+}
+
+// V-cycle on level l; `zero_guess`: phi of level l is (treated as) zero on entry.
+template <typename T>
+void mg_vcycle(fugu_ctx* c, Work<T>& w, size_t l, bool zero_guess) {
+  MgLevel<T>& L = w.mg[l];
+  const int nu = 2;
+  if (l + 1 == w.mg.size()) {
+    if ((size_t)L.d.nx * L.d.ny <= (size_t)MG_COARSE_MAX) {
+      LAUNCH((k_mg_coarse<T>), dim3(1), dim3(MG_COARSE_MAX), c->s, L.d, (const T*)L.cx, (const T*)L.cy,
+             (const T*)L.dinv, (const T*)L.b, L.phi, 60, 1);
+    } else {
+      for (int s = 0; s < 40; ++s) {
+        mg_smooth(c, L, 0, (s == 0 && zero_guess) ? 1 : 0);
+        mg_smooth(c, L, 1, 0);
+      }
+    }
+    return;
+  }
+  for (int s = 0; s < nu; ++s) {
+    mg_smooth(c, L, 0, (s == 0 && zero_guess) ? 1 : 0);
+    mg_smooth(c, L, 1, 0);
+  }
+  MgLevel<T>& C = w.mg[l + 1];
+  TLAUNCH(c, sizeof(T) == 8 ? "mg_resid_f64" : "mg_resid_f32", (k_mg_restrict<T>), g2(C.d.nx, C.d.ny),
+          dim3(64, 4), c->s, L.d, (const T*)L.cx, (const T*)L.cy, (const T*)L.b, (const T*)L.phi, C.b, C.phi);
+  mg_vcycle(c, w, l + 1, true);
+  LAUNCH((k_mg_prolong<T>), g2(L.d.nx, L.d.ny), dim3(64, 4), c->s, L.d, (const T*)C.phi, L.phi);
+  for (int s = 0; s < nu; ++s) {
+    mg_smooth(c, L, 1, 0);
+    mg_smooth(c, L, 0, 0);
+  }
+}
+
+// phi (padded, interior written) ~= A_p^-1 (z_p - mean) by mg_cycles V-cycles from zero.
+// Leaves S_P_MEAN = mean(phi) (both directions walls) and F_P_DONE = 0 for k_pcd_rp.
+template <typename T>
+void mg_pressure(fugu_ctx* c, Work<T>& w, T* phi, const T* Zp) {
+  const Grid& g = c->g;
+  const int walls_both = (!g.per_x && !g.per_y) ? 1 : 0;
+  c->pcg_calls++;
+  HIPCHK(hipMemsetAsync(c->fl + F_P_DONE, 0, sizeof(int), c->s));
+  if (walls_both) cell_mean<T>(c, Zp, (int)S_P_MEAN, 0);
+  LAUNCH((k_mg_load<T>), g2(g.nx, g.ny), dim3(64, 4), c->s, g, Zp, w.mg[0].b, walls_both, (const double*)c->sc,
+         (int)S_P_MEAN);
+  for (int k = 0; k < c->mg_cycles; ++k) mg_vcycle(c, w, 0, k == 0);   // later cycles smooth from phi
+  LAUNCH((k_mg_store<T>), g2(g.nx, g.ny), dim3(64, 4), c->s, g, (const T*)w.mg[0].phi, phi);
+  if (walls_both) cell_mean<T>(c, (const T*)phi, (int)S_P_MEAN, 0);
 }
 
 // H6: block PCD preconditioner R = P^-1 Z (triplets).
@@ -280,11 +466,15 @@ void pcd(fugu_ctx* c, Work<T>& w, const T* Z, T* R, T dt) {
   const T *Zu = Z, *Zv = Z + g.nu, *Zp = Z + g.nu + g.nv;
   T *Ru = R, *Rv = R + g.nu, *Rp = R + g.nu + g.nv;
   const dim3 gu = grid_union(g), gc = grid_cells(g), bl(TX, TY);
-  pcg<T>(c, w, w.phi, Zp, true, 1.0e-5, 150);
-  if (walls_both)   // mean of phi, unless the zero-rhs early exit fired (skip when F_P_DONE >= 2)
-    LAUNCH((k_pcg_sum<T>), gc, bl, c->s, g, (const T*)w.phi, c->partials, c->ticket, c->sc, (int)S_P_MEAN,
-           2, c->fl);
-  LAUNCH((k_pcd_rp<T>), gu, bl, c->s, g, w.phi, w.uadv, w.vadv, dt, walls_both, Rp, c->sc, c->fl);
+  if (c->use_mg) {
+    mg_pressure<T>(c, w, w.phi, Zp);
+  } else {
+    pcg<T>(c, w, w.phi, Zp, true, 1.0e-5, 150);
+    if (walls_both)   // mean of phi, unless the zero-rhs early exit fired (skip when F_P_DONE >= 2)
+      cell_mean<T>(c, (const T*)w.phi, (int)S_P_MEAN, 2);
+  }
+  LAUNCH((k_pcd_rp<T>), gu, bl, c->s, g, w.phi, w.uadv, w.vadv, dt, walls_both, Rp, c->sc, c->fl, Zp, w.kap,
+         (T)c->pcd_visc);
   halo(c, Rp, 2);
   LAUNCH((k_pcd_zwork<T>), gu, bl, c->s, g, Zu, Zv, (const T*)Rp, w.zuw, w.zvw);
   LAUNCH((k_jacobi_v<T>), gu, bl, c->s, g, (const T*)nullptr, w.jv0, w.zvw, w.rho_v, w.kap, w.zeta, w.eta, dt, w.penv);
@@ -301,7 +491,7 @@ void pcd(fugu_ctx* c, Work<T>& w, const T* Z, T* R, T dt) {
 // ---------------------------------------------------------------------------------
 // H5: BiCGStab (src/mod_bicgstab2.f90:77-264), exact reference recurrences.
 template <typename T>
-void bicgstab1(fugu_ctx* c, Work<T>& w, T dt, double tol, int max_iter, int mvflags, bool has_lid,
+void bicgstab1(fugu_ctx* c, Work<T>& w, double dt, double tol, int max_iter, int mvflags, bool has_lid,
                double lid, int* iters, double* res) {
   const int* brk = c->fl + F_BRK;
   T *X = w.X, *r = w.r, *r0 = w.rt, *p = w.uu, *v = w.vv, *s = w.ss, *t = w.tt, *z = w.z;
@@ -309,11 +499,12 @@ void bicgstab1(fugu_ctx* c, Work<T>& w, T dt, double tol, int max_iter, int mvfl
   set_slot(c, S_TOL, tol);
   set_slot(c, S_P_TOL, 1.0e-5);
   pcg_setup(c, w);
+  if (c->use_mg) mg_setup(c, w);
   bcs_trip(c, X);
-  pcd(c, w, w.B, w.bh, dt);
+  pcd(c, w, w.B, w.bh, (T)dt);
   cdot1<T>(c, w.bh, w.bh, PH_NORM_B);
   matvec(c, w, X, z, dt, mvflags, nullptr);
-  pcd(c, w, z, v, dt);
+  pcd(c, w, z, v, (T)dt);
   lin<T>(c, L_DIFF, r, w.bh, v);
   lin<T>(c, L_COPY, r0, r);
   lin<T>(c, L_COPY, p, r);
@@ -328,13 +519,13 @@ void bicgstab1(fugu_ctx* c, Work<T>& w, T dt, double tol, int max_iter, int mvfl
     for (k = 1; k <= max_iter; ++k) {
       bcs_trip(c, p, 0.0, brk);
       matvec(c, w, p, z, dt, mvflags, brk);
-      pcd(c, w, z, v, dt);
+      pcd(c, w, z, v, (T)dt);
       bcs_trip(c, v, 0.0, brk);
       cdot1<T>(c, r0, v, PH_B1_RV, k, max_iter);
       lin<T>(c, L_SUB1, s, r, v, nullptr, nullptr, S_ALPHA);
       bcs_trip(c, s, 0.0, brk);
       matvec(c, w, s, z, dt, mvflags, brk);
-      pcd(c, w, z, t, dt);
+      pcd(c, w, z, t, (T)dt);
       bcs_trip(c, t, 0.0, brk);
       {
         const T* A[2] = {t, t};
@@ -364,19 +555,20 @@ void bicgstab1(fugu_ctx* c, Work<T>& w, T dt, double tol, int max_iter, int mvfl
 // H4: BiCGStab(2) (src/mod_bicgstab2.f90:270-570).  consistent = BCs on r and s before
 // their matvecs (not in the reference; see fugu2d.h FUGU_B2_CONSISTENT_BCS).
 template <typename T>
-void bicgstab2(fugu_ctx* c, Work<T>& w, T dt, double tol, int max_iter, int mvflags, bool has_lid,
-               double lid, bool consistent, int* iters, double* res) {
+void bicgstab2(fugu_ctx* c, Work<T>& w, double dt, double tol, int max_iter, int mvflags, bool has_lid,
+               double lid, bool consistent, int* iters, double* res, bool relative_guards = false) {
   const int* brk = c->fl + F_BRK;
   T *X = w.X, *r = w.r, *rt = w.rt, *uu = w.uu, *vv = w.vv, *ww = w.ww, *ss = w.ss, *tt = w.tt, *z = w.z;
-  reset_flags(c);
+  reset_flags(c, relative_guards);
   set_slot(c, S_TOL, tol);
   set_slot(c, S_P_TOL, 1.0e-5);
   pcg_setup(c, w);
+  if (c->use_mg) mg_setup(c, w);
   bcs_trip(c, X);
-  pcd(c, w, w.B, w.bh, dt);
+  pcd(c, w, w.B, w.bh, (T)dt);
   cdot1<T>(c, w.bh, w.bh, PH_NORM_B);
   matvec(c, w, X, z, dt, mvflags, nullptr);
-  pcd(c, w, z, w.q, dt);
+  pcd(c, w, z, w.q, (T)dt);
   lin<T>(c, L_DIFF, r, w.bh, w.q);
   lin<T>(c, L_COPY, rt, r);
   HIPCHK(hipMemsetAsync(uu, 0, c->ntrip * sizeof(T), c->s));
@@ -390,13 +582,13 @@ void bicgstab2(fugu_ctx* c, Work<T>& w, T dt, double tol, int max_iter, int mvfl
       lin<T>(c, L_SUB1, uu, r, uu, nullptr, nullptr, S_BETA);
       bcs_trip(c, uu, 0.0, brk);
       matvec(c, w, uu, z, dt, mvflags, brk);
-      pcd(c, w, z, vv, dt);
+      pcd(c, w, z, vv, (T)dt);
       bcs_trip(c, vv, 0.0, brk);
       cdot1<T>(c, rt, vv, PH_B2_GAMMA, k, max_iter);
       lin<T>(c, L_SUB1, r, r, vv, nullptr, nullptr, S_ALPHA);
       if (consistent) bcs_trip(c, r, 0.0, brk);
       matvec(c, w, r, z, dt, mvflags, brk);
-      pcd(c, w, z, ss, dt);
+      pcd(c, w, z, ss, (T)dt);
       bcs_trip(c, ss, 0.0, brk);
       lin<T>(c, L_ADD1, X, X, uu, nullptr, nullptr, S_ALPHA);
       // odd step
@@ -404,7 +596,7 @@ void bicgstab2(fugu_ctx* c, Work<T>& w, T dt, double tol, int max_iter, int mvfl
       lin<T>(c, L_SUB1, vv, ss, vv, nullptr, nullptr, S_BETA);
       bcs_trip(c, vv, 0.0, brk);
       matvec(c, w, vv, z, dt, mvflags, brk);
-      pcd(c, w, z, ww, dt);
+      pcd(c, w, z, ww, (T)dt);
       bcs_trip(c, ww, 0.0, brk);
       cdot1<T>(c, rt, ww, PH_B2_GAMMA, k, max_iter);
       lin<T>(c, L_SUB1, uu, r, uu, nullptr, nullptr, S_BETA);
@@ -412,7 +604,7 @@ void bicgstab2(fugu_ctx* c, Work<T>& w, T dt, double tol, int max_iter, int mvfl
       lin<T>(c, L_SUB1, ss, ss, ww, nullptr, nullptr, S_ALPHA);
       if (consistent) bcs_trip(c, ss, 0.0, brk);
       matvec(c, w, ss, z, dt, mvflags, brk);
-      pcd(c, w, z, tt, dt);
+      pcd(c, w, z, tt, (T)dt);
       bcs_trip(c, tt, 0.0, brk);
       {
         const T* A[5] = {r, ss, ss, tt, r};
@@ -454,47 +646,120 @@ double true_residual(fugu_ctx* c, const double* X, const double* B, int mvflags,
   return nb > 0 ? sqrt(c->h_sc[S_DOT0] / nb) : sqrt(c->h_sc[S_DOT0]);
 }
 
-// M1: FP64 iterative refinement on the true residual; inner BiCGStab(2)+PCD (consistent
-// BCs) on the normalised residual in precision T.  X (FP64 resident triplet) in place,
-// B = wd.B.  Returns total inner iterations, outer count, final true residual.
+// M1: the chosen fast solve.  Outer flexible GMRES(m) in FP64 on the true residual of the
+// coupled system (X BC'd, coupled_dot norm); its preconditioner is an inner BiCGStab(2)
+// + PCD solve of A z = v in precision T (FP32 for the mixed form, FP64 for the plain FP64
+// form) with a capped iteration count.  Flexibility absorbs the inner solve's inexactness
+// (FP32 rounding, non-converged recurrences), and every outer step minimises the FP64
+// residual, so the stopping test ||B - A X|| / ||B|| <= tol is met in FP64.
 template <typename T>
-void solve_refined(fugu_ctx* c, Work<T>& w, double* X, double dt, double tol_true, double inner_tol,
-                   int max_outer, int max_inner, int mvflags, bool has_lid, double lid,
-                   int* iters, int* outer, double* res) {
+void solve_fgmres(fugu_ctx* c, Work<T>& w, double* X, double dt, double tol, double inner_tol, int max_steps,
+                  int max_inner, int m, int mvflags, bool has_lid, double lid, int* iters, int* outer,
+                  double* res) {
   Work<double>& wd = c->wd;
   const size_t n3 = c->ntrip;
-  int total = 0, o = 0;
-  double rr = 0.0;
+  if (m > FG_M - 1) m = FG_M - 1;
+  if (m < 1) m = 1;
+  while ((int)c->kv.size() < m + 1) c->kv.push_back(dalloc<double>(c, n3));
+  while ((int)c->kz.size() < m) c->kz.push_back(dalloc<double>(c, n3));
+  if (!c->fg) c->fg = dalloc<double>(c, FG_SIZE);
+  double* B = c->Bst;
+  HIPCHK(hipMemcpyAsync(B, wd.B, n3 * sizeof(double), hipMemcpyDeviceToDevice, c->s));
   bcs_trip(c, X);
-  for (o = 0; o <= max_outer; ++o) {
-    // R = B - A X in FP64 (wd.q), ||R||^2 and ||B||^2
+  reset_flags(c);
+  if (max_inner == 0) {   // the PCD itself is the flexible preconditioner
+    set_slot(c, S_P_TOL, 1.0e-5);
+    pcg_setup(c, w);
+    if (c->use_mg) mg_setup(c, w);
+  }
+  cdot1<double>(c, B, B, PH_STORE);
+  read_slots(c);
+  const double bn = sqrt(c->h_sc[S_DOT0]) > 0 ? sqrt(c->h_sc[S_DOT0]) : 1.0;
+  int total_inner = 0, steps = 0;
+  double rr = 1.0;
+  auto maxpy = [&](double* o, const double* x0, double* const* xs, int nq, double sgn, int c_off) {
+    MaxpyArgs ma{};
+    ma.out = o; ma.x0 = x0; ma.nq = nq; ma.c_off = c_off; ma.sgn = sgn; ma.n = n3;
+    for (int q = 0; q < nq; ++q) ma.x[q] = xs[q];
+    LAUNCH(k_maxpy, dim3(nblk(n3)), dim3(256), c->s, ma, (const double*)c->sc);
+  };
+  auto dots_into = [&](const double* W, int k, int off) {   // <W, V_i>, i = 0..k -> S_HBUF + off + i
+    for (int i0 = 0; i0 <= k; i0 += 5) {
+      const int np = (k + 1 - i0) < 5 ? (k + 1 - i0) : 5;
+      const double* A[5];
+      const double* Bq[5];
+      for (int q = 0; q < 5; ++q) { A[q] = W; Bq[q] = c->kv[i0 + (q < np ? q : 0)]; }
+      if (np == 1) cdot<double>(c, 1, A, Bq, PH_HSTORE + off + i0, 0, 0);
+      else if (np == 2) cdot<double>(c, 2, A, Bq, PH_HSTORE + off + i0, 0, 0);
+      else cdot<double>(c, 5, A, Bq, PH_HSTORE + off + i0, 0, 0);   // padded lanes land on unused slots
+    }
+  };
+  while (true) {
+    reset_flags(c);
     HIPCHK(hipMemcpyAsync(wd.X, X, n3 * sizeof(double), hipMemcpyDeviceToDevice, c->s));
     matvec(c, wd, wd.X, wd.z, dt, mvflags, nullptr);
-    lin<double>(c, L_DIFF, wd.q, wd.B, wd.z, nullptr, nullptr, -1, -1, -1, 1.0, false);
-    {
-      const double* A[2] = {wd.q, wd.B};
-      const double* Bq[2] = {wd.q, wd.B};
-      cdot<double>(c, 2, A, Bq, PH_STORE, 0, 0);
-    }
+    lin<double>(c, L_DIFF, c->kv[0], B, wd.z, nullptr, nullptr, -1, -1, -1, 1.0, false);
+    cdot1<double>(c, c->kv[0], c->kv[0], PH_STORE);
     read_slots(c);
-    const double rn2 = c->h_sc[S_DOT0], bn2 = c->h_sc[S_DOT1];
-    rr = bn2 > 0 ? sqrt(rn2 / bn2) : sqrt(rn2);
-    if (rr <= tol_true || o == max_outer || rn2 == 0.0) break;
-    // inner system A D = R / ||R||
-    const double rn = sqrt(rn2);
-    set_slot(c, S_SCALE, rn);
-    LAUNCH((k_convert<T, double>), dim3(nblk(n3)), dim3(256), c->s, w.B, wd.q, n3, c->sc, (int)S_SCALE, 1.0, 1);
-    HIPCHK(hipMemsetAsync(w.X, 0, n3 * sizeof(T), c->s));
-    int it = 0;
-    double ires = 0.0;
-    bicgstab2<T>(c, w, (T)dt, inner_tol, max_inner, mvflags, false, 0.0, true, &it, &ires);
-    total += it;
-    LAUNCH((k_refine_add<T>), dim3(nblk(n3)), dim3(256), c->s, X, (const T*)w.X, n3, c->sc, (int)S_SCALE);
+    const double beta = sqrt(c->h_sc[S_DOT0]);
+    rr = beta / bn;
+    if (c->debug) std::fprintf(stderr, "[fugu] fgmres restart: true res %.3e after %d steps\n", rr, steps);
+    if (rr <= tol || beta == 0.0 || steps >= max_steps) break;
+    LAUNCH(k_fgmres_init, dim3(1), dim3(1), c->s, c->sc, c->fg, c->fl);
+    lin<double>(c, L_SCALE, c->kv[0], c->kv[0], nullptr, nullptr, nullptr, S_INVH, -1, -1, 1.0, false);
+    for (int k = 0; k < m && steps < max_steps; ++k) {
+      // preconditioner z_k ~ A^-1 v_k in precision T
+      int it = 1;
+      if (max_inner > 0) {
+        LAUNCH((k_convert<T, double>), dim3(nblk(n3)), dim3(256), c->s, w.B, (const double*)c->kv[k], n3, c->sc,
+               -1, 1.0, 0);
+        HIPCHK(hipMemsetAsync(w.X, 0, n3 * sizeof(T), c->s));
+        double ires = 0.0;
+        bicgstab2<T>(c, w, dt, inner_tol, max_inner, mvflags, false, 0.0, true, &it, &ires, true);
+        LAUNCH((k_convert<double, T>), dim3(nblk(n3)), dim3(256), c->s, c->kz[k], (const T*)w.X, n3, c->sc, -1,
+               1.0, 0);
+        reset_flags(c);
+      } else if (sizeof(T) == sizeof(double)) {
+        pcd(c, w, (const T*)c->kv[k], (T*)c->kz[k], (T)dt);
+      } else {
+        LAUNCH((k_convert<T, double>), dim3(nblk(n3)), dim3(256), c->s, w.B, (const double*)c->kv[k], n3, c->sc,
+               -1, 1.0, 0);
+        pcd(c, w, (const T*)w.B, w.X, (T)dt);
+        LAUNCH((k_convert<double, T>), dim3(nblk(n3)), dim3(256), c->s, c->kz[k], (const T*)w.X, n3, c->sc, -1,
+               1.0, 0);
+      }
+      total_inner += it;
+      bcs_trip(c, c->kz[k]);
+      // W = A z_k, then classical Gram-Schmidt twice, all coefficients device-resident
+      HIPCHK(hipMemcpyAsync(wd.X, c->kz[k], n3 * sizeof(double), hipMemcpyDeviceToDevice, c->s));
+      matvec(c, wd, wd.X, wd.z, dt, mvflags, nullptr);
+      double* W = wd.z;
+      dots_into(W, k, 0);
+      maxpy(W, W, c->kv.data(), k + 1, -1.0, 0);
+      dots_into(W, k, 64);
+      maxpy(W, W, c->kv.data(), k + 1, -1.0, 64);
+      cdot1<double>(c, W, W, PH_STORE);
+      LAUNCH(k_fgmres_step, dim3(1), dim3(1), c->s, k, 0.5 * tol * bn, c->sc, c->fg, c->fl);
+      lin<double>(c, L_SCALE, c->kv[k + 1], W, nullptr, nullptr, nullptr, S_INVH, -1, -1, 1.0, false);
+      ++steps;
+      // convergence is read one step behind, so the device always has the next step queued
+      HIPCHK(hipMemcpyAsync(c->h_fl + F_FG_DONE, c->fl + F_FG_DONE, sizeof(int), hipMemcpyDeviceToHost, c->s));
+      HIPCHK(hipEventRecord(c->fg_ev[k & 1], c->s));
+      if (k >= 1) {
+        HIPCHK(hipEventSynchronize(c->fg_ev[(k - 1) & 1]));
+        if (c->h_fl[F_FG_DONE]) break;
+      }
+    }
+    LAUNCH(k_fgmres_solve, dim3(1), dim3(1), c->s, c->sc, (const double*)c->fg);
+    read_slots(c);
+    const int kk = (int)c->h_sc[S_HBUF + 127];
+    if (c->debug) std::fprintf(stderr, "[fugu] fgmres cycle: %d directions\n", kk);
+    if (kk > 0) maxpy(X, X, c->kz.data(), kk, 1.0, 128);
     bcs_trip(c, X);
   }
   if (has_lid) bcs_trip(c, X, lid);
-  *iters = total;
-  *outer = o;
+  *iters = total_inner;
+  *outer = steps;
   *res = rr;
 }
 
@@ -508,9 +773,13 @@ void svof(fugu_ctx* c, const double* craw, double* cs_out, double delta, int max
   HIPCHK(hipMemcpyAsync(c->csA, craw, g.np * sizeof(double), hipMemcpyDeviceToDevice, c->s));
   HIPCHK(hipMemsetAsync(c->fl + F_S_DONE, 0, 2 * sizeof(int), c->s));
   double* B[2] = {c->csA, c->csB};
-  for (int it = 1; it <= max_iter; ++it)
-    LAUNCH(k_svof_sweep, grid_cells(g), dim3(TX, TY), c->s, g, it, craw, (const double*)B[(it - 1) & 1],
-           B[it & 1], cx, cy, diag, tol, c->partials, c->ticket, c->sc, c->fl);
+  const dim3 gr = grid_cells_red(g);
+  for (int it = 1; it <= max_iter; ++it) {
+    LAUNCH(k_svof_sweep, gr, dim3(TX, TY), c->s, g, craw, (const double*)B[(it - 1) & 1], B[it & 1], cx, cy,
+           diag, c->partials, (const int*)c->fl);
+    LAUNCH(k_fold_svof, dim3(1), dim3(NT), c->s, (int)nblocks(gr), it, tol, (const double*)c->partials, c->sc,
+           c->fl);
+  }
   // the last iterate is in B[iter & 1]
   read_flags(c);
   const int itf = c->h_fl[F_S_ITER];
@@ -559,7 +828,7 @@ void vof(fugu_ctx* c, const double* cin, double* cout, const double* u, const do
 int step_impl(fugu_ctx* c, double dt, int step_idx, const fugu_step_opts* o, fugu_step_stats* st) {
   const Grid& g = c->g;
   const fugu_params& P = c->prm;
-  fugu_step_opts def = {1, 0, 0.0, 1.0e-6, 500, 0, 0.0, 0.0, 0u};
+  fugu_step_opts def = {1, 0, 0.0, 1.0e-6, 500, 0, 0.0, 0.0, 0u, 0.0, 0, 0, 0};
   if (!o) o = &def;
   const bool bdf2 = (P.time_order == 2 && step_idx >= 2);
   const double dt_eff = bdf2 ? dt / 1.5 : dt;
@@ -618,13 +887,21 @@ int step_impl(fugu_ctx* c, double dt, int step_idx, const fugu_step_opts* o, fug
   int iters = 0, outer = 0;
   double res = 0.0;
   c->pcg_calls = 0;
+  c->use_mg = (o->solver_flags & FUGU_PRECOND_MG) != 0;
+  c->pcd_visc = (o->solver_flags & FUGU_PRECOND_VISC) ? 1.0 : 0.0;
+  c->mg_cycles = o->mg_cycles > 0 ? o->mg_cycles : 1;
+  const double itol = o->inner_tol > 0 ? o->inner_tol : 1.0e-3;
+  const int imax = o->inner_max_iter > 0 ? o->inner_max_iter : (o->inner_max_iter < 0 ? 0 : 25);
+  const int rm = o->restart > 0 ? o->restart : 30;
   if (o->solver_flags & FUGU_SOLVE_REFINED) {
     if (o->solver_flags & FUGU_SOLVE_MIXED) {
       work_alloc<float>(c, c->wf, true);
       bind_props_f(c, pen, amp);
-      solve_refined<float>(c, c->wf, c->X, dt_eff, o->tol, 1.0e-2, 200, 25, mvflags, true, lid, &iters, &outer, &res);
+      solve_fgmres<float>(c, c->wf, c->X, dt_eff, o->tol, itol, o->max_iter, imax, rm, mvflags, true, lid,
+                          &iters, &outer, &res);
     } else {
-      solve_refined<double>(c, c->wd, c->X, dt_eff, o->tol, 1.0e-2, 200, 25, mvflags, true, lid, &iters, &outer, &res);
+      solve_fgmres<double>(c, c->wd, c->X, dt_eff, o->tol, itol, o->max_iter, imax, rm, mvflags, true, lid,
+                           &iters, &outer, &res);
     }
   } else {
     HIPCHK(hipMemcpyAsync(w.X, c->X, c->ntrip * sizeof(double), hipMemcpyDeviceToDevice, c->s));
@@ -643,6 +920,9 @@ int step_impl(fugu_ctx* c, double dt, int step_idx, const fugu_step_opts* o, fug
   bcs(c, c->u, c->v, lid);
   HIPCHK(hipEventRecord(c->ev[3], c->s));
   sync(c);
+  kt_resolve(c);
+  c->use_mg = false;
+  c->pcd_visc = 0.0;
   if (st) {
     float ms_solve = 0.f, ms_step = 0.f;
     HIPCHK(hipEventElapsedTime(&ms_solve, c->ev[1], c->ev[2]));
@@ -707,12 +987,10 @@ int fugu_ctx_create(const fugu_grid* grid, const fugu_params* prm, fugu_ctx** ou
     c->prm = *prm;
     HIPCHK(hipStreamCreateWithFlags(&c->s, hipStreamNonBlocking));
     for (auto& e : c->ev) HIPCHK(hipEventCreate(&e));
+    for (auto& e : c->fg_ev) HIPCHK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
     const Grid& g = c->g;
     c->ntrip = g.nu + g.nv + g.np;
-    const dim3 gu = grid_union(g);
-    const size_t maxb = (size_t)gu.x * gu.y + 64;
-    c->partials = dalloc<double>(c, maxb * 10);
-    c->ticket = (unsigned*)dalloc<double>(c, 1);
+    c->partials = dalloc<double>(c, (size_t)RED_MAXB * 10);
     c->sc = dalloc<double>(c, S_COUNT);
     c->fl = (int*)dalloc<double>(c, F_COUNT);
     HIPCHK(hipHostMalloc((void**)&c->h_sc, S_COUNT * sizeof(double), hipHostMallocDefault));
@@ -732,6 +1010,7 @@ int fugu_ctx_create(const fugu_grid* grid, const fugu_params* prm, fugu_ctx** ou
     c->fx = dalloc<double>(c, g.nu); c->fy = dalloc<double>(c, g.nv);
     c->mdu = dalloc<double>(c, g.nu); c->mdv = dalloc<double>(c, g.nv);
     c->penu = dalloc<double>(c, g.nu); c->penv = dalloc<double>(c, g.nv);
+    c->Bst = dalloc<double>(c, c->ntrip);
     for (int k = 0; k < 4; ++k) {
       c->su[k] = dalloc<double>(c, g.nu); c->sv[k] = dalloc<double>(c, g.nv); c->sp[k] = dalloc<double>(c, g.np);
     }
@@ -752,9 +1031,13 @@ int fugu_ctx_destroy(fugu_ctx* c) {
   if (!c) return 0;
   if (c->s) (void)hipStreamSynchronize(c->s);
   for (void* p : c->allocs) (void)hipFree(p);
+  for (auto& kv : c->kt)
+    for (auto& e : kv.second.ev) (void)hipEventDestroy(e);
   if (c->h_sc) (void)hipHostFree(c->h_sc);
   if (c->h_fl) (void)hipHostFree(c->h_fl);
   for (auto& e : c->ev)
+    if (e) (void)hipEventDestroy(e);
+  for (auto& e : c->fg_ev)
     if (e) (void)hipEventDestroy(e);
   if (c->s) (void)hipStreamDestroy(c->s);
   delete c;
@@ -887,8 +1170,7 @@ int fugu_solve_poisson_pressure(fugu_ctx* c, double* phi, const double* rhs, con
     if (!g.per_x || !g.per_y) LAUNCH(k_mirror_p<double>, grid_union(g), dim3(TX, TY), c->s, g, dphi);
     pcg<double>(c, w, dphi, drhs, false, tol, max_iter);
     if (!g.per_x && !g.per_y) {
-      LAUNCH((k_pcg_sum<double>), grid_cells(g), dim3(TX, TY), c->s, g, (const double*)dphi, c->partials,
-             c->ticket, c->sc, (int)S_P_MEAN, 2, c->fl);
+      cell_mean<double>(c, (const double*)dphi, (int)S_P_MEAN, 2);
       LAUNCH((k_pcg_submean<double>), grid_cells(g), dim3(TX, TY), c->s, g, dphi, (const double*)c->sc,
              (const int*)c->fl);
     }
@@ -898,6 +1180,30 @@ int fugu_solve_poisson_pressure(fugu_ctx* c, double* phi, const double* rhs, con
     read_slots(c);
     if (iter_out) *iter_out = c->h_fl[F_P_ITER];
     if (res_out) *res_out = (c->h_fl[F_P_DONE] == 2) ? 0.0 : c->h_sc[S_P_RES] / c->h_sc[S_P_RHSN];
+    return 0;
+  });
+}
+
+int fugu_solve_pressure_mg(fugu_ctx* c, double* phi, const double* rhs, const double* rho_u,
+                           const double* rho_v, int n_cycles) {
+  NEED(c);
+  return guarded([&]() {
+    const Grid& g = c->g;
+    Work<double>& w = c->wd;
+    in(c, c->rho_u, rho_u, g.nu); in(c, c->rho_v, rho_v, g.nv);
+    bind_props_d(c, false, false);
+    double* drhs = c->sp[1];
+    double* dphi = c->sp[0];
+    in(c, drhs, rhs, g.np);
+    HIPCHK(hipMemsetAsync(dphi, 0, g.np * sizeof(double), c->s));
+    mg_setup(c, w);
+    c->mg_cycles = n_cycles > 0 ? n_cycles : 1;
+    mg_pressure<double>(c, w, dphi, drhs);
+    if (!g.per_x && !g.per_y)
+      LAUNCH((k_pcg_submean<double>), grid_cells(g), dim3(TX, TY), c->s, g, dphi, (const double*)c->sc,
+             (const int*)c->fl);
+    out(c, phi, dphi, g.np);
+    sync(c);
     return 0;
   });
 }
@@ -1103,8 +1409,9 @@ int fugu_compute_adaptive_dt(fugu_ctx* c, const double* u, const double* v, doub
     const Grid& g = c->g;
     const fugu_params& P = c->prm;
     in(c, c->su[0], u, g.nu); in(c, c->sv[0], v, g.nv);
-    LAUNCH(k_absmax, grid_union(g), dim3(TX, TY), c->s, g, (const double*)c->su[0], (const double*)c->sv[0],
-           c->partials, c->ticket, c->sc);
+    const dim3 gr = grid_union_red(g);
+    LAUNCH(k_absmax, gr, dim3(TX, TY), c->s, g, (const double*)c->su[0], (const double*)c->sv[0], c->partials);
+    LAUNCH(k_fold_absmax, dim3(1), dim3(NT), c->s, (int)nblocks(gr), (const double*)c->partials, c->sc);
     read_slots(c);
     const double um = c->h_sc[S_DOT0], vm = c->h_sc[S_DOT1];
     const double pi = 3.141592653589793238462643383279502884197;
@@ -1176,6 +1483,30 @@ int fugu_state_init_properties(fugu_ctx* c) {
   return guarded([&]() {
     mixture(c, c->c);
     sync(c);
+    return 0;
+  });
+}
+
+int fugu_kernel_timing(fugu_ctx* c, int enable) {
+  NEED(c);
+  return guarded([&]() {
+    sync(c);
+    kt_resolve(c);
+    c->kt_on = enable != 0;
+    if (enable)
+      for (auto& kv : c->kt) { kv.second.total_ms = 0.0; kv.second.count = 0; }
+    return 0;
+  });
+}
+
+int fugu_kernel_time(fugu_ctx* c, const char* name, double* total_ms, long long* count) {
+  NEED(c);
+  return guarded([&]() {
+    sync(c);
+    kt_resolve(c);
+    auto it = c->kt.find(name);
+    if (total_ms) *total_ms = it == c->kt.end() ? 0.0 : it->second.total_ms;
+    if (count) *count = it == c->kt.end() ? 0 : it->second.count;
     return 0;
   });
 }

@@ -90,9 +90,8 @@ __device__ inline void cell_ij(int m, int& i, int& j) {
 }
 
 // ---------------------------------------------------------------------------------
-// Deterministic grid reductions: wave shuffle -> LDS -> one partial per workgroup ->
-// the last workgroup to finish (ticket counter) folds the partials in a fixed order.
-// Result is bitwise reproducible for a given launch shape.
+// Deterministic reductions: wave shuffle -> LDS -> one partial per workgroup; a second
+// single-workgroup kernel folds the partials in a fixed order (two-pass scheme below).
 
 __device__ inline double wave_sum(double v) {
 #pragma unroll
@@ -152,52 +151,6 @@ __device__ inline void block_max(double (&v)[NV], double (*lds)[NV]) {
   __syncthreads();
 }
 
-// Publish this workgroup's NV partials; returns true in exactly one workgroup (the
-// last to arrive), in which all threads then hold the folded totals in `tot`.
-// `partials` holds nblocks*NV doubles, `ticket` one counter that is reset here.
-template <int NV, bool MAX = false>
-__device__ inline bool grid_fold(double (&v)[NV], double* partials, unsigned* ticket,
-                                 double (&tot)[NV]) {
-  __shared__ double lds[16][NV];
-  __shared__ bool last;
-  const int tid = threadIdx.x + threadIdx.y * blockDim.x;
-  const int nthr = blockDim.x * blockDim.y;
-  const int nb = gridDim.x * gridDim.y;
-  const int bid = blockIdx.x + blockIdx.y * gridDim.x;
-  if (MAX) block_max<NV>(v, lds); else block_sum<NV>(v, lds);
-  if (tid == 0) {
-#pragma unroll
-    for (int k = 0; k < NV; ++k) partials[(size_t)bid * NV + k] = v[k];
-    __threadfence();
-    const unsigned t = atomicAdd(ticket, 1u);
-    last = (t == (unsigned)(nb - 1));
-  }
-  __syncthreads();
-  if (!last) return false;
-  __threadfence();
-  double acc[NV];
-#pragma unroll
-  for (int k = 0; k < NV; ++k) acc[k] = MAX ? 0.0 : 0.0;
-  for (int b = tid; b < nb; b += nthr) {
-#pragma unroll
-    for (int k = 0; k < NV; ++k) {
-      const double x = __builtin_nontemporal_load(&partials[(size_t)b * NV + k]);
-      acc[k] = MAX ? fmax(acc[k], x) : acc[k] + x;
-    }
-  }
-  if (MAX) block_max<NV>(acc, lds); else block_sum<NV>(acc, lds);
-  __shared__ double out[NV];
-  if (tid == 0) {
-#pragma unroll
-    for (int k = 0; k < NV; ++k) out[k] = acc[k];
-    *ticket = 0u;
-  }
-  __syncthreads();
-#pragma unroll
-  for (int k = 0; k < NV; ++k) tot[k] = out[k];
-  return true;
-}
-
 // ---------------------------------------------------------------------------------
 // Compensated (double-double) reductions.  Every FP64 sum of the solver is accumulated
 // as an unevaluated pair (hi, lo) with TwoSum, so the final fl(hi + lo) is the correctly
@@ -255,30 +208,52 @@ __device__ inline void block_dd(DD (&v)[NV], DD (*lds)[NV]) {
   __syncthreads();
 }
 
-// As grid_fold, with compensated sums of per-lane double-double values; `tot` receives
-// fl(hi + lo) of each total.  `partials` holds nblocks*NV*2 doubles.
+// ---------------------------------------------------------------------------------
+// Two-pass reductions.  Pass 1: a capped grid (<= RED_MAXB workgroups, rows strided)
+// reduces to one partial per workgroup -- no fences, no atomics.  Pass 2: a single
+// workgroup (k_fold_*) folds the partials in a fixed order and runs the scalar logic.
+// The kernel boundary orders the two passes across all XCDs.
+constexpr int RED_MAXB = 1024;
+
+inline dim3 cap_rows(dim3 gr) {
+  const unsigned cap = (unsigned)(RED_MAXB / (gr.x > 0 ? gr.x : 1));
+  if (gr.y > (cap > 0 ? cap : 1)) gr.y = cap > 0 ? cap : 1;
+  return gr;
+}
+inline dim3 grid_union_red(const Grid& g) { return cap_rows(grid_union(g)); }
+inline dim3 grid_cells_red(const Grid& g) { return cap_rows(grid_cells(g)); }
+inline unsigned nblocks(dim3 gr) { return gr.x * gr.y; }
+
 template <int NV>
-__device__ inline bool grid_fold_dd2(DD (&v)[NV], double* partials, unsigned* ticket, double (&tot)[NV]) {
+__device__ inline void publish_dd(DD (&v)[NV], double* partials) {
   __shared__ DD lds[16][NV];
-  __shared__ bool last;
-  const int tid = threadIdx.x + threadIdx.y * blockDim.x;
-  const int nthr = blockDim.x * blockDim.y;
-  const int nb = gridDim.x * gridDim.y;
-  const int bid = blockIdx.x + blockIdx.y * gridDim.x;
   block_dd<NV>(v, lds);
-  if (tid == 0) {
+  if (threadIdx.x == 0 && threadIdx.y == 0) {
+    const int bid = blockIdx.x + blockIdx.y * gridDim.x;
 #pragma unroll
     for (int k = 0; k < NV; ++k) {
       partials[((size_t)bid * NV + k) * 2] = v[k].hi;
       partials[((size_t)bid * NV + k) * 2 + 1] = v[k].lo;
     }
-    __threadfence();
-    const unsigned t = atomicAdd(ticket, 1u);
-    last = (t == (unsigned)(nb - 1));
   }
-  __syncthreads();
-  if (!last) return false;
-  __threadfence();
+}
+template <int NV>
+__device__ inline void publish_max(double (&v)[NV], double* partials) {
+  __shared__ double lds[16][NV];
+  block_max<NV>(v, lds);
+  if (threadIdx.x == 0 && threadIdx.y == 0) {
+    const int bid = blockIdx.x + blockIdx.y * gridDim.x;
+#pragma unroll
+    for (int k = 0; k < NV; ++k) partials[(size_t)bid * NV + k] = v[k];
+  }
+}
+// Single-workgroup fold; every thread returns the totals.
+template <int NV>
+__device__ inline void fold_dd(const double* partials, int nb, double (&tot)[NV]) {
+  __shared__ DD lds[16][NV];
+  __shared__ double out[NV];
+  const int tid = threadIdx.x + threadIdx.y * blockDim.x;
+  const int nthr = blockDim.x * blockDim.y;
   DD acc[NV];
 #pragma unroll
   for (int k = 0; k < NV; ++k) acc[k] = dd_from(0.0);
@@ -286,31 +261,36 @@ __device__ inline bool grid_fold_dd2(DD (&v)[NV], double* partials, unsigned* ti
 #pragma unroll
     for (int k = 0; k < NV; ++k) {
       DD p;
-      p.hi = __builtin_nontemporal_load(&partials[((size_t)b * NV + k) * 2]);
-      p.lo = __builtin_nontemporal_load(&partials[((size_t)b * NV + k) * 2 + 1]);
+      p.hi = partials[((size_t)b * NV + k) * 2];
+      p.lo = partials[((size_t)b * NV + k) * 2 + 1];
       acc[k] = dd_add(acc[k], p);
     }
   }
   block_dd<NV>(acc, lds);
-  __shared__ double out[NV];
-  if (tid == 0) {
-#pragma unroll
+  if (tid == 0)
     for (int k = 0; k < NV; ++k) out[k] = dd_val(acc[k]);
-    *ticket = 0u;
-  }
   __syncthreads();
 #pragma unroll
   for (int k = 0; k < NV; ++k) tot[k] = out[k];
-  return true;
 }
-
 template <int NV>
-__device__ inline bool grid_fold_dd(const double (&x)[NV], double* partials, unsigned* ticket,
-                                    double (&tot)[NV]) {
-  DD v[NV];
+__device__ inline void fold_max(const double* partials, int nb, double (&tot)[NV]) {
+  __shared__ double lds[16][NV];
+  __shared__ double out[NV];
+  const int tid = threadIdx.x + threadIdx.y * blockDim.x;
+  const int nthr = blockDim.x * blockDim.y;
+  double acc[NV];
 #pragma unroll
-  for (int k = 0; k < NV; ++k) v[k] = dd_from(x[k]);
-  return grid_fold_dd2<NV>(v, partials, ticket, tot);
+  for (int k = 0; k < NV; ++k) acc[k] = 0.0;
+  for (int b = tid; b < nb; b += nthr)
+#pragma unroll
+    for (int k = 0; k < NV; ++k) acc[k] = fmax(acc[k], partials[(size_t)b * NV + k]);
+  block_max<NV>(acc, lds);
+  if (tid == 0)
+    for (int k = 0; k < NV; ++k) out[k] = acc[k];
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < NV; ++k) tot[k] = out[k];
 }
 
 }  // namespace fugu

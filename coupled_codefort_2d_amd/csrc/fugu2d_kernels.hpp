@@ -21,7 +21,10 @@ enum Slot : int {
   S_P_RZ, S_P_PAP, S_P_ALPHA, S_P_BETA, S_P_RHSN, S_P_RES, S_P_MEAN, S_P_TOL,
   // generic reduction outputs
   S_DOT0, S_DOT1, S_DOT2, S_DOT3, S_DOT4, S_MAX0,
-  S_COUNT
+  S_GSCALE,      // scale of the breakdown guards (1 = the reference's absolute 1e-30 tests)
+  S_INVH,        // 1 / h_{k+1,k} of the outer FGMRES
+  S_HBUF,        // 256 slots: dot products / combination coefficients of the outer FGMRES
+  S_COUNT = S_HBUF + 256
 };
 enum Flag : int {
   F_BRK = 0,     // exit in the middle of a Krylov iteration: skip everything after
@@ -31,6 +34,8 @@ enum Flag : int {
   F_P_ITER,      // PCG iteration count
   F_S_DONE,      // SVOF converged
   F_S_ITER,
+  F_RELG,        // 1: breakdown guards relative to ||P^-1 b||^2 (chosen fast forms only)
+  F_FG_DONE,     // outer FGMRES cycle converged
   F_COUNT
 };
 
@@ -49,16 +54,23 @@ enum Phase : int {
   PH_B2_XI_ODD,   // :449-452   tot = {(rt,s)}
   PH_B2_GCR,      // :504-530   tot = {(r,s),(s,s),(s,t),(t,t),(r,t)}
   PH_B2_RES,      // :546-547 (+ the loop-top test :385 of the next iteration)
-  PH_STORE        // plain: sc[S_DOT0..] = tot
+  PH_STORE,       // plain: sc[S_DOT0..] = tot
+  PH_HSTORE = 64  // PH_HSTORE + o: sc[S_HBUF + o ..] = tot
 };
 
 __device__ inline void krylov_phase(int phase, int k, int max_iter, double* sc, int* fl,
                                     const double* tot) {
+  // guard thresholds for dot-product-valued scalars: the reference's absolute 1e-30, or
+  // 1e-30 * ||P^-1 b||^2 when relative guards are on
+  const double G = sc[S_GSCALE];
+  const double tiny = 1.0e-30 * G;
   switch (phase) {
     case PH_NORM_B: {
       double nb = sqrt(tot[0]);
-      if (nb < 1.0e-14) nb = 1.0;
+      if (nb < 1.0e-14 && !fl[F_RELG]) nb = 1.0;
+      if (nb == 0.0) nb = 1.0;
       sc[S_NORM_B] = nb;
+      sc[S_GSCALE] = fl[F_RELG] ? nb * nb : 1.0;
     } break;
     case PH_B1_INIT: {
       sc[S_RHO_OLD] = tot[0];
@@ -70,12 +82,12 @@ __device__ inline void krylov_phase(int phase, int k, int max_iter, double* sc, 
     } break;
     case PH_B1_RV: {
       const double rv = tot[0];
-      if (fabs(rv) < 1.0e-30) { fl[F_BRK] = 1; fl[F_DONE] = 1; fl[F_ITER] = k; break; }
+      if (fabs(rv) < tiny) { fl[F_BRK] = 1; fl[F_DONE] = 1; fl[F_ITER] = k; break; }
       sc[S_ALPHA] = sc[S_RHO_OLD] / rv;
     } break;
     case PH_B1_TT_TS: {
       const double tt = tot[0], ts = tot[1];
-      sc[S_OMEGA] = (tt > 1.0e-30) ? ts / tt : 0.0;
+      sc[S_OMEGA] = (tt > tiny) ? ts / tt : 0.0;
     } break;
     case PH_B1_RES: {
       sc[S_NORM_R] = sqrt(tot[0]);
@@ -84,7 +96,7 @@ __device__ inline void krylov_phase(int phase, int k, int max_iter, double* sc, 
         fl[F_BRK] = 1; fl[F_DONE] = 1; fl[F_ITER] = k; break;
       }
       const double rho_new = tot[1];
-      if (fabs(sc[S_RHO_OLD]) < 1.0e-30) { fl[F_BRK] = 1; fl[F_DONE] = 1; fl[F_ITER] = k; break; }
+      if (fabs(sc[S_RHO_OLD]) < tiny) { fl[F_BRK] = 1; fl[F_DONE] = 1; fl[F_ITER] = k; break; }
       sc[S_BETA] = (rho_new / sc[S_RHO_OLD]) * (sc[S_ALPHA] / sc[S_OMEGA]);
       sc[S_RHO_OLD] = rho_new;
       if (k == max_iter) { fl[F_DONE] = 1; fl[F_ITER] = max_iter; }
@@ -101,18 +113,18 @@ __device__ inline void krylov_phase(int phase, int k, int max_iter, double* sc, 
       else if (fabs(sc[S_OM1]) > 1.0e-14) xi0 = -sc[S_OM1] * xi0;
       else xi0 = -xi0;
       const double xi1 = tot[0];
-      if (fabs(xi0) < 1.0e-30) { sc[S_XI0] = xi0; fl[F_BRK] = 1; fl[F_DONE] = 1; fl[F_ITER] = k; break; }
+      if (fabs(xi0) < tiny) { sc[S_XI0] = xi0; fl[F_BRK] = 1; fl[F_DONE] = 1; fl[F_ITER] = k; break; }
       sc[S_BETA] = sc[S_ALPHA] * (xi1 / xi0);
       sc[S_XI0] = xi1;
     } break;
     case PH_B2_GAMMA: {
       const double gam = tot[0];
-      if (fabs(gam) < 1.0e-30) { fl[F_BRK] = 1; fl[F_DONE] = 1; fl[F_ITER] = k; break; }
+      if (fabs(gam) < tiny) { fl[F_BRK] = 1; fl[F_DONE] = 1; fl[F_ITER] = k; break; }
       sc[S_ALPHA] = sc[S_XI0] / gam;
     } break;
     case PH_B2_XI_ODD: {
       const double xi1 = tot[0];
-      if (fabs(sc[S_XI0]) < 1.0e-30) { fl[F_BRK] = 1; fl[F_DONE] = 1; fl[F_ITER] = k; break; }
+      if (fabs(sc[S_XI0]) < tiny) { fl[F_BRK] = 1; fl[F_DONE] = 1; fl[F_ITER] = k; break; }
       sc[S_BETA] = sc[S_ALPHA] * (xi1 / sc[S_XI0]);
       sc[S_XI0] = xi1;
     } break;
@@ -120,18 +132,18 @@ __device__ inline void krylov_phase(int phase, int k, int max_iter, double* sc, 
       double om1 = tot[0];
       const double lam = tot[1], nu = tot[2], tau = tot[3];
       double om2 = tot[4];
-      if (lam * tau > 1.0e-30) {
+      if (lam * tau > tiny * G) {
         const double det = lam * tau - nu * nu;
         if (det > 1.0e-14 * (lam * tau)) {
           const double c1 = om1, c2 = om2;
           om1 = (c1 * tau - c2 * nu) / det;
           om2 = (c2 * lam - c1 * nu) / det;
-        } else if (lam > 1.0e-30) {
+        } else if (lam > tiny) {
           om1 = om1 / lam; om2 = 0.0;
         } else {
           om1 = 0.0; om2 = 0.0;
         }
-      } else if (lam > 1.0e-30) {
+      } else if (lam > tiny) {
         om1 = om1 / lam; om2 = 0.0;
       } else {
         om1 = 0.0; om2 = 0.0;
@@ -146,8 +158,12 @@ __device__ inline void krylov_phase(int phase, int k, int max_iter, double* sc, 
       else if (k == max_iter) { fl[F_DONE] = 1; fl[F_ITER] = max_iter; }
     } break;
     case PH_STORE:
-    default:
       for (int q = 0; q < 5; ++q) sc[S_DOT0 + q] = tot[q];
+      break;
+    default:   // PH_HSTORE + offset: sc[S_HBUF + offset + q] = tot[q]
+      if (phase >= PH_HSTORE)
+        for (int q = 0; q < 5; ++q)
+          if (S_HBUF + (phase - PH_HSTORE) + q < (int)S_COUNT) sc[S_HBUF + (phase - PH_HSTORE) + q] = tot[q];
       break;
   }
 }
@@ -278,117 +294,124 @@ struct MatvecArgs {
   T *Yu, *Yv, *Yp;
   const T *uadv, *vadv, *rho_c, *rho_u, *rho_v, *kap, *zeta, *eta;
   const T *penu, *penv, *mdu, *mdv;
-  T dt;
+  double dt;
   int flags;
 };
 
+// Storage type T (double: the drop-in; float: the mixed form's inner operator); the
+// arithmetic is always FP64 in registers, so FP32 storage does not turn the cancellation
+// inside a row (rho/dt X against the stress and pressure terms) into operator error.
 template <typename T>
 __global__ void __launch_bounds__(NT) k_matvec(Grid g, MatvecArgs<T> a, const int* __restrict__ skip) {
   if (skip && *skip) return;
+  using R = double;
   int i, j;
   union_ij(g, i, j);
   const int nx = g.nx, ny = g.ny, nh = g.nh;
-  const T inv_dx = (T)g.inv_dx, inv_dy = (T)g.inv_dy, half = T(0.5);
+  const R inv_dx = g.inv_dx, inv_dy = g.inv_dy, half = 0.5;
   const int i_u_max = g.per_x ? nx : nx - 1;
   const int j_v_max = g.per_y ? ny : ny - 1;
   const bool with_adv = (a.flags & MV_ADV) != 0;
   const bool cons = (a.flags & MV_CONS) != 0;
   const T* Xu = a.Xu;
   const T* Xv = a.Xv;
+  const R dt = (R)a.dt;
+  auto XU = [&](int ii, int jj) -> R { return (R)Xu[IU(g, ii, jj)]; };
+  auto XV = [&](int ii, int jj) -> R { return (R)Xv[IV(g, ii, jj)]; };
 
-  auto Txx = [&](int ii, int jj) -> T { return a.kap[IP(g, ii, jj)] * (Xu[IU(g, ii, jj)] - Xu[IU(g, ii - 1, jj)]) * inv_dx; };
-  auto Tyy = [&](int ii, int jj) -> T { return a.kap[IP(g, ii, jj)] * (Xv[IV(g, ii, jj)] - Xv[IV(g, ii, jj - 1)]) * inv_dy; };
-  auto dudy = [&](int ii, int jj) -> T { return (Xu[IU(g, ii, jj + 1)] - Xu[IU(g, ii, jj)]) * inv_dy; };
-  auto dvdx = [&](int ii, int jj) -> T { return (Xv[IV(g, ii + 1, jj)] - Xv[IV(g, ii, jj)]) * inv_dx; };
-  auto Txy = [&](int ii, int jj) -> T {
-    const T ze = a.zeta[IW(g, ii, jj)], et = a.eta[IW(g, ii, jj)];
+  auto Txx = [&](int ii, int jj) -> R { return (R)a.kap[IP(g, ii, jj)] * (XU(ii, jj) - XU(ii - 1, jj)) * inv_dx; };
+  auto Tyy = [&](int ii, int jj) -> R { return (R)a.kap[IP(g, ii, jj)] * (XV(ii, jj) - XV(ii, jj - 1)) * inv_dy; };
+  auto dudy = [&](int ii, int jj) -> R { return (XU(ii, jj + 1) - XU(ii, jj)) * inv_dy; };
+  auto dvdx = [&](int ii, int jj) -> R { return (XV(ii + 1, jj) - XV(ii, jj)) * inv_dx; };
+  auto Txy = [&](int ii, int jj) -> R {
+    const R ze = a.zeta[IW(g, ii, jj)], et = a.eta[IW(g, ii, jj)];
     return (ze - et) * dudy(ii, jj) + et * dvdx(ii, jj);
   };
-  auto Tyx = [&](int ii, int jj) -> T {
-    const T ze = a.zeta[IW(g, ii, jj)], et = a.eta[IW(g, ii, jj)];
+  auto Tyx = [&](int ii, int jj) -> R {
+    const R ze = a.zeta[IW(g, ii, jj)], et = a.eta[IW(g, ii, jj)];
     return et * dudy(ii, jj) + (ze - et) * dvdx(ii, jj);
   };
 
   // ---- u rows
   if (i >= -nh && i <= nx + nh && j >= 1 - nh && j <= ny + nh) {
-    T y = T(0);
+    R y = 0.0;
     if (j >= 1 && j <= ny && i >= 1 && i <= i_u_max) {
-      const T visc = -((Txx(i + 1, j) - Txx(i, j)) * inv_dx + (Txy(i, j) - Txy(i, j - 1)) * inv_dy);
-      const T grad = (a.Xp[IP(g, i + 1, j)] - a.Xp[IP(g, i, j)]) * inv_dx;
-      T adv = T(0);
+      const R visc = -((Txx(i + 1, j) - Txx(i, j)) * inv_dx + (Txy(i, j) - Txy(i, j - 1)) * inv_dy);
+      const R grad = ((R)a.Xp[IP(g, i + 1, j)] - (R)a.Xp[IP(g, i, j)]) * inv_dx;
+      R adv = 0.0;
       if (with_adv) {
-        T mE, mW, mN, mS;
+        R mE, mW, mN, mS;
         if (cons) {
-          mE = half * (a.mdu[IU(g, i, j)] + a.mdu[IU(g, i + 1, j)]);
-          mW = half * (a.mdu[IU(g, i - 1, j)] + a.mdu[IU(g, i, j)]);
-          mN = half * (a.mdv[IV(g, i, j)] + a.mdv[IV(g, i + 1, j)]);
-          mS = half * (a.mdv[IV(g, i, j - 1)] + a.mdv[IV(g, i + 1, j - 1)]);
+          mE = half * ((R)a.mdu[IU(g, i, j)] + (R)a.mdu[IU(g, i + 1, j)]);
+          mW = half * ((R)a.mdu[IU(g, i - 1, j)] + (R)a.mdu[IU(g, i, j)]);
+          mN = half * ((R)a.mdv[IV(g, i, j)] + (R)a.mdv[IV(g, i + 1, j)]);
+          mS = half * ((R)a.mdv[IV(g, i, j - 1)] + (R)a.mdv[IV(g, i + 1, j - 1)]);
         } else {
-          mE = a.rho_c[IP(g, i + 1, j)] * half * (a.uadv[IU(g, i, j)] + a.uadv[IU(g, i + 1, j)]);
-          mW = a.rho_c[IP(g, i, j)] * half * (a.uadv[IU(g, i - 1, j)] + a.uadv[IU(g, i, j)]);
-          mN = half * (a.rho_v[IV(g, i, j)] + a.rho_v[IV(g, i + 1, j)]) * half *
-               (a.vadv[IV(g, i, j)] + a.vadv[IV(g, i + 1, j)]);
-          mS = half * (a.rho_v[IV(g, i, j - 1)] + a.rho_v[IV(g, i + 1, j - 1)]) * half *
-               (a.vadv[IV(g, i, j - 1)] + a.vadv[IV(g, i + 1, j - 1)]);
+          mE = (R)a.rho_c[IP(g, i + 1, j)] * half * ((R)a.uadv[IU(g, i, j)] + (R)a.uadv[IU(g, i + 1, j)]);
+          mW = (R)a.rho_c[IP(g, i, j)] * half * ((R)a.uadv[IU(g, i - 1, j)] + (R)a.uadv[IU(g, i, j)]);
+          mN = half * ((R)a.rho_v[IV(g, i, j)] + (R)a.rho_v[IV(g, i + 1, j)]) * half *
+               ((R)a.vadv[IV(g, i, j)] + (R)a.vadv[IV(g, i + 1, j)]);
+          mS = half * ((R)a.rho_v[IV(g, i, j - 1)] + (R)a.rho_v[IV(g, i + 1, j - 1)]) * half *
+               ((R)a.vadv[IV(g, i, j - 1)] + (R)a.vadv[IV(g, i + 1, j - 1)]);
         }
-        const T uc = Xu[IU(g, i, j)];
-        const T uE = half * (uc + Xu[IU(g, i + 1, j)]);
-        const T uW = half * (Xu[IU(g, i - 1, j)] + uc);
-        const T uN = half * (uc + Xu[IU(g, i, j + 1)]);
-        const T uS = half * (Xu[IU(g, i, j - 1)] + uc);
+        const R uc = XU(i, j);
+        const R uE = half * (uc + XU(i + 1, j));
+        const R uW = half * (XU(i - 1, j) + uc);
+        const R uN = half * (uc + XU(i, j + 1));
+        const R uS = half * (XU(i, j - 1) + uc);
         adv = (mE * uE - mW * uW) * inv_dx + (mN * uN - mS * uS) * inv_dy;
       }
-      const T x = Xu[IU(g, i, j)];
-      y = (a.rho_u[IU(g, i, j)] / a.dt) * x + adv + visc + grad;
-      if (a.flags & MV_PEN) y = y + a.penu[IU(g, i, j)] * x;
+      const R x = XU(i, j);
+      y = ((R)a.rho_u[IU(g, i, j)] / dt) * x + adv + visc + grad;
+      if (a.flags & MV_PEN) y = y + (R)a.penu[IU(g, i, j)] * x;
     } else if (!g.per_x && j >= 1 && j <= ny && (i == 0 || i == nx)) {
-      y = Xu[IU(g, i, j)];
+      y = XU(i, j);
     }
-    a.Yu[IU(g, i, j)] = y;
+    a.Yu[IU(g, i, j)] = (T)y;
   }
   // ---- v rows
   if (i >= 1 - nh && i <= nx + nh && j >= -nh && j <= ny + nh) {
-    T y = T(0);
+    R y = 0.0;
     if (j >= 1 && j <= j_v_max && i >= 1 && i <= nx) {
-      const T visc = -((Tyx(i, j) - Tyx(i - 1, j)) * inv_dx + (Tyy(i, j + 1) - Tyy(i, j)) * inv_dy);
-      const T grad = (a.Xp[IP(g, i, j + 1)] - a.Xp[IP(g, i, j)]) * inv_dy;
-      T adv = T(0);
+      const R visc = -((Tyx(i, j) - Tyx(i - 1, j)) * inv_dx + (Tyy(i, j + 1) - Tyy(i, j)) * inv_dy);
+      const R grad = ((R)a.Xp[IP(g, i, j + 1)] - (R)a.Xp[IP(g, i, j)]) * inv_dy;
+      R adv = 0.0;
       if (with_adv) {
-        T mE, mW, mN, mS;
+        R mE, mW, mN, mS;
         if (cons) {
-          mE = half * (a.mdu[IU(g, i, j)] + a.mdu[IU(g, i, j + 1)]);
-          mW = half * (a.mdu[IU(g, i - 1, j)] + a.mdu[IU(g, i - 1, j + 1)]);
-          mN = half * (a.mdv[IV(g, i, j)] + a.mdv[IV(g, i, j + 1)]);
-          mS = half * (a.mdv[IV(g, i, j - 1)] + a.mdv[IV(g, i, j)]);
+          mE = half * ((R)a.mdu[IU(g, i, j)] + (R)a.mdu[IU(g, i, j + 1)]);
+          mW = half * ((R)a.mdu[IU(g, i - 1, j)] + (R)a.mdu[IU(g, i - 1, j + 1)]);
+          mN = half * ((R)a.mdv[IV(g, i, j)] + (R)a.mdv[IV(g, i, j + 1)]);
+          mS = half * ((R)a.mdv[IV(g, i, j - 1)] + (R)a.mdv[IV(g, i, j)]);
         } else {
-          mE = half * (a.rho_u[IU(g, i, j)] + a.rho_u[IU(g, i, j + 1)]) * half *
-               (a.uadv[IU(g, i, j)] + a.uadv[IU(g, i, j + 1)]);
-          mW = half * (a.rho_u[IU(g, i - 1, j)] + a.rho_u[IU(g, i - 1, j + 1)]) * half *
-               (a.uadv[IU(g, i - 1, j)] + a.uadv[IU(g, i - 1, j + 1)]);
-          mN = a.rho_c[IP(g, i, j + 1)] * half * (a.vadv[IV(g, i, j)] + a.vadv[IV(g, i, j + 1)]);
-          mS = a.rho_c[IP(g, i, j)] * half * (a.vadv[IV(g, i, j - 1)] + a.vadv[IV(g, i, j)]);
+          mE = half * ((R)a.rho_u[IU(g, i, j)] + (R)a.rho_u[IU(g, i, j + 1)]) * half *
+               ((R)a.uadv[IU(g, i, j)] + (R)a.uadv[IU(g, i, j + 1)]);
+          mW = half * ((R)a.rho_u[IU(g, i - 1, j)] + (R)a.rho_u[IU(g, i - 1, j + 1)]) * half *
+               ((R)a.uadv[IU(g, i - 1, j)] + (R)a.uadv[IU(g, i - 1, j + 1)]);
+          mN = (R)a.rho_c[IP(g, i, j + 1)] * half * ((R)a.vadv[IV(g, i, j)] + (R)a.vadv[IV(g, i, j + 1)]);
+          mS = (R)a.rho_c[IP(g, i, j)] * half * ((R)a.vadv[IV(g, i, j - 1)] + (R)a.vadv[IV(g, i, j)]);
         }
-        const T vc = Xv[IV(g, i, j)];
-        const T vE = half * (vc + Xv[IV(g, i + 1, j)]);
-        const T vW = half * (Xv[IV(g, i - 1, j)] + vc);
-        const T vN = half * (vc + Xv[IV(g, i, j + 1)]);
-        const T vS = half * (Xv[IV(g, i, j - 1)] + vc);
+        const R vc = XV(i, j);
+        const R vE = half * (vc + XV(i + 1, j));
+        const R vW = half * (XV(i - 1, j) + vc);
+        const R vN = half * (vc + XV(i, j + 1));
+        const R vS = half * (XV(i, j - 1) + vc);
         adv = (mE * vE - mW * vW) * inv_dx + (mN * vN - mS * vS) * inv_dy;
       }
-      const T x = Xv[IV(g, i, j)];
-      y = (a.rho_v[IV(g, i, j)] / a.dt) * x + adv + visc + grad;
-      if (a.flags & MV_PEN) y = y + a.penv[IV(g, i, j)] * x;
+      const R x = XV(i, j);
+      y = ((R)a.rho_v[IV(g, i, j)] / dt) * x + adv + visc + grad;
+      if (a.flags & MV_PEN) y = y + (R)a.penv[IV(g, i, j)] * x;
     } else if (!g.per_y && i >= 1 && i <= nx && (j == 0 || j == ny)) {
-      y = Xv[IV(g, i, j)];
+      y = XV(i, j);
     }
-    a.Yv[IV(g, i, j)] = y;
+    a.Yv[IV(g, i, j)] = (T)y;
   }
   // ---- p rows: divergence
   if (i >= 1 - nh && i <= nx + nh && j >= 1 - nh && j <= ny + nh) {
-    T y = T(0);
+    R y = 0.0;
     if (i >= 1 && i <= nx && j >= 1 && j <= ny)
-      y = (Xu[IU(g, i, j)] - Xu[IU(g, i - 1, j)]) * inv_dx + (Xv[IV(g, i, j)] - Xv[IV(g, i, j - 1)]) * inv_dy;
-    a.Yp[IP(g, i, j)] = y;
+      y = (XU(i, j) - XU(i - 1, j)) * inv_dx + (XV(i, j) - XV(i, j - 1)) * inv_dy;
+    a.Yp[IP(g, i, j)] = (T)y;
   }
 }
 
@@ -406,46 +429,52 @@ struct DotArgs {
 
 template <typename T, int NP>
 __global__ void __launch_bounds__(NT) k_cdot(Grid g, DotArgs<T> d, double* __restrict__ partials,
-                                             unsigned* __restrict__ ticket, double* __restrict__ sc,
-                                             int* __restrict__ fl) {
+                                             const int* __restrict__ fl) {
   if (fl[F_BRK]) return;
-  int i, j;
-  union_ij(g, i, j);
-  const int nx = g.nx, ny = g.ny;
+  const int nx = g.nx, ny = g.ny, nh = g.nh;
   const int i_u_max = g.per_x ? nx : nx - 1;
   const int j_v_max = g.per_y ? ny : ny - 1;
   const double vol = g.vol;
-  // terms (a*b)*vol as in the reference; each workgroup lane holds up to three of them,
-  // folded into one double-double per pair so the total is the correctly rounded sum.
+  // terms (a*b)*vol as in the reference, compensated per pair so the total is the
+  // correctly rounded sum
   DD v[NP];
 #pragma unroll
   for (int q = 0; q < NP; ++q) v[q] = dd_from(0.0);
-  const bool own_u = (j >= 1 && j <= ny && i >= 1 && i <= i_u_max);
-  const bool own_v = (j >= 1 && j <= j_v_max && i >= 1 && i <= nx);
-  const bool own_p = (j >= 1 && j <= ny && i >= 1 && i <= nx);
-  if (own_u) {
-    const size_t o = IU(g, i, j);
+  const int i = blockIdx.x * TX + (int)threadIdx.x - nh;
+  const int tiles = (g.rw + TY - 1) / TY;
+  for (int t = blockIdx.y; t < tiles; t += gridDim.y) {
+    const int j = t * TY + (int)threadIdx.y - nh;
+    if (j >= 1 && j <= ny && i >= 1 && i <= i_u_max) {
+      const size_t o = IU(g, i, j);
 #pragma unroll
-    for (int q = 0; q < NP; ++q) v[q] = dd_add(v[q], dd_from((double)d.a[q][o] * (double)d.b[q][o] * vol));
-  }
-  if (own_v) {
-    const size_t o = g.nu + IV(g, i, j);
-#pragma unroll
-    for (int q = 0; q < NP; ++q) v[q] = dd_add(v[q], dd_from((double)d.a[q][o] * (double)d.b[q][o] * vol));
-  }
-  if (own_p) {
-    const size_t o = g.nu + g.nv + IP(g, i, j);
-#pragma unroll
-    for (int q = 0; q < NP; ++q) v[q] = dd_add(v[q], dd_from((double)d.a[q][o] * (double)d.b[q][o] * vol));
-  }
-  double tot[NP];
-  if (grid_fold_dd2<NP>(v, partials, ticket, tot)) {
-    if (threadIdx.x == 0 && threadIdx.y == 0) {
-      double t5[5] = {0, 0, 0, 0, 0};
-#pragma unroll
-      for (int q = 0; q < NP; ++q) t5[q] = tot[q];
-      krylov_phase(d.phase, d.k, d.max_iter, sc, fl, t5);
+      for (int q = 0; q < NP; ++q) v[q] = dd_add(v[q], dd_from((double)d.a[q][o] * (double)d.b[q][o] * vol));
     }
+    if (j >= 1 && j <= j_v_max && i >= 1 && i <= nx) {
+      const size_t o = g.nu + IV(g, i, j);
+#pragma unroll
+      for (int q = 0; q < NP; ++q) v[q] = dd_add(v[q], dd_from((double)d.a[q][o] * (double)d.b[q][o] * vol));
+    }
+    if (j >= 1 && j <= ny && i >= 1 && i <= nx) {
+      const size_t o = g.nu + g.nv + IP(g, i, j);
+#pragma unroll
+      for (int q = 0; q < NP; ++q) v[q] = dd_add(v[q], dd_from((double)d.a[q][o] * (double)d.b[q][o] * vol));
+    }
+  }
+  publish_dd<NP>(v, partials);
+}
+
+template <int NP>
+__global__ void __launch_bounds__(NT) k_fold_krylov(int nb, int phase, int k, int max_iter,
+                                                    const double* __restrict__ partials, double* __restrict__ sc,
+                                                    int* __restrict__ fl) {
+  if (fl[F_BRK]) return;
+  double tot[NP];
+  fold_dd<NP>(partials, nb, tot);
+  if (threadIdx.x == 0 && threadIdx.y == 0) {
+    double t5[5] = {0, 0, 0, 0, 0};
+#pragma unroll
+    for (int q = 0; q < NP; ++q) t5[q] = tot[q];
+    krylov_phase(phase, k, max_iter, sc, fl, t5);
   }
 }
 
@@ -500,6 +529,89 @@ __global__ void __launch_bounds__(256) k_lin(LinArgs<T> a, const double* __restr
     }
     a.out[k] = o;
   }
+}
+
+// Multi-vector update o = x0 + sgn * sum_q c_q x_q (coefficients sc[S_HBUF + c_off + q]),
+// all operands FP64 triplets; o may alias x0.  One pass reads every operand once.
+constexpr int MAXPY_MAX = 48;
+struct MaxpyArgs {
+  double* out;
+  const double* x0;
+  const double* x[MAXPY_MAX];
+  int nq, c_off;
+  double sgn;
+  size_t n;
+};
+__global__ void __launch_bounds__(256) k_maxpy(MaxpyArgs a, const double* __restrict__ sc) {
+  __shared__ double c[MAXPY_MAX];
+  if ((int)threadIdx.x < a.nq) c[threadIdx.x] = a.sgn * sc[S_HBUF + a.c_off + threadIdx.x];
+  __syncthreads();
+  const size_t stride = (size_t)gridDim.x * blockDim.x;
+  for (size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x; k < a.n; k += stride) {
+    double o = a.x0[k];
+#pragma unroll 4
+    for (int q = 0; q < a.nq; ++q) o += c[q] * a.x[q][k];
+    a.out[k] = o;
+  }
+}
+
+// ---------------------------------------------------------------------------------
+// Outer FGMRES scalar work on the device (one thread): the Hessenberg column from the
+// two classical Gram-Schmidt passes (slots S_HBUF+i and S_HBUF+64+i) and ||w||^2
+// (S_DOT0), Givens rotations, residual estimate and the 1/h scaling of the next basis
+// vector -- no host round trip per step.  fg layout (ld = M+1): H[0 .. (M+1)M), then
+// cs[M], sn[M], g[M+1], kk.
+constexpr int FG_M = 48;
+constexpr int FG_CS = (FG_M + 1) * FG_M, FG_SN = FG_CS + FG_M, FG_G = FG_SN + FG_M, FG_KK = FG_G + FG_M + 1;
+constexpr int FG_SIZE = FG_KK + 8;
+
+__global__ void k_fgmres_init(double* __restrict__ sc, double* __restrict__ fg, int* __restrict__ fl) {
+  const double beta = sqrt(sc[S_DOT0]);
+  for (int i = 0; i <= FG_M; ++i) fg[FG_G + i] = 0.0;
+  fg[FG_G] = beta;
+  fg[FG_KK] = 0.0;
+  sc[S_INVH] = beta > 0.0 ? 1.0 / beta : 0.0;
+  fl[F_FG_DONE] = beta > 0.0 ? 0 : 1;
+}
+
+__global__ void k_fgmres_step(int k, double tol_abs, double* __restrict__ sc, double* __restrict__ fg,
+                              int* __restrict__ fl) {
+  if (fl[F_FG_DONE]) return;
+  double* H = fg + (size_t)k * (FG_M + 1);
+  for (int i = 0; i <= k; ++i) H[i] = sc[S_HBUF + i] + sc[S_HBUF + 64 + i];
+  const double hn = sqrt(sc[S_DOT0]);
+  H[k + 1] = hn;
+  for (int i = 0; i < k; ++i) {
+    const double c = fg[FG_CS + i], s = fg[FG_SN + i];
+    const double t = c * H[i] + s * H[i + 1];
+    H[i + 1] = -s * H[i] + c * H[i + 1];
+    H[i] = t;
+  }
+  const double den = hypot(H[k], H[k + 1]);
+  const double c = den > 0.0 ? H[k] / den : 1.0, s = den > 0.0 ? H[k + 1] / den : 0.0;
+  fg[FG_CS + k] = c;
+  fg[FG_SN + k] = s;
+  H[k] = den;
+  H[k + 1] = 0.0;
+  fg[FG_G + k + 1] = -s * fg[FG_G + k];
+  fg[FG_G + k] = c * fg[FG_G + k];
+  fg[FG_KK] = (double)(k + 1);
+  sc[S_INVH] = hn > 0.0 ? 1.0 / hn : 0.0;
+  if (fabs(fg[FG_G + k + 1]) <= tol_abs || hn == 0.0) fl[F_FG_DONE] = 1;
+}
+
+// y = H^-1 g (back substitution) into slots S_HBUF + 128 + i; returns kk in S_HBUF + 127.
+__global__ void k_fgmres_solve(double* __restrict__ sc, const double* __restrict__ fg) {
+  const int kk = (int)fg[FG_KK];
+  double y[FG_M];
+  for (int i = kk - 1; i >= 0; --i) {
+    double s = fg[FG_G + i];
+    for (int j = i + 1; j < kk; ++j) s -= fg[(size_t)j * (FG_M + 1) + i] * y[j];
+    const double d = fg[(size_t)i * (FG_M + 1) + i];
+    y[i] = d != 0.0 ? s / d : 0.0;
+  }
+  for (int i = 0; i < kk; ++i) sc[S_HBUF + 128 + i] = y[i];
+  sc[S_HBUF + 127] = (double)kk;
 }
 
 // Precision conversion / scaled copy: out[k] = (To)(s * in[k]), s = sc[slot] * lit.
@@ -573,21 +685,32 @@ __device__ inline Nbr cell_nbrs(const Grid& g, int i, int j) {
 }
 
 // Sum of rhs over the interior (for the zero-mean projection, :91-99) -> S_P_MEAN.
+// Interior sum of a cell field (zero-mean projection, :91-99) -> sc[slot] = sum / (nx ny).
 // skip_mode: 0 never, 1 when a Krylov exit is pending (F_BRK), 2 when the PCG did not
 // iterate (zero-rhs early exit or skipped: F_P_DONE >= 2).
+__device__ inline bool pcg_sum_skip(int skip_mode, const int* fl) {
+  return (skip_mode == 1 && fl[F_BRK]) || (skip_mode == 2 && fl[F_P_DONE] >= 2);
+}
 template <typename T>
-__global__ void __launch_bounds__(NT) k_pcg_sum(Grid g, const T* __restrict__ a, double* partials,
-                                                unsigned* ticket, double* sc, int slot, int skip_mode,
-                                                const int* __restrict__ fl) {
-  if (skip_mode == 1 && fl[F_BRK]) return;
-  if (skip_mode == 2 && fl[F_P_DONE] >= 2) return;
-  int i, j;
-  cell_ij(0, i, j);
-  double v[1] = {0.0};
-  if (i <= g.nx && j <= g.ny) v[0] = (double)a[IP(g, i, j)];
+__global__ void __launch_bounds__(NT) k_pcg_sum(Grid g, const T* __restrict__ a, double* __restrict__ partials,
+                                                int skip_mode, const int* __restrict__ fl) {
+  if (pcg_sum_skip(skip_mode, fl)) return;
+  DD v[1] = {dd_from(0.0)};
+  const int i = blockIdx.x * TX + (int)threadIdx.x + 1;
+  const int tiles = (g.ny + TY - 1) / TY;
+  for (int t = blockIdx.y; t < tiles; t += gridDim.y) {
+    const int j = t * TY + (int)threadIdx.y + 1;
+    if (i <= g.nx && j <= g.ny) v[0] = dd_add(v[0], dd_from((double)a[IP(g, i, j)]));
+  }
+  publish_dd<1>(v, partials);
+}
+__global__ void __launch_bounds__(NT) k_fold_sum(Grid g, int nb, int slot, int skip_mode,
+                                                 const double* __restrict__ partials, double* __restrict__ sc,
+                                                 const int* __restrict__ fl) {
+  if (pcg_sum_skip(skip_mode, fl)) return;
   double tot[1];
-  if (grid_fold_dd<1>(v, partials, ticket, tot))
-    if (threadIdx.x == 0 && threadIdx.y == 0) sc[slot] = tot[0] / (double)(g.nx * g.ny);
+  fold_dd<1>(partials, nb, tot);
+  if (threadIdx.x == 0 && threadIdx.y == 0) sc[slot] = tot[0] / (double)(g.nx * g.ny);
 }
 
 // PCG start: rhs_work = rhs - mean (walls) (:90-105), ||rhs_work|| with early exit
@@ -599,44 +722,51 @@ __global__ void __launch_bounds__(NT) k_pcg_init(Grid g, const T* __restrict__ r
                                                  const T* __restrict__ cx, const T* __restrict__ cy,
                                                  const T* __restrict__ dinv, T* __restrict__ r,
                                                  T* __restrict__ z, T* __restrict__ p, int walls_both,
-                                                 int phi_zero, double* partials, unsigned* ticket,
-                                                 double* sc, int* fl) {
+                                                 int phi_zero, double* __restrict__ partials,
+                                                 const double* __restrict__ sc, const int* __restrict__ fl) {
+  if (fl[F_BRK]) return;
+  DD v[2] = {dd_from(0.0), dd_from(0.0)};
+  const T mean = walls_both ? (T)sc[S_P_MEAN] : T(0);
+  const int i = blockIdx.x * TX + (int)threadIdx.x + 1;
+  const int tiles = (g.ny + TY - 1) / TY;
+  for (int t = blockIdx.y; t < tiles; t += gridDim.y) {
+    const int j = t * TY + (int)threadIdx.y + 1;
+    if (i <= g.nx && j <= g.ny) {
+      const Nbr nb = cell_nbrs(g, i, j);
+      const T rw = walls_both ? rhs[nb.c] - mean : rhs[nb.c];
+      T ap = T(0);
+      if (!phi_zero) {
+        const T cE = cx[IU(g, i, j)], cW = cx[IU(g, i - 1, j)];
+        const T cN = cy[IV(g, i, j)], cS = cy[IV(g, i, j - 1)];
+        ap = (cE + cW + cN + cS) * phi[nb.c] -
+             (cE * phi[nb.e] + cW * phi[nb.w] + cN * phi[nb.n] + cS * phi[nb.s]);
+      } else {
+        phi[nb.c] = T(0);
+      }
+      const T rr = rw - ap;
+      const T zz = dinv[nb.c] * rr;
+      r[nb.c] = rr; z[nb.c] = zz; p[nb.c] = zz;
+      v[0] = dd_add(v[0], dd_from((double)rw * (double)rw));
+      v[1] = dd_add(v[1], dd_from((double)rr * (double)zz));
+    }
+  }
+  publish_dd<2>(v, partials);
+}
+__global__ void __launch_bounds__(NT) k_fold_pcg_init(int nb, const double* __restrict__ partials,
+                                                      double* __restrict__ sc, int* __restrict__ fl) {
   if (fl[F_BRK]) {   // pending Krylov exit: mark the solve as skipped
-    if (blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x == 0 && threadIdx.y == 0) fl[F_P_DONE] = 3;
+    if (threadIdx.x == 0 && threadIdx.y == 0) fl[F_P_DONE] = 3;
     return;
   }
-  int i, j;
-  cell_ij(0, i, j);
-  double v[2] = {0.0, 0.0};
-  if (i <= g.nx && j <= g.ny) {
-    const Nbr nb = cell_nbrs(g, i, j);
-    const T mean = walls_both ? (T)sc[S_P_MEAN] : T(0);
-    const T rw = walls_both ? rhs[nb.c] - mean : rhs[nb.c];
-    T ap = T(0);
-    if (!phi_zero) {
-      const T cE = cx[IU(g, i, j)], cW = cx[IU(g, i - 1, j)];
-      const T cN = cy[IV(g, i, j)], cS = cy[IV(g, i, j - 1)];
-      ap = (cE + cW + cN + cS) * phi[nb.c] -
-           (cE * phi[nb.e] + cW * phi[nb.w] + cN * phi[nb.n] + cS * phi[nb.s]);
-    } else {
-      phi[nb.c] = T(0);
-    }
-    const T rr = rw - ap;
-    const T zz = dinv[nb.c] * rr;
-    r[nb.c] = rr; z[nb.c] = zz; p[nb.c] = zz;
-    v[0] = (double)rw * (double)rw;
-    v[1] = (double)rr * (double)zz;
-  }
   double tot[2];
-  if (grid_fold_dd<2>(v, partials, ticket, tot)) {
-    if (threadIdx.x == 0 && threadIdx.y == 0) {
-      const double rn = sqrt(tot[0]);
-      sc[S_P_RHSN] = rn;
-      sc[S_P_RES] = 0.0;
-      fl[F_P_ITER] = 0;
-      if (rn < 1.0e-14) fl[F_P_DONE] = 2;
-      else { fl[F_P_DONE] = 0; sc[S_P_RZ] = tot[1]; }
-    }
+  fold_dd<2>(partials, nb, tot);
+  if (threadIdx.x == 0 && threadIdx.y == 0) {
+    const double rn = sqrt(tot[0]);
+    sc[S_P_RHSN] = rn;
+    sc[S_P_RES] = 0.0;
+    fl[F_P_ITER] = 0;
+    if (rn < 1.0e-14) fl[F_P_DONE] = 2;
+    else { fl[F_P_DONE] = 0; sc[S_P_RZ] = tot[1]; }
   }
 }
 
@@ -646,77 +776,91 @@ template <typename T>
 __global__ void __launch_bounds__(NT) k_pcg_iter1(Grid g, int it, const T* __restrict__ z,
                                                   const T* __restrict__ p_old, T* __restrict__ p_new,
                                                   T* __restrict__ Ap, const T* __restrict__ cx,
-                                                  const T* __restrict__ cy, double* partials,
-                                                  unsigned* ticket, double* sc, int* fl) {
+                                                  const T* __restrict__ cy, double* __restrict__ partials,
+                                                  const double* __restrict__ sc, const int* __restrict__ fl) {
   if (fl[F_P_DONE]) return;
-  int i, j;
-  cell_ij(0, i, j);
-  double v[1] = {0.0};
-  if (i <= g.nx && j <= g.ny) {
-    const Nbr nb = cell_nbrs(g, i, j);
-    T pc, pe, pw, pn, ps;
-    if (it == 1) {
-      pc = p_old[nb.c]; pe = p_old[nb.e]; pw = p_old[nb.w]; pn = p_old[nb.n]; ps = p_old[nb.s];
-    } else {
-      const T beta = (T)sc[S_P_BETA];
-      pc = z[nb.c] + beta * p_old[nb.c];
-      pe = z[nb.e] + beta * p_old[nb.e];
-      pw = z[nb.w] + beta * p_old[nb.w];
-      pn = z[nb.n] + beta * p_old[nb.n];
-      ps = z[nb.s] + beta * p_old[nb.s];
+  DD v[1] = {dd_from(0.0)};
+  const T beta = (T)sc[S_P_BETA];
+  const int i = blockIdx.x * TX + (int)threadIdx.x + 1;
+  const int tiles = (g.ny + TY - 1) / TY;
+  for (int t = blockIdx.y; t < tiles; t += gridDim.y) {
+    const int j = t * TY + (int)threadIdx.y + 1;
+    if (i <= g.nx && j <= g.ny) {
+      const Nbr nb = cell_nbrs(g, i, j);
+      T pc, pe, pw, pn, ps;
+      if (it == 1) {
+        pc = p_old[nb.c]; pe = p_old[nb.e]; pw = p_old[nb.w]; pn = p_old[nb.n]; ps = p_old[nb.s];
+      } else {
+        pc = z[nb.c] + beta * p_old[nb.c];
+        pe = z[nb.e] + beta * p_old[nb.e];
+        pw = z[nb.w] + beta * p_old[nb.w];
+        pn = z[nb.n] + beta * p_old[nb.n];
+        ps = z[nb.s] + beta * p_old[nb.s];
+      }
+      const T cE = cx[IU(g, i, j)], cW = cx[IU(g, i - 1, j)];
+      const T cN = cy[IV(g, i, j)], cS = cy[IV(g, i, j - 1)];
+      const T ap = (cE + cW + cN + cS) * pc - (cE * pe + cW * pw + cN * pn + cS * ps);
+      p_new[nb.c] = pc;
+      Ap[nb.c] = ap;
+      v[0] = dd_add(v[0], dd_from((double)pc * (double)ap));
     }
-    const T cE = cx[IU(g, i, j)], cW = cx[IU(g, i - 1, j)];
-    const T cN = cy[IV(g, i, j)], cS = cy[IV(g, i, j - 1)];
-    const T ap = (cE + cW + cN + cS) * pc - (cE * pe + cW * pw + cN * pn + cS * ps);
-    p_new[nb.c] = pc;
-    Ap[nb.c] = ap;
-    v[0] = (double)pc * (double)ap;
   }
+  publish_dd<1>(v, partials);
+}
+__global__ void __launch_bounds__(NT) k_fold_pcg_iter1(int nb, int it, const double* __restrict__ partials,
+                                                       double* __restrict__ sc, int* __restrict__ fl) {
+  if (fl[F_P_DONE]) return;
   double tot[1];
-  if (grid_fold_dd<1>(v, partials, ticket, tot)) {
-    if (threadIdx.x == 0 && threadIdx.y == 0) {
-      const double pAp = tot[0];
-      sc[S_P_PAP] = pAp;
-      if (fabs(pAp) < 1.0e-30) { fl[F_P_DONE] = 1; fl[F_P_ITER] = it; }
-      else sc[S_P_ALPHA] = sc[S_P_RZ] / pAp;
-    }
+  fold_dd<1>(partials, nb, tot);
+  if (threadIdx.x == 0 && threadIdx.y == 0) {
+    const double pAp = tot[0];
+    sc[S_P_PAP] = pAp;
+    if (fabs(pAp) < 1.0e-30) { fl[F_P_DONE] = 1; fl[F_P_ITER] = it; }
+    else sc[S_P_ALPHA] = sc[S_P_RZ] / pAp;
   }
 }
 
 // Iteration pass 2: phi += alpha p; r -= alpha Ap; z = D^-1 r; ||r||, (r, z) (:194-218).
 template <typename T>
-__global__ void __launch_bounds__(NT) k_pcg_iter2(Grid g, int it, int max_iter, T* __restrict__ phi,
-                                                  const T* __restrict__ p, T* __restrict__ r,
-                                                  const T* __restrict__ Ap, T* __restrict__ z,
-                                                  const T* __restrict__ dinv, double* partials,
-                                                  unsigned* ticket, double* sc, int* fl) {
+__global__ void __launch_bounds__(NT) k_pcg_iter2(Grid g, T* __restrict__ phi, const T* __restrict__ p,
+                                                  T* __restrict__ r, const T* __restrict__ Ap, T* __restrict__ z,
+                                                  const T* __restrict__ dinv, double* __restrict__ partials,
+                                                  const double* __restrict__ sc, const int* __restrict__ fl) {
   if (fl[F_P_DONE]) return;
-  int i, j;
-  cell_ij(0, i, j);
-  double v[2] = {0.0, 0.0};
-  if (i <= g.nx && j <= g.ny) {
-    const size_t c = IP(g, i, j);
-    const T alpha = (T)sc[S_P_ALPHA];
-    phi[c] = phi[c] + alpha * p[c];
-    const T rr = r[c] - alpha * Ap[c];
-    const T zz = dinv[c] * rr;
-    r[c] = rr;
-    z[c] = zz;
-    v[0] = (double)rr * (double)rr;
-    v[1] = (double)rr * (double)zz;
+  DD v[2] = {dd_from(0.0), dd_from(0.0)};
+  const T alpha = (T)sc[S_P_ALPHA];
+  const int i = blockIdx.x * TX + (int)threadIdx.x + 1;
+  const int tiles = (g.ny + TY - 1) / TY;
+  for (int t = blockIdx.y; t < tiles; t += gridDim.y) {
+    const int j = t * TY + (int)threadIdx.y + 1;
+    if (i <= g.nx && j <= g.ny) {
+      const size_t c = IP(g, i, j);
+      phi[c] = phi[c] + alpha * p[c];
+      const T rr = r[c] - alpha * Ap[c];
+      const T zz = dinv[c] * rr;
+      r[c] = rr;
+      z[c] = zz;
+      v[0] = dd_add(v[0], dd_from((double)rr * (double)rr));
+      v[1] = dd_add(v[1], dd_from((double)rr * (double)zz));
+    }
   }
+  publish_dd<2>(v, partials);
+}
+__global__ void __launch_bounds__(NT) k_fold_pcg_iter2(int nb, int it, int max_iter,
+                                                       const double* __restrict__ partials, double* __restrict__ sc,
+                                                       int* __restrict__ fl) {
+  if (fl[F_P_DONE]) return;
   double tot[2];
-  if (grid_fold_dd<2>(v, partials, ticket, tot)) {
-    if (threadIdx.x == 0 && threadIdx.y == 0) {
-      const double res = sqrt(tot[0]);
-      sc[S_P_RES] = res;
-      if (res / sc[S_P_RHSN] < sc[S_P_TOL]) { fl[F_P_DONE] = 1; fl[F_P_ITER] = it; }
-      else {
-        const double rz_new = tot[1];
-        sc[S_P_BETA] = rz_new / sc[S_P_RZ];
-        sc[S_P_RZ] = rz_new;
-        if (it == max_iter) fl[F_P_ITER] = max_iter;
-      }
+  fold_dd<2>(partials, nb, tot);
+  if (threadIdx.x == 0 && threadIdx.y == 0) {
+    const double res = sqrt(tot[0]);
+    sc[S_P_RES] = res;
+    if (res / sc[S_P_RHSN] < sc[S_P_TOL]) { fl[F_P_DONE] = 1; fl[F_P_ITER] = it; }
+    else {
+      const double rz_new = tot[1];
+      sc[S_P_BETA] = rz_new / sc[S_P_RZ];
+      sc[S_P_RZ] = rz_new;
+      if (it == max_iter) fl[F_P_ITER] = max_iter;
     }
   }
 }
@@ -747,11 +891,14 @@ __global__ void k_zero_if(T* __restrict__ a, size_t n, const int* __restrict__ f
 // r_p = -((1/dt) phi + u.grad phi) on cells, zero ghosts (:54-62).  phi is the PCG
 // iterate before mean removal: the mean is subtracted here on the interior; wall
 // ghosts of phi are 0; periodic ghosts wrap.  A zero-rhs early exit means phi = 0.
+// visc > 0 (chosen fast forms only): the Schur approximation also carries the viscous
+// term, r_p -= visc * kappa * z_p (Cahouet-Chabard type), for viscous-dominated regions.
 template <typename T>
 __global__ void __launch_bounds__(NT) k_pcd_rp(Grid g, const T* __restrict__ phi, const T* __restrict__ uadv,
                                                const T* __restrict__ vadv, T dt, int walls_both,
                                                T* __restrict__ rp, const double* __restrict__ sc,
-                                               const int* __restrict__ fl) {
+                                               const int* __restrict__ fl, const T* __restrict__ zp,
+                                               const T* __restrict__ kap, T visc) {
   int i, j;
   union_ij(g, i, j);
   const int nx = g.nx, ny = g.ny, nh = g.nh;
@@ -773,6 +920,7 @@ __global__ void __launch_bounds__(NT) k_pcd_rp(Grid g, const T* __restrict__ phi
     const T conv = u_avg * (ph(i + 1, j) - ph(i - 1, j)) * (half * (T)g.inv_dx) +
                    v_avg * (ph(i, j + 1) - ph(i, j - 1)) * (half * (T)g.inv_dy);
     out = -((T(1) / dt) * ph(i, j) + conv);
+    if (visc > T(0)) out = out - visc * kap[IP(g, i, j)] * zp[IP(g, i, j)];
   }
   rp[IP(g, i, j)] = out;
 }
@@ -1015,28 +1163,35 @@ __global__ void __launch_bounds__(NT) k_vof(Grid g, const double* __restrict__ c
 
 // H10: one SVOF Jacobi sweep (src/mod_vof.f90:168-190) with mirror/wrap neighbour reads
 // of the previous iterate and a max-reduction of the update; predicated on F_S_DONE.
-__global__ void __launch_bounds__(NT) k_svof_sweep(Grid g, int it, const double* __restrict__ craw,
+__global__ void __launch_bounds__(NT) k_svof_sweep(Grid g, const double* __restrict__ craw,
                                                    const double* __restrict__ cold, double* __restrict__ cnew,
-                                                   double cx, double cy, double diag, double tol,
-                                                   double* partials, unsigned* ticket, double* sc, int* fl) {
+                                                   double cx, double cy, double diag, double* __restrict__ partials,
+                                                   const int* __restrict__ fl) {
   if (fl[F_S_DONE]) return;
-  int i, j;
-  cell_ij(0, i, j);
   double v[1] = {0.0};
-  if (i <= g.nx && j <= g.ny) {
-    const Nbr nb = cell_nbrs(g, i, j);
-    // at walls the mirrored ghost equals the cell itself: cell_nbrs already maps to it
-    const double o = (craw[nb.c] + cx * (cold[nb.e] + cold[nb.w]) + cy * (cold[nb.n] + cold[nb.s])) / diag;
-    cnew[nb.c] = o;
-    v[0] = fabs(o - cold[nb.c]);
-  }
-  double tot[1];
-  if (grid_fold<1, true>(v, partials, ticket, tot)) {
-    if (threadIdx.x == 0 && threadIdx.y == 0) {
-      sc[S_MAX0] = tot[0];
-      fl[F_S_ITER] = it;
-      if (tot[0] < tol) fl[F_S_DONE] = 1;
+  const int i = blockIdx.x * TX + (int)threadIdx.x + 1;
+  const int tiles = (g.ny + TY - 1) / TY;
+  for (int t = blockIdx.y; t < tiles; t += gridDim.y) {
+    const int j = t * TY + (int)threadIdx.y + 1;
+    if (i <= g.nx && j <= g.ny) {
+      const Nbr nb = cell_nbrs(g, i, j);
+      // at walls the mirrored ghost equals the cell itself: cell_nbrs already maps to it
+      const double o = (craw[nb.c] + cx * (cold[nb.e] + cold[nb.w]) + cy * (cold[nb.n] + cold[nb.s])) / diag;
+      cnew[nb.c] = o;
+      v[0] = fmax(v[0], fabs(o - cold[nb.c]));
     }
+  }
+  publish_max<1>(v, partials);
+}
+__global__ void __launch_bounds__(NT) k_fold_svof(int nb, int it, double tol, const double* __restrict__ partials,
+                                                  double* __restrict__ sc, int* __restrict__ fl) {
+  if (fl[F_S_DONE]) return;
+  double tot[1];
+  fold_max<1>(partials, nb, tot);
+  if (threadIdx.x == 0 && threadIdx.y == 0) {
+    sc[S_MAX0] = tot[0];
+    fl[F_S_ITER] = it;
+    if (tot[0] < tol) fl[F_S_DONE] = 1;
   }
 }
 
@@ -1159,15 +1314,21 @@ __global__ void __launch_bounds__(NT) k_rhs(Grid g, RhsArgs a) {
 
 // max |u|, |v| over faces for the adaptive dt (src/mod_solver_coupled.f90:207-219).
 __global__ void __launch_bounds__(NT) k_absmax(Grid g, const double* __restrict__ u, const double* __restrict__ v,
-                                               double* partials, unsigned* ticket, double* sc) {
-  int i, j;
-  union_ij(g, i, j);
+                                               double* __restrict__ partials) {
   double m[2] = {0.0, 0.0};
-  if (j >= 1 && j <= g.ny && i >= 0 && i <= g.nx) m[0] = fabs(u[IU(g, i, j)]);
-  if (i >= 1 && i <= g.nx && j >= 0 && j <= g.ny) m[1] = fabs(v[IV(g, i, j)]);
+  const int i = blockIdx.x * TX + (int)threadIdx.x - g.nh;
+  const int tiles = (g.rw + TY - 1) / TY;
+  for (int t = blockIdx.y; t < tiles; t += gridDim.y) {
+    const int j = t * TY + (int)threadIdx.y - g.nh;
+    if (j >= 1 && j <= g.ny && i >= 0 && i <= g.nx) m[0] = fmax(m[0], fabs(u[IU(g, i, j)]));
+    if (i >= 1 && i <= g.nx && j >= 0 && j <= g.ny) m[1] = fmax(m[1], fabs(v[IV(g, i, j)]));
+  }
+  publish_max<2>(m, partials);
+}
+__global__ void __launch_bounds__(NT) k_fold_absmax(int nb, const double* __restrict__ partials, double* __restrict__ sc) {
   double tot[2];
-  if (grid_fold<2, true>(m, partials, ticket, tot))
-    if (threadIdx.x == 0 && threadIdx.y == 0) { sc[S_DOT0] = tot[0]; sc[S_DOT1] = tot[1]; }
+  fold_max<2>(partials, nb, tot);
+  if (threadIdx.x == 0 && threadIdx.y == 0) { sc[S_DOT0] = tot[0]; sc[S_DOT1] = tot[1]; }
 }
 
 }  // namespace fugu

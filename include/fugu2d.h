@@ -52,7 +52,12 @@ typedef struct {
   int max_iter;             /* max_iter_in (reference default 500) */
   int use_amp;              /* consistent_amp */
   double u_solid, v_solid;  /* u_solid_in, v_solid_in */
-  unsigned solver_flags;    /* FUGU_SOLVE_* (0 = the reference solver exactly) */
+  unsigned solver_flags;    /* FUGU_SOLVE_* / FUGU_PRECOND_* (0 = the reference solver exactly) */
+  double inner_tol;         /* refined solve: inner Krylov tolerance (<= 0: 1e-3) */
+  int inner_max_iter;       /* refined solve: inner Krylov iteration cap (0: 25; < 0: no inner Krylov,
+                               the PCD alone preconditions the outer FGMRES) */
+  int mg_cycles;            /* FUGU_PRECOND_MG: V-cycles per preconditioner application (<= 0: 1) */
+  int restart;              /* refined solve: outer FGMRES restart length (<= 0: 30, max 47) */
 } fugu_step_opts;
 
 enum {
@@ -62,7 +67,12 @@ enum {
   /* Step solve by FP64 iterative refinement on the true residual, inner BiCGStab(2)+PCD. */
   FUGU_SOLVE_REFINED = 1u << 1,
   /* With FUGU_SOLVE_REFINED: inner Krylov + preconditioner in FP32 (mixed precision). */
-  FUGU_SOLVE_MIXED = 1u << 2
+  FUGU_SOLVE_MIXED = 1u << 2,
+  /* Pressure block of the PCD preconditioner by geometric multigrid V-cycles (a fixed linear
+   * operator) instead of the reference's Jacobi-PCG to 1e-5 / 150 iterations. */
+  FUGU_PRECOND_MG = 1u << 3,
+  /* Schur approximation of the PCD also carries the viscous term: r_p -= kappa * z_p. */
+  FUGU_PRECOND_VISC = 1u << 4
 };
 
 enum { FUGU_W_MAX_ITER = 1, FUGU_W_NONFINITE = 2 };
@@ -99,6 +109,12 @@ int fugu_coupled_dot(fugu_ctx* ctx, const double* u1, const double* v1, const do
 int fugu_solve_poisson_pressure(fugu_ctx* ctx, double* phi, const double* rhs,
                                 const double* rho_u, const double* rho_v,
                                 double tol, int max_iter, int* iter_out, double* res_out);
+/* No counterpart in the reference: the pressure operator of solve_poisson_pressure (same
+ * coefficients, zero-mean projection with walls) solved by n_cycles multigrid V-cycles from
+ * zero -- the pressure block of the FUGU_PRECOND_MG preconditioner.  phi: interior written,
+ * ghosts zero. */
+int fugu_solve_pressure_mg(fugu_ctx* ctx, double* phi, const double* rhs, const double* rho_u,
+                           const double* rho_v, int n_cycles);
 /* src/mod_subsolvers.f90:252-309 and 314-372 (n_sweeps <= 0 selects the default 5) */
 int fugu_solve_helmholtz_u(fugu_ctx* ctx, double* u_sol, const double* rhs_u, const double* rho_u,
                            const double* kappa, const double* zeta, const double* eta,
@@ -181,6 +197,13 @@ typedef struct {
   double step_seconds;    /* device time of the whole step */
   int pcg_calls;          /* preconditioner applications */
 } fugu_step_stats;
+
+/* Per-kernel device timing: when enabled, launches of the instrumented kernels ("pcg_iter1_f64",
+ * "pcg_iter2_f64", "matvec_f64", their _f32 twins, ...) are bracketed by HIP events on the context
+ * stream (up to 1024 per kernel between synchronisations).  Enabling resets the totals.
+ * No counterpart in the reference (it has no timers); used to measure kernel rooflines. */
+int fugu_kernel_timing(fugu_ctx* ctx, int enable);
+int fugu_kernel_time(fugu_ctx* ctx, const char* kernel, double* total_ms, long long* count);
 
 /* One advance_coupled_step on the resident state.  stats may be NULL. */
 int fugu_step(fugu_ctx* ctx, double dt_step, int step_idx, const fugu_step_opts* opts,
