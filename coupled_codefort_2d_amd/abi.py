@@ -114,7 +114,7 @@ def _declare(L):
     s("fugu_step", [P, C.c_double, C.c_int, C.POINTER(FuguStepOpts), C.POINTER(FuguStepStats)])
     s("fugu_kernel_timing", [P, C.c_int])
     s("fugu_solve_pressure_mg", [P, _D, _D, _D, _D, C.c_int])
-    s("fugu_kernel_time", [P, C.c_char_p, _D, C.POINTER(C.c_longlong)])
+    s("fugu_kernel_time", [P, C.c_char_p, _D, C.POINTER(C.c_longlong), _D])
 
 
 def ptr(a):
@@ -285,10 +285,10 @@ class Context:
         check(load().fugu_kernel_timing(self.h, int(enable)), "kernel_timing")
 
     def kernel_time(self, name):
-        """(total_ms, launches) of an instrumented kernel since timing was enabled."""
-        t, n = C.c_double(0), C.c_longlong(0)
-        check(load().fugu_kernel_time(self.h, name.encode(), C.byref(t), C.byref(n)), "kernel_time")
-        return t.value, n.value
+        """(total_ms, launches, algorithmic bytes) of an instrumented kernel since timing was enabled."""
+        t, n, b = C.c_double(0), C.c_longlong(0), C.c_double(0)
+        check(load().fugu_kernel_time(self.h, name.encode(), C.byref(t), C.byref(n), C.byref(b)), "kernel_time")
+        return t.value, n.value, b.value
 
     def step(self, dt, step_idx, opts: FuguStepOpts):
         st = FuguStepStats()

@@ -41,12 +41,18 @@ struct Error : std::runtime_error {
   } while (0)
 
 // LAUNCH bracketed by timing events when the context's kernel timing is on.
-#define TLAUNCH(ctx, name, kern, grid, block, stream, ...)                       \
+// Timed launch: HIP events around the kernel on its stream, plus the launch's algorithmic
+// bytes (unique fields read + written once) for the roofline.
+#define TLAUNCH(ctx, name, abytes, kern, grid, block, stream, ...)                \
   do {                                                                           \
     KTimer* kt_ = (ctx)->kt_on ? kt_slot(ctx, name) : nullptr;                   \
     if (kt_) HIPCHK(hipEventRecord(kt_->ev[2 * kt_->used], stream));             \
     LAUNCH(kern, grid, block, stream, __VA_ARGS__);                              \
-    if (kt_) { HIPCHK(hipEventRecord(kt_->ev[2 * kt_->used + 1], stream)); kt_->used++; } \
+    if (kt_) {                                                                   \
+      HIPCHK(hipEventRecord(kt_->ev[2 * kt_->used + 1], stream));                \
+      kt_->bytes[kt_->used] = (double)(abytes);                                  \
+      kt_->used++;                                                               \
+    }                                                                            \
   } while (0)
 
 template <typename T>
@@ -76,8 +82,9 @@ struct Work {
 // context stream); resolved after each synchronising call.
 struct KTimer {
   std::vector<hipEvent_t> ev;   // start/stop pairs
+  std::vector<double> bytes;    // algorithmic bytes of each pending launch
   int used = 0;
-  double total_ms = 0.0;
+  double total_ms = 0.0, total_bytes = 0.0;
   long long count = 0;
 };
 constexpr int KT_POOL = 1024;
@@ -127,6 +134,7 @@ KTimer* kt_slot(fugu_ctx* c, const char* name) {
   KTimer& t = c->kt[name];
   if (t.ev.empty()) {
     t.ev.resize(2 * KT_POOL);
+    t.bytes.resize(KT_POOL);
     for (auto& e : t.ev) HIPCHK(hipEventCreate(&e));
   }
   return t.used < KT_POOL ? &t : nullptr;
@@ -138,6 +146,7 @@ void kt_resolve(fugu_ctx* c) {
       float ms = 0.f;
       HIPCHK(hipEventElapsedTime(&ms, t.ev[2 * k], t.ev[2 * k + 1]));
       t.total_ms += ms;
+      t.total_bytes += t.bytes[k];
       t.count++;
     }
     t.used = 0;
@@ -224,7 +233,10 @@ void matvec(fugu_ctx* c, Work<T>& w, T* X, T* Y, double dt, int flags, const int
   a.kap = w.kap; a.zeta = w.zeta; a.eta = w.eta;
   a.penu = w.penu; a.penv = w.penv; a.mdu = w.mdu; a.mdv = w.mdv;
   a.dt = dt; a.flags = flags;
-  TLAUNCH(c, sizeof(T) == 8 ? "matvec_f64" : "matvec_f32", (k_matvec<T>), grid_union(g), dim3(TX, TY), c->s,
+  // 13 fields per cell (X_u, X_v, X_p, rho_u, rho_v, rho_c, kappa, vertex mu, u_adv, v_adv -> Y_u, Y_v,
+  // Y_p); AMP swaps rho_c, u_adv, v_adv for mdot_u, mdot_v (-1); the penalty adds pen_u, pen_v
+  const double mv_fields = 13.0 - ((flags & MV_CONS) ? 1.0 : 0.0) + ((flags & MV_PEN) ? 2.0 : 0.0);
+  TLAUNCH(c, sizeof(T) == 8 ? "matvec_f64" : "matvec_f32", mv_fields * sizeof(T) * g.nx * g.ny, (k_matvec<T>), grid_union(g), dim3(TX, TY), c->s,
           g, a, skip);
 }
 
@@ -373,10 +385,12 @@ void pcg(fugu_ctx* c, Work<T>& w, T* phi, const T* rhs, bool phi_zero, double to
   LAUNCH(k_fold_pcg_init, dim3(1), b1, c->s, nb, pa, c->sc, c->fl);
   T* P[2] = {w.pp0, w.pp1};
   for (int it = 1; it <= max_iter; ++it) {
-    TLAUNCH(c, sizeof(T) == 8 ? "pcg_iter1_f64" : "pcg_iter1_f32", (k_pcg_iter1<T>), gc, bl, c->s, g, it,
+    // z, p_old, c_x, c_y read; p_new, Ap written
+    TLAUNCH(c, sizeof(T) == 8 ? "pcg_iter1_f64" : "pcg_iter1_f32", 6.0 * sizeof(T) * g.nx * g.ny, (k_pcg_iter1<T>), gc, bl, c->s, g, it,
             w.pz, P[(it - 1) & 1], P[it & 1], w.Ap, w.cx, w.cy, c->partials, sc, fl);
     LAUNCH(k_fold_pcg_iter1, dim3(1), b1, c->s, nb, it, pa, c->sc, c->fl);
-    TLAUNCH(c, sizeof(T) == 8 ? "pcg_iter2_f64" : "pcg_iter2_f32", (k_pcg_iter2<T>), gc, bl, c->s, g, phi,
+    // phi, p, r, Ap, D^-1 read; phi, r, z written
+    TLAUNCH(c, sizeof(T) == 8 ? "pcg_iter2_f64" : "pcg_iter2_f32", 8.0 * sizeof(T) * g.nx * g.ny, (k_pcg_iter2<T>), gc, bl, c->s, g, phi,
             P[it & 1], w.pr, w.Ap, w.pz, w.dinv, c->partials, sc, fl);
     LAUNCH(k_fold_pcg_iter2, dim3(1), b1, c->s, nb, it, max_iter, pa, c->sc, c->fl);
   }
@@ -406,7 +420,9 @@ void mg_setup(fugu_ctx* c, Work<T>& w) {
 template <typename T>
 void mg_smooth(fugu_ctx* c, MgLevel<T>& L, int colour, int zero_other) {
   const int half = (L.d.nx + 1) / 2;
-  TLAUNCH(c, sizeof(T) == 8 ? "mg_smooth_f64" : "mg_smooth_f32", (k_mg_rbgs<T>), g2(half, L.d.ny), dim3(64, 4),
+  // one colour: b, D^-1 and phi (written) of half the cells, phi of the other half (neighbours),
+  // all of c_x and c_y
+  TLAUNCH(c, sizeof(T) == 8 ? "mg_smooth_f64" : "mg_smooth_f32", 4.0 * sizeof(T) * L.d.nx * L.d.ny, (k_mg_rbgs<T>), g2(half, L.d.ny), dim3(64, 4),
           c->s, L.d, colour, (const T*)L.cx, (const T*)L.cy, (const T*)L.dinv, (const T*)L.b, L.phi, zero_other);
 }
 
@@ -432,7 +448,8 @@ void mg_vcycle(fugu_ctx* c, Work<T>& w, size_t l, bool zero_guess) {
     mg_smooth(c, L, 1, 0);
   }
   MgLevel<T>& C = w.mg[l + 1];
-  TLAUNCH(c, sizeof(T) == 8 ? "mg_resid_f64" : "mg_resid_f32", (k_mg_restrict<T>), g2(C.d.nx, C.d.ny),
+  // fine c_x, c_y, b, phi read; coarse b, phi written (a quarter of the cells each)
+  TLAUNCH(c, sizeof(T) == 8 ? "mg_resid_f64" : "mg_resid_f32", 4.5 * sizeof(T) * L.d.nx * L.d.ny, (k_mg_restrict<T>), g2(C.d.nx, C.d.ny),
           dim3(64, 4), c->s, L.d, (const T*)L.cx, (const T*)L.cy, (const T*)L.b, (const T*)L.phi, C.b, C.phi);
   mg_vcycle(c, w, l + 1, true);
   LAUNCH((k_mg_prolong<T>), g2(L.d.nx, L.d.ny), dim3(64, 4), c->s, L.d, (const T*)C.phi, L.phi);
@@ -677,22 +694,38 @@ void solve_fgmres(fugu_ctx* c, Work<T>& w, double* X, double dt, double tol, dou
   const double bn = sqrt(c->h_sc[S_DOT0]) > 0 ? sqrt(c->h_sc[S_DOT0]) : 1.0;
   int total_inner = 0, steps = 0;
   double rr = 1.0;
-  auto maxpy = [&](double* o, const double* x0, double* const* xs, int nq, double sgn, int c_off) {
+  // o = x0 + sgn sum_q c_q xs[q].  gs_mode 0: plain; 1: first Gram-Schmidt pass (norm and the
+  // re-orthogonalisation test); 2: second pass, predicated on F_REORTH (norm when it runs).
+  auto maxpy = [&](double* o, const double* x0, double* const* xs, int nq, double sgn, int c_off, int gs_mode) {
     MaxpyArgs ma{};
     ma.out = o; ma.x0 = x0; ma.nq = nq; ma.c_off = c_off; ma.sgn = sgn; ma.n = n3;
     for (int q = 0; q < nq; ++q) ma.x[q] = xs[q];
-    LAUNCH(k_maxpy, dim3(nblk(n3)), dim3(256), c->s, ma, (const double*)c->sc);
+    unsigned nb = nblk(n3);
+    if (nb > (unsigned)RED_MAXB) nb = RED_MAXB;
+    double* pa = gs_mode ? c->partials : (double*)nullptr;
+    if (gs_mode == 2)   // predicated: not timed (its bytes depend on a device-side flag)
+      LAUNCH(k_maxpy, dim3(nb), dim3(256), c->s, ma, (const double*)c->sc, pa, (const int*)c->fl, (int)F_REORTH);
+    else
+      TLAUNCH(c, "maxpy", (nq + 2.0) * sizeof(double) * n3, k_maxpy, dim3(nb), dim3(256), c->s, ma,
+              (const double*)c->sc, pa, (const int*)c->fl, -1);
+    if (gs_mode)
+      LAUNCH(k_fold_gs_norm, dim3(1), dim3(NT), c->s, (int)nb, gs_mode, c->g.vol, (const double*)c->partials,
+             c->sc, c->fl);
   };
-  auto dots_into = [&](const double* W, int k, int off) {   // <W, V_i>, i = 0..k -> S_HBUF + off + i
-    for (int i0 = 0; i0 <= k; i0 += 5) {
-      const int np = (k + 1 - i0) < 5 ? (k + 1 - i0) : 5;
-      const double* A[5];
-      const double* Bq[5];
-      for (int q = 0; q < 5; ++q) { A[q] = W; Bq[q] = c->kv[i0 + (q < np ? q : 0)]; }
-      if (np == 1) cdot<double>(c, 1, A, Bq, PH_HSTORE + off + i0, 0, 0);
-      else if (np == 2) cdot<double>(c, 2, A, Bq, PH_HSTORE + off + i0, 0, 0);
-      else cdot<double>(c, 5, A, Bq, PH_HSTORE + off + i0, 0, 0);   // padded lanes land on unused slots
-    }
+  // <W, V_i>, i = 0..k -> S_HBUF + off + i (and vol <W, W> -> S_DOT1); pass 2 predicated on F_REORTH
+  auto dots_into = [&](const double* W, int k, int off, bool second) {
+    MdotArgs md{};
+    md.w = W; md.nv = k + 1; md.n = n3;
+    for (int q = 0; q <= k; ++q) md.v[q] = c->kv[q];
+    const unsigned nch = (unsigned)((k + 1 + MD_CH - 1) / MD_CH);
+    const int flag = second ? (int)F_REORTH : -1;
+    if (second)   // predicated: not timed
+      LAUNCH(k_mdot, dim3(MD_BX, nch), dim3(256), c->s, md, c->partials, (const int*)c->fl, flag);
+    else
+      TLAUNCH(c, "mdot", (k + 2.0) * sizeof(double) * n3, k_mdot, dim3(MD_BX, nch), dim3(256), c->s, md,
+              c->partials, (const int*)c->fl, flag);
+    LAUNCH(k_mdot_fold, dim3(1), dim3(1024), c->s, k + 1, off, c->g.vol, (const double*)c->partials, c->sc,
+           (const int*)c->fl, flag);
   };
   while (true) {
     reset_flags(c);
@@ -734,11 +767,11 @@ void solve_fgmres(fugu_ctx* c, Work<T>& w, double* X, double dt, double tol, dou
       HIPCHK(hipMemcpyAsync(wd.X, c->kz[k], n3 * sizeof(double), hipMemcpyDeviceToDevice, c->s));
       matvec(c, wd, wd.X, wd.z, dt, mvflags, nullptr);
       double* W = wd.z;
-      dots_into(W, k, 0);
-      maxpy(W, W, c->kv.data(), k + 1, -1.0, 0);
-      dots_into(W, k, 64);
-      maxpy(W, W, c->kv.data(), k + 1, -1.0, 64);
-      cdot1<double>(c, W, W, PH_STORE);
+      // classical Gram-Schmidt, second pass only when the first lost more than half of ||W||^2
+      dots_into(W, k, 0, false);
+      maxpy(W, W, c->kv.data(), k + 1, -1.0, 0, 1);
+      dots_into(W, k, 64, true);
+      maxpy(W, W, c->kv.data(), k + 1, -1.0, 64, 2);
       LAUNCH(k_fgmres_step, dim3(1), dim3(1), c->s, k, 0.5 * tol * bn, c->sc, c->fg, c->fl);
       lin<double>(c, L_SCALE, c->kv[k + 1], W, nullptr, nullptr, nullptr, S_INVH, -1, -1, 1.0, false);
       ++steps;
@@ -754,7 +787,7 @@ void solve_fgmres(fugu_ctx* c, Work<T>& w, double* X, double dt, double tol, dou
     read_slots(c);
     const int kk = (int)c->h_sc[S_HBUF + 127];
     if (c->debug) std::fprintf(stderr, "[fugu] fgmres cycle: %d directions\n", kk);
-    if (kk > 0) maxpy(X, X, c->kz.data(), kk, 1.0, 128);
+    if (kk > 0) maxpy(X, X, c->kz.data(), kk, 1.0, 128, 0);
     bcs_trip(c, X);
   }
   if (has_lid) bcs_trip(c, X, lid);
@@ -990,7 +1023,9 @@ int fugu_ctx_create(const fugu_grid* grid, const fugu_params* prm, fugu_ctx** ou
     for (auto& e : c->fg_ev) HIPCHK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
     const Grid& g = c->g;
     c->ntrip = g.nu + g.nv + g.np;
-    c->partials = dalloc<double>(c, (size_t)RED_MAXB * 10);
+    // two-pass reductions: RED_MAXB workgroups x up to 5 double-double values; multi-dot: 6
+    // chunks x MD_BX workgroups x (MD_CH + 1) values
+    c->partials = dalloc<double>(c, std::max((size_t)RED_MAXB * 10, (size_t)6 * MD_BX * (MD_CH + 1)));
     c->sc = dalloc<double>(c, S_COUNT);
     c->fl = (int*)dalloc<double>(c, F_COUNT);
     HIPCHK(hipHostMalloc((void**)&c->h_sc, S_COUNT * sizeof(double), hipHostMallocDefault));
@@ -1499,7 +1534,7 @@ int fugu_kernel_timing(fugu_ctx* c, int enable) {
   });
 }
 
-int fugu_kernel_time(fugu_ctx* c, const char* name, double* total_ms, long long* count) {
+int fugu_kernel_time(fugu_ctx* c, const char* name, double* total_ms, long long* count, double* total_bytes) {
   NEED(c);
   return guarded([&]() {
     sync(c);
@@ -1507,6 +1542,7 @@ int fugu_kernel_time(fugu_ctx* c, const char* name, double* total_ms, long long*
     auto it = c->kt.find(name);
     if (total_ms) *total_ms = it == c->kt.end() ? 0.0 : it->second.total_ms;
     if (count) *count = it == c->kt.end() ? 0 : it->second.count;
+    if (total_bytes) *total_bytes = it == c->kt.end() ? 0.0 : it->second.total_bytes;
     return 0;
   });
 }

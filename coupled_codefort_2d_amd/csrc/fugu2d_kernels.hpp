@@ -36,6 +36,7 @@ enum Flag : int {
   F_S_ITER,
   F_RELG,        // 1: breakdown guards relative to ||P^-1 b||^2 (chosen fast forms only)
   F_FG_DONE,     // outer FGMRES cycle converged
+  F_REORTH,      // outer FGMRES: second Gram-Schmidt pass needed
   F_COUNT
 };
 
@@ -542,17 +543,115 @@ struct MaxpyArgs {
   double sgn;
   size_t n;
 };
-__global__ void __launch_bounds__(256) k_maxpy(MaxpyArgs a, const double* __restrict__ sc) {
+// With `partials` set, also publishes per-workgroup sums of out^2 (grid capped at RED_MAXB).
+__global__ void __launch_bounds__(256) k_maxpy(MaxpyArgs a, const double* __restrict__ sc,
+                                               double* __restrict__ partials, const int* __restrict__ fl, int flag) {
+  if (flag >= 0 && !fl[flag]) return;   // predicated pass (second Gram-Schmidt pass)
   __shared__ double c[MAXPY_MAX];
   if ((int)threadIdx.x < a.nq) c[threadIdx.x] = a.sgn * sc[S_HBUF + a.c_off + threadIdx.x];
   __syncthreads();
+  double nrm[1] = {0.0};
   const size_t stride = (size_t)gridDim.x * blockDim.x;
   for (size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x; k < a.n; k += stride) {
     double o = a.x0[k];
 #pragma unroll 4
     for (int q = 0; q < a.nq; ++q) o += c[q] * a.x[q][k];
     a.out[k] = o;
+    nrm[0] += o * o;
   }
+  if (partials) {
+    __shared__ double lds[16][1];
+    block_sum<1>(nrm, lds);
+    if (threadIdx.x == 0) partials[blockIdx.x] = nrm[0];
+  }
+}
+
+// Multi-dot of the outer FGMRES Gram-Schmidt: <w, v_q> for q < nv in one launch.
+// blockIdx.y selects a chunk of MD_CH basis vectors, blockIdx.x one of MD_BX slices of the
+// flat arrays; plain FP64 sums in a fixed order.  The triplets the outer iteration builds
+// (residuals B - A X and A z) are exactly zero outside the coupled_dot owned sets (the
+// matvec zeroes ghosts and the wall-normal faces carry the zero BC value), so the flat
+// product times vol is the coupled_dot inner product.
+constexpr int MD_CH = 8;
+constexpr int MD_BX = 256;
+struct MdotArgs {
+  const double* w;
+  const double* v[MAXPY_MAX];
+  int nv;
+  size_t n;
+};
+// Chunk 0 also sums <w, w> (partial slot MD_CH).  flag >= 0: run only when fl[flag] is set.
+__global__ void __launch_bounds__(256) k_mdot(MdotArgs a, double* __restrict__ partials, const int* __restrict__ fl,
+                                              int flag) {
+  if (flag >= 0 && !fl[flag]) return;
+  const int c0 = (int)blockIdx.y * MD_CH;
+  const double* vp[MD_CH];
+#pragma unroll
+  for (int q = 0; q < MD_CH; ++q) vp[q] = (c0 + q < a.nv) ? a.v[c0 + q] : a.w;   // padding lanes reread w
+  double acc[MD_CH + 1];
+#pragma unroll
+  for (int q = 0; q <= MD_CH; ++q) acc[q] = 0.0;
+  const size_t stride = (size_t)MD_BX * blockDim.x;
+  for (size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x; k < a.n; k += stride) {
+    const double wk = a.w[k];
+    double x[MD_CH];
+#pragma unroll
+    for (int q = 0; q < MD_CH; ++q) x[q] = vp[q][k];
+#pragma unroll
+    for (int q = 0; q < MD_CH; ++q) acc[q] += wk * x[q];
+    acc[MD_CH] += wk * wk;
+  }
+  __shared__ double lds[16][MD_CH + 1];
+  block_sum<MD_CH + 1>(acc, lds);
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int q = 0; q <= MD_CH; ++q)
+      partials[((size_t)blockIdx.y * MD_BX + blockIdx.x) * (MD_CH + 1) + q] = acc[q];
+  }
+}
+// Fold of k_mdot: one wavefront per basis vector; sc[S_HBUF + off + q] = vol * sum, and
+// sc[S_DOT1] = vol * <w, w>.  flag >= 0: when fl[flag] is clear the pass was skipped and the
+// coefficients are written as zero.
+__global__ void __launch_bounds__(1024) k_mdot_fold(int nv, int off, double vol, const double* __restrict__ partials,
+                                                    double* __restrict__ sc, const int* __restrict__ fl, int flag) {
+  const bool skip = flag >= 0 && !fl[flag];
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  for (int q = wid; q <= nv; q += (int)(blockDim.x >> 6)) {
+    const int ch = q < nv ? q / MD_CH : 0, qq = q < nv ? q % MD_CH : MD_CH;
+    double s = 0.0;
+    if (!skip)
+      for (int b = lane; b < MD_BX; b += 64) s += partials[((size_t)ch * MD_BX + b) * (MD_CH + 1) + qq];
+    s = wave_sum(s);
+    if (lane == 0) {
+      if (q < nv) sc[S_HBUF + off + q] = s * vol;
+      else if (!skip) sc[S_DOT1] = s * vol;
+    }
+  }
+}
+// Norm fold after a Gram-Schmidt pass: sc[S_DOT0] = vol * sum.  mode 1 (after the first
+// pass): fl[F_REORTH] = (||w'||^2 < ||w||^2 / 2), the Kahan-Parlett "twice is enough" test
+// with its usual 1/sqrt(2) threshold.  mode 2 (after the second pass): only when it ran.
+__global__ void __launch_bounds__(NT) k_fold_gs_norm(int nb, int mode, double vol, const double* __restrict__ partials,
+                                                     double* __restrict__ sc, int* __restrict__ fl) {
+  if (mode == 2 && !fl[F_REORTH]) return;
+  double v[1] = {0.0};
+  for (int b = threadIdx.x; b < nb; b += blockDim.x) v[0] += partials[b];
+  __shared__ double lds[16][1];
+  block_sum<1>(v, lds);
+  if (threadIdx.x == 0) {
+    sc[S_DOT0] = vol * v[0];
+    if (mode == 1) fl[F_REORTH] = (vol * v[0] < 0.5 * sc[S_DOT1]) ? 1 : 0;
+  }
+}
+
+// Fold of per-workgroup plain sums: sc[slot] = scale * sum.
+__global__ void __launch_bounds__(NT) k_fold_plain(int nb, int slot, double scale, const double* __restrict__ partials,
+                                                   double* __restrict__ sc) {
+  double v[1] = {0.0};
+  for (int b = threadIdx.x; b < nb; b += blockDim.x) v[0] += partials[b];
+  __shared__ double lds[16][1];
+  block_sum<1>(v, lds);
+  if (threadIdx.x == 0) sc[slot] = scale * v[0];
 }
 
 // ---------------------------------------------------------------------------------

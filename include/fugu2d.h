@@ -199,11 +199,14 @@ typedef struct {
 } fugu_step_stats;
 
 /* Per-kernel device timing: when enabled, launches of the instrumented kernels ("pcg_iter1_f64",
- * "pcg_iter2_f64", "matvec_f64", their _f32 twins, ...) are bracketed by HIP events on the context
- * stream (up to 1024 per kernel between synchronisations).  Enabling resets the totals.
+ * "pcg_iter2_f64", "matvec_f64", their _f32 twins, "mg_smooth_*", "mg_resid_*", "mdot", "maxpy")
+ * are bracketed by HIP events on the context stream (up to 1024 per kernel between
+ * synchronisations); total_bytes sums each timed launch's algorithmic bytes (unique fields read
+ * and written once).  Enabling resets the totals.
  * No counterpart in the reference (it has no timers); used to measure kernel rooflines. */
 int fugu_kernel_timing(fugu_ctx* ctx, int enable);
-int fugu_kernel_time(fugu_ctx* ctx, const char* kernel, double* total_ms, long long* count);
+int fugu_kernel_time(fugu_ctx* ctx, const char* kernel, double* total_ms, long long* count,
+                     double* total_bytes);
 
 /* One advance_coupled_step on the resident state.  stats may be NULL. */
 int fugu_step(fugu_ctx* ctx, double dt_step, int step_idx, const fugu_step_opts* opts,
