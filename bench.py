@@ -115,9 +115,20 @@ def roofline(ktimes):
     avg_s = ms * 1e-3 / n
     bytes_launch = nbytes / n
     achieved = bytes_launch / avg_s / 1e9
-    return {"bound": "hbm", "kernel": device_name(name), "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-            "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-            "bytes_per_launch": round(bytes_launch), "avg_launch_us": round(avg_s * 1e6, 3), "launches": n}
+    out = {"bound": "hbm", "kernel": device_name(name), "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+           "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+           "bytes_per_launch": round(bytes_launch), "avg_launch_us": round(avg_s * 1e6, 3), "launches": n}
+    # measured HBM bytes / algorithmic bytes of this kernel on this workload (two rocprofv3
+    # --pmc runs, FETCH_SIZE and WRITE_SIZE, scaled by a plain copy; hbm_traffic.py)
+    prof = os.path.join(ROOT, "profiles", f"hbm_traffic_{device_name(name)}_rising_bubble_{NX}x{NY}.txt")
+    if os.path.exists(prof):
+        for ln in open(prof):
+            if ln.startswith("ratio:"):
+                ratio = float(ln.split()[1])
+                out["traffic"] = round(ratio * bytes_launch)
+                out["traffic_ratio"] = ratio
+                out["traffic_source"] = os.path.relpath(prof, ROOT)
+    return out
 
 
 def cpu_baseline(ref_iters):
@@ -169,6 +180,19 @@ def run_reference_form(abi, steps, warmup, max_iter):
     return dict(wall=wall, stats=stats, ktimes={})
 
 
+def copy_probe(mib):
+    """A plain streaming copy of known size (torch elementwise kernel, FP64): the calibration
+    for the HBM byte counters (FETCH_SIZE, WRITE_SIZE) read on the solver's kernels."""
+    import torch
+    n = mib * (1 << 20) // 8
+    x = torch.rand(n, dtype=torch.float64, device="cuda")
+    y = torch.empty_like(x)
+    for _ in range(3):
+        torch.mul(x, 1.0, out=y)
+    torch.cuda.synchronize()
+    print(f"# copy probe: {mib} MiB read + {mib} MiB written per launch, 3 launches", file=sys.stderr, flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -178,17 +202,21 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--forms", default="fp64,mixed,ref")
     ap.add_argument("--ref-steps", type=int, default=1)
+    ap.add_argument("--copy-probe-mib", type=int, default=0,
+                    help="first run a plain FP64 copy of this size (counter calibration)")
     ap.add_argument("--ref-max-iter", type=int, default=400)
     ap.add_argument("--inner-tol", type=float, default=1e-3)
     ap.add_argument("--mg-cycles", type=int, default=1)
     ap.add_argument("--inner-max", type=int, default=-1, help="inner BiCGStab(2) cap; -1: PCD only")
-    ap.add_argument("--restart", type=int, default=40)
+    ap.add_argument("--restart", type=int, default=30)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: --gpus {args.gpus} is not supported by this build "
               "(single-device domain; see DESIGN.md for how the path would shard)", file=sys.stderr)
         sys.exit(2)
 
+    if args.copy_probe_mib:
+        copy_probe(args.copy_probe_mib)
     from coupled_codefort_2d_amd import abi
     abi.load()
     forms = {}

@@ -126,6 +126,9 @@ struct fugu_ctx {
   bool kt_on = false;
   std::map<std::string, KTimer> kt;
   bool debug = std::getenv("FUGU_DEBUG") != nullptr;
+  // FUGU_LAUNCH_LOG=<path>: one line "kernel algorithmic_bytes timed" per launch of the
+  // multi-vector kernels, in launch order, to pair with per-dispatch hardware counters
+  FILE* launch_log = nullptr;
 };
 
 namespace {
@@ -703,6 +706,8 @@ void solve_fgmres(fugu_ctx* c, Work<T>& w, double* X, double dt, double tol, dou
     unsigned nb = nblk(n3);
     if (nb > (unsigned)RED_MAXB) nb = RED_MAXB;
     double* pa = gs_mode ? c->partials : (double*)nullptr;
+    if (c->launch_log)
+      std::fprintf(c->launch_log, "k_maxpy %.0f %d\n", (nq + 2.0) * sizeof(double) * n3, gs_mode == 2 ? 0 : 1);
     if (gs_mode == 2)   // predicated: not timed (its bytes depend on a device-side flag)
       LAUNCH(k_maxpy, dim3(nb), dim3(256), c->s, ma, (const double*)c->sc, pa, (const int*)c->fl, (int)F_REORTH);
     else
@@ -719,11 +724,13 @@ void solve_fgmres(fugu_ctx* c, Work<T>& w, double* X, double dt, double tol, dou
     for (int q = 0; q <= k; ++q) md.v[q] = c->kv[q];
     const unsigned nch = (unsigned)((k + 1 + MD_CH - 1) / MD_CH);
     const int flag = second ? (int)F_REORTH : -1;
-    if (second)   // predicated: not timed
+    if (c->launch_log) std::fprintf(c->launch_log, "k_mdot %.0f %d\n", (k + 2.0) * sizeof(double) * n3, second ? 0 : 1);
+    if (second) {   // predicated: not timed
       LAUNCH(k_mdot, dim3(MD_BX, nch), dim3(256), c->s, md, c->partials, (const int*)c->fl, flag);
-    else
+    } else {
       TLAUNCH(c, "mdot", (k + 2.0) * sizeof(double) * n3, k_mdot, dim3(MD_BX, nch), dim3(256), c->s, md,
               c->partials, (const int*)c->fl, flag);
+    }
     LAUNCH(k_mdot_fold, dim3(1), dim3(1024), c->s, k + 1, off, c->g.vol, (const double*)c->partials, c->sc,
            (const int*)c->fl, flag);
   };
@@ -1018,6 +1025,7 @@ int fugu_ctx_create(const fugu_grid* grid, const fugu_params* prm, fugu_ctx** ou
     c->g = make_grid(grid->nx, grid->ny, grid->nh, grid->lx / grid->nx, grid->ly / grid->ny,
                      grid->periodic_x ? 1 : 0, grid->periodic_y ? 1 : 0);
     c->prm = *prm;
+    if (const char* lp = std::getenv("FUGU_LAUNCH_LOG")) c->launch_log = std::fopen(lp, "a");
     HIPCHK(hipStreamCreateWithFlags(&c->s, hipStreamNonBlocking));
     for (auto& e : c->ev) HIPCHK(hipEventCreate(&e));
     for (auto& e : c->fg_ev) HIPCHK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
@@ -1075,6 +1083,7 @@ int fugu_ctx_destroy(fugu_ctx* c) {
   for (auto& e : c->fg_ev)
     if (e) (void)hipEventDestroy(e);
   if (c->s) (void)hipStreamDestroy(c->s);
+  if (c->launch_log) std::fclose(c->launch_log);
   delete c;
   return 0;
 }
