@@ -366,7 +366,7 @@ void bind_props_f(fugu_ctx* c, bool pen, bool cons) {
 }
 
 // ---------------------------------------------------------------------------------
-// H7: PCG pressure solve.
+// PCG pressure solve.
 template <typename T>
 void pcg_setup(fugu_ctx* c, Work<T>& w) {
   LAUNCH((k_pcg_coef<T>), grid_union(c->g), dim3(TX, TY), c->s, c->g, w.rho_u, w.rho_v, w.cx, w.cy, w.dinv);
@@ -478,7 +478,7 @@ void mg_pressure(fugu_ctx* c, Work<T>& w, T* phi, const T* Zp) {
   if (walls_both) cell_mean<T>(c, (const T*)phi, (int)S_P_MEAN, 0);
 }
 
-// H6: block PCD preconditioner R = P^-1 Z (triplets).
+// block PCD preconditioner R = P^-1 Z (triplets).
 template <typename T>
 void pcd(fugu_ctx* c, Work<T>& w, const T* Z, T* R, T dt) {
   const Grid& g = c->g;
@@ -509,7 +509,7 @@ void pcd(fugu_ctx* c, Work<T>& w, const T* Z, T* R, T dt) {
 }
 
 // ---------------------------------------------------------------------------------
-// H5: BiCGStab (src/mod_bicgstab2.f90:77-264), exact reference recurrences.
+// BiCGStab (src/mod_bicgstab2.f90:77-264), exact reference recurrences.
 template <typename T>
 void bicgstab1(fugu_ctx* c, Work<T>& w, double dt, double tol, int max_iter, int mvflags, bool has_lid,
                double lid, int* iters, double* res) {
@@ -572,7 +572,7 @@ void bicgstab1(fugu_ctx* c, Work<T>& w, double dt, double tol, int max_iter, int
   *res = c->h_sc[S_REL_RES];
 }
 
-// H4: BiCGStab(2) (src/mod_bicgstab2.f90:270-570).  consistent = BCs on r and s before
+// BiCGStab(2) (src/mod_bicgstab2.f90:270-570).  consistent = BCs on r and s before
 // their matvecs (not in the reference; see fugu2d.h FUGU_B2_CONSISTENT_BCS).
 template <typename T>
 void bicgstab2(fugu_ctx* c, Work<T>& w, double dt, double tol, int max_iter, int mvflags, bool has_lid,
@@ -666,7 +666,7 @@ double true_residual(fugu_ctx* c, const double* X, const double* B, int mvflags,
   return nb > 0 ? sqrt(c->h_sc[S_DOT0] / nb) : sqrt(c->h_sc[S_DOT0]);
 }
 
-// M1: the chosen fast solve.  Outer flexible GMRES(m) in FP64 on the true residual of the
+// the chosen fast solve.  Outer flexible GMRES(m) in FP64 on the true residual of the
 // coupled system (X BC'd, coupled_dot norm); its preconditioner is an inner BiCGStab(2)
 // + PCD solve of A z = v in precision T (FP32 for the mixed form, FP64 for the plain FP64
 // form) with a capped iteration count.  Flexibility absorbs the inner solve's inexactness
@@ -804,7 +804,7 @@ void solve_fgmres(fugu_ctx* c, Work<T>& w, double* X, double dt, double tol, dou
 }
 
 // ---------------------------------------------------------------------------------
-// H10 / H11 / H12 / H9 helpers on resident buffers
+// SVOF, curvature, CSF, mixture and VOF helpers on resident buffers
 void svof(fugu_ctx* c, const double* craw, double* cs_out, double delta, int max_iter, double tol, int* it_out) {
   const Grid& g = c->g;
   const double cx = (delta / g.dx) * (delta / g.dx);
@@ -864,7 +864,7 @@ void vof(fugu_ctx* c, const double* cin, double* cout, const double* u, const do
 }
 
 // ---------------------------------------------------------------------------------
-// H13: one advance_coupled_step on the resident state.
+// one advance_coupled_step on the resident state.
 int step_impl(fugu_ctx* c, double dt, int step_idx, const fugu_step_opts* o, fugu_step_stats* st) {
   const Grid& g = c->g;
   const fugu_params& P = c->prm;

@@ -283,7 +283,7 @@ __global__ void k_mirror_p(Grid g, T* __restrict__ a) {
 }
 
 // ---------------------------------------------------------------------------------
-// H1: coupled matvec Y = A X (src/mod_operators.f90:341-443 with :99-162, :60-88,
+// coupled matvec Y = A X (src/mod_operators.f90:341-443 with :99-162, :60-88,
 // :168-249, :34-53).  One pass over the union grid writes Y_u, Y_v, Y_p including
 // zeroed ghosts; the stress tensors are recomputed per use (same arithmetic as the
 // reference's temporaries).  The periodic halo of X is done before (k_halo).
@@ -417,7 +417,7 @@ __global__ void __launch_bounds__(NT) k_matvec(Grid g, MatvecArgs<T> a, const in
 }
 
 // ---------------------------------------------------------------------------------
-// H3: coupled dots (src/mod_bicgstab2.f90:32-72), up to 5 pairs in one pass over the
+// coupled dots (src/mod_bicgstab2.f90:32-72), up to 5 pairs in one pass over the
 // owned sets (u: j=1..ny, i=1..i_u_max; v: j=1..j_v_max, i=1..nx; p: interior), each
 // term (a*b)*vol, FP64 accumulation; the last workgroup runs the Krylov phase.
 template <typename T>
@@ -736,7 +736,7 @@ __global__ void __launch_bounds__(256) k_refine_add(double* __restrict__ X, cons
 }
 
 // ---------------------------------------------------------------------------------
-// H7: Jacobi-PCG pressure solve (src/mod_subsolvers.f90:32-246), two fused passes per
+// Jacobi-PCG pressure solve (src/mod_subsolvers.f90:32-246), two fused passes per
 // iteration with device-resident scalars and predication on F_P_DONE.
 
 // Face coefficients c = inv_h2 / rho_face with zeroed wall faces (:65-78), and D^-1 (:61-87).
@@ -985,7 +985,7 @@ __global__ void k_zero_if(T* __restrict__ a, size_t n, const int* __restrict__ f
 }
 
 // ---------------------------------------------------------------------------------
-// H6: PCD preconditioner pieces (src/mod_pcd_precond.f90).
+// PCD preconditioner pieces (src/mod_pcd_precond.f90).
 
 // r_p = -((1/dt) phi + u.grad phi) on cells, zero ghosts (:54-62).  phi is the PCG
 // iterate before mean removal: the mean is subtracted here on the interior; wall
@@ -1143,7 +1143,7 @@ __global__ void k_pcd_cross(Grid g, const T* __restrict__ rv, const T* __restric
 }
 
 // ---------------------------------------------------------------------------------
-// H12: mixture properties over the full padded arrays (src/mod_viscosity.f90:28-151).
+// mixture properties over the full padded arrays (src/mod_viscosity.f90:28-151).
 __device__ inline double clamp01(double x) {
   const double m = (1.0 < x) ? 1.0 : x;     // Fortran min(one, x)
   return (m > 0.0) ? m : 0.0;               // Fortran max(zero, .)
@@ -1197,7 +1197,7 @@ __global__ void k_mixture(Grid g, const double* __restrict__ c, double rho1, dou
 }
 
 // ---------------------------------------------------------------------------------
-// H9: VOF-TVD advection (src/mod_vof.f90:25-140).  Face fluxes are recomputed per use
+// VOF-TVD advection (src/mod_vof.f90:25-140).  Face fluxes are recomputed per use
 // (each face by its two cells) with the same arithmetic as the reference's Flux arrays.
 __device__ inline double van_leer(double r) { return (r <= 0.0) ? 0.0 : (2.0 * r) / (1.0 + r); }
 
@@ -1260,7 +1260,7 @@ __global__ void __launch_bounds__(NT) k_vof(Grid g, const double* __restrict__ c
     fy[IV(g, i, j)] = (i >= 1 && i <= nx && j >= 0 && j <= ny) ? vof_flux_y(g, cin, v, dt, i, j) : 0.0;
 }
 
-// H10: one SVOF Jacobi sweep (src/mod_vof.f90:168-190) with mirror/wrap neighbour reads
+// one SVOF Jacobi sweep (src/mod_vof.f90:168-190) with mirror/wrap neighbour reads
 // of the previous iterate and a max-reduction of the update; predicated on F_S_DONE.
 __global__ void __launch_bounds__(NT) k_svof_sweep(Grid g, const double* __restrict__ craw,
                                                    const double* __restrict__ cold, double* __restrict__ cnew,
@@ -1294,7 +1294,7 @@ __global__ void __launch_bounds__(NT) k_fold_svof(int nb, int it, double tol, co
   }
 }
 
-// H11: face normals from the smoothed C (src/mod_surface_tension.f90:41-60).
+// face normals from the smoothed C (src/mod_surface_tension.f90:41-60).
 __global__ void k_normals(Grid g, const double* __restrict__ cs, double* __restrict__ nxu, double* __restrict__ nyv) {
   int i, j;
   union_ij(g, i, j);
@@ -1349,7 +1349,7 @@ __global__ void k_csf(Grid g, const double* __restrict__ craw, const double* __r
 }
 
 // ---------------------------------------------------------------------------------
-// H13: step preparation (src/mod_solver_coupled.f90:362-456).
+// step preparation (src/mod_solver_coupled.f90:209-222).
 // u_adv = 2 u^n - u^(n-1) (or u^n at step 1) and u_prev = u^n, over the full arrays.
 __global__ void k_ab2(double* __restrict__ uadv, double* __restrict__ uprev, const double* __restrict__ un,
                       size_t n, int first) {
@@ -1379,7 +1379,7 @@ struct RhsArgs {
   double dt, gx, gy, u_solid, v_solid, u_lid;
   int bdf2, has_lid;
 };
-// B_u, B_v, B_p (:415-456) over the full padded arrays (zero outside owned rows).
+// B_u, B_v, B_p (src/mod_solver_coupled.f90:261-303) over the full padded arrays (zero outside owned rows).
 __global__ void __launch_bounds__(NT) k_rhs(Grid g, RhsArgs a) {
   int i, j;
   union_ij(g, i, j);
@@ -1411,7 +1411,7 @@ __global__ void __launch_bounds__(NT) k_rhs(Grid g, RhsArgs a) {
   if (i >= 1 - nh && i <= nx + nh && j >= 1 - nh && j <= ny + nh) a.Bp[IP(g, i, j)] = 0.0;
 }
 
-// max |u|, |v| over faces for the adaptive dt (src/mod_solver_coupled.f90:207-219).
+// max |u|, |v| over faces for the adaptive dt (src/mod_solver_coupled.f90:54-66).
 __global__ void __launch_bounds__(NT) k_absmax(Grid g, const double* __restrict__ u, const double* __restrict__ v,
                                                double* __restrict__ partials) {
   double m[2] = {0.0, 0.0};

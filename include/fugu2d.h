@@ -43,7 +43,7 @@ typedef struct {
   double cfl_target;
 } fugu_params;
 
-/* Options of advance_coupled_step (src/mod_solver_coupled.f90:290-297). */
+/* Options of advance_coupled_step (src/mod_solver_coupled.f90:179-194). */
 typedef struct {
   int use_bicgstab2;        /* use_bicgstab2 */
   int has_lid;              /* present(u_top_lid) */
@@ -163,10 +163,10 @@ int fugu_compute_mixture_densities(fugu_ctx* ctx, const double* c, double* rho_c
                                    double* rho_v);
 int fugu_compute_mixture_viscosities(fugu_ctx* ctx, const double* c, double* kappa, double* zeta,
                                      double* eta);
-/* src/mod_solver_coupled.f90:194-244 */
+/* src/mod_solver_coupled.f90:41-91 */
 int fugu_compute_adaptive_dt(fugu_ctx* ctx, const double* u, const double* v, double dt_max,
                              double* dt_out);
-/* src/mod_solver_coupled.f90:290-533: every state array in/out (host or device). */
+/* src/mod_solver_coupled.f90:137-380: every state array in/out (host or device). */
 int fugu_advance_coupled_step(fugu_ctx* ctx, double* u, double* v, double* p, double* c,
                               double* u_prev, double* v_prev,
                               double* rho_c, double* rho_u, double* rho_v,

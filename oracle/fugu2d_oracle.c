@@ -1061,7 +1061,7 @@ void og_mixture_viscosities(const og_grid* g, const double* c, double mu1, doubl
   }
 }
 
-/* src/mod_solver_coupled.f90:194-244 */
+/* src/mod_solver_coupled.f90:41-91 */
 double og_adaptive_dt(const og_grid* g, const og_phys* ph, const double* u, const double* v,
                       double dt_max) {
   DECL;
@@ -1087,7 +1087,7 @@ double og_adaptive_dt(const og_grid* g, const og_phys* ph, const double* u, cons
   return dt_loc;
 }
 
-/* src/mod_solver_coupled.f90:290-533 */
+/* src/mod_solver_coupled.f90:137-380 */
 void og_advance_step(const og_grid* g, const og_phys* ph, const og_step_opts* o,
                      double* u, double* v, double* p, double* c,
                      double* u_prev, double* v_prev,

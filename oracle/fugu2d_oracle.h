@@ -146,11 +146,11 @@ void og_mixture_densities(const og_grid* g, const double* c, double rho1, double
 void og_mixture_viscosities(const og_grid* g, const double* c, double mu1, double mu2,
                             double* kappa, double* zeta, double* eta);
 
-/* src/mod_solver_coupled.f90:194-244 */
+/* src/mod_solver_coupled.f90:41-91 */
 double og_adaptive_dt(const og_grid* g, const og_phys* ph, const double* u, const double* v,
                       double dt_max);
 
-/* src/mod_solver_coupled.f90:290-533.  Optional arguments are nullable pointers /
+/* src/mod_solver_coupled.f90:137-380.  Optional arguments are nullable pointers /
  * flags: has_lid = present(u_top_lid); pen_* = present(phi_s_*). */
 typedef struct {
   int use_bicgstab2;

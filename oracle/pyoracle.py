@@ -280,7 +280,7 @@ def csf(g, c_raw, kc, sigma):
 
 
 class State:
-    """The padded state arrays of advance_coupled_step (src/mod_solver_coupled.f90:290-313)."""
+    """The padded state arrays of advance_coupled_step (src/mod_solver_coupled.f90:137-160)."""
 
     NAMES_U = ("u", "u_prev", "rho_u", "f_u")
     NAMES_V = ("v", "v_prev", "rho_v", "f_v")

@@ -1,4 +1,4 @@
-"""GPU parity of advance_coupled_step (src/mod_solver_coupled.f90:290-533) over several steps
+"""GPU parity of advance_coupled_step (src/mod_solver_coupled.f90:137-380) over several steps
 against the oracle: u, v, p within 1e-10 relative after each coupled solve, C within 1e-12,
 same Krylov iteration counts."""
 import numpy as np
