@@ -64,9 +64,11 @@ enum {
   /* BiCGStab(2) with velocity BCs applied to r and s before their matvecs (the reference
    * omits them, src/mod_bicgstab2.f90:430,489, which stalls it on walled domains). */
   FUGU_B2_CONSISTENT_BCS = 1u << 0,
-  /* Step solve by FP64 iterative refinement on the true residual, inner BiCGStab(2)+PCD. */
+  /* Step solve by restarted flexible GMRES in FP64 on the true residual (coupled_dot norm);
+   * the flexible preconditioner is the PCD (inner_max_iter < 0) or an inner BiCGStab(2) + PCD. */
   FUGU_SOLVE_REFINED = 1u << 1,
-  /* With FUGU_SOLVE_REFINED: inner Krylov + preconditioner in FP32 (mixed precision). */
+  /* With FUGU_SOLVE_REFINED: preconditioner (and inner Krylov) stored in FP32, the matvec
+   * arithmetic in FP64 (mixed precision); the outer FGMRES stays FP64. */
   FUGU_SOLVE_MIXED = 1u << 2,
   /* Pressure block of the PCD preconditioner by geometric multigrid V-cycles (a fixed linear
    * operator) instead of the reference's Jacobi-PCG to 1e-5 / 150 iterations. */

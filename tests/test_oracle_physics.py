@@ -1,0 +1,198 @@
+"""Pins of the oracle's VOF / SVOF / surface-tension / time-step / step routines by closed
+forms and invariants (no retyped formulas):
+
+- van Leer limiter values and its symmetry psi(r)/r = psi(1/r) (src/mod_vof.f90:25-33);
+- VOF-TVD: at CFL 1 the scheme is an exact one-cell shift; at CFL < 1 with uniform
+  velocity it conserves sum(C) on a periodic domain, keeps constants and stays within the
+  old extrema (src/mod_vof.f90:38-140);
+- SVOF: the converged filter of a cosine mode is the mode divided by 1 + delta^2 lambda_h,
+  lambda_h the discrete Laplacian eigenvalue, for periodic and mirrored (wall) edges
+  (src/mod_vof.f90:145-237);
+- curvature: exactly zero for a linear C; 1/R (R = 0.25, C = 1 inside) within 5 % on the
+  interface cells of a smoothed circle (tests/test_static_droplet.f90:99-137);
+- CSF: sigma * kappa * dC/dx exactly for constant kappa and linear C on an anisotropic
+  grid, and linear in sigma (src/mod_surface_tension.f90:78-109);
+- adaptive dt: each of the CFL, capillary and gravity bounds selected in turn
+  (src/mod_solver_coupled.f90:41-91);
+- advance_coupled_step: a uniform flow is kept by BDF1 and BDF2 steps (pins the BDF2
+  coefficients with dt/1.5 in the operator), and a resting single fluid under gravity
+  builds the hydrostatic pressure dp/dy = rho g_y (src/mod_solver_coupled.f90:137-380).
+"""
+import math
+
+import numpy as np
+import pytest
+
+import pyoracle as O
+from cases import droplet_c, phys
+
+
+def test_van_leer_values_and_symmetry():
+    vl = O.lib().og_van_leer
+    for r in (-5.0, -1.0, -1e-3, 0.0):
+        assert vl(r) == 0.0
+    assert vl(1.0) == 1.0
+    assert abs(vl(2.0) - 4.0 / 3.0) < 1e-15
+    assert abs(vl(3.0) - 1.5) < 1e-15
+    for r in (0.25, 0.5, 2.0, 7.0):
+        assert abs(vl(r) / r - vl(1.0 / r)) < 1e-15
+    assert vl(1e12) < 2.0
+
+
+def _uniform(g, kind, val):
+    a = g.zeros(kind)
+    a[...] = val
+    return a
+
+
+def test_vof_cfl_one_is_exact_shift():
+    g = O.Grid(32, 16, 1.0, 0.5, per_x=True, per_y=True)
+    rng = np.random.default_rng(1)
+    prof = rng.uniform(0.0, 1.0, g.nx)
+    c = g.zeros("p")
+    c[g.nh:g.nh + g.ny, g.nh:g.nh + g.nx] = prof[None, :]
+    O.halo(g, c, "p")
+    dt = 1.0 / 64.0
+    u = _uniform(g, "u", g.dx / dt)       # u dt / dx = 1 exactly
+    v = g.zeros("v")
+    out = O.vof_advection(g, c, u, v, dt)
+    inner = out[g.nh:g.nh + g.ny, g.nh:g.nh + g.nx]
+    assert np.array_equal(inner, np.broadcast_to(np.roll(prof, 1)[None, :], inner.shape))
+    # the same at u < 0: shift to the other side
+    out = O.vof_advection(g, c, -u, v, dt)
+    inner = out[g.nh:g.nh + g.ny, g.nh:g.nh + g.nx]
+    assert np.array_equal(inner, np.broadcast_to(np.roll(prof, -1)[None, :], inner.shape))
+
+
+def test_vof_conservative_tvd_and_constant():
+    g = O.Grid(24, 24, per_x=True, per_y=True)
+    X, Y = g.coords("p")
+    c = 0.5 + 0.3 * np.sin(2 * np.pi * X) * np.cos(2 * np.pi * Y)
+    O.halo(g, c, "p")
+    dt = 0.4 * g.dx / 1.0
+    u, v = _uniform(g, "u", 0.7), _uniform(g, "v", -0.5)
+    out = O.vof_advection(g, c, u, v, dt)
+    s = (slice(g.nh, g.nh + g.ny), slice(g.nh, g.nh + g.nx))
+    assert abs(out[s].sum() - c[s].sum()) <= 1e-13 * c[s].sum()
+    assert out[s].min() >= c[s].min() - 1e-15 and out[s].max() <= c[s].max() + 1e-15
+    assert not np.array_equal(out[s], c[s])
+    k = _uniform(g, "p", 0.3)
+    assert np.all(O.vof_advection(g, k, u, v, dt)[s] == 0.3)
+
+
+@pytest.mark.parametrize("walls", [False, True])
+def test_svof_cosine_mode_closed_form(walls):
+    nx, ny = 32, 16
+    g = O.Grid(nx, ny, 1.0, 0.5, per_x=not walls, per_y=not walls)
+    X, _ = g.coords("p")
+    m = 3
+    if walls:   # Neumann cosine on the cell-centred grid (mirror ghosts)
+        mode = np.cos(m * np.pi * X / g.lx)
+        lam = (2.0 - 2.0 * math.cos(m * math.pi / nx)) / g.dx ** 2
+    else:
+        mode = np.cos(2 * m * np.pi * X / g.lx)
+        lam = (2.0 - 2.0 * math.cos(2 * m * math.pi / nx)) / g.dx ** 2
+    c = 0.5 + 0.25 * mode
+    delta = 1.5 * min(g.dx, g.dy)
+    cs, it = O.svof(g, c, delta, max_iter=3000, tol=1e-15)
+    s = (slice(g.nh, g.nh + ny), slice(g.nh, g.nh + nx))
+    expect = 0.5 + 0.25 * mode / (1.0 + delta ** 2 * lam)
+    assert np.abs(cs[s] - expect[s]).max() < 1e-12
+    # a planted error in the filter strength (delta^2 -> 2 delta^2) moves it far off
+    wrong = 0.5 + 0.25 * mode / (1.0 + 2 * delta ** 2 * lam)
+    assert np.abs(cs[s] - wrong[s]).max() > 1e-3
+
+
+def test_curvature_zero_for_linear_c():
+    g = O.Grid(32, 32, per_x=True, per_y=True)
+    X, Y = g.coords("p")
+    c = 0.2 + 0.3 * X + 0.2 * Y        # not wrapped: only the interior away from ghosts is checked
+    kc = O.curvature(g, c)
+    s = (slice(g.nh + 2, g.nh + g.ny - 2), slice(g.nh + 2, g.nh + g.nx - 2))
+    assert np.abs(kc[s]).max() < 1e-9
+
+
+def test_curvature_of_circle():
+    g = O.Grid(64, 64)
+    c = droplet_c(g, R=0.25, inside=1.0)
+    cs, _ = O.svof(g, c, 1.5 * g.dx, 150, 1e-7)
+    kc = O.curvature(g, cs)
+    band = (cs > 0.1) & (cs < 0.9)
+    band[:g.nh, :] = band[-g.nh:, :] = False
+    band[:, :g.nh] = band[:, -g.nh:] = False
+    k_mean = kc[band].mean()
+    assert abs(k_mean - 4.0) < 0.05 * 4.0, k_mean
+
+
+def test_csf_closed_form_and_linear_in_sigma():
+    g = O.Grid(16, 32, 1.0, 0.5)          # dx = 1/16, dy = 1/64: transposing dx and dy fails
+    X, Y = g.coords("p")
+    a, b = 0.7, -0.4
+    c = 0.3 + a * X + b * Y
+    K = 3.0
+    kc = _uniform(g, "p", K)
+    sigma = 0.5
+    fu, fv = O.csf(g, c, kc, sigma)
+    su = (slice(g.nh, g.nh + g.ny), slice(g.nh + 1, g.nh + g.nx))       # i = 1..nx-1
+    sv = (slice(g.nh + 1, g.nh + g.ny), slice(g.nh, g.nh + g.nx))       # j = 1..ny-1
+    assert np.abs(fu[su] - sigma * K * a).max() < 1e-12
+    assert np.abs(fv[sv] - sigma * K * b).max() < 1e-12
+    fu2, fv2 = O.csf(g, c, kc, 2 * sigma)
+    assert np.array_equal(fu2, 2 * fu) and np.array_equal(fv2, 2 * fv)
+
+
+def test_adaptive_dt_branches():
+    g = O.Grid(32, 16, 1.0, 0.5)
+    u, v = g.zeros("u"), g.zeros("v")
+    u[g.iu(5, 7)] = -2.0
+    v[g.iv(3, 4)] = 1.0
+    h = min(g.dx, g.dy)
+    # CFL bound: no surface tension, no gravity
+    p = phys(sigma=0.0, gy=0.0, cfl_target=0.5)
+    dt = O.adaptive_dt(g, p, u, v, 1.0)
+    assert abs(dt - 0.5 / (2.0 / g.dx + 1.0 / g.dy)) < 1e-15
+    # dt_max bound
+    assert O.adaptive_dt(g, p, u, v, 1e-6) == 1e-6
+    # capillary bound (Brackbill)
+    p = phys(rho1=1000.0, rho2=1.0, sigma=50.0, gy=0.0, cfl_target=0.5)
+    dt = O.adaptive_dt(g, p, g.zeros("u"), g.zeros("v"), 1.0)
+    assert abs(dt - 0.5 * math.sqrt(1001.0 * h ** 3 / (4 * math.pi * 50.0))) < 1e-15
+    # gravity bound
+    p = phys(sigma=0.0, gx=3.0, gy=-4.0, cfl_target=0.5)
+    dt = O.adaptive_dt(g, p, g.zeros("u"), g.zeros("v"), 1.0)
+    assert abs(dt - 0.5 * math.sqrt(h / 5.0)) < 1e-15
+
+
+@pytest.mark.parametrize("order", [1, 2])
+def test_step_keeps_uniform_flow(order):
+    g = O.Grid(16, 16, per_x=True, per_y=True)
+    ph = phys(rho1=3.0, rho2=3.0, mu1=0.2, mu2=0.2, time_order=order)
+    st = O.State(g)
+    st.c[...] = 1.0
+    st.init_properties(ph)
+    U, V = 0.3, -0.2
+    for a, val in ((st.u, U), (st.u_prev, U), (st.v, V), (st.v_prev, V)):
+        a[...] = val
+    for step in (1, 2, 3):
+        it, res = O.advance_step(st, ph, 1e-2, step, use_bicgstab2=False, tol=1e-12, max_iter=50)
+        s = (slice(g.nh, g.nh + g.ny), slice(g.nh, g.nh + g.nx))
+        assert np.abs(st.u[s] - U).max() < 1e-9, (step, np.abs(st.u[s] - U).max())
+        assert np.abs(st.v[s] - V).max() < 1e-9
+
+
+def test_step_hydrostatic_pressure():
+    g = O.Grid(16, 16)
+    ph = phys(rho1=2.0, rho2=2.0, mu1=0.1, mu2=0.1, gy=-3.0, time_order=1)
+    st = O.State(g)
+    st.c[...] = 1.0
+    st.init_properties(ph)
+    # the reference BiCGStab + PCD (PCG truncated at 1e-5) stops on its preconditioned
+    # recurrence residual; the true solution error it leaves is ~1e-4 relative here
+    O.advance_step(st, ph, 1e-2, 1, use_bicgstab2=False, tol=1e-12, max_iter=100)
+    s = (slice(g.nh, g.nh + g.ny), slice(g.nh, g.nh + g.nx))
+    vscale = 3.0 * 1e-2                     # |g| dt: the velocity an unbalanced column would gain
+    assert np.abs(st.u[s]).max() < 1e-3 * vscale and np.abs(st.v[s]).max() < 1e-3 * vscale
+    p = st.p[s]
+    dpdy = (p[1:, :] - p[:-1, :]) / g.dy
+    assert np.abs(dpdy - 2.0 * -3.0).max() < 1e-2 * 6.0
+    assert np.abs(p[:, 1:] - p[:, :-1]).max() < 1e-2 * 6.0 * g.dx
