@@ -172,6 +172,43 @@ void og_advance_step(const og_grid* g, const og_phys* ph, const og_step_opts* o,
                      const double* phi_s_u, const double* phi_s_v,
                      int* iter_out, double* res_out);
 
+/* ---- adjoint building blocks (fugu2d_adjoint.c; no counterpart in the reference) ---- */
+void og_mask_owned(const og_grid* g, double* u, double* v, double* p);
+void og_halo_p_T(const og_grid* g, double* a);
+void og_halo_u_T(const og_grid* g, double* a);
+void og_halo_v_T(const og_grid* g, double* a);
+void og_velocity_bcs_T(const og_grid* g, double* u, double* v);
+void og_extend(const og_grid* g, double* Xu, double* Xv, double* Xp);
+void og_extend_T(const og_grid* g, double* Xu, double* Xv, double* Xp);
+void og_matvec_eff(const og_grid* g, const double* Xu, const double* Xv, const double* Xp,
+                   double* Yu, double* Yv, double* Yp, const double* u_adv, const double* v_adv,
+                   const double* rho_c, const double* rho_u, const double* rho_v,
+                   const double* kappa, const double* zeta, const double* eta, double dt, int include_adv);
+void og_matvec_T(const og_grid* g, const double* Lu, const double* Lv, const double* Lp,
+                 double* Xu, double* Xv, double* Xp, const double* u_adv, const double* v_adv,
+                 const double* rho_c, const double* rho_u, const double* rho_v,
+                 const double* kappa, const double* zeta, const double* eta, double dt, int include_adv);
+void og_pcd_lin(const og_grid* g, const double* zu, const double* zv, const double* zp,
+                double* ru, double* rv, double* rp, const double* u_adv, const double* v_adv,
+                const double* rho_u, const double* rho_v, const double* kappa, const double* zeta,
+                const double* eta, double dt, double ptol, int pmax, double visc);
+void og_pcd_lin_T(const og_grid* g, const double* yu, const double* yv, const double* yp,
+                  double* zu, double* zv, double* zp, const double* u_adv, const double* v_adv,
+                  const double* rho_u, const double* rho_v, const double* kappa, const double* zeta,
+                  const double* eta, double dt, double ptol, int pmax, double visc);
+void og_operator_coef_grad(const og_grid* g, const double* Lu, const double* Lv,
+                           const double* Xu, const double* Xv,
+                           const double* u_adv, const double* v_adv, const double* rho_c,
+                           const double* rho_u, const double* rho_v, const double* kappa,
+                           const double* zeta, const double* eta, double dt, int include_adv, double s,
+                           double* g_rho_c, double* g_rho_u, double* g_rho_v, double* g_kappa,
+                           double* g_zeta, double* g_eta, double* g_uadv, double* g_vadv);
+void og_cell_properties(const og_grid* g, const double* mu_cell, const double* rho_cell, double* rho_c,
+                        double* rho_u, double* rho_v, double* kappa, double* zeta, double* eta);
+void og_cell_properties_T(const og_grid* g, const double* mu_cell, double* g_rho_c, double* g_rho_u,
+                          double* g_rho_v, double* g_kappa, double* g_zeta, double* g_eta,
+                          double* g_mu, double* g_rho);
+
 #ifdef __cplusplus
 }
 #endif

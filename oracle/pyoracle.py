@@ -100,6 +100,19 @@ def _declare(L):
     s("og_adaptive_dt", [_G, C.POINTER(OgPhys), _D, _D, C.c_double], C.c_double)
     s("og_advance_step", [_G, C.POINTER(OgPhys), C.POINTER(OgStepOpts)] + [_D] * 15 +
       [C.c_double, C.c_int, _D, _D, _I, _D])
+    # adjoint building blocks
+    s("og_mask_owned", [_G, _D, _D, _D])
+    for k in ("p", "u", "v"):
+        s(f"og_halo_{k}_T", [_G, _D])
+    s("og_velocity_bcs_T", [_G, _D, _D])
+    s("og_extend", [_G, _D, _D, _D]); s("og_extend_T", [_G, _D, _D, _D])
+    s("og_matvec_eff", [_G] + [_D] * 14 + [C.c_double, C.c_int])
+    s("og_matvec_T", [_G] + [_D] * 14 + [C.c_double, C.c_int])
+    s("og_pcd_lin", [_G] + [_D] * 13 + [C.c_double, C.c_double, C.c_int, C.c_double])
+    s("og_pcd_lin_T", [_G] + [_D] * 13 + [C.c_double, C.c_double, C.c_int, C.c_double])
+    s("og_operator_coef_grad", [_G] + [_D] * 12 + [C.c_double, C.c_int, C.c_double] + [_D] * 8)
+    s("og_cell_properties", [_G] + [_D] * 8)
+    s("og_cell_properties_T", [_G] + [_D] * 9)
 
 
 def ptr(a):
@@ -331,3 +344,98 @@ def advance_step(st: State, phys: OgPhys, dt, step_idx, use_bicgstab2=True, u_li
 
 def adaptive_dt(g, phys, u, v, dt_max):
     return lib().og_adaptive_dt(C.byref(g.c), C.byref(phys), ptr(u), ptr(v), dt_max)
+
+
+# ---- adjoint building blocks (fugu2d_adjoint.c) -----------------------------------------
+PROP_NAMES = ("rho_c", "rho_u", "rho_v", "kappa", "zeta", "eta")
+
+
+def mask_owned(g, X):
+    """Zero the entries outside the owned degrees of freedom, in place (triplet list)."""
+    lib().og_mask_owned(C.byref(g.c), ptr(X[0]), ptr(X[1]), ptr(X[2]))
+    return X
+
+
+def halo_T(g, a, kind):
+    getattr(lib(), f"og_halo_{kind}_T")(C.byref(g.c), ptr(a))
+
+
+def velocity_bcs_T(g, u, v):
+    lib().og_velocity_bcs_T(C.byref(g.c), ptr(u), ptr(v))
+
+
+def extend(g, X):
+    """E: owned triplet -> the padded operand the matvec sees (copy)."""
+    Y = [a.copy() for a in X]
+    lib().og_extend(C.byref(g.c), ptr(Y[0]), ptr(Y[1]), ptr(Y[2]))
+    return Y
+
+
+def extend_T(g, X):
+    Y = [a.copy() for a in X]
+    lib().og_extend_T(C.byref(g.c), ptr(Y[0]), ptr(Y[1]), ptr(Y[2]))
+    return Y
+
+
+def _props_args(pr):
+    return [ptr(pr[k]) for k in PROP_NAMES]
+
+
+def matvec_eff(g, X, props, u_adv, v_adv, dt, include_advection=True):
+    Y = [g.zeros("u"), g.zeros("v"), g.zeros("p")]
+    lib().og_matvec_eff(C.byref(g.c), *[ptr(a) for a in X], *[ptr(a) for a in Y], ptr(u_adv), ptr(v_adv),
+                        *_props_args(props), dt, int(include_advection))
+    return Y
+
+
+def matvec_T(g, L, props, u_adv, v_adv, dt, include_advection=True):
+    Y = [g.zeros("u"), g.zeros("v"), g.zeros("p")]
+    lib().og_matvec_T(C.byref(g.c), *[ptr(a) for a in L], *[ptr(a) for a in Y], ptr(u_adv), ptr(v_adv),
+                      *_props_args(props), dt, int(include_advection))
+    return Y
+
+
+def _pcd_props(pr):
+    return [ptr(pr[k]) for k in ("rho_u", "rho_v", "kappa", "zeta", "eta")]
+
+
+def pcd_lin(g, Z, props, u_adv, v_adv, dt, ptol=1e-14, pmax=5000, visc=0.0):
+    R = [g.zeros("u"), g.zeros("v"), g.zeros("p")]
+    lib().og_pcd_lin(C.byref(g.c), *[ptr(a) for a in Z], *[ptr(a) for a in R], ptr(u_adv), ptr(v_adv),
+                     *_pcd_props(props), dt, ptol, pmax, visc)
+    return R
+
+
+def pcd_lin_T(g, Y, props, u_adv, v_adv, dt, ptol=1e-14, pmax=5000, visc=0.0):
+    Z = [g.zeros("u"), g.zeros("v"), g.zeros("p")]
+    lib().og_pcd_lin_T(C.byref(g.c), *[ptr(a) for a in Y], *[ptr(a) for a in Z], ptr(u_adv), ptr(v_adv),
+                       *_pcd_props(props), dt, ptol, pmax, visc)
+    return Z
+
+
+def operator_coef_grad(g, L, Xext, props, u_adv, v_adv, dt, include_advection=True, s=1.0):
+    """Gradients of s * lam^T A(coef; X) with respect to rho_c, rho_u, rho_v, kappa, zeta, eta
+    and the advecting velocity (u_adv, v_adv); Xext is the padded operand (extend(X))."""
+    out = dict(rho_c=g.zeros("p"), rho_u=g.zeros("u"), rho_v=g.zeros("v"), kappa=g.zeros("p"),
+               zeta=g.zeros("w"), eta=g.zeros("w"), u_adv=g.zeros("u"), v_adv=g.zeros("v"))
+    lib().og_operator_coef_grad(C.byref(g.c), ptr(L[0]), ptr(L[1]), ptr(Xext[0]), ptr(Xext[1]), ptr(u_adv),
+                                ptr(v_adv), *_props_args(props), dt, int(include_advection), s,
+                                *[ptr(out[k]) for k in ("rho_c", "rho_u", "rho_v", "kappa", "zeta", "eta",
+                                                         "u_adv", "v_adv")])
+    return out
+
+
+def cell_properties(g, mu_cell, rho_cell):
+    pr = dict(rho_c=g.zeros("p"), rho_u=g.zeros("u"), rho_v=g.zeros("v"), kappa=g.zeros("p"),
+              zeta=g.zeros("w"), eta=g.zeros("w"))
+    lib().og_cell_properties(C.byref(g.c), ptr(mu_cell), ptr(rho_cell), *_props_args(pr))
+    return pr
+
+
+def cell_properties_T(g, mu_cell, gcoef):
+    """Transpose of the Jacobian of cell_properties at mu_cell: coefficient gradients ->
+    (g_mu, g_rho) per padded cell."""
+    gc = {k: gcoef[k].copy() for k in PROP_NAMES}
+    g_mu, g_rho = g.zeros("p"), g.zeros("p")
+    lib().og_cell_properties_T(C.byref(g.c), ptr(mu_cell), *_props_args(gc), ptr(g_mu), ptr(g_rho))
+    return g_mu, g_rho
