@@ -42,6 +42,20 @@ class FuguStepStats(C.Structure):
                 ("solve_seconds", C.c_double), ("step_seconds", C.c_double), ("pcg_calls", C.c_int)]
 
 
+class FuguAdjointOpts(C.Structure):
+    _fields_ = [("nsteps", C.c_int), ("dt", C.c_double), ("y0", C.c_double), ("tol", C.c_double),
+                ("max_iter", C.c_int), ("restart", C.c_int), ("mg_cycles", C.c_int), ("flags", C.c_uint)]
+
+
+class FuguAdjointStats(C.Structure):
+    _fields_ = [("J", C.c_double), ("forward_seconds", C.c_double), ("reverse_seconds", C.c_double),
+                ("forward_iters", C.c_int), ("adjoint_iters", C.c_int), ("max_forward_res", C.c_double),
+                ("max_adjoint_res", C.c_double)]
+
+
+ADJ_FORWARD_ONLY = 1
+ADJ_REUSE_INTERFACE = 2
+
 B2_CONSISTENT_BCS = 1
 SOLVE_REFINED = 2
 SOLVE_MIXED = 4
@@ -57,7 +71,8 @@ EXPORTS = [
     "fugu_compute_mixture_densities", "fugu_compute_mixture_viscosities", "fugu_compute_adaptive_dt",
     "fugu_advance_coupled_step", "fugu_state_upload", "fugu_state_download", "fugu_field_download",
     "fugu_state_init_properties", "fugu_step", "fugu_kernel_timing", "fugu_kernel_time",
-    "fugu_solve_pressure_mg",
+    "fugu_solve_pressure_mg", "fugu_matvec_transpose", "fugu_apply_pcd_transpose", "fugu_adjoint_solve",
+    "fugu_cell_properties", "fugu_adjoint_gradient",
 ]
 
 _lib = None
@@ -115,6 +130,12 @@ def _declare(L):
     s("fugu_kernel_timing", [P, C.c_int])
     s("fugu_solve_pressure_mg", [P, _D, _D, _D, _D, C.c_int])
     s("fugu_kernel_time", [P, C.c_char_p, _D, C.POINTER(C.c_longlong), _D])
+    s("fugu_matvec_transpose", [P] + [_D] * 14 + [C.c_double, C.c_int])
+    s("fugu_apply_pcd_transpose", [P] + [_D] * 13 + [C.c_double, C.c_double, C.c_int, C.c_double])
+    s("fugu_adjoint_solve", [P] + [_D] * 14 + [C.c_double, C.c_double, C.c_int, _I, _D])
+    s("fugu_cell_properties", [P] + [_D] * 8)
+    s("fugu_adjoint_gradient", [P, C.POINTER(FuguAdjointOpts), _D, _D, C.c_double, _D, _D, _D, _D, _D, _D,
+                                C.POINTER(FuguAdjointStats)])
 
 
 def ptr(a):
@@ -182,6 +203,42 @@ class Context:
                                          ptr(props["rho_v"]), ptr(props["kappa"]), ptr(props["zeta"]),
                                          ptr(props["eta"]), dt, int(include_advection), ptr(pu), ptr(pv),
                                          ptr(mu), ptr(mv)), "matvec")
+
+    # -- adjoint (no counterpart in the reference) ---------------------------------------
+    def matvec_transpose(self, L, O, props, u_adv, v_adv, dt, include_advection=True):
+        check(load().fugu_matvec_transpose(self.h, ptr(L[0]), ptr(L[1]), ptr(L[2]), ptr(O[0]), ptr(O[1]), ptr(O[2]),
+                                           ptr(u_adv), ptr(v_adv), ptr(props["rho_c"]), ptr(props["rho_u"]),
+                                           ptr(props["rho_v"]), ptr(props["kappa"]), ptr(props["zeta"]),
+                                           ptr(props["eta"]), dt, int(include_advection)), "matvec_transpose")
+
+    def pcd_transpose(self, Y, Z, props, u_adv, v_adv, dt, ptol=1e-14, pmax=5000, visc=0.0):
+        check(load().fugu_apply_pcd_transpose(self.h, ptr(Y[0]), ptr(Y[1]), ptr(Y[2]), ptr(Z[0]), ptr(Z[1]),
+                                              ptr(Z[2]), ptr(u_adv), ptr(v_adv), ptr(props["rho_u"]),
+                                              ptr(props["rho_v"]), ptr(props["kappa"]), ptr(props["zeta"]),
+                                              ptr(props["eta"]), dt, ptol, pmax, visc), "pcd_transpose")
+
+    def adjoint_solve(self, L, G, props, u_adv, v_adv, dt, tol=1e-10, max_iter=400):
+        it, res = C.c_int(0), C.c_double(0)
+        rc = check(load().fugu_adjoint_solve(self.h, ptr(L[0]), ptr(L[1]), ptr(L[2]), ptr(G[0]), ptr(G[1]),
+                                             ptr(G[2]), ptr(u_adv), ptr(v_adv), ptr(props["rho_c"]),
+                                             ptr(props["rho_u"]), ptr(props["rho_v"]), ptr(props["kappa"]),
+                                             ptr(props["zeta"]), ptr(props["eta"]), dt, tol, max_iter,
+                                             C.byref(it), C.byref(res)), "adjoint_solve")
+        return it.value, res.value, rc
+
+    def cell_properties(self, mu_cell, rho_cell, out):
+        check(load().fugu_cell_properties(self.h, ptr(mu_cell), ptr(rho_cell), ptr(out["rho_c"]), ptr(out["rho_u"]),
+                                          ptr(out["rho_v"]), ptr(out["kappa"]), ptr(out["zeta"]), ptr(out["eta"])),
+              "cell_properties")
+
+    def adjoint_gradient(self, opts: FuguAdjointOpts, mu_cell, rho_cell, sigma, c0, u0, v0, g_mu=None, g_rho=None):
+        """Returns (J, g_sigma, stats, rc); g_mu, g_rho (padded p-kind arrays) are filled in place."""
+        gs = C.c_double(0)
+        st = FuguAdjointStats()
+        rc = check(load().fugu_adjoint_gradient(self.h, C.byref(opts), ptr(mu_cell), ptr(rho_cell), sigma, ptr(c0),
+                                                ptr(u0), ptr(v0), ptr(g_mu), ptr(g_rho), C.byref(gs), C.byref(st)),
+                   "adjoint_gradient")
+        return st.J, gs.value, st, rc
 
     def coupled_dot(self, a, b):
         out = C.c_double(0)

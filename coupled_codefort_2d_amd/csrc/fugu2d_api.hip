@@ -6,6 +6,7 @@
 #include "fugu2d.h"
 #include "fugu2d_kernels.hpp"
 #include "fugu2d_mg.hpp"
+#include "fugu2d_adjoint.hpp"
 
 #include <cmath>
 #include <cstdlib>
@@ -123,6 +124,19 @@ struct fugu_ctx {
   bool use_mg = false;   // pressure block by multigrid V-cycles instead of the reference PCG
   int mg_cycles = 1;
   double pcd_visc = 0.0; // viscous term of the Schur approximation (0 = the reference PCD)
+  // adjoint: solve_fgmres applies A_eff^T and the transposed PCD instead of A and the PCD
+  bool adjoint = false;
+  struct AdjWork {
+    double *y = nullptr, *hu0 = nullptr, *hu1 = nullptr, *zuwb = nullptr, *rvb = nullptr, *hv0 = nullptr;
+    double *hv1 = nullptr, *zvwb = nullptr, *rpb = nullptr, *phib = nullptr, *phi = nullptr;
+    double *mu = nullptr, *rho = nullptr, *bmu = nullptr, *bmv = nullptr, *t2 = nullptr, *ga = nullptr;
+    double *G = nullptr, *lam[3] = {nullptr, nullptr, nullptr}, *h = nullptr;
+    double *grc = nullptr, *gru = nullptr, *grv = nullptr, *gk = nullptr, *gz = nullptr, *ge = nullptr;
+    double *gmu = nullptr, *grho = nullptr;
+    bool ready = false;
+  } aw;
+  std::vector<double*> traj_u, traj_v, traj_fu, traj_fv;   // forward trajectory of the adjoint horizon
+  int traj_n = 0;
   bool kt_on = false;
   std::map<std::string, KTimer> kt;
   bool debug = std::getenv("FUGU_DEBUG") != nullptr;
@@ -373,9 +387,11 @@ void pcg_setup(fugu_ctx* c, Work<T>& w) {
 }
 
 template <typename T>
-void pcg(fugu_ctx* c, Work<T>& w, T* phi, const T* rhs, bool phi_zero, double tol, int max_iter) {
+void pcg(fugu_ctx* c, Work<T>& w, T* phi, const T* rhs, bool phi_zero, double tol, int max_iter,
+         bool force_proj = false) {
   const Grid& g = c->g;
-  const int walls_both = (!g.per_x && !g.per_y) ? 1 : 0;
+  // zero-mean projection of the rhs: both directions walls (the reference), or forced
+  const int walls_both = (force_proj || (!g.per_x && !g.per_y)) ? 1 : 0;
   const dim3 gc = grid_cells_red(g), bl(TX, TY), b1(NT);
   const int nb = (int)nblocks(gc);
   const double* pa = c->partials;
@@ -465,9 +481,9 @@ void mg_vcycle(fugu_ctx* c, Work<T>& w, size_t l, bool zero_guess) {
 // phi (padded, interior written) ~= A_p^-1 (z_p - mean) by mg_cycles V-cycles from zero.
 // Leaves S_P_MEAN = mean(phi) (both directions walls) and F_P_DONE = 0 for k_pcd_rp.
 template <typename T>
-void mg_pressure(fugu_ctx* c, Work<T>& w, T* phi, const T* Zp) {
+void mg_pressure(fugu_ctx* c, Work<T>& w, T* phi, const T* Zp, bool force_proj = false) {
   const Grid& g = c->g;
-  const int walls_both = (!g.per_x && !g.per_y) ? 1 : 0;
+  const int walls_both = (force_proj || (!g.per_x && !g.per_y)) ? 1 : 0;
   c->pcg_calls++;
   HIPCHK(hipMemsetAsync(c->fl + F_P_DONE, 0, sizeof(int), c->s));
   if (walls_both) cell_mean<T>(c, Zp, (int)S_P_MEAN, 0);
@@ -666,6 +682,120 @@ double true_residual(fugu_ctx* c, const double* X, const double* B, int mvflags,
   return nb > 0 ? sqrt(c->h_sc[S_DOT0] / nb) : sqrt(c->h_sc[S_DOT0]);
 }
 
+// ---------------------------------------------------------------------------------
+// Adjoint building blocks (no counterpart in the reference).
+void adj_alloc(fugu_ctx* c) {
+  auto& a = c->aw;
+  if (a.ready) return;
+  const Grid& g = c->g;
+  a.y = dalloc<double>(c, c->ntrip);
+  a.hu0 = dalloc<double>(c, g.nu); a.hu1 = dalloc<double>(c, g.nu); a.zuwb = dalloc<double>(c, g.nu);
+  a.rvb = dalloc<double>(c, g.nv); a.hv0 = dalloc<double>(c, g.nv); a.hv1 = dalloc<double>(c, g.nv);
+  a.zvwb = dalloc<double>(c, g.nv); a.rpb = dalloc<double>(c, g.np); a.phib = dalloc<double>(c, g.np);
+  a.phi = dalloc<double>(c, g.np);
+  a.mu = dalloc<double>(c, g.np); a.rho = dalloc<double>(c, g.np);
+  a.bmu = dalloc<double>(c, g.nu); a.bmv = dalloc<double>(c, g.nv);
+  a.t2 = dalloc<double>(c, c->ntrip); a.ga = dalloc<double>(c, c->ntrip); a.G = dalloc<double>(c, c->ntrip);
+  for (auto& l : a.lam) l = dalloc<double>(c, c->ntrip);
+  a.h = dalloc<double>(c, g.nu + g.nv);
+  a.grc = dalloc<double>(c, g.np); a.gru = dalloc<double>(c, g.nu); a.grv = dalloc<double>(c, g.nv);
+  a.gk = dalloc<double>(c, g.np); a.gz = dalloc<double>(c, g.nw); a.ge = dalloc<double>(c, g.nw);
+  a.gmu = dalloc<double>(c, g.np); a.grho = dalloc<double>(c, g.np);
+  a.ready = true;
+}
+
+void zero(fugu_ctx* c, double* a, size_t n) {
+  LAUNCH(k_axpby, dim3(nblk(n)), dim3(256), c->s, a, (const double*)nullptr, 0.0, (const double*)nullptr, 0.0, n);
+}
+void axpby(fugu_ctx* c, double* o, const double* x, double a, const double* y, double b, size_t n) {
+  LAUNCH(k_axpby, dim3(nblk(n)), dim3(256), c->s, o, x, a, y, b, n);
+}
+void acc_prod(fugu_ctx* c, double* o, double s, const double* w, const double* x, const double* y, size_t n) {
+  LAUNCH(k_acc_prod, dim3(nblk(n)), dim3(256), c->s, o, s, w, x, y, n);
+}
+
+// H^T of one padded field (y pass undone first, then x).
+void halo_T(fugu_ctx* c, double* a, int kind) {
+  const Grid& g = c->g;
+  if (g.per_y) {
+    const int ni = (kind == 0) ? g.nx + 1 + 2 * g.nh : g.nx + 2 * g.nh;
+    LAUNCH(k_halo_T_y, dim3(nblk((size_t)ni * g.nh)), dim3(256), c->s, g, a, kind);
+  }
+  if (g.per_x) {
+    const int nj = (kind == 1) ? g.ny + 1 + 2 * g.nh : g.ny + 2 * g.nh;
+    LAUNCH(k_halo_T_x, dim3(nblk((size_t)nj * g.nh)), dim3(256), c->s, g, a, kind);
+  }
+}
+// E^T of a triplet: H^T, then B^T (y walls, then x walls), then the owned-set mask.
+void extend_T(fugu_ctx* c, double* X) {
+  const Grid& g = c->g;
+  double *u = X, *v = X + g.nu, *p = X + g.nu + g.nv;
+  halo_T(c, u, 0); halo_T(c, v, 1); halo_T(c, p, 2);
+  if (!g.per_y) LAUNCH(k_bcs_T_y, dim3(nblk((size_t)g.nx + 1 + 2 * g.nh)), dim3(256), c->s, g, u, v);
+  if (!g.per_x) LAUNCH(k_bcs_T_x, dim3(nblk((size_t)g.ny + 1 + 2 * g.nh)), dim3(256), c->s, g, u, v);
+  LAUNCH(k_mask_trip, grid_union(g), dim3(TX, TY), c->s, g, u, v, p);
+}
+void mask_trip(fugu_ctx* c, double* X) {
+  const Grid& g = c->g;
+  LAUNCH(k_mask_trip, grid_union(g), dim3(TX, TY), c->s, g, X, X + g.nu, X + g.nu + g.nv);
+}
+
+// O = A_eff^T L with the context's FP64 operator fields (c->wd bindings).
+void matvec_T(fugu_ctx* c, const double* L, double* O, double dt, int adv) {
+  const Grid& g = c->g;
+  Work<double>& w = c->wd;
+  MatvecTArgs a;
+  a.Lu = L; a.Lv = L + g.nu; a.Lp = L + g.nu + g.nv;
+  a.Ou = O; a.Ov = O + g.nu; a.Op = O + g.nu + g.nv;
+  a.uadv = w.uadv; a.vadv = w.vadv; a.rho_c = w.rho_c; a.rho_u = w.rho_u; a.rho_v = w.rho_v;
+  a.kap = w.kap; a.zeta = w.zeta; a.eta = w.eta; a.dt = dt; a.adv = adv;
+  TLAUNCH(c, "matvec_T", 13.0 * sizeof(double) * g.nx * g.ny, k_matvec_T, grid_union(g), dim3(TX, TY), c->s, g, a);
+  extend_T(c, O);
+}
+
+// Z = P_L^T Y: the transposed linearised PCD.  Pressure block: ptol > 0 -> projected PCG to
+// ptol (pmax iterations at most); ptol <= 0 -> c->mg_cycles multigrid V-cycles (needs mg_setup).
+void pcd_T(fugu_ctx* c, const double* Y, double* Z, double dt, double ptol, int pmax, double visc) {
+  const Grid& g = c->g;
+  Work<double>& w = c->wd;
+  auto& a = c->aw;
+  const dim3 gu = grid_union(g), bl(TX, TY);
+  HIPCHK(hipMemcpyAsync(a.y, Y, c->ntrip * sizeof(double), hipMemcpyDeviceToDevice, c->s));
+  mask_trip(c, a.y);
+  const double *yu = a.y, *yv = a.y + g.nu;
+  double *Zu = Z, *Zv = Z + g.nu, *Zp = Z + g.nu + g.nv;
+  // r_u = J_u^3(z_uw): three transposed sweeps, z_uw,b accumulated
+  zero(c, a.zuwb, g.nu);
+  LAUNCH(k_jacobi_u_T, gu, bl, c->s, g, yu, a.hu0, a.zuwb, w.rho_u, w.kap, w.zeta, w.eta, dt);
+  LAUNCH(k_jacobi_u_T, gu, bl, c->s, g, (const double*)a.hu0, a.hu1, a.zuwb, w.rho_u, w.kap, w.zeta, w.eta, dt);
+  LAUNCH(k_jacobi_u_T, gu, bl, c->s, g, (const double*)a.hu1, a.hu0, a.zuwb, w.rho_u, w.kap, w.zeta, w.eta, dt);
+  // r_v from the output and from the u block's eta cross term
+  HIPCHK(hipMemcpyAsync(a.rvb, yv, g.nv * sizeof(double), hipMemcpyDeviceToDevice, c->s));
+  LAUNCH(k_cross_T, gu, bl, c->s, g, (const double*)a.zuwb, w.eta, a.rvb);
+  zero(c, a.zvwb, g.nv);
+  LAUNCH(k_jacobi_v_T, gu, bl, c->s, g, (const double*)a.rvb, a.hv0, a.zvwb, w.rho_v, w.kap, w.zeta, w.eta, dt);
+  LAUNCH(k_jacobi_v_T, gu, bl, c->s, g, (const double*)a.hv0, a.hv1, a.zvwb, w.rho_v, w.kap, w.zeta, w.eta, dt);
+  LAUNCH(k_jacobi_v_T, gu, bl, c->s, g, (const double*)a.hv1, a.hv0, a.zvwb, w.rho_v, w.kap, w.zeta, w.eta, dt);
+  // z_u, z_v and r_p,b from z_uw,b and z_vw,b
+  LAUNCH(k_zwork_T, gu, bl, c->s, g, (const double*)a.zuwb, (const double*)a.zvwb,
+         (const double*)(a.y + g.nu + g.nv), Zu, Zv, a.rpb);
+  zero(c, Zp, g.np);
+  LAUNCH(k_rp_T, gu, bl, c->s, g, (const double*)a.rpb, w.uadv, w.vadv, dt, w.kap, visc, a.phib, Zp);
+  // z_p += Pi A_p^+ Pi phi_b (symmetric), centred
+  zero(c, a.phi, g.np);
+  if (ptol > 0.0) {
+    reset_flags(c);
+    set_slot(c, S_P_TOL, ptol);
+    pcg_setup(c, w);
+    pcg<double>(c, w, a.phi, (const double*)a.phib, true, ptol, pmax, true);
+  } else {   // c->mg_cycles V-cycles from zero on the projected rhs (mg_setup done by the caller)
+    mg_pressure<double>(c, w, a.phi, (const double*)a.phib, true);
+  }
+  cell_mean<double>(c, (const double*)a.phi, (int)S_P_MEAN, 0);
+  LAUNCH(k_add_centred, gu, bl, c->s, g, Zp, (const double*)a.phi, (const double*)c->sc, (int)S_P_MEAN);
+  mask_trip(c, Z);
+}
+
 // the chosen fast solve.  Outer flexible GMRES(m) in FP64 on the true residual of the
 // coupled system (X BC'd, coupled_dot norm); its preconditioner is an inner BiCGStab(2)
 // + PCD solve of A z = v in precision T (FP32 for the mixed form, FP64 for the plain FP64
@@ -737,7 +867,8 @@ void solve_fgmres(fugu_ctx* c, Work<T>& w, double* X, double dt, double tol, dou
   while (true) {
     reset_flags(c);
     HIPCHK(hipMemcpyAsync(wd.X, X, n3 * sizeof(double), hipMemcpyDeviceToDevice, c->s));
-    matvec(c, wd, wd.X, wd.z, dt, mvflags, nullptr);
+    if (c->adjoint) matvec_T(c, wd.X, wd.z, dt, (mvflags & MV_ADV) ? 1 : 0);
+    else matvec(c, wd, wd.X, wd.z, dt, mvflags, nullptr);
     lin<double>(c, L_DIFF, c->kv[0], B, wd.z, nullptr, nullptr, -1, -1, -1, 1.0, false);
     cdot1<double>(c, c->kv[0], c->kv[0], PH_STORE);
     read_slots(c);
@@ -759,6 +890,8 @@ void solve_fgmres(fugu_ctx* c, Work<T>& w, double* X, double dt, double tol, dou
         LAUNCH((k_convert<double, T>), dim3(nblk(n3)), dim3(256), c->s, c->kz[k], (const T*)w.X, n3, c->sc, -1,
                1.0, 0);
         reset_flags(c);
+      } else if (c->adjoint) {
+        pcd_T(c, (const double*)c->kv[k], c->kz[k], dt, -1.0, c->mg_cycles, c->pcd_visc);
       } else if (sizeof(T) == sizeof(double)) {
         pcd(c, w, (const T*)c->kv[k], (T*)c->kz[k], (T)dt);
       } else {
@@ -772,7 +905,8 @@ void solve_fgmres(fugu_ctx* c, Work<T>& w, double* X, double dt, double tol, dou
       bcs_trip(c, c->kz[k]);
       // W = A z_k, then classical Gram-Schmidt twice, all coefficients device-resident
       HIPCHK(hipMemcpyAsync(wd.X, c->kz[k], n3 * sizeof(double), hipMemcpyDeviceToDevice, c->s));
-      matvec(c, wd, wd.X, wd.z, dt, mvflags, nullptr);
+      if (c->adjoint) matvec_T(c, wd.X, wd.z, dt, (mvflags & MV_ADV) ? 1 : 0);
+      else matvec(c, wd, wd.X, wd.z, dt, mvflags, nullptr);
       double* W = wd.z;
       // classical Gram-Schmidt, second pass only when the first lost more than half of ||W||^2
       dots_into(W, k, 0, false);
@@ -876,12 +1010,12 @@ int step_impl(fugu_ctx* c, double dt, int step_idx, const fugu_step_opts* o, fug
   const bool pen = c->has_solid;
   const bool amp = o->use_amp != 0;
   HIPCHK(hipEventRecord(c->ev[0], c->s));
-  // 1. u_adv, u_prev (:363-375): unm1 <- u_prev, u_prev <- u, u_adv <- 2u - unm1
+  // 1. u_adv, u_prev (src/mod_solver_coupled.f90:209-222): unm1 <- u_prev, u_prev <- u, u_adv <- 2u - unm1
   HIPCHK(hipMemcpyAsync(c->unm1, c->u_prev, g.nu * sizeof(double), hipMemcpyDeviceToDevice, c->s));
   HIPCHK(hipMemcpyAsync(c->vnm1, c->v_prev, g.nv * sizeof(double), hipMemcpyDeviceToDevice, c->s));
   LAUNCH(k_ab2, dim3(nblk(g.nu)), dim3(256), c->s, c->uadv, c->u_prev, (const double*)c->u, g.nu, step_idx <= 1 ? 1 : 0);
   LAUNCH(k_ab2, dim3(nblk(g.nv)), dim3(256), c->s, c->vadv, c->v_prev, (const double*)c->v, g.nv, step_idx <= 1 ? 1 : 0);
-  // 2. VOF (:377-388) with u^n
+  // 2. VOF (:224-235) with u^n
   vof(c, c->c, c->cnew, c->u, c->v, dt, amp ? c->fx : nullptr, amp ? c->fy : nullptr);
   if (amp) {
     LAUNCH(k_amp, dim3(nblk(g.nu)), dim3(256), c->s, c->mdu, (const double*)c->fx, (const double*)c->uadv, g.nu,
@@ -890,9 +1024,9 @@ int step_impl(fugu_ctx* c, double dt, int step_idx, const fugu_step_opts* o, fug
            P.rho1 - P.rho2, P.rho2);
   }
   std::swap(c->c, c->cnew);
-  // 3. mixture (:391-392)
+  // 3. mixture (:237-239)
   mixture(c, c->c);
-  // 4. SVOF + curvature + CSF (:395-405)
+  // 4. SVOF + curvature + CSF (:241-252)
   if (P.sigma > 1.0e-14) {
     const double delta = 1.5 * (g.dx < g.dy ? g.dx : g.dy);
     svof(c, c->c, c->cs, delta, 150, 1.0e-7, nullptr);
@@ -903,12 +1037,12 @@ int step_impl(fugu_ctx* c, double dt, int step_idx, const fugu_step_opts* o, fug
     HIPCHK(hipMemsetAsync(c->fu, 0, g.nu * sizeof(double), c->s));
     HIPCHK(hipMemsetAsync(c->fv, 0, g.nv * sizeof(double), c->s));
   }
-  // 5. penalty (:408-412)
+  // 5. penalty (:254-259)
   if (pen) {
     LAUNCH(k_scale, dim3(nblk(g.nu)), dim3(256), c->s, c->penu, (const double*)c->phis_u, g.nu, P.alpha_penalty);
     LAUNCH(k_scale, dim3(nblk(g.nv)), dim3(256), c->s, c->penv, (const double*)c->phis_v, g.nv, P.alpha_penalty);
   }
-  // 6. RHS (:415-456) into wd.B
+  // 6. RHS (:261-303) into wd.B
   Work<double>& w = c->wd;
   {
     RhsArgs a;
@@ -920,7 +1054,7 @@ int step_impl(fugu_ctx* c, double dt, int step_idx, const fugu_step_opts* o, fug
     a.bdf2 = bdf2 ? 1 : 0; a.has_lid = o->has_lid ? 1 : 0;
     LAUNCH(k_rhs, grid_union(g), dim3(TX, TY), c->s, g, a);
   }
-  // 7. solve (:459-519)
+  // 7. solve (:305-366)
   bind_props_d(c, pen, amp);
   const int mvflags = MV_ADV | (pen ? MV_PEN : 0) | (amp ? MV_CONS : 0);
   HIPCHK(hipEventRecord(c->ev[1], c->s));
@@ -953,7 +1087,7 @@ int step_impl(fugu_ctx* c, double dt, int step_idx, const fugu_step_opts* o, fug
     HIPCHK(hipMemcpyAsync(c->X, w.X, c->ntrip * sizeof(double), hipMemcpyDeviceToDevice, c->s));
   }
   HIPCHK(hipEventRecord(c->ev[2], c->s));
-  // 8. final halos + BCs (:522-525)
+  // 8. final halos + BCs (:368-372)
   halo(c, c->u, 0);
   halo(c, c->v, 1);
   halo(c, c->p, 2);
@@ -978,6 +1112,226 @@ int step_impl(fugu_ctx* c, double dt, int step_idx, const fugu_step_opts* o, fug
   int rc = 0;
   if (!(res == res)) rc |= FUGU_W_NONFINITE;
   if (!(o->solver_flags & FUGU_SOLVE_REFINED) && iters >= o->max_iter) rc |= FUGU_W_MAX_ITER;
+  return rc;
+}
+
+// ---------------------------------------------------------------------------------
+// Multi-step adjoint of the frozen-interface BDF2 horizon (no counterpart in the reference).
+//   forward: properties from the per-cell fields (fixed over the horizon); per step n
+//     u_adv = u^{n-1} (n = 1) or 2u^{n-1} - u^{n-2}; C^n = VOF(C^{n-1}, u^{n-1}); unit-sigma CSF
+//     force fhat^n from C^n (recorded); B = (rho/dt) hist + rho g + sigma fhat; A_eff(dt_eff) X^n
+//     = B by FP64 FGMRES + PCD (multigrid pressure block, viscous Schur term) to opts->tol;
+//     u^n = E X^n.  J = 1/2 sum_band rho_face u^2 vol at step N.
+//   reverse: A_n^T lam_n = dJ/dX^n + couplings of steps n+1, n+2 (BDF history and the AB2
+//     advecting velocity), by FGMRES on A_eff^T with the transposed PCD; gradients
+//     -lam^T dA/dcoef X + lam^T dB/dcoef chained through the per-cell property map.
+int adjoint_run(fugu_ctx* c, const fugu_adjoint_opts* o, const double* mu_in, const double* rho_in, double sigma,
+                const double* c0, const double* u0, const double* v0, double* g_mu, double* g_rho,
+                double* g_sigma, fugu_adjoint_stats* st) {
+  const Grid& g = c->g;
+  const fugu_params& P = c->prm;
+  const int N = o->nsteps;
+  const double dt = o->dt;
+  if (N < 1 || !(dt > 0.0)) throw Error("adjoint: nsteps >= 1 and dt > 0 required");
+  const bool reuse = (o->flags & FUGU_ADJ_REUSE_INTERFACE) != 0;
+  const bool fwd_only = (o->flags & FUGU_ADJ_FORWARD_ONLY) != 0;
+  if (reuse && c->traj_n != N) throw Error("adjoint: no recorded interface of this horizon to reuse");
+  if (!mu_in || !rho_in || !u0 || !v0 || (!reuse && !c0)) throw Error("adjoint: null input field");
+  adj_alloc(c);
+  auto& a = c->aw;
+  Work<double>& wd = c->wd;
+  const size_t n3 = c->ntrip, nu = g.nu, nv = g.nv;
+  while ((int)c->traj_u.size() < N + 1) {
+    c->traj_u.push_back(dalloc<double>(c, nu)); c->traj_v.push_back(dalloc<double>(c, nv));
+    c->traj_fu.push_back(dalloc<double>(c, nu)); c->traj_fv.push_back(dalloc<double>(c, nv));
+  }
+  auto& U = c->traj_u;
+  auto& V = c->traj_v;
+  auto& FU = c->traj_fu;
+  auto& FV = c->traj_fv;
+  const double vol = g.vol;
+  const int rm = o->restart > 0 ? o->restart : 30;
+  const int mi = o->max_iter > 0 ? o->max_iter : 400;
+  const double tol = o->tol > 0 ? o->tol : 1.0e-12;
+  // properties
+  in(c, a.mu, mu_in, g.np); in(c, a.rho, rho_in, g.np);
+  LAUNCH(k_cell_props, grid_union(g), dim3(TX, TY), c->s, g, (const double*)a.mu, (const double*)a.rho, c->rho_c,
+         c->rho_u, c->rho_v, c->kap, c->zeta, c->eta);
+  bind_props_d(c, false, false);
+  c->use_mg = true;
+  c->pcd_visc = 1.0;
+  c->mg_cycles = o->mg_cycles > 0 ? o->mg_cycles : 1;
+  // initial state (BCs and halos as the end of a step leaves them)
+  in(c, U[0], u0, nu); in(c, V[0], v0, nv);
+  bcs(c, U[0], V[0], 0.0); halo(c, U[0], 0); halo(c, V[0], 1);
+  if (!reuse) in(c, c->c, c0, g.np);
+  HIPCHK(hipMemcpyAsync(c->u, U[0], nu * sizeof(double), hipMemcpyDeviceToDevice, c->s));
+  HIPCHK(hipMemcpyAsync(c->v, V[0], nv * sizeof(double), hipMemcpyDeviceToDevice, c->s));
+  zero(c, c->p, g.np);
+  auto set_uadv = [&](int n) {   // u_adv of step n
+    if (n <= 1) {
+      HIPCHK(hipMemcpyAsync(c->uadv, U[0], nu * sizeof(double), hipMemcpyDeviceToDevice, c->s));
+      HIPCHK(hipMemcpyAsync(c->vadv, V[0], nv * sizeof(double), hipMemcpyDeviceToDevice, c->s));
+    } else {
+      axpby(c, c->uadv, U[n - 1], 2.0, U[n - 2], -1.0, nu);
+      axpby(c, c->vadv, V[n - 1], 2.0, V[n - 2], -1.0, nv);
+    }
+  };
+  auto bdf2_of = [&](int n) { return P.time_order == 2 && n >= 2; };
+  auto dte_of = [&](int n) { return bdf2_of(n) ? dt / 1.5 : dt; };
+  int fwd_it = 0, adj_it = 0;
+  double fwd_res = 0.0, adj_res = 0.0;
+  HIPCHK(hipEventRecord(c->ev[0], c->s));
+  for (int n = 1; n <= N; ++n) {
+    set_uadv(n);
+    if (!reuse) {
+      vof(c, c->c, c->cnew, U[n - 1], V[n - 1], dt, nullptr, nullptr);
+      std::swap(c->c, c->cnew);
+      if (sigma > 1.0e-14) {
+        const double delta = 1.5 * (g.dx < g.dy ? g.dx : g.dy);
+        svof(c, c->c, c->cs, delta, 150, 1.0e-7, nullptr);
+        curvature(c, c->cs, c->kcurv);
+        LAUNCH(k_csf, grid_union(g), dim3(TX, TY), c->s, g, (const double*)c->c, (const double*)c->kcurv, 1.0,
+               FU[n], FV[n]);
+      } else {
+        zero(c, FU[n], nu); zero(c, FV[n], nv);
+      }
+    }
+    axpby(c, c->fu, FU[n], sigma, nullptr, 0.0, nu);
+    axpby(c, c->fv, FV[n], sigma, nullptr, 0.0, nv);
+    RhsArgs r;
+    r.un = U[n - 1]; r.unm1 = (n >= 2) ? U[n - 2] : U[0]; r.vn = V[n - 1]; r.vnm1 = (n >= 2) ? V[n - 2] : V[0];
+    r.rho_u = c->rho_u; r.rho_v = c->rho_v; r.fu = c->fu; r.fv = c->fv; r.penu = nullptr; r.penv = nullptr;
+    r.zeta = c->zeta; r.eta = c->eta;
+    r.Bu = wd.B; r.Bv = wd.B + nu; r.Bp = wd.B + nu + nv;
+    r.dt = dt; r.gx = P.gx; r.gy = P.gy; r.u_solid = 0.0; r.v_solid = 0.0; r.u_lid = 0.0;
+    r.bdf2 = bdf2_of(n) ? 1 : 0; r.has_lid = 0;
+    LAUNCH(k_rhs, grid_union(g), dim3(TX, TY), c->s, g, r);
+    int it = 0, outer = 0;
+    double res = 0.0;
+    solve_fgmres<double>(c, wd, c->X, dte_of(n), tol, 0.0, mi, 0, rm, MV_ADV, false, 0.0, &it, &outer, &res);
+    fwd_it += outer;
+    fwd_res = std::max(fwd_res, res);
+    // u^n = E X^n
+    mask_trip(c, c->X);
+    bcs(c, c->u, c->v, 0.0); halo(c, c->u, 0); halo(c, c->v, 1); halo(c, c->p, 2);
+    HIPCHK(hipMemcpyAsync(U[n], c->u, nu * sizeof(double), hipMemcpyDeviceToDevice, c->s));
+    HIPCHK(hipMemcpyAsync(V[n], c->v, nv * sizeof(double), hipMemcpyDeviceToDevice, c->s));
+  }
+  // J = 1/2 coupled_dot((band rho u, band rho v, 0), (u, v, 0))
+  LAUNCH(k_band_mask, grid_union(g), dim3(TX, TY), c->s, g, o->y0, a.bmu, a.bmv);
+  zero(c, a.y, n3); zero(c, a.t2, n3);
+  acc_prod(c, a.y, 1.0, a.bmu, c->rho_u, U[N], nu);
+  acc_prod(c, a.y + nu, 1.0, a.bmv, c->rho_v, V[N], nv);
+  HIPCHK(hipMemcpyAsync(a.t2, U[N], nu * sizeof(double), hipMemcpyDeviceToDevice, c->s));
+  HIPCHK(hipMemcpyAsync(a.t2 + nu, V[N], nv * sizeof(double), hipMemcpyDeviceToDevice, c->s));
+  reset_flags(c);
+  cdot1<double>(c, a.y, a.t2, PH_STORE);
+  read_slots(c);
+  const double J = 0.5 * c->h_sc[S_DOT0];
+  HIPCHK(hipEventRecord(c->ev[1], c->s));
+  c->traj_n = N;
+  double gs = 0.0;
+  if (!fwd_only) {
+    zero(c, a.grc, g.np); zero(c, a.gru, nu); zero(c, a.grv, nv); zero(c, a.gk, g.np);
+    zero(c, a.gz, g.nw); zero(c, a.ge, g.nw); zero(c, a.gmu, g.np); zero(c, a.grho, g.np);
+    auto coef_grad = [&](const double* L, int n, double s, int what, double* gua, double* gva) {
+      CoefGradArgs q;
+      q.Lu = L; q.Lv = L + nu; q.Xu = U[n]; q.Xv = V[n]; q.uadv = c->uadv; q.vadv = c->vadv;
+      q.rho_c = c->rho_c; q.rho_u = c->rho_u; q.rho_v = c->rho_v; q.kap = c->kap; q.zeta = c->zeta; q.eta = c->eta;
+      q.grc = a.grc; q.gru = a.gru; q.grv = a.grv; q.gk = a.gk; q.gz = a.gz; q.ge = a.ge; q.gua = gua; q.gva = gva;
+      q.dt = dte_of(n); q.s = s; q.adv = 1; q.what = what;
+      LAUNCH(k_coef_grad, grid_union(g), dim3(TX, TY), c->s, g, q);
+    };
+    c->adjoint = true;
+    for (int n = N; n >= 1; --n) {
+      double* G = a.G;
+      zero(c, G, n3);
+      if (n == N) {
+        acc_prod(c, G, vol, a.bmu, c->rho_u, U[N], nu);
+        acc_prod(c, G + nu, vol, a.bmv, c->rho_v, V[N], nv);
+      }
+      for (int m = n + 1; m <= n + 2 && m <= N; ++m) {
+        const double* L = a.lam[m % 3];
+        const bool b2 = bdf2_of(m);
+        const double cb = (m == n + 1) ? (b2 ? 2.0 : 1.0) : (b2 ? -0.5 : 0.0);
+        if (cb != 0.0) {
+          acc_prod(c, G, cb / dt, c->rho_u, L, nullptr, nu);
+          acc_prod(c, G + nu, cb / dt, c->rho_v, L + nu, nullptr, nv);
+        }
+        // through u_adv^m = 2 u^{m-1} - u^{m-2} (m >= 2)
+        const double cadv = (m == n + 1) ? 2.0 : -1.0;
+        set_uadv(m);
+        zero(c, a.ga, n3);
+        coef_grad(L, m, 1.0, CG_ADV, a.ga, a.ga + nu);
+        extend_T(c, a.ga);
+        acc_prod(c, G, -cadv, nullptr, a.ga, nullptr, nu);
+        acc_prod(c, G + nu, -cadv, nullptr, a.ga + nu, nullptr, nv);
+      }
+      mask_trip(c, G);
+      // A_n^T lam_n = G
+      set_uadv(n);
+      HIPCHK(hipMemcpyAsync(wd.B, G, n3 * sizeof(double), hipMemcpyDeviceToDevice, c->s));
+      double* Ln = a.lam[n % 3];
+      zero(c, Ln, n3);
+      int it = 0, outer = 0;
+      double res = 0.0;
+      solve_fgmres<double>(c, wd, Ln, dte_of(n), tol, 0.0, mi, 0, rm, MV_ADV, false, 0.0, &it, &outer, &res);
+      adj_it += outer;
+      adj_res = std::max(adj_res, res);
+      mask_trip(c, Ln);
+      // - lam^T dA/dcoef X^n
+      coef_grad(Ln, n, -1.0, CG_COEF, nullptr, nullptr);
+      // + lam^T dB/dcoef: rho_u (hist / dt + g_x), rho_v alike
+      const bool b2 = bdf2_of(n);
+      const double* um2 = (n >= 2) ? U[n - 2] : U[0];
+      const double* vm2 = (n >= 2) ? V[n - 2] : V[0];
+      axpby(c, a.h, U[n - 1], b2 ? 2.0 : 1.0, b2 ? um2 : nullptr, -0.5, nu);
+      axpby(c, a.h + nu, V[n - 1], b2 ? 2.0 : 1.0, b2 ? vm2 : nullptr, -0.5, nv);
+      acc_prod(c, a.gru, 1.0 / dt, nullptr, Ln, a.h, nu);
+      acc_prod(c, a.grv, 1.0 / dt, nullptr, Ln + nu, a.h + nu, nv);
+      acc_prod(c, a.gru, P.gx, nullptr, Ln, nullptr, nu);
+      acc_prod(c, a.grv, P.gy, nullptr, Ln + nu, nullptr, nv);
+      // sigma: lam^T fhat
+      zero(c, a.t2, n3);
+      HIPCHK(hipMemcpyAsync(a.t2, FU[n], nu * sizeof(double), hipMemcpyDeviceToDevice, c->s));
+      HIPCHK(hipMemcpyAsync(a.t2 + nu, FV[n], nv * sizeof(double), hipMemcpyDeviceToDevice, c->s));
+      reset_flags(c);
+      cdot1<double>(c, Ln, a.t2, PH_STORE);
+      read_slots(c);
+      gs += c->h_sc[S_DOT0] / vol;
+    }
+    c->adjoint = false;
+    // explicit dJ/drho_face, then the property map
+    acc_prod(c, a.gru, 0.5 * vol, a.bmu, U[N], U[N], nu);
+    acc_prod(c, a.grv, 0.5 * vol, a.bmv, V[N], V[N], nv);
+    LAUNCH(k_cell_props_T, grid_union(g), dim3(TX, TY), c->s, g, (const double*)a.mu, (const double*)a.grc,
+           (const double*)a.gru, (const double*)a.grv, (const double*)a.gk, (const double*)a.gz,
+           (const double*)a.ge, a.gmu, a.grho);
+    out(c, g_mu, a.gmu, g.np);
+    out(c, g_rho, a.grho, g.np);
+  }
+  HIPCHK(hipEventRecord(c->ev[2], c->s));
+  sync(c);
+  kt_resolve(c);
+  c->use_mg = false;
+  c->pcd_visc = 0.0;
+  if (g_sigma) *g_sigma = gs;
+  if (st) {
+    float f_ms = 0.f, r_ms = 0.f;
+    HIPCHK(hipEventElapsedTime(&f_ms, c->ev[0], c->ev[1]));
+    HIPCHK(hipEventElapsedTime(&r_ms, c->ev[1], c->ev[2]));
+    st->J = J;
+    st->forward_seconds = f_ms * 1e-3;
+    st->reverse_seconds = r_ms * 1e-3;
+    st->forward_iters = fwd_it;
+    st->adjoint_iters = adj_it;
+    st->max_forward_res = fwd_res;
+    st->max_adjoint_res = adj_res;
+  }
+  int rc = 0;
+  if (!(fwd_res == fwd_res) || !(adj_res == adj_res) || !(J == J)) rc |= FUGU_W_NONFINITE;
+  if (fwd_res > tol || adj_res > tol) rc |= FUGU_W_MAX_ITER;
   return rc;
 }
 
@@ -1584,6 +1938,115 @@ int fugu_advance_coupled_step(fugu_ctx* c, double* u, double* v, double* p, doub
     if (iter_out) *iter_out = st.iters;
     if (res_out) *res_out = st.res;
     return rc;
+  });
+}
+
+// ---- adjoint (no counterpart in the reference) --------------------------------------
+int fugu_matvec_transpose(fugu_ctx* c, const double* Lu, const double* Lv, const double* Lp, double* Ou, double* Ov,
+                          double* Op, const double* uadv, const double* vadv, const double* rho_c,
+                          const double* rho_u, const double* rho_v, const double* kap, const double* zeta,
+                          const double* eta, double dt, int include_adv) {
+  NEED(c);
+  return guarded([&]() {
+    const Grid& g = c->g;
+    Work<double>& w = c->wd;
+    upload_props(c, uadv, vadv, rho_c, rho_u, rho_v, kap, zeta, eta, nullptr, nullptr, nullptr, nullptr);
+    in(c, w.X, Lu, g.nu); in(c, w.X + g.nu, Lv, g.nv); in(c, w.X + g.nu + g.nv, Lp, g.np);
+    matvec_T(c, w.X, w.z, dt, include_adv ? 1 : 0);
+    out(c, Ou, w.z, g.nu); out(c, Ov, w.z + g.nu, g.nv); out(c, Op, w.z + g.nu + g.nv, g.np);
+    sync(c);
+    return 0;
+  });
+}
+
+int fugu_apply_pcd_transpose(fugu_ctx* c, const double* yu, const double* yv, const double* yp, double* zu,
+                             double* zv, double* zp, const double* uadv, const double* vadv, const double* rho_u,
+                             const double* rho_v, const double* kap, const double* zeta, const double* eta,
+                             double dt, double ptol, int pmax, double visc) {
+  NEED(c);
+  return guarded([&]() {
+    const Grid& g = c->g;
+    Work<double>& w = c->wd;
+    upload_props(c, uadv, vadv, nullptr, rho_u, rho_v, kap, zeta, eta, nullptr, nullptr, nullptr, nullptr);
+    adj_alloc(c);
+    in(c, w.B, yu, g.nu); in(c, w.B + g.nu, yv, g.nv); in(c, w.B + g.nu + g.nv, yp, g.np);
+    if (ptol <= 0.0) {
+      c->mg_cycles = pmax > 0 ? pmax : 1;
+      mg_setup(c, w);
+    }
+    pcd_T(c, w.B, w.X, dt, ptol, pmax > 0 ? pmax : 5000, visc);
+    out(c, zu, w.X, g.nu); out(c, zv, w.X + g.nu, g.nv); out(c, zp, w.X + g.nu + g.nv, g.np);
+    sync(c);
+    c->mg_cycles = 1;
+    return 0;
+  });
+}
+
+int fugu_adjoint_solve(fugu_ctx* c, double* Lu, double* Lv, double* Lp, const double* Gu, const double* Gv,
+                       const double* Gp, const double* uadv, const double* vadv, const double* rho_c,
+                       const double* rho_u, const double* rho_v, const double* kap, const double* zeta,
+                       const double* eta, double dt, double tol, int max_iter, int* iter_out, double* res_out) {
+  NEED(c);
+  return guarded([&]() {
+    const Grid& g = c->g;
+    Work<double>& w = c->wd;
+    upload_props(c, uadv, vadv, rho_c, rho_u, rho_v, kap, zeta, eta, nullptr, nullptr, nullptr, nullptr);
+    adj_alloc(c);
+    in(c, w.B, Gu, g.nu); in(c, w.B + g.nu, Gv, g.nv); in(c, w.B + g.nu + g.nv, Gp, g.np);
+    mask_trip(c, w.B);
+    double* L = c->aw.G;
+    zero(c, L, c->ntrip);
+    in(c, L, Lu, g.nu); in(c, L + g.nu, Lv, g.nv); in(c, L + g.nu + g.nv, Lp, g.np);
+    mask_trip(c, L);
+    c->use_mg = true; c->pcd_visc = 1.0; c->mg_cycles = 1; c->adjoint = true;
+    int it = 0, outer = 0;
+    double res = 0.0;
+    try {
+      solve_fgmres<double>(c, w, L, dt, tol > 0 ? tol : 1.0e-10, 0.0, max_iter > 0 ? max_iter : 400, 0, 30, MV_ADV,
+                           false, 0.0, &it, &outer, &res);
+    } catch (...) {
+      c->adjoint = false; c->use_mg = false; c->pcd_visc = 0.0;
+      throw;
+    }
+    c->adjoint = false; c->use_mg = false; c->pcd_visc = 0.0;
+    mask_trip(c, L);
+    out(c, Lu, L, g.nu); out(c, Lv, L + g.nu, g.nv); out(c, Lp, L + g.nu + g.nv, g.np);
+    sync(c);
+    if (iter_out) *iter_out = outer;
+    if (res_out) *res_out = res;
+    return (res <= (tol > 0 ? tol : 1.0e-10)) ? 0 : (int)FUGU_W_MAX_ITER;
+  });
+}
+
+int fugu_cell_properties(fugu_ctx* c, const double* mu_cell, const double* rho_cell, double* rho_c, double* rho_u,
+                         double* rho_v, double* kap, double* zeta, double* eta) {
+  NEED(c);
+  return guarded([&]() {
+    const Grid& g = c->g;
+    adj_alloc(c);
+    if (!mu_cell || !rho_cell) throw Error("cell_properties: null input");
+    in(c, c->aw.mu, mu_cell, g.np); in(c, c->aw.rho, rho_cell, g.np);
+    LAUNCH(k_cell_props, grid_union(g), dim3(TX, TY), c->s, g, (const double*)c->aw.mu, (const double*)c->aw.rho,
+           c->rho_c, c->rho_u, c->rho_v, c->kap, c->zeta, c->eta);
+    out(c, rho_c, c->rho_c, g.np); out(c, rho_u, c->rho_u, g.nu); out(c, rho_v, c->rho_v, g.nv);
+    out(c, kap, c->kap, g.np); out(c, zeta, c->zeta, g.nw); out(c, eta, c->eta, g.nw);
+    sync(c);
+    return 0;
+  });
+}
+
+int fugu_adjoint_gradient(fugu_ctx* c, const fugu_adjoint_opts* o, const double* mu_cell, const double* rho_cell,
+                          double sigma, const double* c0, const double* u0, const double* v0, double* g_mu,
+                          double* g_rho, double* g_sigma, fugu_adjoint_stats* st) {
+  NEED(c);
+  if (!o) { g_err = "adjoint_gradient: null options"; return -2; }
+  return guarded([&]() {
+    try {
+      return adjoint_run(c, o, mu_cell, rho_cell, sigma, c0, u0, v0, g_mu, g_rho, g_sigma, st);
+    } catch (...) {
+      c->adjoint = false; c->use_mg = false; c->pcd_visc = 0.0;
+      throw;
+    }
   });
 }
 

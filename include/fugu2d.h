@@ -214,6 +214,81 @@ int fugu_kernel_time(fugu_ctx* ctx, const char* kernel, double* total_ms, long l
 int fugu_step(fugu_ctx* ctx, double dt_step, int step_idx, const fugu_step_opts* opts,
               fugu_step_stats* stats);
 
+/* ---- adjoint (no counterpart in the reference: it has no adjoint) ----------------------
+ * The step operator as the solvers see it maps the owned degrees of freedom (u: j=1..ny,
+ * i=1..i_u_max; v: i=1..nx, j=1..j_v_max; p: interior) to the owned rows:
+ *     A_eff X = R A_pad H B E0 X
+ * (embed, velocity BCs with lid 0, halo exchange, the padded matvec of
+ * src/mod_operators.f90:341-443, owned rows).  Inputs are read on the owned sets; outputs are
+ * zero outside them. */
+/* A_eff^T L (transpose of src/mod_operators.f90:341-443 with the BC extension folded in). */
+int fugu_matvec_transpose(fugu_ctx* ctx, const double* L_u, const double* L_v, const double* L_p,
+                          double* O_u, double* O_v, double* O_p,
+                          const double* u_adv, const double* v_adv,
+                          const double* rho_c, const double* rho_u, const double* rho_v,
+                          const double* kappa, const double* zeta, const double* eta,
+                          double dt_step, int include_advection);
+/* Transpose of the linearised PCD (src/mod_pcd_precond.f90:14-104 with the truncated PCG of
+ * src/mod_subsolvers.f90:32-246 replaced by the projected pressure solve phi = Pi A_p^+ Pi z_p):
+ * ptol > 0 -> projected Jacobi-PCG to ptol, at most pmax iterations (pmax <= 0: 5000);
+ * ptol <= 0 -> pmax multigrid V-cycles.  visc: coefficient of the viscous Schur term. */
+int fugu_apply_pcd_transpose(fugu_ctx* ctx, const double* y_u, const double* y_v, const double* y_p,
+                             double* z_u, double* z_v, double* z_p,
+                             const double* u_adv, const double* v_adv,
+                             const double* rho_u, const double* rho_v,
+                             const double* kappa, const double* zeta, const double* eta,
+                             double dt_step, double ptol, int pmax, double visc);
+/* A_eff^T L = G by FP64 FGMRES(30) on the true residual with the transposed PCD (multigrid
+ * pressure block, viscous Schur term) as flexible preconditioner; L in: initial guess, out:
+ * solution.  tol <= 0: 1e-10; max_iter <= 0: 400.  Returns FUGU_W_MAX_ITER if not converged. */
+int fugu_adjoint_solve(fugu_ctx* ctx, double* L_u, double* L_v, double* L_p,
+                       const double* G_u, const double* G_v, const double* G_p,
+                       const double* u_adv, const double* v_adv,
+                       const double* rho_c, const double* rho_u, const double* rho_v,
+                       const double* kappa, const double* zeta, const double* eta,
+                       double dt_step, double tol, int max_iter, int* iter_out, double* res_out);
+/* Per-cell property path (the configurations with per-inclusion properties): mu_cell, rho_cell
+ * over the padded cells -> the operator coefficients with the edge conventions of
+ * src/mod_viscosity.f90:54-151; vertex viscosity 4 / sum(1/mu_k), which equals
+ * harmonic_viscosity(mean C) (:28-46) for mixture-law inputs. */
+int fugu_cell_properties(fugu_ctx* ctx, const double* mu_cell, const double* rho_cell,
+                         double* rho_c, double* rho_u, double* rho_v,
+                         double* kappa, double* zeta, double* eta);
+
+typedef struct {
+  int nsteps;           /* horizon N (BDF2 from step 2 when time_order is 2) */
+  double dt;            /* fixed step */
+  double y0;            /* objective band: J = 1/2 sum over owned faces with centre y >= y0 of
+                           rho_face u^2 dx dy at step N */
+  double tol;           /* true relative residual of every forward and adjoint solve (<= 0: 1e-12) */
+  int max_iter;         /* FGMRES steps per solve (<= 0: 400) */
+  int restart;          /* FGMRES restart (<= 0: 30) */
+  int mg_cycles;        /* V-cycles per preconditioner application (<= 0: 1) */
+  unsigned flags;       /* FUGU_ADJ_* */
+} fugu_adjoint_opts;
+enum {
+  FUGU_ADJ_FORWARD_ONLY = 1u << 0,     /* J only (no reverse sweep) */
+  /* use the interface (unit-sigma CSF forces) recorded by the previous call with the same
+   * horizon instead of advecting C: the frozen-interface map the gradient differentiates, as
+   * finite differences of the gradient need it */
+  FUGU_ADJ_REUSE_INTERFACE = 1u << 1
+};
+typedef struct {
+  double J;
+  double forward_seconds, reverse_seconds;   /* device time (HIP events) */
+  int forward_iters, adjoint_iters;           /* FGMRES steps summed over the horizon */
+  double max_forward_res, max_adjoint_res;    /* worst true relative residual */
+} fugu_adjoint_stats;
+/* Gradient of J with respect to the per-cell viscosity and density (padded p-kind arrays)
+ * and the surface tension, over a BDF2 horizon from (u0, v0, C0), properties held at the
+ * per-cell fields, interface motion frozen in the adjoint (VOF, SVOF and CSF of the forward
+ * pass recorded and held fixed), all solves FP64 to opts->tol.  g_mu, g_rho, g_sigma may be
+ * NULL with FUGU_ADJ_FORWARD_ONLY; c0 may be NULL with FUGU_ADJ_REUSE_INTERFACE. */
+int fugu_adjoint_gradient(fugu_ctx* ctx, const fugu_adjoint_opts* opts, const double* mu_cell,
+                          const double* rho_cell, double sigma, const double* c0, const double* u0,
+                          const double* v0, double* g_mu, double* g_rho, double* g_sigma,
+                          fugu_adjoint_stats* stats);
+
 #ifdef __cplusplus
 }
 #endif
