@@ -19,6 +19,7 @@ from __future__ import annotations
 import argparse
 import json
 import os
+import re
 import sys
 import time
 
@@ -38,20 +39,9 @@ SEED = 3
 
 
 def bubble_c(nx, ny, lx, ly, nh=2, r0=0.2, xc=0.5, yc=0.6, seed=SEED):
-    """C = 0 inside r(theta) = r0 (1 + 0.01 sum_m a_m cos(m theta + phi_m)), 1 outside, sampled at
-    cell centres over the full padded array (the initialiser of src/main.f90:137-151 moved
-    to (0.5, 0.6) with the seeded perturbation)."""
-    rng = np.random.default_rng(seed)
-    a = rng.uniform(-1.0, 1.0, 8) / 8.0
-    ph = rng.uniform(0.0, 2.0 * np.pi, 8)
-    dx, dy = lx / nx, ly / ny
-    i = np.arange(1 - nh, nx + nh + 1, dtype=np.float64)
-    j = np.arange(1 - nh, ny + nh + 1, dtype=np.float64)
-    X, Y = np.meshgrid((i - 0.5) * dx, (j - 0.5) * dy)
-    th = np.arctan2(Y - yc, X - xc)
-    r = np.hypot(X - xc, Y - yc)
-    R = r0 * (1.0 + 0.01 * sum(a[m - 1] * np.cos(m * th + ph[m - 1]) for m in range(1, 9)))
-    return np.where(r <= R, 0.0, 1.0)
+    """The rising-bubble volume fraction (coupled_codefort_2d_amd.workloads.rising_bubble_c)."""
+    from coupled_codefort_2d_amd import workloads
+    return workloads.rising_bubble_c(nx, ny, lx, ly, nh, r0, xc, yc, seed)
 
 
 def make_ctx(abi):
@@ -96,8 +86,10 @@ def run_form(abi, flags, steps, warmup, max_outer, inner_tol, mg_cycles, inner_m
 # kernels timed live with HIP events on the library's stream; the library reports each
 # launch's algorithmic bytes (unique fields read + written once)
 KERNELS = ("pcg_iter1_f64", "pcg_iter2_f64", "pcg_iter1_f32", "pcg_iter2_f32", "matvec_f64", "matvec_f32",
-           "mg_smooth_f64", "mg_smooth_f32", "mg_resid_f64", "mg_resid_f32", "mdot", "maxpy")
-DEVICE_NAMES = {"mdot": "k_mdot", "maxpy": "k_maxpy"}
+           "mg_smooth_f64", "mg_smooth_f32", "mg_resid_f64", "mg_resid_f32", "mdot_f64", "mdot_f32", "maxpy_f64",
+           "maxpy_f32", "maxpy_update")
+DEVICE_NAMES = {"maxpy_f64": "k_maxpy<double, double>", "maxpy_f32": "k_maxpy<float, float>",
+                "maxpy_update": "k_maxpy<double, float>"}
 
 
 def device_name(name):
@@ -120,7 +112,8 @@ def roofline(ktimes):
            "bytes_per_launch": round(bytes_launch), "avg_launch_us": round(avg_s * 1e6, 3), "launches": n}
     # measured HBM bytes / algorithmic bytes of this kernel on this workload (two rocprofv3
     # --pmc runs, FETCH_SIZE and WRITE_SIZE, scaled by a plain copy; hbm_traffic.py)
-    prof = os.path.join(ROOT, "profiles", f"hbm_traffic_{device_name(name)}_rising_bubble_{NX}x{NY}.txt")
+    safe = re.sub(r"[^A-Za-z0-9]+", "_", device_name(name)).strip("_")
+    prof = os.path.join(ROOT, "profiles", f"hbm_traffic_{safe}_rising_bubble_{NX}x{NY}.txt")
     if os.path.exists(prof):
         for ln in open(prof):
             if ln.startswith("ratio:"):
@@ -220,7 +213,7 @@ def main():
     from coupled_codefort_2d_amd import abi
     abi.load()
     forms = {}
-    base = abi.SOLVE_REFINED | abi.PRECOND_MG | abi.PRECOND_VISC
+    base = abi.SOLVE_REFINED | abi.PRECOND_MG | abi.PRECOND_VISC | abi.PRECOND_VMG
     kw = dict(steps=args.steps, warmup=args.warmup, max_outer=args.max_outer, inner_tol=args.inner_tol,
               mg_cycles=args.mg_cycles, inner_max=args.inner_max, restart=args.restart)
     if "fp64" in args.forms:
@@ -244,7 +237,9 @@ def main():
     rep = {k: summary(v) for k, v in forms.items()}
     all_k = {}
     for f in forms.values():
-        all_k.update(f["ktimes"])
+        for k, v in f["ktimes"].items():   # names carry the precision; sum over forms that share one
+            o = all_k.get(k, (0.0, 0, 0.0))
+            all_k[k] = (o[0] + v[0], o[1] + v[1], o[2] + v[2])
     line = {
         "metric": "coupled solves/s to 1e-8 residual, FP64 vs mixed; HBM GB/s % peak; adjoint s/grad",
         "value": rep[head]["solves_per_s"],
@@ -258,8 +253,10 @@ def main():
                                "BDF2 dt 2e-4, solves to true relative residual 1e-8",
                    "nx": NX, "ny": NY, "form": head},
         "forms": rep,
-        "form_notes": {"fp64": "FP64 FGMRES + FP64 PCD (MG pressure block), true residual 1e-8",
-                       "mixed": "FP64 FGMRES + FP32 PCD (MG pressure block), true residual 1e-8",
+        "form_notes": {"fp64": "FGMRES(30) in FP64 + FP64 PCD (multigrid pressure and velocity blocks), "
+                               "true residual 1e-8",
+                       "mixed": "FGMRES(30) Arnoldi cycles in FP32 storage (FP64 dots and Hessenberg) + FP32 PCD; "
+                                "true residual and update in FP64 at every restart; true residual 1e-8",
                        "ref": "drop-in: reference BiCGStab + PCD(PCG 150), reference stopping test 1e-8"},
         "roofline": roofline(all_k),
         "kernels": {k: {"avg_us": round(1e3 * v[0] / v[1], 2), "launches": v[1], "total_ms": round(v[0], 2),

@@ -55,12 +55,14 @@ class FuguAdjointStats(C.Structure):
 
 ADJ_FORWARD_ONLY = 1
 ADJ_REUSE_INTERFACE = 2
+ADJ_PRECOND_VMG = 4
 
 B2_CONSISTENT_BCS = 1
 SOLVE_REFINED = 2
 SOLVE_MIXED = 4
 PRECOND_MG = 8
 PRECOND_VISC = 16
+PRECOND_VMG = 32
 
 EXPORTS = [
     "fugu_ctx_create", "fugu_ctx_destroy", "fugu_set_params", "fugu_last_error", "fugu_field_size",

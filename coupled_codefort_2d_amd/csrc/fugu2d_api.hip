@@ -13,6 +13,7 @@
 #include <cstdio>
 #include <algorithm>
 #include <cstring>
+#include <type_traits>
 #include <map>
 #include <stdexcept>
 #include <string>
@@ -59,13 +60,18 @@ struct Error : std::runtime_error {
 template <typename T>
 struct MgLevel {
   MgDims d;
-  T *cx = nullptr, *cy = nullptr, *dinv = nullptr, *b = nullptr, *phi = nullptr;
+  T *cx = nullptr, *cy = nullptr, *m = nullptr, *dinv = nullptr, *b = nullptr, *phi = nullptr;
 };
+
+// A multigrid hierarchy from the finest extents down (mass: allocate the mass terms).
+template <typename T>
+std::vector<MgLevel<T>> mg_levels(fugu_ctx* c, MgDims d, bool mass);
 
 // Per-precision solver workspace.  Triplets are contiguous [u | v | p].
 template <typename T>
 struct Work {
-  std::vector<MgLevel<T>> mg;   // pressure multigrid hierarchy (chosen fast forms)
+  std::vector<MgLevel<T>> mg;       // pressure multigrid hierarchy (chosen fast forms)
+  std::vector<MgLevel<T>> vmg[2];   // u and v velocity-block hierarchies (allocated on first use)
   T *X = nullptr, *B = nullptr, *bh = nullptr, *r = nullptr, *rt = nullptr, *uu = nullptr;
   T *vv = nullptr, *ww = nullptr, *ss = nullptr, *tt = nullptr, *z = nullptr, *q = nullptr;
   // properties (aliases of the context's FP64 fields for T = double)
@@ -118,10 +124,12 @@ struct fugu_ctx {
   Work<float> wf;
   float *f_penu = nullptr, *f_penv = nullptr, *f_mdu = nullptr, *f_mdv = nullptr;
   std::vector<double*> kv, kz;   // outer FGMRES basis and preconditioned directions (FP64 triplets)
+  std::vector<float*> kv32, kz32; // the same in FP32 for the mixed form's Arnoldi cycles
   double* fg = nullptr;          // outer FGMRES Hessenberg / Givens state (device)
   hipEvent_t fg_ev[2] = {nullptr, nullptr};
   int pcg_calls = 0;
   bool use_mg = false;   // pressure block by multigrid V-cycles instead of the reference PCG
+  bool use_vmg = false;  // velocity blocks by multigrid V-cycles instead of three Jacobi sweeps
   int mg_cycles = 1;
   double pcd_visc = 0.0; // viscous term of the Schur approximation (0 = the reference PCD)
   // adjoint: solve_fgmres applies A_eff^T and the transposed PCD instead of A and the PCD
@@ -326,22 +334,7 @@ void work_alloc(fugu_ctx* c, Work<T>& w, bool own_props) {
   w.cx = dalloc<T>(c, g.nu); w.cy = dalloc<T>(c, g.nv);
   w.zuw = dalloc<T>(c, g.nu); w.ju0 = dalloc<T>(c, g.nu); w.ju1 = dalloc<T>(c, g.nu);
   w.zvw = dalloc<T>(c, g.nv); w.jv0 = dalloc<T>(c, g.nv); w.jv1 = dalloc<T>(c, g.nv);
-  // multigrid levels: factor-2 coarsening while both extents are even and >= 8
-  MgDims d{g.nx, g.ny, g.per_x, g.per_y};
-  for (;;) {
-    MgLevel<T> L;
-    L.d = d;
-    const size_t n = (size_t)d.nx * d.ny;
-    L.cx = dalloc<T>(c, (size_t)(d.nx + 1) * d.ny);
-    L.cy = dalloc<T>(c, (size_t)d.nx * (d.ny + 1));
-    L.dinv = dalloc<T>(c, n);
-    L.b = dalloc<T>(c, n);
-    L.phi = dalloc<T>(c, n);
-    w.mg.push_back(L);
-    if (d.nx % 2 || d.ny % 2 || d.nx < 8 || d.ny < 8) break;
-    d.nx /= 2;
-    d.ny /= 2;
-  }
+  w.mg = mg_levels<T>(c, MgDims{g.nx, g.ny, g.per_x, g.per_y}, false);
   w.ready = true;
 }
 
@@ -423,17 +416,61 @@ inline dim3 g2(int nx, int ny, int bx = 64, int by = 4) {
 }
 
 template <typename T>
+std::vector<MgLevel<T>> mg_levels(fugu_ctx* c, MgDims d, bool mass) {
+  std::vector<MgLevel<T>> lv;
+  for (;;) {
+    MgLevel<T> L;
+    L.d = d;
+    const size_t n = (size_t)d.nx * d.ny;
+    L.cx = dalloc<T>(c, (size_t)(d.nx + 1) * d.ny);
+    L.cy = dalloc<T>(c, (size_t)d.nx * (d.ny + 1));
+    if (mass) L.m = dalloc<T>(c, n);
+    L.dinv = dalloc<T>(c, n);
+    L.b = dalloc<T>(c, n);
+    L.phi = dalloc<T>(c, n);
+    lv.push_back(L);
+    if (!mg_can_coarsen(d)) break;
+    d = mg_coarse_dims(d);
+  }
+  return lv;
+}
+
+// Coarse operators and inverse diagonals below a level 0 whose c_x, c_y (and m) are set.
+template <typename T>
+void mg_galerkin(fugu_ctx* c, std::vector<MgLevel<T>>& lv) {
+  const dim3 bl(64, 4);
+  for (size_t l = 0; l + 1 < lv.size(); ++l) {
+    const MgDims& f = lv[l].d;
+    const MgDims& C = lv[l + 1].d;
+    LAUNCH((k_mg_coarsen<T>), g2(C.nx + 1, C.ny + 1), bl, c->s, f, (const T*)lv[l].cx, (const T*)lv[l].cy,
+           lv[l + 1].cx, lv[l + 1].cy);
+    if (lv[l].m) LAUNCH((k_mg_coarsen_m<T>), g2(C.nx, C.ny), bl, c->s, f, (const T*)lv[l].m, lv[l + 1].m);
+  }
+  for (auto& L : lv)
+    LAUNCH((k_mg_dinv<T>), g2(L.d.nx, L.d.ny), bl, c->s, L.d, (const T*)L.cx, (const T*)L.cy, (const T*)L.m,
+           L.dinv);
+}
+
+template <typename T>
 void mg_setup(fugu_ctx* c, Work<T>& w) {
   const Grid& g = c->g;
-  const dim3 bl(64, 4);
-  LAUNCH((k_mg_coef0<T>), g2(g.nx + 1, g.ny + 1), bl, c->s, g, w.rho_u, w.rho_v, w.mg[0].cx, w.mg[0].cy);
-  for (size_t l = 0; l + 1 < w.mg.size(); ++l) {
-    const MgDims& f = w.mg[l].d;
-    LAUNCH((k_mg_coarsen<T>), g2(f.nx / 2 + 1, f.ny / 2 + 1), bl, c->s, f, (const T*)w.mg[l].cx,
-           (const T*)w.mg[l].cy, w.mg[l + 1].cx, w.mg[l + 1].cy);
+  LAUNCH((k_mg_coef0<T>), g2(g.nx + 1, g.ny + 1), dim3(64, 4), c->s, g, w.rho_u, w.rho_v, w.mg[0].cx, w.mg[0].cy);
+  mg_galerkin(c, w.mg);
+}
+
+// Velocity-block hierarchies from the current properties and dt (allocated on first use).
+template <typename T>
+void vmg_setup(fugu_ctx* c, Work<T>& w, T dt) {
+  const Grid& g = c->g;
+  for (int comp = 0; comp < 2; ++comp) {
+    auto& lv = w.vmg[comp];
+    if (lv.empty()) lv = mg_levels<T>(c, vmg_dims(g, comp), true);
+    MgLevel<T>& L = lv[0];
+    LAUNCH((k_vmg_coef0<T>), g2(L.d.nx + 1, L.d.ny + 1), dim3(64, 4), c->s, g, L.d, comp,
+           comp == 0 ? (const T*)w.rho_u : (const T*)w.rho_v, (const T*)w.kap, (const T*)w.zeta, (const T*)w.eta,
+           dt, comp == 0 ? (const T*)w.penu : (const T*)w.penv, L.cx, L.cy, L.m);
+    mg_galerkin(c, lv);
   }
-  for (auto& L : w.mg)
-    LAUNCH((k_mg_dinv<T>), g2(L.d.nx, L.d.ny), bl, c->s, L.d, (const T*)L.cx, (const T*)L.cy, L.dinv);
 }
 
 template <typename T>
@@ -445,15 +482,16 @@ void mg_smooth(fugu_ctx* c, MgLevel<T>& L, int colour, int zero_other) {
           c->s, L.d, colour, (const T*)L.cx, (const T*)L.cy, (const T*)L.dinv, (const T*)L.b, L.phi, zero_other);
 }
 
-// V-cycle on level l; `zero_guess`: phi of level l is (treated as) zero on entry.
+// V-cycle on level l; `zero_guess`: phi of level l is (treated as) zero on entry; `project`:
+// the operator is singular (no mass term, no Dirichlet face) and the coarsest rhs is centred.
 template <typename T>
-void mg_vcycle(fugu_ctx* c, Work<T>& w, size_t l, bool zero_guess) {
-  MgLevel<T>& L = w.mg[l];
+void mg_vcycle(fugu_ctx* c, std::vector<MgLevel<T>>& lv, size_t l, bool zero_guess, int project) {
+  MgLevel<T>& L = lv[l];
   const int nu = 2;
-  if (l + 1 == w.mg.size()) {
+  if (l + 1 == lv.size()) {
     if ((size_t)L.d.nx * L.d.ny <= (size_t)MG_COARSE_MAX) {
       LAUNCH((k_mg_coarse<T>), dim3(1), dim3(MG_COARSE_MAX), c->s, L.d, (const T*)L.cx, (const T*)L.cy,
-             (const T*)L.dinv, (const T*)L.b, L.phi, 60, 1);
+             (const T*)L.dinv, (const T*)L.b, L.phi, 60, project);
     } else {
       for (int s = 0; s < 40; ++s) {
         mg_smooth(c, L, 0, (s == 0 && zero_guess) ? 1 : 0);
@@ -466,11 +504,12 @@ void mg_vcycle(fugu_ctx* c, Work<T>& w, size_t l, bool zero_guess) {
     mg_smooth(c, L, 0, (s == 0 && zero_guess) ? 1 : 0);
     mg_smooth(c, L, 1, 0);
   }
-  MgLevel<T>& C = w.mg[l + 1];
-  // fine c_x, c_y, b, phi read; coarse b, phi written (a quarter of the cells each)
-  TLAUNCH(c, sizeof(T) == 8 ? "mg_resid_f64" : "mg_resid_f32", 4.5 * sizeof(T) * L.d.nx * L.d.ny, (k_mg_restrict<T>), g2(C.d.nx, C.d.ny),
-          dim3(64, 4), c->s, L.d, (const T*)L.cx, (const T*)L.cy, (const T*)L.b, (const T*)L.phi, C.b, C.phi);
-  mg_vcycle(c, w, l + 1, true);
+  MgLevel<T>& C = lv[l + 1];
+  // fine c_x, c_y, (m,) b, phi read; coarse b, phi written (a quarter of the cells each)
+  TLAUNCH(c, sizeof(T) == 8 ? "mg_resid_f64" : "mg_resid_f32", (L.m ? 5.5 : 4.5) * sizeof(T) * L.d.nx * L.d.ny,
+          (k_mg_restrict<T>), g2(C.d.nx, C.d.ny), dim3(64, 4), c->s, L.d, (const T*)L.cx, (const T*)L.cy,
+          (const T*)L.m, (const T*)L.b, (const T*)L.phi, C.b, C.phi);
+  mg_vcycle(c, lv, l + 1, true, project);
   LAUNCH((k_mg_prolong<T>), g2(L.d.nx, L.d.ny), dim3(64, 4), c->s, L.d, (const T*)C.phi, L.phi);
   for (int s = 0; s < nu; ++s) {
     mg_smooth(c, L, 1, 0);
@@ -489,9 +528,21 @@ void mg_pressure(fugu_ctx* c, Work<T>& w, T* phi, const T* Zp, bool force_proj =
   if (walls_both) cell_mean<T>(c, Zp, (int)S_P_MEAN, 0);
   LAUNCH((k_mg_load<T>), g2(g.nx, g.ny), dim3(64, 4), c->s, g, Zp, w.mg[0].b, walls_both, (const double*)c->sc,
          (int)S_P_MEAN);
-  for (int k = 0; k < c->mg_cycles; ++k) mg_vcycle(c, w, 0, k == 0);   // later cycles smooth from phi
+  for (int k = 0; k < c->mg_cycles; ++k) mg_vcycle(c, w.mg, 0, k == 0, 1);   // later cycles smooth from phi
   LAUNCH((k_mg_store<T>), g2(g.nx, g.ny), dim3(64, 4), c->s, g, (const T*)w.mg[0].phi, phi);
   if (walls_both) cell_mean<T>(c, (const T*)phi, (int)S_P_MEAN, 0);
+}
+
+// out (padded u or v array) ~= A_comp^-1 rhs by one V-cycle from zero on the velocity block
+// (rows of the block; wall faces copy rhs, ghosts zero, as the Jacobi blocks leave them).
+template <typename T>
+void vmg_solve(fugu_ctx* c, Work<T>& w, int comp, const T* rhs, T* out) {
+  const Grid& g = c->g;
+  auto& lv = w.vmg[comp];
+  MgLevel<T>& L = lv[0];
+  LAUNCH((k_vmg_load<T>), g2(L.d.nx, L.d.ny), dim3(64, 4), c->s, g, L.d, rhs, comp, L.b);
+  mg_vcycle(c, lv, 0, true, 0);
+  LAUNCH((k_vmg_store<T>), grid_union(g), dim3(TX, TY), c->s, g, comp, (const T*)L.phi, rhs, out);
 }
 
 // block PCD preconditioner R = P^-1 Z (triplets).
@@ -513,14 +564,22 @@ void pcd(fugu_ctx* c, Work<T>& w, const T* Z, T* R, T dt) {
          (T)c->pcd_visc);
   halo(c, Rp, 2);
   LAUNCH((k_pcd_zwork<T>), gu, bl, c->s, g, Zu, Zv, (const T*)Rp, w.zuw, w.zvw);
-  LAUNCH((k_jacobi_v<T>), gu, bl, c->s, g, (const T*)nullptr, w.jv0, w.zvw, w.rho_v, w.kap, w.zeta, w.eta, dt, w.penv);
-  LAUNCH((k_jacobi_v<T>), gu, bl, c->s, g, (const T*)w.jv0, w.jv1, w.zvw, w.rho_v, w.kap, w.zeta, w.eta, dt, w.penv);
-  LAUNCH((k_jacobi_v<T>), gu, bl, c->s, g, (const T*)w.jv1, Rv, w.zvw, w.rho_v, w.kap, w.zeta, w.eta, dt, w.penv);
+  if (c->use_vmg) {
+    vmg_solve<T>(c, w, 1, w.zvw, Rv);
+  } else {
+    LAUNCH((k_jacobi_v<T>), gu, bl, c->s, g, (const T*)nullptr, w.jv0, w.zvw, w.rho_v, w.kap, w.zeta, w.eta, dt, w.penv);
+    LAUNCH((k_jacobi_v<T>), gu, bl, c->s, g, (const T*)w.jv0, w.jv1, w.zvw, w.rho_v, w.kap, w.zeta, w.eta, dt, w.penv);
+    LAUNCH((k_jacobi_v<T>), gu, bl, c->s, g, (const T*)w.jv1, Rv, w.zvw, w.rho_v, w.kap, w.zeta, w.eta, dt, w.penv);
+  }
   halo(c, Rv, 1);
   LAUNCH((k_pcd_cross<T>), gc, bl, c->s, g, (const T*)Rv, w.eta, w.zuw);
-  LAUNCH((k_jacobi_u<T>), gu, bl, c->s, g, (const T*)nullptr, w.ju0, w.zuw, w.rho_u, w.kap, w.zeta, w.eta, dt, w.penu);
-  LAUNCH((k_jacobi_u<T>), gu, bl, c->s, g, (const T*)w.ju0, w.ju1, w.zuw, w.rho_u, w.kap, w.zeta, w.eta, dt, w.penu);
-  LAUNCH((k_jacobi_u<T>), gu, bl, c->s, g, (const T*)w.ju1, Ru, w.zuw, w.rho_u, w.kap, w.zeta, w.eta, dt, w.penu);
+  if (c->use_vmg) {
+    vmg_solve<T>(c, w, 0, w.zuw, Ru);
+  } else {
+    LAUNCH((k_jacobi_u<T>), gu, bl, c->s, g, (const T*)nullptr, w.ju0, w.zuw, w.rho_u, w.kap, w.zeta, w.eta, dt, w.penu);
+    LAUNCH((k_jacobi_u<T>), gu, bl, c->s, g, (const T*)w.ju0, w.ju1, w.zuw, w.rho_u, w.kap, w.zeta, w.eta, dt, w.penu);
+    LAUNCH((k_jacobi_u<T>), gu, bl, c->s, g, (const T*)w.ju1, Ru, w.zuw, w.rho_u, w.kap, w.zeta, w.eta, dt, w.penu);
+  }
   halo(c, Ru, 0);
 }
 
@@ -536,6 +595,7 @@ void bicgstab1(fugu_ctx* c, Work<T>& w, double dt, double tol, int max_iter, int
   set_slot(c, S_P_TOL, 1.0e-5);
   pcg_setup(c, w);
   if (c->use_mg) mg_setup(c, w);
+  if (c->use_vmg) vmg_setup(c, w, (T)dt);
   bcs_trip(c, X);
   pcd(c, w, w.B, w.bh, (T)dt);
   cdot1<T>(c, w.bh, w.bh, PH_NORM_B);
@@ -600,6 +660,7 @@ void bicgstab2(fugu_ctx* c, Work<T>& w, double dt, double tol, int max_iter, int
   set_slot(c, S_P_TOL, 1.0e-5);
   pcg_setup(c, w);
   if (c->use_mg) mg_setup(c, w);
+  if (c->use_vmg) vmg_setup(c, w, (T)dt);
   bcs_trip(c, X);
   pcd(c, w, w.B, w.bh, (T)dt);
   cdot1<T>(c, w.bh, w.bh, PH_NORM_B);
@@ -764,18 +825,27 @@ void pcd_T(fugu_ctx* c, const double* Y, double* Z, double dt, double ptol, int 
   mask_trip(c, a.y);
   const double *yu = a.y, *yv = a.y + g.nu;
   double *Zu = Z, *Zv = Z + g.nu, *Zp = Z + g.nu + g.nv;
-  // r_u = J_u^3(z_uw): three transposed sweeps, z_uw,b accumulated
-  zero(c, a.zuwb, g.nu);
-  LAUNCH(k_jacobi_u_T, gu, bl, c->s, g, yu, a.hu0, a.zuwb, w.rho_u, w.kap, w.zeta, w.eta, dt);
-  LAUNCH(k_jacobi_u_T, gu, bl, c->s, g, (const double*)a.hu0, a.hu1, a.zuwb, w.rho_u, w.kap, w.zeta, w.eta, dt);
-  LAUNCH(k_jacobi_u_T, gu, bl, c->s, g, (const double*)a.hu1, a.hu0, a.zuwb, w.rho_u, w.kap, w.zeta, w.eta, dt);
+  // r_u = J_u^3(z_uw): three transposed sweeps, z_uw,b accumulated (velocity multigrid: the
+  // symmetric V-cycle of the symmetric u block stands for its own transpose)
+  if (c->use_vmg) {
+    vmg_solve<double>(c, w, 0, yu, a.zuwb);
+  } else {
+    zero(c, a.zuwb, g.nu);
+    LAUNCH(k_jacobi_u_T, gu, bl, c->s, g, yu, a.hu0, a.zuwb, w.rho_u, w.kap, w.zeta, w.eta, dt);
+    LAUNCH(k_jacobi_u_T, gu, bl, c->s, g, (const double*)a.hu0, a.hu1, a.zuwb, w.rho_u, w.kap, w.zeta, w.eta, dt);
+    LAUNCH(k_jacobi_u_T, gu, bl, c->s, g, (const double*)a.hu1, a.hu0, a.zuwb, w.rho_u, w.kap, w.zeta, w.eta, dt);
+  }
   // r_v from the output and from the u block's eta cross term
   HIPCHK(hipMemcpyAsync(a.rvb, yv, g.nv * sizeof(double), hipMemcpyDeviceToDevice, c->s));
   LAUNCH(k_cross_T, gu, bl, c->s, g, (const double*)a.zuwb, w.eta, a.rvb);
-  zero(c, a.zvwb, g.nv);
-  LAUNCH(k_jacobi_v_T, gu, bl, c->s, g, (const double*)a.rvb, a.hv0, a.zvwb, w.rho_v, w.kap, w.zeta, w.eta, dt);
-  LAUNCH(k_jacobi_v_T, gu, bl, c->s, g, (const double*)a.hv0, a.hv1, a.zvwb, w.rho_v, w.kap, w.zeta, w.eta, dt);
-  LAUNCH(k_jacobi_v_T, gu, bl, c->s, g, (const double*)a.hv1, a.hv0, a.zvwb, w.rho_v, w.kap, w.zeta, w.eta, dt);
+  if (c->use_vmg) {
+    vmg_solve<double>(c, w, 1, (const double*)a.rvb, a.zvwb);
+  } else {
+    zero(c, a.zvwb, g.nv);
+    LAUNCH(k_jacobi_v_T, gu, bl, c->s, g, (const double*)a.rvb, a.hv0, a.zvwb, w.rho_v, w.kap, w.zeta, w.eta, dt);
+    LAUNCH(k_jacobi_v_T, gu, bl, c->s, g, (const double*)a.hv0, a.hv1, a.zvwb, w.rho_v, w.kap, w.zeta, w.eta, dt);
+    LAUNCH(k_jacobi_v_T, gu, bl, c->s, g, (const double*)a.hv1, a.hv0, a.zvwb, w.rho_v, w.kap, w.zeta, w.eta, dt);
+  }
   // z_u, z_v and r_p,b from z_uw,b and z_vw,b
   LAUNCH(k_zwork_T, gu, bl, c->s, g, (const double*)a.zuwb, (const double*)a.zvwb,
          (const double*)(a.y + g.nu + g.nv), Zu, Zv, a.rpb);
@@ -796,22 +866,32 @@ void pcd_T(fugu_ctx* c, const double* Y, double* Z, double dt, double ptol, int 
   mask_trip(c, Z);
 }
 
-// the chosen fast solve.  Outer flexible GMRES(m) in FP64 on the true residual of the
-// coupled system (X BC'd, coupled_dot norm); its preconditioner is an inner BiCGStab(2)
-// + PCD solve of A z = v in precision T (FP32 for the mixed form, FP64 for the plain FP64
-// form) with a capped iteration count.  Flexibility absorbs the inner solve's inexactness
-// (FP32 rounding, non-converged recurrences), and every outer step minimises the FP64
-// residual, so the stopping test ||B - A X|| / ||B|| <= tol is met in FP64.
+// the chosen fast solve: restarted flexible GMRES(m) on the coupled system (coupled_dot
+// norm), each restart recomputing the true residual B - A X and updating X in FP64, so the
+// stopping test ||B - A X|| / ||B|| <= tol is met in FP64.  Preconditioner (max_inner > 0):
+// an inner BiCGStab(2) + PCD solve of A z = v in precision T with a capped iteration count;
+// (max_inner == 0) the PCD itself, in precision T.  With T = float and the PCD preconditioner
+// the whole Arnoldi cycle runs in FP32 storage (GMRES-based iterative refinement): basis,
+// directions, matvec operands and the preconditioner are FP32 triplets, the Gram-Schmidt
+// products, sums and the Hessenberg/Givens scalars FP64; the cycle's correction is added to
+// the FP64 X and the next restart's residual is FP64 again.  Flexibility absorbs the
+// preconditioner's inexactness (FP32 rounding, non-converged inner recurrences).
 template <typename T>
 void solve_fgmres(fugu_ctx* c, Work<T>& w, double* X, double dt, double tol, double inner_tol, int max_steps,
                   int max_inner, int m, int mvflags, bool has_lid, double lid, int* iters, int* outer,
                   double* res) {
   Work<double>& wd = c->wd;
   const size_t n3 = c->ntrip;
+  const bool ir32 = sizeof(T) == sizeof(float) && max_inner == 0 && !c->adjoint;
   if (m > FG_M - 1) m = FG_M - 1;
   if (m < 1) m = 1;
   while ((int)c->kv.size() < m + 1) c->kv.push_back(dalloc<double>(c, n3));
-  while ((int)c->kz.size() < m) c->kz.push_back(dalloc<double>(c, n3));
+  if (ir32) {
+    while ((int)c->kv32.size() < m + 1) c->kv32.push_back(dalloc<float>(c, n3));
+    while ((int)c->kz32.size() < m) c->kz32.push_back(dalloc<float>(c, n3));
+  } else {
+    while ((int)c->kz.size() < m) c->kz.push_back(dalloc<double>(c, n3));
+  }
   if (!c->fg) c->fg = dalloc<double>(c, FG_SIZE);
   double* B = c->Bst;
   HIPCHK(hipMemcpyAsync(B, wd.B, n3 * sizeof(double), hipMemcpyDeviceToDevice, c->s));
@@ -821,6 +901,7 @@ void solve_fgmres(fugu_ctx* c, Work<T>& w, double* X, double dt, double tol, dou
     set_slot(c, S_P_TOL, 1.0e-5);
     pcg_setup(c, w);
     if (c->use_mg) mg_setup(c, w);
+    if (c->use_vmg) vmg_setup(c, w, (T)dt);
   }
   cdot1<double>(c, B, B, PH_STORE);
   read_slots(c);
@@ -829,37 +910,47 @@ void solve_fgmres(fugu_ctx* c, Work<T>& w, double* X, double dt, double tol, dou
   double rr = 1.0;
   // o = x0 + sgn sum_q c_q xs[q].  gs_mode 0: plain; 1: first Gram-Schmidt pass (norm and the
   // re-orthogonalisation test); 2: second pass, predicated on F_REORTH (norm when it runs).
-  auto maxpy = [&](double* o, const double* x0, double* const* xs, int nq, double sgn, int c_off, int gs_mode) {
-    MaxpyArgs ma{};
+  auto maxpy = [&](auto* o, const auto* x0, auto* const* xs, int nq, double sgn, int c_off, int gs_mode) {
+    using O = std::remove_cv_t<std::remove_pointer_t<decltype(o)>>;
+    using V = std::remove_cv_t<std::remove_pointer_t<std::remove_cv_t<std::remove_pointer_t<decltype(xs)>>>>;
+    MaxpyArgs<O, V> ma{};
     ma.out = o; ma.x0 = x0; ma.nq = nq; ma.c_off = c_off; ma.sgn = sgn; ma.n = n3;
     for (int q = 0; q < nq; ++q) ma.x[q] = xs[q];
     unsigned nb = nblk(n3);
     if (nb > (unsigned)RED_MAXB) nb = RED_MAXB;
     double* pa = gs_mode ? c->partials : (double*)nullptr;
-    if (c->launch_log)
-      std::fprintf(c->launch_log, "k_maxpy %.0f %d\n", (nq + 2.0) * sizeof(double) * n3, gs_mode == 2 ? 0 : 1);
+    const double ab = (2.0 * sizeof(O) + nq * sizeof(V)) * n3;
+    const char* dev = sizeof(O) == 4 ? "k_maxpy<float, float>"
+                      : (sizeof(V) == 4 ? "k_maxpy<double, float>" : "k_maxpy<double, double>");
+    const char* tname = sizeof(O) == 4 ? "maxpy_f32" : (sizeof(V) == 4 ? "maxpy_update" : "maxpy_f64");
+    if (c->launch_log) std::fprintf(c->launch_log, "%s %.0f %d\n", dev, ab, gs_mode == 2 ? 0 : 1);
     if (gs_mode == 2)   // predicated: not timed (its bytes depend on a device-side flag)
-      LAUNCH(k_maxpy, dim3(nb), dim3(256), c->s, ma, (const double*)c->sc, pa, (const int*)c->fl, (int)F_REORTH);
+      LAUNCH((k_maxpy<O, V>), dim3(nb), dim3(256), c->s, ma, (const double*)c->sc, pa, (const int*)c->fl,
+             (int)F_REORTH);
     else
-      TLAUNCH(c, "maxpy", (nq + 2.0) * sizeof(double) * n3, k_maxpy, dim3(nb), dim3(256), c->s, ma,
-              (const double*)c->sc, pa, (const int*)c->fl, -1);
+      TLAUNCH(c, tname, ab, (k_maxpy<O, V>), dim3(nb), dim3(256), c->s, ma, (const double*)c->sc, pa,
+              (const int*)c->fl, -1);
     if (gs_mode)
       LAUNCH(k_fold_gs_norm, dim3(1), dim3(NT), c->s, (int)nb, gs_mode, c->g.vol, (const double*)c->partials,
              c->sc, c->fl);
   };
   // <W, V_i>, i = 0..k -> S_HBUF + off + i (and vol <W, W> -> S_DOT1); pass 2 predicated on F_REORTH
-  auto dots_into = [&](const double* W, int k, int off, bool second) {
-    MdotArgs md{};
+  auto dots_into = [&](const auto* W, auto* const* basis, int k, int off, bool second) {
+    using V = std::remove_cv_t<std::remove_pointer_t<decltype(W)>>;
+    MdotArgs<V> md{};
     md.w = W; md.nv = k + 1; md.n = n3;
-    for (int q = 0; q <= k; ++q) md.v[q] = c->kv[q];
+    for (int q = 0; q <= k; ++q) md.v[q] = basis[q];
     const unsigned nch = (unsigned)((k + 1 + MD_CH - 1) / MD_CH);
     const int flag = second ? (int)F_REORTH : -1;
-    if (c->launch_log) std::fprintf(c->launch_log, "k_mdot %.0f %d\n", (k + 2.0) * sizeof(double) * n3, second ? 0 : 1);
+    const double ab = (k + 2.0) * sizeof(V) * n3;
+    if (c->launch_log)
+      std::fprintf(c->launch_log, "%s %.0f %d\n", sizeof(V) == 4 ? "k_mdot<float>" : "k_mdot<double>", ab,
+                   second ? 0 : 1);
     if (second) {   // predicated: not timed
-      LAUNCH(k_mdot, dim3(MD_BX, nch), dim3(256), c->s, md, c->partials, (const int*)c->fl, flag);
+      LAUNCH((k_mdot<V>), dim3(MD_BX, nch), dim3(256), c->s, md, c->partials, (const int*)c->fl, flag);
     } else {
-      TLAUNCH(c, "mdot", (k + 2.0) * sizeof(double) * n3, k_mdot, dim3(MD_BX, nch), dim3(256), c->s, md,
-              c->partials, (const int*)c->fl, flag);
+      TLAUNCH(c, sizeof(V) == 4 ? "mdot_f32" : "mdot_f64", ab, (k_mdot<V>), dim3(MD_BX, nch), dim3(256), c->s,
+              md, c->partials, (const int*)c->fl, flag);
     }
     LAUNCH(k_mdot_fold, dim3(1), dim3(1024), c->s, k + 1, off, c->g.vol, (const double*)c->partials, c->sc,
            (const int*)c->fl, flag);
@@ -877,58 +968,76 @@ void solve_fgmres(fugu_ctx* c, Work<T>& w, double* X, double dt, double tol, dou
     if (c->debug) std::fprintf(stderr, "[fugu] fgmres restart: true res %.3e after %d steps\n", rr, steps);
     if (rr <= tol || beta == 0.0 || steps >= max_steps) break;
     LAUNCH(k_fgmres_init, dim3(1), dim3(1), c->s, c->sc, c->fg, c->fl);
-    lin<double>(c, L_SCALE, c->kv[0], c->kv[0], nullptr, nullptr, nullptr, S_INVH, -1, -1, 1.0, false);
+    if (ir32)   // v_0 = r / beta rounded to FP32
+      LAUNCH((k_convert<float, double>), dim3(nblk(n3)), dim3(256), c->s, c->kv32[0], (const double*)c->kv[0], n3,
+             c->sc, (int)S_INVH, 1.0, 0);
+    else
+      lin<double>(c, L_SCALE, c->kv[0], c->kv[0], nullptr, nullptr, nullptr, S_INVH, -1, -1, 1.0, false);
     for (int k = 0; k < m && steps < max_steps; ++k) {
       // preconditioner z_k ~ A^-1 v_k in precision T
       int it = 1;
-      if (max_inner > 0) {
-        LAUNCH((k_convert<T, double>), dim3(nblk(n3)), dim3(256), c->s, w.B, (const double*)c->kv[k], n3, c->sc,
-               -1, 1.0, 0);
-        HIPCHK(hipMemsetAsync(w.X, 0, n3 * sizeof(T), c->s));
-        double ires = 0.0;
-        bicgstab2<T>(c, w, dt, inner_tol, max_inner, mvflags, false, 0.0, true, &it, &ires, true);
-        LAUNCH((k_convert<double, T>), dim3(nblk(n3)), dim3(256), c->s, c->kz[k], (const T*)w.X, n3, c->sc, -1,
-               1.0, 0);
-        reset_flags(c);
-      } else if (c->adjoint) {
-        pcd_T(c, (const double*)c->kv[k], c->kz[k], dt, -1.0, c->mg_cycles, c->pcd_visc);
-      } else if (sizeof(T) == sizeof(double)) {
-        pcd(c, w, (const T*)c->kv[k], (T*)c->kz[k], (T)dt);
+      if (ir32) {
+        float* z = c->kz32[k];
+        pcd<float>(c, c->wf, (const float*)c->kv32[k], z, (float)dt);
+        bcs_trip(c, z);
+        // W = A z_k (FP32 operands), then classical Gram-Schmidt as below on FP32 storage
+        HIPCHK(hipMemcpyAsync(c->wf.X, z, n3 * sizeof(float), hipMemcpyDeviceToDevice, c->s));
+        matvec(c, c->wf, c->wf.X, c->wf.z, dt, mvflags, nullptr);
+        float* W = c->wf.z;
+        dots_into(W, c->kv32.data(), k, 0, false);
+        maxpy(W, W, c->kv32.data(), k + 1, -1.0, 0, 1);
+        dots_into(W, c->kv32.data(), k, 64, true);
+        maxpy(W, W, c->kv32.data(), k + 1, -1.0, 64, 2);
+        LAUNCH(k_fgmres_step, dim3(1), dim3(1), c->s, k, 0.5 * tol * bn, c->sc, c->fg, c->fl);
+        lin<float>(c, L_SCALE, c->kv32[k + 1], W, nullptr, nullptr, nullptr, S_INVH, -1, -1, 1.0, false);
       } else {
-        LAUNCH((k_convert<T, double>), dim3(nblk(n3)), dim3(256), c->s, w.B, (const double*)c->kv[k], n3, c->sc,
-               -1, 1.0, 0);
-        pcd(c, w, (const T*)w.B, w.X, (T)dt);
-        LAUNCH((k_convert<double, T>), dim3(nblk(n3)), dim3(256), c->s, c->kz[k], (const T*)w.X, n3, c->sc, -1,
-               1.0, 0);
+        if (max_inner > 0) {
+          LAUNCH((k_convert<T, double>), dim3(nblk(n3)), dim3(256), c->s, w.B, (const double*)c->kv[k], n3, c->sc,
+                 -1, 1.0, 0);
+          HIPCHK(hipMemsetAsync(w.X, 0, n3 * sizeof(T), c->s));
+          double ires = 0.0;
+          bicgstab2<T>(c, w, dt, inner_tol, max_inner, mvflags, false, 0.0, true, &it, &ires, true);
+          LAUNCH((k_convert<double, T>), dim3(nblk(n3)), dim3(256), c->s, c->kz[k], (const T*)w.X, n3, c->sc, -1,
+                 1.0, 0);
+          reset_flags(c);
+        } else if (c->adjoint) {
+          pcd_T(c, (const double*)c->kv[k], c->kz[k], dt, -1.0, c->mg_cycles, c->pcd_visc);
+        } else if (sizeof(T) == sizeof(double)) {
+          pcd(c, w, (const T*)c->kv[k], (T*)c->kz[k], (T)dt);
+        }
+        bcs_trip(c, c->kz[k]);
+        // W = A z_k, then classical Gram-Schmidt twice, all coefficients device-resident
+        HIPCHK(hipMemcpyAsync(wd.X, c->kz[k], n3 * sizeof(double), hipMemcpyDeviceToDevice, c->s));
+        if (c->adjoint) matvec_T(c, wd.X, wd.z, dt, (mvflags & MV_ADV) ? 1 : 0);
+        else matvec(c, wd, wd.X, wd.z, dt, mvflags, nullptr);
+        double* W = wd.z;
+        // classical Gram-Schmidt, second pass only when the first lost more than half of ||W||^2
+        dots_into(W, c->kv.data(), k, 0, false);
+        maxpy(W, W, c->kv.data(), k + 1, -1.0, 0, 1);
+        dots_into(W, c->kv.data(), k, 64, true);
+        maxpy(W, W, c->kv.data(), k + 1, -1.0, 64, 2);
+        LAUNCH(k_fgmres_step, dim3(1), dim3(1), c->s, k, 0.5 * tol * bn, c->sc, c->fg, c->fl);
+        lin<double>(c, L_SCALE, c->kv[k + 1], W, nullptr, nullptr, nullptr, S_INVH, -1, -1, 1.0, false);
       }
       total_inner += it;
-      bcs_trip(c, c->kz[k]);
-      // W = A z_k, then classical Gram-Schmidt twice, all coefficients device-resident
-      HIPCHK(hipMemcpyAsync(wd.X, c->kz[k], n3 * sizeof(double), hipMemcpyDeviceToDevice, c->s));
-      if (c->adjoint) matvec_T(c, wd.X, wd.z, dt, (mvflags & MV_ADV) ? 1 : 0);
-      else matvec(c, wd, wd.X, wd.z, dt, mvflags, nullptr);
-      double* W = wd.z;
-      // classical Gram-Schmidt, second pass only when the first lost more than half of ||W||^2
-      dots_into(W, k, 0, false);
-      maxpy(W, W, c->kv.data(), k + 1, -1.0, 0, 1);
-      dots_into(W, k, 64, true);
-      maxpy(W, W, c->kv.data(), k + 1, -1.0, 64, 2);
-      LAUNCH(k_fgmres_step, dim3(1), dim3(1), c->s, k, 0.5 * tol * bn, c->sc, c->fg, c->fl);
-      lin<double>(c, L_SCALE, c->kv[k + 1], W, nullptr, nullptr, nullptr, S_INVH, -1, -1, 1.0, false);
       ++steps;
       // convergence is read one step behind, so the device always has the next step queued
-      HIPCHK(hipMemcpyAsync(c->h_fl + F_FG_DONE, c->fl + F_FG_DONE, sizeof(int), hipMemcpyDeviceToHost, c->s));
+      HIPCHK(hipMemcpyAsync(c->h_fl + F_COUNT + (k & 1), c->fl + F_FG_DONE, sizeof(int), hipMemcpyDeviceToHost,
+                            c->s));
       HIPCHK(hipEventRecord(c->fg_ev[k & 1], c->s));
       if (k >= 1) {
         HIPCHK(hipEventSynchronize(c->fg_ev[(k - 1) & 1]));
-        if (c->h_fl[F_FG_DONE]) break;
+        if (c->h_fl[F_COUNT + ((k - 1) & 1)]) break;
       }
     }
     LAUNCH(k_fgmres_solve, dim3(1), dim3(1), c->s, c->sc, (const double*)c->fg);
     read_slots(c);
     const int kk = (int)c->h_sc[S_HBUF + 127];
     if (c->debug) std::fprintf(stderr, "[fugu] fgmres cycle: %d directions\n", kk);
-    if (kk > 0) maxpy(X, X, c->kz.data(), kk, 1.0, 128, 0);
+    if (kk > 0) {
+      if (ir32) maxpy(X, (const double*)X, c->kz32.data(), kk, 1.0, 128, 0);
+      else maxpy(X, (const double*)X, c->kz.data(), kk, 1.0, 128, 0);
+    }
     bcs_trip(c, X);
   }
   if (has_lid) bcs_trip(c, X, lid);
@@ -1062,6 +1171,7 @@ int step_impl(fugu_ctx* c, double dt, int step_idx, const fugu_step_opts* o, fug
   double res = 0.0;
   c->pcg_calls = 0;
   c->use_mg = (o->solver_flags & FUGU_PRECOND_MG) != 0;
+  c->use_vmg = (o->solver_flags & FUGU_PRECOND_VMG) != 0;
   c->pcd_visc = (o->solver_flags & FUGU_PRECOND_VISC) ? 1.0 : 0.0;
   c->mg_cycles = o->mg_cycles > 0 ? o->mg_cycles : 1;
   const double itol = o->inner_tol > 0 ? o->inner_tol : 1.0e-3;
@@ -1095,7 +1205,7 @@ int step_impl(fugu_ctx* c, double dt, int step_idx, const fugu_step_opts* o, fug
   HIPCHK(hipEventRecord(c->ev[3], c->s));
   sync(c);
   kt_resolve(c);
-  c->use_mg = false;
+  c->use_mg = false; c->use_vmg = false;
   c->pcd_visc = 0.0;
   if (st) {
     float ms_solve = 0.f, ms_step = 0.f;
@@ -1159,6 +1269,7 @@ int adjoint_run(fugu_ctx* c, const fugu_adjoint_opts* o, const double* mu_in, co
          c->rho_u, c->rho_v, c->kap, c->zeta, c->eta);
   bind_props_d(c, false, false);
   c->use_mg = true;
+  c->use_vmg = (o->flags & FUGU_ADJ_PRECOND_VMG) != 0;
   c->pcd_visc = 1.0;
   c->mg_cycles = o->mg_cycles > 0 ? o->mg_cycles : 1;
   // initial state (BCs and halos as the end of a step leaves them)
@@ -1314,7 +1425,7 @@ int adjoint_run(fugu_ctx* c, const fugu_adjoint_opts* o, const double* mu_in, co
   HIPCHK(hipEventRecord(c->ev[2], c->s));
   sync(c);
   kt_resolve(c);
-  c->use_mg = false;
+  c->use_mg = false; c->use_vmg = false;
   c->pcd_visc = 0.0;
   if (g_sigma) *g_sigma = gs;
   if (st) {
@@ -1391,7 +1502,9 @@ int fugu_ctx_create(const fugu_grid* grid, const fugu_params* prm, fugu_ctx** ou
     c->sc = dalloc<double>(c, S_COUNT);
     c->fl = (int*)dalloc<double>(c, F_COUNT);
     HIPCHK(hipHostMalloc((void**)&c->h_sc, S_COUNT * sizeof(double), hipHostMallocDefault));
-    HIPCHK(hipHostMalloc((void**)&c->h_fl, F_COUNT * sizeof(int), hipHostMallocDefault));
+    // two landing slots past the flags for the FGMRES convergence flag (read_flags copies
+    // only the first F_COUNT, so an inner solve cannot overwrite them)
+    HIPCHK(hipHostMalloc((void**)&c->h_fl, (F_COUNT + 2) * sizeof(int), hipHostMallocDefault));
     c->X = dalloc<double>(c, c->ntrip);
     c->u = c->X; c->v = c->X + g.nu; c->p = c->X + g.nu + g.nv;
     c->c = dalloc<double>(c, g.np); c->cnew = dalloc<double>(c, g.np);
@@ -2005,10 +2118,10 @@ int fugu_adjoint_solve(fugu_ctx* c, double* Lu, double* Lv, double* Lp, const do
       solve_fgmres<double>(c, w, L, dt, tol > 0 ? tol : 1.0e-10, 0.0, max_iter > 0 ? max_iter : 400, 0, 30, MV_ADV,
                            false, 0.0, &it, &outer, &res);
     } catch (...) {
-      c->adjoint = false; c->use_mg = false; c->pcd_visc = 0.0;
+      c->adjoint = false; c->use_mg = false; c->use_vmg = false; c->pcd_visc = 0.0;
       throw;
     }
-    c->adjoint = false; c->use_mg = false; c->pcd_visc = 0.0;
+    c->adjoint = false; c->use_mg = false; c->use_vmg = false; c->pcd_visc = 0.0;
     mask_trip(c, L);
     out(c, Lu, L, g.nu); out(c, Lv, L + g.nu, g.nv); out(c, Lp, L + g.nu + g.nv, g.np);
     sync(c);
@@ -2044,7 +2157,7 @@ int fugu_adjoint_gradient(fugu_ctx* c, const fugu_adjoint_opts* o, const double*
     try {
       return adjoint_run(c, o, mu_cell, rho_cell, sigma, c0, u0, v0, g_mu, g_rho, g_sigma, st);
     } catch (...) {
-      c->adjoint = false; c->use_mg = false; c->pcd_visc = 0.0;
+      c->adjoint = false; c->use_mg = false; c->use_vmg = false; c->pcd_visc = 0.0;
       throw;
     }
   });

@@ -532,19 +532,23 @@ __global__ void __launch_bounds__(256) k_lin(LinArgs<T> a, const double* __restr
   }
 }
 
-// Multi-vector update o = x0 + sgn * sum_q c_q x_q (coefficients sc[S_HBUF + c_off + q]),
-// all operands FP64 triplets; o may alias x0.  One pass reads every operand once.
+// Multi-vector update o = x0 + sgn * sum_q c_q x_q (coefficients sc[S_HBUF + c_off + q]);
+// o, x0 stored as O, the x_q as V (FP64 or FP32 triplets), arithmetic in FP64; o may alias
+// x0.  One pass reads every operand once.
 constexpr int MAXPY_MAX = 48;
+template <typename O, typename V>
 struct MaxpyArgs {
-  double* out;
-  const double* x0;
-  const double* x[MAXPY_MAX];
+  O* out;
+  const O* x0;
+  const V* x[MAXPY_MAX];
   int nq, c_off;
   double sgn;
   size_t n;
 };
-// With `partials` set, also publishes per-workgroup sums of out^2 (grid capped at RED_MAXB).
-__global__ void __launch_bounds__(256) k_maxpy(MaxpyArgs a, const double* __restrict__ sc,
+// With `partials` set, also publishes per-workgroup sums of out^2 (of the stored values; grid
+// capped at RED_MAXB).
+template <typename O, typename V>
+__global__ void __launch_bounds__(256) k_maxpy(MaxpyArgs<O, V> a, const double* __restrict__ sc,
                                                double* __restrict__ partials, const int* __restrict__ fl, int flag) {
   if (flag >= 0 && !fl[flag]) return;   // predicated pass (second Gram-Schmidt pass)
   __shared__ double c[MAXPY_MAX];
@@ -553,11 +557,12 @@ __global__ void __launch_bounds__(256) k_maxpy(MaxpyArgs a, const double* __rest
   double nrm[1] = {0.0};
   const size_t stride = (size_t)gridDim.x * blockDim.x;
   for (size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x; k < a.n; k += stride) {
-    double o = a.x0[k];
+    double o = (double)a.x0[k];
 #pragma unroll 4
-    for (int q = 0; q < a.nq; ++q) o += c[q] * a.x[q][k];
-    a.out[k] = o;
-    nrm[0] += o * o;
+    for (int q = 0; q < a.nq; ++q) o += c[q] * (double)a.x[q][k];
+    const O os = (O)o;
+    a.out[k] = os;
+    nrm[0] += (double)os * (double)os;
   }
   if (partials) {
     __shared__ double lds[16][1];
@@ -566,26 +571,28 @@ __global__ void __launch_bounds__(256) k_maxpy(MaxpyArgs a, const double* __rest
   }
 }
 
-// Multi-dot of the outer FGMRES Gram-Schmidt: <w, v_q> for q < nv in one launch.
-// blockIdx.y selects a chunk of MD_CH basis vectors, blockIdx.x one of MD_BX slices of the
-// flat arrays; plain FP64 sums in a fixed order.  The triplets the outer iteration builds
+// Multi-dot of the FGMRES Gram-Schmidt: <w, v_q> for q < nv in one launch (FP64 or FP32
+// storage, FP64 products and sums).  blockIdx.y selects a chunk of MD_CH basis vectors,
+// blockIdx.x one of MD_BX slices of the flat arrays; plain FP64 sums in a fixed order.  The triplets the outer iteration builds
 // (residuals B - A X and A z) are exactly zero outside the coupled_dot owned sets (the
 // matvec zeroes ghosts and the wall-normal faces carry the zero BC value), so the flat
 // product times vol is the coupled_dot inner product.
 constexpr int MD_CH = 8;
 constexpr int MD_BX = 256;
+template <typename V>
 struct MdotArgs {
-  const double* w;
-  const double* v[MAXPY_MAX];
+  const V* w;
+  const V* v[MAXPY_MAX];
   int nv;
   size_t n;
 };
 // Chunk 0 also sums <w, w> (partial slot MD_CH).  flag >= 0: run only when fl[flag] is set.
-__global__ void __launch_bounds__(256) k_mdot(MdotArgs a, double* __restrict__ partials, const int* __restrict__ fl,
+template <typename V>
+__global__ void __launch_bounds__(256) k_mdot(MdotArgs<V> a, double* __restrict__ partials, const int* __restrict__ fl,
                                               int flag) {
   if (flag >= 0 && !fl[flag]) return;
   const int c0 = (int)blockIdx.y * MD_CH;
-  const double* vp[MD_CH];
+  const V* vp[MD_CH];
 #pragma unroll
   for (int q = 0; q < MD_CH; ++q) vp[q] = (c0 + q < a.nv) ? a.v[c0 + q] : a.w;   // padding lanes reread w
   double acc[MD_CH + 1];
@@ -593,10 +600,10 @@ __global__ void __launch_bounds__(256) k_mdot(MdotArgs a, double* __restrict__ p
   for (int q = 0; q <= MD_CH; ++q) acc[q] = 0.0;
   const size_t stride = (size_t)MD_BX * blockDim.x;
   for (size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x; k < a.n; k += stride) {
-    const double wk = a.w[k];
+    const double wk = (double)a.w[k];
     double x[MD_CH];
 #pragma unroll
-    for (int q = 0; q < MD_CH; ++q) x[q] = vp[q][k];
+    for (int q = 0; q < MD_CH; ++q) x[q] = (double)vp[q][k];
 #pragma unroll
     for (int q = 0; q < MD_CH; ++q) acc[q] += wk * x[q];
     acc[MD_CH] += wk * wk;

@@ -74,7 +74,10 @@ enum {
    * operator) instead of the reference's Jacobi-PCG to 1e-5 / 150 iterations. */
   FUGU_PRECOND_MG = 1u << 3,
   /* Schur approximation of the PCD also carries the viscous term: r_p -= kappa * z_p. */
-  FUGU_PRECOND_VISC = 1u << 4
+  FUGU_PRECOND_VISC = 1u << 4,
+  /* Velocity blocks of the PCD preconditioner by one geometric-multigrid V-cycle each (the
+   * same Helmholtz operators) instead of the reference's three damped-Jacobi sweeps. */
+  FUGU_PRECOND_VMG = 1u << 5
 };
 
 enum { FUGU_W_MAX_ITER = 1, FUGU_W_NONFINITE = 2 };
@@ -271,7 +274,10 @@ enum {
   /* use the interface (unit-sigma CSF forces) recorded by the previous call with the same
    * horizon instead of advecting C: the frozen-interface map the gradient differentiates, as
    * finite differences of the gradient need it */
-  FUGU_ADJ_REUSE_INTERFACE = 1u << 1
+  FUGU_ADJ_REUSE_INTERFACE = 1u << 1,
+  /* velocity blocks of the forward and transposed preconditioners by multigrid V-cycles
+   * (FUGU_PRECOND_VMG) */
+  FUGU_ADJ_PRECOND_VMG = 1u << 2
 };
 typedef struct {
   double J;

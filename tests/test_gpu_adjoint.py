@@ -158,14 +158,15 @@ def gpu_opts(case, flags=0, tol=1e-12):
     return abi.FuguAdjointOpts(case.nsteps, case.dt, case.y0, tol, 400, 30, 1, flags)
 
 
-@pytest.mark.parametrize("per_x,time_order", [(False, 2), (True, 2), (False, 1)],
-                         ids=["walls_bdf2", "periodic_x_bdf2", "walls_bdf1"])
-def test_adjoint_gradient_against_cpu_reference(per_x, time_order):
+@pytest.mark.parametrize("per_x,time_order,flags", [(False, 2, 0), (True, 2, 0), (False, 1, 0),
+                                                    (False, 2, abi.ADJ_PRECOND_VMG), (True, 2, abi.ADJ_PRECOND_VMG)],
+                         ids=["walls_bdf2", "periodic_x_bdf2", "walls_bdf1", "walls_bdf2_vmg", "periodic_x_bdf2_vmg"])
+def test_adjoint_gradient_against_cpu_reference(per_x, time_order, flags):
     case = make_case(per_x, time_order)
     J, g_mu, g_rho, g_sigma, _ = AR.gradient(case)
     ctx = gpu_ctx(case)
     gm, gr = np.zeros_like(case.mu), np.zeros_like(case.rho)
-    Jg, gs, st, rc = ctx.adjoint_gradient(gpu_opts(case), case.mu, case.rho, case.sigma, case.c0, case.u0,
+    Jg, gs, st, rc = ctx.adjoint_gradient(gpu_opts(case, flags), case.mu, case.rho, case.sigma, case.c0, case.u0,
                                           case.v0, gm, gr)
     assert rc == 0, (st.max_forward_res, st.max_adjoint_res)
     assert abs(Jg - J) <= 1e-10 * abs(J), (Jg, J)

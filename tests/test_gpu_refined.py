@@ -1,8 +1,9 @@
 """GPU tests of the chosen fast solve forms (FUGU_SOLVE_REFINED, FP64 and mixed FP32/FP64,
-FUGU_PRECOND_MG, with an inner BiCGStab(2) or the PCD alone + FUGU_PRECOND_VISC as the
-flexible preconditioner): every step's system is solved to a true relative residual <= 1e-8, the
-residual being recomputed here with the oracle's matvec_coupled; the mixed and FP64 forms
-agree per field to the stated bound."""
+FUGU_PRECOND_MG, with an inner BiCGStab(2) or the PCD alone + FUGU_PRECOND_VISC (+ the
+multigrid velocity blocks, FUGU_PRECOND_VMG) as the flexible preconditioner): every step's
+system is solved to a true relative residual <= 1e-8, the residual being recomputed here with
+the oracle's matvec_coupled; both forms' per-field errors against a 1e-13 solve stay within
+the stated bounds."""
 import numpy as np
 import pytest
 
@@ -54,39 +55,52 @@ def oracle_true_residual(g, ph, st_before, u, v, p, dt, step_idx):
     return np.sqrt(O.coupled_dot(g, R, R) / O.coupled_dot(g, B, B))
 
 
-# (inner BiCGStab(2) cap, extra preconditioner flags, restart): inner Krylov inside FGMRES,
-# and the PCD alone as the flexible preconditioner with the viscous Schur correction
+# (inner BiCGStab(2) cap, extra preconditioner flags, restart, periodic in x): inner Krylov
+# inside FGMRES, the PCD alone as the flexible preconditioner with the viscous Schur correction,
+# and the same with multigrid velocity blocks (walls: odd u-block width; periodic x: even)
 VARIANTS = {
-    "inner_b2": (40, 0, 0),
-    "pcd_visc": (-1, abi.PRECOND_VISC, 40),
+    "inner_b2": (40, 0, 0, False),
+    "pcd_visc": (-1, abi.PRECOND_VISC, 40, False),
+    "pcd_visc_vmg": (-1, abi.PRECOND_VISC | abi.PRECOND_VMG, 40, False),
+    "pcd_visc_vmg_periodic_x": (-1, abi.PRECOND_VISC | abi.PRECOND_VMG, 40, True),
 }
 
 
 @pytest.mark.parametrize("variant", sorted(VARIANTS))
 def test_refined_forms_reach_1e8_and_agree(variant):
-    inner_max, extra, restart = VARIANTS[variant]
+    """Both forms reach the true residual 1e-8 (recomputed by the oracle) every step; their
+    forward errors against a 1e-13 solve of the same steps are of the size the 1e-8 residual
+    allows, and the mixed form's is within a small factor of the FP64 form's (the FP32
+    Arnoldi cycles do not cost accuracy at the stated residual)."""
+    inner_max, extra, restart, per_x = VARIANTS[variant]
     nx, ny = 64, 128
-    g = O.Grid(nx, ny, 1.0, 2.0)
+    g = O.Grid(nx, ny, 1.0, 2.0, per_x=per_x)
     ph = phys(rho1=1000.0, rho2=1.0, mu1=10.0, mu2=0.1, sigma=2.0, gy=-1.0, time_order=2)
     st0 = O.State(g)
     st0.c[...] = bench.bubble_c(nx, ny, 1.0, 2.0)
     st0.init_properties(ph)
     results = {}
     base = abi.SOLVE_REFINED | abi.PRECOND_MG | extra
-    for name, flags in (("fp64", base), ("mixed", base | abi.SOLVE_MIXED)):
-        ctx = abi.Context(nx, ny, 1.0, 2.0, params=abi.FuguParams(1000.0, 1.0, 10.0, 0.1, 2.0, 0.0, -1.0, 2, 1e12, 0.5))
+    for name, flags, tol in (("tight", base, 1e-13), ("fp64", base, 1e-8), ("mixed", base | abi.SOLVE_MIXED, 1e-8)):
+        ctx = abi.Context(nx, ny, 1.0, 2.0, per_x, False, params=abi.FuguParams(1000.0, 1.0, 10.0, 0.1, 2.0, 0.0, -1.0, 2, 1e12, 0.5))
         st = st0.copy()
-        opts = abi.step_opts(tol=1e-8, max_iter=200, solver_flags=flags, inner_tol=1e-3, inner_max_iter=inner_max,
+        opts = abi.step_opts(tol=tol, max_iter=400, solver_flags=flags, inner_tol=1e-3, inner_max_iter=inner_max,
                              restart=restart)
+        fields = []
         for step in (1, 2, 3):
             before = st.copy()
             it, res, rc = ctx.advance_step(st, 2e-4, step, opts)
-            assert rc == 0 and res <= 1e-8, (name, step, res)
-            tr = oracle_true_residual(g, ph, before, st.u, st.v, st.p, 2e-4, step)
-            assert tr <= 2e-8, (name, step, tr)
+            assert rc == 0 and res <= tol, (name, step, res)
+            if name != "tight":
+                tr = oracle_true_residual(g, ph, before, st.u, st.v, st.p, 2e-4, step)
+                assert tr <= 2e-8, (name, step, tr)
+            fields.append([st.u.copy(), st.v.copy(), st.p.copy()])
         ctx.close()
-        results[name] = st
-    a, b = results["mixed"], results["fp64"]
-    for f in ("u", "v"):
-        d = np.abs(getattr(a, f) - getattr(b, f)).max() / np.abs(getattr(b, f)).max()
-        assert d < 1e-5, (f, d)
+        results[name] = fields
+    for step in range(3):
+        ref = results["tight"][step]
+        err = {name: [np.abs(a - b).max() / np.abs(b).max() for a, b in zip(results[name][step], ref)]
+               for name in ("fp64", "mixed")}
+        for k, f in enumerate("uvp"):
+            assert err["fp64"][k] < 1e-3 and err["mixed"][k] < 1e-3, (step, f, err)
+            assert err["mixed"][k] <= 5.0 * err["fp64"][k] + 1e-7, (step, f, err)
