@@ -61,6 +61,8 @@ template <typename T>
 struct MgLevel {
   MgDims d;
   T *cx = nullptr, *cy = nullptr, *m = nullptr, *dinv = nullptr, *b = nullptr, *phi = nullptr;
+  T* cinv = nullptr;   // coarsest level of at most MG_DENSE_MAX cells: its dense inverse
+  T* phi2 = nullptr;   // the fused post-smoothing's output (swapped with phi after the launch)
 };
 
 // A multigrid hierarchy from the finest extents down (mass: allocate the mass terms).
@@ -131,6 +133,8 @@ struct fugu_ctx {
   bool use_mg = false;   // pressure block by multigrid V-cycles instead of the reference PCG
   bool use_vmg = false;  // velocity blocks by multigrid V-cycles instead of three Jacobi sweeps
   int mg_cycles = 1;
+  int mg_tail_cells = 4096;  // multigrid levels of at most this many cells run as one launch (0: never)
+  bool mg_fused = true;      // fused (temporal-blocking) level kernels instead of one launch per half-sweep
   double pcd_visc = 0.0; // viscous term of the Schur approximation (0 = the reference PCD)
   // adjoint: solve_fgmres applies A_eff^T and the transposed PCD instead of A and the PCD
   bool adjoint = false;
@@ -428,8 +432,11 @@ std::vector<MgLevel<T>> mg_levels(fugu_ctx* c, MgDims d, bool mass) {
     L.dinv = dalloc<T>(c, n);
     L.b = dalloc<T>(c, n);
     L.phi = dalloc<T>(c, n);
+    L.phi2 = dalloc<T>(c, n);
+    const bool last = !mg_can_coarsen(d);
+    if (last && n <= (size_t)MG_DENSE_MAX) L.cinv = dalloc<T>(c, n * n);
     lv.push_back(L);
-    if (!mg_can_coarsen(d)) break;
+    if (last) break;
     d = mg_coarse_dims(d);
   }
   return lv;
@@ -449,6 +456,10 @@ void mg_galerkin(fugu_ctx* c, std::vector<MgLevel<T>>& lv) {
   for (auto& L : lv)
     LAUNCH((k_mg_dinv<T>), g2(L.d.nx, L.d.ny), bl, c->s, L.d, (const T*)L.cx, (const T*)L.cy, (const T*)L.m,
            L.dinv);
+  const MgLevel<T>& C = lv.back();
+  if (C.cinv)   // singular (pressure) block: no mass term
+    LAUNCH((k_mg_coarse_inv<T>), dim3(1), dim3(1024), c->s, C.d, (const T*)C.cx, (const T*)C.cy, (const T*)C.m,
+           C.m ? 0 : 1, C.cinv);
 }
 
 template <typename T>
@@ -482,16 +493,39 @@ void mg_smooth(fugu_ctx* c, MgLevel<T>& L, int colour, int zero_other) {
           c->s, L.d, colour, (const T*)L.cx, (const T*)L.cy, (const T*)L.dinv, (const T*)L.b, L.phi, zero_other);
 }
 
+// First level of the coarse tail: the first of at most mg_tail_cells cells (no more than
+// MG_TAIL_LEVELS from the bottom), when the coarsest level fits the LDS solve; lv.size(): none.
+template <typename T>
+size_t mg_tail_start(const fugu_ctx* c, const std::vector<MgLevel<T>>& lv) {
+  const size_t nl = lv.size();
+  const MgDims& dc = lv[nl - 1].d;
+  if (c->mg_tail_cells <= 0 || (size_t)dc.nx * dc.ny > (size_t)MG_COARSE_MAX) return nl;
+  size_t ts = 0;
+  while (ts < nl && (size_t)lv[ts].d.nx * lv[ts].d.ny > (size_t)c->mg_tail_cells) ++ts;
+  if (ts + MG_TAIL_LEVELS < nl) ts = nl - MG_TAIL_LEVELS;
+  return ts;
+}
+
 // V-cycle on level l; `zero_guess`: phi of level l is (treated as) zero on entry; `project`:
 // the operator is singular (no mass term, no Dirichlet face) and the coarsest rhs is centred.
 template <typename T>
 void mg_vcycle(fugu_ctx* c, std::vector<MgLevel<T>>& lv, size_t l, bool zero_guess, int project) {
   MgLevel<T>& L = lv[l];
   const int nu = 2;
+  const size_t nl = lv.size();
+  if (l == mg_tail_start(c, lv)) {   // the coarse tail in one launch
+    MgTailArgs<T> ta{};
+    ta.nlev = (int)(nl - l);
+    for (size_t q = l; q < nl; ++q)
+      ta.L[q - l] = MgTailLevel<T>{lv[q].d, lv[q].cx, lv[q].cy, lv[q].m, lv[q].dinv, lv[q].cinv, lv[q].b, lv[q].phi};
+    ta.nu = nu; ta.coarse_sweeps = 60; ta.project = project; ta.zero_guess = zero_guess ? 1 : 0;
+    LAUNCH((k_mg_tail<T>), dim3(1), dim3(1024), c->s, ta);
+    return;
+  }
   if (l + 1 == lv.size()) {
     if ((size_t)L.d.nx * L.d.ny <= (size_t)MG_COARSE_MAX) {
       LAUNCH((k_mg_coarse<T>), dim3(1), dim3(MG_COARSE_MAX), c->s, L.d, (const T*)L.cx, (const T*)L.cy,
-             (const T*)L.dinv, (const T*)L.b, L.phi, 60, project);
+             (const T*)L.dinv, (const T*)L.cinv, (const T*)L.b, L.phi, 60, project);
     } else {
       for (int s = 0; s < 40; ++s) {
         mg_smooth(c, L, 0, (s == 0 && zero_guess) ? 1 : 0);
@@ -500,20 +534,37 @@ void mg_vcycle(fugu_ctx* c, std::vector<MgLevel<T>>& lv, size_t l, bool zero_gue
     }
     return;
   }
-  for (int s = 0; s < nu; ++s) {
-    mg_smooth(c, L, 0, (s == 0 && zero_guess) ? 1 : 0);
-    mg_smooth(c, L, 1, 0);
-  }
   MgLevel<T>& C = lv[l + 1];
-  // fine c_x, c_y, (m,) b, phi read; coarse b, phi written (a quarter of the cells each)
-  TLAUNCH(c, sizeof(T) == 8 ? "mg_resid_f64" : "mg_resid_f32", (L.m ? 5.5 : 4.5) * sizeof(T) * L.d.nx * L.d.ny,
-          (k_mg_restrict<T>), g2(C.d.nx, C.d.ny), dim3(64, 4), c->s, L.d, (const T*)L.cx, (const T*)L.cy,
-          (const T*)L.m, (const T*)L.b, (const T*)L.phi, C.b, C.phi);
+  const double ncell = (double)L.d.nx * L.d.ny;
+  const dim3 gt((unsigned)((L.d.nx + MGT_W - 1) / MGT_W), (unsigned)((L.d.ny + MGT_H - 1) / MGT_H));
+  if (c->mg_fused && zero_guess) {
+    // b, D^-1, c_x, c_y (, m) read; phi written; coarse b and phi written (a quarter each)
+    TLAUNCH(c, sizeof(T) == 8 ? "mg_pre_f64" : "mg_pre_f32", (L.m ? 6.5 : 5.5) * sizeof(T) * ncell,
+            (k_mg_pre<T, 2>), gt, dim3(256), c->s, L.d, (const T*)L.cx, (const T*)L.cy, (const T*)L.m,
+            (const T*)L.dinv, (const T*)L.b, L.phi, C.b, C.phi);
+  } else {
+    for (int s = 0; s < nu; ++s) {
+      mg_smooth(c, L, 0, (s == 0 && zero_guess) ? 1 : 0);
+      mg_smooth(c, L, 1, 0);
+    }
+    // fine c_x, c_y, (m,) b, phi read; coarse b, phi written (a quarter of the cells each)
+    TLAUNCH(c, sizeof(T) == 8 ? "mg_resid_f64" : "mg_resid_f32", (L.m ? 5.5 : 4.5) * sizeof(T) * ncell,
+            (k_mg_restrict<T>), g2(C.d.nx, C.d.ny), dim3(64, 4), c->s, L.d, (const T*)L.cx, (const T*)L.cy,
+            (const T*)L.m, (const T*)L.b, (const T*)L.phi, C.b, C.phi);
+  }
   mg_vcycle(c, lv, l + 1, true, project);
-  LAUNCH((k_mg_prolong<T>), g2(L.d.nx, L.d.ny), dim3(64, 4), c->s, L.d, (const T*)C.phi, L.phi);
-  for (int s = 0; s < nu; ++s) {
-    mg_smooth(c, L, 1, 0);
-    mg_smooth(c, L, 0, 0);
+  if (c->mg_fused) {
+    // phi, coarse phi (a quarter), b, D^-1, c_x, c_y read; phi written
+    TLAUNCH(c, sizeof(T) == 8 ? "mg_post_f64" : "mg_post_f32", 6.25 * sizeof(T) * ncell, (k_mg_post<T, 2>), gt,
+            dim3(256), c->s, L.d, (const T*)L.cx, (const T*)L.cy, (const T*)L.dinv, (const T*)L.b,
+            (const T*)L.phi, (const T*)C.phi, L.phi2);
+    std::swap(L.phi, L.phi2);
+  } else {
+    LAUNCH((k_mg_prolong<T>), g2(L.d.nx, L.d.ny), dim3(64, 4), c->s, L.d, (const T*)C.phi, L.phi);
+    for (int s = 0; s < nu; ++s) {
+      mg_smooth(c, L, 1, 0);
+      mg_smooth(c, L, 0, 0);
+    }
   }
 }
 
@@ -1491,6 +1542,8 @@ int fugu_ctx_create(const fugu_grid* grid, const fugu_params* prm, fugu_ctx** ou
                      grid->periodic_x ? 1 : 0, grid->periodic_y ? 1 : 0);
     c->prm = *prm;
     if (const char* lp = std::getenv("FUGU_LAUNCH_LOG")) c->launch_log = std::fopen(lp, "a");
+    if (const char* tc = std::getenv("FUGU_MG_TAIL_CELLS")) c->mg_tail_cells = std::atoi(tc);
+    if (const char* fu = std::getenv("FUGU_MG_FUSED")) c->mg_fused = std::atoi(fu) != 0;
     HIPCHK(hipStreamCreateWithFlags(&c->s, hipStreamNonBlocking));
     for (auto& e : c->ev) HIPCHK(hipEventCreate(&e));
     for (auto& e : c->fg_ev) HIPCHK(hipEventCreateWithFlags(&e, hipEventDisableTiming));

@@ -46,14 +46,17 @@ __global__ void k_mg_coef0(Grid g, const T* __restrict__ rho_u, const T* __restr
   const int fi = blockIdx.x * blockDim.x + threadIdx.x;
   const int fj = blockIdx.y * blockDim.y + threadIdx.y;
   const int nx = g.nx, ny = g.ny;
+  // a periodic direction has one value per wrapped face: face n is face 0
   if (fi <= nx && fj < ny) {
     double c = 0.0;
-    if (g.per_x || (fi > 0 && fi < nx)) c = g.inv_dx2 / (double)rho_u[IU(g, fi, fj + 1)];
+    const int ff = (g.per_x && fi == nx) ? 0 : fi;
+    if (g.per_x || (fi > 0 && fi < nx)) c = g.inv_dx2 / (double)rho_u[IU(g, ff, fj + 1)];
     cx[fi + (size_t)fj * (nx + 1)] = (T)c;
   }
   if (fi < nx && fj <= ny) {
     double c = 0.0;
-    if (g.per_y || (fj > 0 && fj < ny)) c = g.inv_dy2 / (double)rho_v[IV(g, fi + 1, fj)];
+    const int ff = (g.per_y && fj == ny) ? 0 : fj;
+    if (g.per_y || (fj > 0 && fj < ny)) c = g.inv_dy2 / (double)rho_v[IV(g, fi + 1, ff)];
     cy[fi + (size_t)fj * nx] = (T)c;
   }
 }
@@ -171,16 +174,17 @@ __global__ void __launch_bounds__(256) k_mg_prolong(MgDims f, const T* __restric
   phi[o] = phi[o] + phic[(i >> 1) + (size_t)(j >> 1) * ((f.nx + 1) / 2)];
 }
 
-// Coarsest level: one workgroup, LDS-resident, optional zero-mean projection of the rhs
-// (singular Neumann operator), then `sweeps` red-black sweeps from zero.
+// Coarsest level: one workgroup.  The rhs is centred first when the operator is singular
+// (project: the Neumann/periodic pressure block).  With at most MG_DENSE_MAX cells the level is
+// solved exactly by its dense inverse (k_mg_coarse_inv, once per set-up; for the singular
+// block the inverse of A + s 11^T / n, which maps a centred rhs to the zero-mean solution),
+// otherwise by `sweeps` LDS-resident red-black sweeps from zero.
 constexpr int MG_COARSE_MAX = 1024;
+constexpr int MG_DENSE_MAX = 64;
+
 template <typename T>
-__global__ void __launch_bounds__(1024) k_mg_coarse(MgDims d, const T* __restrict__ cx, const T* __restrict__ cy,
-                                                    const T* __restrict__ dinv, const T* __restrict__ b,
-                                                    T* __restrict__ phi, int sweeps, int project) {
-  __shared__ T sphi[MG_COARSE_MAX];
-  __shared__ T sb[MG_COARSE_MAX];
-  __shared__ double red[16];
+__device__ inline void mg_coarse_solve(const MgDims& d, const T* cx, const T* cy, const T* dinv, const T* cinv,
+                                       const T* b, T* phi, int sweeps, int project, T* sphi, T* sb, double* red) {
   const int n = d.nx * d.ny;
   const int t = threadIdx.x;
   double my = 0.0;
@@ -195,6 +199,15 @@ __global__ void __launch_bounds__(1024) k_mg_coarse(MgDims d, const T* __restric
     if (t < n) sb[t] = sb[t] - (T)m;
   }
   __syncthreads();
+  if (cinv) {
+    if (t < n) {
+      double s = 0.0;
+      for (int q = 0; q < n; ++q) s += (double)cinv[(size_t)t * n + q] * (double)sb[q];
+      phi[t] = (T)s;
+    }
+    __syncthreads();
+    return;
+  }
   const int i = (t < n) ? t % d.nx : 0, j = (t < n) ? t / d.nx : 0;
   for (int s = 0; s < sweeps; ++s) {
     for (int colour = 0; colour < 2; ++colour) {
@@ -203,6 +216,338 @@ __global__ void __launch_bounds__(1024) k_mg_coarse(MgDims d, const T* __restric
     }
   }
   if (t < n) phi[t] = sphi[t];
+  __syncthreads();
+}
+
+template <typename T>
+__global__ void __launch_bounds__(1024) k_mg_coarse(MgDims d, const T* __restrict__ cx, const T* __restrict__ cy,
+                                                    const T* __restrict__ dinv, const T* __restrict__ cinv,
+                                                    const T* __restrict__ b, T* __restrict__ phi, int sweeps,
+                                                    int project) {
+  __shared__ T sphi[MG_COARSE_MAX];
+  __shared__ T sb[MG_COARSE_MAX];
+  __shared__ double red[16];
+  mg_coarse_solve(d, cx, cy, dinv, cinv, b, phi, sweeps, project, sphi, sb, red);
+}
+
+// Dense inverse of the coarsest operator (n <= MG_DENSE_MAX cells), by Gauss-Jordan
+// elimination in FP64 without pivoting (the operator, with the rank-one term when singular, is
+// symmetric positive definite).  One workgroup; inv is n x n row-major.
+template <typename T>
+__global__ void __launch_bounds__(1024) k_mg_coarse_inv(MgDims d, const T* __restrict__ cx, const T* __restrict__ cy,
+                                                        const T* __restrict__ m, int project, T* __restrict__ inv) {
+  __shared__ double A[MG_DENSE_MAX][2 * MG_DENSE_MAX];
+  __shared__ double fcol[MG_DENSE_MAX];
+  __shared__ double sdiag;
+  const int n = d.nx * d.ny, w = 2 * n, t = threadIdx.x;
+  for (int q = t; q < n * w; q += blockDim.x) A[q / w][q % w] = ((q % w) == n + q / w) ? 1.0 : 0.0;
+  __syncthreads();
+  if (t < n) {
+    const int i = t % d.nx, j = t / d.nx;
+    const size_t rx = (size_t)(d.nx + 1);
+    const double cW = (double)cx[i + j * rx], cE = (double)cx[i + 1 + j * rx];
+    const double cS = (double)cy[i + (size_t)j * d.nx], cN = (double)cy[i + (size_t)(j + 1) * d.nx];
+    A[t][t] = cW + cE + cS + cN + (m ? (double)m[t] : 0.0);
+    // neighbours through interior or periodic faces (boundary faces enter the diagonal only)
+    if (i > 0 || d.per_x) A[t][mg_wrap(i - 1, d.nx, d.per_x) + j * d.nx] -= cW;
+    if (i < d.nx - 1 || d.per_x) A[t][mg_wrap(i + 1, d.nx, d.per_x) + j * d.nx] -= cE;
+    if (j > 0 || d.per_y) A[t][i + mg_wrap(j - 1, d.ny, d.per_y) * d.nx] -= cS;
+    if (j < d.ny - 1 || d.per_y) A[t][i + mg_wrap(j + 1, d.ny, d.per_y) * d.nx] -= cN;
+  }
+  __syncthreads();
+  if (project) {
+    if (t == 0) {
+      double s = 0.0;
+      for (int q = 0; q < n; ++q) s += A[q][q];
+      sdiag = s / ((double)n * (double)n);
+    }
+    __syncthreads();
+    for (int q = t; q < n * n; q += blockDim.x) A[q / n][q % n] += sdiag;
+    __syncthreads();
+  }
+  for (int k = 0; k < n; ++k) {
+    const double piv = A[k][k];
+    if (t < n) fcol[t] = A[t][k];   // multipliers, read before column k is eliminated
+    __syncthreads();
+    for (int q = t; q < w; q += blockDim.x) A[k][q] /= piv;
+    __syncthreads();
+    for (int q = t; q < n * w; q += blockDim.x) {
+      const int r = q / w, col = q % w;
+      if (r != k) A[r][col] -= fcol[r] * A[k][col];
+    }
+    __syncthreads();
+  }
+  for (int q = t; q < n * n; q += blockDim.x) inv[q] = (T)A[q / n][n + q % n];
+}
+
+// ---------------------------------------------------------------------------------
+// Fused level kernels (temporal blocking): one launch does a level's NU pre-smoothing sweeps
+// and the residual restriction, another the prolongation and NU post-smoothing sweeps, instead
+// of 2 NU + 1 launches each.  A 256-thread workgroup owns a MGT_W x MGT_H tile plus a halo of
+// H cells; each thread owns fixed cells of that extended tile (slot k: cell tid + 256 k, row
+// major) and loads their b, D^-1 and four face conductances into registers once, in one round
+// of independent loads; phi of the extended tile lives in LDS.  Half-sweep h updates its colour
+// on the tile grown by H - 1 - h rings, so every value the owner needs is final when read, and
+// the owner writes its tile only.  Tiles are numbered so that each XCD gets a contiguous band
+// of them (its L2 then holds the shared halos).  No launch reads what it writes: the
+// pre-smoothing starts from zero (phi is not read) and the post-smoothing writes a second
+// array, which the caller swaps in.  A cell's colour is that of its wrapped global index and
+// each update performs the one-launch-per-half-sweep smoother's arithmetic with the same
+// operands, so the result is bit-identical to it.
+constexpr int MGT_W = 64, MGT_H = 16;
+
+__device__ inline int mgt_wrap(int g, int n, int per) {
+  if (g >= 0 && g < n) return g;
+  if (!per) return -1;
+  return ((g % n) + n) % n;
+}
+
+// Tile of this workgroup: dispatch order places workgroup w on XCD w mod 8; renumber so that
+// XCD x owns a contiguous range of tiles.
+__device__ inline void mgt_tile(int& i0, int& j0) {
+  const int W = gridDim.x * gridDim.y;
+  const int w = blockIdx.x + blockIdx.y * gridDim.x;
+  const int per = W / 8, rem = W % 8, x = w % 8, k = w / 8;
+  const int t = (x < rem) ? x * (per + 1) + k : rem * (per + 1) + (x - rem) * per + k;
+  i0 = (t % gridDim.x) * MGT_W;
+  j0 = (t / gridDim.x) * MGT_H;
+}
+
+template <typename T, int H>
+struct MgtCells {
+  static constexpr int EW = MGT_W + 2 * H, EH = MGT_H + 2 * H, NS = (EW * EH + 255) / 256;
+  T b[NS], dinv[NS], cW[NS], cE[NS], cS[NS], cN[NS];
+  int li[NS], lj[NS], ring[NS];   // ring: distance outside the tile (0 inside); -1: no cell
+  int col[NS];                    // colour of the wrapped global index
+};
+
+// Load the thread's cells (registers) and phi (+ the parent's coarse correction) into LDS.
+template <typename T, int H>
+__device__ inline void mgt_init(const MgDims& f, int i0, int j0, const T* cx, const T* cy, const T* dinv,
+                                const T* b, const T* phi, const T* phic, MgtCells<T, H>& m,
+                                T (*s)[MgtCells<T, H>::EW]) {
+  using M = MgtCells<T, H>;
+  const int tid = threadIdx.x;
+  const int nxc = (f.nx + 1) / 2;
+  const size_t rx = (size_t)(f.nx + 1);
+  T pv[M::NS];
+#pragma unroll
+  for (int k = 0; k < M::NS; ++k) {
+    const int idx = tid + 256 * k;
+    const int lj = idx / M::EW, li = idx - lj * M::EW;
+    m.li[k] = li; m.lj[k] = lj;
+    const int i = mgt_wrap(i0 - H + li, f.nx, f.per_x), j = mgt_wrap(j0 - H + lj, f.ny, f.per_y);
+    int dist = 0;
+    dist = max(dist, H - li);
+    dist = max(dist, li - (H + MGT_W - 1));
+    dist = max(dist, H - lj);
+    dist = max(dist, lj - (H + MGT_H - 1));
+    const bool cell = idx < M::EW * M::EH && i >= 0 && j >= 0;
+    m.ring[k] = cell ? dist : -1;
+    m.col[k] = (i + j) & 1;
+    T bv = T(0), dv = T(0), w = T(0), e = T(0), so = T(0), no = T(0), p = T(0);
+    if (cell) {
+      const size_t o = i + (size_t)j * f.nx;
+      bv = b[o];
+      dv = dinv[o];
+      w = (i > 0 || f.per_x) ? cx[i + j * rx] : T(0);
+      e = (i < f.nx - 1 || f.per_x) ? cx[i + 1 + j * rx] : T(0);
+      so = (j > 0 || f.per_y) ? cy[o] : T(0);
+      no = (j < f.ny - 1 || f.per_y) ? cy[o + f.nx] : T(0);
+      if (phi) {
+        p = phi[o];
+        if (phic) p = p + phic[(i >> 1) + (size_t)(j >> 1) * nxc];
+      }
+    }
+    m.b[k] = bv; m.dinv[k] = dv; m.cW[k] = w; m.cE[k] = e; m.cS[k] = so; m.cN[k] = no;
+    pv[k] = p;
+  }
+#pragma unroll
+  for (int k = 0; k < M::NS; ++k)
+    if (tid + 256 * k < M::EW * M::EH) s[m.lj[k]][m.li[k]] = pv[k];
+  __syncthreads();
+}
+
+// One half-sweep of colour `col` over the tile grown by r rings.
+template <typename T, int H>
+__device__ inline void mgt_half(int r, int col, int zero_other, const MgtCells<T, H>& m, T (*s)[MgtCells<T, H>::EW]) {
+  using M = MgtCells<T, H>;
+#pragma unroll
+  for (int k = 0; k < M::NS; ++k) {
+    if (m.ring[k] < 0 || m.ring[k] > r || m.col[k] != col) continue;
+    const int li = m.li[k], lj = m.lj[k];
+    T sum = T(0);
+    if (!zero_other) sum = m.cW[k] * s[lj][li - 1] + m.cE[k] * s[lj][li + 1] + m.cS[k] * s[lj - 1][li] + m.cN[k] * s[lj + 1][li];
+    s[lj][li] = (m.b[k] + sum) * m.dinv[k];
+  }
+  __syncthreads();
+}
+
+template <typename T, int H>
+__device__ inline void mgt_store(const MgDims& f, int i0, int j0, T* out, const T (*s)[MgtCells<T, H>::EW]) {
+  const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
+  const int gi = i0 + tx;
+  if (gi >= f.nx) return;
+#pragma unroll
+  for (int qj = ty; qj < MGT_H; qj += 4) {
+    const int gj = j0 + qj;
+    if (gj < f.ny) out[gi + (size_t)gj * f.nx] = s[H + qj][H + tx];
+  }
+}
+
+// NU pre-smoothing sweeps from zero (red first), then the residual summed onto the coarse
+// cells of the tile (coarse phi zeroed).
+template <typename T, int NU>
+__global__ void __launch_bounds__(256) k_mg_pre(MgDims f, const T* __restrict__ cx, const T* __restrict__ cy,
+                                                const T* __restrict__ m, const T* __restrict__ dinv,
+                                                const T* __restrict__ b, T* __restrict__ phi, T* __restrict__ bc,
+                                                T* __restrict__ phic) {
+  constexpr int H = 2 * NU + 1;
+  using M = MgtCells<T, H>;
+  __shared__ T s[M::EH][M::EW];
+  int i0, j0;
+  mgt_tile(i0, j0);
+  M cells;
+  mgt_init<T, H>(f, i0, j0, cx, cy, dinv, b, (const T*)nullptr, (const T*)nullptr, cells, s);
+#pragma unroll
+  for (int h = 0; h < 2 * NU; ++h) mgt_half<T, H>(H - 1 - h, h & 1, h == 0 ? 1 : 0, cells, s);
+  mgt_store<T, H>(f, i0, j0, phi, s);
+  const int nxc = (f.nx + 1) / 2, nyc = (f.ny + 1) / 2;
+  const size_t rx = (size_t)(f.nx + 1);
+  const int q = threadIdx.x;   // one coarse cell of the 32 x 8 coarse tile
+  const int I = i0 / 2 + (q & 31), J = j0 / 2 + (q >> 5);
+  if (I >= nxc || J >= nyc) return;
+  T acc = T(0);
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    const int i = 2 * I + (k & 1), j = 2 * J + (k >> 1);
+    if (i >= f.nx || j >= f.ny) continue;
+    const int li = H + i - i0, lj = H + j - j0;
+    const size_t o = i + (size_t)j * f.nx;
+    T diag = cx[i + j * rx] + cx[i + 1 + j * rx] + cy[o] + cy[o + f.nx];
+    if (m) diag += m[o];
+    const T cW = (i > 0 || f.per_x) ? cx[i + j * rx] : T(0);
+    const T cE = (i < f.nx - 1 || f.per_x) ? cx[i + 1 + j * rx] : T(0);
+    const T cS = (j > 0 || f.per_y) ? cy[o] : T(0);
+    const T cN = (j < f.ny - 1 || f.per_y) ? cy[o + f.nx] : T(0);
+    const T nb = cW * s[lj][li - 1] + cE * s[lj][li + 1] + cS * s[lj - 1][li] + cN * s[lj + 1][li];
+    acc += b[o] - (diag * s[lj][li] - nb);
+  }
+  bc[I + (size_t)J * nxc] = acc;
+  phic[I + (size_t)J * nxc] = T(0);
+}
+
+// phi_out = phi + phi_c(parent), then NU post-smoothing sweeps (black first).
+template <typename T, int NU>
+__global__ void __launch_bounds__(256) k_mg_post(MgDims f, const T* __restrict__ cx, const T* __restrict__ cy,
+                                                 const T* __restrict__ dinv, const T* __restrict__ b,
+                                                 const T* __restrict__ phi, const T* __restrict__ phic,
+                                                 T* __restrict__ phi_out) {
+  constexpr int H = 2 * NU;
+  using M = MgtCells<T, H>;
+  __shared__ T s[M::EH][M::EW];
+  int i0, j0;
+  mgt_tile(i0, j0);
+  M cells;
+  mgt_init<T, H>(f, i0, j0, cx, cy, dinv, b, phi, phic, cells, s);
+#pragma unroll
+  for (int h = 0; h < 2 * NU; ++h) mgt_half<T, H>(H - 1 - h, (h & 1) ? 0 : 1, 0, cells, s);
+  mgt_store<T, H>(f, i0, j0, phi_out, s);
+}
+
+// ---------------------------------------------------------------------------------
+// Coarse tail: the V-cycle over the coarsest levels (the first one holding at most a few
+// thousand cells and everything below it) in ONE workgroup and one launch, with a barrier
+// between phases instead of a launch per half-sweep.  Phase for phase it performs the
+// multi-launch cycle's arithmetic (same smoother update, residual sum, prolongation and
+// coarsest solve), so its result is bit-identical; the coarse levels' arrays stay in L2.
+constexpr int MG_TAIL_LEVELS = 16;
+template <typename T>
+struct MgTailLevel {
+  MgDims d;
+  const T *cx, *cy, *m, *dinv, *cinv;
+  T *b, *phi;
+};
+template <typename T>
+struct MgTailArgs {
+  MgTailLevel<T> L[MG_TAIL_LEVELS];
+  int nlev, nu, coarse_sweeps, project, zero_guess;
+};
+
+template <typename T>
+__device__ inline void tail_smooth(const MgTailLevel<T>& L, int colour, int zero_other) {
+  const MgDims d = L.d;
+  const int half = (d.nx + 1) / 2;
+  const int tot = half * d.ny;
+  for (int q = (int)threadIdx.x; q < tot; q += (int)blockDim.x) {
+    const int j = q / half, t = q - j * half;
+    const int i = 2 * t + ((j + colour) & 1);
+    if (i >= d.nx) continue;
+    const size_t o = i + (size_t)j * d.nx;
+    const T s = zero_other ? T(0) : mg_nbr_sum(d, L.cx, L.cy, (const T*)L.phi, i, j);
+    L.phi[o] = (L.b[o] + s) * L.dinv[o];
+  }
+  __syncthreads();
+}
+
+template <typename T>
+__global__ void __launch_bounds__(1024) k_mg_tail(MgTailArgs<T> a) {
+  __shared__ T sphi[MG_COARSE_MAX];
+  __shared__ T sb[MG_COARSE_MAX];
+  __shared__ double red[16];
+  const int nu = a.nu;
+  // descent: pre-smoothing and residual restriction
+  for (int l = 0; l + 1 < a.nlev; ++l) {
+    const MgTailLevel<T>& L = a.L[l];
+    const int zg = (l == 0) ? a.zero_guess : 1;
+    for (int s = 0; s < nu; ++s) {
+      tail_smooth(L, 0, (s == 0 && zg) ? 1 : 0);
+      tail_smooth(L, 1, 0);
+    }
+    const MgDims f = L.d;
+    const int nxc = (f.nx + 1) / 2, nyc = (f.ny + 1) / 2;
+    T* bc = a.L[l + 1].b;
+    T* phic = a.L[l + 1].phi;
+    for (int q = (int)threadIdx.x; q < nxc * nyc; q += (int)blockDim.x) {
+      const int J = q / nxc, I = q - J * nxc;
+      T acc = T(0);
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        const int i = 2 * I + (k & 1), j = 2 * J + (k >> 1);
+        if (i >= f.nx || j >= f.ny) continue;
+        const size_t o = i + (size_t)j * f.nx;
+        const size_t rx = (size_t)(f.nx + 1);
+        T diag = L.cx[i + j * rx] + L.cx[i + 1 + j * rx] + L.cy[o] + L.cy[o + f.nx];
+        if (L.m) diag += L.m[o];
+        acc += L.b[o] - (diag * L.phi[o] - mg_nbr_sum(f, L.cx, L.cy, (const T*)L.phi, i, j));
+      }
+      bc[I + (size_t)J * nxc] = acc;
+      phic[I + (size_t)J * nxc] = T(0);
+    }
+    __syncthreads();
+  }
+  // coarsest level
+  {
+    const MgTailLevel<T>& L = a.L[a.nlev - 1];
+    mg_coarse_solve(L.d, L.cx, L.cy, L.dinv, L.cinv, (const T*)L.b, L.phi, a.coarse_sweeps, a.project, sphi, sb, red);
+  }
+  // ascent: prolongation and post-smoothing
+  for (int l = a.nlev - 2; l >= 0; --l) {
+    const MgTailLevel<T>& L = a.L[l];
+    const MgDims f = L.d;
+    const T* phic = a.L[l + 1].phi;
+    const int nxc = (f.nx + 1) / 2;
+    for (int q = (int)threadIdx.x; q < f.nx * f.ny; q += (int)blockDim.x) {
+      const int j = q / f.nx, i = q - j * f.nx;
+      const size_t o = i + (size_t)j * f.nx;
+      L.phi[o] = L.phi[o] + phic[(i >> 1) + (size_t)(j >> 1) * nxc];
+    }
+    __syncthreads();
+    for (int s = 0; s < nu; ++s) {
+      tail_smooth(L, 1, 0);
+      tail_smooth(L, 0, 0);
+    }
+  }
 }
 
 // Level-0 load: b0 = z_p (- mean when both directions are walls), compact layout.
@@ -251,16 +596,19 @@ __global__ void k_vmg_coef0(Grid g, MgDims d, int comp, const T* __restrict__ rh
   const int fi = blockIdx.x * blockDim.x + threadIdx.x;
   const int fj = blockIdx.y * blockDim.y + threadIdx.y;
   const double ix2 = g.inv_dx2, iy2 = g.inv_dy2;
+  // a periodic direction has one value per wrapped face: face n is face 0
   if (fi <= d.nx && fj < d.ny) {   // x face fi (west face of node fi)
+    const int ff = (d.per_x && fi == d.nx) ? 0 : fi;
     const double c = comp == 0
-        ? (double)kap[IP(g, fi + 1, fj + 1)] * ix2                                        // u(fi) | u(fi+1)
-        : ((double)zeta[IW(g, fi, fj + 1)] - (double)eta[IW(g, fi, fj + 1)]) * ix2;      // v(fi) | v(fi+1)
+        ? (double)kap[IP(g, ff + 1, fj + 1)] * ix2                                        // u(fi) | u(fi+1)
+        : ((double)zeta[IW(g, ff, fj + 1)] - (double)eta[IW(g, ff, fj + 1)]) * ix2;      // v(fi) | v(fi+1)
     cx[fi + (size_t)fj * (d.nx + 1)] = (T)c;
   }
   if (fi < d.nx && fj <= d.ny) {   // y face fj (south face of node row fj)
+    const int ff = (d.per_y && fj == d.ny) ? 0 : fj;
     const double c = comp == 0
-        ? ((double)zeta[IW(g, fi + 1, fj)] - (double)eta[IW(g, fi + 1, fj)]) * iy2       // u(., fj) | u(., fj+1)
-        : (double)kap[IP(g, fi + 1, fj + 1)] * iy2;                                       // v(., fj) | v(., fj+1)
+        ? ((double)zeta[IW(g, fi + 1, ff)] - (double)eta[IW(g, fi + 1, ff)]) * iy2       // u(., fj) | u(., fj+1)
+        : (double)kap[IP(g, fi + 1, ff + 1)] * iy2;                                       // v(., fj) | v(., fj+1)
     cy[fi + (size_t)fj * d.nx] = (T)c;
   }
   if (fi < d.nx && fj < d.ny) {

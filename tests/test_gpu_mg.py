@@ -50,3 +50,63 @@ def test_mg_vcycle_is_linear():
         out.append(phi)
     lin = 3.0 * out[0] - 2.0 * out[1]
     assert np.abs(out[2] - lin).max() <= 1e-12 * np.abs(lin).max()
+
+
+@pytest.mark.parametrize("nx,ny,per_x,per_y", [(32, 24, False, False), (48, 40, True, False), (40, 64, False, True),
+                                               (96, 80, True, True), (130, 70, False, False)])
+def test_coarse_tail_is_bitwise_the_multilaunch_cycle(monkeypatch, nx, ny, per_x, per_y):
+    """The single-launch coarse tail and the fused (temporal-blocking) level kernels perform the
+    one-launch-per-half-sweep V-cycle's arithmetic: results equal bit for bit with both off, at
+    the default threshold and with the whole hierarchy in the tail (odd extents included)."""
+    g = O.Grid(nx, ny, 1.0, ny / nx, per_x=per_x, per_y=per_y)
+    rng = np.random.default_rng(202)
+    c, pr, _, _ = two_phase_case(g, rng, rho=(100.0, 1.0))
+    O.halo(g, pr["rho_u"], "u"); O.halo(g, pr["rho_v"], "v")
+    rhs = rand(g, "p", rng)
+    out = {}
+    for cells, fused in (("0", "0"), ("0", "1"), ("4096", "1"), ("1000000", "1"), ("4096", "0")):
+        monkeypatch.setenv("FUGU_MG_TAIL_CELLS", cells)
+        monkeypatch.setenv("FUGU_MG_FUSED", fused)
+        ctx = abi.Context(g.nx, g.ny, g.lx, g.ly, per_x=per_x, per_y=per_y)
+        res = []
+        for n in (1, 3):
+            phi = g.zeros("p")
+            ctx.pressure_mg(phi, rhs, pr["rho_u"], pr["rho_v"], n_cycles=n)
+            res.append(phi)
+        ctx.close()
+        out[(cells, fused)] = res
+    for key in out:
+        for a, b in zip(out[key], out[("0", "0")]):
+            assert np.array_equal(a, b), key
+
+
+@pytest.mark.parametrize("mixed", [False, True])
+def test_coarse_tail_bitwise_in_the_step_solve(monkeypatch, mixed):
+    """A refined step solve (pressure and velocity hierarchies: mass terms, Dirichlet faces, FP64
+    or FP32) is bit-identical with the coarse tail and fused level kernels on and off."""
+    import bench
+    nx, ny = 96, 192
+    outs = []
+    for cells, fused in (("0", "0"), ("4096", "1")):
+        monkeypatch.setenv("FUGU_MG_TAIL_CELLS", cells)
+        monkeypatch.setenv("FUGU_MG_FUSED", fused)
+        p = abi.FuguParams(1000.0, 1.0, 10.0, 0.1, 2.0, 0.0, -1.0, 2, 1e12, 0.5)
+        ctx = abi.Context(nx, ny, 1.0, 2.0, False, False, params=p)
+        nh = 2
+        u = np.zeros((ny + 2 * nh, nx + 1 + 2 * nh)); v = np.zeros((ny + 1 + 2 * nh, nx + 2 * nh))
+        pp = np.zeros((ny + 2 * nh, nx + 2 * nh))
+        ctx.upload(u, v, u.copy(), v.copy(), pp, bench.bubble_c(nx, ny, 1.0, 2.0))
+        ctx.init_properties()
+        flags = abi.SOLVE_REFINED | abi.PRECOND_MG | abi.PRECOND_VISC | abi.PRECOND_VMG | (abi.SOLVE_MIXED if mixed else 0)
+        opts = abi.step_opts(tol=1e-8, max_iter=200, solver_flags=flags, inner_max_iter=-1, restart=30)
+        its = []
+        for s in (1, 2):
+            st, rc = ctx.step(2e-4, s, opts)
+            assert rc == 0 and st.res <= 1e-8
+            its.append(st.outer)
+        f = [ctx.field(k, np.zeros(shape)) for k, shape in (("u", u.shape), ("v", v.shape), ("p", pp.shape))]
+        ctx.close()
+        outs.append((its, f))
+    assert outs[0][0] == outs[1][0]
+    for a, b in zip(outs[0][1], outs[1][1]):
+        assert np.array_equal(a, b)
