@@ -1,16 +1,25 @@
 #!/usr/bin/env python3
-"""Coupled solves/s to a 1e-8 true relative residual, FP64 vs mixed FP32/FP64, on one MI355X.
+"""Coupled solves/s to a 1e-8 true relative residual, FP64 vs mixed FP32/FP64, on one MI355X;
+the same at 2048 x 2048; adjoint seconds per gradient.
 
-Workload: the rising-bubble case (domain 1x2, 512x1024 cells, bubble r = 0.2 at (0.5, 0.6)
-with a seeded 1 % radial perturbation over 8 Fourier modes, rho 1000/1, mu 10/0.1, sigma 2,
-g = -1, BDF2, dt = 2e-4).  One bench step = one full advance_coupled_step (VOF, mixture,
-SVOF/CSF, RHS and one coupled (u, v, p) solve) on HBM-resident state through the C ABI.
+Headline workload (`value`): the rising-bubble case (domain 1x2, 512x1024 cells, bubble r = 0.2
+at (0.5, 0.6) with a seeded 1 % radial perturbation over 8 Fourier modes, rho 1000/1, mu
+10/0.1, sigma 2, g = -1, BDF2, dt = 2e-4).  One bench step = one full advance_coupled_step
+(VOF, mixture, SVOF/CSF, RHS and one coupled (u, v, p) solve) on HBM-resident state through
+the C ABI.  `value` is the mixed form's solves/s; the FP64 form is reported beside it.
 
-Both forms solve every step's system to ||B - A X|| / ||B|| <= 1e-8 (coupled_dot norm,
-FP64) with restarted flexible GMRES in FP64 on the true residual.  The preconditioner is
-the block PCD (geometric-multigrid pressure block, viscous Schur correction, Jacobi
-velocity blocks): in FP64 for the FP64 form, in FP32 (FP64 arithmetic in the matvec) for
-the mixed form.  `value` is the mixed form's solves/s; the FP64 form is reported beside it.
+Both forms solve every step's system to ||B - A X|| / ||B|| <= 1e-8 (coupled_dot norm, FP64)
+with restarted flexible GMRES(30) whose every restart recomputes the true residual and updates
+X in FP64.  The preconditioner is the block PCD with geometric-multigrid pressure and velocity
+blocks and the viscous Schur correction.  FP64 form: everything FP64.  Mixed form: the Arnoldi
+cycles (basis, directions, matvec operands, preconditioner) in FP32 storage, FP64 products,
+sums and Hessenberg.
+
+"m1": the same two forms on 2048 x 2048 (unit square, 256 seeded inclusions r ~ U[0.005,
+0.02], carrier rho 1000 / mu 1e-2, inclusions rho 1 / mu 10, sigma 1, g -1, BDF2 dt 5e-5).
+"adjoint": seconds per gradient of J (kinetic energy in the band y >= 0.9 after 16 BDF2 steps,
+dt 5e-5) with respect to per-cell mu, per-cell rho and sigma on 1024 x 1024 with 64 seeded
+inclusions (carrier rho 100 / mu 1e-2, inclusions rho 1 / mu 1e-2 * 10^U[0,3], sigma 0.5).
 
 Prints one JSON line.  --gpus N > 1 exits: this build measures one GPU.
 """
@@ -30,12 +39,18 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
 
-# rising bubble (512 x 1024)
+# rising bubble (512 x 1024): the headline workload
 NX, NY, LX, LY = 512, 1024, 1.0, 2.0
 RHO1, RHO2, MU1, MU2, SIGMA, GY = 1000.0, 1.0, 10.0, 0.1, 2.0, -1.0
 DT = 2.0e-4
 TOL = 1.0e-8
 SEED = 3
+
+
+class Workload:
+    def __init__(self, name, nx, ny, lx, ly, phys, dt, cfield):
+        self.name, self.nx, self.ny, self.lx, self.ly = name, nx, ny, lx, ly
+        self.phys, self.dt, self.cfield = phys, dt, cfield   # phys: rho1, rho2, mu1, mu2, sigma, gy
 
 
 def bubble_c(nx, ny, lx, ly, nh=2, r0=0.2, xc=0.5, yc=0.6, seed=SEED):
@@ -44,36 +59,59 @@ def bubble_c(nx, ny, lx, ly, nh=2, r0=0.2, xc=0.5, yc=0.6, seed=SEED):
     return workloads.rising_bubble_c(nx, ny, lx, ly, nh, r0, xc, yc, seed)
 
 
-def make_ctx(abi):
-    p = abi.FuguParams(RHO1, RHO2, MU1, MU2, SIGMA, 0.0, GY, 2, 1.0e12, 0.5)
-    ctx = abi.Context(NX, NY, LX, LY, False, False, params=p)
+def inclusions_c(n, count, rmin, rmax, seed):
+    from coupled_codefort_2d_amd import workloads
+    return workloads.inclusions(n, count, rmin, rmax, seed, 1.0, 1.0, 1.0, 1.0)[0]
+
+
+BUBBLE = Workload("rising bubble 512x1024 (1x2 domain), rho 1000/1, mu 10/0.1, sigma 2, g -1, BDF2 dt 2e-4",
+                  NX, NY, LX, LY, (RHO1, RHO2, MU1, MU2, SIGMA, GY), DT, lambda: bubble_c(NX, NY, LX, LY))
+INCL2048 = Workload("2048x2048 unit square, 256 seeded inclusions r~U[0.005,0.02] (seed 5), carrier rho 1000 "
+                    "mu 1e-2, inclusions rho 1 mu 10, sigma 1, g -1, BDF2 dt 5e-5",
+                    2048, 2048, 1.0, 1.0, (1000.0, 1.0, 1e-2, 10.0, 1.0, -1.0), 5e-5,
+                    lambda: inclusions_c(2048, 256, 0.005, 0.02, 5))
+
+
+def make_ctx(abi, wl=None):
+    wl = wl or BUBBLE
+    r1, r2, m1, m2, sg, gy = wl.phys
+    p = abi.FuguParams(r1, r2, m1, m2, sg, 0.0, gy, 2, 1.0e12, 0.5)
+    ctx = abi.Context(wl.nx, wl.ny, wl.lx, wl.ly, False, False, params=p)
     nh = 2
-    u = np.zeros((NY + 2 * nh, NX + 1 + 2 * nh))
-    v = np.zeros((NY + 1 + 2 * nh, NX + 2 * nh))
-    pp = np.zeros((NY + 2 * nh, NX + 2 * nh))
-    c = bubble_c(NX, NY, LX, LY)
-    ctx.upload(u, v, u.copy(), v.copy(), pp, c)
+    u = np.zeros((wl.ny + 2 * nh, wl.nx + 1 + 2 * nh))
+    v = np.zeros((wl.ny + 1 + 2 * nh, wl.nx + 2 * nh))
+    pp = np.zeros((wl.ny + 2 * nh, wl.nx + 2 * nh))
+    ctx.upload(u, v, u.copy(), v.copy(), pp, wl.cfield())
     ctx.init_properties()
     return ctx
 
 
-def run_form(abi, flags, steps, warmup, max_outer, inner_tol, mg_cycles, inner_max, restart):
-    ctx = make_ctx(abi)
+def run_form(abi, flags, steps, warmup, max_outer, inner_tol, mg_cycles, inner_max, restart, wl=None,
+             instr_steps=None):
+    """Warm-up steps, then `steps` timed steps (the rate), then `instr_steps` more steps with
+    every named kernel bracketed by HIP events on the library's stream (per-kernel device time
+    and algorithmic bytes; kept out of the timed steps, whose rate the events would lower)."""
+    wl = wl or BUBBLE
+    instr_steps = steps if instr_steps is None else instr_steps
+    ctx = make_ctx(abi, wl)
     opts = abi.step_opts(use_bicgstab2=True, tol=TOL, max_iter=max_outer, solver_flags=flags,
                          inner_tol=inner_tol, inner_max_iter=inner_max, mg_cycles=mg_cycles, restart=restart)
     stats = []
     for s in range(1, warmup + 1):
-        st, _ = ctx.step(DT, s, opts)
+        st, _ = ctx.step(wl.dt, s, opts)
         print(f"# warmup step {s}: outer {st.outer} inner {st.iters} res {st.res:.2e} "
               f"solve {st.solve_seconds:.3f}s", file=sys.stderr, flush=True)
-    ctx.kernel_timing(True)
     t0 = time.perf_counter()
     for s in range(warmup + 1, warmup + steps + 1):
-        st, _ = ctx.step(DT, s, opts)
+        st, _ = ctx.step(wl.dt, s, opts)
         stats.append(st)
         print(f"# step {s}: outer {st.outer} inner {st.iters} res {st.res:.2e} solve {st.solve_seconds:.3f}s "
               f"step {st.step_seconds:.3f}s", file=sys.stderr, flush=True)
     wall = time.perf_counter() - t0
+    ctx.kernel_timing(True)
+    for s in range(warmup + steps + 1, warmup + steps + instr_steps + 1):
+        st, _ = ctx.step(wl.dt, s, opts)
+        print(f"# instrumented step {s}: outer {st.outer} solve {st.solve_seconds:.3f}s", file=sys.stderr, flush=True)
     ktimes = {}
     for name in KERNELS:
         ms, n, nbytes = ctx.kernel_time(name)
@@ -87,7 +125,8 @@ def run_form(abi, flags, steps, warmup, max_outer, inner_tol, mg_cycles, inner_m
 # launch's algorithmic bytes (unique fields read + written once)
 KERNELS = ("pcg_iter1_f64", "pcg_iter2_f64", "pcg_iter1_f32", "pcg_iter2_f32", "matvec_f64", "matvec_f32",
            "mg_smooth_f64", "mg_smooth_f32", "mg_resid_f64", "mg_resid_f32", "mdot_f64", "mdot_f32", "maxpy_f64",
-           "maxpy_f32", "maxpy_update")
+           "maxpy_f32", "maxpy_update", "maxpy_dot_f64", "maxpy_dot_f32", "mg_pre_f64", "mg_pre_f32", "mg_post_f64",
+           "mg_post_f32")
 DEVICE_NAMES = {"maxpy_f64": "k_maxpy<double, double>", "maxpy_f32": "k_maxpy<float, float>",
                 "maxpy_update": "k_maxpy<double, float>"}
 
@@ -99,7 +138,7 @@ def device_name(name):
     return f"k_{base}<{'double' if prec == 'f64' else 'float'}>"
 
 
-def roofline(ktimes):
+def roofline(ktimes, tag="rising_bubble_512x1024"):
     """Roofline object for the timed kernel with the largest total device time."""
     if not ktimes:
         return None
@@ -113,7 +152,7 @@ def roofline(ktimes):
     # measured HBM bytes / algorithmic bytes of this kernel on this workload (two rocprofv3
     # --pmc runs, FETCH_SIZE and WRITE_SIZE, scaled by a plain copy; hbm_traffic.py)
     safe = re.sub(r"[^A-Za-z0-9]+", "_", device_name(name)).strip("_")
-    prof = os.path.join(ROOT, "profiles", f"hbm_traffic_{safe}_rising_bubble_{NX}x{NY}.txt")
+    prof = os.path.join(ROOT, "profiles", f"hbm_traffic_{safe}_{tag}.txt")
     if os.path.exists(prof):
         for ln in open(prof):
             if ln.startswith("ratio:"):
@@ -173,6 +212,42 @@ def run_reference_form(abi, steps, warmup, max_iter):
     return dict(wall=wall, stats=stats, ktimes={})
 
 
+def timed_rate(ktimes):
+    """Algorithmic bytes / device time summed over every timed kernel (GB/s)."""
+    ms = sum(v[0] for v in ktimes.values())
+    nb = sum(v[2] for v in ktimes.values())
+    return round(nb / (ms * 1e-3) / 1e9, 1) if ms > 0 else None
+
+
+def run_adjoint(abi, n=1024, nsteps=16, dt=5e-5, count=64, seed=4):
+    """Seconds per gradient on the adjoint workload (forward horizon, recorded interface, reverse
+    sweep, property-path gradients), after a 2-step warm-up call on the same context."""
+    from coupled_codefort_2d_amd import workloads
+    c, mu, rho, _ = workloads.inclusions(n, count, 0.01, 0.04, seed, 100.0, 1e-2, 1.0, 1e3)
+    ctx = abi.Context(n, n, 1.0, 1.0, False, False,
+                      params=abi.FuguParams(1.0, 1.0, 1.0, 1.0, 0.5, 0.0, -1.0, 2, 1e12, 0.5))
+    u0 = np.zeros((n + 4, n + 5))
+    v0 = np.zeros((n + 5, n + 4))
+    gm, gr = np.zeros_like(mu), np.zeros_like(rho)
+    flags = abi.ADJ_PRECOND_VMG
+    ctx.adjoint_gradient(abi.FuguAdjointOpts(2, dt, 0.9, 1e-10, 3000, 30, 1, flags), mu, rho, 0.5, c, u0, v0, gm, gr)
+    t0 = time.perf_counter()
+    J, gs, st, rc = ctx.adjoint_gradient(abi.FuguAdjointOpts(nsteps, dt, 0.9, 1e-10, 3000, 30, 1, flags), mu, rho,
+                                         0.5, c, u0, v0, gm, gr)
+    wall = time.perf_counter() - t0
+    ctx.close()
+    print(f"# adjoint {n}x{n}, {nsteps} steps: J {J:.6e} dJ/dsigma {gs:.6e} forward {st.forward_seconds:.2f}s "
+          f"reverse {st.reverse_seconds:.2f}s iters {st.forward_iters}/{st.adjoint_iters}", file=sys.stderr, flush=True)
+    return {"workload": f"{n}x{n} unit square, {count} seeded inclusions r~U[0.01,0.04] (seed {seed}), carrier rho 100 "
+                        f"mu 1e-2, inclusions rho 1 mu 1e-2*10^U[0,3], sigma 0.5, g -1; J = kinetic energy in "
+                        f"y >= 0.9 after {nsteps} BDF2 steps (dt {dt}); gradients w.r.t. per-cell mu, rho and sigma",
+            "seconds_per_gradient": round(wall, 3), "forward_seconds": round(st.forward_seconds, 3),
+            "reverse_seconds": round(st.reverse_seconds, 3), "forward_iters": st.forward_iters,
+            "adjoint_iters": st.adjoint_iters, "max_forward_res": float(f"{st.max_forward_res:.3e}"),
+            "max_adjoint_res": float(f"{st.max_adjoint_res:.3e}"), "rc": rc, "J": J, "dJ_dsigma": gs,
+            "solves": "FP64 FGMRES(30) to 1e-10, PCD with multigrid pressure and velocity blocks"}
+
+
 def copy_probe(mib):
     """A plain streaming copy of known size (torch elementwise kernel, FP64): the calibration
     for the HBM byte counters (FETCH_SIZE, WRITE_SIZE) read on the solver's kernels."""
@@ -202,6 +277,9 @@ def main():
     ap.add_argument("--mg-cycles", type=int, default=1)
     ap.add_argument("--inner-max", type=int, default=-1, help="inner BiCGStab(2) cap; -1: PCD only")
     ap.add_argument("--restart", type=int, default=30)
+    ap.add_argument("--no-m1", action="store_true", help="skip the 2048 x 2048 FP64-vs-mixed section")
+    ap.add_argument("--m1-steps", type=int, default=3)
+    ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint seconds-per-gradient section")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: --gpus {args.gpus} is not supported by this build "
@@ -222,6 +300,12 @@ def main():
         forms["mixed"] = run_form(abi, base | abi.SOLVE_MIXED, **kw)
     if "ref" in args.forms:
         forms["ref"] = run_reference_form(abi, args.ref_steps, 1, args.ref_max_iter)
+    m1 = {}
+    if not args.no_m1:
+        kw1 = dict(kw, steps=args.m1_steps, wl=INCL2048)
+        m1["fp64"] = run_form(abi, base, **kw1)
+        m1["mixed"] = run_form(abi, base | abi.SOLVE_MIXED, **kw1)
+    adj = None if args.no_adjoint else run_adjoint(abi)
 
     def summary(f):
         sts = f["stats"]
@@ -264,6 +348,15 @@ def main():
     }
     if "fp64" in rep and "mixed" in rep and rep["fp64"]["solves_per_s"]:
         line["mixed_speedup_vs_fp64"] = round(rep["mixed"]["solves_per_s"] / rep["fp64"]["solves_per_s"], 3)
+    if m1:
+        r1 = {k: summary(v) for k, v in m1.items()}
+        line["m1"] = {"workload": INCL2048.name + ", solves to true relative residual 1e-8",
+                      "forms": r1,
+                      "mixed_speedup_vs_fp64": round(r1["mixed"]["solves_per_s"] / r1["fp64"]["solves_per_s"], 3),
+                      "roofline": {k: roofline(v["ktimes"], "inclusions_2048x2048") for k, v in m1.items()},
+                      "timed_kernels_GBps": {k: timed_rate(v["ktimes"]) for k, v in m1.items()}}
+    if adj:
+        line["adjoint"] = adj
     if not args.no_cpu:
         # iterations the reference solver needs per solve on this workload (GPU drop-in form)
         ref_iters = float(np.mean(rep["ref"]["inner_iters"])) if "ref" in rep else float(args.ref_max_iter)

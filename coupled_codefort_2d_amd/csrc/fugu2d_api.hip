@@ -133,8 +133,9 @@ struct fugu_ctx {
   bool use_mg = false;   // pressure block by multigrid V-cycles instead of the reference PCG
   bool use_vmg = false;  // velocity blocks by multigrid V-cycles instead of three Jacobi sweeps
   int mg_cycles = 1;
-  int mg_tail_cells = 4096;  // multigrid levels of at most this many cells run as one launch (0: never)
-  bool mg_fused = true;      // fused (temporal-blocking) level kernels instead of one launch per half-sweep
+  int mg_tail_cells = -1;    // coarse-tail start: < 0 the LDS-pool fit, > 0 a cell count, 0 never
+  bool mg_fused = true;
+  bool mg_tail_stage = true; // stage the coarse tail in LDS when it fits      // fused (temporal-blocking) level kernels instead of one launch per half-sweep
   double pcd_visc = 0.0; // viscous term of the Schur approximation (0 = the reference PCD)
   // adjoint: solve_fgmres applies A_eff^T and the transposed PCD instead of A and the PCD
   bool adjoint = false;
@@ -465,6 +466,8 @@ void mg_galerkin(fugu_ctx* c, std::vector<MgLevel<T>>& lv) {
 template <typename T>
 void mg_setup(fugu_ctx* c, Work<T>& w) {
   const Grid& g = c->g;
+  if (mg_odd_periodic(w.mg[0].d))
+    throw Error("multigrid preconditioner: a periodic direction needs an even cell count");
   LAUNCH((k_mg_coef0<T>), g2(g.nx + 1, g.ny + 1), dim3(64, 4), c->s, g, w.rho_u, w.rho_v, w.mg[0].cx, w.mg[0].cy);
   mg_galerkin(c, w.mg);
 }
@@ -474,6 +477,8 @@ template <typename T>
 void vmg_setup(fugu_ctx* c, Work<T>& w, T dt) {
   const Grid& g = c->g;
   for (int comp = 0; comp < 2; ++comp) {
+    if (mg_odd_periodic(vmg_dims(g, comp)))
+      throw Error("velocity multigrid: a periodic direction needs an even cell count");
     auto& lv = w.vmg[comp];
     if (lv.empty()) lv = mg_levels<T>(c, vmg_dims(g, comp), true);
     MgLevel<T>& L = lv[0];
@@ -493,15 +498,31 @@ void mg_smooth(fugu_ctx* c, MgLevel<T>& L, int colour, int zero_other) {
           c->s, L.d, colour, (const T*)L.cx, (const T*)L.cy, (const T*)L.dinv, (const T*)L.b, L.phi, zero_other);
 }
 
-// First level of the coarse tail: the first of at most mg_tail_cells cells (no more than
-// MG_TAIL_LEVELS from the bottom), when the coarsest level fits the LDS solve; lv.size(): none.
+// LDS elements the tail needs from level l down (coefficients, mass, D^-1, rhs, iterate).
+template <typename T>
+size_t mg_tail_footprint(const std::vector<MgLevel<T>>& lv, size_t l) {
+  size_t e = 0;
+  for (size_t q = l; q < lv.size(); ++q) {
+    const size_t nx = lv[q].d.nx, ny = lv[q].d.ny;
+    e += (nx + 1) * ny + nx * (ny + 1) + (lv[q].m ? 4 : 3) * nx * ny;
+  }
+  return e;
+}
+
+// First level of the coarse tail (lv.size(): none).  mg_tail_cells < 0 (default): the highest
+// level whose tail fits the LDS pool; > 0: the first level of at most that many cells; 0: no
+// tail.  At most MG_TAIL_LEVELS levels, and the coarsest level must fit the LDS solve.
 template <typename T>
 size_t mg_tail_start(const fugu_ctx* c, const std::vector<MgLevel<T>>& lv) {
   const size_t nl = lv.size();
   const MgDims& dc = lv[nl - 1].d;
-  if (c->mg_tail_cells <= 0 || (size_t)dc.nx * dc.ny > (size_t)MG_COARSE_MAX) return nl;
+  if (c->mg_tail_cells == 0 || (size_t)dc.nx * dc.ny > (size_t)MG_COARSE_MAX) return nl;
   size_t ts = 0;
-  while (ts < nl && (size_t)lv[ts].d.nx * lv[ts].d.ny > (size_t)c->mg_tail_cells) ++ts;
+  if (c->mg_tail_cells > 0) {
+    while (ts < nl && (size_t)lv[ts].d.nx * lv[ts].d.ny > (size_t)c->mg_tail_cells) ++ts;
+  } else {
+    while (ts + 1 < nl && mg_tail_footprint(lv, ts) * sizeof(T) > (size_t)MG_TAIL_POOL) ++ts;
+  }
   if (ts + MG_TAIL_LEVELS < nl) ts = nl - MG_TAIL_LEVELS;
   return ts;
 }
@@ -516,10 +537,23 @@ void mg_vcycle(fugu_ctx* c, std::vector<MgLevel<T>>& lv, size_t l, bool zero_gue
   if (l == mg_tail_start(c, lv)) {   // the coarse tail in one launch
     MgTailArgs<T> ta{};
     ta.nlev = (int)(nl - l);
-    for (size_t q = l; q < nl; ++q)
-      ta.L[q - l] = MgTailLevel<T>{lv[q].d, lv[q].cx, lv[q].cy, lv[q].m, lv[q].dinv, lv[q].cinv, lv[q].b, lv[q].phi};
+    int off = 0;
+    for (size_t q = l; q < nl; ++q) {
+      MgTailLevel<T> t{lv[q].d, lv[q].cx, lv[q].cy, lv[q].m, lv[q].dinv, lv[q].cinv, lv[q].b, lv[q].phi};
+      const int nx = lv[q].d.nx, ny = lv[q].d.ny, n = nx * ny;
+      t.ocx = off; off += (nx + 1) * ny;
+      t.ocy = off; off += nx * (ny + 1);
+      t.om = lv[q].m ? off : -1; off += lv[q].m ? n : 0;
+      t.odinv = off; off += n;
+      t.ob = off; off += n;
+      t.ophi = off; off += n;
+      ta.L[q - l] = t;
+    }
     ta.nu = nu; ta.coarse_sweeps = 60; ta.project = project; ta.zero_guess = zero_guess ? 1 : 0;
-    LAUNCH((k_mg_tail<T>), dim3(1), dim3(1024), c->s, ta);
+    if (c->mg_tail_stage && (size_t)off * sizeof(T) <= (size_t)MG_TAIL_POOL)
+      LAUNCH((k_mg_tail<T, true>), dim3(1), dim3(1024), c->s, ta);
+    else
+      LAUNCH((k_mg_tail<T, false>), dim3(1), dim3(1024), c->s, ta);
     return;
   }
   if (l + 1 == lv.size()) {
@@ -985,6 +1019,32 @@ void solve_fgmres(fugu_ctx* c, Work<T>& w, double* X, double dt, double tol, dou
       LAUNCH(k_fold_gs_norm, dim3(1), dim3(NT), c->s, (int)nb, gs_mode, c->g.vol, (const double*)c->partials,
              c->sc, c->fl);
   };
+  // W -= V h1 (coefficients at S_HBUF) fused with <W', V_i> -> S_HBUF + 64 + i when the
+  // re-orthogonalisation test fires (zero otherwise), ||W'||^2 -> S_DOT0, the test -> F_REORTH
+  auto maxpy_dot = [&](auto* W, auto* const* basis, int nq) {
+    using V = std::remove_cv_t<std::remove_pointer_t<decltype(W)>>;
+    MaxpyArgs<V, V> ma{};
+    ma.out = W; ma.x0 = W; ma.nq = nq; ma.c_off = 0; ma.sgn = -1.0; ma.n = n3;
+    for (int q = 0; q < MAXPY_MAX; ++q) ma.x[q] = q < nq ? basis[q] : W;   // padding slots: x0, coefficient 0
+    unsigned nb = nblk(n3);
+    if (nb > (unsigned)RED_MAXB) nb = RED_MAXB;
+    const double ab = (nq + 2.0) * sizeof(V) * n3;
+    const char* tname = sizeof(V) == 4 ? "maxpy_dot_f32" : "maxpy_dot_f64";
+    int stride = 0;
+#define FUGU_MAXPY_DOT(NQ)                                                                                  \
+    if (nq <= NQ && !stride) {                                                                             \
+      stride = NQ + 1;                                                                                      \
+      if (c->launch_log)                                                                                    \
+        std::fprintf(c->launch_log, "%s %.0f 1\n", sizeof(V) == 4 ? "k_maxpy_dot<float, " #NQ ">"          \
+                                                                  : "k_maxpy_dot<double, " #NQ ">", ab);    \
+      TLAUNCH(c, tname, ab, (k_maxpy_dot<V, NQ>), dim3(nb), dim3(256), c->s, ma, (const double*)c->sc,     \
+              c->partials);                                                                                 \
+    }
+    FUGU_MAXPY_DOT(8) FUGU_MAXPY_DOT(16) FUGU_MAXPY_DOT(32) FUGU_MAXPY_DOT(48)
+#undef FUGU_MAXPY_DOT
+    LAUNCH(k_fold_maxpy_dot, dim3(1), dim3(1024), c->s, (int)nb, nq, stride, c->g.vol, (const double*)c->partials,
+           c->sc, c->fl);
+  };
   // <W, V_i>, i = 0..k -> S_HBUF + off + i (and vol <W, W> -> S_DOT1); pass 2 predicated on F_REORTH
   auto dots_into = [&](const auto* W, auto* const* basis, int k, int off, bool second) {
     using V = std::remove_cv_t<std::remove_pointer_t<decltype(W)>>;
@@ -1036,8 +1096,7 @@ void solve_fgmres(fugu_ctx* c, Work<T>& w, double* X, double dt, double tol, dou
         matvec(c, c->wf, c->wf.X, c->wf.z, dt, mvflags, nullptr);
         float* W = c->wf.z;
         dots_into(W, c->kv32.data(), k, 0, false);
-        maxpy(W, W, c->kv32.data(), k + 1, -1.0, 0, 1);
-        dots_into(W, c->kv32.data(), k, 64, true);
+        maxpy_dot(W, c->kv32.data(), k + 1);
         maxpy(W, W, c->kv32.data(), k + 1, -1.0, 64, 2);
         LAUNCH(k_fgmres_step, dim3(1), dim3(1), c->s, k, 0.5 * tol * bn, c->sc, c->fg, c->fl);
         lin<float>(c, L_SCALE, c->kv32[k + 1], W, nullptr, nullptr, nullptr, S_INVH, -1, -1, 1.0, false);
@@ -1062,10 +1121,10 @@ void solve_fgmres(fugu_ctx* c, Work<T>& w, double* X, double dt, double tol, dou
         if (c->adjoint) matvec_T(c, wd.X, wd.z, dt, (mvflags & MV_ADV) ? 1 : 0);
         else matvec(c, wd, wd.X, wd.z, dt, mvflags, nullptr);
         double* W = wd.z;
-        // classical Gram-Schmidt, second pass only when the first lost more than half of ||W||^2
+        // classical Gram-Schmidt, second pass only when the first lost more than half of ||W||^2;
+        // the second pass's products come out of the first pass's sweep
         dots_into(W, c->kv.data(), k, 0, false);
-        maxpy(W, W, c->kv.data(), k + 1, -1.0, 0, 1);
-        dots_into(W, c->kv.data(), k, 64, true);
+        maxpy_dot(W, c->kv.data(), k + 1);
         maxpy(W, W, c->kv.data(), k + 1, -1.0, 64, 2);
         LAUNCH(k_fgmres_step, dim3(1), dim3(1), c->s, k, 0.5 * tol * bn, c->sc, c->fg, c->fl);
         lin<double>(c, L_SCALE, c->kv[k + 1], W, nullptr, nullptr, nullptr, S_INVH, -1, -1, 1.0, false);
@@ -1544,6 +1603,7 @@ int fugu_ctx_create(const fugu_grid* grid, const fugu_params* prm, fugu_ctx** ou
     if (const char* lp = std::getenv("FUGU_LAUNCH_LOG")) c->launch_log = std::fopen(lp, "a");
     if (const char* tc = std::getenv("FUGU_MG_TAIL_CELLS")) c->mg_tail_cells = std::atoi(tc);
     if (const char* fu = std::getenv("FUGU_MG_FUSED")) c->mg_fused = std::atoi(fu) != 0;
+    if (const char* ts = std::getenv("FUGU_MG_TAIL_STAGE")) c->mg_tail_stage = std::atoi(ts) != 0;
     HIPCHK(hipStreamCreateWithFlags(&c->s, hipStreamNonBlocking));
     for (auto& e : c->ev) HIPCHK(hipEventCreate(&e));
     for (auto& e : c->fg_ev) HIPCHK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
@@ -1551,7 +1611,8 @@ int fugu_ctx_create(const fugu_grid* grid, const fugu_params* prm, fugu_ctx** ou
     c->ntrip = g.nu + g.nv + g.np;
     // two-pass reductions: RED_MAXB workgroups x up to 5 double-double values; multi-dot: 6
     // chunks x MD_BX workgroups x (MD_CH + 1) values
-    c->partials = dalloc<double>(c, std::max((size_t)RED_MAXB * 10, (size_t)6 * MD_BX * (MD_CH + 1)));
+    c->partials = dalloc<double>(c, std::max({(size_t)RED_MAXB * 10, (size_t)6 * MD_BX * (MD_CH + 1),
+                                              (size_t)RED_MAXB * (MAXPY_MAX + 1)}));
     c->sc = dalloc<double>(c, S_COUNT);
     c->fl = (int*)dalloc<double>(c, F_COUNT);
     HIPCHK(hipHostMalloc((void**)&c->h_sc, S_COUNT * sizeof(double), hipHostMallocDefault));

@@ -571,6 +571,69 @@ __global__ void __launch_bounds__(256) k_maxpy(MaxpyArgs<O, V> a, const double* 
   }
 }
 
+// First Gram-Schmidt pass fused with the second pass's products: o = x0 - sum_q c_q x_q (as
+// k_maxpy, same operation order) and, in the same sweep, the per-workgroup sums of o x_q
+// (q < nq) and o^2 -- the inner products the re-orthogonalisation pass needs, read while the
+// basis is streamed once instead of twice.  NQ >= nq (compile-time accumulator count); the
+// caller points the slots nq..NQ-1 at x0 (valid, reread from cache) and their coefficients are
+// zero, so the loop has no branches and every load of an element is issued at once.
+template <typename V, int NQ>
+__global__ void __launch_bounds__(256) k_maxpy_dot(MaxpyArgs<V, V> a, const double* __restrict__ sc,
+                                                   double* __restrict__ partials) {
+  __shared__ double c[MAXPY_MAX];
+  if ((int)threadIdx.x < NQ) c[threadIdx.x] = (int)threadIdx.x < a.nq ? a.sgn * sc[S_HBUF + a.c_off + threadIdx.x] : 0.0;
+  __syncthreads();
+  double acc[NQ + 1];
+#pragma unroll
+  for (int q = 0; q <= NQ; ++q) acc[q] = 0.0;
+  const size_t stride = (size_t)gridDim.x * blockDim.x;
+  for (size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x; k < a.n; k += stride) {
+    double x[NQ];
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) x[q] = (double)a.x[q][k];
+    double o = (double)a.x0[k];
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) o += c[q] * x[q];
+    const V os = (V)o;
+    a.out[k] = os;
+    const double od = (double)os;
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) acc[q] += od * x[q];
+    acc[NQ] += od * od;
+  }
+  __shared__ double lds[16][NQ + 1];
+  block_sum<NQ + 1>(acc, lds);
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int q = 0; q <= NQ; ++q) partials[(size_t)blockIdx.x * (NQ + 1) + q] = acc[q];
+  }
+}
+// Fold of k_maxpy_dot: sc[S_DOT0] = vol ||o||^2, fl[F_REORTH] = (||o||^2 < ||w||^2 / 2, the
+// Kahan-Parlett test against sc[S_DOT1] = vol ||w||^2), and the second pass's coefficients
+// sc[S_HBUF + 64 + q] = vol <o, x_q> when it runs, zero otherwise.
+__global__ void __launch_bounds__(1024) k_fold_maxpy_dot(int nb, int nq, int stride, double vol,
+                                                         const double* __restrict__ partials,
+                                                         double* __restrict__ sc, int* __restrict__ fl) {
+  __shared__ double red[MAXPY_MAX + 1];
+  __shared__ int reorth;
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  for (int q = wid; q <= nq; q += (int)(blockDim.x >> 6)) {
+    const int col = q < nq ? q : stride - 1;
+    double s = 0.0;
+    for (int b = lane; b < nb; b += 64) s += partials[(size_t)b * stride + col];
+    s = wave_sum(s);
+    if (lane == 0) red[q] = s * vol;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    sc[S_DOT0] = red[nq];
+    reorth = (red[nq] < 0.5 * sc[S_DOT1]) ? 1 : 0;
+    fl[F_REORTH] = reorth;
+  }
+  __syncthreads();
+  if ((int)threadIdx.x < nq) sc[S_HBUF + 64 + threadIdx.x] = reorth ? red[threadIdx.x] : 0.0;
+}
+
 // Multi-dot of the FGMRES Gram-Schmidt: <w, v_q> for q < nv in one launch (FP64 or FP32
 // storage, FP64 products and sums).  blockIdx.y selects a chunk of MD_CH basis vectors,
 // blockIdx.x one of MD_BX slices of the flat arrays; plain FP64 sums in a fixed order.  The triplets the outer iteration builds

@@ -28,9 +28,18 @@ struct MgDims {
   int nx, ny, per_x, per_y;
 };
 
+constexpr int MG_COARSE_MAX = 1024;   // coarsest level solved in one workgroup
+constexpr int MG_DENSE_MAX = 64;      // coarsest level solved by its dense inverse
+
 inline MgDims mg_coarse_dims(const MgDims& f) { return MgDims{(f.nx + 1) / 2, (f.ny + 1) / 2, f.per_x, f.per_y}; }
+inline bool mg_odd_periodic(const MgDims& d) { return (d.per_x && (d.nx & 1)) || (d.per_y && (d.ny & 1)); }
+// Red-black colouring needs even periodic extents (an odd one puts two neighbours across the
+// seam in one colour), so a level with an odd periodic extent is only ever the coarsest one,
+// and then only when the dense solve takes it.
 inline bool mg_can_coarsen(const MgDims& d) {
-  return d.nx >= 8 && d.ny >= 8 && !(d.per_x && (d.nx & 1)) && !(d.per_y && (d.ny & 1));
+  if (d.nx < 8 || d.ny < 8 || mg_odd_periodic(d)) return false;
+  const MgDims c = mg_coarse_dims(d);
+  return !mg_odd_periodic(c) || c.nx * c.ny <= MG_DENSE_MAX;
 }
 
 __device__ inline int mg_wrap(int a, int n, int per) {
@@ -179,9 +188,6 @@ __global__ void __launch_bounds__(256) k_mg_prolong(MgDims f, const T* __restric
 // solved exactly by its dense inverse (k_mg_coarse_inv, once per set-up; for the singular
 // block the inverse of A + s 11^T / n, which maps a centred rhs to the zero-mean solution),
 // otherwise by `sweeps` LDS-resident red-black sweeps from zero.
-constexpr int MG_COARSE_MAX = 1024;
-constexpr int MG_DENSE_MAX = 64;
-
 template <typename T>
 __device__ inline void mg_coarse_solve(const MgDims& d, const T* cx, const T* cy, const T* dinv, const T* cinv,
                                        const T* b, T* phi, int sweeps, int project, T* sphi, T* sb, double* red) {
@@ -456,17 +462,21 @@ __global__ void __launch_bounds__(256) k_mg_post(MgDims f, const T* __restrict__
 }
 
 // ---------------------------------------------------------------------------------
-// Coarse tail: the V-cycle over the coarsest levels (the first one holding at most a few
-// thousand cells and everything below it) in ONE workgroup and one launch, with a barrier
-// between phases instead of a launch per half-sweep.  Phase for phase it performs the
-// multi-launch cycle's arithmetic (same smoother update, residual sum, prolongation and
-// coarsest solve), so its result is bit-identical; the coarse levels' arrays stay in L2.
+// Coarse tail: the V-cycle over the coarsest levels (from a level of at most a few thousand
+// cells down) in ONE workgroup and one launch, with a barrier between phases instead of a
+// launch per half-sweep.  STAGE: the levels' coefficients, rhs and iterates are first copied
+// into LDS (when they fit MG_TAIL_POOL bytes), so every phase runs from LDS; otherwise they
+// stay in global memory (L2-resident).  Phase for phase the kernel performs the multi-launch
+// cycle's arithmetic (same smoother update, residual sum, prolongation and coarsest solve), so
+// its result is bit-identical; only the top level's phi is written back.
 constexpr int MG_TAIL_LEVELS = 16;
+constexpr int MG_TAIL_POOL = 136 * 1024;
 template <typename T>
 struct MgTailLevel {
   MgDims d;
   const T *cx, *cy, *m, *dinv, *cinv;
   T *b, *phi;
+  int ocx, ocy, om, odinv, ob, ophi;   // offsets in the LDS pool (elements; om < 0: no mass term)
 };
 template <typename T>
 struct MgTailArgs {
@@ -475,8 +485,8 @@ struct MgTailArgs {
 };
 
 template <typename T>
-__device__ inline void tail_smooth(const MgTailLevel<T>& L, int colour, int zero_other) {
-  const MgDims d = L.d;
+__device__ inline void tail_smooth(const MgDims d, const T* cx, const T* cy, const T* dinv, const T* b, T* phi,
+                                   int colour, int zero_other) {
   const int half = (d.nx + 1) / 2;
   const int tot = half * d.ny;
   for (int q = (int)threadIdx.x; q < tot; q += (int)blockDim.x) {
@@ -484,30 +494,64 @@ __device__ inline void tail_smooth(const MgTailLevel<T>& L, int colour, int zero
     const int i = 2 * t + ((j + colour) & 1);
     if (i >= d.nx) continue;
     const size_t o = i + (size_t)j * d.nx;
-    const T s = zero_other ? T(0) : mg_nbr_sum(d, L.cx, L.cy, (const T*)L.phi, i, j);
-    L.phi[o] = (L.b[o] + s) * L.dinv[o];
+    const T s = zero_other ? T(0) : mg_nbr_sum(d, cx, cy, (const T*)phi, i, j);
+    phi[o] = (b[o] + s) * dinv[o];
   }
   __syncthreads();
 }
 
 template <typename T>
+__device__ inline void tail_copy(T* dst, const T* src, int n) {
+  for (int q = (int)threadIdx.x; q < n; q += (int)blockDim.x) dst[q] = src[q];
+}
+
+template <typename T, bool STAGE>
 __global__ void __launch_bounds__(1024) k_mg_tail(MgTailArgs<T> a) {
   __shared__ T sphi[MG_COARSE_MAX];
   __shared__ T sb[MG_COARSE_MAX];
   __shared__ double red[16];
+  __shared__ T pool[STAGE ? MG_TAIL_POOL / sizeof(T) : 1];
   const int nu = a.nu;
+  auto CX = [&](int l) -> const T* { return STAGE ? pool + a.L[l].ocx : a.L[l].cx; };
+  auto CY = [&](int l) -> const T* { return STAGE ? pool + a.L[l].ocy : a.L[l].cy; };
+  auto MM = [&](int l) -> const T* { return STAGE ? (a.L[l].om >= 0 ? pool + a.L[l].om : nullptr) : a.L[l].m; };
+  auto DI = [&](int l) -> const T* { return STAGE ? pool + a.L[l].odinv : a.L[l].dinv; };
+  auto BB = [&](int l) -> T* { return STAGE ? pool + a.L[l].ob : a.L[l].b; };
+  auto PH = [&](int l) -> T* { return STAGE ? pool + a.L[l].ophi : a.L[l].phi; };
+  if (STAGE) {
+    for (int l = 0; l < a.nlev; ++l) {
+      const MgDims d = a.L[l].d;
+      const int n = d.nx * d.ny;
+      tail_copy(pool + a.L[l].ocx, a.L[l].cx, (d.nx + 1) * d.ny);
+      tail_copy(pool + a.L[l].ocy, a.L[l].cy, d.nx * (d.ny + 1));
+      tail_copy(pool + a.L[l].odinv, a.L[l].dinv, n);
+      if (a.L[l].om >= 0) tail_copy(pool + a.L[l].om, a.L[l].m, n);
+    }
+    const MgDims d0 = a.L[0].d;
+    tail_copy(pool + a.L[0].ob, (const T*)a.L[0].b, d0.nx * d0.ny);
+    // a zero guess still needs finite values: at a wall the smoother's neighbour sum reads the
+    // cell itself times a zero conductance (LDS holds whatever the last kernel left there)
+    if (a.zero_guess) {
+      for (int q = (int)threadIdx.x; q < d0.nx * d0.ny; q += (int)blockDim.x) pool[a.L[0].ophi + q] = T(0);
+    } else {
+      tail_copy(pool + a.L[0].ophi, (const T*)a.L[0].phi, d0.nx * d0.ny);
+    }
+    __syncthreads();
+  }
   // descent: pre-smoothing and residual restriction
   for (int l = 0; l + 1 < a.nlev; ++l) {
-    const MgTailLevel<T>& L = a.L[l];
+    const MgDims f = a.L[l].d;
+    const T *cx = CX(l), *cy = CY(l), *m = MM(l), *dinv = DI(l);
+    const T* b = BB(l);
+    T* phi = PH(l);
     const int zg = (l == 0) ? a.zero_guess : 1;
     for (int s = 0; s < nu; ++s) {
-      tail_smooth(L, 0, (s == 0 && zg) ? 1 : 0);
-      tail_smooth(L, 1, 0);
+      tail_smooth(f, cx, cy, dinv, b, phi, 0, (s == 0 && zg) ? 1 : 0);
+      tail_smooth(f, cx, cy, dinv, b, phi, 1, 0);
     }
-    const MgDims f = L.d;
     const int nxc = (f.nx + 1) / 2, nyc = (f.ny + 1) / 2;
-    T* bc = a.L[l + 1].b;
-    T* phic = a.L[l + 1].phi;
+    T* bc = BB(l + 1);
+    T* phic = PH(l + 1);
     for (int q = (int)threadIdx.x; q < nxc * nyc; q += (int)blockDim.x) {
       const int J = q / nxc, I = q - J * nxc;
       T acc = T(0);
@@ -517,9 +561,9 @@ __global__ void __launch_bounds__(1024) k_mg_tail(MgTailArgs<T> a) {
         if (i >= f.nx || j >= f.ny) continue;
         const size_t o = i + (size_t)j * f.nx;
         const size_t rx = (size_t)(f.nx + 1);
-        T diag = L.cx[i + j * rx] + L.cx[i + 1 + j * rx] + L.cy[o] + L.cy[o + f.nx];
-        if (L.m) diag += L.m[o];
-        acc += L.b[o] - (diag * L.phi[o] - mg_nbr_sum(f, L.cx, L.cy, (const T*)L.phi, i, j));
+        T diag = cx[i + j * rx] + cx[i + 1 + j * rx] + cy[o] + cy[o + f.nx];
+        if (m) diag += m[o];
+        acc += b[o] - (diag * phi[o] - mg_nbr_sum(f, cx, cy, (const T*)phi, i, j));
       }
       bc[I + (size_t)J * nxc] = acc;
       phic[I + (size_t)J * nxc] = T(0);
@@ -528,25 +572,32 @@ __global__ void __launch_bounds__(1024) k_mg_tail(MgTailArgs<T> a) {
   }
   // coarsest level
   {
-    const MgTailLevel<T>& L = a.L[a.nlev - 1];
-    mg_coarse_solve(L.d, L.cx, L.cy, L.dinv, L.cinv, (const T*)L.b, L.phi, a.coarse_sweeps, a.project, sphi, sb, red);
+    const int l = a.nlev - 1;
+    mg_coarse_solve(a.L[l].d, CX(l), CY(l), DI(l), a.L[l].cinv, (const T*)BB(l), PH(l), a.coarse_sweeps, a.project,
+                    sphi, sb, red);
   }
   // ascent: prolongation and post-smoothing
   for (int l = a.nlev - 2; l >= 0; --l) {
-    const MgTailLevel<T>& L = a.L[l];
-    const MgDims f = L.d;
-    const T* phic = a.L[l + 1].phi;
+    const MgDims f = a.L[l].d;
+    const T *cx = CX(l), *cy = CY(l), *dinv = DI(l);
+    const T* b = BB(l);
+    T* phi = PH(l);
+    const T* phic = PH(l + 1);
     const int nxc = (f.nx + 1) / 2;
     for (int q = (int)threadIdx.x; q < f.nx * f.ny; q += (int)blockDim.x) {
       const int j = q / f.nx, i = q - j * f.nx;
       const size_t o = i + (size_t)j * f.nx;
-      L.phi[o] = L.phi[o] + phic[(i >> 1) + (size_t)(j >> 1) * nxc];
+      phi[o] = phi[o] + phic[(i >> 1) + (size_t)(j >> 1) * nxc];
     }
     __syncthreads();
     for (int s = 0; s < nu; ++s) {
-      tail_smooth(L, 1, 0);
-      tail_smooth(L, 0, 0);
+      tail_smooth(f, cx, cy, dinv, b, phi, 1, 0);
+      tail_smooth(f, cx, cy, dinv, b, phi, 0, 0);
     }
+  }
+  if (STAGE) {
+    const MgDims d0 = a.L[0].d;
+    tail_copy(a.L[0].phi, (const T*)(pool + a.L[0].ophi), d0.nx * d0.ny);
   }
 }
 

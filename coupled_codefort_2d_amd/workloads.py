@@ -48,13 +48,20 @@ def inclusions(n, count, rmin, rmax, seed, rho_carrier, mu_carrier, rho_incl, mu
         if all(np.hypot(xc - x, yc - y) >= r + q + 2 * h for x, y, q in circles):
             circles.append((xc, yc, r))
     mus = mu_carrier * 10.0 ** rng.uniform(0.0, np.log10(mu_ratio), count)
-    X, Y = cell_centres(n, n, 1.0, 1.0, nh)
-    c = np.ones_like(X)
-    mu = np.full_like(X, mu_carrier)
-    rho = np.full_like(X, rho_carrier)
+    shape = (n + 2 * nh, n + 2 * nh)
+    c = np.ones(shape)
+    mu = np.full(shape, mu_carrier)
+    rho = np.full(shape, rho_carrier)
     for (xc, yc, r), m in zip(circles, mus):
-        inside = np.hypot(X - xc, Y - yc) <= r
-        c[inside] = 0.0
-        mu[inside] = m
-        rho[inside] = rho_incl
+        # padded index k <-> cell i = k + 1 - nh, centre (i - 1/2) h; test only the bounding box
+        k0 = max(int(np.floor((xc - r) / h + 0.5)) - 1 + nh - 1, 0)
+        k1 = min(int(np.ceil((xc + r) / h + 0.5)) + 1 + nh, shape[1])
+        l0 = max(int(np.floor((yc - r) / h + 0.5)) - 1 + nh - 1, 0)
+        l1 = min(int(np.ceil((yc + r) / h + 0.5)) + 1 + nh, shape[0])
+        xs = (np.arange(k0, k1) + 1 - nh - 0.5) * h
+        ys = (np.arange(l0, l1) + 1 - nh - 0.5) * h
+        inside = np.hypot(xs[None, :] - xc, ys[:, None] - yc) <= r
+        c[l0:l1, k0:k1][inside] = 0.0
+        mu[l0:l1, k0:k1][inside] = m
+        rho[l0:l1, k0:k1][inside] = rho_incl
     return c, mu, rho, circles

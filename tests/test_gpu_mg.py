@@ -55,16 +55,17 @@ def test_mg_vcycle_is_linear():
 @pytest.mark.parametrize("nx,ny,per_x,per_y", [(32, 24, False, False), (48, 40, True, False), (40, 64, False, True),
                                                (96, 80, True, True), (130, 70, False, False)])
 def test_coarse_tail_is_bitwise_the_multilaunch_cycle(monkeypatch, nx, ny, per_x, per_y):
-    """The single-launch coarse tail and the fused (temporal-blocking) level kernels perform the
-    one-launch-per-half-sweep V-cycle's arithmetic: results equal bit for bit with both off, at
-    the default threshold and with the whole hierarchy in the tail (odd extents included)."""
+    """The single-launch coarse tail (LDS-staged or global) and the fused (temporal-blocking)
+    level kernels perform the one-launch-per-half-sweep V-cycle's arithmetic: results equal bit
+    for bit with both off, at a 4096-cell threshold, with the default (LDS-fit) start and with the
+    whole hierarchy in the tail (odd extents included)."""
     g = O.Grid(nx, ny, 1.0, ny / nx, per_x=per_x, per_y=per_y)
     rng = np.random.default_rng(202)
     c, pr, _, _ = two_phase_case(g, rng, rho=(100.0, 1.0))
     O.halo(g, pr["rho_u"], "u"); O.halo(g, pr["rho_v"], "v")
     rhs = rand(g, "p", rng)
     out = {}
-    for cells, fused in (("0", "0"), ("0", "1"), ("4096", "1"), ("1000000", "1"), ("4096", "0")):
+    for cells, fused in (("0", "0"), ("0", "1"), ("4096", "1"), ("1000000", "1"), ("4096", "0"), ("-1", "1")):
         monkeypatch.setenv("FUGU_MG_TAIL_CELLS", cells)
         monkeypatch.setenv("FUGU_MG_FUSED", fused)
         ctx = abi.Context(g.nx, g.ny, g.lx, g.ly, per_x=per_x, per_y=per_y)
@@ -87,7 +88,7 @@ def test_coarse_tail_bitwise_in_the_step_solve(monkeypatch, mixed):
     import bench
     nx, ny = 96, 192
     outs = []
-    for cells, fused in (("0", "0"), ("4096", "1")):
+    for cells, fused in (("0", "0"), ("-1", "1")):
         monkeypatch.setenv("FUGU_MG_TAIL_CELLS", cells)
         monkeypatch.setenv("FUGU_MG_FUSED", fused)
         p = abi.FuguParams(1000.0, 1.0, 10.0, 0.1, 2.0, 0.0, -1.0, 2, 1e12, 0.5)
@@ -110,3 +111,15 @@ def test_coarse_tail_bitwise_in_the_step_solve(monkeypatch, mixed):
     assert outs[0][0] == outs[1][0]
     for a, b in zip(outs[0][1], outs[1][1]):
         assert np.array_equal(a, b)
+
+
+def test_odd_periodic_extent_is_refused():
+    """Red-black smoothing needs even periodic extents: the multigrid block says so instead of
+    running a colouring whose seam neighbours share a colour."""
+    g = O.Grid(25, 16, 1.0, 0.64, per_x=True)
+    rng = np.random.default_rng(203)
+    c, pr, _, _ = two_phase_case(g, rng, rho=(10.0, 1.0))
+    ctx = abi.Context(g.nx, g.ny, g.lx, g.ly, per_x=True)
+    with pytest.raises(Exception, match="even"):
+        ctx.pressure_mg(g.zeros("p"), rand(g, "p", rng), pr["rho_u"], pr["rho_v"], n_cycles=1)
+    ctx.close()
