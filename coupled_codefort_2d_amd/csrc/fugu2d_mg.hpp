@@ -323,6 +323,7 @@ template <typename T, int H>
 struct MgtCells {
   static constexpr int EW = MGT_W + 2 * H, EH = MGT_H + 2 * H, NS = (EW * EH + 255) / 256;
   T b[NS], dinv[NS], cW[NS], cE[NS], cS[NS], cN[NS];
+  T dg[NS];                       // unmasked diagonal (+ m): the residual of the restriction
   int li[NS], lj[NS], ring[NS];   // ring: distance outside the tile (0 inside); -1: no cell
   int col[NS];                    // colour of the wrapped global index
 };
@@ -330,8 +331,8 @@ struct MgtCells {
 // Load the thread's cells (registers) and phi (+ the parent's coarse correction) into LDS.
 template <typename T, int H>
 __device__ inline void mgt_init(const MgDims& f, int i0, int j0, const T* cx, const T* cy, const T* dinv,
-                                const T* b, const T* phi, const T* phic, MgtCells<T, H>& m,
-                                T (*s)[MgtCells<T, H>::EW]) {
+                                const T* b, const T* phi, const T* phic, const T* mass, bool want_dg,
+                                MgtCells<T, H>& m, T (*s)[MgtCells<T, H>::EW]) {
   using M = MgtCells<T, H>;
   const int tid = threadIdx.x;
   const int nxc = (f.nx + 1) / 2;
@@ -351,21 +352,26 @@ __device__ inline void mgt_init(const MgDims& f, int i0, int j0, const T* cx, co
     const bool cell = idx < M::EW * M::EH && i >= 0 && j >= 0;
     m.ring[k] = cell ? dist : -1;
     m.col[k] = (i + j) & 1;
-    T bv = T(0), dv = T(0), w = T(0), e = T(0), so = T(0), no = T(0), p = T(0);
+    T bv = T(0), dv = T(0), w = T(0), e = T(0), so = T(0), no = T(0), p = T(0), dg = T(0);
     if (cell) {
       const size_t o = i + (size_t)j * f.nx;
       bv = b[o];
       dv = dinv[o];
-      w = (i > 0 || f.per_x) ? cx[i + j * rx] : T(0);
-      e = (i < f.nx - 1 || f.per_x) ? cx[i + 1 + j * rx] : T(0);
-      so = (j > 0 || f.per_y) ? cy[o] : T(0);
-      no = (j < f.ny - 1 || f.per_y) ? cy[o + f.nx] : T(0);
+      const T rw = cx[i + j * rx], re = cx[i + 1 + j * rx], rs = cy[o], rn = cy[o + f.nx];
+      w = (i > 0 || f.per_x) ? rw : T(0);
+      e = (i < f.nx - 1 || f.per_x) ? re : T(0);
+      so = (j > 0 || f.per_y) ? rs : T(0);
+      no = (j < f.ny - 1 || f.per_y) ? rn : T(0);
+      if (want_dg && dist == 0) {
+        dg = rw + re + rs + rn;
+        if (mass) dg += mass[o];
+      }
       if (phi) {
         p = phi[o];
         if (phic) p = p + phic[(i >> 1) + (size_t)(j >> 1) * nxc];
       }
     }
-    m.b[k] = bv; m.dinv[k] = dv; m.cW[k] = w; m.cE[k] = e; m.cS[k] = so; m.cN[k] = no;
+    m.b[k] = bv; m.dinv[k] = dv; m.cW[k] = w; m.cE[k] = e; m.cS[k] = so; m.cN[k] = no; m.dg[k] = dg;
     pv[k] = p;
   }
 #pragma unroll
@@ -402,7 +408,8 @@ __device__ inline void mgt_store(const MgDims& f, int i0, int j0, T* out, const 
 }
 
 // NU pre-smoothing sweeps from zero (red first), then the residual summed onto the coarse
-// cells of the tile (coarse phi zeroed).
+// cells of the tile (coarse phi zeroed).  The residual of each tile cell is formed by the thread
+// holding its coefficients and summed from LDS (children in the restriction's order).
 template <typename T, int NU>
 __global__ void __launch_bounds__(256) k_mg_pre(MgDims f, const T* __restrict__ cx, const T* __restrict__ cy,
                                                 const T* __restrict__ m, const T* __restrict__ dinv,
@@ -411,15 +418,24 @@ __global__ void __launch_bounds__(256) k_mg_pre(MgDims f, const T* __restrict__ 
   constexpr int H = 2 * NU + 1;
   using M = MgtCells<T, H>;
   __shared__ T s[M::EH][M::EW];
+  __shared__ T sres[MGT_H][MGT_W];
   int i0, j0;
   mgt_tile(i0, j0);
   M cells;
-  mgt_init<T, H>(f, i0, j0, cx, cy, dinv, b, (const T*)nullptr, (const T*)nullptr, cells, s);
+  mgt_init<T, H>(f, i0, j0, cx, cy, dinv, b, (const T*)nullptr, (const T*)nullptr, m, true, cells, s);
 #pragma unroll
   for (int h = 0; h < 2 * NU; ++h) mgt_half<T, H>(H - 1 - h, h & 1, h == 0 ? 1 : 0, cells, s);
   mgt_store<T, H>(f, i0, j0, phi, s);
+#pragma unroll
+  for (int k = 0; k < M::NS; ++k) {
+    if (cells.ring[k] != 0) continue;
+    const int li = cells.li[k], lj = cells.lj[k];
+    const T nb = cells.cW[k] * s[lj][li - 1] + cells.cE[k] * s[lj][li + 1] + cells.cS[k] * s[lj - 1][li] +
+                 cells.cN[k] * s[lj + 1][li];
+    sres[lj - H][li - H] = cells.b[k] - (cells.dg[k] * s[lj][li] - nb);
+  }
+  __syncthreads();
   const int nxc = (f.nx + 1) / 2, nyc = (f.ny + 1) / 2;
-  const size_t rx = (size_t)(f.nx + 1);
   const int q = threadIdx.x;   // one coarse cell of the 32 x 8 coarse tile
   const int I = i0 / 2 + (q & 31), J = j0 / 2 + (q >> 5);
   if (I >= nxc || J >= nyc) return;
@@ -428,16 +444,7 @@ __global__ void __launch_bounds__(256) k_mg_pre(MgDims f, const T* __restrict__ 
   for (int k = 0; k < 4; ++k) {
     const int i = 2 * I + (k & 1), j = 2 * J + (k >> 1);
     if (i >= f.nx || j >= f.ny) continue;
-    const int li = H + i - i0, lj = H + j - j0;
-    const size_t o = i + (size_t)j * f.nx;
-    T diag = cx[i + j * rx] + cx[i + 1 + j * rx] + cy[o] + cy[o + f.nx];
-    if (m) diag += m[o];
-    const T cW = (i > 0 || f.per_x) ? cx[i + j * rx] : T(0);
-    const T cE = (i < f.nx - 1 || f.per_x) ? cx[i + 1 + j * rx] : T(0);
-    const T cS = (j > 0 || f.per_y) ? cy[o] : T(0);
-    const T cN = (j < f.ny - 1 || f.per_y) ? cy[o + f.nx] : T(0);
-    const T nb = cW * s[lj][li - 1] + cE * s[lj][li + 1] + cS * s[lj - 1][li] + cN * s[lj + 1][li];
-    acc += b[o] - (diag * s[lj][li] - nb);
+    acc += sres[j - j0][i - i0];
   }
   bc[I + (size_t)J * nxc] = acc;
   phic[I + (size_t)J * nxc] = T(0);
@@ -455,7 +462,7 @@ __global__ void __launch_bounds__(256) k_mg_post(MgDims f, const T* __restrict__
   int i0, j0;
   mgt_tile(i0, j0);
   M cells;
-  mgt_init<T, H>(f, i0, j0, cx, cy, dinv, b, phi, phic, cells, s);
+  mgt_init<T, H>(f, i0, j0, cx, cy, dinv, b, phi, phic, (const T*)nullptr, false, cells, s);
 #pragma unroll
   for (int h = 0; h < 2 * NU; ++h) mgt_half<T, H>(H - 1 - h, (h & 1) ? 0 : 1, 0, cells, s);
   mgt_store<T, H>(f, i0, j0, phi_out, s);
@@ -484,18 +491,41 @@ struct MgTailArgs {
   int nlev, nu, coarse_sweeps, project, zero_guess;
 };
 
+// One colour of a red-black sweep in the tail (1024 threads as 32 x 32).  phic (the first
+// post-smoothing half-sweep): the neighbours' values are phi + phi_c(parent), the prolongation
+// done on the fly -- the only reader of the prolonged values of this colour's neighbours, whose
+// own cells are overwritten before anything else reads them.
 template <typename T>
 __device__ inline void tail_smooth(const MgDims d, const T* cx, const T* cy, const T* dinv, const T* b, T* phi,
-                                   int colour, int zero_other) {
+                                   int colour, int zero_other, const T* phic = nullptr) {
   const int half = (d.nx + 1) / 2;
-  const int tot = half * d.ny;
-  for (int q = (int)threadIdx.x; q < tot; q += (int)blockDim.x) {
-    const int j = q / half, t = q - j * half;
-    const int i = 2 * t + ((j + colour) & 1);
-    if (i >= d.nx) continue;
-    const size_t o = i + (size_t)j * d.nx;
-    const T s = zero_other ? T(0) : mg_nbr_sum(d, cx, cy, (const T*)phi, i, j);
-    phi[o] = (b[o] + s) * dinv[o];
+  const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
+  const int nxc = (d.nx + 1) / 2;
+  for (int j = ty; j < d.ny; j += 32) {
+    for (int t = tx; t < half; t += 32) {
+      const int i = 2 * t + ((j + colour) & 1);
+      if (i >= d.nx) continue;
+      const size_t o = i + (size_t)j * d.nx;
+      T s = T(0);
+      if (!zero_other) {
+        if (phic) {
+          const size_t rx = (size_t)(d.nx + 1), ro = (size_t)j * d.nx;
+          const T cW = (i > 0 || d.per_x) ? cx[i + j * rx] : T(0);
+          const T cE = (i < d.nx - 1 || d.per_x) ? cx[i + 1 + j * rx] : T(0);
+          const T cS = (j > 0 || d.per_y) ? cy[i + ro] : T(0);
+          const T cN = (j < d.ny - 1 || d.per_y) ? cy[i + ro + d.nx] : T(0);
+          const int iw = mg_wrap(i - 1, d.nx, d.per_x), ie = mg_wrap(i + 1, d.nx, d.per_x);
+          const int js = mg_wrap(j - 1, d.ny, d.per_y), jn = mg_wrap(j + 1, d.ny, d.per_y);
+          auto pv = [&](int ii, int jj) -> T {
+            return phi[ii + (size_t)jj * d.nx] + phic[(ii >> 1) + (size_t)(jj >> 1) * nxc];
+          };
+          s = cW * pv(iw, j) + cE * pv(ie, j) + cS * pv(i, js) + cN * pv(i, jn);
+        } else {
+          s = mg_nbr_sum(d, cx, cy, (const T*)phi, i, j);
+        }
+      }
+      phi[o] = (b[o] + s) * dinv[o];
+    }
   }
   __syncthreads();
 }
@@ -552,21 +582,22 @@ __global__ void __launch_bounds__(1024) k_mg_tail(MgTailArgs<T> a) {
     const int nxc = (f.nx + 1) / 2, nyc = (f.ny + 1) / 2;
     T* bc = BB(l + 1);
     T* phic = PH(l + 1);
-    for (int q = (int)threadIdx.x; q < nxc * nyc; q += (int)blockDim.x) {
-      const int J = q / nxc, I = q - J * nxc;
-      T acc = T(0);
+    for (int J = (int)threadIdx.x >> 5; J < nyc; J += 32) {
+      for (int I = (int)threadIdx.x & 31; I < nxc; I += 32) {
+        T acc = T(0);
 #pragma unroll
-      for (int k = 0; k < 4; ++k) {
-        const int i = 2 * I + (k & 1), j = 2 * J + (k >> 1);
-        if (i >= f.nx || j >= f.ny) continue;
-        const size_t o = i + (size_t)j * f.nx;
-        const size_t rx = (size_t)(f.nx + 1);
-        T diag = cx[i + j * rx] + cx[i + 1 + j * rx] + cy[o] + cy[o + f.nx];
-        if (m) diag += m[o];
-        acc += b[o] - (diag * phi[o] - mg_nbr_sum(f, cx, cy, (const T*)phi, i, j));
+        for (int k = 0; k < 4; ++k) {
+          const int i = 2 * I + (k & 1), j = 2 * J + (k >> 1);
+          if (i >= f.nx || j >= f.ny) continue;
+          const size_t o = i + (size_t)j * f.nx;
+          const size_t rx = (size_t)(f.nx + 1);
+          T diag = cx[i + j * rx] + cx[i + 1 + j * rx] + cy[o] + cy[o + f.nx];
+          if (m) diag += m[o];
+          acc += b[o] - (diag * phi[o] - mg_nbr_sum(f, cx, cy, (const T*)phi, i, j));
+        }
+        bc[I + (size_t)J * nxc] = acc;
+        phic[I + (size_t)J * nxc] = T(0);
       }
-      bc[I + (size_t)J * nxc] = acc;
-      phic[I + (size_t)J * nxc] = T(0);
     }
     __syncthreads();
   }
@@ -576,22 +607,15 @@ __global__ void __launch_bounds__(1024) k_mg_tail(MgTailArgs<T> a) {
     mg_coarse_solve(a.L[l].d, CX(l), CY(l), DI(l), a.L[l].cinv, (const T*)BB(l), PH(l), a.coarse_sweeps, a.project,
                     sphi, sb, red);
   }
-  // ascent: prolongation and post-smoothing
+  // ascent: post-smoothing, the prolongation folded into its first half-sweep
   for (int l = a.nlev - 2; l >= 0; --l) {
     const MgDims f = a.L[l].d;
     const T *cx = CX(l), *cy = CY(l), *dinv = DI(l);
     const T* b = BB(l);
     T* phi = PH(l);
     const T* phic = PH(l + 1);
-    const int nxc = (f.nx + 1) / 2;
-    for (int q = (int)threadIdx.x; q < f.nx * f.ny; q += (int)blockDim.x) {
-      const int j = q / f.nx, i = q - j * f.nx;
-      const size_t o = i + (size_t)j * f.nx;
-      phi[o] = phi[o] + phic[(i >> 1) + (size_t)(j >> 1) * nxc];
-    }
-    __syncthreads();
     for (int s = 0; s < nu; ++s) {
-      tail_smooth(f, cx, cy, dinv, b, phi, 1, 0);
+      tail_smooth(f, cx, cy, dinv, b, phi, 1, 0, s == 0 ? phic : (const T*)nullptr);
       tail_smooth(f, cx, cy, dinv, b, phi, 0, 0);
     }
   }
