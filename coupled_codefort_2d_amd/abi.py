@@ -63,6 +63,7 @@ SOLVE_MIXED = 4
 PRECOND_MG = 8
 PRECOND_VISC = 16
 PRECOND_VMG = 32
+SOLVE_EXTRAPOLATE = 64
 
 EXPORTS = [
     "fugu_ctx_create", "fugu_ctx_destroy", "fugu_set_params", "fugu_last_error", "fugu_field_size",

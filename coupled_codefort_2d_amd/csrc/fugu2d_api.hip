@@ -120,6 +120,8 @@ struct fugu_ctx {
   double *uadv = nullptr, *vadv = nullptr, *cnew = nullptr, *csA = nullptr, *csB = nullptr, *cs = nullptr;
   double *nxu = nullptr, *nyv = nullptr, *fx = nullptr, *fy = nullptr, *mdu = nullptr, *mdv = nullptr;
   double *penu = nullptr, *penv = nullptr, *Bst = nullptr;
+  double* p_prev = nullptr;   // p^(n-1) for FUGU_SOLVE_EXTRAPOLATE
+  bool have_pprev = false;
   // fine-API staging
   double *su[4] = {}, *sv[4] = {}, *sp[4] = {};
   Work<double> wd;
@@ -135,7 +137,13 @@ struct fugu_ctx {
   int mg_cycles = 1;
   int mg_tail_cells = -1;    // coarse-tail start: < 0 the LDS-pool fit, > 0 a cell count, 0 never
   bool mg_fused = true;
-  bool mg_tail_stage = true; // stage the coarse tail in LDS when it fits      // fused (temporal-blocking) level kernels instead of one launch per half-sweep
+  bool mg_tail_stage = true; // stage the coarse tail in LDS when it fits
+  double fg_est = 0.9;       // FGMRES ends a cycle when its residual estimate is below fg_est * tol
+  // second Gram-Schmidt pass when ||w'||^2 < reorth_frac ||w||^2: only on severe cancellation
+  // (more than 99 % of ||w||^2 removed by the first pass).  With the PCD preconditioner the
+  // Kahan-Parlett 1/2 fires on almost every step without changing a single iteration count or
+  // final residual on the rising bubble (512x1024) or the 2048^2 inclusions, FP64 or mixed.
+  double reorth_frac = 0.01;      // fused (temporal-blocking) level kernels instead of one launch per half-sweep
   double pcd_visc = 0.0; // viscous term of the Schur approximation (0 = the reference PCD)
   // adjoint: solve_fgmres applies A_eff^T and the transposed PCD instead of A and the PCD
   bool adjoint = false;
@@ -510,8 +518,11 @@ size_t mg_tail_footprint(const std::vector<MgLevel<T>>& lv, size_t l) {
 }
 
 // First level of the coarse tail (lv.size(): none).  mg_tail_cells < 0 (default): the highest
-// level whose tail fits the LDS pool; > 0: the first level of at most that many cells; 0: no
-// tail.  At most MG_TAIL_LEVELS levels, and the coarsest level must fit the LDS solve.
+// level of at most MG_TAIL_AUTO_CELLS cells whose tail fits the LDS pool (a single workgroup is
+// slower than the fused level kernels above that size); > 0: the first level of at most that
+// many cells; 0: no tail.  At most MG_TAIL_LEVELS levels, and the coarsest level must fit the
+// LDS solve.
+constexpr int MG_TAIL_AUTO_CELLS = 2048;
 template <typename T>
 size_t mg_tail_start(const fugu_ctx* c, const std::vector<MgLevel<T>>& lv) {
   const size_t nl = lv.size();
@@ -521,7 +532,9 @@ size_t mg_tail_start(const fugu_ctx* c, const std::vector<MgLevel<T>>& lv) {
   if (c->mg_tail_cells > 0) {
     while (ts < nl && (size_t)lv[ts].d.nx * lv[ts].d.ny > (size_t)c->mg_tail_cells) ++ts;
   } else {
-    while (ts + 1 < nl && mg_tail_footprint(lv, ts) * sizeof(T) > (size_t)MG_TAIL_POOL) ++ts;
+    while (ts + 1 < nl && (mg_tail_footprint(lv, ts) * sizeof(T) > (size_t)MG_TAIL_POOL ||
+                           (size_t)lv[ts].d.nx * lv[ts].d.ny > (size_t)MG_TAIL_AUTO_CELLS))
+      ++ts;
   }
   if (ts + MG_TAIL_LEVELS < nl) ts = nl - MG_TAIL_LEVELS;
   return ts;
@@ -572,10 +585,10 @@ void mg_vcycle(fugu_ctx* c, std::vector<MgLevel<T>>& lv, size_t l, bool zero_gue
   const double ncell = (double)L.d.nx * L.d.ny;
   const dim3 gt((unsigned)((L.d.nx + MGT_W - 1) / MGT_W), (unsigned)((L.d.ny + MGT_H - 1) / MGT_H));
   if (c->mg_fused && zero_guess) {
-    // b, D^-1, c_x, c_y (, m) read; phi written; coarse b and phi written (a quarter each)
-    TLAUNCH(c, sizeof(T) == 8 ? "mg_pre_f64" : "mg_pre_f32", (L.m ? 6.5 : 5.5) * sizeof(T) * ncell,
+    // b, c_x, c_y (, m) read; phi written; coarse b and phi written (a quarter each)
+    TLAUNCH(c, sizeof(T) == 8 ? "mg_pre_f64" : "mg_pre_f32", (L.m ? 5.5 : 4.5) * sizeof(T) * ncell,
             (k_mg_pre<T, 2>), gt, dim3(256), c->s, L.d, (const T*)L.cx, (const T*)L.cy, (const T*)L.m,
-            (const T*)L.dinv, (const T*)L.b, L.phi, C.b, C.phi);
+            (const T*)L.b, L.phi, C.b, C.phi);
   } else {
     for (int s = 0; s < nu; ++s) {
       mg_smooth(c, L, 0, (s == 0 && zero_guess) ? 1 : 0);
@@ -588,10 +601,10 @@ void mg_vcycle(fugu_ctx* c, std::vector<MgLevel<T>>& lv, size_t l, bool zero_gue
   }
   mg_vcycle(c, lv, l + 1, true, project);
   if (c->mg_fused) {
-    // phi, coarse phi (a quarter), b, D^-1, c_x, c_y read; phi written
-    TLAUNCH(c, sizeof(T) == 8 ? "mg_post_f64" : "mg_post_f32", 6.25 * sizeof(T) * ncell, (k_mg_post<T, 2>), gt,
-            dim3(256), c->s, L.d, (const T*)L.cx, (const T*)L.cy, (const T*)L.dinv, (const T*)L.b,
-            (const T*)L.phi, (const T*)C.phi, L.phi2);
+    // phi, coarse phi (a quarter), b, c_x, c_y (, m) read; phi written
+    TLAUNCH(c, sizeof(T) == 8 ? "mg_post_f64" : "mg_post_f32", (L.m ? 6.25 : 5.25) * sizeof(T) * ncell,
+            (k_mg_post<T, 2>), gt, dim3(256), c->s, L.d, (const T*)L.cx, (const T*)L.cy, (const T*)L.m,
+            (const T*)L.b, (const T*)L.phi, (const T*)C.phi, L.phi2);
     std::swap(L.phi, L.phi2);
   } else {
     LAUNCH((k_mg_prolong<T>), g2(L.d.nx, L.d.ny), dim3(64, 4), c->s, L.d, (const T*)C.phi, L.phi);
@@ -1042,8 +1055,8 @@ void solve_fgmres(fugu_ctx* c, Work<T>& w, double* X, double dt, double tol, dou
     }
     FUGU_MAXPY_DOT(8) FUGU_MAXPY_DOT(16) FUGU_MAXPY_DOT(32) FUGU_MAXPY_DOT(48)
 #undef FUGU_MAXPY_DOT
-    LAUNCH(k_fold_maxpy_dot, dim3(1), dim3(1024), c->s, (int)nb, nq, stride, c->g.vol, (const double*)c->partials,
-           c->sc, c->fl);
+    LAUNCH(k_fold_maxpy_dot, dim3(1), dim3(1024), c->s, (int)nb, nq, stride, c->g.vol, c->reorth_frac,
+           (const double*)c->partials, c->sc, c->fl);
   };
   // <W, V_i>, i = 0..k -> S_HBUF + off + i (and vol <W, W> -> S_DOT1); pass 2 predicated on F_REORTH
   auto dots_into = [&](const auto* W, auto* const* basis, int k, int off, bool second) {
@@ -1098,7 +1111,7 @@ void solve_fgmres(fugu_ctx* c, Work<T>& w, double* X, double dt, double tol, dou
         dots_into(W, c->kv32.data(), k, 0, false);
         maxpy_dot(W, c->kv32.data(), k + 1);
         maxpy(W, W, c->kv32.data(), k + 1, -1.0, 64, 2);
-        LAUNCH(k_fgmres_step, dim3(1), dim3(1), c->s, k, 0.5 * tol * bn, c->sc, c->fg, c->fl);
+        LAUNCH(k_fgmres_step, dim3(1), dim3(1), c->s, k, c->fg_est * tol * bn, c->sc, c->fg, c->fl);
         lin<float>(c, L_SCALE, c->kv32[k + 1], W, nullptr, nullptr, nullptr, S_INVH, -1, -1, 1.0, false);
       } else {
         if (max_inner > 0) {
@@ -1126,7 +1139,7 @@ void solve_fgmres(fugu_ctx* c, Work<T>& w, double* X, double dt, double tol, dou
         dots_into(W, c->kv.data(), k, 0, false);
         maxpy_dot(W, c->kv.data(), k + 1);
         maxpy(W, W, c->kv.data(), k + 1, -1.0, 64, 2);
-        LAUNCH(k_fgmres_step, dim3(1), dim3(1), c->s, k, 0.5 * tol * bn, c->sc, c->fg, c->fl);
+        LAUNCH(k_fgmres_step, dim3(1), dim3(1), c->s, k, c->fg_est * tol * bn, c->sc, c->fg, c->fl);
         lin<double>(c, L_SCALE, c->kv[k + 1], W, nullptr, nullptr, nullptr, S_INVH, -1, -1, 1.0, false);
       }
       total_inner += it;
@@ -1273,7 +1286,20 @@ int step_impl(fugu_ctx* c, double dt, int step_idx, const fugu_step_opts* o, fug
     a.bdf2 = bdf2 ? 1 : 0; a.has_lid = o->has_lid ? 1 : 0;
     LAUNCH(k_rhs, grid_union(g), dim3(TX, TY), c->s, g, a);
   }
-  // 7. solve (:305-366)
+  // 7. solve (:305-366); initial guess X^n, or 2 X^n - X^(n-1) (u_adv, v_adv are 2u^n - u^(n-1))
+  if (o->solver_flags & FUGU_SOLVE_EXTRAPOLATE) {
+    if (!c->p_prev) c->p_prev = dalloc<double>(c, g.np);
+    if (step_idx >= 2 && c->have_pprev) {
+      HIPCHK(hipMemcpyAsync(c->u, c->uadv, g.nu * sizeof(double), hipMemcpyDeviceToDevice, c->s));
+      HIPCHK(hipMemcpyAsync(c->v, c->vadv, g.nv * sizeof(double), hipMemcpyDeviceToDevice, c->s));
+      LAUNCH(k_extrap, dim3(nblk(g.np)), dim3(256), c->s, c->p, c->p_prev, g.np);
+    } else {
+      HIPCHK(hipMemcpyAsync(c->p_prev, c->p, g.np * sizeof(double), hipMemcpyDeviceToDevice, c->s));
+    }
+    c->have_pprev = true;
+  } else {
+    c->have_pprev = false;
+  }
   bind_props_d(c, pen, amp);
   const int mvflags = MV_ADV | (pen ? MV_PEN : 0) | (amp ? MV_CONS : 0);
   HIPCHK(hipEventRecord(c->ev[1], c->s));
@@ -1604,6 +1630,8 @@ int fugu_ctx_create(const fugu_grid* grid, const fugu_params* prm, fugu_ctx** ou
     if (const char* tc = std::getenv("FUGU_MG_TAIL_CELLS")) c->mg_tail_cells = std::atoi(tc);
     if (const char* fu = std::getenv("FUGU_MG_FUSED")) c->mg_fused = std::atoi(fu) != 0;
     if (const char* ts = std::getenv("FUGU_MG_TAIL_STAGE")) c->mg_tail_stage = std::atoi(ts) != 0;
+    if (const char* fe = std::getenv("FUGU_FGMRES_EST")) c->fg_est = std::atof(fe);
+    if (const char* rf = std::getenv("FUGU_REORTH_FRAC")) c->reorth_frac = std::atof(rf);
     HIPCHK(hipStreamCreateWithFlags(&c->s, hipStreamNonBlocking));
     for (auto& e : c->ev) HIPCHK(hipEventCreate(&e));
     for (auto& e : c->fg_ev) HIPCHK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
@@ -2064,6 +2092,7 @@ int fugu_state_upload(fugu_ctx* c, const double* u, const double* v, const doubl
     const Grid& g = c->g;
     in(c, c->u, u, g.nu); in(c, c->v, v, g.nv); in(c, c->u_prev, u_prev, g.nu); in(c, c->v_prev, v_prev, g.nv);
     in(c, c->p, p, g.np); in(c, c->c, cc, g.np);
+    c->have_pprev = false;
     c->has_solid = (psu && psv);
     if (c->has_solid) { in(c, c->phis_u, psu, g.nu); in(c, c->phis_v, psv, g.nv); }
     sync(c);

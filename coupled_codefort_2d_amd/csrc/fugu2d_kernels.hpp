@@ -608,10 +608,11 @@ __global__ void __launch_bounds__(256) k_maxpy_dot(MaxpyArgs<V, V> a, const doub
     for (int q = 0; q <= NQ; ++q) partials[(size_t)blockIdx.x * (NQ + 1) + q] = acc[q];
   }
 }
-// Fold of k_maxpy_dot: sc[S_DOT0] = vol ||o||^2, fl[F_REORTH] = (||o||^2 < ||w||^2 / 2, the
-// Kahan-Parlett test against sc[S_DOT1] = vol ||w||^2), and the second pass's coefficients
+// Fold of k_maxpy_dot: sc[S_DOT0] = vol ||o||^2, fl[F_REORTH] = (||o||^2 < frac ||w||^2, the
+// Kahan-Parlett test against sc[S_DOT1] = vol ||w||^2, frac = 1/2 its usual threshold), and the
+// second pass's coefficients
 // sc[S_HBUF + 64 + q] = vol <o, x_q> when it runs, zero otherwise.
-__global__ void __launch_bounds__(1024) k_fold_maxpy_dot(int nb, int nq, int stride, double vol,
+__global__ void __launch_bounds__(1024) k_fold_maxpy_dot(int nb, int nq, int stride, double vol, double frac,
                                                          const double* __restrict__ partials,
                                                          double* __restrict__ sc, int* __restrict__ fl) {
   __shared__ double red[MAXPY_MAX + 1];
@@ -627,7 +628,7 @@ __global__ void __launch_bounds__(1024) k_fold_maxpy_dot(int nb, int nq, int str
   __syncthreads();
   if (threadIdx.x == 0) {
     sc[S_DOT0] = red[nq];
-    reorth = (red[nq] < 0.5 * sc[S_DOT1]) ? 1 : 0;
+    reorth = (red[nq] < frac * sc[S_DOT1]) ? 1 : 0;
     fl[F_REORTH] = reorth;
   }
   __syncthreads();
@@ -1428,6 +1429,15 @@ __global__ void k_ab2(double* __restrict__ uadv, double* __restrict__ uprev, con
     const double a = un[k];
     uadv[k] = first ? a : 2.0 * a - uprev[k];
     uprev[k] = a;
+  }
+}
+// x <- 2 x - x_prev and x_prev <- x (old value), elementwise (the extrapolated initial guess).
+__global__ void k_extrap(double* x, double* x_prev, size_t n) {
+  const size_t stride = (size_t)gridDim.x * blockDim.x;
+  for (size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x; k < n; k += stride) {
+    const double a = x[k];
+    x[k] = 2.0 * a - x_prev[k];
+    x_prev[k] = a;
   }
 }
 // mdot = (rho1 - rho2) flux + rho2 u_adv (:383-384)

@@ -329,10 +329,13 @@ struct MgtCells {
 };
 
 // Load the thread's cells (registers) and phi (+ the parent's coarse correction) into LDS.
+// Cells up to ring rc get b and their four conductances (and m), from which D^-1 and the
+// diagonal are formed exactly as k_mg_dinv forms them (no D^-1 array is read); cells beyond
+// ring rc are only ever read as neighbours (their phi) or not at all, and load nothing else.
 template <typename T, int H>
-__device__ inline void mgt_init(const MgDims& f, int i0, int j0, const T* cx, const T* cy, const T* dinv,
-                                const T* b, const T* phi, const T* phic, const T* mass, bool want_dg,
-                                MgtCells<T, H>& m, T (*s)[MgtCells<T, H>::EW]) {
+__device__ inline void mgt_init(const MgDims& f, int i0, int j0, const T* cx, const T* cy, const T* mass,
+                                const T* b, const T* phi, const T* phic, int rc, MgtCells<T, H>& m,
+                                T (*s)[MgtCells<T, H>::EW]) {
   using M = MgtCells<T, H>;
   const int tid = threadIdx.x;
   const int nxc = (f.nx + 1) / 2;
@@ -350,29 +353,37 @@ __device__ inline void mgt_init(const MgDims& f, int i0, int j0, const T* cx, co
     dist = max(dist, H - lj);
     dist = max(dist, lj - (H + MGT_H - 1));
     const bool cell = idx < M::EW * M::EH && i >= 0 && j >= 0;
-    m.ring[k] = cell ? dist : -1;
+    m.ring[k] = (cell && dist <= rc) ? dist : -1;
     m.col[k] = (i + j) & 1;
-    T bv = T(0), dv = T(0), w = T(0), e = T(0), so = T(0), no = T(0), p = T(0), dg = T(0);
+    // raw loads only here (all slots' loads in flight together); D^-1 and masks below
+    T bv = T(0), rw = T(0), re = T(0), rs = T(0), rn = T(0), mm = T(0), p = T(0);
     if (cell) {
       const size_t o = i + (size_t)j * f.nx;
-      bv = b[o];
-      dv = dinv[o];
-      const T rw = cx[i + j * rx], re = cx[i + 1 + j * rx], rs = cy[o], rn = cy[o + f.nx];
-      w = (i > 0 || f.per_x) ? rw : T(0);
-      e = (i < f.nx - 1 || f.per_x) ? re : T(0);
-      so = (j > 0 || f.per_y) ? rs : T(0);
-      no = (j < f.ny - 1 || f.per_y) ? rn : T(0);
-      if (want_dg && dist == 0) {
-        dg = rw + re + rs + rn;
-        if (mass) dg += mass[o];
+      if (dist <= rc) {
+        bv = b[o];
+        rw = cx[i + j * rx]; re = cx[i + 1 + j * rx]; rs = cy[o]; rn = cy[o + f.nx];
+        if (mass) mm = mass[o];
       }
       if (phi) {
         p = phi[o];
         if (phic) p = p + phic[(i >> 1) + (size_t)(j >> 1) * nxc];
       }
     }
-    m.b[k] = bv; m.dinv[k] = dv; m.cW[k] = w; m.cE[k] = e; m.cS[k] = so; m.cN[k] = no; m.dg[k] = dg;
+    m.b[k] = bv; m.cW[k] = rw; m.cE[k] = re; m.cS[k] = rs; m.cN[k] = rn; m.dg[k] = mm;
     pv[k] = p;
+  }
+#pragma unroll
+  for (int k = 0; k < M::NS; ++k) {
+    // diagonal and D^-1 exactly as k_mg_dinv forms them, then the boundary-face masks
+    T dg = m.cW[k] + m.cE[k] + m.cS[k] + m.cN[k];
+    if (mass) dg += m.dg[k];
+    m.dg[k] = dg;
+    m.dinv[k] = (dg > T(1.0e-14)) ? T(1) / dg : T(0);
+    const int i = mgt_wrap(i0 - H + m.li[k], f.nx, f.per_x), j = mgt_wrap(j0 - H + m.lj[k], f.ny, f.per_y);
+    if (!(i > 0 || f.per_x)) m.cW[k] = T(0);
+    if (!(i < f.nx - 1 || f.per_x)) m.cE[k] = T(0);
+    if (!(j > 0 || f.per_y)) m.cS[k] = T(0);
+    if (!(j < f.ny - 1 || f.per_y)) m.cN[k] = T(0);
   }
 #pragma unroll
   for (int k = 0; k < M::NS; ++k)
@@ -412,9 +423,8 @@ __device__ inline void mgt_store(const MgDims& f, int i0, int j0, T* out, const 
 // holding its coefficients and summed from LDS (children in the restriction's order).
 template <typename T, int NU>
 __global__ void __launch_bounds__(256) k_mg_pre(MgDims f, const T* __restrict__ cx, const T* __restrict__ cy,
-                                                const T* __restrict__ m, const T* __restrict__ dinv,
-                                                const T* __restrict__ b, T* __restrict__ phi, T* __restrict__ bc,
-                                                T* __restrict__ phic) {
+                                                const T* __restrict__ m, const T* __restrict__ b,
+                                                T* __restrict__ phi, T* __restrict__ bc, T* __restrict__ phic) {
   constexpr int H = 2 * NU + 1;
   using M = MgtCells<T, H>;
   __shared__ T s[M::EH][M::EW];
@@ -422,7 +432,9 @@ __global__ void __launch_bounds__(256) k_mg_pre(MgDims f, const T* __restrict__ 
   int i0, j0;
   mgt_tile(i0, j0);
   M cells;
-  mgt_init<T, H>(f, i0, j0, cx, cy, dinv, b, (const T*)nullptr, (const T*)nullptr, m, true, cells, s);
+  // rings 0..H-1 are updated (ring H-1 only by the first, neighbour-free half-sweep); ring H is
+  // never read
+  mgt_init<T, H>(f, i0, j0, cx, cy, m, b, (const T*)nullptr, (const T*)nullptr, H - 1, cells, s);
 #pragma unroll
   for (int h = 0; h < 2 * NU; ++h) mgt_half<T, H>(H - 1 - h, h & 1, h == 0 ? 1 : 0, cells, s);
   mgt_store<T, H>(f, i0, j0, phi, s);
@@ -453,7 +465,7 @@ __global__ void __launch_bounds__(256) k_mg_pre(MgDims f, const T* __restrict__ 
 // phi_out = phi + phi_c(parent), then NU post-smoothing sweeps (black first).
 template <typename T, int NU>
 __global__ void __launch_bounds__(256) k_mg_post(MgDims f, const T* __restrict__ cx, const T* __restrict__ cy,
-                                                 const T* __restrict__ dinv, const T* __restrict__ b,
+                                                 const T* __restrict__ m, const T* __restrict__ b,
                                                  const T* __restrict__ phi, const T* __restrict__ phic,
                                                  T* __restrict__ phi_out) {
   constexpr int H = 2 * NU;
@@ -462,7 +474,7 @@ __global__ void __launch_bounds__(256) k_mg_post(MgDims f, const T* __restrict__
   int i0, j0;
   mgt_tile(i0, j0);
   M cells;
-  mgt_init<T, H>(f, i0, j0, cx, cy, dinv, b, phi, phic, (const T*)nullptr, false, cells, s);
+  mgt_init<T, H>(f, i0, j0, cx, cy, m, b, phi, phic, H - 1, cells, s);
 #pragma unroll
   for (int h = 0; h < 2 * NU; ++h) mgt_half<T, H>(H - 1 - h, (h & 1) ? 0 : 1, 0, cells, s);
   mgt_store<T, H>(f, i0, j0, phi_out, s);

@@ -51,7 +51,7 @@ module fugu2d_hip
   ! solver_flags bits of fugu_step_opts (include/fugu2d.h)
   integer(c_int), parameter, public :: FUGU_B2_CONSISTENT_BCS = 1, FUGU_SOLVE_REFINED = 2, &
                                        FUGU_SOLVE_MIXED = 4, FUGU_PRECOND_MG = 8, FUGU_PRECOND_VISC = 16, &
-                                       FUGU_PRECOND_VMG = 32
+                                       FUGU_PRECOND_VMG = 32, FUGU_SOLVE_EXTRAPOLATE = 64
   integer(c_int), parameter, public :: FUGU_KIND_U = 0, FUGU_KIND_V = 1, FUGU_KIND_P = 2, FUGU_KIND_W = 3
 
   type(c_ptr), save :: g_ctx = c_null_ptr

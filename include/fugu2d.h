@@ -77,7 +77,11 @@ enum {
   FUGU_PRECOND_VISC = 1u << 4,
   /* Velocity blocks of the PCD preconditioner by one geometric-multigrid V-cycle each (the
    * same Helmholtz operators) instead of the reference's three damped-Jacobi sweeps. */
-  FUGU_PRECOND_VMG = 1u << 5
+  FUGU_PRECOND_VMG = 1u << 5,
+  /* Initial guess of the step solve extrapolated in time, X0 = 2 X^n - X^(n-1) (u and v are the
+   * AB2 advection velocities; p^(n-1) is kept by the context), instead of X^n.  From the second
+   * consecutive step with the flag on; fugu_state_upload forgets p^(n-1). */
+  FUGU_SOLVE_EXTRAPOLATE = 1u << 6
 };
 
 enum { FUGU_W_MAX_ITER = 1, FUGU_W_NONFINITE = 2 };

@@ -38,20 +38,18 @@ def oracle_true_residual(g, ph, st_before, u, v, p, dt, step_idx):
         fu, fv = g.zeros("u"), g.zeros("v")
     bdf2 = ph.time_order == 2 and step_idx >= 2
     dte = dt / 1.5 if bdf2 else dt
-    nh, nx, ny = g.nh, g.nx, g.ny
-    B = [g.zeros("u"), g.zeros("v"), g.zeros("p")]
-    su = (slice(nh, nh + ny), slice(nh + 1, nh + nx))
-    sv = (slice(nh + 1, nh + ny), slice(nh, nh + nx))
+    # B on the owned rows (u: i = 1..nx-1, or 1..nx when periodic in x; v likewise in y)
     if bdf2:
-        B[0][su] = (pr["rho_u"] / dt * (2 * un - 0.5 * unm1) + pr["rho_u"] * ph.gx + fu)[su]
-        B[1][sv] = (pr["rho_v"] / dt * (2 * vn - 0.5 * vnm1) + pr["rho_v"] * ph.gy + fv)[sv]
+        B = [pr["rho_u"] / dt * (2 * un - 0.5 * unm1) + pr["rho_u"] * ph.gx + fu,
+             pr["rho_v"] / dt * (2 * vn - 0.5 * vnm1) + pr["rho_v"] * ph.gy + fv, g.zeros("p")]
     else:
-        B[0][su] = (pr["rho_u"] / dt * un + pr["rho_u"] * ph.gx + fu)[su]
-        B[1][sv] = (pr["rho_v"] / dt * vn + pr["rho_v"] * ph.gy + fv)[sv]
+        B = [pr["rho_u"] / dt * un + pr["rho_u"] * ph.gx + fu, pr["rho_v"] / dt * vn + pr["rho_v"] * ph.gy + fv,
+             g.zeros("p")]
+    O.mask_owned(g, B)
     X = [u.copy(), v.copy(), p.copy()]
     O.velocity_bcs(g, X[0], X[1])
     Y = O.matvec(g, X, pr, ua, va, dte)
-    R = [b - y for b, y in zip(B, Y)]
+    R = O.mask_owned(g, [b - y for b, y in zip(B, Y)])
     return np.sqrt(O.coupled_dot(g, R, R) / O.coupled_dot(g, B, B))
 
 
