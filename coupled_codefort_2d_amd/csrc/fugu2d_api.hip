@@ -622,10 +622,9 @@ void mg_pressure(fugu_ctx* c, Work<T>& w, T* phi, const T* Zp, bool force_proj =
   const Grid& g = c->g;
   const int walls_both = (force_proj || (!g.per_x && !g.per_y)) ? 1 : 0;
   c->pcg_calls++;
-  HIPCHK(hipMemsetAsync(c->fl + F_P_DONE, 0, sizeof(int), c->s));
   if (walls_both) cell_mean<T>(c, Zp, (int)S_P_MEAN, 0);
   LAUNCH((k_mg_load<T>), g2(g.nx, g.ny), dim3(64, 4), c->s, g, Zp, w.mg[0].b, walls_both, (const double*)c->sc,
-         (int)S_P_MEAN);
+         (int)S_P_MEAN, c->fl, (int)F_P_DONE);
   for (int k = 0; k < c->mg_cycles; ++k) mg_vcycle(c, w.mg, 0, k == 0, 1);   // later cycles smooth from phi
   LAUNCH((k_mg_store<T>), g2(g.nx, g.ny), dim3(64, 4), c->s, g, (const T*)w.mg[0].phi, phi);
   if (walls_both) cell_mean<T>(c, (const T*)phi, (int)S_P_MEAN, 0);
@@ -633,12 +632,13 @@ void mg_pressure(fugu_ctx* c, Work<T>& w, T* phi, const T* Zp, bool force_proj =
 
 // out (padded u or v array) ~= A_comp^-1 rhs by one V-cycle from zero on the velocity block
 // (rows of the block; wall faces copy rhs, ghosts zero, as the Jacobi blocks leave them).
+// loaded: level 0's b already holds the rhs rows (written by the kernel that formed them).
 template <typename T>
-void vmg_solve(fugu_ctx* c, Work<T>& w, int comp, const T* rhs, T* out) {
+void vmg_solve(fugu_ctx* c, Work<T>& w, int comp, const T* rhs, T* out, bool loaded = false) {
   const Grid& g = c->g;
   auto& lv = w.vmg[comp];
   MgLevel<T>& L = lv[0];
-  LAUNCH((k_vmg_load<T>), g2(L.d.nx, L.d.ny), dim3(64, 4), c->s, g, L.d, rhs, comp, L.b);
+  if (!loaded) LAUNCH((k_vmg_load<T>), g2(L.d.nx, L.d.ny), dim3(64, 4), c->s, g, L.d, rhs, comp, L.b);
   mg_vcycle(c, lv, 0, true, 0);
   LAUNCH((k_vmg_store<T>), grid_union(g), dim3(TX, TY), c->s, g, comp, (const T*)L.phi, rhs, out);
 }
@@ -661,18 +661,19 @@ void pcd(fugu_ctx* c, Work<T>& w, const T* Z, T* R, T dt) {
   LAUNCH((k_pcd_rp<T>), gu, bl, c->s, g, w.phi, w.uadv, w.vadv, dt, walls_both, Rp, c->sc, c->fl, Zp, w.kap,
          (T)c->pcd_visc);
   halo(c, Rp, 2);
-  LAUNCH((k_pcd_zwork<T>), gu, bl, c->s, g, Zu, Zv, (const T*)Rp, w.zuw, w.zvw);
+  LAUNCH((k_pcd_zwork<T>), gu, bl, c->s, g, Zu, Zv, (const T*)Rp, w.zuw, w.zvw,
+         c->use_vmg ? w.vmg[1][0].b : (T*)nullptr);
   if (c->use_vmg) {
-    vmg_solve<T>(c, w, 1, w.zvw, Rv);
+    vmg_solve<T>(c, w, 1, w.zvw, Rv, true);
   } else {
     LAUNCH((k_jacobi_v<T>), gu, bl, c->s, g, (const T*)nullptr, w.jv0, w.zvw, w.rho_v, w.kap, w.zeta, w.eta, dt, w.penv);
     LAUNCH((k_jacobi_v<T>), gu, bl, c->s, g, (const T*)w.jv0, w.jv1, w.zvw, w.rho_v, w.kap, w.zeta, w.eta, dt, w.penv);
     LAUNCH((k_jacobi_v<T>), gu, bl, c->s, g, (const T*)w.jv1, Rv, w.zvw, w.rho_v, w.kap, w.zeta, w.eta, dt, w.penv);
   }
   halo(c, Rv, 1);
-  LAUNCH((k_pcd_cross<T>), gc, bl, c->s, g, (const T*)Rv, w.eta, w.zuw);
+  LAUNCH((k_pcd_cross<T>), gc, bl, c->s, g, (const T*)Rv, w.eta, w.zuw, c->use_vmg ? w.vmg[0][0].b : (T*)nullptr);
   if (c->use_vmg) {
-    vmg_solve<T>(c, w, 0, w.zuw, Ru);
+    vmg_solve<T>(c, w, 0, w.zuw, Ru, true);
   } else {
     LAUNCH((k_jacobi_u<T>), gu, bl, c->s, g, (const T*)nullptr, w.ju0, w.zuw, w.rho_u, w.kap, w.zeta, w.eta, dt, w.penu);
     LAUNCH((k_jacobi_u<T>), gu, bl, c->s, g, (const T*)w.ju0, w.ju1, w.zuw, w.rho_u, w.kap, w.zeta, w.eta, dt, w.penu);
@@ -1104,9 +1105,9 @@ void solve_fgmres(fugu_ctx* c, Work<T>& w, double* X, double dt, double tol, dou
         float* z = c->kz32[k];
         pcd<float>(c, c->wf, (const float*)c->kv32[k], z, (float)dt);
         bcs_trip(c, z);
-        // W = A z_k (FP32 operands), then classical Gram-Schmidt as below on FP32 storage
-        HIPCHK(hipMemcpyAsync(c->wf.X, z, n3 * sizeof(float), hipMemcpyDeviceToDevice, c->s));
-        matvec(c, c->wf, c->wf.X, c->wf.z, dt, mvflags, nullptr);
+        // W = A z_k (FP32 operands; the matvec's halo exchange only rewrites z_k's periodic
+        // ghosts with the images the PCD already put there), then Gram-Schmidt on FP32 storage
+        matvec(c, c->wf, z, c->wf.z, dt, mvflags, nullptr);
         float* W = c->wf.z;
         dots_into(W, c->kv32.data(), k, 0, false);
         maxpy_dot(W, c->kv32.data(), k + 1);
@@ -1130,9 +1131,12 @@ void solve_fgmres(fugu_ctx* c, Work<T>& w, double* X, double dt, double tol, dou
         }
         bcs_trip(c, c->kz[k]);
         // W = A z_k, then classical Gram-Schmidt twice, all coefficients device-resident
-        HIPCHK(hipMemcpyAsync(wd.X, c->kz[k], n3 * sizeof(double), hipMemcpyDeviceToDevice, c->s));
-        if (c->adjoint) matvec_T(c, wd.X, wd.z, dt, (mvflags & MV_ADV) ? 1 : 0);
-        else matvec(c, wd, wd.X, wd.z, dt, mvflags, nullptr);
+        if (c->adjoint) {
+          HIPCHK(hipMemcpyAsync(wd.X, c->kz[k], n3 * sizeof(double), hipMemcpyDeviceToDevice, c->s));
+          matvec_T(c, wd.X, wd.z, dt, (mvflags & MV_ADV) ? 1 : 0);
+        } else {
+          matvec(c, wd, c->kz[k], wd.z, dt, mvflags, nullptr);
+        }
         double* W = wd.z;
         // classical Gram-Schmidt, second pass only when the first lost more than half of ||W||^2;
         // the second pass's products come out of the first pass's sweep

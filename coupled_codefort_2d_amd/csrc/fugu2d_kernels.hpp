@@ -1096,11 +1096,12 @@ __global__ void __launch_bounds__(NT) k_pcd_rp(Grid g, const T* __restrict__ phi
 }
 
 // z_u - d r_p/dx on faces i=1..nx-1 (copies at i=0, nx), z_v - d r_p/dy on j=1..ny-1
-// (copies at j=0, ny), zero elsewhere (:66-84, single-rank loop bounds).
+// (copies at j=0, ny), zero elsewhere (:66-84, single-rank loop bounds).  bv (optional): the
+// v block's rows (i = 1..nx, j = 1..j_v_max) also written compactly, the multigrid's rhs.
 template <typename T>
 __global__ void __launch_bounds__(NT) k_pcd_zwork(Grid g, const T* __restrict__ zu, const T* __restrict__ zv,
                                                   const T* __restrict__ rp, T* __restrict__ zuw,
-                                                  T* __restrict__ zvw) {
+                                                  T* __restrict__ zvw, T* __restrict__ bv = nullptr) {
   int i, j;
   union_ij(g, i, j);
   const int nx = g.nx, ny = g.ny, nh = g.nh;
@@ -1119,6 +1120,7 @@ __global__ void __launch_bounds__(NT) k_pcd_zwork(Grid g, const T* __restrict__ 
       else if (j == 0 || j == ny) o = zv[IV(g, i, j)];
     }
     zvw[IV(g, i, j)] = o;
+    if (bv && i >= 1 && i <= nx && j >= 1 && j <= (g.per_y ? ny : ny - 1)) bv[(i - 1) + (size_t)(j - 1) * nx] = o;
   }
 }
 
@@ -1200,11 +1202,28 @@ __global__ void __launch_bounds__(NT) k_jacobi_u(Grid g, const T* __restrict__ u
   u_new[IU(g, i, j)] = out;
 }
 
-// u gets the eta cross term of the solved v block, i=1..nx-1, j=1..ny (:91-98).
+// u gets the eta cross term of the solved v block, i=1..nx-1, j=1..ny (:91-98).  bu (optional):
+// instead of updating z_uw, write the u block's rows (i = 1..i_u_max, j = 1..ny, the cross term
+// on i <= nx-1 only) compactly, the multigrid's rhs.
 template <typename T>
-__global__ void k_pcd_cross(Grid g, const T* __restrict__ rv, const T* __restrict__ eta, T* __restrict__ zuw) {
+__global__ void k_pcd_cross(Grid g, const T* __restrict__ rv, const T* __restrict__ eta, T* __restrict__ zuw,
+                            T* __restrict__ bu = nullptr) {
   int i, j;
   cell_ij(0, i, j);
+  const int i_u_max = g.per_x ? g.nx : g.nx - 1;
+  if (bu) {
+    if (i > i_u_max || j > g.ny) return;
+    const size_t o = IU(g, i, j);
+    T val = zuw[o];
+    if (i <= g.nx - 1) {
+      const T inv_dx = (T)g.inv_dx, inv_dy = (T)g.inv_dy;
+      const T cross = -inv_dy * (eta[IW(g, i, j)] * (rv[IV(g, i + 1, j)] - rv[IV(g, i, j)]) * inv_dx -
+                                 eta[IW(g, i, j - 1)] * (rv[IV(g, i + 1, j - 1)] - rv[IV(g, i, j - 1)]) * inv_dx);
+      val = val - cross;
+    }
+    bu[(i - 1) + (size_t)(j - 1) * i_u_max] = val;
+    return;
+  }
   if (i > g.nx - 1 || j > g.ny) return;
   const T inv_dx = (T)g.inv_dx, inv_dy = (T)g.inv_dy;
   const T cross = -inv_dy * (eta[IW(g, i, j)] * (rv[IV(g, i + 1, j)] - rv[IV(g, i, j)]) * inv_dx -

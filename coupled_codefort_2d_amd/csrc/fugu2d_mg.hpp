@@ -637,12 +637,14 @@ __global__ void __launch_bounds__(1024) k_mg_tail(MgTailArgs<T> a) {
   }
 }
 
-// Level-0 load: b0 = z_p (- mean when both directions are walls), compact layout.
+// Level-0 load: b0 = z_p (- mean when both directions are walls), compact layout; also clears
+// fl[done_flag] (the pressure block's early-exit flag, which the PCD reads after it).
 template <typename T>
 __global__ void k_mg_load(Grid g, const T* __restrict__ zp, T* __restrict__ b0, int walls_both,
-                          const double* __restrict__ sc, int mean_slot) {
+                          const double* __restrict__ sc, int mean_slot, int* __restrict__ fl, int done_flag) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   const int j = blockIdx.y * blockDim.y + threadIdx.y;
+  if (i == 0 && j == 0) fl[done_flag] = 0;
   if (i >= g.nx || j >= g.ny) return;
   const T z = zp[IP(g, i + 1, j + 1)];
   b0[i + (size_t)j * g.nx] = walls_both ? z - (T)sc[mean_slot] : z;
