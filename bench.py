@@ -28,7 +28,6 @@ from __future__ import annotations
 import argparse
 import json
 import os
-import re
 import sys
 import time
 
@@ -150,13 +149,14 @@ def roofline(ktimes, tag="rising_bubble_512x1024"):
            "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
            "bytes_per_launch": round(bytes_launch), "avg_launch_us": round(avg_s * 1e6, 3), "launches": n}
     # measured HBM bytes / algorithmic bytes of this kernel on this workload (two rocprofv3
-    # --pmc runs, FETCH_SIZE and WRITE_SIZE, scaled by a plain copy; hbm_traffic.py)
-    safe = re.sub(r"[^A-Za-z0-9]+", "_", device_name(name)).strip("_")
-    prof = os.path.join(ROOT, "profiles", f"hbm_traffic_{safe}_{tag}.txt")
+    # --pmc runs, FETCH_SIZE and WRITE_SIZE, scaled by a plain copy; hbm_traffic.py), one line
+    # per kernel: "name kernel launches traffic algorithmic ratio fetch_factor write_factor"
+    prof = os.path.join(ROOT, "profiles", f"hbm_traffic_{tag}.txt")
     if os.path.exists(prof):
         for ln in open(prof):
-            if ln.startswith("ratio:"):
-                ratio = float(ln.split()[1])
+            f = ln.split()
+            if f and not ln.startswith("#") and f[0] == name and len(f) >= 6:
+                ratio = float(f[-3])
                 out["traffic"] = round(ratio * bytes_launch)
                 out["traffic_ratio"] = ratio
                 out["traffic_source"] = os.path.relpath(prof, ROOT)
@@ -277,6 +277,8 @@ def main():
     ap.add_argument("--mg-cycles", type=int, default=1)
     ap.add_argument("--inner-max", type=int, default=-1, help="inner BiCGStab(2) cap; -1: PCD only")
     ap.add_argument("--restart", type=int, default=30)
+    ap.add_argument("--sections", default="headline,m1,adjoint",
+                    help="headline (rising bubble, value), m1 (2048 x 2048 FP64 vs mixed), adjoint (s/grad)")
     ap.add_argument("--no-m1", action="store_true", help="skip the 2048 x 2048 FP64-vs-mixed section")
     ap.add_argument("--m1-steps", type=int, default=3)
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint seconds-per-gradient section")
@@ -290,22 +292,28 @@ def main():
         copy_probe(args.copy_probe_mib)
     from coupled_codefort_2d_amd import abi
     abi.load()
+    sections = set(args.sections.split(","))
+    if args.no_m1:
+        sections.discard("m1")
+    if args.no_adjoint:
+        sections.discard("adjoint")
     forms = {}
     base = abi.SOLVE_REFINED | abi.PRECOND_MG | abi.PRECOND_VISC | abi.PRECOND_VMG
     kw = dict(steps=args.steps, warmup=args.warmup, max_outer=args.max_outer, inner_tol=args.inner_tol,
               mg_cycles=args.mg_cycles, inner_max=args.inner_max, restart=args.restart)
-    if "fp64" in args.forms:
-        forms["fp64"] = run_form(abi, base, **kw)
-    if "mixed" in args.forms:
-        forms["mixed"] = run_form(abi, base | abi.SOLVE_MIXED, **kw)
-    if "ref" in args.forms:
-        forms["ref"] = run_reference_form(abi, args.ref_steps, 1, args.ref_max_iter)
+    if "headline" in sections:
+        if "fp64" in args.forms:
+            forms["fp64"] = run_form(abi, base, **kw)
+        if "mixed" in args.forms:
+            forms["mixed"] = run_form(abi, base | abi.SOLVE_MIXED, **kw)
+        if "ref" in args.forms:
+            forms["ref"] = run_reference_form(abi, args.ref_steps, 1, args.ref_max_iter)
     m1 = {}
-    if not args.no_m1:
+    if "m1" in sections:
         kw1 = dict(kw, steps=args.m1_steps, wl=INCL2048)
         m1["fp64"] = run_form(abi, base, **kw1)
         m1["mixed"] = run_form(abi, base | abi.SOLVE_MIXED, **kw1)
-    adj = None if args.no_adjoint else run_adjoint(abi)
+    adj = run_adjoint(abi) if "adjoint" in sections else None
 
     def summary(f):
         sts = f["stats"]
@@ -316,9 +324,10 @@ def main():
                 "inner_iters": [s.iters for s in sts], "outer": [s.outer for s in sts],
                 "res": [float(f"{s.res:.3e}") for s in sts]}
 
-    head = "mixed" if "mixed" in forms else ("fp64" if "fp64" in forms else "ref")
-    main_f = forms[head]
+    head = "mixed" if "mixed" in forms else ("fp64" if "fp64" in forms else ("ref" if "ref" in forms else None))
     rep = {k: summary(v) for k, v in forms.items()}
+    if head is None:   # headline section not run (profiling another section): no value
+        rep[None] = {"solves_per_s": None, "ms_per_step": None}
     all_k = {}
     for f in forms.values():
         for k, v in f["ktimes"].items():   # names carry the precision; sum over forms that share one
@@ -336,7 +345,7 @@ def main():
         "config": {"workload": "rising bubble 512x1024 (1x2 domain), rho 1000/1, mu 10/0.1, sigma 2, g -1, "
                                "BDF2 dt 2e-4, solves to true relative residual 1e-8",
                    "nx": NX, "ny": NY, "form": head},
-        "forms": rep,
+        "forms": {k: v for k, v in rep.items() if k is not None},
         "form_notes": {"fp64": "FGMRES(30) in FP64 + FP64 PCD (multigrid pressure and velocity blocks), "
                                "true residual 1e-8",
                        "mixed": "FGMRES(30) Arnoldi cycles in FP32 storage (FP64 dots and Hessenberg) + FP32 PCD; "
@@ -359,7 +368,7 @@ def main():
         line["adjoint"] = adj
     if not args.no_cpu:
         # iterations the reference solver needs per solve on this workload (GPU drop-in form)
-        ref_iters = float(np.mean(rep["ref"]["inner_iters"])) if "ref" in rep else float(args.ref_max_iter)
+        ref_iters = float(np.mean(rep["ref"]["inner_iters"])) if "ref" in forms else float(args.ref_max_iter)
         line["cpu_baseline"] = cpu_baseline(ref_iters)
     print(json.dumps(line), flush=True)
 

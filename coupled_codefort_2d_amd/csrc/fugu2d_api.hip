@@ -44,9 +44,11 @@ struct Error : std::runtime_error {
 
 // LAUNCH bracketed by timing events when the context's kernel timing is on.
 // Timed launch: HIP events around the kernel on its stream, plus the launch's algorithmic
-// bytes (unique fields read + written once) for the roofline.
+// bytes (unique fields read + written once) for the roofline.  With a launch log open, one
+// line "name bytes 1" per launch (the HBM-counter pairing of hbm_traffic.py).
 #define TLAUNCH(ctx, name, abytes, kern, grid, block, stream, ...)                \
   do {                                                                           \
+    if ((ctx)->launch_log) std::fprintf((ctx)->launch_log, "%s %.0f 1\n", (name), (double)(abytes)); \
     KTimer* kt_ = (ctx)->kt_on ? kt_slot(ctx, name) : nullptr;                   \
     if (kt_) HIPCHK(hipEventRecord(kt_->ev[2 * kt_->used], stream));             \
     LAUNCH(kern, grid, block, stream, __VA_ARGS__);                              \
@@ -138,6 +140,7 @@ struct fugu_ctx {
   int mg_tail_cells = -1;    // coarse-tail start: < 0 the LDS-pool fit, > 0 a cell count, 0 never
   bool mg_fused = true;
   bool mg_tail_stage = true; // stage the coarse tail in LDS when it fits
+  bool pcd_cross = true;     // the PCD's eta cross term from the solved v block into the u block
   double fg_est = 0.9;       // FGMRES ends a cycle when its residual estimate is below fg_est * tol
   // second Gram-Schmidt pass when ||w'||^2 < reorth_frac ||w||^2: only on severe cancellation
   // (more than 99 % of ||w||^2 removed by the first pass).  With the PCD preconditioner the
@@ -671,9 +674,10 @@ void pcd(fugu_ctx* c, Work<T>& w, const T* Z, T* R, T dt) {
     LAUNCH((k_jacobi_v<T>), gu, bl, c->s, g, (const T*)w.jv1, Rv, w.zvw, w.rho_v, w.kap, w.zeta, w.eta, dt, w.penv);
   }
   halo(c, Rv, 1);
-  LAUNCH((k_pcd_cross<T>), gc, bl, c->s, g, (const T*)Rv, w.eta, w.zuw, c->use_vmg ? w.vmg[0][0].b : (T*)nullptr);
+  if (c->pcd_cross || !c->use_vmg)
+    LAUNCH((k_pcd_cross<T>), gc, bl, c->s, g, (const T*)Rv, w.eta, w.zuw, c->use_vmg ? w.vmg[0][0].b : (T*)nullptr);
   if (c->use_vmg) {
-    vmg_solve<T>(c, w, 0, w.zuw, Ru, true);
+    vmg_solve<T>(c, w, 0, w.zuw, Ru, c->pcd_cross);
   } else {
     LAUNCH((k_jacobi_u<T>), gu, bl, c->s, g, (const T*)nullptr, w.ju0, w.zuw, w.rho_u, w.kap, w.zeta, w.eta, dt, w.penu);
     LAUNCH((k_jacobi_u<T>), gu, bl, c->s, g, (const T*)w.ju0, w.ju1, w.zuw, w.rho_u, w.kap, w.zeta, w.eta, dt, w.penu);
@@ -1019,10 +1023,8 @@ void solve_fgmres(fugu_ctx* c, Work<T>& w, double* X, double dt, double tol, dou
     if (nb > (unsigned)RED_MAXB) nb = RED_MAXB;
     double* pa = gs_mode ? c->partials : (double*)nullptr;
     const double ab = (2.0 * sizeof(O) + nq * sizeof(V)) * n3;
-    const char* dev = sizeof(O) == 4 ? "k_maxpy<float, float>"
-                      : (sizeof(V) == 4 ? "k_maxpy<double, float>" : "k_maxpy<double, double>");
     const char* tname = sizeof(O) == 4 ? "maxpy_f32" : (sizeof(V) == 4 ? "maxpy_update" : "maxpy_f64");
-    if (c->launch_log) std::fprintf(c->launch_log, "%s %.0f %d\n", dev, ab, gs_mode == 2 ? 0 : 1);
+    if (gs_mode == 2 && c->launch_log) std::fprintf(c->launch_log, "%s %.0f 0\n", tname, ab);
     if (gs_mode == 2)   // predicated: not timed (its bytes depend on a device-side flag)
       LAUNCH((k_maxpy<O, V>), dim3(nb), dim3(256), c->s, ma, (const double*)c->sc, pa, (const int*)c->fl,
              (int)F_REORTH);
@@ -1048,9 +1050,6 @@ void solve_fgmres(fugu_ctx* c, Work<T>& w, double* X, double dt, double tol, dou
 #define FUGU_MAXPY_DOT(NQ)                                                                                  \
     if (nq <= NQ && !stride) {                                                                             \
       stride = NQ + 1;                                                                                      \
-      if (c->launch_log)                                                                                    \
-        std::fprintf(c->launch_log, "%s %.0f 1\n", sizeof(V) == 4 ? "k_maxpy_dot<float, " #NQ ">"          \
-                                                                  : "k_maxpy_dot<double, " #NQ ">", ab);    \
       TLAUNCH(c, tname, ab, (k_maxpy_dot<V, NQ>), dim3(nb), dim3(256), c->s, ma, (const double*)c->sc,     \
               c->partials);                                                                                 \
     }
@@ -1068,9 +1067,8 @@ void solve_fgmres(fugu_ctx* c, Work<T>& w, double* X, double dt, double tol, dou
     const unsigned nch = (unsigned)((k + 1 + MD_CH - 1) / MD_CH);
     const int flag = second ? (int)F_REORTH : -1;
     const double ab = (k + 2.0) * sizeof(V) * n3;
-    if (c->launch_log)
-      std::fprintf(c->launch_log, "%s %.0f %d\n", sizeof(V) == 4 ? "k_mdot<float>" : "k_mdot<double>", ab,
-                   second ? 0 : 1);
+    if (second && c->launch_log)
+      std::fprintf(c->launch_log, "%s %.0f 0\n", sizeof(V) == 4 ? "mdot_f32" : "mdot_f64", ab);
     if (second) {   // predicated: not timed
       LAUNCH((k_mdot<V>), dim3(MD_BX, nch), dim3(256), c->s, md, c->partials, (const int*)c->fl, flag);
     } else {
@@ -1635,6 +1633,7 @@ int fugu_ctx_create(const fugu_grid* grid, const fugu_params* prm, fugu_ctx** ou
     if (const char* fu = std::getenv("FUGU_MG_FUSED")) c->mg_fused = std::atoi(fu) != 0;
     if (const char* ts = std::getenv("FUGU_MG_TAIL_STAGE")) c->mg_tail_stage = std::atoi(ts) != 0;
     if (const char* fe = std::getenv("FUGU_FGMRES_EST")) c->fg_est = std::atof(fe);
+    if (const char* pc = std::getenv("FUGU_PCD_CROSS")) c->pcd_cross = std::atoi(pc) != 0;
     if (const char* rf = std::getenv("FUGU_REORTH_FRAC")) c->reorth_frac = std::atof(rf);
     HIPCHK(hipStreamCreateWithFlags(&c->s, hipStreamNonBlocking));
     for (auto& e : c->ev) HIPCHK(hipEventCreate(&e));
