@@ -219,33 +219,38 @@ def timed_rate(ktimes):
     return round(nb / (ms * 1e-3) / 1e9, 1) if ms > 0 else None
 
 
-def run_adjoint(abi, n=1024, nsteps=16, dt=5e-5, count=64, seed=4):
-    """Seconds per gradient on the adjoint workload (forward horizon, recorded interface, reverse
-    sweep, property-path gradients), after a 2-step warm-up call on the same context."""
+def run_adjoint(abi, n=1024, nsteps=16, dt=5e-5, count=64, seed=4, rmin=0.01, rmax=0.04, rho_c=100.0,
+                mu_ratio=1e3, sigma=0.5, checkpoint_every=4, mixed=False):
+    """Seconds per gradient on an adjoint workload: the forward horizon with checkpoints (restart
+    states every `checkpoint_every` steps), the reverse sweep with each segment's forward states
+    recomputed, and the property-path gradients; after a 2-step warm-up call on the same
+    context."""
     from coupled_codefort_2d_amd import workloads
-    c, mu, rho, _ = workloads.inclusions(n, count, 0.01, 0.04, seed, 100.0, 1e-2, 1.0, 1e3)
+    c, mu, rho, _ = workloads.inclusions(n, count, rmin, rmax, seed, rho_c, 1e-2, 1.0, mu_ratio)
     ctx = abi.Context(n, n, 1.0, 1.0, False, False,
-                      params=abi.FuguParams(1.0, 1.0, 1.0, 1.0, 0.5, 0.0, -1.0, 2, 1e12, 0.5))
+                      params=abi.FuguParams(1.0, 1.0, 1.0, 1.0, sigma, 0.0, -1.0, 2, 1e12, 0.5))
     u0 = np.zeros((n + 4, n + 5))
     v0 = np.zeros((n + 5, n + 4))
     gm, gr = np.zeros_like(mu), np.zeros_like(rho)
-    flags = abi.ADJ_PRECOND_VMG
-    ctx.adjoint_gradient(abi.FuguAdjointOpts(2, dt, 0.9, 1e-10, 3000, 30, 1, flags), mu, rho, 0.5, c, u0, v0, gm, gr)
+    flags = abi.ADJ_PRECOND_VMG | (abi.ADJ_MIXED_FORWARD if mixed else 0)
+    ctx.adjoint_gradient(abi.FuguAdjointOpts(2, dt, 0.9, 1e-10, 3000, 30, 1, flags), mu, rho, sigma, c, u0, v0, gm, gr)
+    opts = abi.FuguAdjointOpts(nsteps, dt, 0.9, 1e-10, 3000, 30, 1, flags, checkpoint_every)
     t0 = time.perf_counter()
-    J, gs, st, rc = ctx.adjoint_gradient(abi.FuguAdjointOpts(nsteps, dt, 0.9, 1e-10, 3000, 30, 1, flags), mu, rho,
-                                         0.5, c, u0, v0, gm, gr)
+    J, gs, st, rc = ctx.adjoint_gradient(opts, mu, rho, sigma, c, u0, v0, gm, gr)
     wall = time.perf_counter() - t0
     ctx.close()
     print(f"# adjoint {n}x{n}, {nsteps} steps: J {J:.6e} dJ/dsigma {gs:.6e} forward {st.forward_seconds:.2f}s "
           f"reverse {st.reverse_seconds:.2f}s iters {st.forward_iters}/{st.adjoint_iters}", file=sys.stderr, flush=True)
-    return {"workload": f"{n}x{n} unit square, {count} seeded inclusions r~U[0.01,0.04] (seed {seed}), carrier rho 100 "
-                        f"mu 1e-2, inclusions rho 1 mu 1e-2*10^U[0,3], sigma 0.5, g -1; J = kinetic energy in "
-                        f"y >= 0.9 after {nsteps} BDF2 steps (dt {dt}); gradients w.r.t. per-cell mu, rho and sigma",
+    return {"workload": f"{n}x{n} unit square, {count} seeded inclusions r~U[{rmin},{rmax}] (seed {seed}), carrier rho "
+                        f"{rho_c:g} mu 1e-2, inclusions rho 1 mu 1e-2*10^U[0,{np.log10(mu_ratio):g}], sigma {sigma}, "
+                        f"g -1; J = kinetic energy in y >= 0.9 after {nsteps} BDF2 steps (dt {dt}); gradients w.r.t. "
+                        f"per-cell mu, rho and sigma; checkpoints every {checkpoint_every} steps",
             "seconds_per_gradient": round(wall, 3), "forward_seconds": round(st.forward_seconds, 3),
-            "reverse_seconds": round(st.reverse_seconds, 3), "forward_iters": st.forward_iters,
+            "reverse_seconds_incl_recompute": round(st.reverse_seconds, 3), "forward_iters": st.forward_iters,
             "adjoint_iters": st.adjoint_iters, "max_forward_res": float(f"{st.max_forward_res:.3e}"),
             "max_adjoint_res": float(f"{st.max_adjoint_res:.3e}"), "rc": rc, "J": J, "dJ_dsigma": gs,
-            "solves": "FP64 FGMRES(30) to 1e-10, PCD with multigrid pressure and velocity blocks"}
+            "solves": ("mixed (FP32 cycles, FP64 restarts)" if mixed else "FP64") + " forward and FP64 adjoint "
+                      "FGMRES(30) to 1e-10, PCD with multigrid pressure and velocity blocks"}
 
 
 def copy_probe(mib):
@@ -278,7 +283,8 @@ def main():
     ap.add_argument("--inner-max", type=int, default=-1, help="inner BiCGStab(2) cap; -1: PCD only")
     ap.add_argument("--restart", type=int, default=30)
     ap.add_argument("--sections", default="headline,m1,adjoint",
-                    help="headline (rising bubble, value), m1 (2048 x 2048 FP64 vs mixed), adjoint (s/grad)")
+                    help="headline (rising bubble, value), m1 (2048 x 2048 FP64 vs mixed), adjoint (s/grad, "
+                         "1024^2, 16 steps), adjoint2048 (2048^2, 256 inclusions, 32 steps, mixed forward)")
     ap.add_argument("--no-m1", action="store_true", help="skip the 2048 x 2048 FP64-vs-mixed section")
     ap.add_argument("--m1-steps", type=int, default=3)
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint seconds-per-gradient section")
@@ -314,6 +320,10 @@ def main():
         m1["fp64"] = run_form(abi, base, **kw1)
         m1["mixed"] = run_form(abi, base | abi.SOLVE_MIXED, **kw1)
     adj = run_adjoint(abi) if "adjoint" in sections else None
+    adj2048 = None
+    if "adjoint2048" in sections:
+        adj2048 = run_adjoint(abi, n=2048, nsteps=32, count=256, seed=5, rmin=0.005, rmax=0.02, rho_c=1000.0,
+                              sigma=1.0, mixed=True)
 
     def summary(f):
         sts = f["stats"]
@@ -366,6 +376,8 @@ def main():
                       "timed_kernels_GBps": {k: timed_rate(v["ktimes"]) for k, v in m1.items()}}
     if adj:
         line["adjoint"] = adj
+    if adj2048:
+        line["adjoint_2048"] = adj2048
     if not args.no_cpu:
         # iterations the reference solver needs per solve on this workload (GPU drop-in form)
         ref_iters = float(np.mean(rep["ref"]["inner_iters"])) if "ref" in forms else float(args.ref_max_iter)

@@ -44,7 +44,8 @@ class FuguStepStats(C.Structure):
 
 class FuguAdjointOpts(C.Structure):
     _fields_ = [("nsteps", C.c_int), ("dt", C.c_double), ("y0", C.c_double), ("tol", C.c_double),
-                ("max_iter", C.c_int), ("restart", C.c_int), ("mg_cycles", C.c_int), ("flags", C.c_uint)]
+                ("max_iter", C.c_int), ("restart", C.c_int), ("mg_cycles", C.c_int), ("flags", C.c_uint),
+                ("checkpoint_every", C.c_int)]
 
 
 class FuguAdjointStats(C.Structure):
@@ -56,6 +57,7 @@ class FuguAdjointStats(C.Structure):
 ADJ_FORWARD_ONLY = 1
 ADJ_REUSE_INTERFACE = 2
 ADJ_PRECOND_VMG = 4
+ADJ_MIXED_FORWARD = 8
 
 B2_CONSISTENT_BCS = 1
 SOLVE_REFINED = 2

@@ -159,7 +159,11 @@ struct fugu_ctx {
     double *gmu = nullptr, *grho = nullptr;
     bool ready = false;
   } aw;
-  std::vector<double*> traj_u, traj_v, traj_fu, traj_fv;   // forward trajectory of the adjoint horizon
+  std::vector<double*> traj_u, traj_v, traj_fu, traj_fv;   // forward trajectory (or window) of the adjoint horizon
+  struct Checkpoint {
+    double *u1 = nullptr, *u2 = nullptr, *v1 = nullptr, *v2 = nullptr, *p = nullptr, *c = nullptr;
+  };
+  std::vector<Checkpoint> cp;   // restart states of the checkpointed adjoint horizon
   int traj_n = 0;
   bool kt_on = false;
   std::map<std::string, KTimer> kt;
@@ -1383,20 +1387,34 @@ int adjoint_run(fugu_ctx* c, const fugu_adjoint_opts* o, const double* mu_in, co
   if (N < 1 || !(dt > 0.0)) throw Error("adjoint: nsteps >= 1 and dt > 0 required");
   const bool reuse = (o->flags & FUGU_ADJ_REUSE_INTERFACE) != 0;
   const bool fwd_only = (o->flags & FUGU_ADJ_FORWARD_ONLY) != 0;
+  const bool mixed = (o->flags & FUGU_ADJ_MIXED_FORWARD) != 0;
+  // checkpointing: every K-th step's restart state kept, a window of W states recomputed per
+  // segment in the reverse sweep (K <= 1: the whole horizon stays resident)
+  const int K = (o->checkpoint_every > 1 && o->checkpoint_every < N) ? o->checkpoint_every : 0;
+  if (K && reuse) throw Error("adjoint: FUGU_ADJ_REUSE_INTERFACE needs the whole horizon resident (no checkpointing)");
+  const int W = K ? K + 5 : N + 1;
+  auto slot = [&](int n) { return K ? n % W : n; };
   if (reuse && c->traj_n != N) throw Error("adjoint: no recorded interface of this horizon to reuse");
   if (!mu_in || !rho_in || !u0 || !v0 || (!reuse && !c0)) throw Error("adjoint: null input field");
   adj_alloc(c);
   auto& a = c->aw;
   Work<double>& wd = c->wd;
   const size_t n3 = c->ntrip, nu = g.nu, nv = g.nv;
-  while ((int)c->traj_u.size() < N + 1) {
+  while ((int)c->traj_u.size() < W) {
     c->traj_u.push_back(dalloc<double>(c, nu)); c->traj_v.push_back(dalloc<double>(c, nv));
     c->traj_fu.push_back(dalloc<double>(c, nu)); c->traj_fv.push_back(dalloc<double>(c, nv));
   }
-  auto& U = c->traj_u;
-  auto& V = c->traj_v;
-  auto& FU = c->traj_fu;
-  auto& FV = c->traj_fv;
+  const int ncp = K ? (N + K - 1) / K : 0;
+  while ((int)c->cp.size() < ncp) {
+    fugu_ctx::Checkpoint q;
+    q.u1 = dalloc<double>(c, nu); q.u2 = dalloc<double>(c, nu); q.v1 = dalloc<double>(c, nv);
+    q.v2 = dalloc<double>(c, nv); q.p = dalloc<double>(c, g.np); q.c = dalloc<double>(c, g.np);
+    c->cp.push_back(q);
+  }
+  auto Us = [&](int n) { return c->traj_u[slot(n)]; };
+  auto Vs = [&](int n) { return c->traj_v[slot(n)]; };
+  auto FUs = [&](int n) { return c->traj_fu[slot(n)]; };
+  auto FVs = [&](int n) { return c->traj_fv[slot(n)]; };
   const double vol = g.vol;
   const int rm = o->restart > 0 ? o->restart : 30;
   const int mi = o->max_iter > 0 ? o->max_iter : 400;
@@ -1406,50 +1424,55 @@ int adjoint_run(fugu_ctx* c, const fugu_adjoint_opts* o, const double* mu_in, co
   LAUNCH(k_cell_props, grid_union(g), dim3(TX, TY), c->s, g, (const double*)a.mu, (const double*)a.rho, c->rho_c,
          c->rho_u, c->rho_v, c->kap, c->zeta, c->eta);
   bind_props_d(c, false, false);
+  if (mixed) work_alloc<float>(c, c->wf, true);
   c->use_mg = true;
   c->use_vmg = (o->flags & FUGU_ADJ_PRECOND_VMG) != 0;
   c->pcd_visc = 1.0;
   c->mg_cycles = o->mg_cycles > 0 ? o->mg_cycles : 1;
   // initial state (BCs and halos as the end of a step leaves them)
-  in(c, U[0], u0, nu); in(c, V[0], v0, nv);
-  bcs(c, U[0], V[0], 0.0); halo(c, U[0], 0); halo(c, V[0], 1);
+  in(c, Us(0), u0, nu); in(c, Vs(0), v0, nv);
+  bcs(c, Us(0), Vs(0), 0.0); halo(c, Us(0), 0); halo(c, Vs(0), 1);
   if (!reuse) in(c, c->c, c0, g.np);
-  HIPCHK(hipMemcpyAsync(c->u, U[0], nu * sizeof(double), hipMemcpyDeviceToDevice, c->s));
-  HIPCHK(hipMemcpyAsync(c->v, V[0], nv * sizeof(double), hipMemcpyDeviceToDevice, c->s));
+  HIPCHK(hipMemcpyAsync(c->u, Us(0), nu * sizeof(double), hipMemcpyDeviceToDevice, c->s));
+  HIPCHK(hipMemcpyAsync(c->v, Vs(0), nv * sizeof(double), hipMemcpyDeviceToDevice, c->s));
   zero(c, c->p, g.np);
   auto set_uadv = [&](int n) {   // u_adv of step n
     if (n <= 1) {
-      HIPCHK(hipMemcpyAsync(c->uadv, U[0], nu * sizeof(double), hipMemcpyDeviceToDevice, c->s));
-      HIPCHK(hipMemcpyAsync(c->vadv, V[0], nv * sizeof(double), hipMemcpyDeviceToDevice, c->s));
+      HIPCHK(hipMemcpyAsync(c->uadv, Us(0), nu * sizeof(double), hipMemcpyDeviceToDevice, c->s));
+      HIPCHK(hipMemcpyAsync(c->vadv, Vs(0), nv * sizeof(double), hipMemcpyDeviceToDevice, c->s));
     } else {
-      axpby(c, c->uadv, U[n - 1], 2.0, U[n - 2], -1.0, nu);
-      axpby(c, c->vadv, V[n - 1], 2.0, V[n - 2], -1.0, nv);
+      axpby(c, c->uadv, Us(n - 1), 2.0, Us(n - 2), -1.0, nu);
+      axpby(c, c->vadv, Vs(n - 1), 2.0, Vs(n - 2), -1.0, nv);
     }
   };
   auto bdf2_of = [&](int n) { return P.time_order == 2 && n >= 2; };
   auto dte_of = [&](int n) { return bdf2_of(n) ? dt / 1.5 : dt; };
+  auto cpy = [&](double* dst, const double* src, size_t n) {
+    HIPCHK(hipMemcpyAsync(dst, src, n * sizeof(double), hipMemcpyDeviceToDevice, c->s));
+  };
   int fwd_it = 0, adj_it = 0;
   double fwd_res = 0.0, adj_res = 0.0;
-  HIPCHK(hipEventRecord(c->ev[0], c->s));
-  for (int n = 1; n <= N; ++n) {
+  // one forward step n from the state (c->u, c->v, c->p, c->c) = step n-1; stores u^n, v^n and
+  // the unit-sigma force of step n in their slots
+  auto fwd_step = [&](int n, bool count) {
     set_uadv(n);
     if (!reuse) {
-      vof(c, c->c, c->cnew, U[n - 1], V[n - 1], dt, nullptr, nullptr);
+      vof(c, c->c, c->cnew, Us(n - 1), Vs(n - 1), dt, nullptr, nullptr);
       std::swap(c->c, c->cnew);
       if (sigma > 1.0e-14) {
         const double delta = 1.5 * (g.dx < g.dy ? g.dx : g.dy);
         svof(c, c->c, c->cs, delta, 150, 1.0e-7, nullptr);
         curvature(c, c->cs, c->kcurv);
         LAUNCH(k_csf, grid_union(g), dim3(TX, TY), c->s, g, (const double*)c->c, (const double*)c->kcurv, 1.0,
-               FU[n], FV[n]);
+               FUs(n), FVs(n));
       } else {
-        zero(c, FU[n], nu); zero(c, FV[n], nv);
+        zero(c, FUs(n), nu); zero(c, FVs(n), nv);
       }
     }
-    axpby(c, c->fu, FU[n], sigma, nullptr, 0.0, nu);
-    axpby(c, c->fv, FV[n], sigma, nullptr, 0.0, nv);
+    axpby(c, c->fu, FUs(n), sigma, nullptr, 0.0, nu);
+    axpby(c, c->fv, FVs(n), sigma, nullptr, 0.0, nv);
     RhsArgs r;
-    r.un = U[n - 1]; r.unm1 = (n >= 2) ? U[n - 2] : U[0]; r.vn = V[n - 1]; r.vnm1 = (n >= 2) ? V[n - 2] : V[0];
+    r.un = Us(n - 1); r.unm1 = (n >= 2) ? Us(n - 2) : Us(0); r.vn = Vs(n - 1); r.vnm1 = (n >= 2) ? Vs(n - 2) : Vs(0);
     r.rho_u = c->rho_u; r.rho_v = c->rho_v; r.fu = c->fu; r.fv = c->fv; r.penu = nullptr; r.penv = nullptr;
     r.zeta = c->zeta; r.eta = c->eta;
     r.Bu = wd.B; r.Bv = wd.B + nu; r.Bp = wd.B + nu + nv;
@@ -1458,35 +1481,63 @@ int adjoint_run(fugu_ctx* c, const fugu_adjoint_opts* o, const double* mu_in, co
     LAUNCH(k_rhs, grid_union(g), dim3(TX, TY), c->s, g, r);
     int it = 0, outer = 0;
     double res = 0.0;
-    solve_fgmres<double>(c, wd, c->X, dte_of(n), tol, 0.0, mi, 0, rm, MV_ADV, false, 0.0, &it, &outer, &res);
-    fwd_it += outer;
-    fwd_res = std::max(fwd_res, res);
+    if (mixed) {
+      bind_props_f(c, false, false);
+      solve_fgmres<float>(c, c->wf, c->X, dte_of(n), tol, 0.0, mi, 0, rm, MV_ADV, false, 0.0, &it, &outer, &res);
+    } else {
+      solve_fgmres<double>(c, wd, c->X, dte_of(n), tol, 0.0, mi, 0, rm, MV_ADV, false, 0.0, &it, &outer, &res);
+    }
+    if (count) {
+      fwd_it += outer;
+      fwd_res = std::max(fwd_res, res);
+    }
     // u^n = E X^n
     mask_trip(c, c->X);
     bcs(c, c->u, c->v, 0.0); halo(c, c->u, 0); halo(c, c->v, 1); halo(c, c->p, 2);
-    HIPCHK(hipMemcpyAsync(U[n], c->u, nu * sizeof(double), hipMemcpyDeviceToDevice, c->s));
-    HIPCHK(hipMemcpyAsync(V[n], c->v, nv * sizeof(double), hipMemcpyDeviceToDevice, c->s));
+    cpy(Us(n), c->u, nu);
+    cpy(Vs(n), c->v, nv);
+  };
+  // restart state of step n (the state after step n-1): u^(n-1), u^(n-2), v alike, p^(n-1), C^(n-1)
+  auto save_cp = [&](int n) {
+    fugu_ctx::Checkpoint& q = c->cp[(n - 1) / K];
+    cpy(q.u1, Us(n - 1), nu); cpy(q.u2, Us(n >= 2 ? n - 2 : 0), nu);
+    cpy(q.v1, Vs(n - 1), nv); cpy(q.v2, Vs(n >= 2 ? n - 2 : 0), nv);
+    cpy(q.p, c->p, g.np); cpy(q.c, c->c, g.np);
+  };
+  auto load_cp = [&](int n) {
+    const fugu_ctx::Checkpoint& q = c->cp[(n - 1) / K];
+    cpy(Us(n - 1), q.u1, nu); cpy(Vs(n - 1), q.v1, nv);
+    if (n >= 2) { cpy(Us(n - 2), q.u2, nu); cpy(Vs(n - 2), q.v2, nv); }
+    cpy(c->u, q.u1, nu); cpy(c->v, q.v1, nv); cpy(c->p, q.p, g.np); cpy(c->c, q.c, g.np);
+  };
+  HIPCHK(hipEventRecord(c->ev[0], c->s));
+  for (int n = 1; n <= N; ++n) {
+    if (K && (n - 1) % K == 0) save_cp(n);
+    fwd_step(n, true);
   }
   // J = 1/2 coupled_dot((band rho u, band rho v, 0), (u, v, 0))
   LAUNCH(k_band_mask, grid_union(g), dim3(TX, TY), c->s, g, o->y0, a.bmu, a.bmv);
   zero(c, a.y, n3); zero(c, a.t2, n3);
-  acc_prod(c, a.y, 1.0, a.bmu, c->rho_u, U[N], nu);
-  acc_prod(c, a.y + nu, 1.0, a.bmv, c->rho_v, V[N], nv);
-  HIPCHK(hipMemcpyAsync(a.t2, U[N], nu * sizeof(double), hipMemcpyDeviceToDevice, c->s));
-  HIPCHK(hipMemcpyAsync(a.t2 + nu, V[N], nv * sizeof(double), hipMemcpyDeviceToDevice, c->s));
+  acc_prod(c, a.y, 1.0, a.bmu, c->rho_u, Us(N), nu);
+  acc_prod(c, a.y + nu, 1.0, a.bmv, c->rho_v, Vs(N), nv);
+  HIPCHK(hipMemcpyAsync(a.t2, Us(N), nu * sizeof(double), hipMemcpyDeviceToDevice, c->s));
+  HIPCHK(hipMemcpyAsync(a.t2 + nu, Vs(N), nv * sizeof(double), hipMemcpyDeviceToDevice, c->s));
   reset_flags(c);
   cdot1<double>(c, a.y, a.t2, PH_STORE);
   read_slots(c);
   const double J = 0.5 * c->h_sc[S_DOT0];
   HIPCHK(hipEventRecord(c->ev[1], c->s));
-  c->traj_n = N;
+  c->traj_n = K ? 0 : N;   // a checkpointed horizon is not kept whole: nothing to reuse
   double gs = 0.0;
   if (!fwd_only) {
     zero(c, a.grc, g.np); zero(c, a.gru, nu); zero(c, a.grv, nv); zero(c, a.gk, g.np);
     zero(c, a.gz, g.nw); zero(c, a.ge, g.nw); zero(c, a.gmu, g.np); zero(c, a.grho, g.np);
+    // explicit dJ/drho_face (first: u^N may leave the window when segments are recomputed)
+    acc_prod(c, a.gru, 0.5 * vol, a.bmu, Us(N), Us(N), nu);
+    acc_prod(c, a.grv, 0.5 * vol, a.bmv, Vs(N), Vs(N), nv);
     auto coef_grad = [&](const double* L, int n, double s, int what, double* gua, double* gva) {
       CoefGradArgs q;
-      q.Lu = L; q.Lv = L + nu; q.Xu = U[n]; q.Xv = V[n]; q.uadv = c->uadv; q.vadv = c->vadv;
+      q.Lu = L; q.Lv = L + nu; q.Xu = Us(n); q.Xv = Vs(n); q.uadv = c->uadv; q.vadv = c->vadv;
       q.rho_c = c->rho_c; q.rho_u = c->rho_u; q.rho_v = c->rho_v; q.kap = c->kap; q.zeta = c->zeta; q.eta = c->eta;
       q.grc = a.grc; q.gru = a.gru; q.grv = a.grv; q.gk = a.gk; q.gz = a.gz; q.ge = a.ge; q.gua = gua; q.gva = gva;
       q.dt = dte_of(n); q.s = s; q.adv = 1; q.what = what;
@@ -1494,11 +1545,20 @@ int adjoint_run(fugu_ctx* c, const fugu_adjoint_opts* o, const double* mu_in, co
     };
     c->adjoint = true;
     for (int n = N; n >= 1; --n) {
+      // checkpointed horizon: on entering a segment [a, b] (below the last), recompute its
+      // forward states from the checkpoint at a; states b+1, b+2 are still in the window
+      if (K && n % K == 0 && n < N) {
+        const int sa = n - K + 1;
+        c->adjoint = false;
+        load_cp(sa);
+        for (int q = sa; q <= n; ++q) fwd_step(q, false);
+        c->adjoint = true;
+      }
       double* G = a.G;
       zero(c, G, n3);
       if (n == N) {
-        acc_prod(c, G, vol, a.bmu, c->rho_u, U[N], nu);
-        acc_prod(c, G + nu, vol, a.bmv, c->rho_v, V[N], nv);
+        acc_prod(c, G, vol, a.bmu, c->rho_u, Us(N), nu);
+        acc_prod(c, G + nu, vol, a.bmv, c->rho_v, Vs(N), nv);
       }
       for (int m = n + 1; m <= n + 2 && m <= N; ++m) {
         const double* L = a.lam[m % 3];
@@ -1533,27 +1593,25 @@ int adjoint_run(fugu_ctx* c, const fugu_adjoint_opts* o, const double* mu_in, co
       coef_grad(Ln, n, -1.0, CG_COEF, nullptr, nullptr);
       // + lam^T dB/dcoef: rho_u (hist / dt + g_x), rho_v alike
       const bool b2 = bdf2_of(n);
-      const double* um2 = (n >= 2) ? U[n - 2] : U[0];
-      const double* vm2 = (n >= 2) ? V[n - 2] : V[0];
-      axpby(c, a.h, U[n - 1], b2 ? 2.0 : 1.0, b2 ? um2 : nullptr, -0.5, nu);
-      axpby(c, a.h + nu, V[n - 1], b2 ? 2.0 : 1.0, b2 ? vm2 : nullptr, -0.5, nv);
+      const double* um2 = (n >= 2) ? Us(n - 2) : Us(0);
+      const double* vm2 = (n >= 2) ? Vs(n - 2) : Vs(0);
+      axpby(c, a.h, Us(n - 1), b2 ? 2.0 : 1.0, b2 ? um2 : nullptr, -0.5, nu);
+      axpby(c, a.h + nu, Vs(n - 1), b2 ? 2.0 : 1.0, b2 ? vm2 : nullptr, -0.5, nv);
       acc_prod(c, a.gru, 1.0 / dt, nullptr, Ln, a.h, nu);
       acc_prod(c, a.grv, 1.0 / dt, nullptr, Ln + nu, a.h + nu, nv);
       acc_prod(c, a.gru, P.gx, nullptr, Ln, nullptr, nu);
       acc_prod(c, a.grv, P.gy, nullptr, Ln + nu, nullptr, nv);
       // sigma: lam^T fhat
       zero(c, a.t2, n3);
-      HIPCHK(hipMemcpyAsync(a.t2, FU[n], nu * sizeof(double), hipMemcpyDeviceToDevice, c->s));
-      HIPCHK(hipMemcpyAsync(a.t2 + nu, FV[n], nv * sizeof(double), hipMemcpyDeviceToDevice, c->s));
+      HIPCHK(hipMemcpyAsync(a.t2, FUs(n), nu * sizeof(double), hipMemcpyDeviceToDevice, c->s));
+      HIPCHK(hipMemcpyAsync(a.t2 + nu, FVs(n), nv * sizeof(double), hipMemcpyDeviceToDevice, c->s));
       reset_flags(c);
       cdot1<double>(c, Ln, a.t2, PH_STORE);
       read_slots(c);
       gs += c->h_sc[S_DOT0] / vol;
     }
     c->adjoint = false;
-    // explicit dJ/drho_face, then the property map
-    acc_prod(c, a.gru, 0.5 * vol, a.bmu, U[N], U[N], nu);
-    acc_prod(c, a.grv, 0.5 * vol, a.bmv, V[N], V[N], nv);
+    // the property map
     LAUNCH(k_cell_props_T, grid_union(g), dim3(TX, TY), c->s, g, (const double*)a.mu, (const double*)a.grc,
            (const double*)a.gru, (const double*)a.grv, (const double*)a.gk, (const double*)a.gz,
            (const double*)a.ge, a.gmu, a.grho);

@@ -272,6 +272,9 @@ typedef struct {
   int restart;          /* FGMRES restart (<= 0: 30) */
   int mg_cycles;        /* V-cycles per preconditioner application (<= 0: 1) */
   unsigned flags;       /* FUGU_ADJ_* */
+  int checkpoint_every; /* > 1: keep the restart state of every checkpoint_every-th step only and
+                           recompute each segment's forward states in the reverse sweep (bit-identical
+                           to keeping them all); <= 1: the whole horizon stays resident */
 } fugu_adjoint_opts;
 enum {
   FUGU_ADJ_FORWARD_ONLY = 1u << 0,     /* J only (no reverse sweep) */
@@ -281,7 +284,10 @@ enum {
   FUGU_ADJ_REUSE_INTERFACE = 1u << 1,
   /* velocity blocks of the forward and transposed preconditioners by multigrid V-cycles
    * (FUGU_PRECOND_VMG) */
-  FUGU_ADJ_PRECOND_VMG = 1u << 2
+  FUGU_ADJ_PRECOND_VMG = 1u << 2,
+  /* forward solves by the mixed form (FP32 Arnoldi cycles, FP64 true residual and update at
+   * every restart: FUGU_SOLVE_MIXED); the adjoint solves stay FP64 */
+  FUGU_ADJ_MIXED_FORWARD = 1u << 3
 };
 typedef struct {
   double J;

@@ -199,3 +199,50 @@ def test_adjoint_gradient_central_differences_on_gpu():
     f = (Jat(case.mu, case.rho, case.sigma + eps) - Jat(case.mu, case.rho, case.sigma - eps)) / (2 * eps)
     assert abs(f - gs) <= 1e-6 * abs(f), (f, gs)
     ctx.close()
+
+
+def test_checkpointed_horizon_matches_resident_horizon():
+    """Checkpointing (restart states every 3rd step, segments recomputed in the reverse sweep)
+    gives the resident horizon's J bit for bit and its gradients to rounding (the coefficient
+    gradients accumulate with atomics, so their order is not fixed)."""
+    case = make_case(nsteps=7)
+    ctx = gpu_ctx(case)
+    out = {}
+    for k in (0, 3):
+        gm, gr = np.zeros_like(case.mu), np.zeros_like(case.rho)
+        o = gpu_opts(case, abi.ADJ_PRECOND_VMG)
+        o.checkpoint_every = k
+        J, gs, st, rc = ctx.adjoint_gradient(o, case.mu, case.rho, case.sigma, case.c0, case.u0, case.v0, gm, gr)
+        assert rc == 0, (k, st.max_forward_res, st.max_adjoint_res)
+        out[k] = (J, gs, gm, gr)
+    (J0, s0, m0, r0), (J3, s3, m3, r3) = out[0], out[3]
+    assert J3 == J0
+    assert abs(s3 - s0) <= 1e-12 * abs(s0)
+    assert np.abs(m3 - m0).max() <= 1e-12 * np.abs(m0).max()
+    assert np.abs(r3 - r0).max() <= 1e-12 * np.abs(r0).max()
+    # reusing the interface needs the whole horizon
+    fwd = gpu_opts(case, abi.ADJ_FORWARD_ONLY | abi.ADJ_REUSE_INTERFACE)
+    fwd.checkpoint_every = 3
+    with pytest.raises(Exception, match="whole horizon"):
+        ctx.adjoint_gradient(fwd, case.mu, case.rho, case.sigma, None, case.u0, case.v0)
+    ctx.close()
+
+
+def test_mixed_forward_gradient_matches_fp64_forward():
+    """Forward solves by the mixed form (FP32 Arnoldi cycles, FP64 restarts) to the same 1e-12
+    true residual: J and the gradients agree with the FP64 forward to the solve tolerance."""
+    case = make_case(nsteps=4)
+    ctx = gpu_ctx(case)
+    out = {}
+    for fl in (0, abi.ADJ_MIXED_FORWARD):
+        gm, gr = np.zeros_like(case.mu), np.zeros_like(case.rho)
+        J, gs, st, rc = ctx.adjoint_gradient(gpu_opts(case, fl | abi.ADJ_PRECOND_VMG), case.mu, case.rho, case.sigma,
+                                             case.c0, case.u0, case.v0, gm, gr)
+        assert rc == 0, (fl, st.max_forward_res, st.max_adjoint_res)
+        out[fl] = (J, gs, gm, gr)
+    (J0, s0, m0, r0), (J1, s1, m1, r1) = out[0], out[abi.ADJ_MIXED_FORWARD]
+    assert abs(J1 - J0) <= 1e-9 * abs(J0), (J1, J0)
+    assert abs(s1 - s0) <= 1e-7 * abs(s0)
+    assert np.abs(m1 - m0).max() <= 1e-7 * np.abs(m0).max()
+    assert np.abs(r1 - r0).max() <= 1e-7 * np.abs(r0).max()
+    ctx.close()
