@@ -69,13 +69,14 @@ struct MgLevel {
 
 // A multigrid hierarchy from the finest extents down (mass: allocate the mass terms).
 template <typename T>
-std::vector<MgLevel<T>> mg_levels(fugu_ctx* c, MgDims d, bool mass);
+std::vector<MgLevel<T>> mg_levels(fugu_ctx* c, MgDims d, bool mass, int max_levels = 0);
 
 // Per-precision solver workspace.  Triplets are contiguous [u | v | p].
 template <typename T>
 struct Work {
   std::vector<MgLevel<T>> mg;       // pressure multigrid hierarchy (chosen fast forms)
   std::vector<MgLevel<T>> vmg[2];   // u and v velocity-block hierarchies (allocated on first use)
+  size_t vmg_nl[2] = {0, 0};        // their levels in use (cut where the mass term dominates)
   T *X = nullptr, *B = nullptr, *bh = nullptr, *r = nullptr, *rt = nullptr, *uu = nullptr;
   T *vv = nullptr, *ww = nullptr, *ss = nullptr, *tt = nullptr, *z = nullptr, *q = nullptr;
   // properties (aliases of the context's FP64 fields for T = double)
@@ -140,7 +141,9 @@ struct fugu_ctx {
   int mg_tail_cells = -1;    // coarse-tail start: < 0 the LDS-pool fit, > 0 a cell count, 0 never
   bool mg_fused = true;
   bool mg_tail_stage = true; // stage the coarse tail in LDS when it fits
-  bool pcd_cross = true;     // the PCD's eta cross term from the solved v block into the u block
+  bool pcd_cross = true;
+  int vmg_levels = 0;        // velocity hierarchies: at most this many levels (0: coarsen fully)
+  double vmg_theta = 3.0;    // velocity hierarchies cut where diffusion / mass < vmg_theta (0: never)     // the PCD's eta cross term from the solved v block into the u block
   double fg_est = 0.9;       // FGMRES ends a cycle when its residual estimate is below fg_est * tol
   // second Gram-Schmidt pass when ||w'||^2 < reorth_frac ||w||^2: only on severe cancellation
   // (more than 99 % of ||w||^2 removed by the first pass).  With the PCD preconditioner the
@@ -436,7 +439,7 @@ inline dim3 g2(int nx, int ny, int bx = 64, int by = 4) {
 }
 
 template <typename T>
-std::vector<MgLevel<T>> mg_levels(fugu_ctx* c, MgDims d, bool mass) {
+std::vector<MgLevel<T>> mg_levels(fugu_ctx* c, MgDims d, bool mass, int max_levels) {
   std::vector<MgLevel<T>> lv;
   for (;;) {
     MgLevel<T> L;
@@ -449,7 +452,7 @@ std::vector<MgLevel<T>> mg_levels(fugu_ctx* c, MgDims d, bool mass) {
     L.b = dalloc<T>(c, n);
     L.phi = dalloc<T>(c, n);
     L.phi2 = dalloc<T>(c, n);
-    const bool last = !mg_can_coarsen(d);
+    const bool last = !mg_can_coarsen(d) || (max_levels > 0 && (int)lv.size() + 1 >= max_levels);
     if (last && n <= (size_t)MG_DENSE_MAX) L.cinv = dalloc<T>(c, n * n);
     lv.push_back(L);
     if (last) break;
@@ -460,19 +463,21 @@ std::vector<MgLevel<T>> mg_levels(fugu_ctx* c, MgDims d, bool mass) {
 
 // Coarse operators and inverse diagonals below a level 0 whose c_x, c_y (and m) are set.
 template <typename T>
-void mg_galerkin(fugu_ctx* c, std::vector<MgLevel<T>>& lv) {
+void mg_galerkin(fugu_ctx* c, std::vector<MgLevel<T>>& lv, size_t nl) {
   const dim3 bl(64, 4);
-  for (size_t l = 0; l + 1 < lv.size(); ++l) {
+  for (size_t l = 0; l + 1 < nl; ++l) {
     const MgDims& f = lv[l].d;
     const MgDims& C = lv[l + 1].d;
     LAUNCH((k_mg_coarsen<T>), g2(C.nx + 1, C.ny + 1), bl, c->s, f, (const T*)lv[l].cx, (const T*)lv[l].cy,
            lv[l + 1].cx, lv[l + 1].cy);
     if (lv[l].m) LAUNCH((k_mg_coarsen_m<T>), g2(C.nx, C.ny), bl, c->s, f, (const T*)lv[l].m, lv[l + 1].m);
   }
-  for (auto& L : lv)
+  for (size_t l = 0; l < nl; ++l) {
+    MgLevel<T>& L = lv[l];
     LAUNCH((k_mg_dinv<T>), g2(L.d.nx, L.d.ny), bl, c->s, L.d, (const T*)L.cx, (const T*)L.cy, (const T*)L.m,
            L.dinv);
-  const MgLevel<T>& C = lv.back();
+  }
+  const MgLevel<T>& C = lv[nl - 1];
   if (C.cinv)   // singular (pressure) block: no mass term
     LAUNCH((k_mg_coarse_inv<T>), dim3(1), dim3(1024), c->s, C.d, (const T*)C.cx, (const T*)C.cy, (const T*)C.m,
            C.m ? 0 : 1, C.cinv);
@@ -484,10 +489,15 @@ void mg_setup(fugu_ctx* c, Work<T>& w) {
   if (mg_odd_periodic(w.mg[0].d))
     throw Error("multigrid preconditioner: a periodic direction needs an even cell count");
   LAUNCH((k_mg_coef0<T>), g2(g.nx + 1, g.ny + 1), dim3(64, 4), c->s, g, w.rho_u, w.rho_v, w.mg[0].cx, w.mg[0].cy);
-  mg_galerkin(c, w.mg);
+  mg_galerkin(c, w.mg, w.mg.size());
 }
 
 // Velocity-block hierarchies from the current properties and dt (allocated on first use).
+// Depth: a coarse level's diffusion-to-mass ratio is a quarter of its parent's (conductances
+// c_H ~ c, mass m_H = 4m), so once the finest ratio r0 = max (sum c) / m has dropped below
+// vmg_theta the level is mass-dominated and two red-black sweeps solve it well enough: the
+// hierarchy is cut at 1 + ceil(log4(r0 / vmg_theta)) levels (one host read per set-up, so the
+// depth is a function of the current properties alone and recomputed steps repeat bit for bit).
 template <typename T>
 void vmg_setup(fugu_ctx* c, Work<T>& w, T dt) {
   const Grid& g = c->g;
@@ -495,12 +505,33 @@ void vmg_setup(fugu_ctx* c, Work<T>& w, T dt) {
     if (mg_odd_periodic(vmg_dims(g, comp)))
       throw Error("velocity multigrid: a periodic direction needs an even cell count");
     auto& lv = w.vmg[comp];
-    if (lv.empty()) lv = mg_levels<T>(c, vmg_dims(g, comp), true);
+    if (lv.empty()) lv = mg_levels<T>(c, vmg_dims(g, comp), true, c->vmg_levels);
     MgLevel<T>& L = lv[0];
     LAUNCH((k_vmg_coef0<T>), g2(L.d.nx + 1, L.d.ny + 1), dim3(64, 4), c->s, g, L.d, comp,
            comp == 0 ? (const T*)w.rho_u : (const T*)w.rho_v, (const T*)w.kap, (const T*)w.zeta, (const T*)w.eta,
            dt, comp == 0 ? (const T*)w.penu : (const T*)w.penv, L.cx, L.cy, L.m);
-    mg_galerkin(c, lv);
+  }
+  double r0[2] = {1.0e300, 1.0e300};
+  if (c->vmg_theta > 0.0) {
+    const MgLevel<T>& U = w.vmg[0][0];
+    const MgLevel<T>& V = w.vmg[1][0];
+    const unsigned nb = std::min<unsigned>(nblk(std::max((size_t)U.d.nx * U.d.ny, (size_t)V.d.nx * V.d.ny)),
+                                           (unsigned)RED_MAXB);
+    LAUNCH((k_vmg_ratio<T>), dim3(nb), dim3(256), c->s, U.d, (const T*)U.cx, (const T*)U.cy, (const T*)U.m, V.d,
+           (const T*)V.cx, (const T*)V.cy, (const T*)V.m, c->partials);
+    LAUNCH(k_fold_absmax, dim3(1), dim3(NT), c->s, (int)nb, (const double*)c->partials, c->sc);
+    read_slots(c);
+    r0[0] = c->h_sc[S_DOT0];
+    r0[1] = c->h_sc[S_DOT1];
+  }
+  for (int comp = 0; comp < 2; ++comp) {
+    auto& lv = w.vmg[comp];
+    size_t nl = 1;
+    for (double r = r0[comp]; r > c->vmg_theta && nl < lv.size(); r *= 0.25) ++nl;
+    w.vmg_nl[comp] = nl;
+    if (c->debug) std::fprintf(stderr, "[fugu] velocity block %d: ratio %.3g, %zu of %zu levels\n", comp, r0[comp], nl,
+                               lv.size());
+    mg_galerkin(c, lv, nl);
   }
 }
 
@@ -513,11 +544,11 @@ void mg_smooth(fugu_ctx* c, MgLevel<T>& L, int colour, int zero_other) {
           c->s, L.d, colour, (const T*)L.cx, (const T*)L.cy, (const T*)L.dinv, (const T*)L.b, L.phi, zero_other);
 }
 
-// LDS elements the tail needs from level l down (coefficients, mass, D^-1, rhs, iterate).
+// LDS elements the tail needs from level l down to nl-1 (coefficients, mass, D^-1, rhs, iterate).
 template <typename T>
-size_t mg_tail_footprint(const std::vector<MgLevel<T>>& lv, size_t l) {
+size_t mg_tail_footprint(const std::vector<MgLevel<T>>& lv, size_t l, size_t nl) {
   size_t e = 0;
-  for (size_t q = l; q < lv.size(); ++q) {
+  for (size_t q = l; q < nl; ++q) {
     const size_t nx = lv[q].d.nx, ny = lv[q].d.ny;
     e += (nx + 1) * ny + nx * (ny + 1) + (lv[q].m ? 4 : 3) * nx * ny;
   }
@@ -531,15 +562,14 @@ size_t mg_tail_footprint(const std::vector<MgLevel<T>>& lv, size_t l) {
 // LDS solve.
 constexpr int MG_TAIL_AUTO_CELLS = 2048;
 template <typename T>
-size_t mg_tail_start(const fugu_ctx* c, const std::vector<MgLevel<T>>& lv) {
-  const size_t nl = lv.size();
+size_t mg_tail_start(const fugu_ctx* c, const std::vector<MgLevel<T>>& lv, size_t nl) {
   const MgDims& dc = lv[nl - 1].d;
   if (c->mg_tail_cells == 0 || (size_t)dc.nx * dc.ny > (size_t)MG_COARSE_MAX) return nl;
   size_t ts = 0;
   if (c->mg_tail_cells > 0) {
     while (ts < nl && (size_t)lv[ts].d.nx * lv[ts].d.ny > (size_t)c->mg_tail_cells) ++ts;
   } else {
-    while (ts + 1 < nl && (mg_tail_footprint(lv, ts) * sizeof(T) > (size_t)MG_TAIL_POOL ||
+    while (ts + 1 < nl && (mg_tail_footprint(lv, ts, nl) * sizeof(T) > (size_t)MG_TAIL_POOL ||
                            (size_t)lv[ts].d.nx * lv[ts].d.ny > (size_t)MG_TAIL_AUTO_CELLS))
       ++ts;
   }
@@ -549,12 +579,12 @@ size_t mg_tail_start(const fugu_ctx* c, const std::vector<MgLevel<T>>& lv) {
 
 // V-cycle on level l; `zero_guess`: phi of level l is (treated as) zero on entry; `project`:
 // the operator is singular (no mass term, no Dirichlet face) and the coarsest rhs is centred.
+// nl: the levels in use (a velocity hierarchy may be cut short, its last one mass-dominated).
 template <typename T>
-void mg_vcycle(fugu_ctx* c, std::vector<MgLevel<T>>& lv, size_t l, bool zero_guess, int project) {
+void mg_vcycle(fugu_ctx* c, std::vector<MgLevel<T>>& lv, size_t l, bool zero_guess, int project, size_t nl) {
   MgLevel<T>& L = lv[l];
   const int nu = 2;
-  const size_t nl = lv.size();
-  if (l == mg_tail_start(c, lv)) {   // the coarse tail in one launch
+  if (l == mg_tail_start(c, lv, nl)) {   // the coarse tail in one launch
     MgTailArgs<T> ta{};
     ta.nlev = (int)(nl - l);
     int off = 0;
@@ -576,8 +606,22 @@ void mg_vcycle(fugu_ctx* c, std::vector<MgLevel<T>>& lv, size_t l, bool zero_gue
       LAUNCH((k_mg_tail<T, false>), dim3(1), dim3(1024), c->s, ta);
     return;
   }
-  if (l + 1 == lv.size()) {
-    if ((size_t)L.d.nx * L.d.ny <= (size_t)MG_COARSE_MAX) {
+  if (l + 1 == nl) {
+    if (L.m && (size_t)L.d.nx * L.d.ny > (size_t)MG_COARSE_MAX) {
+      // truncated velocity hierarchy: the mass term dominates its coarsest level, which gets
+      // two red-black sweeps from zero (one fused launch)
+      if (c->mg_fused && zero_guess) {
+        const dim3 gt((unsigned)((L.d.nx + MGT_W - 1) / MGT_W), (unsigned)((L.d.ny + MGT_H - 1) / MGT_H));
+        TLAUNCH(c, sizeof(T) == 8 ? "mg_pre_f64" : "mg_pre_f32", 5.0 * sizeof(T) * L.d.nx * L.d.ny,
+                (k_mg_pre<T, 2>), gt, dim3(256), c->s, L.d, (const T*)L.cx, (const T*)L.cy, (const T*)L.m,
+                (const T*)L.b, L.phi, (T*)nullptr, (T*)nullptr);
+      } else {
+        for (int s = 0; s < 2; ++s) {
+          mg_smooth(c, L, 0, (s == 0 && zero_guess) ? 1 : 0);
+          mg_smooth(c, L, 1, 0);
+        }
+      }
+    } else if ((size_t)L.d.nx * L.d.ny <= (size_t)MG_COARSE_MAX) {
       LAUNCH((k_mg_coarse<T>), dim3(1), dim3(MG_COARSE_MAX), c->s, L.d, (const T*)L.cx, (const T*)L.cy,
              (const T*)L.dinv, (const T*)L.cinv, (const T*)L.b, L.phi, 60, project);
     } else {
@@ -606,7 +650,7 @@ void mg_vcycle(fugu_ctx* c, std::vector<MgLevel<T>>& lv, size_t l, bool zero_gue
             (k_mg_restrict<T>), g2(C.d.nx, C.d.ny), dim3(64, 4), c->s, L.d, (const T*)L.cx, (const T*)L.cy,
             (const T*)L.m, (const T*)L.b, (const T*)L.phi, C.b, C.phi);
   }
-  mg_vcycle(c, lv, l + 1, true, project);
+  mg_vcycle(c, lv, l + 1, true, project, nl);
   if (c->mg_fused) {
     // phi, coarse phi (a quarter), b, c_x, c_y (, m) read; phi written
     TLAUNCH(c, sizeof(T) == 8 ? "mg_post_f64" : "mg_post_f32", (L.m ? 6.25 : 5.25) * sizeof(T) * ncell,
@@ -632,7 +676,7 @@ void mg_pressure(fugu_ctx* c, Work<T>& w, T* phi, const T* Zp, bool force_proj =
   if (walls_both) cell_mean<T>(c, Zp, (int)S_P_MEAN, 0);
   LAUNCH((k_mg_load<T>), g2(g.nx, g.ny), dim3(64, 4), c->s, g, Zp, w.mg[0].b, walls_both, (const double*)c->sc,
          (int)S_P_MEAN, c->fl, (int)F_P_DONE);
-  for (int k = 0; k < c->mg_cycles; ++k) mg_vcycle(c, w.mg, 0, k == 0, 1);   // later cycles smooth from phi
+  for (int k = 0; k < c->mg_cycles; ++k) mg_vcycle(c, w.mg, 0, k == 0, 1, w.mg.size());   // later cycles smooth from phi
   LAUNCH((k_mg_store<T>), g2(g.nx, g.ny), dim3(64, 4), c->s, g, (const T*)w.mg[0].phi, phi);
   if (walls_both) cell_mean<T>(c, (const T*)phi, (int)S_P_MEAN, 0);
 }
@@ -646,7 +690,7 @@ void vmg_solve(fugu_ctx* c, Work<T>& w, int comp, const T* rhs, T* out, bool loa
   auto& lv = w.vmg[comp];
   MgLevel<T>& L = lv[0];
   if (!loaded) LAUNCH((k_vmg_load<T>), g2(L.d.nx, L.d.ny), dim3(64, 4), c->s, g, L.d, rhs, comp, L.b);
-  mg_vcycle(c, lv, 0, true, 0);
+  mg_vcycle(c, lv, 0, true, 0, w.vmg_nl[comp]);
   LAUNCH((k_vmg_store<T>), grid_union(g), dim3(TX, TY), c->s, g, comp, (const T*)L.phi, rhs, out);
 }
 
@@ -1692,6 +1736,8 @@ int fugu_ctx_create(const fugu_grid* grid, const fugu_params* prm, fugu_ctx** ou
     if (const char* ts = std::getenv("FUGU_MG_TAIL_STAGE")) c->mg_tail_stage = std::atoi(ts) != 0;
     if (const char* fe = std::getenv("FUGU_FGMRES_EST")) c->fg_est = std::atof(fe);
     if (const char* pc = std::getenv("FUGU_PCD_CROSS")) c->pcd_cross = std::atoi(pc) != 0;
+    if (const char* vl = std::getenv("FUGU_VMG_LEVELS")) c->vmg_levels = std::atoi(vl);
+    if (const char* vt = std::getenv("FUGU_VMG_THETA")) c->vmg_theta = std::atof(vt);
     if (const char* rf = std::getenv("FUGU_REORTH_FRAC")) c->reorth_frac = std::atof(rf);
     HIPCHK(hipStreamCreateWithFlags(&c->s, hipStreamNonBlocking));
     for (auto& e : c->ev) HIPCHK(hipEventCreate(&e));

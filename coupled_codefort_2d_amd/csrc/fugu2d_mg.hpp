@@ -418,8 +418,8 @@ __device__ inline void mgt_store(const MgDims& f, int i0, int j0, T* out, const 
   }
 }
 
-// NU pre-smoothing sweeps from zero (red first), then the residual summed onto the coarse
-// cells of the tile (coarse phi zeroed).  The residual of each tile cell is formed by the thread
+// NU pre-smoothing sweeps from zero (red first), then (bc set) the residual summed onto the
+// coarse cells of the tile (coarse phi zeroed).  The residual of each tile cell is formed by the thread
 // holding its coefficients and summed from LDS (children in the restriction's order).
 template <typename T, int NU>
 __global__ void __launch_bounds__(256) k_mg_pre(MgDims f, const T* __restrict__ cx, const T* __restrict__ cy,
@@ -438,6 +438,7 @@ __global__ void __launch_bounds__(256) k_mg_pre(MgDims f, const T* __restrict__ 
 #pragma unroll
   for (int h = 0; h < 2 * NU; ++h) mgt_half<T, H>(H - 1 - h, h & 1, h == 0 ? 1 : 0, cells, s);
   mgt_store<T, H>(f, i0, j0, phi, s);
+  if (!bc) return;   // smoothing only (a truncated hierarchy's coarsest level)
 #pragma unroll
   for (int k = 0; k < M::NS; ++k) {
     if (cells.ring[k] != 0) continue;
@@ -711,6 +712,35 @@ __global__ void k_vmg_coef0(Grid g, MgDims d, int comp, const T* __restrict__ rh
     }
     m[fi + (size_t)fj * d.nx] = (T)mm;
   }
+}
+
+// max over the u and v blocks' level-0 cells of (sum of face conductances) / mass: how far the
+// diffusion dominates the mass term, which sets how deep the velocity hierarchies must go.
+template <typename T>
+__global__ void __launch_bounds__(256) k_vmg_ratio(MgDims du, const T* __restrict__ cxu, const T* __restrict__ cyu,
+                                                   const T* __restrict__ mu, MgDims dv, const T* __restrict__ cxv,
+                                                   const T* __restrict__ cyv, const T* __restrict__ mv,
+                                                   double* __restrict__ partials) {
+  double r[2] = {0.0, 0.0};
+  const size_t stride = (size_t)gridDim.x * blockDim.x;
+  const size_t nuc = (size_t)du.nx * du.ny, nvc = (size_t)dv.nx * dv.ny;
+  for (size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x; q < nuc || q < nvc; q += stride) {
+    if (q < nuc) {
+      const size_t i = q % du.nx, j = q / du.nx;
+      const double s = (double)cxu[i + j * (du.nx + 1)] + (double)cxu[i + 1 + j * (du.nx + 1)] + (double)cyu[q] +
+                       (double)cyu[q + du.nx];
+      const double m = (double)mu[q];
+      if (m > 0.0) r[0] = fmax(r[0], s / m);
+    }
+    if (q < nvc) {
+      const size_t i = q % dv.nx, j = q / dv.nx;
+      const double s = (double)cxv[i + j * (dv.nx + 1)] + (double)cxv[i + 1 + j * (dv.nx + 1)] + (double)cyv[q] +
+                       (double)cyv[q + dv.nx];
+      const double m = (double)mv[q];
+      if (m > 0.0) r[1] = fmax(r[1], s / m);
+    }
+  }
+  publish_max<2>(r, partials);
 }
 
 // rhs rows -> compact b.
