@@ -64,7 +64,7 @@ struct MgLevel {
   MgDims d;
   T *cx = nullptr, *cy = nullptr, *m = nullptr, *dinv = nullptr, *b = nullptr, *phi = nullptr;
   T* cinv = nullptr;   // coarsest level of at most MG_DENSE_MAX cells: its dense inverse
-  T* phi2 = nullptr;   // the fused post-smoothing's output (swapped with phi after the launch)
+  T* phi2 = nullptr;   // the fused post-smoothing's output (the level's result when it ran)
 };
 
 // A multigrid hierarchy from the finest extents down (mass: allocate the mass terms).
@@ -110,6 +110,7 @@ struct fugu_ctx {
   double* sc = nullptr;
   int* fl = nullptr;
   double* partials = nullptr;
+  size_t partials_n = 0;   // its length in doubles
   double* h_sc = nullptr;
   int* h_fl = nullptr;
   size_t ntrip = 0;
@@ -139,18 +140,37 @@ struct fugu_ctx {
   bool use_vmg = false;  // velocity blocks by multigrid V-cycles instead of three Jacobi sweeps
   int mg_cycles = 1;
   int mg_tail_cells = -1;    // coarse-tail start: < 0 the LDS-pool fit, > 0 a cell count, 0 never
-  bool mg_fused = true;
+  bool mg_fused = true;       // fused (temporal-blocking) level kernels instead of one launch per half-sweep
   bool mg_tail_stage = true; // stage the coarse tail in LDS when it fits
-  bool pcd_cross = true;
+  bool pcd_cross = true;     // the PCD's eta cross term from the solved v block into the u block
   int vmg_levels = 0;        // velocity hierarchies: at most this many levels (0: coarsen fully)
-  double vmg_theta = 3.0;    // velocity hierarchies cut where diffusion / mass < vmg_theta (0: never)     // the PCD's eta cross term from the solved v block into the u block
+  double vmg_theta = 3.0;    // velocity hierarchies cut where diffusion / mass < vmg_theta (0: never)
   double fg_est = 0.9;       // FGMRES ends a cycle when its residual estimate is below fg_est * tol
   // second Gram-Schmidt pass when ||w'||^2 < reorth_frac ||w||^2: only on severe cancellation
   // (more than 99 % of ||w||^2 removed by the first pass).  With the PCD preconditioner the
   // Kahan-Parlett 1/2 fires on almost every step without changing a single iteration count or
   // final residual on the rising bubble (512x1024) or the 2048^2 inclusions, FP64 or mixed.
-  double reorth_frac = 0.01;      // fused (temporal-blocking) level kernels instead of one launch per half-sweep
+  double reorth_frac = 0.01;
   double pcd_visc = 0.0; // viscous term of the Schur approximation (0 = the reference PCD)
+  // FGMRES steps as HIP graphs (FUGU_GRAPHS=1): step k of a cycle (preconditioner, matvec,
+  // Gram-Schmidt, the Hessenberg update, the next basis vector) is captured once and replayed,
+  // so the ~40 launches of a step cost one graph launch on the host.  Every launch of a step
+  // reads its per-solve scalars from device slots, and its buffers (basis k, direction k, the
+  // multigrid levels, the work triplets) are fixed, so a replay repeats the captured work
+  // exactly.  The graphs are rebuilt when anything they bake in changes (fg_key).  Off by
+  // default: on the 512 x 1024 bubble the host keeps ahead of the device without them, and the
+  // per-kernel cost (a few microseconds between dependent kernels) is on the device side.
+  bool graphs = false;
+  struct FgKey {
+    int prec = 0, mvflags = 0, nl0 = 0, nl1 = 0, flags = 0, mg_cycles = 0;
+    double dt = 0.0, pcd_visc = 0.0;
+    bool operator==(const FgKey& o) const {
+      return prec == o.prec && mvflags == o.mvflags && nl0 == o.nl0 && nl1 == o.nl1 && flags == o.flags &&
+             mg_cycles == o.mg_cycles && dt == o.dt && pcd_visc == o.pcd_visc;
+    }
+  } fg_key;
+  std::vector<hipGraphExec_t> fg_graph;
+  std::vector<int> fg_graph_pcg;   // pressure-block calls one replay stands for
   // adjoint: solve_fgmres applies A_eff^T and the transposed PCD instead of A and the PCD
   bool adjoint = false;
   struct AdjWork {
@@ -438,25 +458,47 @@ inline dim3 g2(int nx, int ny, int bx = 64, int by = 4) {
   return dim3((unsigned)((nx + bx - 1) / bx), (unsigned)((ny + by - 1) / by));
 }
 
+// Levels of at most MG_TAIL_BLOCK_CELLS cells (the only ones whose tail can fit the LDS pool)
+// keep c_x, c_y (, m), D^-1, b and phi back to back in one allocation, level after level, in
+// the coarse tail's LDS pool order: staging a tail is then one contiguous copy.
+constexpr size_t MG_TAIL_BLOCK_CELLS = 8192;
 template <typename T>
 std::vector<MgLevel<T>> mg_levels(fugu_ctx* c, MgDims d, bool mass, int max_levels) {
-  std::vector<MgLevel<T>> lv;
+  std::vector<MgDims> ds;
   for (;;) {
-    MgLevel<T> L;
-    L.d = d;
-    const size_t n = (size_t)d.nx * d.ny;
-    L.cx = dalloc<T>(c, (size_t)(d.nx + 1) * d.ny);
-    L.cy = dalloc<T>(c, (size_t)d.nx * (d.ny + 1));
-    if (mass) L.m = dalloc<T>(c, n);
-    L.dinv = dalloc<T>(c, n);
-    L.b = dalloc<T>(c, n);
-    L.phi = dalloc<T>(c, n);
-    L.phi2 = dalloc<T>(c, n);
-    const bool last = !mg_can_coarsen(d) || (max_levels > 0 && (int)lv.size() + 1 >= max_levels);
-    if (last && n <= (size_t)MG_DENSE_MAX) L.cinv = dalloc<T>(c, n * n);
-    lv.push_back(L);
-    if (last) break;
+    ds.push_back(d);
+    if (!mg_can_coarsen(d) || (max_levels > 0 && (int)ds.size() >= max_levels)) break;
     d = mg_coarse_dims(d);
+  }
+  size_t first_blk = ds.size(), blk_n = 0;
+  for (size_t q = 0; q < ds.size(); ++q) {
+    const size_t n = (size_t)ds[q].nx * ds[q].ny;
+    if (n > MG_TAIL_BLOCK_CELLS) continue;
+    first_blk = std::min(first_blk, q);
+    blk_n += (size_t)(ds[q].nx + 1) * ds[q].ny + (size_t)ds[q].nx * (ds[q].ny + 1) + (mass ? 4 : 3) * n;
+  }
+  T* blk = blk_n ? dalloc<T>(c, blk_n) : nullptr;
+  std::vector<MgLevel<T>> lv;
+  for (size_t q = 0; q < ds.size(); ++q) {
+    MgLevel<T> L;
+    L.d = ds[q];
+    const size_t n = (size_t)L.d.nx * L.d.ny;
+    const size_t ncx = (size_t)(L.d.nx + 1) * L.d.ny, ncy = (size_t)L.d.nx * (L.d.ny + 1);
+    auto take = [&](size_t k) -> T* {
+      if (q < first_blk) return dalloc<T>(c, k);
+      T* p = blk;
+      blk += k;
+      return p;
+    };
+    L.cx = take(ncx);
+    L.cy = take(ncy);
+    if (mass) L.m = take(n);
+    L.dinv = take(n);
+    L.b = take(n);
+    L.phi = take(n);
+    L.phi2 = dalloc<T>(c, n);
+    if (q + 1 == ds.size() && n <= (size_t)MG_DENSE_MAX) L.cinv = dalloc<T>(c, n * n);
+    lv.push_back(L);
   }
   return lv;
 }
@@ -580,8 +622,14 @@ size_t mg_tail_start(const fugu_ctx* c, const std::vector<MgLevel<T>>& lv, size_
 // V-cycle on level l; `zero_guess`: phi of level l is (treated as) zero on entry; `project`:
 // the operator is singular (no mass term, no Dirichlet face) and the coarsest rhs is centred.
 // nl: the levels in use (a velocity hierarchy may be cut short, its last one mass-dominated).
+// Returns the buffer holding the level's result: phi2 after a fused post-smoothing (whose
+// tiles cannot update in place), phi otherwise -- a function of the path alone, so the same
+// launches always touch the same buffers (a captured cycle replays correctly).
+// psum (level 0 only): where a fused post-smoothing writes the result, its per-tile double-double
+// sums go there and *psum_nb = their count (left 0 when no such launch ran or they do not fit).
 template <typename T>
-void mg_vcycle(fugu_ctx* c, std::vector<MgLevel<T>>& lv, size_t l, bool zero_guess, int project, size_t nl) {
+T* mg_vcycle(fugu_ctx* c, std::vector<MgLevel<T>>& lv, size_t l, bool zero_guess, int project, size_t nl,
+             double* psum = nullptr, int* psum_nb = nullptr) {
   MgLevel<T>& L = lv[l];
   const int nu = 2;
   if (l == mg_tail_start(c, lv, nl)) {   // the coarse tail in one launch
@@ -600,11 +648,22 @@ void mg_vcycle(fugu_ctx* c, std::vector<MgLevel<T>>& lv, size_t l, bool zero_gue
       ta.L[q - l] = t;
     }
     ta.nu = nu; ta.coarse_sweeps = 60; ta.project = project; ta.zero_guess = zero_guess ? 1 : 0;
-    if (c->mg_tail_stage && (size_t)off * sizeof(T) <= (size_t)MG_TAIL_POOL)
+    // staged: the levels' arrays must lie in the pool order from lv[l].cx on (the phi of the
+    // levels below l excepted: the tail writes them before it reads them)
+    bool packed = true;
+    for (size_t q = l; q < nl && packed; ++q) {
+      const MgTailLevel<T>& t = ta.L[q - l];
+      const T* base = lv[l].cx;
+      packed = lv[q].cx == base + t.ocx && lv[q].cy == base + t.ocy && (t.om < 0 || lv[q].m == base + t.om) &&
+               lv[q].dinv == base + t.odinv && lv[q].b == base + t.ob;
+    }
+    ta.blk = lv[l].cx;
+    ta.nblk = off;
+    if (c->mg_tail_stage && packed && (size_t)off * sizeof(T) <= (size_t)MG_TAIL_POOL)
       LAUNCH((k_mg_tail<T, true>), dim3(1), dim3(1024), c->s, ta);
     else
       LAUNCH((k_mg_tail<T, false>), dim3(1), dim3(1024), c->s, ta);
-    return;
+    return L.phi;
   }
   if (l + 1 == nl) {
     if (L.m && (size_t)L.d.nx * L.d.ny > (size_t)MG_COARSE_MAX) {
@@ -630,7 +689,7 @@ void mg_vcycle(fugu_ctx* c, std::vector<MgLevel<T>>& lv, size_t l, bool zero_gue
         mg_smooth(c, L, 1, 0);
       }
     }
-    return;
+    return L.phi;
   }
   MgLevel<T>& C = lv[l + 1];
   const double ncell = (double)L.d.nx * L.d.ny;
@@ -650,20 +709,21 @@ void mg_vcycle(fugu_ctx* c, std::vector<MgLevel<T>>& lv, size_t l, bool zero_gue
             (k_mg_restrict<T>), g2(C.d.nx, C.d.ny), dim3(64, 4), c->s, L.d, (const T*)L.cx, (const T*)L.cy,
             (const T*)L.m, (const T*)L.b, (const T*)L.phi, C.b, C.phi);
   }
-  mg_vcycle(c, lv, l + 1, true, project, nl);
+  const T* cres = mg_vcycle(c, lv, l + 1, true, project, nl);
   if (c->mg_fused) {
     // phi, coarse phi (a quarter), b, c_x, c_y (, m) read; phi written
     TLAUNCH(c, sizeof(T) == 8 ? "mg_post_f64" : "mg_post_f32", (L.m ? 6.25 : 5.25) * sizeof(T) * ncell,
             (k_mg_post<T, 2>), gt, dim3(256), c->s, L.d, (const T*)L.cx, (const T*)L.cy, (const T*)L.m,
-            (const T*)L.b, (const T*)L.phi, (const T*)C.phi, L.phi2);
-    std::swap(L.phi, L.phi2);
-  } else {
-    LAUNCH((k_mg_prolong<T>), g2(L.d.nx, L.d.ny), dim3(64, 4), c->s, L.d, (const T*)C.phi, L.phi);
-    for (int s = 0; s < nu; ++s) {
-      mg_smooth(c, L, 1, 0);
-      mg_smooth(c, L, 0, 0);
-    }
+            (const T*)L.b, (const T*)L.phi, cres, L.phi2, psum ? psum : (double*)nullptr);
+    if (psum && psum_nb) *psum_nb = (int)(gt.x * gt.y);
+    return L.phi2;
   }
+  LAUNCH((k_mg_prolong<T>), g2(L.d.nx, L.d.ny), dim3(64, 4), c->s, L.d, cres, L.phi);
+  for (int s = 0; s < nu; ++s) {
+    mg_smooth(c, L, 1, 0);
+    mg_smooth(c, L, 0, 0);
+  }
+  return L.phi;
 }
 
 // phi (padded, interior written) ~= A_p^-1 (z_p - mean) by mg_cycles V-cycles from zero.
@@ -676,9 +736,50 @@ void mg_pressure(fugu_ctx* c, Work<T>& w, T* phi, const T* Zp, bool force_proj =
   if (walls_both) cell_mean<T>(c, Zp, (int)S_P_MEAN, 0);
   LAUNCH((k_mg_load<T>), g2(g.nx, g.ny), dim3(64, 4), c->s, g, Zp, w.mg[0].b, walls_both, (const double*)c->sc,
          (int)S_P_MEAN, c->fl, (int)F_P_DONE);
-  for (int k = 0; k < c->mg_cycles; ++k) mg_vcycle(c, w.mg, 0, k == 0, 1, w.mg.size());   // later cycles smooth from phi
-  LAUNCH((k_mg_store<T>), g2(g.nx, g.ny), dim3(64, 4), c->s, g, (const T*)w.mg[0].phi, phi);
+  MgLevel<T>& L0 = w.mg[0];
+  const T* res = nullptr;
+  for (int k = 0; k < c->mg_cycles; ++k) {   // later cycles smooth from the last result, in phi
+    if (res && res != L0.phi)
+      HIPCHK(hipMemcpyAsync(L0.phi, res, (size_t)L0.d.nx * L0.d.ny * sizeof(T), hipMemcpyDeviceToDevice, c->s));
+    res = mg_vcycle(c, w.mg, 0, k == 0, 1, w.mg.size());
+  }
+  LAUNCH((k_mg_store<T>), g2(g.nx, g.ny), dim3(64, 4), c->s, g, res, phi);
   if (walls_both) cell_mean<T>(c, (const T*)phi, (int)S_P_MEAN, 0);
+}
+
+// The PCD's pressure block in the chosen fast forms: returns the compact level-0 result
+// phi ~= A_p^-1 (z_p - mean) (mg_cycles V-cycles from zero), and with walls in both directions
+// leaves S_P_MEAN = mean(phi), from the last post-smoothing's per-tile sums where the fused
+// kernel wrote the result (double-double sums: the same mean whichever kernel summed), and
+// F_P_DONE = 0.  No padded copy: k_pcd_rp reads the compact array.
+template <typename T>
+const T* mg_pressure_compact(fugu_ctx* c, Work<T>& w, const T* Zp) {
+  const Grid& g = c->g;
+  const int walls_both = (!g.per_x && !g.per_y) ? 1 : 0;
+  c->pcg_calls++;
+  if (walls_both) cell_mean<T>(c, Zp, (int)S_P_MEAN, 0);
+  MgLevel<T>& L0 = w.mg[0];
+  LAUNCH((k_mg_load<T>), g2(g.nx, g.ny), dim3(64, 4), c->s, g, Zp, L0.b, walls_both, (const double*)c->sc,
+         (int)S_P_MEAN, c->fl, (int)F_P_DONE);
+  const size_t tiles = (size_t)((L0.d.nx + MGT_W - 1) / MGT_W) * ((L0.d.ny + MGT_H - 1) / MGT_H);
+  double* psum = (walls_both && 2 * tiles <= c->partials_n) ? c->partials : nullptr;
+  const T* res = nullptr;
+  int nb = 0;
+  for (int k = 0; k < c->mg_cycles; ++k) {   // later cycles smooth from the last result, in phi
+    if (res && res != L0.phi)
+      HIPCHK(hipMemcpyAsync(L0.phi, res, (size_t)L0.d.nx * L0.d.ny * sizeof(T), hipMemcpyDeviceToDevice, c->s));
+    nb = 0;
+    res = mg_vcycle(c, w.mg, 0, k == 0, 1, w.mg.size(), k + 1 == c->mg_cycles ? psum : nullptr, &nb);
+  }
+  if (walls_both) {
+    if (nb == 0) {
+      nb = (int)std::min<unsigned>(nblk((size_t)L0.d.nx * L0.d.ny), (unsigned)RED_MAXB);
+      LAUNCH((k_mg_sum<T>), dim3((unsigned)nb), dim3(256), c->s, L0.d, res, c->partials);
+    }
+    LAUNCH(k_fold_sum, dim3(1), dim3(NT), c->s, g, nb, (int)S_P_MEAN, 0, (const double*)c->partials, c->sc,
+           (const int*)c->fl);
+  }
+  return res;
 }
 
 // out (padded u or v array) ~= A_comp^-1 rhs by one V-cycle from zero on the velocity block
@@ -690,8 +791,8 @@ void vmg_solve(fugu_ctx* c, Work<T>& w, int comp, const T* rhs, T* out, bool loa
   auto& lv = w.vmg[comp];
   MgLevel<T>& L = lv[0];
   if (!loaded) LAUNCH((k_vmg_load<T>), g2(L.d.nx, L.d.ny), dim3(64, 4), c->s, g, L.d, rhs, comp, L.b);
-  mg_vcycle(c, lv, 0, true, 0, w.vmg_nl[comp]);
-  LAUNCH((k_vmg_store<T>), grid_union(g), dim3(TX, TY), c->s, g, comp, (const T*)L.phi, rhs, out);
+  const T* res = mg_vcycle(c, lv, 0, true, 0, w.vmg_nl[comp]);
+  LAUNCH((k_vmg_store<T>), grid_union(g), dim3(TX, TY), c->s, g, comp, res, rhs, out);
 }
 
 // block PCD preconditioner R = P^-1 Z (triplets).
@@ -702,15 +803,16 @@ void pcd(fugu_ctx* c, Work<T>& w, const T* Z, T* R, T dt) {
   const T *Zu = Z, *Zv = Z + g.nu, *Zp = Z + g.nu + g.nv;
   T *Ru = R, *Rv = R + g.nu, *Rp = R + g.nu + g.nv;
   const dim3 gu = grid_union(g), gc = grid_cells(g), bl(TX, TY);
+  const T* phc = nullptr;
   if (c->use_mg) {
-    mg_pressure<T>(c, w, w.phi, Zp);
+    phc = mg_pressure_compact<T>(c, w, Zp);
   } else {
     pcg<T>(c, w, w.phi, Zp, true, 1.0e-5, 150);
     if (walls_both)   // mean of phi, unless the zero-rhs early exit fired (skip when F_P_DONE >= 2)
       cell_mean<T>(c, (const T*)w.phi, (int)S_P_MEAN, 2);
   }
   LAUNCH((k_pcd_rp<T>), gu, bl, c->s, g, w.phi, w.uadv, w.vadv, dt, walls_both, Rp, c->sc, c->fl, Zp, w.kap,
-         (T)c->pcd_visc);
+         (T)c->pcd_visc, phc);
   halo(c, Rp, 2);
   LAUNCH((k_pcd_zwork<T>), gu, bl, c->s, g, Zu, Zv, (const T*)Rp, w.zuw, w.zvw,
          c->use_vmg ? w.vmg[1][0].b : (T*)nullptr);
@@ -1017,6 +1119,48 @@ void pcd_T(fugu_ctx* c, const double* Y, double* Z, double dt, double ptol, int 
   mask_trip(c, Z);
 }
 
+void fg_graphs_clear(fugu_ctx* c) {
+  for (auto& e : c->fg_graph)
+    if (e) (void)hipGraphExecDestroy(e);
+  c->fg_graph.clear();
+  c->fg_graph_pcg.clear();
+}
+
+// Run FGMRES step k (`emit` enqueues its launches) from its graph, capturing it on first use.
+template <typename F>
+void fg_graph_run(fugu_ctx* c, const fugu_ctx::FgKey& key, int k, F&& emit) {
+  if (!(key == c->fg_key)) {
+    fg_graphs_clear(c);
+    c->fg_key = key;
+  }
+  if ((int)c->fg_graph.size() <= k) {
+    c->fg_graph.resize(k + 1, nullptr);
+    c->fg_graph_pcg.resize(k + 1, 0);
+  }
+  if (!c->fg_graph[k]) {
+    const int pc0 = c->pcg_calls;
+    hipGraph_t gr = nullptr;
+    HIPCHK(hipStreamBeginCapture(c->s, hipStreamCaptureModeThreadLocal));
+    try {
+      emit();
+    } catch (...) {
+      (void)hipStreamEndCapture(c->s, &gr);
+      if (gr) (void)hipGraphDestroy(gr);
+      throw;
+    }
+    HIPCHK(hipStreamEndCapture(c->s, &gr));
+    hipGraphExec_t ge = nullptr;
+    const hipError_t e = hipGraphInstantiate(&ge, gr, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(gr);
+    HIPCHK(e);
+    c->fg_graph[k] = ge;
+    c->fg_graph_pcg[k] = c->pcg_calls - pc0;
+  } else {
+    c->pcg_calls += c->fg_graph_pcg[k];
+  }
+  HIPCHK(hipGraphLaunch(c->fg_graph[k], c->s));
+}
+
 // the chosen fast solve: restarted flexible GMRES(m) on the coupled system (coupled_dot
 // norm), each restart recomputing the true residual B - A X and updating X in FP64, so the
 // stopping test ||B - A X|| / ||B|| <= tol is met in FP64.  Preconditioner (max_inner > 0):
@@ -1057,11 +1201,28 @@ void solve_fgmres(fugu_ctx* c, Work<T>& w, double* X, double dt, double tol, dou
   cdot1<double>(c, B, B, PH_STORE);
   read_slots(c);
   const double bn = sqrt(c->h_sc[S_DOT0]) > 0 ? sqrt(c->h_sc[S_DOT0]) : 1.0;
+  set_slot(c, S_FG_TOL, c->fg_est * tol * bn);
+  // steps as graphs: the PCD forms only (an inner Krylov solve reads flags on the host), not
+  // while kernels are timed or logged one by one
+  const bool graphed = c->graphs && max_inner == 0 && !c->adjoint && c->use_mg && !c->kt_on && !c->launch_log;
+  fugu_ctx::FgKey key;
+  if (graphed) {
+    key.prec = ir32 ? 4 : (int)sizeof(T);
+    key.mvflags = mvflags;
+    key.nl0 = c->use_vmg ? (int)w.vmg_nl[0] : 0;
+    key.nl1 = c->use_vmg ? (int)w.vmg_nl[1] : 0;
+    key.flags = (c->use_vmg ? 1 : 0) | (c->pcd_cross ? 2 : 0) | (c->mg_fused ? 4 : 0) | (c->mg_tail_stage ? 8 : 0);
+    key.mg_cycles = c->mg_cycles;
+    key.dt = dt;
+    key.pcd_visc = c->pcd_visc;
+  }
   int total_inner = 0, steps = 0;
   double rr = 1.0;
   // o = x0 + sgn sum_q c_q xs[q].  gs_mode 0: plain; 1: first Gram-Schmidt pass (norm and the
   // re-orthogonalisation test); 2: second pass, predicated on F_REORTH (norm when it runs).
-  auto maxpy = [&](auto* o, const auto* x0, auto* const* xs, int nq, double sgn, int c_off, int gs_mode) {
+  // fg_k >= 0 (gs_mode 2): the FGMRES step k follows the second pass's norm when that pass runs
+  auto maxpy = [&](auto* o, const auto* x0, auto* const* xs, int nq, double sgn, int c_off, int gs_mode,
+                   int fg_k = -1) {
     using O = std::remove_cv_t<std::remove_pointer_t<decltype(o)>>;
     using V = std::remove_cv_t<std::remove_pointer_t<std::remove_cv_t<std::remove_pointer_t<decltype(xs)>>>>;
     MaxpyArgs<O, V> ma{};
@@ -1081,11 +1242,12 @@ void solve_fgmres(fugu_ctx* c, Work<T>& w, double* X, double dt, double tol, dou
               (const int*)c->fl, -1);
     if (gs_mode)
       LAUNCH(k_fold_gs_norm, dim3(1), dim3(NT), c->s, (int)nb, gs_mode, c->g.vol, (const double*)c->partials,
-             c->sc, c->fl);
+             c->sc, c->fl, fg_k, c->fg);
   };
   // W -= V h1 (coefficients at S_HBUF) fused with <W', V_i> -> S_HBUF + 64 + i when the
-  // re-orthogonalisation test fires (zero otherwise), ||W'||^2 -> S_DOT0, the test -> F_REORTH
-  auto maxpy_dot = [&](auto* W, auto* const* basis, int nq) {
+  // re-orthogonalisation test fires (zero otherwise), ||W'||^2 -> S_DOT0, the test -> F_REORTH;
+  // when it does not fire, the FGMRES step fg_k follows in the fold
+  auto maxpy_dot = [&](auto* W, auto* const* basis, int nq, int fg_k) {
     using V = std::remove_cv_t<std::remove_pointer_t<decltype(W)>>;
     MaxpyArgs<V, V> ma{};
     ma.out = W; ma.x0 = W; ma.nq = nq; ma.c_off = 0; ma.sgn = -1.0; ma.n = n3;
@@ -1104,7 +1266,7 @@ void solve_fgmres(fugu_ctx* c, Work<T>& w, double* X, double dt, double tol, dou
     FUGU_MAXPY_DOT(8) FUGU_MAXPY_DOT(16) FUGU_MAXPY_DOT(32) FUGU_MAXPY_DOT(48)
 #undef FUGU_MAXPY_DOT
     LAUNCH(k_fold_maxpy_dot, dim3(1), dim3(1024), c->s, (int)nb, nq, stride, c->g.vol, c->reorth_frac,
-           (const double*)c->partials, c->sc, c->fl);
+           (const double*)c->partials, c->sc, c->fl, fg_k, c->fg);
   };
   // <W, V_i>, i = 0..k -> S_HBUF + off + i (and vol <W, W> -> S_DOT1); pass 2 predicated on F_REORTH
   auto dots_into = [&](const auto* W, auto* const* basis, int k, int off, bool second) {
@@ -1147,6 +1309,7 @@ void solve_fgmres(fugu_ctx* c, Work<T>& w, double* X, double dt, double tol, dou
     for (int k = 0; k < m && steps < max_steps; ++k) {
       // preconditioner z_k ~ A^-1 v_k in precision T
       int it = 1;
+      auto step_k = [&]() {
       if (ir32) {
         float* z = c->kz32[k];
         pcd<float>(c, c->wf, (const float*)c->kv32[k], z, (float)dt);
@@ -1156,9 +1319,8 @@ void solve_fgmres(fugu_ctx* c, Work<T>& w, double* X, double dt, double tol, dou
         matvec(c, c->wf, z, c->wf.z, dt, mvflags, nullptr);
         float* W = c->wf.z;
         dots_into(W, c->kv32.data(), k, 0, false);
-        maxpy_dot(W, c->kv32.data(), k + 1);
-        maxpy(W, W, c->kv32.data(), k + 1, -1.0, 64, 2);
-        LAUNCH(k_fgmres_step, dim3(1), dim3(1), c->s, k, c->fg_est * tol * bn, c->sc, c->fg, c->fl);
+        maxpy_dot(W, c->kv32.data(), k + 1, k);
+        maxpy(W, W, c->kv32.data(), k + 1, -1.0, 64, 2, k);   // the step k runs in one of the two folds
         lin<float>(c, L_SCALE, c->kv32[k + 1], W, nullptr, nullptr, nullptr, S_INVH, -1, -1, 1.0, false);
       } else {
         if (max_inner > 0) {
@@ -1187,11 +1349,13 @@ void solve_fgmres(fugu_ctx* c, Work<T>& w, double* X, double dt, double tol, dou
         // classical Gram-Schmidt, second pass only when the first lost more than half of ||W||^2;
         // the second pass's products come out of the first pass's sweep
         dots_into(W, c->kv.data(), k, 0, false);
-        maxpy_dot(W, c->kv.data(), k + 1);
-        maxpy(W, W, c->kv.data(), k + 1, -1.0, 64, 2);
-        LAUNCH(k_fgmres_step, dim3(1), dim3(1), c->s, k, c->fg_est * tol * bn, c->sc, c->fg, c->fl);
+        maxpy_dot(W, c->kv.data(), k + 1, k);
+        maxpy(W, W, c->kv.data(), k + 1, -1.0, 64, 2, k);   // the step k runs in one of the two folds
         lin<double>(c, L_SCALE, c->kv[k + 1], W, nullptr, nullptr, nullptr, S_INVH, -1, -1, 1.0, false);
       }
+      };
+      if (graphed) fg_graph_run(c, key, k, step_k);
+      else step_k();
       total_inner += it;
       ++steps;
       // convergence is read one step behind, so the device always has the next step queued
@@ -1739,6 +1903,7 @@ int fugu_ctx_create(const fugu_grid* grid, const fugu_params* prm, fugu_ctx** ou
     if (const char* vl = std::getenv("FUGU_VMG_LEVELS")) c->vmg_levels = std::atoi(vl);
     if (const char* vt = std::getenv("FUGU_VMG_THETA")) c->vmg_theta = std::atof(vt);
     if (const char* rf = std::getenv("FUGU_REORTH_FRAC")) c->reorth_frac = std::atof(rf);
+    if (const char* gr = std::getenv("FUGU_GRAPHS")) c->graphs = std::atoi(gr) != 0;
     HIPCHK(hipStreamCreateWithFlags(&c->s, hipStreamNonBlocking));
     for (auto& e : c->ev) HIPCHK(hipEventCreate(&e));
     for (auto& e : c->fg_ev) HIPCHK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
@@ -1746,8 +1911,9 @@ int fugu_ctx_create(const fugu_grid* grid, const fugu_params* prm, fugu_ctx** ou
     c->ntrip = g.nu + g.nv + g.np;
     // two-pass reductions: RED_MAXB workgroups x up to 5 double-double values; multi-dot: 6
     // chunks x MD_BX workgroups x (MD_CH + 1) values
-    c->partials = dalloc<double>(c, std::max({(size_t)RED_MAXB * 10, (size_t)6 * MD_BX * (MD_CH + 1),
-                                              (size_t)RED_MAXB * (MAXPY_MAX + 1)}));
+    c->partials_n = std::max({(size_t)RED_MAXB * 10, (size_t)6 * MD_BX * (MD_CH + 1),
+                              (size_t)RED_MAXB * (MAXPY_MAX + 1)});
+    c->partials = dalloc<double>(c, c->partials_n);
     c->sc = dalloc<double>(c, S_COUNT);
     c->fl = (int*)dalloc<double>(c, F_COUNT);
     HIPCHK(hipHostMalloc((void**)&c->h_sc, S_COUNT * sizeof(double), hipHostMallocDefault));
@@ -1789,6 +1955,7 @@ int fugu_ctx_create(const fugu_grid* grid, const fugu_params* prm, fugu_ctx** ou
 int fugu_ctx_destroy(fugu_ctx* c) {
   if (!c) return 0;
   if (c->s) (void)hipStreamSynchronize(c->s);
+  fg_graphs_clear(c);
   for (void* p : c->allocs) (void)hipFree(p);
   for (auto& kv : c->kt)
     for (auto& e : kv.second.ev) (void)hipEventDestroy(e);

@@ -23,6 +23,7 @@ enum Slot : int {
   S_DOT0, S_DOT1, S_DOT2, S_DOT3, S_DOT4, S_MAX0,
   S_GSCALE,      // scale of the breakdown guards (1 = the reference's absolute 1e-30 tests)
   S_INVH,        // 1 / h_{k+1,k} of the outer FGMRES
+  S_FG_TOL,      // the outer FGMRES cycle's absolute stopping level for its residual estimate
   S_HBUF,        // 256 slots: dot products / combination coefficients of the outer FGMRES
   S_COUNT = S_HBUF + 256
 };
@@ -536,6 +537,42 @@ __global__ void __launch_bounds__(256) k_lin(LinArgs<T> a, const double* __restr
 // o, x0 stored as O, the x_q as V (FP64 or FP32 triplets), arithmetic in FP64; o may alias
 // x0.  One pass reads every operand once.
 constexpr int MAXPY_MAX = 48;
+// ---------------------------------------------------------------------------------
+// Outer FGMRES scalar work on the device (one thread): the Hessenberg column from the
+// two classical Gram-Schmidt passes (slots S_HBUF+i and S_HBUF+64+i) and ||w||^2
+// (S_DOT0), Givens rotations, residual estimate and the 1/h scaling of the next basis
+// vector -- no host round trip per step.  fg layout (ld = M+1): H[0 .. (M+1)M), then
+// cs[M], sn[M], g[M+1], kk.
+constexpr int FG_M = 48;
+constexpr int FG_CS = (FG_M + 1) * FG_M, FG_SN = FG_CS + FG_M, FG_G = FG_SN + FG_M, FG_KK = FG_G + FG_M + 1;
+constexpr int FG_SIZE = FG_KK + 8;
+
+__device__ inline void fgmres_step_dev(int k, double* __restrict__ sc, double* __restrict__ fg, int* __restrict__ fl) {
+  if (fl[F_FG_DONE]) return;
+  const double tol_abs = sc[S_FG_TOL];
+  double* H = fg + (size_t)k * (FG_M + 1);
+  for (int i = 0; i <= k; ++i) H[i] = sc[S_HBUF + i] + sc[S_HBUF + 64 + i];
+  const double hn = sqrt(sc[S_DOT0]);
+  H[k + 1] = hn;
+  for (int i = 0; i < k; ++i) {
+    const double c = fg[FG_CS + i], s = fg[FG_SN + i];
+    const double t = c * H[i] + s * H[i + 1];
+    H[i + 1] = -s * H[i] + c * H[i + 1];
+    H[i] = t;
+  }
+  const double den = hypot(H[k], H[k + 1]);
+  const double c = den > 0.0 ? H[k] / den : 1.0, s = den > 0.0 ? H[k + 1] / den : 0.0;
+  fg[FG_CS + k] = c;
+  fg[FG_SN + k] = s;
+  H[k] = den;
+  H[k + 1] = 0.0;
+  fg[FG_G + k + 1] = -s * fg[FG_G + k];
+  fg[FG_G + k] = c * fg[FG_G + k];
+  fg[FG_KK] = (double)(k + 1);
+  sc[S_INVH] = hn > 0.0 ? 1.0 / hn : 0.0;
+  if (fabs(fg[FG_G + k + 1]) <= tol_abs || hn == 0.0) fl[F_FG_DONE] = 1;
+}
+
 template <typename O, typename V>
 struct MaxpyArgs {
   O* out;
@@ -611,10 +648,12 @@ __global__ void __launch_bounds__(256) k_maxpy_dot(MaxpyArgs<V, V> a, const doub
 // Fold of k_maxpy_dot: sc[S_DOT0] = vol ||o||^2, fl[F_REORTH] = (||o||^2 < frac ||w||^2, the
 // Kahan-Parlett test against sc[S_DOT1] = vol ||w||^2, frac = 1/2 its usual threshold), and the
 // second pass's coefficients
-// sc[S_HBUF + 64 + q] = vol <o, x_q> when it runs, zero otherwise.
+// sc[S_HBUF + 64 + q] = vol <o, x_q> when it runs, zero otherwise.  fg_k >= 0: when the second
+// pass does not run, the FGMRES step k (fgmres_step_dev) follows here, in the same launch.
 __global__ void __launch_bounds__(1024) k_fold_maxpy_dot(int nb, int nq, int stride, double vol, double frac,
                                                          const double* __restrict__ partials,
-                                                         double* __restrict__ sc, int* __restrict__ fl) {
+                                                         double* __restrict__ sc, int* __restrict__ fl, int fg_k,
+                                                         double* __restrict__ fg) {
   __shared__ double red[MAXPY_MAX + 1];
   __shared__ int reorth;
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
@@ -633,6 +672,10 @@ __global__ void __launch_bounds__(1024) k_fold_maxpy_dot(int nb, int nq, int str
   }
   __syncthreads();
   if ((int)threadIdx.x < nq) sc[S_HBUF + 64 + threadIdx.x] = reorth ? red[threadIdx.x] : 0.0;
+  if (fg_k >= 0 && !reorth) {
+    __syncthreads();
+    if (threadIdx.x == 0) fgmres_step_dev(fg_k, sc, fg, fl);
+  }
 }
 
 // Multi-dot of the FGMRES Gram-Schmidt: <w, v_q> for q < nv in one launch (FP64 or FP32
@@ -701,9 +744,11 @@ __global__ void __launch_bounds__(1024) k_mdot_fold(int nv, int off, double vol,
 }
 // Norm fold after a Gram-Schmidt pass: sc[S_DOT0] = vol * sum.  mode 1 (after the first
 // pass): fl[F_REORTH] = (||w'||^2 < ||w||^2 / 2), the Kahan-Parlett "twice is enough" test
-// with its usual 1/sqrt(2) threshold.  mode 2 (after the second pass): only when it ran.
+// with its usual 1/sqrt(2) threshold.  mode 2 (after the second pass): only when it ran, and
+// then (fg_k >= 0) the FGMRES step k follows in the same launch.
 __global__ void __launch_bounds__(NT) k_fold_gs_norm(int nb, int mode, double vol, const double* __restrict__ partials,
-                                                     double* __restrict__ sc, int* __restrict__ fl) {
+                                                     double* __restrict__ sc, int* __restrict__ fl, int fg_k,
+                                                     double* __restrict__ fg) {
   if (mode == 2 && !fl[F_REORTH]) return;
   double v[1] = {0.0};
   for (int b = threadIdx.x; b < nb; b += blockDim.x) v[0] += partials[b];
@@ -712,6 +757,7 @@ __global__ void __launch_bounds__(NT) k_fold_gs_norm(int nb, int mode, double vo
   if (threadIdx.x == 0) {
     sc[S_DOT0] = vol * v[0];
     if (mode == 1) fl[F_REORTH] = (vol * v[0] < 0.5 * sc[S_DOT1]) ? 1 : 0;
+    if (mode == 2 && fg_k >= 0) fgmres_step_dev(fg_k, sc, fg, fl);
   }
 }
 
@@ -726,15 +772,8 @@ __global__ void __launch_bounds__(NT) k_fold_plain(int nb, int slot, double scal
 }
 
 // ---------------------------------------------------------------------------------
-// Outer FGMRES scalar work on the device (one thread): the Hessenberg column from the
-// two classical Gram-Schmidt passes (slots S_HBUF+i and S_HBUF+64+i) and ||w||^2
-// (S_DOT0), Givens rotations, residual estimate and the 1/h scaling of the next basis
-// vector -- no host round trip per step.  fg layout (ld = M+1): H[0 .. (M+1)M), then
-// cs[M], sn[M], g[M+1], kk.
-constexpr int FG_M = 48;
-constexpr int FG_CS = (FG_M + 1) * FG_M, FG_SN = FG_CS + FG_M, FG_G = FG_SN + FG_M, FG_KK = FG_G + FG_M + 1;
-constexpr int FG_SIZE = FG_KK + 8;
-
+// Outer FGMRES cycle start and end (one thread each); the steps run inside the Gram-Schmidt
+// folds above (fgmres_step_dev).
 __global__ void k_fgmres_init(double* __restrict__ sc, double* __restrict__ fg, int* __restrict__ fl) {
   const double beta = sqrt(sc[S_DOT0]);
   for (int i = 0; i <= FG_M; ++i) fg[FG_G + i] = 0.0;
@@ -744,30 +783,8 @@ __global__ void k_fgmres_init(double* __restrict__ sc, double* __restrict__ fg, 
   fl[F_FG_DONE] = beta > 0.0 ? 0 : 1;
 }
 
-__global__ void k_fgmres_step(int k, double tol_abs, double* __restrict__ sc, double* __restrict__ fg,
-                              int* __restrict__ fl) {
-  if (fl[F_FG_DONE]) return;
-  double* H = fg + (size_t)k * (FG_M + 1);
-  for (int i = 0; i <= k; ++i) H[i] = sc[S_HBUF + i] + sc[S_HBUF + 64 + i];
-  const double hn = sqrt(sc[S_DOT0]);
-  H[k + 1] = hn;
-  for (int i = 0; i < k; ++i) {
-    const double c = fg[FG_CS + i], s = fg[FG_SN + i];
-    const double t = c * H[i] + s * H[i + 1];
-    H[i + 1] = -s * H[i] + c * H[i + 1];
-    H[i] = t;
-  }
-  const double den = hypot(H[k], H[k + 1]);
-  const double c = den > 0.0 ? H[k] / den : 1.0, s = den > 0.0 ? H[k + 1] / den : 0.0;
-  fg[FG_CS + k] = c;
-  fg[FG_SN + k] = s;
-  H[k] = den;
-  H[k + 1] = 0.0;
-  fg[FG_G + k + 1] = -s * fg[FG_G + k];
-  fg[FG_G + k] = c * fg[FG_G + k];
-  fg[FG_KK] = (double)(k + 1);
-  sc[S_INVH] = hn > 0.0 ? 1.0 / hn : 0.0;
-  if (fabs(fg[FG_G + k + 1]) <= tol_abs || hn == 0.0) fl[F_FG_DONE] = 1;
+__global__ void k_fgmres_step(int k, double* __restrict__ sc, double* __restrict__ fg, int* __restrict__ fl) {
+  fgmres_step_dev(k, sc, fg, fl);
 }
 
 // y = H^-1 g (back substitution) into slots S_HBUF + 128 + i; returns kk in S_HBUF + 127.
@@ -1063,12 +1080,13 @@ __global__ void k_zero_if(T* __restrict__ a, size_t n, const int* __restrict__ f
 // ghosts of phi are 0; periodic ghosts wrap.  A zero-rhs early exit means phi = 0.
 // visc > 0 (chosen fast forms only): the Schur approximation also carries the viscous
 // term, r_p -= visc * kappa * z_p (Cahouet-Chabard type), for viscous-dominated regions.
+// phic (chosen fast forms): phi read from the multigrid's compact level-0 array instead.
 template <typename T>
 __global__ void __launch_bounds__(NT) k_pcd_rp(Grid g, const T* __restrict__ phi, const T* __restrict__ uadv,
                                                const T* __restrict__ vadv, T dt, int walls_both,
                                                T* __restrict__ rp, const double* __restrict__ sc,
                                                const int* __restrict__ fl, const T* __restrict__ zp,
-                                               const T* __restrict__ kap, T visc) {
+                                               const T* __restrict__ kap, T visc, const T* __restrict__ phic) {
   int i, j;
   union_ij(g, i, j);
   const int nx = g.nx, ny = g.ny, nh = g.nh;
@@ -1081,7 +1099,7 @@ __global__ void __launch_bounds__(NT) k_pcd_rp(Grid g, const T* __restrict__ phi
       if (early) return T(0);
       if (ii < 1 || ii > nx) { if (!g.per_x) return T(0); ii = (ii < 1) ? ii + nx : ii - nx; }
       if (jj < 1 || jj > ny) { if (!g.per_y) return T(0); jj = (jj < 1) ? jj + ny : jj - ny; }
-      const T x = phi[IP(g, ii, jj)];
+      const T x = phic ? phic[(ii - 1) + (size_t)(jj - 1) * nx] : phi[IP(g, ii, jj)];
       return walls_both ? x - mean : x;
     };
     const T half = T(0.5);

@@ -465,10 +465,12 @@ __global__ void __launch_bounds__(256) k_mg_pre(MgDims f, const T* __restrict__ 
 
 // phi_out = phi + phi_c(parent), then NU post-smoothing sweeps (black first).
 template <typename T, int NU>
+// psum (optional): the tile's sum of its outputs as a double-double partial, one per workgroup
+// (the mean of the level-0 result without another pass over it).
 __global__ void __launch_bounds__(256) k_mg_post(MgDims f, const T* __restrict__ cx, const T* __restrict__ cy,
                                                  const T* __restrict__ m, const T* __restrict__ b,
                                                  const T* __restrict__ phi, const T* __restrict__ phic,
-                                                 T* __restrict__ phi_out) {
+                                                 T* __restrict__ phi_out, double* __restrict__ psum) {
   constexpr int H = 2 * NU;
   using M = MgtCells<T, H>;
   __shared__ T s[M::EH][M::EW];
@@ -479,6 +481,26 @@ __global__ void __launch_bounds__(256) k_mg_post(MgDims f, const T* __restrict__
 #pragma unroll
   for (int h = 0; h < 2 * NU; ++h) mgt_half<T, H>(H - 1 - h, (h & 1) ? 0 : 1, 0, cells, s);
   mgt_store<T, H>(f, i0, j0, phi_out, s);
+  if (psum) {
+    DD v[1] = {dd_from(0.0)};
+    const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
+    if (i0 + tx < f.nx) {
+#pragma unroll
+      for (int qj = ty; qj < MGT_H; qj += 4)
+        if (j0 + qj < f.ny) v[0] = dd_add(v[0], dd_from((double)s[H + qj][H + tx]));
+    }
+    publish_dd<1>(v, psum);
+  }
+}
+
+// Double-double partial sums of a compact level array (one per workgroup, grid-stride).
+template <typename T>
+__global__ void __launch_bounds__(256) k_mg_sum(MgDims d, const T* __restrict__ phi, double* __restrict__ partials) {
+  DD v[1] = {dd_from(0.0)};
+  const size_t n = (size_t)d.nx * d.ny, stride = (size_t)gridDim.x * blockDim.x;
+  for (size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x; k < n; k += stride)
+    v[0] = dd_add(v[0], dd_from((double)phi[k]));
+  publish_dd<1>(v, partials);
 }
 
 // ---------------------------------------------------------------------------------
@@ -486,7 +508,10 @@ __global__ void __launch_bounds__(256) k_mg_post(MgDims f, const T* __restrict__
 // cells down) in ONE workgroup and one launch, with a barrier between phases instead of a
 // launch per half-sweep.  STAGE: the levels' coefficients, rhs and iterates are first copied
 // into LDS (when they fit MG_TAIL_POOL bytes), so every phase runs from LDS; otherwise they
-// stay in global memory (L2-resident).  Phase for phase the kernel performs the multi-launch
+// stay in global memory (L2-resident).  The staged levels lie in global memory in the pool's
+// own order (blk, nblk elements), so the copy is one contiguous range whose loads are all in
+// flight together: the data are cold (the finer levels streamed through the caches since the
+// set-up wrote them) and a copy per array would pay the memory latency once per array.  Phase for phase the kernel performs the multi-launch
 // cycle's arithmetic (same smoother update, residual sum, prolongation and coarsest solve), so
 // its result is bit-identical; only the top level's phi is written back.
 constexpr int MG_TAIL_LEVELS = 16;
@@ -501,6 +526,8 @@ struct MgTailLevel {
 template <typename T>
 struct MgTailArgs {
   MgTailLevel<T> L[MG_TAIL_LEVELS];
+  const T* blk;   // STAGE: the pool's image in global memory (the top level's phi excepted)
+  int nblk;       // its length in elements
   int nlev, nu, coarse_sweeps, project, zero_guess;
 };
 
@@ -562,22 +589,29 @@ __global__ void __launch_bounds__(1024) k_mg_tail(MgTailArgs<T> a) {
   auto BB = [&](int l) -> T* { return STAGE ? pool + a.L[l].ob : a.L[l].b; };
   auto PH = [&](int l) -> T* { return STAGE ? pool + a.L[l].ophi : a.L[l].phi; };
   if (STAGE) {
-    for (int l = 0; l < a.nlev; ++l) {
-      const MgDims d = a.L[l].d;
-      const int n = d.nx * d.ny;
-      tail_copy(pool + a.L[l].ocx, a.L[l].cx, (d.nx + 1) * d.ny);
-      tail_copy(pool + a.L[l].ocy, a.L[l].cy, d.nx * (d.ny + 1));
-      tail_copy(pool + a.L[l].odinv, a.L[l].dinv, n);
-      if (a.L[l].om >= 0) tail_copy(pool + a.L[l].om, a.L[l].m, n);
+    // every load issued before the first LDS store (1024 threads, a fixed number per thread);
+    // the top level's phi comes from its own array, or is zero for a zero guess (finite values
+    // are needed even then: at a wall the smoother's neighbour sum reads the cell itself times a
+    // zero conductance)
+    constexpr int KMAX = (MG_TAIL_POOL / (int)sizeof(T) + 1023) / 1024;
+    const int p0 = a.L[0].ophi, n0 = a.L[0].d.nx * a.L[0].d.ny;
+    T v[KMAX];
+#pragma unroll
+    for (int k = 0; k < KMAX; ++k) {
+      const int q = (int)threadIdx.x + 1024 * k;
+      const int r = q - p0;
+      v[k] = T(0);
+      if (q < a.nblk) {
+        if (r < 0 || r >= n0)
+          v[k] = a.blk[q];
+        else if (!a.zero_guess)
+          v[k] = a.L[0].phi[r];
+      }
     }
-    const MgDims d0 = a.L[0].d;
-    tail_copy(pool + a.L[0].ob, (const T*)a.L[0].b, d0.nx * d0.ny);
-    // a zero guess still needs finite values: at a wall the smoother's neighbour sum reads the
-    // cell itself times a zero conductance (LDS holds whatever the last kernel left there)
-    if (a.zero_guess) {
-      for (int q = (int)threadIdx.x; q < d0.nx * d0.ny; q += (int)blockDim.x) pool[a.L[0].ophi + q] = T(0);
-    } else {
-      tail_copy(pool + a.L[0].ophi, (const T*)a.L[0].phi, d0.nx * d0.ny);
+#pragma unroll
+    for (int k = 0; k < KMAX; ++k) {
+      const int q = (int)threadIdx.x + 1024 * k;
+      if (q < a.nblk) pool[q] = v[k];
     }
     __syncthreads();
   }
