@@ -161,6 +161,7 @@ struct fugu_ctx {
   // default: on the 512 x 1024 bubble the host keeps ahead of the device without them, and the
   // per-kernel cost (a few microseconds between dependent kernels) is on the device side.
   bool graphs = false;
+  bool svof_blocked = true;   // SVOF sweeps temporally blocked (FUGU_SVOF_BLOCKED=0: one per launch)
   struct FgKey {
     int prec = 0, mvflags = 0, nl0 = 0, nl1 = 0, flags = 0, mg_cycles = 0;
     double dt = 0.0, pcd_visc = 0.0;
@@ -1393,18 +1394,39 @@ void svof(fugu_ctx* c, const double* craw, double* cs_out, double delta, int max
   HIPCHK(hipMemcpyAsync(c->csA, craw, g.np * sizeof(double), hipMemcpyDeviceToDevice, c->s));
   HIPCHK(hipMemsetAsync(c->fl + F_S_DONE, 0, 2 * sizeof(int), c->s));
   double* B[2] = {c->csA, c->csB};
-  const dim3 gr = grid_cells_red(g);
-  for (int it = 1; it <= max_iter; ++it) {
-    LAUNCH(k_svof_sweep, gr, dim3(TX, TY), c->s, g, craw, (const double*)B[(it - 1) & 1], B[it & 1], cx, cy,
-           diag, c->partials, (const int*)c->fl);
-    LAUNCH(k_fold_svof, dim3(1), dim3(NT), c->s, (int)nblocks(gr), it, tol, (const double*)c->partials, c->sc,
-           c->fl);
+  const dim3 tb((unsigned)((g.nx + SVOF_TW - 1) / SVOF_TW), (unsigned)((g.ny + SVOF_TH - 1) / SVOF_TH));
+  const size_t ntb = (size_t)tb.x * tb.y;
+  const double* last = c->csA;
+  if (c->svof_blocked && ntb * SVOF_S <= c->partials_n) {
+    // SVOF_S iterations per launch; block b reads B[b & 1] and writes B[(b + 1) & 1]; the
+    // launch after the last block only recomputes a block that converged before its end
+    const int nbk = (max_iter + SVOF_S - 1) / SVOF_S;
+    for (int b = 0; b <= nbk; ++b) {
+      const int it0 = b * SVOF_S + 1, cnt = b < nbk ? std::min(SVOF_S, max_iter - b * SVOF_S) : 0;
+      LAUNCH(k_svof_block, tb, dim3(SVOF_NT), c->s, g, craw, (const double*)B[b & 1], B[(b + 1) & 1], cx, cy, diag, b,
+             cnt, c->partials, (const int*)c->fl);
+      if (cnt > 0)
+        LAUNCH(k_fold_svof_blk, dim3(1), dim3(NT), c->s, (int)ntb, b, it0, cnt, tol, (const double*)c->partials,
+               c->sc, c->fl);
+    }
+    read_flags(c);
+    const int itf = c->h_fl[F_S_ITER];
+    if (it_out) *it_out = itf;
+    if (max_iter >= 1) last = B[((itf - 1) / SVOF_S + 1) & 1];   // the output of the block holding itf
+  } else {
+    const dim3 gr = grid_cells_red(g);
+    for (int it = 1; it <= max_iter; ++it) {
+      LAUNCH(k_svof_sweep, gr, dim3(TX, TY), c->s, g, craw, (const double*)B[(it - 1) & 1], B[it & 1], cx, cy,
+             diag, c->partials, (const int*)c->fl);
+      LAUNCH(k_fold_svof, dim3(1), dim3(NT), c->s, (int)nblocks(gr), it, tol, (const double*)c->partials, c->sc,
+             c->fl);
+    }
+    // the last iterate is in B[iter & 1]
+    read_flags(c);
+    const int itf = c->h_fl[F_S_ITER];
+    if (it_out) *it_out = itf;
+    if (max_iter >= 1) last = B[itf & 1];
   }
-  // the last iterate is in B[iter & 1]
-  read_flags(c);
-  const int itf = c->h_fl[F_S_ITER];
-  if (it_out) *it_out = itf;
-  const double* last = (max_iter >= 1) ? B[itf & 1] : c->csA;
   // ghosts: C_smooth starts as a copy of C_raw (ghosts included), then exchange + mirror
   HIPCHK(hipMemcpyAsync(cs_out, craw, g.np * sizeof(double), hipMemcpyDeviceToDevice, c->s));
   {
@@ -1904,6 +1926,7 @@ int fugu_ctx_create(const fugu_grid* grid, const fugu_params* prm, fugu_ctx** ou
     if (const char* vt = std::getenv("FUGU_VMG_THETA")) c->vmg_theta = std::atof(vt);
     if (const char* rf = std::getenv("FUGU_REORTH_FRAC")) c->reorth_frac = std::atof(rf);
     if (const char* gr = std::getenv("FUGU_GRAPHS")) c->graphs = std::atoi(gr) != 0;
+    if (const char* sb = std::getenv("FUGU_SVOF_BLOCKED")) c->svof_blocked = std::atoi(sb) != 0;
     HIPCHK(hipStreamCreateWithFlags(&c->s, hipStreamNonBlocking));
     for (auto& e : c->ev) HIPCHK(hipEventCreate(&e));
     for (auto& e : c->fg_ev) HIPCHK(hipEventCreateWithFlags(&e, hipEventDisableTiming));

@@ -38,6 +38,8 @@ enum Flag : int {
   F_RELG,        // 1: breakdown guards relative to ||P^-1 b||^2 (chosen fast forms only)
   F_FG_DONE,     // outer FGMRES cycle converged
   F_REORTH,      // outer FGMRES: second Gram-Schmidt pass needed
+  F_S_RBLK,      // SVOF in blocks: the block whose iterations are redone up to the converged one
+  F_S_RCNT,      // ... and how many of them
   F_COUNT
 };
 
@@ -1399,6 +1401,128 @@ __global__ void __launch_bounds__(NT) k_fold_svof(int nb, int it, double tol, co
     sc[S_MAX0] = tot[0];
     fl[F_S_ITER] = it;
     if (tot[0] < tol) fl[F_S_DONE] = 1;
+  }
+}
+
+// SVOF Jacobi sweeps in blocks of up to SVOF_S iterations per launch (temporal blocking):
+// a 64 x 32 tile of owned cells with an SVOF_S-cell halo in LDS, every cell updated from the
+// previous iterate exactly as k_svof_sweep does (same neighbours: wall neighbours are the cell
+// itself, periodic ones wrap; same expression and order), the valid region shrinking by one
+// cell per iteration.  Per iteration and tile, the max |new - old| over the owned cells goes to
+// partials[tile * SVOF_S + t]; k_fold_svof_blk finds the first iteration below tol.  When that
+// is not the block's last, the next launch (or the extra one after the loop) recomputes the
+// block from its input -- which the ping-pong left intact -- up to that iteration: the result
+// and the iteration count are those of one sweep per launch.
+constexpr int SVOF_S = 8, SVOF_TW = 64, SVOF_TH = 32;
+constexpr int SVOF_EW = SVOF_TW + 2 * SVOF_S, SVOF_EH = SVOF_TH + 2 * SVOF_S;
+constexpr int SVOF_NT = 1024;                                       // threads per tile
+constexpr int SVOF_CPT = (SVOF_EW * SVOF_EH + SVOF_NT - 1) / SVOF_NT;   // cells per thread
+__global__ void __launch_bounds__(SVOF_NT) k_svof_block(Grid g, const double* __restrict__ craw,
+                                                    const double* __restrict__ cin, double* __restrict__ cout,
+                                                    double cx, double cy, double diag, int blk, int cnt,
+                                                    double* __restrict__ partials, const int* __restrict__ fl) {
+  bool redo = false;
+  if (fl[F_S_DONE]) {
+    if (fl[F_S_RBLK] + 1 != blk) return;
+    redo = true;   // block blk - 1 again, from its input (this launch's output buffer)
+    const double* t = cin;
+    cin = cout;
+    cout = const_cast<double*>(t);
+    cnt = fl[F_S_RCNT];
+  }
+  if (cnt <= 0) return;
+  __shared__ double L[SVOF_EW * SVOF_EH];
+  const int nx = g.nx, ny = g.ny;
+  const int i0 = (int)blockIdx.x * SVOF_TW, j0 = (int)blockIdx.y * SVOF_TH;   // 0-based owned corner
+  double cr[SVOF_CPT], nw[SVOF_CPT];
+  unsigned present = 0, owned = 0, clamp_e = 0, clamp_w = 0, clamp_n = 0, clamp_s = 0;
+  // load: every load issued before the first LDS store
+#pragma unroll
+  for (int r = 0; r < SVOF_CPT; ++r) {
+    const int q = (int)threadIdx.x + SVOF_NT * r;
+    cr[r] = 0.0;
+    nw[r] = 0.0;
+    if (q >= SVOF_EW * SVOF_EH) continue;
+    const int a = q % SVOF_EW, b = q / SVOF_EW;
+    int gi = i0 - SVOF_S + a + 1, gj = j0 - SVOF_S + b + 1;   // 1-based cell indices
+    if (g.per_x) gi = ((gi - 1) % nx + nx) % nx + 1;
+    if (g.per_y) gj = ((gj - 1) % ny + ny) % ny + 1;
+    if (gi < 1 || gi > nx || gj < 1 || gj > ny) continue;
+    const unsigned bit = 1u << r;
+    present |= bit;
+    if (a >= SVOF_S && a < SVOF_S + SVOF_TW && b >= SVOF_S && b < SVOF_S + SVOF_TH && i0 + a - SVOF_S < nx &&
+        j0 + b - SVOF_S < ny)
+      owned |= bit;
+    // a wall neighbour is the cell itself (as cell_nbrs); so is one past the region's edge,
+    // which lies outside the valid cone anyway
+    if ((!g.per_x && gi == nx) || a + 1 == SVOF_EW) clamp_e |= bit;
+    if ((!g.per_x && gi == 1) || a == 0) clamp_w |= bit;
+    if ((!g.per_y && gj == ny) || b + 1 == SVOF_EH) clamp_n |= bit;
+    if ((!g.per_y && gj == 1) || b == 0) clamp_s |= bit;
+    cr[r] = craw[IP(g, gi, gj)];
+    nw[r] = cin[IP(g, gi, gj)];
+  }
+#pragma unroll
+  for (int r = 0; r < SVOF_CPT; ++r)
+    if (present & (1u << r)) L[threadIdx.x + SVOF_NT * r] = nw[r];
+  __syncthreads();
+  __shared__ double wmax[SVOF_S][SVOF_NT / 64];   // per iteration and wavefront: max |new - old| over owned cells
+  for (int t = 0; t < cnt; ++t) {
+    double mt = 0.0;
+#pragma unroll
+    for (int r = 0; r < SVOF_CPT; ++r) {
+      const unsigned bit = 1u << r;
+      if (!(present & bit)) continue;
+      const int q = (int)threadIdx.x + SVOF_NT * r;
+      const double e = L[(clamp_e & bit) ? q : q + 1], w = L[(clamp_w & bit) ? q : q - 1];
+      const double n = L[(clamp_n & bit) ? q : q + SVOF_EW], s = L[(clamp_s & bit) ? q : q - SVOF_EW];
+      const double o = (cr[r] + cx * (e + w) + cy * (n + s)) / diag;
+      if (owned & bit) mt = fmax(mt, fabs(o - L[q]));
+      nw[r] = o;
+    }
+    mt = wave_max(mt);
+    if ((threadIdx.x & 63) == 0) wmax[t][threadIdx.x >> 6] = mt;
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < SVOF_CPT; ++r)
+      if (present & (1u << r)) L[threadIdx.x + SVOF_NT * r] = nw[r];
+    __syncthreads();
+  }
+#pragma unroll
+  for (int r = 0; r < SVOF_CPT; ++r) {
+    if (!(owned & (1u << r))) continue;
+    const int q = (int)threadIdx.x + SVOF_NT * r;
+    const int gi = i0 + q % SVOF_EW - SVOF_S + 1, gj = j0 + q / SVOF_EW - SVOF_S + 1;
+    cout[IP(g, gi, gj)] = nw[r];
+  }
+  if (!redo && threadIdx.x < SVOF_S) {
+    const int t = (int)threadIdx.x;
+    double m = 0.0;
+    if (t < cnt)
+      for (int w = 0; w < SVOF_NT / 64; ++w) m = fmax(m, wmax[t][w]);
+    partials[((size_t)blockIdx.x + (size_t)blockIdx.y * gridDim.x) * SVOF_S + t] = m;
+  }
+}
+// Fold of block blk (iterations it0 .. it0 + cnt - 1): the first iteration whose max change
+// is below tol ends the smoothing (fl[F_S_DONE], fl[F_S_ITER], sc[S_MAX0] as k_fold_svof), and
+// when it is not the block's last, fl[F_S_RBLK] / fl[F_S_RCNT] ask for the block's recomputation.
+__global__ void __launch_bounds__(NT) k_fold_svof_blk(int nb, int blk, int it0, int cnt, double tol,
+                                                      const double* __restrict__ partials, double* __restrict__ sc,
+                                                      int* __restrict__ fl) {
+  if (fl[F_S_DONE]) return;
+  double tot[SVOF_S];
+  fold_max<SVOF_S>(partials, nb, tot);
+  if (threadIdx.x == 0 && threadIdx.y == 0) {
+    for (int t = 0; t < cnt; ++t) {
+      sc[S_MAX0] = tot[t];
+      fl[F_S_ITER] = it0 + t;
+      if (tot[t] < tol) {
+        fl[F_S_DONE] = 1;
+        fl[F_S_RBLK] = (t + 1 < cnt) ? blk : -10;
+        fl[F_S_RCNT] = t + 1;
+        break;
+      }
+    }
   }
 }
 

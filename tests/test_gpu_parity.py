@@ -235,6 +235,38 @@ def test_svof_curvature_csf_bitwise(kw):
     same(fu_g, fu_o, "f_u"); same(fv_g, fv_o, "f_v")
 
 
+SVOF_GRIDS = [
+    dict(nx=130, ny=70, lx=1.3, ly=0.7, per_x=False, per_y=False),
+    dict(nx=128, ny=96, lx=1.0, ly=0.75, per_x=True, per_y=False),
+    dict(nx=96, ny=80, lx=1.2, ly=1.0, per_x=True, per_y=True),
+    dict(nx=40, ny=150, lx=0.4, ly=1.5, per_x=False, per_y=True),
+]
+
+
+@pytest.mark.parametrize("kw", SVOF_GRIDS, ids=["walls", "periodic_x", "periodic_xy", "periodic_y"])
+def test_svof_blocked_sweeps_bitwise(kw):
+    """The SVOF sweeps run several iterations per launch on tiles with halos; their iterate and
+    iteration count equal the oracle's one-sweep-at-a-time smoothing bit for bit wherever the
+    stopping test fires: inside a block (the block is recomputed up to that iteration), at a
+    block's last iteration, or never (iteration caps that are and are not block multiples),
+    on multi-tile grids with partial tiles, walls and periodic wrap."""
+    g = grid(kw)
+    ctx = ctx_for(g)
+    rng = np.random.default_rng(110)
+    c = np.clip(droplet_c(g, 0.3) + 0.3 * rng.standard_normal(g.shape("p")), -0.1, 1.1)
+    seen = set()
+    for it_max, tol in ((150, 1e-3), (150, 3e-4), (150, 1e-4), (150, 3e-5), (150, 1e-5), (150, 1e-7),
+                        (1, 1e-12), (8, 1e-12), (9, 1e-12), (16, 1e-12), (23, 1e-12)):
+        cs_o, it_o = O.svof(g, c, 1.5 * g.dx, it_max, tol)
+        cs_g = g.zeros("p")
+        it_g = ctx.svof(c, cs_g, 1.5 * g.dx, it_max, tol)
+        assert it_g == it_o, (it_max, tol, it_g, it_o)
+        same(cs_g, cs_o, f"C_smooth (it_max {it_max}, tol {tol:g}, {it_o} iterations)")
+        seen.add(it_o % 8)
+    assert len(seen) >= 3, seen   # stops at several positions inside a block of eight
+    ctx.close()
+
+
 def test_adaptive_dt():
     g = O.Grid(32, 24, 1.0, 0.75)
     ph = O.OgPhys(1000.0, 1.0, 1.0, 0.01, 0.5, 0.0, -9.81, 1, 1e12, 0.25)
