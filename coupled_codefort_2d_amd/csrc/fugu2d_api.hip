@@ -144,7 +144,12 @@ struct fugu_ctx {
   bool mg_tail_stage = true; // stage the coarse tail in LDS when it fits
   bool pcd_cross = true;     // the PCD's eta cross term from the solved v block into the u block
   int vmg_levels = 0;        // velocity hierarchies: at most this many levels (0: coarsen fully)
-  double vmg_theta = 3.0;    // velocity hierarchies cut where diffusion / mass < vmg_theta (0: never)
+  // velocity hierarchies cut where diffusion / mass < vmg_theta (0: never), the cut coarsest level
+  // smoothed by vmg_coarse_nu red-black sweeps (2, 3 or 4).  Measured: 10 / 4 keeps the FGMRES
+  // step counts of full depth on the 512 x 1024 bubble (2 levels instead of 3 at 3 / 2) and on
+  // the 2048^2 inclusions (6-7 of 10 levels), with fewer launches per step.
+  double vmg_theta = 10.0;
+  int vmg_coarse_nu = 4;
   double fg_est = 0.9;       // FGMRES ends a cycle when its residual estimate is below fg_est * tol
   // second Gram-Schmidt pass when ||w'||^2 < reorth_frac ||w||^2: only on severe cancellation
   // (more than 99 % of ||w||^2 removed by the first pass).  With the PCD preconditioner the
@@ -538,7 +543,7 @@ void mg_setup(fugu_ctx* c, Work<T>& w) {
 // Velocity-block hierarchies from the current properties and dt (allocated on first use).
 // Depth: a coarse level's diffusion-to-mass ratio is a quarter of its parent's (conductances
 // c_H ~ c, mass m_H = 4m), so once the finest ratio r0 = max (sum c) / m has dropped below
-// vmg_theta the level is mass-dominated and two red-black sweeps solve it well enough: the
+// vmg_theta the level is mass-dominated and a few red-black sweeps solve it well enough: the
 // hierarchy is cut at 1 + ceil(log4(r0 / vmg_theta)) levels (one host read per set-up, so the
 // depth is a function of the current properties alone and recomputed steps repeat bit for bit).
 template <typename T>
@@ -661,22 +666,31 @@ T* mg_vcycle(fugu_ctx* c, std::vector<MgLevel<T>>& lv, size_t l, bool zero_guess
     ta.blk = lv[l].cx;
     ta.nblk = off;
     if (c->mg_tail_stage && packed && (size_t)off * sizeof(T) <= (size_t)MG_TAIL_POOL)
-      LAUNCH((k_mg_tail<T, true>), dim3(1), dim3(1024), c->s, ta);
+      LAUNCH((k_mg_tail<T, true, MG_TAIL_NTH>), dim3(1), dim3(MG_TAIL_NTH), c->s, ta);
     else
-      LAUNCH((k_mg_tail<T, false>), dim3(1), dim3(1024), c->s, ta);
+      LAUNCH((k_mg_tail<T, false, MG_TAIL_NTH>), dim3(1), dim3(MG_TAIL_NTH), c->s, ta);
     return L.phi;
   }
   if (l + 1 == nl) {
     if (L.m && (size_t)L.d.nx * L.d.ny > (size_t)MG_COARSE_MAX) {
       // truncated velocity hierarchy: the mass term dominates its coarsest level, which gets
-      // two red-black sweeps from zero (one fused launch)
+      // vmg_coarse_nu red-black sweeps from zero (one fused launch)
+      const int cnu = c->vmg_coarse_nu;
       if (c->mg_fused && zero_guess) {
         const dim3 gt((unsigned)((L.d.nx + MGT_W - 1) / MGT_W), (unsigned)((L.d.ny + MGT_H - 1) / MGT_H));
-        TLAUNCH(c, sizeof(T) == 8 ? "mg_pre_f64" : "mg_pre_f32", 5.0 * sizeof(T) * L.d.nx * L.d.ny,
-                (k_mg_pre<T, 2>), gt, dim3(256), c->s, L.d, (const T*)L.cx, (const T*)L.cy, (const T*)L.m,
-                (const T*)L.b, L.phi, (T*)nullptr, (T*)nullptr);
+        const double ab = 5.0 * sizeof(T) * L.d.nx * L.d.ny;
+        const char* nm = sizeof(T) == 8 ? "mg_pre_f64" : "mg_pre_f32";
+        if (cnu == 3)
+          TLAUNCH(c, nm, ab, (k_mg_pre<T, 3>), gt, dim3(256), c->s, L.d, (const T*)L.cx, (const T*)L.cy,
+                  (const T*)L.m, (const T*)L.b, L.phi, (T*)nullptr, (T*)nullptr);
+        else if (cnu == 4)
+          TLAUNCH(c, nm, ab, (k_mg_pre<T, 4>), gt, dim3(256), c->s, L.d, (const T*)L.cx, (const T*)L.cy,
+                  (const T*)L.m, (const T*)L.b, L.phi, (T*)nullptr, (T*)nullptr);
+        else
+          TLAUNCH(c, nm, ab, (k_mg_pre<T, 2>), gt, dim3(256), c->s, L.d, (const T*)L.cx, (const T*)L.cy,
+                  (const T*)L.m, (const T*)L.b, L.phi, (T*)nullptr, (T*)nullptr);
       } else {
-        for (int s = 0; s < 2; ++s) {
+        for (int s = 0; s < (cnu == 3 || cnu == 4 ? cnu : 2); ++s) {
           mg_smooth(c, L, 0, (s == 0 && zero_guess) ? 1 : 0);
           mg_smooth(c, L, 1, 0);
         }
@@ -1924,6 +1938,7 @@ int fugu_ctx_create(const fugu_grid* grid, const fugu_params* prm, fugu_ctx** ou
     if (const char* pc = std::getenv("FUGU_PCD_CROSS")) c->pcd_cross = std::atoi(pc) != 0;
     if (const char* vl = std::getenv("FUGU_VMG_LEVELS")) c->vmg_levels = std::atoi(vl);
     if (const char* vt = std::getenv("FUGU_VMG_THETA")) c->vmg_theta = std::atof(vt);
+    if (const char* vc = std::getenv("FUGU_VMG_COARSE_NU")) c->vmg_coarse_nu = std::atoi(vc);
     if (const char* rf = std::getenv("FUGU_REORTH_FRAC")) c->reorth_frac = std::atof(rf);
     if (const char* gr = std::getenv("FUGU_GRAPHS")) c->graphs = std::atoi(gr) != 0;
     if (const char* sb = std::getenv("FUGU_SVOF_BLOCKED")) c->svof_blocked = std::atoi(sb) != 0;

@@ -119,17 +119,22 @@ __global__ void k_mg_dinv(MgDims d, const T* __restrict__ cx, const T* __restric
   dinv[i + (size_t)j * d.nx] = (s > T(1.0e-14)) ? T(1) / s : T(0);
 }
 
+// Level arrays stay below 2^31 elements, so indices are 32-bit.  Every face of a level exists
+// (boundary faces too) and wrapped neighbour indices are clamped at a wall, so all eight loads
+// are unconditional -- issued together, no branch per load; a boundary face of a non-periodic
+// edge couples to a zero ghost (its conductance enters the diagonal only) by a select.
 template <typename T>
 __device__ inline T mg_nbr_sum(const MgDims& d, const T* cx, const T* cy, const T* phi, int i, int j) {
-  const size_t rx = (size_t)(d.nx + 1), o = (size_t)j * d.nx;
-  // boundary faces of a non-periodic edge couple to a zero ghost (diagonal only)
-  const T cW = (i > 0 || d.per_x) ? cx[i + j * rx] : T(0);
-  const T cE = (i < d.nx - 1 || d.per_x) ? cx[i + 1 + j * rx] : T(0);
-  const T cS = (j > 0 || d.per_y) ? cy[i + o] : T(0);
-  const T cN = (j < d.ny - 1 || d.per_y) ? cy[i + o + d.nx] : T(0);
+  const int rx = d.nx + 1, o = j * d.nx;
+  const T vW = cx[i + j * rx], vE = cx[i + 1 + j * rx], vS = cy[i + o], vN = cy[i + o + d.nx];
   const int iw = mg_wrap(i - 1, d.nx, d.per_x), ie = mg_wrap(i + 1, d.nx, d.per_x);
   const int js = mg_wrap(j - 1, d.ny, d.per_y), jn = mg_wrap(j + 1, d.ny, d.per_y);
-  return cW * phi[iw + o] + cE * phi[ie + o] + cS * phi[i + (size_t)js * d.nx] + cN * phi[i + (size_t)jn * d.nx];
+  const T pW = phi[iw + o], pE = phi[ie + o], pS = phi[i + js * d.nx], pN = phi[i + jn * d.nx];
+  const T cW = (i > 0 || d.per_x) ? vW : T(0);
+  const T cE = (i < d.nx - 1 || d.per_x) ? vE : T(0);
+  const T cS = (j > 0 || d.per_y) ? vS : T(0);
+  const T cN = (j < d.ny - 1 || d.per_y) ? vN : T(0);
+  return cW * pW + cE * pE + cS * pS + cN * pN;
 }
 
 // One colour of a red-black Gauss-Seidel sweep; thread (t, j) owns cell i = 2t + ((j + colour) & 1).
@@ -188,25 +193,30 @@ __global__ void __launch_bounds__(256) k_mg_prolong(MgDims f, const T* __restric
 // solved exactly by its dense inverse (k_mg_coarse_inv, once per set-up; for the singular
 // block the inverse of A + s 11^T / n, which maps a centred rhs to the zero-mean solution),
 // otherwise by `sweeps` LDS-resident red-black sweeps from zero.
+// Any workgroup size of at least one wavefront (cells strided over the threads); the rhs mean,
+// when centred, is summed by the first wavefront in a fixed order (lane l: cells l, l + 64, ...;
+// then across lanes), the same whatever the workgroup size.
 template <typename T>
 __device__ inline void mg_coarse_solve(const MgDims& d, const T* cx, const T* cy, const T* dinv, const T* cinv,
                                        const T* b, T* phi, int sweeps, int project, T* sphi, T* sb, double* red) {
   const int n = d.nx * d.ny;
-  const int t = threadIdx.x;
-  double my = 0.0;
-  if (t < n) { sb[t] = b[t]; sphi[t] = T(0); my = (double)sb[t]; }
-  if (project) {
-    my = wave_sum(my);
-    if ((t & 63) == 0) red[t >> 6] = my;
-    __syncthreads();
-    double m = 0.0;
-    for (int w = 0; w < (int)(blockDim.x >> 6); ++w) m += red[w];
-    m /= (double)n;
-    if (t < n) sb[t] = sb[t] - (T)m;
-  }
+  const int nt = (int)blockDim.x;
+  for (int t = threadIdx.x; t < n; t += nt) { sb[t] = b[t]; sphi[t] = T(0); }
   __syncthreads();
+  if (project) {
+    if (threadIdx.x < 64) {
+      double my = 0.0;
+      for (int t = threadIdx.x; t < n; t += 64) my += (double)sb[t];
+      my = wave_sum(my);
+      if (threadIdx.x == 0) red[0] = my / (double)n;
+    }
+    __syncthreads();
+    const double m = red[0];
+    for (int t = threadIdx.x; t < n; t += nt) sb[t] = sb[t] - (T)m;
+    __syncthreads();
+  }
   if (cinv) {
-    if (t < n) {
+    for (int t = threadIdx.x; t < n; t += nt) {
       double s = 0.0;
       for (int q = 0; q < n; ++q) s += (double)cinv[(size_t)t * n + q] * (double)sb[q];
       phi[t] = (T)s;
@@ -214,14 +224,16 @@ __device__ inline void mg_coarse_solve(const MgDims& d, const T* cx, const T* cy
     __syncthreads();
     return;
   }
-  const int i = (t < n) ? t % d.nx : 0, j = (t < n) ? t / d.nx : 0;
   for (int s = 0; s < sweeps; ++s) {
     for (int colour = 0; colour < 2; ++colour) {
-      if (t < n && ((i + j) & 1) == colour) sphi[t] = (sb[t] + mg_nbr_sum(d, cx, cy, (const T*)sphi, i, j)) * dinv[t];
+      for (int t = threadIdx.x; t < n; t += nt) {
+        const int i = t % d.nx, j = t / d.nx;
+        if (((i + j) & 1) == colour) sphi[t] = (sb[t] + mg_nbr_sum(d, cx, cy, (const T*)sphi, i, j)) * dinv[t];
+      }
       __syncthreads();
     }
   }
-  if (t < n) phi[t] = sphi[t];
+  for (int t = threadIdx.x; t < n; t += nt) phi[t] = sphi[t];
   __syncthreads();
 }
 
@@ -516,6 +528,7 @@ __global__ void __launch_bounds__(256) k_mg_sum(MgDims d, const T* __restrict__ 
 // its result is bit-identical; only the top level's phi is written back.
 constexpr int MG_TAIL_LEVELS = 16;
 constexpr int MG_TAIL_POOL = 136 * 1024;
+constexpr int MG_TAIL_NTH = 1024;   // threads of the tail workgroup
 template <typename T>
 struct MgTailLevel {
   MgDims d;
@@ -531,35 +544,35 @@ struct MgTailArgs {
   int nlev, nu, coarse_sweeps, project, zero_guess;
 };
 
-// One colour of a red-black sweep in the tail (1024 threads as 32 x 32).  phic (the first
+// One colour of a red-black sweep in the tail (NTH threads as 32 x NTH/32).  phic (the first
 // post-smoothing half-sweep): the neighbours' values are phi + phi_c(parent), the prolongation
 // done on the fly -- the only reader of the prolonged values of this colour's neighbours, whose
 // own cells are overwritten before anything else reads them.
-template <typename T>
+template <typename T, int NTH>
 __device__ inline void tail_smooth(const MgDims d, const T* cx, const T* cy, const T* dinv, const T* b, T* phi,
                                    int colour, int zero_other, const T* phic = nullptr) {
   const int half = (d.nx + 1) / 2;
   const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
   const int nxc = (d.nx + 1) / 2;
-  for (int j = ty; j < d.ny; j += 32) {
+  for (int j = ty; j < d.ny; j += NTH / 32) {
     for (int t = tx; t < half; t += 32) {
       const int i = 2 * t + ((j + colour) & 1);
       if (i >= d.nx) continue;
-      const size_t o = i + (size_t)j * d.nx;
+      const int o = i + j * d.nx;
       T s = T(0);
       if (!zero_other) {
         if (phic) {
-          const size_t rx = (size_t)(d.nx + 1), ro = (size_t)j * d.nx;
-          const T cW = (i > 0 || d.per_x) ? cx[i + j * rx] : T(0);
-          const T cE = (i < d.nx - 1 || d.per_x) ? cx[i + 1 + j * rx] : T(0);
-          const T cS = (j > 0 || d.per_y) ? cy[i + ro] : T(0);
-          const T cN = (j < d.ny - 1 || d.per_y) ? cy[i + ro + d.nx] : T(0);
+          const int rx = d.nx + 1, ro = j * d.nx;
+          const T vW = cx[i + j * rx], vE = cx[i + 1 + j * rx], vS = cy[i + ro], vN = cy[i + ro + d.nx];
           const int iw = mg_wrap(i - 1, d.nx, d.per_x), ie = mg_wrap(i + 1, d.nx, d.per_x);
           const int js = mg_wrap(j - 1, d.ny, d.per_y), jn = mg_wrap(j + 1, d.ny, d.per_y);
-          auto pv = [&](int ii, int jj) -> T {
-            return phi[ii + (size_t)jj * d.nx] + phic[(ii >> 1) + (size_t)(jj >> 1) * nxc];
-          };
-          s = cW * pv(iw, j) + cE * pv(ie, j) + cS * pv(i, js) + cN * pv(i, jn);
+          auto pv = [&](int ii, int jj) -> T { return phi[ii + jj * d.nx] + phic[(ii >> 1) + (jj >> 1) * nxc]; };
+          const T pW = pv(iw, j), pE = pv(ie, j), pS = pv(i, js), pN = pv(i, jn);
+          const T cW = (i > 0 || d.per_x) ? vW : T(0);
+          const T cE = (i < d.nx - 1 || d.per_x) ? vE : T(0);
+          const T cS = (j > 0 || d.per_y) ? vS : T(0);
+          const T cN = (j < d.ny - 1 || d.per_y) ? vN : T(0);
+          s = cW * pW + cE * pE + cS * pS + cN * pN;
         } else {
           s = mg_nbr_sum(d, cx, cy, (const T*)phi, i, j);
         }
@@ -575,8 +588,8 @@ __device__ inline void tail_copy(T* dst, const T* src, int n) {
   for (int q = (int)threadIdx.x; q < n; q += (int)blockDim.x) dst[q] = src[q];
 }
 
-template <typename T, bool STAGE>
-__global__ void __launch_bounds__(1024) k_mg_tail(MgTailArgs<T> a) {
+template <typename T, bool STAGE, int NTH>
+__global__ void __launch_bounds__(NTH) k_mg_tail(MgTailArgs<T> a) {
   __shared__ T sphi[MG_COARSE_MAX];
   __shared__ T sb[MG_COARSE_MAX];
   __shared__ double red[16];
@@ -589,16 +602,16 @@ __global__ void __launch_bounds__(1024) k_mg_tail(MgTailArgs<T> a) {
   auto BB = [&](int l) -> T* { return STAGE ? pool + a.L[l].ob : a.L[l].b; };
   auto PH = [&](int l) -> T* { return STAGE ? pool + a.L[l].ophi : a.L[l].phi; };
   if (STAGE) {
-    // every load issued before the first LDS store (1024 threads, a fixed number per thread);
+    // every load issued before the first LDS store (NTH threads, a fixed number per thread);
     // the top level's phi comes from its own array, or is zero for a zero guess (finite values
     // are needed even then: at a wall the smoother's neighbour sum reads the cell itself times a
     // zero conductance)
-    constexpr int KMAX = (MG_TAIL_POOL / (int)sizeof(T) + 1023) / 1024;
+    constexpr int KMAX = (MG_TAIL_POOL / (int)sizeof(T) + NTH - 1) / NTH;
     const int p0 = a.L[0].ophi, n0 = a.L[0].d.nx * a.L[0].d.ny;
     T v[KMAX];
 #pragma unroll
     for (int k = 0; k < KMAX; ++k) {
-      const int q = (int)threadIdx.x + 1024 * k;
+      const int q = (int)threadIdx.x + NTH * k;
       const int r = q - p0;
       v[k] = T(0);
       if (q < a.nblk) {
@@ -610,7 +623,7 @@ __global__ void __launch_bounds__(1024) k_mg_tail(MgTailArgs<T> a) {
     }
 #pragma unroll
     for (int k = 0; k < KMAX; ++k) {
-      const int q = (int)threadIdx.x + 1024 * k;
+      const int q = (int)threadIdx.x + NTH * k;
       if (q < a.nblk) pool[q] = v[k];
     }
     __syncthreads();
@@ -623,13 +636,13 @@ __global__ void __launch_bounds__(1024) k_mg_tail(MgTailArgs<T> a) {
     T* phi = PH(l);
     const int zg = (l == 0) ? a.zero_guess : 1;
     for (int s = 0; s < nu; ++s) {
-      tail_smooth(f, cx, cy, dinv, b, phi, 0, (s == 0 && zg) ? 1 : 0);
-      tail_smooth(f, cx, cy, dinv, b, phi, 1, 0);
+      tail_smooth<T, NTH>(f, cx, cy, dinv, b, phi, 0, (s == 0 && zg) ? 1 : 0);
+      tail_smooth<T, NTH>(f, cx, cy, dinv, b, phi, 1, 0);
     }
     const int nxc = (f.nx + 1) / 2, nyc = (f.ny + 1) / 2;
     T* bc = BB(l + 1);
     T* phic = PH(l + 1);
-    for (int J = (int)threadIdx.x >> 5; J < nyc; J += 32) {
+    for (int J = (int)threadIdx.x >> 5; J < nyc; J += NTH / 32) {
       for (int I = (int)threadIdx.x & 31; I < nxc; I += 32) {
         T acc = T(0);
 #pragma unroll
@@ -662,8 +675,8 @@ __global__ void __launch_bounds__(1024) k_mg_tail(MgTailArgs<T> a) {
     T* phi = PH(l);
     const T* phic = PH(l + 1);
     for (int s = 0; s < nu; ++s) {
-      tail_smooth(f, cx, cy, dinv, b, phi, 1, 0, s == 0 ? phic : (const T*)nullptr);
-      tail_smooth(f, cx, cy, dinv, b, phi, 0, 0);
+      tail_smooth<T, NTH>(f, cx, cy, dinv, b, phi, 1, 0, s == 0 ? phic : (const T*)nullptr);
+      tail_smooth<T, NTH>(f, cx, cy, dinv, b, phi, 0, 0);
     }
   }
   if (STAGE) {

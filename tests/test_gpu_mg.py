@@ -81,17 +81,19 @@ def test_coarse_tail_is_bitwise_the_multilaunch_cycle(monkeypatch, nx, ny, per_x
             assert np.array_equal(a, b), key
 
 
-@pytest.mark.parametrize("theta", ["0", "3", "1e9"])
+@pytest.mark.parametrize("theta,coarse_nu", [("0", "2"), ("3", "2"), ("10", "4"), ("10", "3"), ("1e9", "4")])
 @pytest.mark.parametrize("mixed", [False, True])
-def test_coarse_tail_bitwise_in_the_step_solve(monkeypatch, mixed, theta):
+def test_coarse_tail_bitwise_in_the_step_solve(monkeypatch, mixed, theta, coarse_nu):
     """A refined step solve (pressure and velocity hierarchies: mass terms, Dirichlet faces, FP64
     or FP32) is bit-identical with the coarse tail and fused level kernels on and off -- with the
-    velocity hierarchies at full depth (theta 0), cut by their diffusion-to-mass ratio (3) and
-    cut to the finest level alone (1e9, whose two sweeps are then the whole velocity block)."""
+    velocity hierarchies at full depth (theta 0), cut by their diffusion-to-mass ratio (3, 10; the
+    cut coarsest level smoothed by 2, 3 or 4 sweeps) and cut to the finest level alone (1e9,
+    whose sweeps are then the whole velocity block)."""
     import bench
     nx, ny = 96, 192
     outs = []
     monkeypatch.setenv("FUGU_VMG_THETA", theta)
+    monkeypatch.setenv("FUGU_VMG_COARSE_NU", coarse_nu)
     for cells, fused in (("0", "0"), ("-1", "1")):
         monkeypatch.setenv("FUGU_MG_TAIL_CELLS", cells)
         monkeypatch.setenv("FUGU_MG_FUSED", fused)
