@@ -679,7 +679,7 @@ T* mg_vcycle(fugu_ctx* c, std::vector<MgLevel<T>>& lv, size_t l, bool zero_guess
       if (c->mg_fused && zero_guess) {
         const dim3 gt((unsigned)((L.d.nx + MGT_W - 1) / MGT_W), (unsigned)((L.d.ny + MGT_H - 1) / MGT_H));
         const double ab = 5.0 * sizeof(T) * L.d.nx * L.d.ny;
-        const char* nm = sizeof(T) == 8 ? "mg_pre_f64" : "mg_pre_f32";
+        const char* nm = sizeof(T) == 8 ? "mg_cut_f64" : "mg_cut_f32";
         if (cnu == 3)
           TLAUNCH(c, nm, ab, (k_mg_pre<T, 3>), gt, dim3(256), c->s, L.d, (const T*)L.cx, (const T*)L.cy,
                   (const T*)L.m, (const T*)L.b, L.phi, (T*)nullptr, (T*)nullptr);

@@ -549,21 +549,37 @@ constexpr int FG_M = 48;
 constexpr int FG_CS = (FG_M + 1) * FG_M, FG_SN = FG_CS + FG_M, FG_G = FG_SN + FG_M, FG_KK = FG_G + FG_M + 1;
 constexpr int FG_SIZE = FG_KK + 8;
 
-__device__ inline void fgmres_step_dev(int k, double* __restrict__ sc, double* __restrict__ fg, int* __restrict__ fl) {
+// Called by every thread of a workgroup (uniformly): the new column's entries and the previous
+// rotations are staged in LDS by all threads, then one thread applies the rotations with the
+// running entry carried in a register (unrolled, so the LDS reads of several steps are in flight
+// together; the dependent chain is arithmetic only) -- the same operations in the same order as
+// rotating the column in place.
+__device__ inline void fgmres_step_blk(int k, double* __restrict__ sc, double* __restrict__ fg, int* __restrict__ fl) {
+  __shared__ double hcol[FG_M + 1], rc[FG_M], rs[FG_M];
   if (fl[F_FG_DONE]) return;
+  for (int i = threadIdx.x; i <= k; i += blockDim.x) {
+    hcol[i] = sc[S_HBUF + i] + sc[S_HBUF + 64 + i];
+    if (i < k) {
+      rc[i] = fg[FG_CS + i];
+      rs[i] = fg[FG_SN + i];
+    }
+  }
+  __syncthreads();
+  if (threadIdx.x != 0) return;
   const double tol_abs = sc[S_FG_TOL];
   double* H = fg + (size_t)k * (FG_M + 1);
-  for (int i = 0; i <= k; ++i) H[i] = sc[S_HBUF + i] + sc[S_HBUF + 64 + i];
   const double hn = sqrt(sc[S_DOT0]);
-  H[k + 1] = hn;
+  double cur = hcol[0];   // H[i] as rotated so far
+#pragma unroll 8
   for (int i = 0; i < k; ++i) {
-    const double c = fg[FG_CS + i], s = fg[FG_SN + i];
-    const double t = c * H[i] + s * H[i + 1];
-    H[i + 1] = -s * H[i] + c * H[i + 1];
+    const double nxt = hcol[i + 1];
+    const double c = rc[i], s = rs[i];
+    const double t = c * cur + s * nxt;
+    cur = -s * cur + c * nxt;
     H[i] = t;
   }
-  const double den = hypot(H[k], H[k + 1]);
-  const double c = den > 0.0 ? H[k] / den : 1.0, s = den > 0.0 ? H[k + 1] / den : 0.0;
+  const double den = hypot(cur, hn);
+  const double c = den > 0.0 ? cur / den : 1.0, s = den > 0.0 ? hn / den : 0.0;
   fg[FG_CS + k] = c;
   fg[FG_SN + k] = s;
   H[k] = den;
@@ -642,16 +658,16 @@ __global__ void __launch_bounds__(256) k_maxpy_dot(MaxpyArgs<V, V> a, const doub
   }
   __shared__ double lds[16][NQ + 1];
   block_sum<NQ + 1>(acc, lds);
-  if (threadIdx.x == 0) {
+  if (threadIdx.x == 0) {   // column-major (value q of every workgroup contiguous): coalesced folds
 #pragma unroll
-    for (int q = 0; q <= NQ; ++q) partials[(size_t)blockIdx.x * (NQ + 1) + q] = acc[q];
+    for (int q = 0; q <= NQ; ++q) partials[(size_t)q * gridDim.x + blockIdx.x] = acc[q];
   }
 }
 // Fold of k_maxpy_dot: sc[S_DOT0] = vol ||o||^2, fl[F_REORTH] = (||o||^2 < frac ||w||^2, the
 // Kahan-Parlett test against sc[S_DOT1] = vol ||w||^2, frac = 1/2 its usual threshold), and the
 // second pass's coefficients
 // sc[S_HBUF + 64 + q] = vol <o, x_q> when it runs, zero otherwise.  fg_k >= 0: when the second
-// pass does not run, the FGMRES step k (fgmres_step_dev) follows here, in the same launch.
+// pass does not run, the FGMRES step k (fgmres_step_blk) follows here, in the same launch.
 __global__ void __launch_bounds__(1024) k_fold_maxpy_dot(int nb, int nq, int stride, double vol, double frac,
                                                          const double* __restrict__ partials,
                                                          double* __restrict__ sc, int* __restrict__ fl, int fg_k,
@@ -661,8 +677,14 @@ __global__ void __launch_bounds__(1024) k_fold_maxpy_dot(int nb, int nq, int str
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
   for (int q = wid; q <= nq; q += (int)(blockDim.x >> 6)) {
     const int col = q < nq ? q : stride - 1;
+    // nb <= RED_MAXB: every load of the lane issued before the first add (same order of adds)
+    double v[RED_MAXB / 64];
+#pragma unroll
+    for (int r = 0; r < RED_MAXB / 64; ++r) v[r] = (lane + 64 * r < nb) ? partials[(size_t)col * nb + lane + 64 * r] : 0.0;
     double s = 0.0;
-    for (int b = lane; b < nb; b += 64) s += partials[(size_t)b * stride + col];
+#pragma unroll
+    for (int r = 0; r < RED_MAXB / 64; ++r)
+      if (lane + 64 * r < nb) s += v[r];
     s = wave_sum(s);
     if (lane == 0) red[q] = s * vol;
   }
@@ -676,7 +698,7 @@ __global__ void __launch_bounds__(1024) k_fold_maxpy_dot(int nb, int nq, int str
   if ((int)threadIdx.x < nq) sc[S_HBUF + 64 + threadIdx.x] = reorth ? red[threadIdx.x] : 0.0;
   if (fg_k >= 0 && !reorth) {
     __syncthreads();
-    if (threadIdx.x == 0) fgmres_step_dev(fg_k, sc, fg, fl);
+    fgmres_step_blk(fg_k, sc, fg, fl);
   }
 }
 
@@ -719,10 +741,10 @@ __global__ void __launch_bounds__(256) k_mdot(MdotArgs<V> a, double* __restrict_
   }
   __shared__ double lds[16][MD_CH + 1];
   block_sum<MD_CH + 1>(acc, lds);
-  if (threadIdx.x == 0) {
+  if (threadIdx.x == 0) {   // column-major per chunk: coalesced folds
 #pragma unroll
     for (int q = 0; q <= MD_CH; ++q)
-      partials[((size_t)blockIdx.y * MD_BX + blockIdx.x) * (MD_CH + 1) + q] = acc[q];
+      partials[((size_t)blockIdx.y * (MD_CH + 1) + q) * MD_BX + blockIdx.x] = acc[q];
   }
 }
 // Fold of k_mdot: one wavefront per basis vector; sc[S_HBUF + off + q] = vol * sum, and
@@ -735,8 +757,13 @@ __global__ void __launch_bounds__(1024) k_mdot_fold(int nv, int off, double vol,
   for (int q = wid; q <= nv; q += (int)(blockDim.x >> 6)) {
     const int ch = q < nv ? q / MD_CH : 0, qq = q < nv ? q % MD_CH : MD_CH;
     double s = 0.0;
-    if (!skip)
-      for (int b = lane; b < MD_BX; b += 64) s += partials[((size_t)ch * MD_BX + b) * (MD_CH + 1) + qq];
+    if (!skip) {
+      double v[MD_BX / 64];   // every load of the lane issued before the first add
+#pragma unroll
+      for (int r = 0; r < MD_BX / 64; ++r) v[r] = partials[((size_t)ch * (MD_CH + 1) + qq) * MD_BX + lane + 64 * r];
+#pragma unroll
+      for (int r = 0; r < MD_BX / 64; ++r) s += v[r];
+    }
     s = wave_sum(s);
     if (lane == 0) {
       if (q < nv) sc[S_HBUF + off + q] = s * vol;
@@ -759,7 +786,10 @@ __global__ void __launch_bounds__(NT) k_fold_gs_norm(int nb, int mode, double vo
   if (threadIdx.x == 0) {
     sc[S_DOT0] = vol * v[0];
     if (mode == 1) fl[F_REORTH] = (vol * v[0] < 0.5 * sc[S_DOT1]) ? 1 : 0;
-    if (mode == 2 && fg_k >= 0) fgmres_step_dev(fg_k, sc, fg, fl);
+  }
+  if (mode == 2 && fg_k >= 0) {
+    __syncthreads();
+    fgmres_step_blk(fg_k, sc, fg, fl);
   }
 }
 
@@ -775,7 +805,7 @@ __global__ void __launch_bounds__(NT) k_fold_plain(int nb, int slot, double scal
 
 // ---------------------------------------------------------------------------------
 // Outer FGMRES cycle start and end (one thread each); the steps run inside the Gram-Schmidt
-// folds above (fgmres_step_dev).
+// folds above (fgmres_step_blk).
 __global__ void k_fgmres_init(double* __restrict__ sc, double* __restrict__ fg, int* __restrict__ fl) {
   const double beta = sqrt(sc[S_DOT0]);
   for (int i = 0; i <= FG_M; ++i) fg[FG_G + i] = 0.0;
@@ -785,9 +815,6 @@ __global__ void k_fgmres_init(double* __restrict__ sc, double* __restrict__ fg, 
   fl[F_FG_DONE] = beta > 0.0 ? 0 : 1;
 }
 
-__global__ void k_fgmres_step(int k, double* __restrict__ sc, double* __restrict__ fg, int* __restrict__ fl) {
-  fgmres_step_dev(k, sc, fg, fl);
-}
 
 // y = H^-1 g (back substitution) into slots S_HBUF + 128 + i; returns kk in S_HBUF + 127.
 __global__ void k_fgmres_solve(double* __restrict__ sc, const double* __restrict__ fg) {
