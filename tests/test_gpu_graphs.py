@@ -14,6 +14,7 @@ abi = pytest.importorskip("coupled_codefort_2d_amd.abi")
 def run_steps(monkeypatch, graphs, nx, ny, per_x, mixed, mg_cycles, dts):
     import bench
     monkeypatch.setenv("FUGU_GRAPHS", "1" if graphs else "0")
+    monkeypatch.setenv("FUGU_VMG_CYCLES", str(mg_cycles))   # two V-cycles: a copy node in the graph
     p = abi.FuguParams(1000.0, 1.0, 10.0, 0.1, 2.0, 0.0, -1.0, 2, 1e12, 0.5)
     ctx = abi.Context(nx, ny, 1.0, 2.0, per_x, False, params=p)
     nh = 2

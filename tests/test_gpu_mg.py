@@ -81,19 +81,22 @@ def test_coarse_tail_is_bitwise_the_multilaunch_cycle(monkeypatch, nx, ny, per_x
             assert np.array_equal(a, b), key
 
 
-@pytest.mark.parametrize("theta,coarse_nu", [("0", "2"), ("3", "2"), ("10", "4"), ("10", "3"), ("1e9", "4")])
+@pytest.mark.parametrize("theta,coarse_nu,cycles", [("0", "2", "1"), ("3", "2", "1"), ("10", "4", "0"), ("10", "3", "2"),
+                                                   ("1e9", "4", "2")])
 @pytest.mark.parametrize("mixed", [False, True])
-def test_coarse_tail_bitwise_in_the_step_solve(monkeypatch, mixed, theta, coarse_nu):
+def test_coarse_tail_bitwise_in_the_step_solve(monkeypatch, mixed, theta, coarse_nu, cycles):
     """A refined step solve (pressure and velocity hierarchies: mass terms, Dirichlet faces, FP64
     or FP32) is bit-identical with the coarse tail and fused level kernels on and off -- with the
     velocity hierarchies at full depth (theta 0), cut by their diffusion-to-mass ratio (3, 10; the
     cut coarsest level smoothed by 2, 3 or 4 sweeps) and cut to the finest level alone (1e9,
-    whose sweeps are then the whole velocity block)."""
+    whose sweeps are then the whole velocity block), with one, two or the ratio-chosen number of
+    V-cycles per velocity block."""
     import bench
     nx, ny = 96, 192
     outs = []
     monkeypatch.setenv("FUGU_VMG_THETA", theta)
     monkeypatch.setenv("FUGU_VMG_COARSE_NU", coarse_nu)
+    monkeypatch.setenv("FUGU_VMG_CYCLES", cycles)
     for cells, fused in (("0", "0"), ("-1", "1")):
         monkeypatch.setenv("FUGU_MG_TAIL_CELLS", cells)
         monkeypatch.setenv("FUGU_MG_FUSED", fused)
