@@ -155,6 +155,27 @@ def roofline(ktimes, tag="rising_bubble_512x1024"):
     # measured HBM bytes / algorithmic bytes of this kernel on this workload (two rocprofv3
     # --pmc runs, FETCH_SIZE and WRITE_SIZE, scaled by a plain copy; hbm_traffic.py), one line
     # per kernel: "name kernel launches traffic algorithmic ratio fetch_factor write_factor"
+    # the same kernel's average duration in the committed rocprofv3 --kernel-trace --stats summary
+    # of this workload: HIP events around a launch also count its dispatch latency (a few us),
+    # which the profiler's begin/end timestamps leave out
+    trace = os.path.join(ROOT, "profiles", f"kernel_trace_{tag}.md")
+    if os.path.exists(trace):
+        key = device_name(name)
+        exact = key if "," in key else None
+        prefix = key[:-1] + "," if "," not in key else None
+        tot_ms, calls = 0.0, 0
+        for ln in open(trace):
+            f = [x.strip() for x in ln.split("|")]
+            if len(f) < 6 or not f[1].startswith("`"):
+                continue
+            kname = f[1].strip("`")
+            if (exact and kname == exact) or (prefix and kname.startswith(prefix)):
+                tot_ms += float(f[3]); calls += int(f[2])
+        if calls:
+            pavg = tot_ms * 1e3 / calls
+            out["profile_avg_launch_us"] = round(pavg, 3)
+            out["frac_at_profile_avg"] = round(bytes_launch / (pavg * 1e-6) / 1e9 / HBM_PEAK_GBS, 4)
+            out["profile_source"] = os.path.relpath(trace, ROOT)
     prof = os.path.join(ROOT, "profiles", f"hbm_traffic_{tag}.txt")
     if os.path.exists(prof):
         for ln in open(prof):
@@ -353,8 +374,9 @@ def main():
     adj = run_adjoint(abi, also_resident=True, mg_cycles=2) if "adjoint" in sections else None
     adj2048 = None
     if "adjoint2048" in sections:
+        # two pressure V-cycles: measured 72 s/grad against 93 s with one
         adj2048 = run_adjoint(abi, n=2048, nsteps=32, count=256, seed=5, rmin=0.005, rmax=0.02, rho_c=1000.0,
-                              sigma=1.0, mixed=True)
+                              sigma=1.0, mixed=True, mg_cycles=2)
 
     def summary(f):
         sts = f["stats"]
