@@ -645,12 +645,21 @@ size_t mg_tail_start(const fugu_ctx* c, const std::vector<MgLevel<T>>& lv, size_
 // launches always touch the same buffers (a captured cycle replays correctly).
 // psum (level 0 only): where a fused post-smoothing writes the result, its per-tile double-double
 // sums go there and *psum_nb = their count (left 0 when no such launch ran or they do not fit).
+// guess (not zero_guess): the buffer holding the initial iterate (nullptr: phi).  The fused
+// kernels read it where it lies and write the other buffer; the in-place paths get it copied
+// into phi first.
 template <typename T>
 T* mg_vcycle(fugu_ctx* c, std::vector<MgLevel<T>>& lv, size_t l, bool zero_guess, int project, size_t nl,
-             double* psum = nullptr, int* psum_nb = nullptr) {
+             double* psum = nullptr, int* psum_nb = nullptr, const T* guess = nullptr) {
   MgLevel<T>& L = lv[l];
   const int nu = 2;
+  const T* g_in = zero_guess ? nullptr : (guess ? guess : L.phi);
+  auto guess_to_phi = [&]() {
+    if (g_in && g_in != L.phi)
+      HIPCHK(hipMemcpyAsync(L.phi, g_in, (size_t)L.d.nx * L.d.ny * sizeof(T), hipMemcpyDeviceToDevice, c->s));
+  };
   if (l == mg_tail_start(c, lv, nl)) {   // the coarse tail in one launch
+    guess_to_phi();
     MgTailArgs<T> ta{};
     ta.nlev = (int)(nl - l);
     int off = 0;
@@ -686,22 +695,25 @@ T* mg_vcycle(fugu_ctx* c, std::vector<MgLevel<T>>& lv, size_t l, bool zero_guess
   if (l + 1 == nl) {
     if (L.m && (size_t)L.d.nx * L.d.ny > (size_t)MG_COARSE_MAX) {
       // truncated velocity hierarchy: the mass term dominates its coarsest level, which gets
-      // vmg_coarse_nu red-black sweeps from zero (one fused launch)
+      // vmg_coarse_nu red-black sweeps (one fused launch)
       const int cnu = c->vmg_coarse_nu;
-      if (c->mg_fused && zero_guess) {
+      if (c->mg_fused) {
         const dim3 gt((unsigned)((L.d.nx + MGT_W - 1) / MGT_W), (unsigned)((L.d.ny + MGT_H - 1) / MGT_H));
-        const double ab = 5.0 * sizeof(T) * L.d.nx * L.d.ny;
+        const double ab = (g_in ? 6.0 : 5.0) * sizeof(T) * L.d.nx * L.d.ny;
         const char* nm = sizeof(T) == 8 ? "mg_cut_f64" : "mg_cut_f32";
+        T* out = (g_in == L.phi) ? L.phi2 : L.phi;
         if (cnu == 3)
           TLAUNCH(c, nm, ab, (k_mg_pre<T, 3>), gt, dim3(256), c->s, L.d, (const T*)L.cx, (const T*)L.cy,
-                  (const T*)L.m, (const T*)L.b, L.phi, (T*)nullptr, (T*)nullptr);
+                  (const T*)L.m, (const T*)L.b, out, (T*)nullptr, (T*)nullptr, g_in);
         else if (cnu == 4)
           TLAUNCH(c, nm, ab, (k_mg_pre<T, 4>), gt, dim3(256), c->s, L.d, (const T*)L.cx, (const T*)L.cy,
-                  (const T*)L.m, (const T*)L.b, L.phi, (T*)nullptr, (T*)nullptr);
+                  (const T*)L.m, (const T*)L.b, out, (T*)nullptr, (T*)nullptr, g_in);
         else
           TLAUNCH(c, nm, ab, (k_mg_pre<T, 2>), gt, dim3(256), c->s, L.d, (const T*)L.cx, (const T*)L.cy,
-                  (const T*)L.m, (const T*)L.b, L.phi, (T*)nullptr, (T*)nullptr);
+                  (const T*)L.m, (const T*)L.b, out, (T*)nullptr, (T*)nullptr, g_in);
+        return out;
       } else {
+        guess_to_phi();
         for (int s = 0; s < (cnu == 3 || cnu == 4 ? cnu : 2); ++s) {
           mg_smooth(c, L, 0, (s == 0 && zero_guess) ? 1 : 0);
           mg_smooth(c, L, 1, 0);
@@ -711,6 +723,7 @@ T* mg_vcycle(fugu_ctx* c, std::vector<MgLevel<T>>& lv, size_t l, bool zero_guess
       LAUNCH((k_mg_coarse<T>), dim3(1), dim3(MG_COARSE_MAX), c->s, L.d, (const T*)L.cx, (const T*)L.cy,
              (const T*)L.dinv, (const T*)L.cinv, (const T*)L.b, L.phi, 60, project);
     } else {
+      guess_to_phi();
       for (int s = 0; s < 40; ++s) {
         mg_smooth(c, L, 0, (s == 0 && zero_guess) ? 1 : 0);
         mg_smooth(c, L, 1, 0);
@@ -721,12 +734,15 @@ T* mg_vcycle(fugu_ctx* c, std::vector<MgLevel<T>>& lv, size_t l, bool zero_guess
   MgLevel<T>& C = lv[l + 1];
   const double ncell = (double)L.d.nx * L.d.ny;
   const dim3 gt((unsigned)((L.d.nx + MGT_W - 1) / MGT_W), (unsigned)((L.d.ny + MGT_H - 1) / MGT_H));
-  if (c->mg_fused && zero_guess) {
-    // b, c_x, c_y (, m) read; phi written; coarse b and phi written (a quarter each)
-    TLAUNCH(c, sizeof(T) == 8 ? "mg_pre_f64" : "mg_pre_f32", (L.m ? 5.5 : 4.5) * sizeof(T) * ncell,
+  T* P = (g_in == L.phi) ? L.phi2 : L.phi;   // the fused pre-smoothing's output
+  if (c->mg_fused) {
+    // b, c_x, c_y (, m) (, the initial iterate) read; phi written; coarse b and phi written (a
+    // quarter each)
+    TLAUNCH(c, sizeof(T) == 8 ? "mg_pre_f64" : "mg_pre_f32", ((L.m ? 5.5 : 4.5) + (g_in ? 1.0 : 0.0)) * sizeof(T) * ncell,
             (k_mg_pre<T, 2>), gt, dim3(256), c->s, L.d, (const T*)L.cx, (const T*)L.cy, (const T*)L.m,
-            (const T*)L.b, L.phi, C.b, C.phi);
+            (const T*)L.b, P, C.b, C.phi, g_in);
   } else {
+    guess_to_phi();
     for (int s = 0; s < nu; ++s) {
       mg_smooth(c, L, 0, (s == 0 && zero_guess) ? 1 : 0);
       mg_smooth(c, L, 1, 0);
@@ -738,12 +754,13 @@ T* mg_vcycle(fugu_ctx* c, std::vector<MgLevel<T>>& lv, size_t l, bool zero_guess
   }
   const T* cres = mg_vcycle(c, lv, l + 1, true, project, nl);
   if (c->mg_fused) {
-    // phi, coarse phi (a quarter), b, c_x, c_y (, m) read; phi written
+    // phi, coarse phi (a quarter), b, c_x, c_y (, m) read; the other buffer written
+    T* Q = (P == L.phi) ? L.phi2 : L.phi;
     TLAUNCH(c, sizeof(T) == 8 ? "mg_post_f64" : "mg_post_f32", (L.m ? 6.25 : 5.25) * sizeof(T) * ncell,
             (k_mg_post<T, 2>), gt, dim3(256), c->s, L.d, (const T*)L.cx, (const T*)L.cy, (const T*)L.m,
-            (const T*)L.b, (const T*)L.phi, cres, L.phi2, psum ? psum : (double*)nullptr);
+            (const T*)L.b, (const T*)P, cres, Q, psum ? psum : (double*)nullptr);
     if (psum && psum_nb) *psum_nb = (int)(gt.x * gt.y);
-    return L.phi2;
+    return Q;
   }
   LAUNCH((k_mg_prolong<T>), g2(L.d.nx, L.d.ny), dim3(64, 4), c->s, L.d, cres, L.phi);
   for (int s = 0; s < nu; ++s) {
@@ -765,11 +782,8 @@ void mg_pressure(fugu_ctx* c, Work<T>& w, T* phi, const T* Zp, bool force_proj =
          (int)S_P_MEAN, c->fl, (int)F_P_DONE);
   MgLevel<T>& L0 = w.mg[0];
   const T* res = nullptr;
-  for (int k = 0; k < c->mg_cycles; ++k) {   // later cycles smooth from the last result, in phi
-    if (res && res != L0.phi)
-      HIPCHK(hipMemcpyAsync(L0.phi, res, (size_t)L0.d.nx * L0.d.ny * sizeof(T), hipMemcpyDeviceToDevice, c->s));
-    res = mg_vcycle(c, w.mg, 0, k == 0, 1, w.mg.size());
-  }
+  for (int k = 0; k < c->mg_cycles; ++k)   // later cycles smooth from the last result
+    res = mg_vcycle(c, w.mg, 0, k == 0, 1, w.mg.size(), nullptr, nullptr, res);
   LAUNCH((k_mg_store<T>), g2(g.nx, g.ny), dim3(64, 4), c->s, g, res, phi);
   if (walls_both) cell_mean<T>(c, (const T*)phi, (int)S_P_MEAN, 0);
 }
@@ -792,11 +806,9 @@ const T* mg_pressure_compact(fugu_ctx* c, Work<T>& w, const T* Zp) {
   double* psum = (walls_both && 2 * tiles <= c->partials_n) ? c->partials : nullptr;
   const T* res = nullptr;
   int nb = 0;
-  for (int k = 0; k < c->mg_cycles; ++k) {   // later cycles smooth from the last result, in phi
-    if (res && res != L0.phi)
-      HIPCHK(hipMemcpyAsync(L0.phi, res, (size_t)L0.d.nx * L0.d.ny * sizeof(T), hipMemcpyDeviceToDevice, c->s));
+  for (int k = 0; k < c->mg_cycles; ++k) {   // later cycles smooth from the last result
     nb = 0;
-    res = mg_vcycle(c, w.mg, 0, k == 0, 1, w.mg.size(), k + 1 == c->mg_cycles ? psum : nullptr, &nb);
+    res = mg_vcycle(c, w.mg, 0, k == 0, 1, w.mg.size(), k + 1 == c->mg_cycles ? psum : nullptr, &nb, res);
   }
   if (walls_both) {
     if (nb == 0) {
@@ -819,11 +831,8 @@ void vmg_solve(fugu_ctx* c, Work<T>& w, int comp, const T* rhs, T* out, bool loa
   MgLevel<T>& L = lv[0];
   if (!loaded) LAUNCH((k_vmg_load<T>), g2(L.d.nx, L.d.ny), dim3(64, 4), c->s, g, L.d, rhs, comp, L.b);
   const T* res = nullptr;
-  for (int k = 0; k < w.vmg_cyc[comp]; ++k) {   // later cycles smooth from the last result, in phi
-    if (res && res != L.phi)
-      HIPCHK(hipMemcpyAsync(L.phi, res, (size_t)L.d.nx * L.d.ny * sizeof(T), hipMemcpyDeviceToDevice, c->s));
-    res = mg_vcycle(c, lv, 0, k == 0, 0, w.vmg_nl[comp]);
-  }
+  for (int k = 0; k < w.vmg_cyc[comp]; ++k)   // later cycles smooth from the last result
+    res = mg_vcycle(c, lv, 0, k == 0, 0, w.vmg_nl[comp], nullptr, nullptr, res);
   LAUNCH((k_vmg_store<T>), grid_union(g), dim3(TX, TY), c->s, g, comp, res, rhs, out);
 }
 

@@ -433,10 +433,13 @@ __device__ inline void mgt_store(const MgDims& f, int i0, int j0, T* out, const 
 // NU pre-smoothing sweeps from zero (red first), then (bc set) the residual summed onto the
 // coarse cells of the tile (coarse phi zeroed).  The residual of each tile cell is formed by the thread
 // holding its coefficients and summed from LDS (children in the restriction's order).
+// phi_in (optional): a non-zero initial iterate (another array than phi: tiles read their halos
+// from it while others write their own cells of phi); nullptr: from zero.
 template <typename T, int NU>
 __global__ void __launch_bounds__(256) k_mg_pre(MgDims f, const T* __restrict__ cx, const T* __restrict__ cy,
                                                 const T* __restrict__ m, const T* __restrict__ b,
-                                                T* __restrict__ phi, T* __restrict__ bc, T* __restrict__ phic) {
+                                                T* __restrict__ phi, T* __restrict__ bc, T* __restrict__ phic,
+                                                const T* __restrict__ phi_in) {
   constexpr int H = 2 * NU + 1;
   using M = MgtCells<T, H>;
   __shared__ T s[M::EH][M::EW];
@@ -444,11 +447,11 @@ __global__ void __launch_bounds__(256) k_mg_pre(MgDims f, const T* __restrict__ 
   int i0, j0;
   mgt_tile(i0, j0);
   M cells;
-  // rings 0..H-1 are updated (ring H-1 only by the first, neighbour-free half-sweep); ring H is
-  // never read
-  mgt_init<T, H>(f, i0, j0, cx, cy, m, b, (const T*)nullptr, (const T*)nullptr, H - 1, cells, s);
+  // rings 0..H-1 are updated; from zero, ring H-1 only by the first, neighbour-free half-sweep and
+  // ring H is never read; from phi_in, ring H is read (its phi) by the first half-sweep
+  mgt_init<T, H>(f, i0, j0, cx, cy, m, b, phi_in, (const T*)nullptr, H - 1, cells, s);
 #pragma unroll
-  for (int h = 0; h < 2 * NU; ++h) mgt_half<T, H>(H - 1 - h, h & 1, h == 0 ? 1 : 0, cells, s);
+  for (int h = 0; h < 2 * NU; ++h) mgt_half<T, H>(H - 1 - h, h & 1, (h == 0 && !phi_in) ? 1 : 0, cells, s);
   mgt_store<T, H>(f, i0, j0, phi, s);
   if (!bc) return;   // smoothing only (a truncated hierarchy's coarsest level)
 #pragma unroll
@@ -476,9 +479,9 @@ __global__ void __launch_bounds__(256) k_mg_pre(MgDims f, const T* __restrict__ 
 }
 
 // phi_out = phi + phi_c(parent), then NU post-smoothing sweeps (black first).
-template <typename T, int NU>
 // psum (optional): the tile's sum of its outputs as a double-double partial, one per workgroup
 // (the mean of the level-0 result without another pass over it).
+template <typename T, int NU>
 __global__ void __launch_bounds__(256) k_mg_post(MgDims f, const T* __restrict__ cx, const T* __restrict__ cy,
                                                  const T* __restrict__ m, const T* __restrict__ b,
                                                  const T* __restrict__ phi, const T* __restrict__ phic,
