@@ -301,18 +301,28 @@ __global__ void __launch_bounds__(1024) k_mg_coarse_inv(MgDims d, const T* __res
 // ---------------------------------------------------------------------------------
 // Fused level kernels (temporal blocking): one launch does a level's NU pre-smoothing sweeps
 // and the residual restriction, another the prolongation and NU post-smoothing sweeps, instead
-// of 2 NU + 1 launches each.  A 256-thread workgroup owns a MGT_W x MGT_H tile plus a halo of
-// H cells; each thread owns fixed cells of that extended tile (slot k: cell tid + 256 k, row
+// of 2 NU + 1 launches each.  A workgroup owns a tile (shape below) plus a halo of
+// H cells; each thread owns fixed cells of that extended tile (slot k: cell tid + NT k, row
 // major) and loads their b, D^-1 and four face conductances into registers once, in one round
 // of independent loads; phi of the extended tile lives in LDS.  Half-sweep h updates its colour
 // on the tile grown by H - 1 - h rings, so every value the owner needs is final when read, and
 // the owner writes its tile only.  Tiles are numbered so that each XCD gets a contiguous band
 // of them (its L2 then holds the shared halos).  No launch reads what it writes: the
-// pre-smoothing starts from zero (phi is not read) and the post-smoothing writes a second
-// array, which the caller swaps in.  A cell's colour is that of its wrapped global index and
+// pre-smoothing starts from zero or reads its initial iterate from the other buffer, and the
+// post-smoothing writes the other buffer.  A cell's colour is that of its wrapped global index and
 // each update performs the one-launch-per-half-sweep smoother's arithmetic with the same
 // operands, so the result is bit-identical to it.
-constexpr int MGT_W = 64, MGT_H = 16;
+// Tile shape per precision: 64 x 32 cells and 512 threads in FP32, 64 x 16 and 256 threads in
+// FP64 (measured: the taller tile cuts the halo overhead from 1.9x to 1.5x and is faster at every
+// FP32 level -- 7 instead of 8.5 us on small levels, 31 instead of 37 us at 2048^2 -- while in FP64
+// its register footprint slows the 2048^2 post-smoothing).  One thread per coarse cell of the
+// restriction: 16 * height threads.
+constexpr int MGT_W = 64;
+template <typename T>
+struct Mgt {
+  static constexpr int H = sizeof(T) == 4 ? 32 : 16;
+  static constexpr int NT = 16 * H;
+};
 
 __device__ inline int mgt_wrap(int g, int n, int per) {
   if (g >= 0 && g < n) return g;
@@ -322,18 +332,20 @@ __device__ inline int mgt_wrap(int g, int n, int per) {
 
 // Tile of this workgroup: dispatch order places workgroup w on XCD w mod 8; renumber so that
 // XCD x owns a contiguous range of tiles.
+template <int TH>
 __device__ inline void mgt_tile(int& i0, int& j0) {
   const int W = gridDim.x * gridDim.y;
   const int w = blockIdx.x + blockIdx.y * gridDim.x;
   const int per = W / 8, rem = W % 8, x = w % 8, k = w / 8;
   const int t = (x < rem) ? x * (per + 1) + k : rem * (per + 1) + (x - rem) * per + k;
   i0 = (t % gridDim.x) * MGT_W;
-  j0 = (t / gridDim.x) * MGT_H;
+  j0 = (t / gridDim.x) * TH;
 }
 
 template <typename T, int H>
 struct MgtCells {
-  static constexpr int EW = MGT_W + 2 * H, EH = MGT_H + 2 * H, NS = (EW * EH + 255) / 256;
+  static constexpr int TH = Mgt<T>::H, NT = Mgt<T>::NT;
+  static constexpr int EW = MGT_W + 2 * H, EH = TH + 2 * H, NS = (EW * EH + NT - 1) / NT;
   T b[NS], dinv[NS], cW[NS], cE[NS], cS[NS], cN[NS];
   T dg[NS];                       // unmasked diagonal (+ m): the residual of the restriction
   int li[NS], lj[NS], ring[NS];   // ring: distance outside the tile (0 inside); -1: no cell
@@ -355,7 +367,7 @@ __device__ inline void mgt_init(const MgDims& f, int i0, int j0, const T* cx, co
   T pv[M::NS];
 #pragma unroll
   for (int k = 0; k < M::NS; ++k) {
-    const int idx = tid + 256 * k;
+    const int idx = tid + M::NT * k;
     const int lj = idx / M::EW, li = idx - lj * M::EW;
     m.li[k] = li; m.lj[k] = lj;
     const int i = mgt_wrap(i0 - H + li, f.nx, f.per_x), j = mgt_wrap(j0 - H + lj, f.ny, f.per_y);
@@ -363,7 +375,7 @@ __device__ inline void mgt_init(const MgDims& f, int i0, int j0, const T* cx, co
     dist = max(dist, H - li);
     dist = max(dist, li - (H + MGT_W - 1));
     dist = max(dist, H - lj);
-    dist = max(dist, lj - (H + MGT_H - 1));
+    dist = max(dist, lj - (H + M::TH - 1));
     const bool cell = idx < M::EW * M::EH && i >= 0 && j >= 0;
     m.ring[k] = (cell && dist <= rc) ? dist : -1;
     m.col[k] = (i + j) & 1;
@@ -399,7 +411,7 @@ __device__ inline void mgt_init(const MgDims& f, int i0, int j0, const T* cx, co
   }
 #pragma unroll
   for (int k = 0; k < M::NS; ++k)
-    if (tid + 256 * k < M::EW * M::EH) s[m.lj[k]][m.li[k]] = pv[k];
+    if (tid + M::NT * k < M::EW * M::EH) s[m.lj[k]][m.li[k]] = pv[k];
   __syncthreads();
 }
 
@@ -424,7 +436,7 @@ __device__ inline void mgt_store(const MgDims& f, int i0, int j0, T* out, const 
   const int gi = i0 + tx;
   if (gi >= f.nx) return;
 #pragma unroll
-  for (int qj = ty; qj < MGT_H; qj += 4) {
+  for (int qj = ty; qj < Mgt<T>::H; qj += Mgt<T>::NT / 64) {
     const int gj = j0 + qj;
     if (gj < f.ny) out[gi + (size_t)gj * f.nx] = s[H + qj][H + tx];
   }
@@ -436,16 +448,16 @@ __device__ inline void mgt_store(const MgDims& f, int i0, int j0, T* out, const 
 // phi_in (optional): a non-zero initial iterate (another array than phi: tiles read their halos
 // from it while others write their own cells of phi); nullptr: from zero.
 template <typename T, int NU>
-__global__ void __launch_bounds__(256) k_mg_pre(MgDims f, const T* __restrict__ cx, const T* __restrict__ cy,
+__global__ void __launch_bounds__(Mgt<T>::NT) k_mg_pre(MgDims f, const T* __restrict__ cx, const T* __restrict__ cy,
                                                 const T* __restrict__ m, const T* __restrict__ b,
                                                 T* __restrict__ phi, T* __restrict__ bc, T* __restrict__ phic,
                                                 const T* __restrict__ phi_in) {
   constexpr int H = 2 * NU + 1;
   using M = MgtCells<T, H>;
   __shared__ T s[M::EH][M::EW];
-  __shared__ T sres[MGT_H][MGT_W];
+  __shared__ T sres[Mgt<T>::H][MGT_W];
   int i0, j0;
-  mgt_tile(i0, j0);
+  mgt_tile<Mgt<T>::H>(i0, j0);
   M cells;
   // rings 0..H-1 are updated; from zero, ring H-1 only by the first, neighbour-free half-sweep and
   // ring H is never read; from phi_in, ring H is read (its phi) by the first half-sweep
@@ -464,7 +476,7 @@ __global__ void __launch_bounds__(256) k_mg_pre(MgDims f, const T* __restrict__ 
   }
   __syncthreads();
   const int nxc = (f.nx + 1) / 2, nyc = (f.ny + 1) / 2;
-  const int q = threadIdx.x;   // one coarse cell of the 32 x 8 coarse tile
+  const int q = threadIdx.x;   // one coarse cell of the 32 x (tile height / 2) coarse tile
   const int I = i0 / 2 + (q & 31), J = j0 / 2 + (q >> 5);
   if (I >= nxc || J >= nyc) return;
   T acc = T(0);
@@ -482,7 +494,7 @@ __global__ void __launch_bounds__(256) k_mg_pre(MgDims f, const T* __restrict__ 
 // psum (optional): the tile's sum of its outputs as a double-double partial, one per workgroup
 // (the mean of the level-0 result without another pass over it).
 template <typename T, int NU>
-__global__ void __launch_bounds__(256) k_mg_post(MgDims f, const T* __restrict__ cx, const T* __restrict__ cy,
+__global__ void __launch_bounds__(Mgt<T>::NT) k_mg_post(MgDims f, const T* __restrict__ cx, const T* __restrict__ cy,
                                                  const T* __restrict__ m, const T* __restrict__ b,
                                                  const T* __restrict__ phi, const T* __restrict__ phic,
                                                  T* __restrict__ phi_out, double* __restrict__ psum) {
@@ -490,7 +502,7 @@ __global__ void __launch_bounds__(256) k_mg_post(MgDims f, const T* __restrict__
   using M = MgtCells<T, H>;
   __shared__ T s[M::EH][M::EW];
   int i0, j0;
-  mgt_tile(i0, j0);
+  mgt_tile<Mgt<T>::H>(i0, j0);
   M cells;
   mgt_init<T, H>(f, i0, j0, cx, cy, m, b, phi, phic, H - 1, cells, s);
 #pragma unroll
@@ -501,7 +513,7 @@ __global__ void __launch_bounds__(256) k_mg_post(MgDims f, const T* __restrict__
     const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
     if (i0 + tx < f.nx) {
 #pragma unroll
-      for (int qj = ty; qj < MGT_H; qj += 4)
+      for (int qj = ty; qj < Mgt<T>::H; qj += Mgt<T>::NT / 64)
         if (j0 + qj < f.ny) v[0] = dd_add(v[0], dd_from((double)s[H + qj][H + tx]));
     }
     publish_dd<1>(v, psum);
