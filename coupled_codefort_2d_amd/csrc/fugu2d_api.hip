@@ -674,7 +674,10 @@ T* mg_vcycle(fugu_ctx* c, std::vector<MgLevel<T>>& lv, size_t l, bool zero_guess
       t.ophi = off; off += n;
       ta.L[q - l] = t;
     }
-    ta.nu = nu; ta.coarse_sweeps = 60; ta.project = project; ta.zero_guess = zero_guess ? 1 : 0;
+    // a cut velocity hierarchy's coarsest level gets the cut treatment's sweeps, whatever its size
+    const int cnu = (c->vmg_coarse_nu == 3 || c->vmg_coarse_nu == 4) ? c->vmg_coarse_nu : 2;
+    ta.nu = nu; ta.coarse_sweeps = (nl < lv.size() && lv[nl - 1].m) ? cnu : 60;
+    ta.project = project; ta.zero_guess = zero_guess ? 1 : 0;
     // staged: the levels' arrays must lie in the pool order from lv[l].cx on (the phi of the
     // levels below l excepted: the tail writes them before it reads them)
     bool packed = true;
@@ -693,7 +696,7 @@ T* mg_vcycle(fugu_ctx* c, std::vector<MgLevel<T>>& lv, size_t l, bool zero_guess
     return L.phi;
   }
   if (l + 1 == nl) {
-    if (L.m && (size_t)L.d.nx * L.d.ny > (size_t)MG_COARSE_MAX) {
+    if (L.m && ((size_t)L.d.nx * L.d.ny > (size_t)MG_COARSE_MAX || nl < lv.size())) {
       // truncated velocity hierarchy: the mass term dominates its coarsest level, which gets
       // vmg_coarse_nu red-black sweeps (one fused launch)
       const int cnu = c->vmg_coarse_nu;
