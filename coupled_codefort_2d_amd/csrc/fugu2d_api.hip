@@ -150,6 +150,8 @@ struct fugu_ctx {
   // step counts of full depth on the 512 x 1024 bubble (2 levels instead of 3 at 3 / 2) and on
   // the 2048^2 inclusions (6-7 of 10 levels), with fewer launches per step.
   double vmg_theta = 10.0;
+  double vmg_theta2 = 40.0;   // the cut for blocks solved by two or more V-cycles (measured at 2048^2:
+                              // the step counts of theta 10, 3-4 % less time)
   int vmg_coarse_nu = 4;
   double fg_est = 0.9;       // FGMRES ends a cycle when its residual estimate is below fg_est * tol
   // second Gram-Schmidt pass when ||w'||^2 < reorth_frac ||w||^2: only on severe cancellation
@@ -584,10 +586,11 @@ void vmg_setup(fugu_ctx* c, Work<T>& w, T dt) {
   }
   for (int comp = 0; comp < 2; ++comp) {
     auto& lv = w.vmg[comp];
-    size_t nl = 1;
-    for (double r = r0[comp]; r > c->vmg_theta && nl < lv.size(); r *= 0.25) ++nl;
-    w.vmg_nl[comp] = nl;
     w.vmg_cyc[comp] = c->vmg_cycles > 0 ? c->vmg_cycles : (r0[comp] > c->vmg_cycle_ratio ? 2 : 1);
+    const double theta = (w.vmg_cyc[comp] >= 2 && c->vmg_theta > 0.0) ? c->vmg_theta2 : c->vmg_theta;
+    size_t nl = 1;
+    for (double r = r0[comp]; r > theta && nl < lv.size(); r *= 0.25) ++nl;
+    w.vmg_nl[comp] = nl;
     if (c->debug)
       std::fprintf(stderr, "[fugu] velocity block %d: ratio %.3g, %zu of %zu levels, %d cycle(s)\n", comp, r0[comp], nl,
                    lv.size(), w.vmg_cyc[comp]);
@@ -1969,6 +1972,7 @@ int fugu_ctx_create(const fugu_grid* grid, const fugu_params* prm, fugu_ctx** ou
     if (const char* pc = std::getenv("FUGU_PCD_CROSS")) c->pcd_cross = std::atoi(pc) != 0;
     if (const char* vl = std::getenv("FUGU_VMG_LEVELS")) c->vmg_levels = std::atoi(vl);
     if (const char* vt = std::getenv("FUGU_VMG_THETA")) c->vmg_theta = std::atof(vt);
+    if (const char* v2 = std::getenv("FUGU_VMG_THETA2")) c->vmg_theta2 = std::atof(v2);
     if (const char* vc = std::getenv("FUGU_VMG_COARSE_NU")) c->vmg_coarse_nu = std::atoi(vc);
     if (const char* pv = std::getenv("FUGU_PCD_VISC_COEF")) c->pcd_visc_coef = std::atof(pv);
     if (const char* vy = std::getenv("FUGU_VMG_CYCLES")) c->vmg_cycles = std::max(0, std::atoi(vy));

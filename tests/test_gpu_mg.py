@@ -95,6 +95,7 @@ def test_coarse_tail_bitwise_in_the_step_solve(monkeypatch, mixed, theta, coarse
     nx, ny = 96, 192
     outs = []
     monkeypatch.setenv("FUGU_VMG_THETA", theta)
+    monkeypatch.setenv("FUGU_VMG_THETA2", theta)
     monkeypatch.setenv("FUGU_VMG_COARSE_NU", coarse_nu)
     monkeypatch.setenv("FUGU_VMG_CYCLES", cycles)
     for cells, fused in (("0", "0"), ("-1", "1")):
