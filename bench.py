@@ -369,14 +369,14 @@ def main():
         m1["fp64"] = run_form(abi, base, **kw1)
         m1["mixed"] = run_form(abi, base | abi.SOLVE_MIXED, **kw1)
     # the 1024^2 horizon (16 states, < 1 GB) is held in HBM; its checkpointed run is timed beside it.
-    # Two pressure V-cycles per PCD: measured 3.7 s/grad against 4.4 s with one (FGMRES steps
-    # 1260 / 1648 forward / adjoint against 1624 / 2277); three gain 1 % more.
+    # Two pressure V-cycles per PCD: measured 3.2 s/grad against 3.7 s with one; four gain < 1 %.
     adj = run_adjoint(abi, also_resident=True, mg_cycles=2) if "adjoint" in sections else None
     adj2048 = None
     if "adjoint2048" in sections:
-        # two pressure V-cycles: measured 72 s/grad against 93 s with one
+        # pressure V-cycles per PCD: measured 61 s/grad with 2, 55 with 3, 51 with 4, 48.6 with 6, 48.5
+        # with 8 (the 1e-10 adjoint solves lean on the pressure block)
         adj2048 = run_adjoint(abi, n=2048, nsteps=32, count=256, seed=5, rmin=0.005, rmax=0.02, rho_c=1000.0,
-                              sigma=1.0, mixed=True, mg_cycles=2)
+                              sigma=1.0, mixed=True, mg_cycles=6)
 
     def summary(f):
         sts = f["stats"]
