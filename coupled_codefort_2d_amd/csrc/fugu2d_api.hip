@@ -482,6 +482,10 @@ void pcg(fugu_ctx* c, Work<T>& w, T* phi, const T* rhs, bool phi_zero, double to
 inline dim3 g2(int nx, int ny, int bx = 64, int by = 4) {
   return dim3((unsigned)((nx + bx - 1) / bx), (unsigned)((ny + by - 1) / by));
 }
+// grid of a fused level launch with tiles of 64 x th cells
+inline dim3 mgt_grid(const MgDims& d, int th) {
+  return dim3((unsigned)((d.nx + MGT_W - 1) / MGT_W), (unsigned)((d.ny + th - 1) / th));
+}
 
 // Levels of at most MG_TAIL_BLOCK_CELLS cells (the only ones whose tail can fit the LDS pool)
 // keep c_x, c_y (, m), D^-1, b and phi back to back in one allocation, level after level, in
@@ -711,18 +715,18 @@ T* mg_vcycle(fugu_ctx* c, std::vector<MgLevel<T>>& lv, size_t l, bool zero_guess
       // vmg_coarse_nu red-black sweeps (one fused launch)
       const int cnu = c->vmg_coarse_nu;
       if (c->mg_fused) {
-        const dim3 gt((unsigned)((L.d.nx + MGT_W - 1) / MGT_W), (unsigned)((L.d.ny + Mgt<T>::H - 1) / Mgt<T>::H));
+        const dim3 gt = mgt_grid(L.d, 32);
         const double ab = (g_in ? 6.0 : 5.0) * sizeof(T) * L.d.nx * L.d.ny;
         const char* nm = sizeof(T) == 8 ? "mg_cut_f64" : "mg_cut_f32";
         T* out = (g_in == L.phi) ? L.phi2 : L.phi;
         if (cnu == 3)
-          TLAUNCH(c, nm, ab, (k_mg_pre<T, 3>), gt, dim3(Mgt<T>::NT), c->s, L.d, (const T*)L.cx, (const T*)L.cy,
+          TLAUNCH(c, nm, ab, (k_mg_pre<T, 3, 32>), gt, dim3(512), c->s, L.d, (const T*)L.cx, (const T*)L.cy,
                   (const T*)L.m, (const T*)L.b, out, (T*)nullptr, (T*)nullptr, g_in);
         else if (cnu == 4)
-          TLAUNCH(c, nm, ab, (k_mg_pre<T, 4>), gt, dim3(Mgt<T>::NT), c->s, L.d, (const T*)L.cx, (const T*)L.cy,
+          TLAUNCH(c, nm, ab, (k_mg_pre<T, 4, 32>), gt, dim3(512), c->s, L.d, (const T*)L.cx, (const T*)L.cy,
                   (const T*)L.m, (const T*)L.b, out, (T*)nullptr, (T*)nullptr, g_in);
         else
-          TLAUNCH(c, nm, ab, (k_mg_pre<T, 2>), gt, dim3(Mgt<T>::NT), c->s, L.d, (const T*)L.cx, (const T*)L.cy,
+          TLAUNCH(c, nm, ab, (k_mg_pre<T, 2, 32>), gt, dim3(512), c->s, L.d, (const T*)L.cx, (const T*)L.cy,
                   (const T*)L.m, (const T*)L.b, out, (T*)nullptr, (T*)nullptr, g_in);
         return out;
       } else {
@@ -746,13 +750,13 @@ T* mg_vcycle(fugu_ctx* c, std::vector<MgLevel<T>>& lv, size_t l, bool zero_guess
   }
   MgLevel<T>& C = lv[l + 1];
   const double ncell = (double)L.d.nx * L.d.ny;
-  const dim3 gt((unsigned)((L.d.nx + MGT_W - 1) / MGT_W), (unsigned)((L.d.ny + Mgt<T>::H - 1) / Mgt<T>::H));
+  const dim3 gt = mgt_grid(L.d, 32);
   T* P = (g_in == L.phi) ? L.phi2 : L.phi;   // the fused pre-smoothing's output
   if (c->mg_fused) {
     // b, c_x, c_y (, m) (, the initial iterate) read; phi written; coarse b and phi written (a
     // quarter each)
     TLAUNCH(c, sizeof(T) == 8 ? "mg_pre_f64" : "mg_pre_f32", ((L.m ? 5.5 : 4.5) + (g_in ? 1.0 : 0.0)) * sizeof(T) * ncell,
-            (k_mg_pre<T, 2>), gt, dim3(Mgt<T>::NT), c->s, L.d, (const T*)L.cx, (const T*)L.cy, (const T*)L.m,
+            (k_mg_pre<T, 2, 32>), gt, dim3(512), c->s, L.d, (const T*)L.cx, (const T*)L.cy, (const T*)L.m,
             (const T*)L.b, P, C.b, C.phi, g_in);
   } else {
     guess_to_phi();
@@ -769,10 +773,17 @@ T* mg_vcycle(fugu_ctx* c, std::vector<MgLevel<T>>& lv, size_t l, bool zero_guess
   if (c->mg_fused) {
     // phi, coarse phi (a quarter), b, c_x, c_y (, m) read; the other buffer written
     T* Q = (P == L.phi) ? L.phi2 : L.phi;
-    TLAUNCH(c, sizeof(T) == 8 ? "mg_post_f64" : "mg_post_f32", (L.m ? 6.25 : 5.25) * sizeof(T) * ncell,
-            (k_mg_post<T, 2>), gt, dim3(Mgt<T>::NT), c->s, L.d, (const T*)L.cx, (const T*)L.cy, (const T*)L.m,
-            (const T*)L.b, (const T*)P, cres, Q, psum ? psum : (double*)nullptr);
-    if (psum && psum_nb) *psum_nb = (int)(gt.x * gt.y);
+    const int thp = mgt_tile_h<T>(L.d, true);
+    const dim3 gp = mgt_grid(L.d, thp);
+    const char* pname = sizeof(T) == 8 ? "mg_post_f64" : "mg_post_f32";
+    const double pbytes = (L.m ? 6.25 : 5.25) * sizeof(T) * ncell;
+    if (thp == 32)
+      TLAUNCH(c, pname, pbytes, (k_mg_post<T, 2, 32>), gp, dim3(512), c->s, L.d, (const T*)L.cx, (const T*)L.cy,
+              (const T*)L.m, (const T*)L.b, (const T*)P, cres, Q, psum ? psum : (double*)nullptr);
+    else
+      TLAUNCH(c, pname, pbytes, (k_mg_post<T, 2, 16>), gp, dim3(256), c->s, L.d, (const T*)L.cx, (const T*)L.cy,
+              (const T*)L.m, (const T*)L.b, (const T*)P, cres, Q, psum ? psum : (double*)nullptr);
+    if (psum && psum_nb) *psum_nb = (int)(gp.x * gp.y);
     return Q;
   }
   LAUNCH((k_mg_prolong<T>), g2(L.d.nx, L.d.ny), dim3(64, 4), c->s, L.d, cres, L.phi);
@@ -816,7 +827,8 @@ const T* mg_pressure_compact(fugu_ctx* c, Work<T>& w, const T* Zp) {
   MgLevel<T>& L0 = w.mg[0];
   LAUNCH((k_mg_load<T>), g2(g.nx, g.ny), dim3(64, 4), c->s, g, Zp, L0.b, centre, (const double*)c->sc,
          (int)S_P_MEAN, c->fl, (int)F_P_DONE);
-  const size_t tiles = (size_t)((L0.d.nx + MGT_W - 1) / MGT_W) * ((L0.d.ny + Mgt<T>::H - 1) / Mgt<T>::H);
+  const dim3 gp0 = mgt_grid(L0.d, mgt_tile_h<T>(L0.d, true));   // the level-0 post-smoothing's tiles
+  const size_t tiles = (size_t)gp0.x * gp0.y;
   double* psum = (walls_both && 2 * tiles <= c->partials_n) ? c->partials : nullptr;
   const T* res = nullptr;
   int nb = 0;

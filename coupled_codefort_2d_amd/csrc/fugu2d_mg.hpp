@@ -312,17 +312,16 @@ __global__ void __launch_bounds__(1024) k_mg_coarse_inv(MgDims d, const T* __res
 // post-smoothing writes the other buffer.  A cell's colour is that of its wrapped global index and
 // each update performs the one-launch-per-half-sweep smoother's arithmetic with the same
 // operands, so the result is bit-identical to it.
-// Tile shape per precision: 64 x 32 cells and 512 threads in FP32, 64 x 16 and 256 threads in
-// FP64 (measured: the taller tile cuts the halo overhead from 1.9x to 1.5x and is faster at every
-// FP32 level -- 7 instead of 8.5 us on small levels, 31 instead of 37 us at 2048^2 -- while in FP64
-// its register footprint slows the 2048^2 post-smoothing).  One thread per coarse cell of the
-// restriction: 16 * height threads.
+// Tile: 64 x TH cells, 16 TH threads (one per coarse cell of the restriction).  TH = 32 (512
+// threads) cuts the halo overhead from 1.9x to 1.5x against TH = 16 and is faster at every level
+// in FP32 (7 instead of 8.5 us on small levels, 31 instead of 37 us at 2048^2) and at every FP64
+// level up to 1024^2; the FP64 post-smoothing above 2^20 cells keeps TH = 16 (its register
+// footprint slows the taller tile there: 70 against 57 us at 2048^2).  mgt_tile_h picks it.
 constexpr int MGT_W = 64;
 template <typename T>
-struct Mgt {
-  static constexpr int H = sizeof(T) == 4 ? 32 : 16;
-  static constexpr int NT = 16 * H;
-};
+inline int mgt_tile_h(const MgDims& d, bool post) {
+  return (sizeof(T) == 8 && post && (size_t)d.nx * d.ny > ((size_t)1 << 20)) ? 16 : 32;
+}
 
 __device__ inline int mgt_wrap(int g, int n, int per) {
   if (g >= 0 && g < n) return g;
@@ -342,9 +341,9 @@ __device__ inline void mgt_tile(int& i0, int& j0) {
   j0 = (t / gridDim.x) * TH;
 }
 
-template <typename T, int H>
+template <typename T, int H, int TH>
 struct MgtCells {
-  static constexpr int TH = Mgt<T>::H, NT = Mgt<T>::NT;
+  static constexpr int TILE_H = TH, NT = 16 * TH;
   static constexpr int EW = MGT_W + 2 * H, EH = TH + 2 * H, NS = (EW * EH + NT - 1) / NT;
   T b[NS], dinv[NS], cW[NS], cE[NS], cS[NS], cN[NS];
   T dg[NS];                       // unmasked diagonal (+ m): the residual of the restriction
@@ -356,11 +355,11 @@ struct MgtCells {
 // Cells up to ring rc get b and their four conductances (and m), from which D^-1 and the
 // diagonal are formed exactly as k_mg_dinv forms them (no D^-1 array is read); cells beyond
 // ring rc are only ever read as neighbours (their phi) or not at all, and load nothing else.
-template <typename T, int H>
+template <typename T, int H, int TH>
 __device__ inline void mgt_init(const MgDims& f, int i0, int j0, const T* cx, const T* cy, const T* mass,
-                                const T* b, const T* phi, const T* phic, int rc, MgtCells<T, H>& m,
-                                T (*s)[MgtCells<T, H>::EW]) {
-  using M = MgtCells<T, H>;
+                                const T* b, const T* phi, const T* phic, int rc, MgtCells<T, H, TH>& m,
+                                T (*s)[MgtCells<T, H, TH>::EW]) {
+  using M = MgtCells<T, H, TH>;
   const int tid = threadIdx.x;
   const int nxc = (f.nx + 1) / 2;
   const size_t rx = (size_t)(f.nx + 1);
@@ -375,7 +374,7 @@ __device__ inline void mgt_init(const MgDims& f, int i0, int j0, const T* cx, co
     dist = max(dist, H - li);
     dist = max(dist, li - (H + MGT_W - 1));
     dist = max(dist, H - lj);
-    dist = max(dist, lj - (H + M::TH - 1));
+    dist = max(dist, lj - (H + M::TILE_H - 1));
     const bool cell = idx < M::EW * M::EH && i >= 0 && j >= 0;
     m.ring[k] = (cell && dist <= rc) ? dist : -1;
     m.col[k] = (i + j) & 1;
@@ -416,9 +415,10 @@ __device__ inline void mgt_init(const MgDims& f, int i0, int j0, const T* cx, co
 }
 
 // One half-sweep of colour `col` over the tile grown by r rings.
-template <typename T, int H>
-__device__ inline void mgt_half(int r, int col, int zero_other, const MgtCells<T, H>& m, T (*s)[MgtCells<T, H>::EW]) {
-  using M = MgtCells<T, H>;
+template <typename T, int H, int TH>
+__device__ inline void mgt_half(int r, int col, int zero_other, const MgtCells<T, H, TH>& m,
+                                T (*s)[MgtCells<T, H, TH>::EW]) {
+  using M = MgtCells<T, H, TH>;
 #pragma unroll
   for (int k = 0; k < M::NS; ++k) {
     if (m.ring[k] < 0 || m.ring[k] > r || m.col[k] != col) continue;
@@ -430,13 +430,13 @@ __device__ inline void mgt_half(int r, int col, int zero_other, const MgtCells<T
   __syncthreads();
 }
 
-template <typename T, int H>
-__device__ inline void mgt_store(const MgDims& f, int i0, int j0, T* out, const T (*s)[MgtCells<T, H>::EW]) {
+template <typename T, int H, int TH>
+__device__ inline void mgt_store(const MgDims& f, int i0, int j0, T* out, const T (*s)[MgtCells<T, H, TH>::EW]) {
   const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
   const int gi = i0 + tx;
   if (gi >= f.nx) return;
 #pragma unroll
-  for (int qj = ty; qj < Mgt<T>::H; qj += Mgt<T>::NT / 64) {
+  for (int qj = ty; qj < TH; qj += TH / 4) {
     const int gj = j0 + qj;
     if (gj < f.ny) out[gi + (size_t)gj * f.nx] = s[H + qj][H + tx];
   }
@@ -447,24 +447,24 @@ __device__ inline void mgt_store(const MgDims& f, int i0, int j0, T* out, const 
 // holding its coefficients and summed from LDS (children in the restriction's order).
 // phi_in (optional): a non-zero initial iterate (another array than phi: tiles read their halos
 // from it while others write their own cells of phi); nullptr: from zero.
-template <typename T, int NU>
-__global__ void __launch_bounds__(Mgt<T>::NT) k_mg_pre(MgDims f, const T* __restrict__ cx, const T* __restrict__ cy,
+template <typename T, int NU, int TH>
+__global__ void __launch_bounds__(16 * TH) k_mg_pre(MgDims f, const T* __restrict__ cx, const T* __restrict__ cy,
                                                 const T* __restrict__ m, const T* __restrict__ b,
                                                 T* __restrict__ phi, T* __restrict__ bc, T* __restrict__ phic,
                                                 const T* __restrict__ phi_in) {
   constexpr int H = 2 * NU + 1;
-  using M = MgtCells<T, H>;
+  using M = MgtCells<T, H, TH>;
   __shared__ T s[M::EH][M::EW];
-  __shared__ T sres[Mgt<T>::H][MGT_W];
+  __shared__ T sres[TH][MGT_W];
   int i0, j0;
-  mgt_tile<Mgt<T>::H>(i0, j0);
+  mgt_tile<TH>(i0, j0);
   M cells;
   // rings 0..H-1 are updated; from zero, ring H-1 only by the first, neighbour-free half-sweep and
   // ring H is never read; from phi_in, ring H is read (its phi) by the first half-sweep
-  mgt_init<T, H>(f, i0, j0, cx, cy, m, b, phi_in, (const T*)nullptr, H - 1, cells, s);
+  mgt_init<T, H, TH>(f, i0, j0, cx, cy, m, b, phi_in, (const T*)nullptr, H - 1, cells, s);
 #pragma unroll
-  for (int h = 0; h < 2 * NU; ++h) mgt_half<T, H>(H - 1 - h, h & 1, (h == 0 && !phi_in) ? 1 : 0, cells, s);
-  mgt_store<T, H>(f, i0, j0, phi, s);
+  for (int h = 0; h < 2 * NU; ++h) mgt_half<T, H, TH>(H - 1 - h, h & 1, (h == 0 && !phi_in) ? 1 : 0, cells, s);
+  mgt_store<T, H, TH>(f, i0, j0, phi, s);
   if (!bc) return;   // smoothing only (a truncated hierarchy's coarsest level)
 #pragma unroll
   for (int k = 0; k < M::NS; ++k) {
@@ -493,27 +493,27 @@ __global__ void __launch_bounds__(Mgt<T>::NT) k_mg_pre(MgDims f, const T* __rest
 // phi_out = phi + phi_c(parent), then NU post-smoothing sweeps (black first).
 // psum (optional): the tile's sum of its outputs as a double-double partial, one per workgroup
 // (the mean of the level-0 result without another pass over it).
-template <typename T, int NU>
-__global__ void __launch_bounds__(Mgt<T>::NT) k_mg_post(MgDims f, const T* __restrict__ cx, const T* __restrict__ cy,
+template <typename T, int NU, int TH>
+__global__ void __launch_bounds__(16 * TH) k_mg_post(MgDims f, const T* __restrict__ cx, const T* __restrict__ cy,
                                                  const T* __restrict__ m, const T* __restrict__ b,
                                                  const T* __restrict__ phi, const T* __restrict__ phic,
                                                  T* __restrict__ phi_out, double* __restrict__ psum) {
   constexpr int H = 2 * NU;
-  using M = MgtCells<T, H>;
+  using M = MgtCells<T, H, TH>;
   __shared__ T s[M::EH][M::EW];
   int i0, j0;
-  mgt_tile<Mgt<T>::H>(i0, j0);
+  mgt_tile<TH>(i0, j0);
   M cells;
-  mgt_init<T, H>(f, i0, j0, cx, cy, m, b, phi, phic, H - 1, cells, s);
+  mgt_init<T, H, TH>(f, i0, j0, cx, cy, m, b, phi, phic, H - 1, cells, s);
 #pragma unroll
-  for (int h = 0; h < 2 * NU; ++h) mgt_half<T, H>(H - 1 - h, (h & 1) ? 0 : 1, 0, cells, s);
-  mgt_store<T, H>(f, i0, j0, phi_out, s);
+  for (int h = 0; h < 2 * NU; ++h) mgt_half<T, H, TH>(H - 1 - h, (h & 1) ? 0 : 1, 0, cells, s);
+  mgt_store<T, H, TH>(f, i0, j0, phi_out, s);
   if (psum) {
     DD v[1] = {dd_from(0.0)};
     const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
     if (i0 + tx < f.nx) {
 #pragma unroll
-      for (int qj = ty; qj < Mgt<T>::H; qj += Mgt<T>::NT / 64)
+      for (int qj = ty; qj < TH; qj += TH / 4)
         if (j0 + qj < f.ny) v[0] = dd_add(v[0], dd_from((double)s[H + qj][H + tx]));
     }
     publish_dd<1>(v, psum);
