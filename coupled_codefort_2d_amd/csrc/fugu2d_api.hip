@@ -168,6 +168,7 @@ struct fugu_ctx {
   // the same step counts on both workloads, 1-2 % less time.  Inner Krylov vectors are centred.
   bool zp_centre = false;
   bool fgmres_pcd = false;
+  bool pcd_fused_glue = true;   // the v store, its halo and the cross term in one launch (FUGU_PCD_FUSED_GLUE=0: three)
   // V-cycles per velocity block solve: vmg_cycles > 0 fixes it; 0 (default) takes 2 where the
   // block's finest diffusion-to-mass ratio exceeds vmg_cycle_ratio, else 1.  Measured: on the
   // 2048^2 inclusions (ratio ~1e4, viscous-dominated inclusions) a second cycle cuts the FGMRES
@@ -850,8 +851,9 @@ const T* mg_pressure_compact(fugu_ctx* c, Work<T>& w, const T* Zp) {
 // out (padded u or v array) ~= A_comp^-1 rhs by one V-cycle from zero on the velocity block
 // (rows of the block; wall faces copy rhs, ghosts zero, as the Jacobi blocks leave them).
 // loaded: level 0's b already holds the rhs rows (written by the kernel that formed them).
+// out == nullptr: no store (the caller fuses it); returns the compact result either way.
 template <typename T>
-void vmg_solve(fugu_ctx* c, Work<T>& w, int comp, const T* rhs, T* out, bool loaded = false) {
+const T* vmg_solve(fugu_ctx* c, Work<T>& w, int comp, const T* rhs, T* out, bool loaded = false) {
   const Grid& g = c->g;
   auto& lv = w.vmg[comp];
   MgLevel<T>& L = lv[0];
@@ -859,7 +861,8 @@ void vmg_solve(fugu_ctx* c, Work<T>& w, int comp, const T* rhs, T* out, bool loa
   const T* res = nullptr;
   for (int k = 0; k < w.vmg_cyc[comp]; ++k)   // later cycles smooth from the last result
     res = mg_vcycle(c, lv, 0, k == 0, 0, w.vmg_nl[comp], nullptr, nullptr, res);
-  LAUNCH((k_vmg_store<T>), grid_union(g), dim3(TX, TY), c->s, g, comp, res, rhs, out);
+  if (out) LAUNCH((k_vmg_store<T>), grid_union(g), dim3(TX, TY), c->s, g, comp, res, rhs, out);
+  return res;
 }
 
 // block PCD preconditioner R = P^-1 Z (triplets).
@@ -883,16 +886,24 @@ void pcd(fugu_ctx* c, Work<T>& w, const T* Z, T* R, T dt) {
   halo(c, Rp, 2);
   LAUNCH((k_pcd_zwork<T>), gu, bl, c->s, g, Zu, Zv, (const T*)Rp, w.zuw, w.zvw,
          c->use_vmg ? w.vmg[1][0].b : (T*)nullptr);
+  // v block, then the u block's rhs with the cross term: store, halo and cross fused in one
+  // launch when the velocity blocks are multigrid solves (pcd_fused_glue)
+  const bool fuse_v = c->use_vmg && c->pcd_cross && c->pcd_fused_glue;
   if (c->use_vmg) {
-    vmg_solve<T>(c, w, 1, w.zvw, Rv, true);
+    const T* resv = vmg_solve<T>(c, w, 1, w.zvw, fuse_v ? (T*)nullptr : Rv, true);
+    if (fuse_v)
+      LAUNCH((k_vmg_store_cross<T>), gu, bl, c->s, g, resv, (const T*)w.zvw, Rv, (const T*)w.eta, (const T*)w.zuw,
+             w.vmg[0][0].b);
   } else {
     LAUNCH((k_jacobi_v<T>), gu, bl, c->s, g, (const T*)nullptr, w.jv0, w.zvw, w.rho_v, w.kap, w.zeta, w.eta, dt, w.penv);
     LAUNCH((k_jacobi_v<T>), gu, bl, c->s, g, (const T*)w.jv0, w.jv1, w.zvw, w.rho_v, w.kap, w.zeta, w.eta, dt, w.penv);
     LAUNCH((k_jacobi_v<T>), gu, bl, c->s, g, (const T*)w.jv1, Rv, w.zvw, w.rho_v, w.kap, w.zeta, w.eta, dt, w.penv);
   }
-  halo(c, Rv, 1);
-  if (c->pcd_cross || !c->use_vmg)
-    LAUNCH((k_pcd_cross<T>), gc, bl, c->s, g, (const T*)Rv, w.eta, w.zuw, c->use_vmg ? w.vmg[0][0].b : (T*)nullptr);
+  if (!fuse_v) {
+    halo(c, Rv, 1);
+    if (c->pcd_cross || !c->use_vmg)
+      LAUNCH((k_pcd_cross<T>), gc, bl, c->s, g, (const T*)Rv, w.eta, w.zuw, c->use_vmg ? w.vmg[0][0].b : (T*)nullptr);
+  }
   if (c->use_vmg) {
     vmg_solve<T>(c, w, 0, w.zuw, Ru, c->pcd_cross);
   } else {
@@ -2002,6 +2013,7 @@ int fugu_ctx_create(const fugu_grid* grid, const fugu_params* prm, fugu_ctx** ou
     if (const char* vc = std::getenv("FUGU_VMG_COARSE_NU")) c->vmg_coarse_nu = std::atoi(vc);
     if (const char* pv = std::getenv("FUGU_PCD_VISC_COEF")) c->pcd_visc_coef = std::atof(pv);
     if (const char* zc = std::getenv("FUGU_ZP_CENTRE")) c->zp_centre = std::atoi(zc) != 0;
+    if (const char* fg = std::getenv("FUGU_PCD_FUSED_GLUE")) c->pcd_fused_glue = std::atoi(fg) != 0;
     if (const char* vy = std::getenv("FUGU_VMG_CYCLES")) c->vmg_cycles = std::max(0, std::atoi(vy));
     if (const char* vr = std::getenv("FUGU_VMG_CYCLE_RATIO")) c->vmg_cycle_ratio = std::atof(vr);
     if (const char* rf = std::getenv("FUGU_REORTH_FRAC")) c->reorth_frac = std::atof(rf);

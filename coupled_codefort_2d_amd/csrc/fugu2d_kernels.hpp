@@ -1458,7 +1458,8 @@ __global__ void __launch_bounds__(SVOF_NT) k_svof_block(Grid g, const double* __
     cnt = fl[F_S_RCNT];
   }
   if (cnt <= 0) return;
-  __shared__ double L[SVOF_EW * SVOF_EH];
+  // two iterates in LDS (read one, write the other): one barrier per iteration
+  __shared__ double LB[2][SVOF_EW * SVOF_EH];
   const int nx = g.nx, ny = g.ny;
   const int i0 = (int)blockIdx.x * SVOF_TW, j0 = (int)blockIdx.y * SVOF_TH;   // 0-based owned corner
   double cr[SVOF_CPT], nw[SVOF_CPT];
@@ -1491,10 +1492,12 @@ __global__ void __launch_bounds__(SVOF_NT) k_svof_block(Grid g, const double* __
   }
 #pragma unroll
   for (int r = 0; r < SVOF_CPT; ++r)
-    if (present & (1u << r)) L[threadIdx.x + SVOF_NT * r] = nw[r];
+    if (present & (1u << r)) LB[0][threadIdx.x + SVOF_NT * r] = nw[r];
   __syncthreads();
   __shared__ double wmax[SVOF_S][SVOF_NT / 64];   // per iteration and wavefront: max |new - old| over owned cells
   for (int t = 0; t < cnt; ++t) {
+    const double* L = LB[t & 1];
+    double* Lo = LB[(t + 1) & 1];
     double mt = 0.0;
 #pragma unroll
     for (int r = 0; r < SVOF_CPT; ++r) {
@@ -1504,15 +1507,12 @@ __global__ void __launch_bounds__(SVOF_NT) k_svof_block(Grid g, const double* __
       const double e = L[(clamp_e & bit) ? q : q + 1], w = L[(clamp_w & bit) ? q : q - 1];
       const double n = L[(clamp_n & bit) ? q : q + SVOF_EW], s = L[(clamp_s & bit) ? q : q - SVOF_EW];
       const double o = (cr[r] + cx * (e + w) + cy * (n + s)) / diag;
-      if (owned & bit) mt = fmax(mt, fabs(o - L[q]));
+      if (owned & bit) mt = fmax(mt, fabs(o - nw[r]));   // nw[r]: this cell's previous iterate
       nw[r] = o;
+      Lo[q] = o;
     }
     mt = wave_max(mt);
     if ((threadIdx.x & 63) == 0) wmax[t][threadIdx.x >> 6] = mt;
-    __syncthreads();
-#pragma unroll
-    for (int r = 0; r < SVOF_CPT; ++r)
-      if (present & (1u << r)) L[threadIdx.x + SVOF_NT * r] = nw[r];
     __syncthreads();
   }
 #pragma unroll

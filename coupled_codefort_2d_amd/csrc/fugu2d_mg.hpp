@@ -839,4 +839,43 @@ __global__ void __launch_bounds__(NT) k_vmg_store(Grid g, int comp, const T* __r
   }
 }
 
+// The PCD's v-block store, its periodic halo and the cross term of the u block in one launch:
+// rv (padded v array) = what k_vmg_store (comp 1) followed by the v halo leaves; bu (compact u
+// rows) = zuw - cross(eta, rv), each thread reading the stored v values it needs straight from
+// the compact result (same values, same operations and order as k_pcd_cross).
+template <typename T>
+__global__ void __launch_bounds__(NT) k_vmg_store_cross(Grid g, const T* __restrict__ phi, const T* __restrict__ rhs,
+                                                        T* __restrict__ rv, const T* __restrict__ eta,
+                                                        const T* __restrict__ zuw, T* __restrict__ bu) {
+  int i, j;
+  union_ij(g, i, j);
+  const int nx = g.nx, ny = g.ny, nh = g.nh;
+  const int j_v_max = g.per_y ? ny : ny - 1;
+  auto vs = [&](int ii, int jj) -> T {   // the stored v at (ii, jj), periodic ghosts from their images
+    if (g.per_x) {
+      if (ii >= 1 - nh && ii <= 0) ii += nx;
+      else if (ii >= nx + 1 && ii <= nx + nh) ii -= nx;
+    }
+    if (g.per_y) {
+      if (jj >= 1 - nh && jj <= 0) jj += ny;
+      else if (jj >= ny + 1 && jj <= ny + nh) jj -= ny;
+    }
+    if (ii >= 1 && ii <= nx && jj >= 1 && jj <= j_v_max) return phi[(ii - 1) + (size_t)(jj - 1) * nx];
+    if (!g.per_y && ii >= 1 && ii <= nx && (jj == 0 || jj == ny)) return rhs[IV(g, ii, jj)];
+    return T(0);
+  };
+  if (i >= 1 - nh && i <= nx + nh && j >= -nh && j <= ny + nh) rv[IV(g, i, j)] = vs(i, j);
+  const int i_u_max = g.per_x ? nx : nx - 1;
+  if (i >= 1 && i <= i_u_max && j >= 1 && j <= ny) {
+    T val = zuw[IU(g, i, j)];
+    if (i <= nx - 1) {
+      const T inv_dx = (T)g.inv_dx, inv_dy = (T)g.inv_dy;
+      const T cross = -inv_dy * (eta[IW(g, i, j)] * (vs(i + 1, j) - vs(i, j)) * inv_dx -
+                                 eta[IW(g, i, j - 1)] * (vs(i + 1, j - 1) - vs(i, j - 1)) * inv_dx);
+      val = val - cross;
+    }
+    bu[(i - 1) + (size_t)(j - 1) * i_u_max] = val;
+  }
+}
+
 }  // namespace fugu

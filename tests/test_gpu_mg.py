@@ -133,3 +133,35 @@ def test_odd_periodic_extent_is_refused():
     with pytest.raises(Exception, match="even"):
         ctx.pressure_mg(g.zeros("p"), rand(g, "p", rng), pr["rho_u"], pr["rho_v"], n_cycles=1)
     ctx.close()
+
+
+@pytest.mark.parametrize("per_x", [False, True])
+@pytest.mark.parametrize("mixed", [False, True])
+def test_fused_pcd_glue_is_bitwise(monkeypatch, mixed, per_x):
+    """The PCD's v-block store, periodic halo and cross term fused into one launch give the fields
+    and step counts of the three separate launches, bit for bit (walls and periodic x)."""
+    import bench
+    nx, ny = 96, 192
+    outs = []
+    for fused in ("0", "1"):
+        monkeypatch.setenv("FUGU_PCD_FUSED_GLUE", fused)
+        p = abi.FuguParams(1000.0, 1.0, 10.0, 0.1, 2.0, 0.0, -1.0, 2, 1e12, 0.5)
+        ctx = abi.Context(nx, ny, 1.0, 2.0, per_x, False, params=p)
+        nh = 2
+        u = np.zeros((ny + 2 * nh, nx + 1 + 2 * nh)); v = np.zeros((ny + 1 + 2 * nh, nx + 2 * nh))
+        pp = np.zeros((ny + 2 * nh, nx + 2 * nh))
+        ctx.upload(u, v, u.copy(), v.copy(), pp, bench.bubble_c(nx, ny, 1.0, 2.0))
+        ctx.init_properties()
+        flags = abi.SOLVE_REFINED | abi.PRECOND_MG | abi.PRECOND_VISC | abi.PRECOND_VMG | (abi.SOLVE_MIXED if mixed else 0)
+        opts = abi.step_opts(tol=1e-8, max_iter=200, solver_flags=flags, inner_max_iter=-1, restart=30)
+        its = []
+        for s in (1, 2, 3):
+            st, rc = ctx.step(2e-4, s, opts)
+            assert rc == 0 and st.res <= 1e-8
+            its.append(st.outer)
+        f = [ctx.field(k, np.zeros(shape)) for k, shape in (("u", u.shape), ("v", v.shape), ("p", pp.shape))]
+        ctx.close()
+        outs.append((its, f))
+    assert outs[0][0] == outs[1][0]
+    for a, b in zip(outs[0][1], outs[1][1]):
+        assert np.array_equal(a, b)
