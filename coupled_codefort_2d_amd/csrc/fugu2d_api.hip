@@ -865,9 +865,11 @@ const T* vmg_solve(fugu_ctx* c, Work<T>& w, int comp, const T* rhs, T* out, bool
   return res;
 }
 
-// block PCD preconditioner R = P^-1 Z (triplets).
+// block PCD preconditioner R = P^-1 Z (triplets).  bcs_after: the caller applies the homogeneous
+// velocity BCs to R next; then (multigrid velocity blocks, fused glue) the u store applies them
+// itself and the return value is true (the caller skips its own).
 template <typename T>
-void pcd(fugu_ctx* c, Work<T>& w, const T* Z, T* R, T dt) {
+bool pcd(fugu_ctx* c, Work<T>& w, const T* Z, T* R, T dt, bool bcs_after = false) {
   const Grid& g = c->g;
   const int walls_both = (!g.per_x && !g.per_y) ? 1 : 0;
   const T *Zu = Z, *Zv = Z + g.nu, *Zp = Z + g.nu + g.nv;
@@ -904,14 +906,17 @@ void pcd(fugu_ctx* c, Work<T>& w, const T* Z, T* R, T dt) {
     if (c->pcd_cross || !c->use_vmg)
       LAUNCH((k_pcd_cross<T>), gc, bl, c->s, g, (const T*)Rv, w.eta, w.zuw, c->use_vmg ? w.vmg[0][0].b : (T*)nullptr);
   }
+  const bool fuse_u = c->use_vmg && c->pcd_fused_glue && bcs_after;
   if (c->use_vmg) {
-    vmg_solve<T>(c, w, 0, w.zuw, Ru, c->pcd_cross);
+    const T* resu = vmg_solve<T>(c, w, 0, w.zuw, fuse_u ? (T*)nullptr : Ru, c->pcd_cross);
+    if (fuse_u) LAUNCH((k_vmg_store_bcs<T>), gu, bl, c->s, g, resu, (const T*)w.zuw, Ru, Rv);
   } else {
     LAUNCH((k_jacobi_u<T>), gu, bl, c->s, g, (const T*)nullptr, w.ju0, w.zuw, w.rho_u, w.kap, w.zeta, w.eta, dt, w.penu);
     LAUNCH((k_jacobi_u<T>), gu, bl, c->s, g, (const T*)w.ju0, w.ju1, w.zuw, w.rho_u, w.kap, w.zeta, w.eta, dt, w.penu);
     LAUNCH((k_jacobi_u<T>), gu, bl, c->s, g, (const T*)w.ju1, Ru, w.zuw, w.rho_u, w.kap, w.zeta, w.eta, dt, w.penu);
   }
-  halo(c, Ru, 0);
+  if (!fuse_u) halo(c, Ru, 0);
+  return fuse_u;
 }
 
 // ---------------------------------------------------------------------------------
@@ -1398,8 +1403,7 @@ void solve_fgmres(fugu_ctx* c, Work<T>& w, double* X, double dt, double tol, dou
       auto step_k = [&]() {
       if (ir32) {
         float* z = c->kz32[k];
-        pcd<float>(c, c->wf, (const float*)c->kv32[k], z, (float)dt);
-        bcs_trip(c, z);
+        if (!pcd<float>(c, c->wf, (const float*)c->kv32[k], z, (float)dt, true)) bcs_trip(c, z);
         // W = A z_k (FP32 operands; the matvec's halo exchange only rewrites z_k's periodic
         // ghosts with the images the PCD already put there), then Gram-Schmidt on FP32 storage
         matvec(c, c->wf, z, c->wf.z, dt, mvflags, nullptr);
@@ -1409,6 +1413,7 @@ void solve_fgmres(fugu_ctx* c, Work<T>& w, double* X, double dt, double tol, dou
         maxpy(W, W, c->kv32.data(), k + 1, -1.0, 64, 2, k);   // the step k runs in one of the two folds
         lin<float>(c, L_SCALE, c->kv32[k + 1], W, nullptr, nullptr, nullptr, S_INVH, -1, -1, 1.0, false);
       } else {
+        bool bcs_done = false;
         if (max_inner > 0) {
           LAUNCH((k_convert<T, double>), dim3(nblk(n3)), dim3(256), c->s, w.B, (const double*)c->kv[k], n3, c->sc,
                  -1, 1.0, 0);
@@ -1421,9 +1426,9 @@ void solve_fgmres(fugu_ctx* c, Work<T>& w, double* X, double dt, double tol, dou
         } else if (c->adjoint) {
           pcd_T(c, (const double*)c->kv[k], c->kz[k], dt, -1.0, c->mg_cycles, c->pcd_visc);
         } else if (sizeof(T) == sizeof(double)) {
-          pcd(c, w, (const T*)c->kv[k], (T*)c->kz[k], (T)dt);
+          bcs_done = pcd(c, w, (const T*)c->kv[k], (T*)c->kz[k], (T)dt, true);
         }
-        bcs_trip(c, c->kz[k]);
+        if (!bcs_done) bcs_trip(c, c->kz[k]);
         // W = A z_k, then classical Gram-Schmidt twice, all coefficients device-resident
         if (c->adjoint) {
           HIPCHK(hipMemcpyAsync(wd.X, c->kz[k], n3 * sizeof(double), hipMemcpyDeviceToDevice, c->s));

@@ -878,4 +878,60 @@ __global__ void __launch_bounds__(NT) k_vmg_store_cross(Grid g, const T* __restr
   }
 }
 
+// The PCD's u-block store, its periodic halo and the homogeneous velocity BCs of the whole
+// output (k_velocity_bcs with a zero lid on ru and rv) in one launch.  ru's final values are
+// formed from the compact result as store + halo + BCs leave them (the BCs read only positions
+// they do not write, so each is a function of the stored values); rv, stored by the previous
+// launch, gets its BC positions exactly as k_velocity_bcs sets them.
+template <typename T>
+__global__ void __launch_bounds__(NT) k_vmg_store_bcs(Grid g, const T* __restrict__ phi, const T* __restrict__ rhs,
+                                                      T* __restrict__ ru, T* __restrict__ rv) {
+  int i, j;
+  union_ij(g, i, j);
+  const int nx = g.nx, ny = g.ny, nh = g.nh;
+  const int i_u_max = g.per_x ? nx : nx - 1;
+  auto us = [&](int ii, int jj) -> T {   // the stored u at (ii, jj), periodic ghosts from their images
+    if (g.per_x) {
+      if (ii >= 1 - nh && ii <= 0) ii += nx;
+      else if (ii >= nx + 1 && ii <= nx + nh) ii -= nx;
+    }
+    if (g.per_y) {
+      if (jj >= 1 - nh && jj <= 0) jj += ny;
+      else if (jj >= ny + 1 && jj <= ny + nh) jj -= ny;
+    }
+    if (jj >= 1 && jj <= ny && ii >= 1 && ii <= i_u_max) return phi[(ii - 1) + (size_t)(jj - 1) * i_u_max];
+    if (!g.per_x && jj >= 1 && jj <= ny && (ii == 0 || ii == nx)) return rhs[IU(g, ii, jj)];
+    return T(0);
+  };
+  auto uax = [&](int ii, int jj) -> T {   // u_after_x on the stored values
+    if (g.per_x) return us(ii, jj);
+    if (ii == 0 || ii == nx) return T(0);
+    if (ii < 0 && ii >= -nh) return -us(-ii, jj);
+    if (ii > nx && ii <= nx + nh) return -us(2 * nx - ii, jj);
+    return us(ii, jj);
+  };
+  if (i >= -nh && i <= nx + nh && j >= 1 - nh && j <= ny + nh) {
+    const bool xt = !g.per_x && (i <= 0 || i >= nx);
+    const bool yt = !g.per_y && (j <= 0 || j >= ny + 1);
+    T val;
+    if (yt) val = (j <= 0) ? -uax(i, 1 - j) : (T)(2.0 * 0.0) - uax(i, 2 * ny + 1 - j);
+    else if (xt) val = uax(i, j);
+    else val = us(i, j);
+    ru[IU(g, i, j)] = val;
+  }
+  if (i >= 1 - nh && i <= nx + nh && j >= -nh && j <= ny + nh) {
+    const bool xt = !g.per_x && (i <= 0 || i >= nx + 1);
+    const bool yt = !g.per_y && (j <= 0 || j >= ny);
+    if (yt) {
+      T val;
+      if (j == 0 || j == ny) val = T(0);
+      else if (j < 0) val = -v_after_x(g, (const T*)rv, i, -j);
+      else val = -v_after_x(g, (const T*)rv, i, 2 * ny - j);
+      rv[IV(g, i, j)] = val;
+    } else if (xt) {
+      rv[IV(g, i, j)] = v_after_x(g, (const T*)rv, i, j);
+    }
+  }
+}
+
 }  // namespace fugu

@@ -135,18 +135,19 @@ def test_odd_periodic_extent_is_refused():
     ctx.close()
 
 
-@pytest.mark.parametrize("per_x", [False, True])
+@pytest.mark.parametrize("per_x,per_y", [(False, False), (True, False), (False, True)])
 @pytest.mark.parametrize("mixed", [False, True])
-def test_fused_pcd_glue_is_bitwise(monkeypatch, mixed, per_x):
-    """The PCD's v-block store, periodic halo and cross term fused into one launch give the fields
-    and step counts of the three separate launches, bit for bit (walls and periodic x)."""
+def test_fused_pcd_glue_is_bitwise(monkeypatch, mixed, per_x, per_y):
+    """The PCD's fused glue -- the v-block store, periodic halo and cross term in one launch, and
+    the u-block store, periodic halo and the velocity BCs FGMRES applies to z_k in another -- gives
+    the fields and step counts of the separate launches, bit for bit (walls, periodic x, periodic y)."""
     import bench
     nx, ny = 96, 192
     outs = []
     for fused in ("0", "1"):
         monkeypatch.setenv("FUGU_PCD_FUSED_GLUE", fused)
         p = abi.FuguParams(1000.0, 1.0, 10.0, 0.1, 2.0, 0.0, -1.0, 2, 1e12, 0.5)
-        ctx = abi.Context(nx, ny, 1.0, 2.0, per_x, False, params=p)
+        ctx = abi.Context(nx, ny, 1.0, 2.0, per_x, per_y, params=p)
         nh = 2
         u = np.zeros((ny + 2 * nh, nx + 1 + 2 * nh)); v = np.zeros((ny + 1 + 2 * nh, nx + 2 * nh))
         pp = np.zeros((ny + 2 * nh, nx + 2 * nh))
@@ -157,11 +158,11 @@ def test_fused_pcd_glue_is_bitwise(monkeypatch, mixed, per_x):
         its = []
         for s in (1, 2, 3):
             st, rc = ctx.step(2e-4, s, opts)
-            assert rc == 0 and st.res <= 1e-8
-            its.append(st.outer)
+            its.append((rc, st.outer, st.res))
         f = [ctx.field(k, np.zeros(shape)) for k, shape in (("u", u.shape), ("v", v.shape), ("p", pp.shape))]
         ctx.close()
         outs.append((its, f))
+    assert all(rc == 0 and res <= 1e-8 for rc, _, res in outs[0][0]), outs[0][0]
     assert outs[0][0] == outs[1][0]
-    for a, b in zip(outs[0][1], outs[1][1]):
-        assert np.array_equal(a, b)
+    for a, b, name in zip(outs[0][1], outs[1][1], "uvp"):
+        assert np.array_equal(a, b), (name, np.abs(a - b).max())
