@@ -125,11 +125,14 @@ def run_form(abi, flags, steps, warmup, max_outer, inner_tol, mg_cycles, inner_m
 KERNELS = ("pcg_iter1_f64", "pcg_iter2_f64", "pcg_iter1_f32", "pcg_iter2_f32", "matvec_f64", "matvec_f32",
            "mg_smooth_f64", "mg_smooth_f32", "mg_resid_f64", "mg_resid_f32", "mdot_f64", "mdot_f32", "maxpy_f64",
            "maxpy_f32", "maxpy_update", "maxpy_dot_f64", "maxpy_dot_f32", "mg_pre_f64", "mg_pre_f32", "mg_post_f64",
-           "mg_post_f32", "mg_cut_f64", "mg_cut_f32")
+           "mg_post_f32", "mg_pre3_f64", "mg_pre3_f32", "mg_post3_f64", "mg_post3_f32", "mg_cut_f64", "mg_cut_f32")
 DEVICE_NAMES = {"maxpy_f64": "k_maxpy<double, double>", "maxpy_f32": "k_maxpy<float, float>",
                 "maxpy_update": "k_maxpy<double, float>",
+                # fused multigrid levels: two red-black sweeps per half (pressure), three (velocity)
                 "mg_pre_f64": "k_mg_pre<double, 2>", "mg_pre_f32": "k_mg_pre<float, 2>",
                 "mg_post_f64": "k_mg_post<double, 2>", "mg_post_f32": "k_mg_post<float, 2>",
+                "mg_pre3_f64": "k_mg_pre<double, 3>", "mg_pre3_f32": "k_mg_pre<float, 3>",
+                "mg_post3_f64": "k_mg_post<double, 3>", "mg_post3_f32": "k_mg_post<float, 3>",
                 # the cut velocity hierarchy's coarsest level: k_mg_pre with NU sweeps, no restriction
                 "mg_cut_f64": "k_mg_pre<double, NU> (cut coarsest)", "mg_cut_f32": "k_mg_pre<float, NU> (cut coarsest)"}
 
@@ -160,16 +163,16 @@ def roofline(ktimes, tag="rising_bubble_512x1024"):
     # which the profiler's begin/end timestamps leave out
     trace = os.path.join(ROOT, "profiles", f"kernel_trace_{tag}.md")
     if os.path.exists(trace):
+        # the kernel itself, or its instantiations with further template arguments (tile shapes)
         key = device_name(name)
-        exact = key if "," in key else None
-        prefix = key[:-1] + "," if "," not in key else None
+        prefix = key[:-1] + ","
         tot_ms, calls = 0.0, 0
         for ln in open(trace):
             f = [x.strip() for x in ln.split("|")]
             if len(f) < 6 or not f[1].startswith("`"):
                 continue
             kname = f[1].strip("`")
-            if (exact and kname == exact) or (prefix and kname.startswith(prefix)):
+            if kname == key or kname.startswith(prefix):
                 tot_ms += float(f[3]); calls += int(f[2])
         if calls:
             pavg = tot_ms * 1e3 / calls

@@ -559,18 +559,29 @@ struct MgTailArgs {
   int nlev, nu, coarse_sweeps, project, zero_guess;
 };
 
-// One colour of a red-black sweep in the tail (NTH threads as 32 x NTH/32).  phic (the first
-// post-smoothing half-sweep): the neighbours' values are phi + phi_c(parent), the prolongation
-// done on the fly -- the only reader of the prolonged values of this colour's neighbours, whose
-// own cells are overwritten before anything else reads them.
-template <typename T, int NTH>
+// log2 of the lanes a tail phase puts along a row of w cells: the smallest power of two >= w, at
+// most 64 (the rest of the threads take further rows), so that a level with no more cells per
+// phase than threads gives every thread at most one.
+__device__ inline int tail_lx_log2(int w) {
+  int s = 0;
+  while ((1 << s) < w && s < 6) ++s;
+  return s;
+}
+
+// One colour of a red-black sweep in the tail (NT threads).  The cells of one colour depend only
+// on the other colour, so the result does not depend on how they are spread over the threads.
+// phic (the first post-smoothing half-sweep): the neighbours' values are phi + phi_c(parent), the
+// prolongation done on the fly -- the only reader of the prolonged values of this colour's
+// neighbours, whose own cells are overwritten before anything else reads them.
+template <typename T, int NT>
 __device__ inline void tail_smooth(const MgDims d, const T* cx, const T* cy, const T* dinv, const T* b, T* phi,
                                    int colour, int zero_other, const T* phic = nullptr) {
   const int half = (d.nx + 1) / 2;
-  const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
+  const int sx = tail_lx_log2(half);
+  const int tx = threadIdx.x & ((1 << sx) - 1), ty = threadIdx.x >> sx;
   const int nxc = (d.nx + 1) / 2;
-  for (int j = ty; j < d.ny; j += NTH / 32) {
-    for (int t = tx; t < half; t += 32) {
+  for (int j = ty; j < d.ny; j += NT >> sx) {
+    for (int t = tx; t < half; t += 1 << sx) {
       const int i = 2 * t + ((j + colour) & 1);
       if (i >= d.nx) continue;
       const int o = i + j * d.nx;
@@ -593,6 +604,33 @@ __device__ inline void tail_smooth(const MgDims d, const T* cx, const T* cy, con
         }
       }
       phi[o] = (b[o] + s) * dinv[o];
+    }
+  }
+  __syncthreads();
+}
+
+// Residual restriction of a tail level: b_c = sum of the children's residuals, phi_c = 0.
+template <typename T, int NT>
+__device__ inline void tail_restrict(const MgDims f, const T* cx, const T* cy, const T* m, const T* b, const T* phi,
+                                     T* bc, T* phic) {
+  const int nxc = (f.nx + 1) / 2, nyc = (f.ny + 1) / 2;
+  const int sx = tail_lx_log2(nxc);
+  const int tx = threadIdx.x & ((1 << sx) - 1), ty = threadIdx.x >> sx;
+  for (int J = ty; J < nyc; J += NT >> sx) {
+    for (int I = tx; I < nxc; I += 1 << sx) {
+      T acc = T(0);
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        const int i = 2 * I + (k & 1), j = 2 * J + (k >> 1);
+        if (i >= f.nx || j >= f.ny) continue;
+        const size_t o = i + (size_t)j * f.nx;
+        const size_t rx = (size_t)(f.nx + 1);
+        T diag = cx[i + j * rx] + cx[i + 1 + j * rx] + cy[o] + cy[o + f.nx];
+        if (m) diag += m[o];
+        acc += b[o] - (diag * phi[o] - mg_nbr_sum(f, cx, cy, phi, i, j));
+      }
+      bc[I + (size_t)J * nxc] = acc;
+      phic[I + (size_t)J * nxc] = T(0);
     }
   }
   __syncthreads();
@@ -654,27 +692,7 @@ __global__ void __launch_bounds__(NTH) k_mg_tail(MgTailArgs<T> a) {
       tail_smooth<T, NTH>(f, cx, cy, dinv, b, phi, 0, (s == 0 && zg) ? 1 : 0);
       tail_smooth<T, NTH>(f, cx, cy, dinv, b, phi, 1, 0);
     }
-    const int nxc = (f.nx + 1) / 2, nyc = (f.ny + 1) / 2;
-    T* bc = BB(l + 1);
-    T* phic = PH(l + 1);
-    for (int J = (int)threadIdx.x >> 5; J < nyc; J += NTH / 32) {
-      for (int I = (int)threadIdx.x & 31; I < nxc; I += 32) {
-        T acc = T(0);
-#pragma unroll
-        for (int k = 0; k < 4; ++k) {
-          const int i = 2 * I + (k & 1), j = 2 * J + (k >> 1);
-          if (i >= f.nx || j >= f.ny) continue;
-          const size_t o = i + (size_t)j * f.nx;
-          const size_t rx = (size_t)(f.nx + 1);
-          T diag = cx[i + j * rx] + cx[i + 1 + j * rx] + cy[o] + cy[o + f.nx];
-          if (m) diag += m[o];
-          acc += b[o] - (diag * phi[o] - mg_nbr_sum(f, cx, cy, (const T*)phi, i, j));
-        }
-        bc[I + (size_t)J * nxc] = acc;
-        phic[I + (size_t)J * nxc] = T(0);
-      }
-    }
-    __syncthreads();
+    tail_restrict<T, NTH>(f, cx, cy, m, b, (const T*)phi, BB(l + 1), PH(l + 1));
   }
   // coarsest level
   {

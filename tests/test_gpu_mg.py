@@ -123,6 +123,43 @@ def test_coarse_tail_bitwise_in_the_step_solve(monkeypatch, mixed, theta, coarse
         assert np.array_equal(a, b)
 
 
+@pytest.mark.parametrize("nu_p,nu_v", [("1", "2"), ("3", "4"), ("2", "3"), ("4", "1")])
+def test_sweep_counts_fused_is_bitwise_the_multilaunch_cycle(monkeypatch, nu_p, nu_v):
+    """Other red-black sweep counts per V-cycle half (pressure and velocity hierarchies set apart):
+    the fused level kernels and the coarse tail still perform the multi-launch cycle's arithmetic,
+    bit for bit, in a refined step solve."""
+    import bench
+    nx, ny = 96, 192
+    monkeypatch.setenv("FUGU_MG_NU_P", nu_p)
+    monkeypatch.setenv("FUGU_MG_NU_V", nu_v)
+    monkeypatch.setenv("FUGU_VMG_THETA", "0")     # full-depth velocity hierarchies: their tail runs too
+    monkeypatch.setenv("FUGU_VMG_THETA2", "0")
+    outs = []
+    for cells, fused in (("0", "0"), ("-1", "1")):
+        monkeypatch.setenv("FUGU_MG_TAIL_CELLS", cells)
+        monkeypatch.setenv("FUGU_MG_FUSED", fused)
+        p = abi.FuguParams(1000.0, 1.0, 10.0, 0.1, 2.0, 0.0, -1.0, 2, 1e12, 0.5)
+        ctx = abi.Context(nx, ny, 1.0, 2.0, False, False, params=p)
+        nh = 2
+        u = np.zeros((ny + 2 * nh, nx + 1 + 2 * nh)); v = np.zeros((ny + 1 + 2 * nh, nx + 2 * nh))
+        pp = np.zeros((ny + 2 * nh, nx + 2 * nh))
+        ctx.upload(u, v, u.copy(), v.copy(), pp, bench.bubble_c(nx, ny, 1.0, 2.0))
+        ctx.init_properties()
+        flags = abi.SOLVE_REFINED | abi.PRECOND_MG | abi.PRECOND_VISC | abi.PRECOND_VMG
+        opts = abi.step_opts(tol=1e-8, max_iter=300, solver_flags=flags, inner_max_iter=-1, restart=30)
+        its = []
+        for s in (1, 2):
+            st, rc = ctx.step(2e-4, s, opts)
+            its.append((rc, st.outer, st.res))
+        f = [ctx.field(k, np.zeros(shape)) for k, shape in (("u", u.shape), ("v", v.shape), ("p", pp.shape))]
+        ctx.close()
+        outs.append((its, f))
+    assert all(rc == 0 and res <= 1e-8 for rc, _, res in outs[0][0]), outs[0][0]
+    assert outs[0][0] == outs[1][0]
+    for a, b, name in zip(outs[0][1], outs[1][1], "uvp"):
+        assert np.array_equal(a, b), (name, np.abs(a - b).max())
+
+
 def test_odd_periodic_extent_is_refused():
     """Red-black smoothing needs even periodic extents: the multigrid block says so instead of
     running a colouring whose seam neighbours share a colour."""
