@@ -341,14 +341,20 @@ __device__ inline void mgt_tile(int& i0, int& j0) {
   j0 = (t / gridDim.x) * TH;
 }
 
+// A thread's cells of the tile grown by H rings, by colour: slots 0..NSC-1 hold colour-0 cells,
+// slots NSC..NS-1 colour-1 cells, so that a half-sweep visits only its own colour's slots with
+// every lane busy.  The grown tile is EW (even) wide, so each of its rows holds EW/2 cells of
+// either colour, and the colour of a cell, (i + j) & 1 of its wrapped global index, is (li + lj)
+// & 1: i0, j0 and 2H are even, and a periodic wrap moves by an even extent.
 template <typename T, int H, int TH>
 struct MgtCells {
   static constexpr int TILE_H = TH, NT = 16 * TH;
-  static constexpr int EW = MGT_W + 2 * H, EH = TH + 2 * H, NS = (EW * EH + NT - 1) / NT;
+  static constexpr int EW = MGT_W + 2 * H, EH = TH + 2 * H;
+  static constexpr int NSC = ((EW / 2) * EH + NT - 1) / NT, NS = 2 * NSC;
+  static_assert(EW % 2 == 0, "the grown tile must be even wide");
   T b[NS], dinv[NS], cW[NS], cE[NS], cS[NS], cN[NS];
   T dg[NS];                       // unmasked diagonal (+ m): the residual of the restriction
   int li[NS], lj[NS], ring[NS];   // ring: distance outside the tile (0 inside); -1: no cell
-  int col[NS];                    // colour of the wrapped global index
 };
 
 // Load the thread's cells (registers) and phi (+ the parent's coarse correction) into LDS.
@@ -366,8 +372,9 @@ __device__ inline void mgt_init(const MgDims& f, int i0, int j0, const T* cx, co
   T pv[M::NS];
 #pragma unroll
   for (int k = 0; k < M::NS; ++k) {
-    const int idx = tid + M::NT * k;
-    const int lj = idx / M::EW, li = idx - lj * M::EW;
+    const int colour = k / M::NSC;
+    const int q = tid + M::NT * (k - colour * M::NSC);   // the colour's q-th cell of the grown tile
+    const int lj = q / (M::EW / 2), li = 2 * (q - lj * (M::EW / 2)) + ((lj + colour) & 1);
     m.li[k] = li; m.lj[k] = lj;
     const int i = mgt_wrap(i0 - H + li, f.nx, f.per_x), j = mgt_wrap(j0 - H + lj, f.ny, f.per_y);
     int dist = 0;
@@ -375,9 +382,9 @@ __device__ inline void mgt_init(const MgDims& f, int i0, int j0, const T* cx, co
     dist = max(dist, li - (H + MGT_W - 1));
     dist = max(dist, H - lj);
     dist = max(dist, lj - (H + M::TILE_H - 1));
-    const bool cell = idx < M::EW * M::EH && i >= 0 && j >= 0;
+    const bool slot = lj < M::EH;
+    const bool cell = slot && i >= 0 && j >= 0;
     m.ring[k] = (cell && dist <= rc) ? dist : -1;
-    m.col[k] = (i + j) & 1;
     // raw loads only here (all slots' loads in flight together); D^-1 and masks below
     T bv = T(0), rw = T(0), re = T(0), rs = T(0), rn = T(0), mm = T(0), p = T(0);
     if (cell) {
@@ -410,18 +417,17 @@ __device__ inline void mgt_init(const MgDims& f, int i0, int j0, const T* cx, co
   }
 #pragma unroll
   for (int k = 0; k < M::NS; ++k)
-    if (tid + M::NT * k < M::EW * M::EH) s[m.lj[k]][m.li[k]] = pv[k];
+    if (m.lj[k] < M::EH) s[m.lj[k]][m.li[k]] = pv[k];
   __syncthreads();
 }
 
-// One half-sweep of colour `col` over the tile grown by r rings.
-template <typename T, int H, int TH>
-__device__ inline void mgt_half(int r, int col, int zero_other, const MgtCells<T, H, TH>& m,
-                                T (*s)[MgtCells<T, H, TH>::EW]) {
+// One half-sweep of colour COL over the tile grown by r rings (the colour's slots only).
+template <typename T, int H, int TH, int COL>
+__device__ inline void mgt_half(int r, int zero_other, const MgtCells<T, H, TH>& m, T (*s)[MgtCells<T, H, TH>::EW]) {
   using M = MgtCells<T, H, TH>;
 #pragma unroll
-  for (int k = 0; k < M::NS; ++k) {
-    if (m.ring[k] < 0 || m.ring[k] > r || m.col[k] != col) continue;
+  for (int k = COL * M::NSC; k < (COL + 1) * M::NSC; ++k) {
+    if (m.ring[k] < 0 || m.ring[k] > r) continue;
     const int li = m.li[k], lj = m.lj[k];
     T sum = T(0);
     if (!zero_other) sum = m.cW[k] * s[lj][li - 1] + m.cE[k] * s[lj][li + 1] + m.cS[k] * s[lj - 1][li] + m.cN[k] * s[lj + 1][li];
@@ -463,7 +469,10 @@ __global__ void __launch_bounds__(16 * TH) k_mg_pre(MgDims f, const T* __restric
   // ring H is never read; from phi_in, ring H is read (its phi) by the first half-sweep
   mgt_init<T, H, TH>(f, i0, j0, cx, cy, m, b, phi_in, (const T*)nullptr, H - 1, cells, s);
 #pragma unroll
-  for (int h = 0; h < 2 * NU; ++h) mgt_half<T, H, TH>(H - 1 - h, h & 1, (h == 0 && !phi_in) ? 1 : 0, cells, s);
+  for (int h = 0; h < 2 * NU; ++h) {   // red (colour 0) first
+    if (h & 1) mgt_half<T, H, TH, 1>(H - 1 - h, 0, cells, s);
+    else mgt_half<T, H, TH, 0>(H - 1 - h, (h == 0 && !phi_in) ? 1 : 0, cells, s);
+  }
   mgt_store<T, H, TH>(f, i0, j0, phi, s);
   if (!bc) return;   // smoothing only (a truncated hierarchy's coarsest level)
 #pragma unroll
@@ -506,7 +515,10 @@ __global__ void __launch_bounds__(16 * TH) k_mg_post(MgDims f, const T* __restri
   M cells;
   mgt_init<T, H, TH>(f, i0, j0, cx, cy, m, b, phi, phic, H - 1, cells, s);
 #pragma unroll
-  for (int h = 0; h < 2 * NU; ++h) mgt_half<T, H, TH>(H - 1 - h, (h & 1) ? 0 : 1, 0, cells, s);
+  for (int h = 0; h < 2 * NU; ++h) {   // black (colour 1) first
+    if (h & 1) mgt_half<T, H, TH, 0>(H - 1 - h, 0, cells, s);
+    else mgt_half<T, H, TH, 1>(H - 1 - h, 0, cells, s);
+  }
   mgt_store<T, H, TH>(f, i0, j0, phi_out, s);
   if (psum) {
     DD v[1] = {dd_from(0.0)};
