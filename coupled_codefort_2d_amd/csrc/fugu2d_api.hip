@@ -729,19 +729,17 @@ T* mg_vcycle(fugu_ctx* c, std::vector<MgLevel<T>>& lv, size_t l, bool zero_guess
       // vmg_coarse_nu red-black sweeps (one fused launch)
       const int cnu = c->vmg_coarse_nu;
       if (c->mg_fused) {
-        const dim3 gt = mgt_grid(L.d, 32);
+        const dim3 gt = mgt_grid(L.d, MGT_TH);
         const double ab = (g_in ? 6.0 : 5.0) * sizeof(T) * L.d.nx * L.d.ny;
         const char* nm = sizeof(T) == 8 ? "mg_cut_f64" : "mg_cut_f32";
         T* out = (g_in == L.phi) ? L.phi2 : L.phi;
-        if (cnu == 3)
-          TLAUNCH(c, nm, ab, (k_mg_pre<T, 3, 32>), gt, dim3(512), c->s, L.d, (const T*)L.cx, (const T*)L.cy,
-                  (const T*)L.m, (const T*)L.b, out, (T*)nullptr, (T*)nullptr, g_in);
-        else if (cnu == 4)
-          TLAUNCH(c, nm, ab, (k_mg_pre<T, 4, 32>), gt, dim3(512), c->s, L.d, (const T*)L.cx, (const T*)L.cy,
-                  (const T*)L.m, (const T*)L.b, out, (T*)nullptr, (T*)nullptr, g_in);
-        else
-          TLAUNCH(c, nm, ab, (k_mg_pre<T, 2, 32>), gt, dim3(512), c->s, L.d, (const T*)L.cx, (const T*)L.cy,
-                  (const T*)L.m, (const T*)L.b, out, (T*)nullptr, (T*)nullptr, g_in);
+#define FUGU_MG_CUT(NUV)                                                                                     \
+  TLAUNCH(c, nm, ab, (k_mg_pre<T, NUV, MGT_TH>), gt, dim3(16 * MGT_TH), c->s, L.d, (const T*)L.cx,             \
+          (const T*)L.cy, (const T*)L.m, (const T*)L.b, out, (T*)nullptr, (T*)nullptr, g_in)
+        if (cnu == 3) FUGU_MG_CUT(3);
+        else if (cnu == 4) FUGU_MG_CUT(4);
+        else FUGU_MG_CUT(2);
+#undef FUGU_MG_CUT
         return out;
       } else {
         guess_to_phi();
@@ -764,16 +762,16 @@ T* mg_vcycle(fugu_ctx* c, std::vector<MgLevel<T>>& lv, size_t l, bool zero_guess
   }
   MgLevel<T>& C = lv[l + 1];
   const double ncell = (double)L.d.nx * L.d.ny;
-  const dim3 gt = mgt_grid(L.d, 32);
+  const dim3 gt = mgt_grid(L.d, MGT_TH);
   T* P = (g_in == L.phi) ? L.phi2 : L.phi;   // the fused pre-smoothing's output
   if (c->mg_fused) {
     // b, c_x, c_y (, m) (, the initial iterate) read; phi written; coarse b and phi written (a
     // quarter each)
     const char* nm = mg_kname(false, nu, sizeof(T) == 8);
     const double ab = ((L.m ? 5.5 : 4.5) + (g_in ? 1.0 : 0.0)) * sizeof(T) * ncell;
-#define FUGU_MG_PRE(NUV)                                                                                           \
-  TLAUNCH(c, nm, ab, (k_mg_pre<T, NUV, 32>), gt, dim3(512), c->s, L.d, (const T*)L.cx, (const T*)L.cy, (const T*)L.m, \
-          (const T*)L.b, P, C.b, C.phi, g_in)
+#define FUGU_MG_PRE(NUV)                                                                                     \
+  TLAUNCH(c, nm, ab, (k_mg_pre<T, NUV, MGT_TH>), gt, dim3(16 * MGT_TH), c->s, L.d, (const T*)L.cx,             \
+          (const T*)L.cy, (const T*)L.m, (const T*)L.b, P, C.b, C.phi, g_in)
     if (nu == 1) FUGU_MG_PRE(1);
     else if (nu == 3) FUGU_MG_PRE(3);
     else if (nu == 4) FUGU_MG_PRE(4);
@@ -794,24 +792,16 @@ T* mg_vcycle(fugu_ctx* c, std::vector<MgLevel<T>>& lv, size_t l, bool zero_guess
   if (c->mg_fused) {
     // phi, coarse phi (a quarter), b, c_x, c_y (, m) read; the other buffer written
     T* Q = (P == L.phi) ? L.phi2 : L.phi;
-    const int thp = mgt_tile_h<T>(L.d, true);
-    const dim3 gp = mgt_grid(L.d, thp);
+    const dim3 gp = mgt_grid(L.d, MGT_TH);
     const char* pname = mg_kname(true, nu, sizeof(T) == 8);
     const double pbytes = (L.m ? 6.25 : 5.25) * sizeof(T) * ncell;
 #define FUGU_MG_POST(NUV, THV)                                                                                      \
   TLAUNCH(c, pname, pbytes, (k_mg_post<T, NUV, THV>), gp, dim3(16 * THV), c->s, L.d, (const T*)L.cx, (const T*)L.cy, \
           (const T*)L.m, (const T*)L.b, (const T*)P, cres, Q, psum ? psum : (double*)nullptr)
-    if (thp == 32) {
-      if (nu == 1) FUGU_MG_POST(1, 32);
-      else if (nu == 3) FUGU_MG_POST(3, 32);
-      else if (nu == 4) FUGU_MG_POST(4, 32);
-      else FUGU_MG_POST(2, 32);
-    } else {
-      if (nu == 1) FUGU_MG_POST(1, 16);
-      else if (nu == 3) FUGU_MG_POST(3, 16);
-      else if (nu == 4) FUGU_MG_POST(4, 16);
-      else FUGU_MG_POST(2, 16);
-    }
+    if (nu == 1) FUGU_MG_POST(1, MGT_TH);
+    else if (nu == 3) FUGU_MG_POST(3, MGT_TH);
+    else if (nu == 4) FUGU_MG_POST(4, MGT_TH);
+    else FUGU_MG_POST(2, MGT_TH);
 #undef FUGU_MG_POST
     if (psum && psum_nb) *psum_nb = (int)(gp.x * gp.y);
     return Q;
@@ -857,7 +847,7 @@ const T* mg_pressure_compact(fugu_ctx* c, Work<T>& w, const T* Zp) {
   MgLevel<T>& L0 = w.mg[0];
   LAUNCH((k_mg_load<T>), g2(g.nx, g.ny), dim3(64, 4), c->s, g, Zp, L0.b, centre, (const double*)c->sc,
          (int)S_P_MEAN, c->fl, (int)F_P_DONE);
-  const dim3 gp0 = mgt_grid(L0.d, mgt_tile_h<T>(L0.d, true));   // the level-0 post-smoothing's tiles
+  const dim3 gp0 = mgt_grid(L0.d, MGT_TH);   // the level-0 post-smoothing's tiles
   const size_t tiles = (size_t)gp0.x * gp0.y;
   double* psum = (walls_both && 2 * tiles <= c->partials_n) ? c->partials : nullptr;
   const T* res = nullptr;

@@ -314,14 +314,11 @@ __global__ void __launch_bounds__(1024) k_mg_coarse_inv(MgDims d, const T* __res
 // operands, so the result is bit-identical to it.
 // Tile: 64 x TH cells, 16 TH threads (one per coarse cell of the restriction).  TH = 32 (512
 // threads) cuts the halo overhead from 1.9x to 1.5x against TH = 16 and is faster at every level
-// in FP32 (7 instead of 8.5 us on small levels, 31 instead of 37 us at 2048^2) and at every FP64
-// level up to 1024^2; the FP64 post-smoothing above 2^20 cells keeps TH = 16 (its register
-// footprint slows the taller tile there: 70 against 57 us at 2048^2).  mgt_tile_h picks it.
+// in both precisions (FP32: 7 instead of 8.5 us on small levels, 31 instead of 37 us at 2048^2;
+// with the cell slots grouped by colour also the FP64 post-smoothing at 2048^2, -3 % on the
+// solve); a 64 x 64 tile of 1024 threads was slower in FP32 at 2048^2 (one workgroup per CU).
 constexpr int MGT_W = 64;
-template <typename T>
-inline int mgt_tile_h(const MgDims& d, bool post) {
-  return (sizeof(T) == 8 && post && (size_t)d.nx * d.ny > ((size_t)1 << 20)) ? 16 : 32;
-}
+constexpr int MGT_TH = 32;
 
 __device__ inline int mgt_wrap(int g, int n, int per) {
   if (g >= 0 && g < n) return g;
