@@ -343,8 +343,8 @@ def main():
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint seconds-per-gradient section")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py measures one GPU: --gpus {args.gpus} is not supported by this build "
-              "(single-device domain; see DESIGN.md for how the path would shard)", file=sys.stderr)
+        print(f"bench.py measures one GPU: --gpus {args.gpus} is not supported "
+              "(the solver runs one device-resident domain per process)", file=sys.stderr)
         sys.exit(2)
 
     if args.copy_probe_mib:
