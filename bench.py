@@ -21,7 +21,7 @@ sums and Hessenberg.
 dt 5e-5) with respect to per-cell mu, per-cell rho and sigma on 1024 x 1024 with 64 seeded
 inclusions (carrier rho 100 / mu 1e-2, inclusions rho 1 / mu 1e-2 * 10^U[0,3], sigma 0.5).
 
-Prints one JSON line.  --gpus N > 1 exits: this build measures one GPU.
+Prints one JSON line.  --gpus N > 1 exits: the bench measures one GPU.
 """
 from __future__ import annotations
 
