@@ -146,6 +146,7 @@ struct fugu_ctx {
   // red-black sweeps before and after each coarse correction: pressure and velocity hierarchies
   // (three on the velocity levels: fewer FGMRES steps, -5 % solve time on viscous-dominated blocks)
   int mg_nu_p = 2, mg_nu_v = 3;
+  bool gs_vec = true;  // Gram-Schmidt multi-dot with 16-byte loads
   bool pcd_cross = true;     // the PCD's eta cross term from the solved v block into the u block
   int vmg_levels = 0;        // velocity hierarchies: at most this many levels (0: coarsen fully)
   // velocity hierarchies cut where diffusion / mass < vmg_theta (0: never), the cut coarsest level
@@ -1387,13 +1388,17 @@ void solve_fgmres(fugu_ctx* c, Work<T>& w, double* X, double dt, double tol, dou
     const unsigned nch = (unsigned)((k + 1 + MD_CH - 1) / MD_CH);
     const int flag = second ? (int)F_REORTH : -1;
     const double ab = (k + 2.0) * sizeof(V) * n3;
+    int vec = c->gs_vec ? 1 : 0;   // 16-byte loads when every array allows them
+    if (((uintptr_t)W & 15) != 0) vec = 0;
+    for (int q = 0; q <= k; ++q)
+      if (((uintptr_t)basis[q] & 15) != 0) vec = 0;
     if (second && c->launch_log)
       std::fprintf(c->launch_log, "%s %.0f 0\n", sizeof(V) == 4 ? "mdot_f32" : "mdot_f64", ab);
     if (second) {   // predicated: not timed
-      LAUNCH((k_mdot<V>), dim3(MD_BX, nch), dim3(256), c->s, md, c->partials, (const int*)c->fl, flag);
+      LAUNCH((k_mdot<V>), dim3(MD_BX, nch), dim3(256), c->s, md, c->partials, (const int*)c->fl, flag, vec);
     } else {
       TLAUNCH(c, sizeof(V) == 4 ? "mdot_f32" : "mdot_f64", ab, (k_mdot<V>), dim3(MD_BX, nch), dim3(256), c->s,
-              md, c->partials, (const int*)c->fl, flag);
+              md, c->partials, (const int*)c->fl, flag, vec);
     }
     LAUNCH(k_mdot_fold, dim3(1), dim3(1024), c->s, k + 1, off, c->g.vol, (const double*)c->partials, c->sc,
            (const int*)c->fl, flag);
@@ -2033,6 +2038,7 @@ int fugu_ctx_create(const fugu_grid* grid, const fugu_params* prm, fugu_ctx** ou
     if (const char* fu = std::getenv("FUGU_MG_FUSED")) c->mg_fused = std::atoi(fu) != 0;
     if (const char* ts = std::getenv("FUGU_MG_TAIL_STAGE")) c->mg_tail_stage = std::atoi(ts) != 0;
     if (const char* s = std::getenv("FUGU_MG_NU_P")) c->mg_nu_p = std::min(4, std::max(1, std::atoi(s)));
+    if (const char* s = std::getenv("FUGU_GS_VEC")) c->gs_vec = std::atoi(s) != 0;
     if (const char* s = std::getenv("FUGU_MG_NU_V")) c->mg_nu_v = std::min(4, std::max(1, std::atoi(s)));
     if (const char* fe = std::getenv("FUGU_FGMRES_EST")) c->fg_est = std::atof(fe);
     if (const char* pc = std::getenv("FUGU_PCD_CROSS")) c->pcd_cross = std::atoi(pc) != 0;

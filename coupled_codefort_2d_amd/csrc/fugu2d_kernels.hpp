@@ -718,9 +718,29 @@ struct MdotArgs {
   size_t n;
 };
 // Chunk 0 also sums <w, w> (partial slot MD_CH).  flag >= 0: run only when fl[flag] is set.
+// 16-byte vectors of V (two doubles or four floats) for the streaming Gram-Schmidt kernels.
+template <typename V>
+struct Vec16;
+template <>
+struct Vec16<double> {
+  using T = double2;
+  static constexpr int W = 2;
+};
+template <>
+struct Vec16<float> {
+  using T = float4;
+  static constexpr int W = 4;
+};
+template <typename V>
+__device__ inline V vcomp(const typename Vec16<V>::T& v, int c) {
+  return reinterpret_cast<const V*>(&v)[c];
+}
+
+// vec: every array is 16-byte aligned; the flat range is then read as 16-byte vectors (a lane
+// sums its W elements in order) and the last n mod W elements one by one.
 template <typename V>
 __global__ void __launch_bounds__(256) k_mdot(MdotArgs<V> a, double* __restrict__ partials, const int* __restrict__ fl,
-                                              int flag) {
+                                              int flag, int vec) {
   if (flag >= 0 && !fl[flag]) return;
   const int c0 = (int)blockIdx.y * MD_CH;
   const V* vp[MD_CH];
@@ -730,7 +750,28 @@ __global__ void __launch_bounds__(256) k_mdot(MdotArgs<V> a, double* __restrict_
 #pragma unroll
   for (int q = 0; q <= MD_CH; ++q) acc[q] = 0.0;
   const size_t stride = (size_t)MD_BX * blockDim.x;
-  for (size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x; k < a.n; k += stride) {
+  const size_t gid = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  size_t k0 = gid;
+  if (vec) {
+    using VT = typename Vec16<V>::T;
+    constexpr int W = Vec16<V>::W;
+    const size_t nw = a.n / W;
+    for (size_t k = gid; k < nw; k += stride) {
+      const VT wv = reinterpret_cast<const VT*>(a.w)[k];
+      VT xv[MD_CH];
+#pragma unroll
+      for (int q = 0; q < MD_CH; ++q) xv[q] = reinterpret_cast<const VT*>(vp[q])[k];
+#pragma unroll
+      for (int e = 0; e < W; ++e) {
+        const double wk = (double)vcomp<V>(wv, e);
+#pragma unroll
+        for (int q = 0; q < MD_CH; ++q) acc[q] += wk * (double)vcomp<V>(xv[q], e);
+        acc[MD_CH] += wk * wk;
+      }
+    }
+    k0 = nw * W + gid;
+  }
+  for (size_t k = k0; k < a.n; k += stride) {
     const double wk = (double)a.w[k];
     double x[MD_CH];
 #pragma unroll
