@@ -146,6 +146,9 @@ struct fugu_ctx {
   // red-black sweeps before and after each coarse correction: pressure and velocity hierarchies
   // (three on the velocity levels: fewer FGMRES steps, -5 % solve time on viscous-dominated blocks)
   int mg_nu_p = 2, mg_nu_v = 3;
+  // matvec from LDS tiles of this many rows (4 or 8; 0: straight from the arrays); 8 rows:
+  // -3.4 % FP64 and -1 % mixed solve time at 2048^2, no change at 512x1024
+  int mv_tile = 8;
   bool gs_vec = true;  // Gram-Schmidt multi-dot with 16-byte loads
   bool pcd_cross = true;     // the PCD's eta cross term from the solved v block into the u block
   int vmg_levels = 0;        // velocity hierarchies: at most this many levels (0: coarsen fully)
@@ -334,8 +337,16 @@ void matvec(fugu_ctx* c, Work<T>& w, T* X, T* Y, double dt, int flags, const int
   // 13 fields per cell (X_u, X_v, X_p, rho_u, rho_v, rho_c, kappa, vertex mu, u_adv, v_adv -> Y_u, Y_v,
   // Y_p); AMP swaps rho_c, u_adv, v_adv for mdot_u, mdot_v (-1); the penalty adds pen_u, pen_v
   const double mv_fields = 13.0 - ((flags & MV_CONS) ? 1.0 : 0.0) + ((flags & MV_PEN) ? 2.0 : 0.0);
-  TLAUNCH(c, sizeof(T) == 8 ? "matvec_f64" : "matvec_f32", mv_fields * sizeof(T) * g.nx * g.ny, (k_matvec<T>), grid_union(g), dim3(TX, TY), c->s,
-          g, a, skip);
+  const char* nm = sizeof(T) == 8 ? "matvec_f64" : "matvec_f32";
+  const double ab = mv_fields * sizeof(T) * g.nx * g.ny;
+  if (c->mv_tile == 8)
+    TLAUNCH(c, nm, ab, (k_matvec_tile<T, 8>), dim3((unsigned)((g.ldw + TX - 1) / TX), (unsigned)((g.rw + 7) / 8)),
+            dim3(TX, 8), c->s, g, a, skip);
+  else if (c->mv_tile == 4)
+    TLAUNCH(c, nm, ab, (k_matvec_tile<T, 4>), dim3((unsigned)((g.ldw + TX - 1) / TX), (unsigned)((g.rw + 3) / 4)),
+            dim3(TX, 4), c->s, g, a, skip);
+  else
+    TLAUNCH(c, nm, ab, (k_matvec<T>), grid_union(g), dim3(TX, TY), c->s, g, a, skip);
 }
 
 template <typename T>
@@ -2038,6 +2049,7 @@ int fugu_ctx_create(const fugu_grid* grid, const fugu_params* prm, fugu_ctx** ou
     if (const char* fu = std::getenv("FUGU_MG_FUSED")) c->mg_fused = std::atoi(fu) != 0;
     if (const char* ts = std::getenv("FUGU_MG_TAIL_STAGE")) c->mg_tail_stage = std::atoi(ts) != 0;
     if (const char* s = std::getenv("FUGU_MG_NU_P")) c->mg_nu_p = std::min(4, std::max(1, std::atoi(s)));
+    if (const char* s = std::getenv("FUGU_MV_TILE")) c->mv_tile = std::atoi(s);
     if (const char* s = std::getenv("FUGU_GS_VEC")) c->gs_vec = std::atoi(s) != 0;
     if (const char* s = std::getenv("FUGU_MG_NU_V")) c->mg_nu_v = std::min(4, std::max(1, std::atoi(s)));
     if (const char* fe = std::getenv("FUGU_FGMRES_EST")) c->fg_est = std::atof(fe);

@@ -302,37 +302,81 @@ struct MatvecArgs {
   int flags;
 };
 
+// Operands of one matvec point, read straight from the padded arrays.
+template <typename T>
+struct MvGlobal {
+  const Grid& g;
+  const MatvecArgs<T>& a;
+  __device__ T xu(int i, int j) const { return a.Xu[IU(g, i, j)]; }
+  __device__ T xv(int i, int j) const { return a.Xv[IV(g, i, j)]; }
+  __device__ T xp(int i, int j) const { return a.Xp[IP(g, i, j)]; }
+  __device__ T kap(int i, int j) const { return a.kap[IP(g, i, j)]; }
+  __device__ T ze(int i, int j) const { return a.zeta[IW(g, i, j)]; }
+  __device__ T et(int i, int j) const { return a.eta[IW(g, i, j)]; }
+  __device__ T ru(int i, int j) const { return a.rho_u[IU(g, i, j)]; }
+  __device__ T rv(int i, int j) const { return a.rho_v[IV(g, i, j)]; }
+  __device__ T rc(int i, int j) const { return a.rho_c[IP(g, i, j)]; }
+  __device__ T ua(int i, int j) const { return a.uadv[IU(g, i, j)]; }
+  __device__ T va(int i, int j) const { return a.vadv[IV(g, i, j)]; }
+  __device__ T pu(int i, int j) const { return a.penu[IU(g, i, j)]; }
+  __device__ T pv(int i, int j) const { return a.penv[IV(g, i, j)]; }
+  __device__ T mdu(int i, int j) const { return a.mdu[IU(g, i, j)]; }
+  __device__ T mdv(int i, int j) const { return a.mdv[IV(g, i, j)]; }
+};
+
+// Operands of one matvec point from LDS tiles of the arrays: the workgroup's block of points
+// grown by one cell on every side (every operand of a point lies within one cell of it).
+constexpr int MV_TW = TX + 2;
+enum MvQ : int { MQ_XU, MQ_XV, MQ_XP, MQ_KAP, MQ_ZE, MQ_ET, MQ_RU, MQ_RV, MQ_RC, MQ_UA, MQ_VA, MQ_PU, MQ_PV,
+                 MQ_MU, MQ_MV, MQ_N };
+template <typename T, int TYT>
+struct MvTile {
+  const T (*s)[TYT + 2][MV_TW];
+  int i0, j0;   // (i, j) of tile element [0][0]
+  __device__ T at(int q, int i, int j) const { return s[q][j - j0][i - i0]; }
+  __device__ T xu(int i, int j) const { return at(MQ_XU, i, j); }
+  __device__ T xv(int i, int j) const { return at(MQ_XV, i, j); }
+  __device__ T xp(int i, int j) const { return at(MQ_XP, i, j); }
+  __device__ T kap(int i, int j) const { return at(MQ_KAP, i, j); }
+  __device__ T ze(int i, int j) const { return at(MQ_ZE, i, j); }
+  __device__ T et(int i, int j) const { return at(MQ_ET, i, j); }
+  __device__ T ru(int i, int j) const { return at(MQ_RU, i, j); }
+  __device__ T rv(int i, int j) const { return at(MQ_RV, i, j); }
+  __device__ T rc(int i, int j) const { return at(MQ_RC, i, j); }
+  __device__ T ua(int i, int j) const { return at(MQ_UA, i, j); }
+  __device__ T va(int i, int j) const { return at(MQ_VA, i, j); }
+  __device__ T pu(int i, int j) const { return at(MQ_PU, i, j); }
+  __device__ T pv(int i, int j) const { return at(MQ_PV, i, j); }
+  __device__ T mdu(int i, int j) const { return at(MQ_MU, i, j); }
+  __device__ T mdv(int i, int j) const { return at(MQ_MV, i, j); }
+};
+
 // Storage type T (double: the drop-in; float: the mixed form's inner operator); the
 // arithmetic is always FP64 in registers, so FP32 storage does not turn the cancellation
 // inside a row (rho/dt X against the stress and pressure terms) into operator error.
-template <typename T>
-__global__ void __launch_bounds__(NT) k_matvec(Grid g, MatvecArgs<T> a, const int* __restrict__ skip) {
-  if (skip && *skip) return;
+template <typename T, typename S>
+__device__ inline void matvec_point(const Grid& g, const MatvecArgs<T>& a, const S& s, int i, int j) {
   using R = double;
-  int i, j;
-  union_ij(g, i, j);
   const int nx = g.nx, ny = g.ny, nh = g.nh;
   const R inv_dx = g.inv_dx, inv_dy = g.inv_dy, half = 0.5;
   const int i_u_max = g.per_x ? nx : nx - 1;
   const int j_v_max = g.per_y ? ny : ny - 1;
   const bool with_adv = (a.flags & MV_ADV) != 0;
   const bool cons = (a.flags & MV_CONS) != 0;
-  const T* Xu = a.Xu;
-  const T* Xv = a.Xv;
   const R dt = (R)a.dt;
-  auto XU = [&](int ii, int jj) -> R { return (R)Xu[IU(g, ii, jj)]; };
-  auto XV = [&](int ii, int jj) -> R { return (R)Xv[IV(g, ii, jj)]; };
+  auto XU = [&](int ii, int jj) -> R { return (R)s.xu(ii, jj); };
+  auto XV = [&](int ii, int jj) -> R { return (R)s.xv(ii, jj); };
 
-  auto Txx = [&](int ii, int jj) -> R { return (R)a.kap[IP(g, ii, jj)] * (XU(ii, jj) - XU(ii - 1, jj)) * inv_dx; };
-  auto Tyy = [&](int ii, int jj) -> R { return (R)a.kap[IP(g, ii, jj)] * (XV(ii, jj) - XV(ii, jj - 1)) * inv_dy; };
+  auto Txx = [&](int ii, int jj) -> R { return (R)s.kap(ii, jj) * (XU(ii, jj) - XU(ii - 1, jj)) * inv_dx; };
+  auto Tyy = [&](int ii, int jj) -> R { return (R)s.kap(ii, jj) * (XV(ii, jj) - XV(ii, jj - 1)) * inv_dy; };
   auto dudy = [&](int ii, int jj) -> R { return (XU(ii, jj + 1) - XU(ii, jj)) * inv_dy; };
   auto dvdx = [&](int ii, int jj) -> R { return (XV(ii + 1, jj) - XV(ii, jj)) * inv_dx; };
   auto Txy = [&](int ii, int jj) -> R {
-    const R ze = a.zeta[IW(g, ii, jj)], et = a.eta[IW(g, ii, jj)];
+    const R ze = s.ze(ii, jj), et = s.et(ii, jj);
     return (ze - et) * dudy(ii, jj) + et * dvdx(ii, jj);
   };
   auto Tyx = [&](int ii, int jj) -> R {
-    const R ze = a.zeta[IW(g, ii, jj)], et = a.eta[IW(g, ii, jj)];
+    const R ze = s.ze(ii, jj), et = s.et(ii, jj);
     return et * dudy(ii, jj) + (ze - et) * dvdx(ii, jj);
   };
 
@@ -341,22 +385,22 @@ __global__ void __launch_bounds__(NT) k_matvec(Grid g, MatvecArgs<T> a, const in
     R y = 0.0;
     if (j >= 1 && j <= ny && i >= 1 && i <= i_u_max) {
       const R visc = -((Txx(i + 1, j) - Txx(i, j)) * inv_dx + (Txy(i, j) - Txy(i, j - 1)) * inv_dy);
-      const R grad = ((R)a.Xp[IP(g, i + 1, j)] - (R)a.Xp[IP(g, i, j)]) * inv_dx;
+      const R grad = ((R)s.xp(i + 1, j) - (R)s.xp(i, j)) * inv_dx;
       R adv = 0.0;
       if (with_adv) {
         R mE, mW, mN, mS;
         if (cons) {
-          mE = half * ((R)a.mdu[IU(g, i, j)] + (R)a.mdu[IU(g, i + 1, j)]);
-          mW = half * ((R)a.mdu[IU(g, i - 1, j)] + (R)a.mdu[IU(g, i, j)]);
-          mN = half * ((R)a.mdv[IV(g, i, j)] + (R)a.mdv[IV(g, i + 1, j)]);
-          mS = half * ((R)a.mdv[IV(g, i, j - 1)] + (R)a.mdv[IV(g, i + 1, j - 1)]);
+          mE = half * ((R)s.mdu(i, j) + (R)s.mdu(i + 1, j));
+          mW = half * ((R)s.mdu(i - 1, j) + (R)s.mdu(i, j));
+          mN = half * ((R)s.mdv(i, j) + (R)s.mdv(i + 1, j));
+          mS = half * ((R)s.mdv(i, j - 1) + (R)s.mdv(i + 1, j - 1));
         } else {
-          mE = (R)a.rho_c[IP(g, i + 1, j)] * half * ((R)a.uadv[IU(g, i, j)] + (R)a.uadv[IU(g, i + 1, j)]);
-          mW = (R)a.rho_c[IP(g, i, j)] * half * ((R)a.uadv[IU(g, i - 1, j)] + (R)a.uadv[IU(g, i, j)]);
-          mN = half * ((R)a.rho_v[IV(g, i, j)] + (R)a.rho_v[IV(g, i + 1, j)]) * half *
-               ((R)a.vadv[IV(g, i, j)] + (R)a.vadv[IV(g, i + 1, j)]);
-          mS = half * ((R)a.rho_v[IV(g, i, j - 1)] + (R)a.rho_v[IV(g, i + 1, j - 1)]) * half *
-               ((R)a.vadv[IV(g, i, j - 1)] + (R)a.vadv[IV(g, i + 1, j - 1)]);
+          mE = (R)s.rc(i + 1, j) * half * ((R)s.ua(i, j) + (R)s.ua(i + 1, j));
+          mW = (R)s.rc(i, j) * half * ((R)s.ua(i - 1, j) + (R)s.ua(i, j));
+          mN = half * ((R)s.rv(i, j) + (R)s.rv(i + 1, j)) * half *
+               ((R)s.va(i, j) + (R)s.va(i + 1, j));
+          mS = half * ((R)s.rv(i, j - 1) + (R)s.rv(i + 1, j - 1)) * half *
+               ((R)s.va(i, j - 1) + (R)s.va(i + 1, j - 1));
         }
         const R uc = XU(i, j);
         const R uE = half * (uc + XU(i + 1, j));
@@ -366,8 +410,8 @@ __global__ void __launch_bounds__(NT) k_matvec(Grid g, MatvecArgs<T> a, const in
         adv = (mE * uE - mW * uW) * inv_dx + (mN * uN - mS * uS) * inv_dy;
       }
       const R x = XU(i, j);
-      y = ((R)a.rho_u[IU(g, i, j)] / dt) * x + adv + visc + grad;
-      if (a.flags & MV_PEN) y = y + (R)a.penu[IU(g, i, j)] * x;
+      y = ((R)s.ru(i, j) / dt) * x + adv + visc + grad;
+      if (a.flags & MV_PEN) y = y + (R)s.pu(i, j) * x;
     } else if (!g.per_x && j >= 1 && j <= ny && (i == 0 || i == nx)) {
       y = XU(i, j);
     }
@@ -378,22 +422,22 @@ __global__ void __launch_bounds__(NT) k_matvec(Grid g, MatvecArgs<T> a, const in
     R y = 0.0;
     if (j >= 1 && j <= j_v_max && i >= 1 && i <= nx) {
       const R visc = -((Tyx(i, j) - Tyx(i - 1, j)) * inv_dx + (Tyy(i, j + 1) - Tyy(i, j)) * inv_dy);
-      const R grad = ((R)a.Xp[IP(g, i, j + 1)] - (R)a.Xp[IP(g, i, j)]) * inv_dy;
+      const R grad = ((R)s.xp(i, j + 1) - (R)s.xp(i, j)) * inv_dy;
       R adv = 0.0;
       if (with_adv) {
         R mE, mW, mN, mS;
         if (cons) {
-          mE = half * ((R)a.mdu[IU(g, i, j)] + (R)a.mdu[IU(g, i, j + 1)]);
-          mW = half * ((R)a.mdu[IU(g, i - 1, j)] + (R)a.mdu[IU(g, i - 1, j + 1)]);
-          mN = half * ((R)a.mdv[IV(g, i, j)] + (R)a.mdv[IV(g, i, j + 1)]);
-          mS = half * ((R)a.mdv[IV(g, i, j - 1)] + (R)a.mdv[IV(g, i, j)]);
+          mE = half * ((R)s.mdu(i, j) + (R)s.mdu(i, j + 1));
+          mW = half * ((R)s.mdu(i - 1, j) + (R)s.mdu(i - 1, j + 1));
+          mN = half * ((R)s.mdv(i, j) + (R)s.mdv(i, j + 1));
+          mS = half * ((R)s.mdv(i, j - 1) + (R)s.mdv(i, j));
         } else {
-          mE = half * ((R)a.rho_u[IU(g, i, j)] + (R)a.rho_u[IU(g, i, j + 1)]) * half *
-               ((R)a.uadv[IU(g, i, j)] + (R)a.uadv[IU(g, i, j + 1)]);
-          mW = half * ((R)a.rho_u[IU(g, i - 1, j)] + (R)a.rho_u[IU(g, i - 1, j + 1)]) * half *
-               ((R)a.uadv[IU(g, i - 1, j)] + (R)a.uadv[IU(g, i - 1, j + 1)]);
-          mN = (R)a.rho_c[IP(g, i, j + 1)] * half * ((R)a.vadv[IV(g, i, j)] + (R)a.vadv[IV(g, i, j + 1)]);
-          mS = (R)a.rho_c[IP(g, i, j)] * half * ((R)a.vadv[IV(g, i, j - 1)] + (R)a.vadv[IV(g, i, j)]);
+          mE = half * ((R)s.ru(i, j) + (R)s.ru(i, j + 1)) * half *
+               ((R)s.ua(i, j) + (R)s.ua(i, j + 1));
+          mW = half * ((R)s.ru(i - 1, j) + (R)s.ru(i - 1, j + 1)) * half *
+               ((R)s.ua(i - 1, j) + (R)s.ua(i - 1, j + 1));
+          mN = (R)s.rc(i, j + 1) * half * ((R)s.va(i, j) + (R)s.va(i, j + 1));
+          mS = (R)s.rc(i, j) * half * ((R)s.va(i, j - 1) + (R)s.va(i, j));
         }
         const R vc = XV(i, j);
         const R vE = half * (vc + XV(i + 1, j));
@@ -403,8 +447,8 @@ __global__ void __launch_bounds__(NT) k_matvec(Grid g, MatvecArgs<T> a, const in
         adv = (mE * vE - mW * vW) * inv_dx + (mN * vN - mS * vS) * inv_dy;
       }
       const R x = XV(i, j);
-      y = ((R)a.rho_v[IV(g, i, j)] / dt) * x + adv + visc + grad;
-      if (a.flags & MV_PEN) y = y + (R)a.penv[IV(g, i, j)] * x;
+      y = ((R)s.rv(i, j) / dt) * x + adv + visc + grad;
+      if (a.flags & MV_PEN) y = y + (R)s.pv(i, j) * x;
     } else if (!g.per_y && i >= 1 && i <= nx && (j == 0 || j == ny)) {
       y = XV(i, j);
     }
@@ -417,6 +461,70 @@ __global__ void __launch_bounds__(NT) k_matvec(Grid g, MatvecArgs<T> a, const in
       y = (XU(i, j) - XU(i - 1, j)) * inv_dx + (XV(i, j) - XV(i, j - 1)) * inv_dy;
     a.Yp[IP(g, i, j)] = (T)y;
   }
+}
+
+template <typename T>
+__global__ void __launch_bounds__(NT) k_matvec(Grid g, MatvecArgs<T> a, const int* __restrict__ skip) {
+  if (skip && *skip) return;
+  int i, j;
+  union_ij(g, i, j);
+  matvec_point(g, a, MvGlobal<T>{g, a}, i, j);
+}
+
+// Array kinds of the padded layout: 0 u-kind, 1 v-kind, 2 p-kind, 3 vertex kind.
+__device__ inline bool kind_has(const Grid& g, int kind, int i, int j) {
+  const int nx = g.nx, ny = g.ny, nh = g.nh;
+  const int ilo = (kind == 0 || kind == 3) ? -nh : 1 - nh, jlo = (kind == 1 || kind == 3) ? -nh : 1 - nh;
+  return i >= ilo && i <= nx + nh && j >= jlo && j <= ny + nh;
+}
+__device__ inline size_t kind_at(const Grid& g, int kind, int i, int j) {
+  return kind == 0 ? IU(g, i, j) : (kind == 1 ? IV(g, i, j) : (kind == 2 ? IP(g, i, j) : IW(g, i, j)));
+}
+
+// The same matvec with every operand staged once per workgroup: TX x TYT points, their arrays'
+// tiles (one halo cell) loaded with all loads in flight, then each point computed from LDS by
+// the same arithmetic (bit-identical output).  Arrays a flag switches off are not loaded.
+template <typename T, int TYT>
+__global__ void __launch_bounds__(TX * TYT) k_matvec_tile(Grid g, MatvecArgs<T> a, const int* __restrict__ skip) {
+  if (skip && *skip) return;
+  __shared__ T s[MQ_N][TYT + 2][MV_TW];
+  int bx, by;
+  tile_coords(bx, by);
+  const int ib = bx * TX - g.nh, jb = by * TYT - g.nh;   // the block's first point
+  const int i0 = ib - 1, j0 = jb - 1;                   // tile origin
+  const int tid = (int)threadIdx.x + TX * (int)threadIdx.y;
+  constexpr int NTILE = (TYT + 2) * MV_TW;
+  constexpr int NE = (NTILE + TX * TYT - 1) / (TX * TYT);
+  const bool adv = (a.flags & MV_ADV) != 0, cons = (a.flags & MV_CONS) != 0, pen = (a.flags & MV_PEN) != 0;
+  const T* src[MQ_N] = {a.Xu, a.Xv, a.Xp, a.kap, a.zeta, a.eta, a.rho_u, a.rho_v,
+                        (adv && !cons) ? a.rho_c : nullptr, (adv && !cons) ? a.uadv : nullptr,
+                        (adv && !cons) ? a.vadv : nullptr, pen ? a.penu : nullptr, pen ? a.penv : nullptr,
+                        (adv && cons) ? a.mdu : nullptr, (adv && cons) ? a.mdv : nullptr};
+  constexpr int kind[MQ_N] = {0, 1, 2, 2, 3, 3, 0, 1, 2, 0, 1, 0, 1, 0, 1};
+  T v[MQ_N][NE];
+#pragma unroll
+  for (int e = 0; e < NE; ++e) {
+    const int idx = tid + TX * TYT * e;
+    const int tj = idx / MV_TW, ti = idx - tj * MV_TW;
+    const int ii = i0 + ti, jj = j0 + tj;
+#pragma unroll
+    for (int q = 0; q < MQ_N; ++q) {
+      T x = T(0);
+      if (src[q] && idx < NTILE && kind_has(g, kind[q], ii, jj)) x = src[q][kind_at(g, kind[q], ii, jj)];
+      v[q][e] = x;
+    }
+  }
+#pragma unroll
+  for (int e = 0; e < NE; ++e) {
+    const int idx = tid + TX * TYT * e;
+    const int tj = idx / MV_TW, ti = idx - tj * MV_TW;
+    if (idx < NTILE) {
+#pragma unroll
+      for (int q = 0; q < MQ_N; ++q) s[q][tj][ti] = v[q][e];
+    }
+  }
+  __syncthreads();
+  matvec_point(g, a, MvTile<T, TYT>{s, i0, j0}, ib + (int)threadIdx.x, jb + (int)threadIdx.y);
 }
 
 // ---------------------------------------------------------------------------------
