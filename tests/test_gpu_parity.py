@@ -65,9 +65,19 @@ def test_halo_and_bcs_bitwise(kw):
     same(u2, u, "bcs u hom"); same(v2, v, "bcs v hom")
 
 
-@pytest.mark.parametrize("kw", GRIDS, ids=IDS)
+# the matvec also on grids whose extents are no multiple of its 64 x 8 LDS tiles
+MV_GRIDS = GRIDS + [dict(nx=131, ny=67, lx=1.31, ly=0.67, per_x=False, per_y=False),
+                    dict(nx=130, ny=70, lx=1.3, ly=0.7, per_x=True, per_y=False)]
+MV_IDS = IDS + ["walls_131x67", "periodic_x_130x70"]
+
+
+@pytest.mark.parametrize("kw", MV_GRIDS, ids=MV_IDS)
 @pytest.mark.parametrize("variant", ["plain", "noadv", "penalty", "amp"])
-def test_matvec_bitwise(kw, variant):
+@pytest.mark.parametrize("tile", ["8", "4", "0"])
+def test_matvec_bitwise(monkeypatch, kw, variant, tile):
+    """Both matvec kernels (operands from LDS tiles of 8 or 4 rows, or straight from the arrays)
+    equal the oracle bit for bit."""
+    monkeypatch.setenv("FUGU_MV_TILE", tile)
     g = grid(kw)
     ctx = ctx_for(g)
     rng = np.random.default_rng(101)
