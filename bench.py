@@ -127,6 +127,8 @@ KERNELS = ("pcg_iter1_f64", "pcg_iter2_f64", "pcg_iter1_f32", "pcg_iter2_f32", "
            "maxpy_f32", "maxpy_update", "maxpy_dot_f64", "maxpy_dot_f32", "mg_pre_f64", "mg_pre_f32", "mg_post_f64",
            "mg_post_f32", "mg_pre3_f64", "mg_pre3_f32", "mg_post3_f64", "mg_post3_f32", "mg_cut_f64", "mg_cut_f32")
 DEVICE_NAMES = {"maxpy_f64": "k_maxpy<double, double>", "maxpy_f32": "k_maxpy<float, float>",
+                # the matvec from LDS tiles of 8 rows (the library's default)
+                "matvec_f64": "k_matvec_tile<double, 8>", "matvec_f32": "k_matvec_tile<float, 8>",
                 "maxpy_update": "k_maxpy<double, float>",
                 # fused multigrid levels: two red-black sweeps per half (pressure), three (velocity)
                 "mg_pre_f64": "k_mg_pre<double, 2>", "mg_pre_f32": "k_mg_pre<float, 2>",
