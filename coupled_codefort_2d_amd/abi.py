@@ -67,8 +67,12 @@ PRECOND_VISC = 16
 PRECOND_VMG = 32
 SOLVE_EXTRAPOLATE = 64
 
+SUM_REFERENCE = 0   # the reference's sequential summation order (default)
+SUM_TREE = 1        # double-double tree (fast)
+
 EXPORTS = [
     "fugu_ctx_create", "fugu_ctx_destroy", "fugu_set_params", "fugu_last_error", "fugu_field_size",
+    "fugu_set_sum_order", "fugu_get_sum_order",
     "fugu_exchange_halos", "fugu_op_divergence", "fugu_apply_velocity_bcs", "fugu_matvec_coupled",
     "fugu_coupled_dot", "fugu_solve_poisson_pressure", "fugu_solve_helmholtz_u", "fugu_solve_helmholtz_v",
     "fugu_apply_pcd_precond", "fugu_bicgstab_solve", "fugu_bicgstab2_solve", "fugu_vof_advection_tvd",
@@ -105,6 +109,8 @@ def _declare(L):
     s("fugu_ctx_create", [C.POINTER(FuguGrid), C.POINTER(FuguParams), C.POINTER(P)])
     s("fugu_ctx_destroy", [P])
     s("fugu_set_params", [P, C.POINTER(FuguParams)])
+    s("fugu_set_sum_order", [P, C.c_int])
+    s("fugu_get_sum_order", [P])
     s("fugu_last_error", [], C.c_char_p)
     s("fugu_field_size", [P, C.c_int], C.c_longlong)
     s("fugu_exchange_halos", [P, _D, C.c_int])
@@ -164,7 +170,7 @@ def check(rc, what):
 class Context:
     """One device-resident FUGU-2D domain (fugu_ctx)."""
 
-    def __init__(self, nx, ny, lx=1.0, ly=1.0, per_x=False, per_y=False, params=None, nh=2):
+    def __init__(self, nx, ny, lx=1.0, ly=1.0, per_x=False, per_y=False, params=None, nh=2, sum_order=None):
         L = load()
         self.nx, self.ny, self.nh, self.lx, self.ly = nx, ny, nh, lx, ly
         self.per_x, self.per_y = per_x, per_y
@@ -173,6 +179,8 @@ class Context:
         h = C.c_void_p()
         check(L.fugu_ctx_create(C.byref(self.grid), C.byref(self.params), C.byref(h)), "fugu_ctx_create")
         self.h = h
+        if sum_order is not None:
+            self.set_sum_order(sum_order)
 
     def close(self):
         if getattr(self, "h", None):
@@ -184,6 +192,13 @@ class Context:
             self.close()
         except Exception:
             pass
+
+    def set_sum_order(self, order):
+        """SUM_REFERENCE (the reference's sequential sums, bit-identical) or SUM_TREE (fast)."""
+        check(load().fugu_set_sum_order(self.h, int(order)), "fugu_set_sum_order")
+
+    def sum_order(self):
+        return check(load().fugu_get_sum_order(self.h), "fugu_get_sum_order")
 
     def set_params(self, params):
         self.params = params

@@ -16,8 +16,6 @@ ROOT = os.path.dirname(HERE)
 INCLUDE = os.path.join(ROOT, "include")
 LIB = os.path.join(HERE, "libfugu2d_hip.so")
 SOURCES = [os.path.join(CSRC, "fugu2d_api.hip")]
-HEADERS = [os.path.join(CSRC, f) for f in ("fugu2d_common.hpp", "fugu2d_kernels.hpp")] + \
-          [os.path.join(INCLUDE, "fugu2d.h")]
 ARCH = os.environ.get("FUGU_OFFLOAD_ARCH", "gfx950")
 
 
@@ -43,13 +41,6 @@ def build(verbose: bool = True) -> str:
     subprocess.run(cmd, check=True)
     os.replace(tmp, LIB)
     return LIB
-
-
-def needs_build() -> bool:
-    if not os.path.exists(LIB):
-        return True
-    t = os.path.getmtime(LIB)
-    return any(os.path.getmtime(f) > t for f in SOURCES + HEADERS)
 
 
 if __name__ == "__main__":

@@ -112,6 +112,14 @@ struct fugu_ctx {
   int* fl = nullptr;
   double* partials = nullptr;
   size_t partials_n = 0;   // its length in doubles
+  // Summation order of the reference-algorithm reductions (coupled_dot, the PCG's dots and
+  // means): true (default) = the reference's sequential order, bit for bit; false = the
+  // double-double tree (fast).  The chosen fast forms and the adjoint always use the tree
+  // (tree_depth > 0 inside them).  terms: the per-term buffer of the reference order,
+  // 5 chains x (owned u + v + p points), allocated on first use.
+  bool sum_ref = true;
+  int tree_depth = 0;
+  double* terms = nullptr;
   double* h_sc = nullptr;
   int* h_fl = nullptr;
   size_t ntrip = 0;
@@ -263,6 +271,33 @@ T* dalloc(fugu_ctx* c, size_t n) {
   return (T*)p;
 }
 
+// Reductions in the reference's order unless a fast form or the adjoint is running.
+bool seq_sums(const fugu_ctx* c) { return c->sum_ref && c->tree_depth == 0; }
+// Sink of a reduction with `nterms` terms per chain: the term buffer (reference order) or the
+// per-workgroup partials (tree).
+Sink sink(fugu_ctx* c, long nterms) {
+  if (!seq_sums(c)) return Sink{c->partials, 0};
+  if (!c->terms) {
+    // at most 5 chains (the fused GCR(2) dots) x every owned point of a triplet
+    const Grid& g = c->g;
+    c->terms = dalloc<double>(c, (size_t)5 * 3 * (size_t)g.nx * (size_t)g.ny);
+  }
+  return Sink{c->terms, nterms};
+}
+long cdot_terms(const Grid& g) {
+  const long iu = g.per_x ? g.nx : g.nx - 1, jv = g.per_y ? g.ny : g.ny - 1;
+  return (long)g.ny * iu + jv * (long)g.nx + (long)g.nx * g.ny;
+}
+long cell_terms(const Grid& g) { return (long)g.nx * g.ny; }
+// The chosen fast forms and the adjoint sum in the tree order (not parity-bound).
+struct TreeSums {
+  fugu_ctx* c;
+  explicit TreeSums(fugu_ctx* ctx) : c(ctx) { c->tree_depth++; }
+  ~TreeSums() { c->tree_depth--; }
+  TreeSums(const TreeSums&) = delete;
+  TreeSums& operator=(const TreeSums&) = delete;
+};
+
 inline unsigned nblk(size_t n) {
   size_t b = (n + 255) / 256;
   if (b > 8192) b = 8192;
@@ -357,19 +392,19 @@ void cdot(fugu_ctx* c, int np, const T* const* a, const T* const* b, int phase, 
   const dim3 gr = grid_union_red(c->g), bl(TX, TY);
   const int nb = (int)nblocks(gr);
   const int* fl = c->fl;
-  const double* pa = c->partials;
+  const Sink sk = sink(c, cdot_terms(c->g));
   switch (np) {
     case 1:
-      LAUNCH((k_cdot<T, 1>), gr, bl, c->s, c->g, d, c->partials, fl);
-      LAUNCH((k_fold_krylov<1>), dim3(1), dim3(NT), c->s, nb, phase, k, max_iter, pa, c->sc, c->fl);
+      LAUNCH((k_cdot<T, 1>), gr, bl, c->s, c->g, d, sk, fl);
+      LAUNCH((k_fold_krylov<1>), dim3(1), dim3(NT), c->s, nb, phase, k, max_iter, sk, c->sc, c->fl);
       break;
     case 2:
-      LAUNCH((k_cdot<T, 2>), gr, bl, c->s, c->g, d, c->partials, fl);
-      LAUNCH((k_fold_krylov<2>), dim3(1), dim3(NT), c->s, nb, phase, k, max_iter, pa, c->sc, c->fl);
+      LAUNCH((k_cdot<T, 2>), gr, bl, c->s, c->g, d, sk, fl);
+      LAUNCH((k_fold_krylov<2>), dim3(1), dim3(NT), c->s, nb, phase, k, max_iter, sk, c->sc, c->fl);
       break;
     case 5:
-      LAUNCH((k_cdot<T, 5>), gr, bl, c->s, c->g, d, c->partials, fl);
-      LAUNCH((k_fold_krylov<5>), dim3(1), dim3(NT), c->s, nb, phase, k, max_iter, pa, c->sc, c->fl);
+      LAUNCH((k_cdot<T, 5>), gr, bl, c->s, c->g, d, sk, fl);
+      LAUNCH((k_fold_krylov<5>), dim3(1), dim3(NT), c->s, nb, phase, k, max_iter, sk, c->sc, c->fl);
       break;
     default: throw Error("cdot: unsupported pair count");
   }
@@ -379,9 +414,10 @@ void cdot(fugu_ctx* c, int np, const T* const* a, const T* const* b, int phase, 
 template <typename T>
 void cell_mean(fugu_ctx* c, const T* a, int slot, int skip_mode) {
   const dim3 gr = grid_cells_red(c->g);
-  LAUNCH((k_pcg_sum<T>), gr, dim3(TX, TY), c->s, c->g, a, c->partials, skip_mode, (const int*)c->fl);
-  LAUNCH(k_fold_sum, dim3(1), dim3(NT), c->s, c->g, (int)nblocks(gr), slot, skip_mode, (const double*)c->partials,
-         c->sc, (const int*)c->fl);
+  const Sink sk = sink(c, cell_terms(c->g));
+  LAUNCH((k_pcg_sum<T>), gr, dim3(TX, TY), c->s, c->g, a, sk, skip_mode, (const int*)c->fl);
+  LAUNCH(k_fold_sum, dim3(1), dim3(NT), c->s, c->g, (int)nblocks(gr), slot, skip_mode, sk, c->sc,
+         (const int*)c->fl);
 }
 template <typename T>
 void cdot1(fugu_ctx* c, const T* a, const T* b, int phase, int k = 0, int mi = 0) {
@@ -471,24 +507,24 @@ void pcg(fugu_ctx* c, Work<T>& w, T* phi, const T* rhs, bool phi_zero, double to
   const int walls_both = (force_proj || (!g.per_x && !g.per_y)) ? 1 : 0;
   const dim3 gc = grid_cells_red(g), bl(TX, TY), b1(NT);
   const int nb = (int)nblocks(gc);
-  const double* pa = c->partials;
+  const Sink sk = sink(c, cell_terms(g));
   const double* sc = c->sc;
   const int* fl = c->fl;
   c->pcg_calls++;
   if (walls_both) cell_mean<T>(c, rhs, (int)S_P_MEAN, 1);
   LAUNCH((k_pcg_init<T>), gc, bl, c->s, g, rhs, phi, w.cx, w.cy, w.dinv, w.pr, w.pz, w.pp0, walls_both,
-         phi_zero ? 1 : 0, c->partials, sc, fl);
-  LAUNCH(k_fold_pcg_init, dim3(1), b1, c->s, nb, pa, c->sc, c->fl);
+         phi_zero ? 1 : 0, sk, sc, fl);
+  LAUNCH(k_fold_pcg_init, dim3(1), b1, c->s, nb, sk, c->sc, c->fl);
   T* P[2] = {w.pp0, w.pp1};
   for (int it = 1; it <= max_iter; ++it) {
     // z, p_old, c_x, c_y read; p_new, Ap written
     TLAUNCH(c, sizeof(T) == 8 ? "pcg_iter1_f64" : "pcg_iter1_f32", 6.0 * sizeof(T) * g.nx * g.ny, (k_pcg_iter1<T>), gc, bl, c->s, g, it,
-            w.pz, P[(it - 1) & 1], P[it & 1], w.Ap, w.cx, w.cy, c->partials, sc, fl);
-    LAUNCH(k_fold_pcg_iter1, dim3(1), b1, c->s, nb, it, pa, c->sc, c->fl);
+            w.pz, P[(it - 1) & 1], P[it & 1], w.Ap, w.cx, w.cy, sk, sc, fl);
+    LAUNCH(k_fold_pcg_iter1, dim3(1), b1, c->s, nb, it, sk, c->sc, c->fl);
     // phi, p, r, Ap, D^-1 read; phi, r, z written
     TLAUNCH(c, sizeof(T) == 8 ? "pcg_iter2_f64" : "pcg_iter2_f32", 8.0 * sizeof(T) * g.nx * g.ny, (k_pcg_iter2<T>), gc, bl, c->s, g, phi,
-            P[it & 1], w.pr, w.Ap, w.pz, w.dinv, c->partials, sc, fl);
-    LAUNCH(k_fold_pcg_iter2, dim3(1), b1, c->s, nb, it, max_iter, pa, c->sc, c->fl);
+            P[it & 1], w.pr, w.Ap, w.pz, w.dinv, sk, sc, fl);
+    LAUNCH(k_fold_pcg_iter2, dim3(1), b1, c->s, nb, it, max_iter, sk, c->sc, c->fl);
   }
   (void)tol;
 }
@@ -873,7 +909,7 @@ const T* mg_pressure_compact(fugu_ctx* c, Work<T>& w, const T* Zp) {
       nb = (int)std::min<unsigned>(nblk((size_t)L0.d.nx * L0.d.ny), (unsigned)RED_MAXB);
       LAUNCH((k_mg_sum<T>), dim3((unsigned)nb), dim3(256), c->s, L0.d, res, c->partials);
     }
-    LAUNCH(k_fold_sum, dim3(1), dim3(NT), c->s, g, nb, (int)S_P_MEAN, 0, (const double*)c->partials, c->sc,
+    LAUNCH(k_fold_sum, dim3(1), dim3(NT), c->s, g, nb, (int)S_P_MEAN, 0, Sink{c->partials, 0}, c->sc,
            (const int*)c->fl);
   }
   return res;
@@ -1095,6 +1131,7 @@ void bicgstab2(fugu_ctx* c, Work<T>& w, double dt, double tol, int max_iter, int
 // True relative residual ||B - A X|| / ||B|| (coupled_dot norm, FP64) of the resident
 // X triplet against the FP64 workspace B.  Uses wd.z (A X) and wd.q (residual).
 double true_residual(fugu_ctx* c, const double* X, const double* B, int mvflags, double dt) {
+  TreeSums tree(c);   // a diagnostic, not a reference quantity
   Work<double>& w = c->wd;
   HIPCHK(hipMemcpyAsync(w.X, X, c->ntrip * sizeof(double), hipMemcpyDeviceToDevice, c->s));
   bcs_trip(c, w.X);
@@ -1289,6 +1326,7 @@ template <typename T>
 void solve_fgmres(fugu_ctx* c, Work<T>& w, double* X, double dt, double tol, double inner_tol, int max_steps,
                   int max_inner, int m, int mvflags, bool has_lid, double lid, int* iters, int* outer,
                   double* res) {
+  TreeSums tree(c);
   Work<double>& wd = c->wd;
   const size_t n3 = c->ntrip;
   const bool ir32 = sizeof(T) == sizeof(float) && max_inner == 0 && !c->adjoint;
@@ -1738,6 +1776,7 @@ int step_impl(fugu_ctx* c, double dt, int step_idx, const fugu_step_opts* o, fug
 int adjoint_run(fugu_ctx* c, const fugu_adjoint_opts* o, const double* mu_in, const double* rho_in, double sigma,
                 const double* c0, const double* u0, const double* v0, double* g_mu, double* g_rho,
                 double* g_sigma, fugu_adjoint_stats* st) {
+  TreeSums tree(c);
   const Grid& g = c->g;
   const fugu_params& P = c->prm;
   const int N = o->nsteps;
@@ -2140,6 +2179,21 @@ int fugu_set_params(fugu_ctx* c, const fugu_params* p) {
   return 0;
 }
 
+int fugu_set_sum_order(fugu_ctx* c, int order) {
+  NEED(c);
+  if (order != FUGU_SUM_REFERENCE && order != FUGU_SUM_TREE) {
+    g_err = "fugu_set_sum_order: order must be FUGU_SUM_REFERENCE (0) or FUGU_SUM_TREE (1)";
+    return -3;
+  }
+  c->sum_ref = order == FUGU_SUM_REFERENCE;
+  return 0;
+}
+
+int fugu_get_sum_order(const fugu_ctx* c) {
+  NEED(c);
+  return c->sum_ref ? FUGU_SUM_REFERENCE : FUGU_SUM_TREE;
+}
+
 int fugu_exchange_halos(fugu_ctx* c, double* a, int kind) {
   NEED(c);
   return guarded([&]() {
@@ -2277,6 +2331,7 @@ int fugu_solve_pressure_mg(fugu_ctx* c, double* phi, const double* rhs, const do
                            const double* rho_v, int n_cycles) {
   NEED(c);
   return guarded([&]() {
+    TreeSums tree(c);
     const Grid& g = c->g;
     Work<double>& w = c->wd;
     in(c, c->rho_u, rho_u, g.nu); in(c, c->rho_v, rho_v, g.nv);
@@ -2657,6 +2712,7 @@ int fugu_apply_pcd_transpose(fugu_ctx* c, const double* yu, const double* yv, co
                              double dt, double ptol, int pmax, double visc) {
   NEED(c);
   return guarded([&]() {
+    TreeSums tree(c);
     const Grid& g = c->g;
     Work<double>& w = c->wd;
     upload_props(c, uadv, vadv, nullptr, rho_u, rho_v, kap, zeta, eta, nullptr, nullptr, nullptr, nullptr);

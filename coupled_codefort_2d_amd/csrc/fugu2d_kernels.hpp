@@ -530,7 +530,7 @@ __global__ void __launch_bounds__(TX * TYT) k_matvec_tile(Grid g, MatvecArgs<T> 
 // ---------------------------------------------------------------------------------
 // coupled dots (src/mod_bicgstab2.f90:32-72), up to 5 pairs in one pass over the
 // owned sets (u: j=1..ny, i=1..i_u_max; v: j=1..j_v_max, i=1..nx; p: interior), each
-// term (a*b)*vol, FP64 accumulation; the last workgroup runs the Krylov phase.
+// term (a*b)*vol in FP64, summed in the Sink's order; the fold runs the Krylov phase.
 template <typename T>
 struct DotArgs {
   const T* a[5];
@@ -540,15 +540,14 @@ struct DotArgs {
 };
 
 template <typename T, int NP>
-__global__ void __launch_bounds__(NT) k_cdot(Grid g, DotArgs<T> d, double* __restrict__ partials,
-                                             const int* __restrict__ fl) {
+__global__ void __launch_bounds__(NT) k_cdot(Grid g, DotArgs<T> d, Sink out, const int* __restrict__ fl) {
   if (fl[F_BRK]) return;
   const int nx = g.nx, ny = g.ny, nh = g.nh;
   const int i_u_max = g.per_x ? nx : nx - 1;
   const int j_v_max = g.per_y ? ny : ny - 1;
   const double vol = g.vol;
-  // terms (a*b)*vol as in the reference, compensated per pair so the total is the
-  // correctly rounded sum
+  // term positions in the reference's loop: u faces, then v faces, then cells (j outer)
+  const long nU = (long)ny * i_u_max, nV = (long)j_v_max * nx;
   DD v[NP];
 #pragma unroll
   for (int q = 0; q < NP; ++q) v[q] = dd_from(0.0);
@@ -558,30 +557,32 @@ __global__ void __launch_bounds__(NT) k_cdot(Grid g, DotArgs<T> d, double* __res
     const int j = t * TY + (int)threadIdx.y - nh;
     if (j >= 1 && j <= ny && i >= 1 && i <= i_u_max) {
       const size_t o = IU(g, i, j);
+      const long pos = (long)(j - 1) * i_u_max + (i - 1);
 #pragma unroll
-      for (int q = 0; q < NP; ++q) v[q] = dd_add(v[q], dd_from((double)d.a[q][o] * (double)d.b[q][o] * vol));
+      for (int q = 0; q < NP; ++q) sink_add<NP>(out, v, q, pos, (double)d.a[q][o] * (double)d.b[q][o] * vol);
     }
     if (j >= 1 && j <= j_v_max && i >= 1 && i <= nx) {
       const size_t o = g.nu + IV(g, i, j);
+      const long pos = nU + (long)(j - 1) * nx + (i - 1);
 #pragma unroll
-      for (int q = 0; q < NP; ++q) v[q] = dd_add(v[q], dd_from((double)d.a[q][o] * (double)d.b[q][o] * vol));
+      for (int q = 0; q < NP; ++q) sink_add<NP>(out, v, q, pos, (double)d.a[q][o] * (double)d.b[q][o] * vol);
     }
     if (j >= 1 && j <= ny && i >= 1 && i <= nx) {
       const size_t o = g.nu + g.nv + IP(g, i, j);
+      const long pos = nU + nV + (long)(j - 1) * nx + (i - 1);
 #pragma unroll
-      for (int q = 0; q < NP; ++q) v[q] = dd_add(v[q], dd_from((double)d.a[q][o] * (double)d.b[q][o] * vol));
+      for (int q = 0; q < NP; ++q) sink_add<NP>(out, v, q, pos, (double)d.a[q][o] * (double)d.b[q][o] * vol);
     }
   }
-  publish_dd<NP>(v, partials);
+  sink_publish<NP>(out, v);
 }
 
 template <int NP>
-__global__ void __launch_bounds__(NT) k_fold_krylov(int nb, int phase, int k, int max_iter,
-                                                    const double* __restrict__ partials, double* __restrict__ sc,
-                                                    int* __restrict__ fl) {
+__global__ void __launch_bounds__(NT) k_fold_krylov(int nb, int phase, int k, int max_iter, Sink in,
+                                                    double* __restrict__ sc, int* __restrict__ fl) {
   if (fl[F_BRK]) return;
   double tot[NP];
-  fold_dd<NP>(partials, nb, tot);
+  fold_sink<NP>(in, nb, tot);
   if (threadIdx.x == 0 && threadIdx.y == 0) {
     double t5[5] = {0, 0, 0, 0, 0};
 #pragma unroll
@@ -1056,8 +1057,11 @@ __device__ inline Nbr cell_nbrs(const Grid& g, int i, int j) {
 __device__ inline bool pcg_sum_skip(int skip_mode, const int* fl) {
   return (skip_mode == 1 && fl[F_BRK]) || (skip_mode == 2 && fl[F_P_DONE] >= 2);
 }
+// Cell sums: term position (j-1) nx + (i-1), the reference's j-outer / i-inner loop.
+__device__ inline long cell_pos(const Grid& g, int i, int j) { return (long)(j - 1) * g.nx + (i - 1); }
+
 template <typename T>
-__global__ void __launch_bounds__(NT) k_pcg_sum(Grid g, const T* __restrict__ a, double* __restrict__ partials,
+__global__ void __launch_bounds__(NT) k_pcg_sum(Grid g, const T* __restrict__ a, Sink out,
                                                 int skip_mode, const int* __restrict__ fl) {
   if (pcg_sum_skip(skip_mode, fl)) return;
   DD v[1] = {dd_from(0.0)};
@@ -1065,16 +1069,15 @@ __global__ void __launch_bounds__(NT) k_pcg_sum(Grid g, const T* __restrict__ a,
   const int tiles = (g.ny + TY - 1) / TY;
   for (int t = blockIdx.y; t < tiles; t += gridDim.y) {
     const int j = t * TY + (int)threadIdx.y + 1;
-    if (i <= g.nx && j <= g.ny) v[0] = dd_add(v[0], dd_from((double)a[IP(g, i, j)]));
+    if (i <= g.nx && j <= g.ny) sink_add<1>(out, v, 0, cell_pos(g, i, j), (double)a[IP(g, i, j)]);
   }
-  publish_dd<1>(v, partials);
+  sink_publish<1>(out, v);
 }
-__global__ void __launch_bounds__(NT) k_fold_sum(Grid g, int nb, int slot, int skip_mode,
-                                                 const double* __restrict__ partials, double* __restrict__ sc,
-                                                 const int* __restrict__ fl) {
+__global__ void __launch_bounds__(NT) k_fold_sum(Grid g, int nb, int slot, int skip_mode, Sink in,
+                                                 double* __restrict__ sc, const int* __restrict__ fl) {
   if (pcg_sum_skip(skip_mode, fl)) return;
   double tot[1];
-  fold_dd<1>(partials, nb, tot);
+  fold_sink<1>(in, nb, tot);
   if (threadIdx.x == 0 && threadIdx.y == 0) sc[slot] = tot[0] / (double)(g.nx * g.ny);
 }
 
@@ -1087,7 +1090,7 @@ __global__ void __launch_bounds__(NT) k_pcg_init(Grid g, const T* __restrict__ r
                                                  const T* __restrict__ cx, const T* __restrict__ cy,
                                                  const T* __restrict__ dinv, T* __restrict__ r,
                                                  T* __restrict__ z, T* __restrict__ p, int walls_both,
-                                                 int phi_zero, double* __restrict__ partials,
+                                                 int phi_zero, Sink out,
                                                  const double* __restrict__ sc, const int* __restrict__ fl) {
   if (fl[F_BRK]) return;
   DD v[2] = {dd_from(0.0), dd_from(0.0)};
@@ -1111,20 +1114,21 @@ __global__ void __launch_bounds__(NT) k_pcg_init(Grid g, const T* __restrict__ r
       const T rr = rw - ap;
       const T zz = dinv[nb.c] * rr;
       r[nb.c] = rr; z[nb.c] = zz; p[nb.c] = zz;
-      v[0] = dd_add(v[0], dd_from((double)rw * (double)rw));
-      v[1] = dd_add(v[1], dd_from((double)rr * (double)zz));
+      const long pos = cell_pos(g, i, j);
+      sink_add<2>(out, v, 0, pos, (double)rw * (double)rw);
+      sink_add<2>(out, v, 1, pos, (double)rr * (double)zz);
     }
   }
-  publish_dd<2>(v, partials);
+  sink_publish<2>(out, v);
 }
-__global__ void __launch_bounds__(NT) k_fold_pcg_init(int nb, const double* __restrict__ partials,
-                                                      double* __restrict__ sc, int* __restrict__ fl) {
+__global__ void __launch_bounds__(NT) k_fold_pcg_init(int nb, Sink in, double* __restrict__ sc,
+                                                      int* __restrict__ fl) {
   if (fl[F_BRK]) {   // pending Krylov exit: mark the solve as skipped
     if (threadIdx.x == 0 && threadIdx.y == 0) fl[F_P_DONE] = 3;
     return;
   }
   double tot[2];
-  fold_dd<2>(partials, nb, tot);
+  fold_sink<2>(in, nb, tot);
   if (threadIdx.x == 0 && threadIdx.y == 0) {
     const double rn = sqrt(tot[0]);
     sc[S_P_RHSN] = rn;
@@ -1141,7 +1145,7 @@ template <typename T>
 __global__ void __launch_bounds__(NT) k_pcg_iter1(Grid g, int it, const T* __restrict__ z,
                                                   const T* __restrict__ p_old, T* __restrict__ p_new,
                                                   T* __restrict__ Ap, const T* __restrict__ cx,
-                                                  const T* __restrict__ cy, double* __restrict__ partials,
+                                                  const T* __restrict__ cy, Sink out,
                                                   const double* __restrict__ sc, const int* __restrict__ fl) {
   if (fl[F_P_DONE]) return;
   DD v[1] = {dd_from(0.0)};
@@ -1167,16 +1171,16 @@ __global__ void __launch_bounds__(NT) k_pcg_iter1(Grid g, int it, const T* __res
       const T ap = (cE + cW + cN + cS) * pc - (cE * pe + cW * pw + cN * pn + cS * ps);
       p_new[nb.c] = pc;
       Ap[nb.c] = ap;
-      v[0] = dd_add(v[0], dd_from((double)pc * (double)ap));
+      sink_add<1>(out, v, 0, cell_pos(g, i, j), (double)pc * (double)ap);
     }
   }
-  publish_dd<1>(v, partials);
+  sink_publish<1>(out, v);
 }
-__global__ void __launch_bounds__(NT) k_fold_pcg_iter1(int nb, int it, const double* __restrict__ partials,
-                                                       double* __restrict__ sc, int* __restrict__ fl) {
+__global__ void __launch_bounds__(NT) k_fold_pcg_iter1(int nb, int it, Sink in, double* __restrict__ sc,
+                                                       int* __restrict__ fl) {
   if (fl[F_P_DONE]) return;
   double tot[1];
-  fold_dd<1>(partials, nb, tot);
+  fold_sink<1>(in, nb, tot);
   if (threadIdx.x == 0 && threadIdx.y == 0) {
     const double pAp = tot[0];
     sc[S_P_PAP] = pAp;
@@ -1189,7 +1193,7 @@ __global__ void __launch_bounds__(NT) k_fold_pcg_iter1(int nb, int it, const dou
 template <typename T>
 __global__ void __launch_bounds__(NT) k_pcg_iter2(Grid g, T* __restrict__ phi, const T* __restrict__ p,
                                                   T* __restrict__ r, const T* __restrict__ Ap, T* __restrict__ z,
-                                                  const T* __restrict__ dinv, double* __restrict__ partials,
+                                                  const T* __restrict__ dinv, Sink out,
                                                   const double* __restrict__ sc, const int* __restrict__ fl) {
   if (fl[F_P_DONE]) return;
   DD v[2] = {dd_from(0.0), dd_from(0.0)};
@@ -1205,18 +1209,18 @@ __global__ void __launch_bounds__(NT) k_pcg_iter2(Grid g, T* __restrict__ phi, c
       const T zz = dinv[c] * rr;
       r[c] = rr;
       z[c] = zz;
-      v[0] = dd_add(v[0], dd_from((double)rr * (double)rr));
-      v[1] = dd_add(v[1], dd_from((double)rr * (double)zz));
+      const long pos = cell_pos(g, i, j);
+      sink_add<2>(out, v, 0, pos, (double)rr * (double)rr);
+      sink_add<2>(out, v, 1, pos, (double)rr * (double)zz);
     }
   }
-  publish_dd<2>(v, partials);
+  sink_publish<2>(out, v);
 }
-__global__ void __launch_bounds__(NT) k_fold_pcg_iter2(int nb, int it, int max_iter,
-                                                       const double* __restrict__ partials, double* __restrict__ sc,
-                                                       int* __restrict__ fl) {
+__global__ void __launch_bounds__(NT) k_fold_pcg_iter2(int nb, int it, int max_iter, Sink in,
+                                                       double* __restrict__ sc, int* __restrict__ fl) {
   if (fl[F_P_DONE]) return;
   double tot[2];
-  fold_dd<2>(partials, nb, tot);
+  fold_sink<2>(in, nb, tot);
   if (threadIdx.x == 0 && threadIdx.y == 0) {
     const double res = sqrt(tot[0]);
     sc[S_P_RES] = res;

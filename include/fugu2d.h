@@ -96,6 +96,22 @@ const char* fugu_last_error(void);
 /* Element count of a padded array of the given kind. */
 long long fugu_field_size(const fugu_ctx* ctx, int kind);
 
+/* Summation order of the reference-algorithm reductions: coupled_dot
+ * (src/mod_bicgstab2.f90:32-72) and the PCG's means and dots (src/mod_subsolvers.f90:91-233).
+ * FUGU_SUM_REFERENCE (the default of a new context): the reference's own order at one rank --
+ * each sum adds its terms one by one from zero, j outer, i inner, u faces then v faces then
+ * cells -- so results are bit-identical to the reference's arithmetic.  The terms are computed
+ * in parallel; the additions are one dependent chain per sum (about one FP64 add latency per
+ * owned point), so this order suits parity work and grids up to a few 10^5 cells.
+ * FUGU_SUM_TREE: per-workgroup double-double partials folded in a fixed order -- fast, and within
+ * a few units of the last place of the reference's sums, but not their exact roundings (the
+ * Krylov trajectories then drift from the reference's by up to ~1e-7 relative over a step).
+ * The chosen fast forms (FUGU_SOLVE_REFINED) and the adjoint always sum in the tree order.
+ * No counterpart in the reference (it has one order). */
+enum { FUGU_SUM_REFERENCE = 0, FUGU_SUM_TREE = 1 };
+int fugu_set_sum_order(fugu_ctx* ctx, int order);
+int fugu_get_sum_order(const fugu_ctx* ctx);
+
 /* ---- fine API: one entry per reference public procedure ----------------------------- */
 /* src/mod_mpi.f90:113-245 at one rank (periodic self wrap-copy; walls untouched). */
 int fugu_exchange_halos(fugu_ctx* ctx, double* a, int kind);

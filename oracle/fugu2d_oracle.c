@@ -6,7 +6,9 @@
  * branch structure follow the reference so that, compiled with
  * -ffp-contract=off, element-wise results are bit-identical to the reference
  * arithmetic.  Reductions are sequential (j outer, i inner) as in the
- * reference at one MPI rank.
+ * reference at one MPI rank by default (og_set_sum_mode(0)); og_set_sum_mode(1)
+ * switches the solver sums to a double-double accumulation, the restatement of the
+ * GPU library's fast tree order.
  *
  * Pinning (how this file is checked against something other than itself) is
  * listed per routine in tests/test_oracle_*.py.
@@ -33,12 +35,13 @@ size_t og_size_v(const og_grid* g) { return (size_t)(g->nx + 2 * g->nh) * (size_
 size_t og_size_p(const og_grid* g) { return (size_t)(g->nx + 2 * g->nh) * (size_t)(g->ny + 2 * g->nh); }
 size_t og_size_w(const og_grid* g) { return (size_t)(g->nx + 1 + 2 * g->nh) * (size_t)(g->ny + 1 + 2 * g->nh); }
 
-/* Summation of the solver's global reductions.  Mode 1 (default): compensated
- * double-double accumulation, whose final fl(hi + lo) is the correctly rounded sum of
- * the terms -- the same double whatever the order, hence the same as the GPU's tree
- * reductions.  Mode 0: plain sequential sums in the reference's loop order (j outer,
- * i inner), kept to measure how much the reference's own summation order moves results. */
-static int g_sum_mode = 1;
+/* Summation of the solver's global reductions (coupled_dot, the PCG's means and dots).
+ * Mode 0 (default): plain sequential sums `s = s + term` from zero in the reference's loop
+ * order (j outer, i inner; u, v, p), i.e. the reference's own roundings.  Mode 1: a
+ * double-double accumulation (TwoSum, then renormalisation), accurate to about
+ * n u^2 sum|x|, which nearly always rounds to the same double as the GPU's double-double
+ * tree of the same terms (not always: it is not a correctly rounded sum). */
+static int g_sum_mode = 0;
 void og_set_sum_mode(int mode) { g_sum_mode = mode; }
 int og_get_sum_mode(void) { return g_sum_mode; }
 typedef struct { double hi, lo; } acc_t;

@@ -179,8 +179,13 @@ class Grid:
 # Thin Python wrappers (allocate outputs, return them)
 
 def set_sum_mode(mode: int):
-    """1: compensated (correctly rounded) reductions; 0: the reference's plain sequential sums."""
+    """0 (default): the reference's plain sequential sums; 1: double-double accumulation (the
+    restatement of the GPU library's fast tree order, FUGU_SUM_TREE)."""
     lib().og_set_sum_mode(int(mode))
+
+
+def get_sum_mode() -> int:
+    return int(lib().og_get_sum_mode())
 
 
 def halo(g: Grid, a, kind):

@@ -21,9 +21,10 @@ GRIDS = [
 IDS = ["walls", "periodic_x", "periodic_xy"]
 
 
-def ctx_for(g: O.Grid, phys=None):
+def ctx_for(g: O.Grid, phys=None, sum_order=None):
     return abi.Context(g.nx, g.ny, g.lx, g.ly, g.per_x, g.per_y,
-                       params=abi.FuguParams(*(phys or (1, 1, 1, 1, 0, 0, 0, 1, 1e12, 0.5))))
+                       params=abi.FuguParams(*(phys or (1, 1, 1, 1, 0, 0, 0, 1, 1e12, 0.5))),
+                       sum_order=sum_order)
 
 
 def same(a, b, what=""):
@@ -38,6 +39,14 @@ def close(a, b, rtol, what=""):
     scale = max(np.abs(b).max(), 1e-300)
     err = np.abs(a - b).max() / scale
     assert err <= rtol, f"{what}: max rel diff {err:.3e} > {rtol:.1e}"
+
+
+def match(a, b, sum_order, rtol, what=""):
+    """Reference summation order: bit-identical.  Tree order: within rtol relative."""
+    if sum_order == abi.SUM_REFERENCE:
+        same(a, b, what)
+    else:
+        close(a, b, rtol, what)
 
 
 def grid(kw):
@@ -97,15 +106,36 @@ def test_matvec_bitwise(monkeypatch, kw, variant, tile):
 
 
 @pytest.mark.parametrize("kw", GRIDS, ids=IDS)
-def test_coupled_dot(kw):
+def test_coupled_dot(kw, sum_order):
+    """Reference order: the same double as the oracle's sequential loop (and as a plain Python
+    loop over the owned points in the reference's order).  Tree order: within 1e-13."""
     g = grid(kw)
-    ctx = ctx_for(g)
+    ctx = ctx_for(g, sum_order=sum_order)
     rng = np.random.default_rng(102)
     a = [rand(g, k, rng) for k in "uvp"]
     b = [rand(g, k, rng) for k in "uvp"]
     ref = O.coupled_dot(g, a, b)
     got = ctx.coupled_dot(a, b)
-    assert abs(got - ref) <= 1e-13 * (abs(ref) + np.sqrt(O.coupled_dot(g, a, a) * O.coupled_dot(g, b, b)))
+    if sum_order == abi.SUM_REFERENCE:
+        assert got == ref, (got, ref)
+        # independent restatement: Python floats added one by one, u then v then p, j outer
+        vol = g.dx * g.dy
+        s = 0.0
+        for k, kind in enumerate("uvp"):
+            A, Bk = a[k][interior(g, kind)], b[k][interior(g, kind)]
+            if kind == "u" and not g.per_x:
+                A, Bk = A[:, 1:-1], Bk[:, 1:-1]        # i = 1..nx-1
+            elif kind == "u":
+                A, Bk = A[:, 1:], Bk[:, 1:]            # i = 1..nx
+            if kind == "v" and not g.per_y:
+                A, Bk = A[1:-1, :], Bk[1:-1, :]        # j = 1..ny-1
+            elif kind == "v":
+                A, Bk = A[1:, :], Bk[1:, :]
+            for x, y in zip(A.ravel().tolist(), Bk.ravel().tolist()):
+                s = s + x * y * vol
+        assert got == s, (got, s)
+    else:
+        assert abs(got - ref) <= 1e-13 * (abs(ref) + np.sqrt(O.coupled_dot(g, a, a) * O.coupled_dot(g, b, b)))
 
 
 @pytest.mark.parametrize("kw", GRIDS, ids=IDS)
@@ -142,11 +172,11 @@ def test_jacobi_sweeps_bitwise(kw):
 
 
 @pytest.mark.parametrize("kw", GRIDS, ids=IDS)
-def test_poisson_pcg(kw):
-    """Truncated 150-iteration PCG: same iteration count, iterate within 1e-10 relative
-    (reductions are summed in a different order than the reference's sequential loop)."""
+def test_poisson_pcg(kw, sum_order):
+    """Truncated 150-iteration PCG: same iteration count; the iterate bit-identical in the
+    reference summation order, within 1e-10 relative in the tree order."""
     g = grid(kw)
-    ctx = ctx_for(g)
+    ctx = ctx_for(g, sum_order=sum_order)
     rng = np.random.default_rng(105)
     c, pr, _, _ = two_phase_case(g, rng, rho=(100.0, 1.0))
     if g.per_x or g.per_y:
@@ -160,8 +190,10 @@ def test_poisson_pcg(kw):
         it_o, res_o = O.poisson_pcg(g, phi_o, rhs, pr["rho_u"], pr["rho_v"], tol=tol, max_iter=150)
         it_g, res_g = ctx.poisson_pcg(phi_g, rhs, pr["rho_u"], pr["rho_v"], tol=tol, max_iter=150)
         assert it_g == it_o
+        if sum_order == abi.SUM_REFERENCE:
+            assert res_g == res_o, (res_g, res_o)
         assert abs(res_g - res_o) <= 1e-12 * max(res_o, 1.0)
-        close(phi_g, phi_o, 1e-10, "phi")
+        match(phi_g, phi_o, sum_order, 1e-10, "phi")
     # zero rhs early exit
     phi = np.ones(g.shape("p"))
     it, res = ctx.poisson_pcg(phi, g.zeros("p"), pr["rho_u"], pr["rho_v"])
@@ -169,9 +201,9 @@ def test_poisson_pcg(kw):
 
 
 @pytest.mark.parametrize("kw", GRIDS, ids=IDS)
-def test_pcd_precond(kw):
+def test_pcd_precond(kw, sum_order):
     g = grid(kw)
-    ctx = ctx_for(g)
+    ctx = ctx_for(g, sum_order=sum_order)
     rng = np.random.default_rng(106)
     c, pr, ua, va = two_phase_case(g, rng)
     if g.per_x or g.per_y:
@@ -184,17 +216,18 @@ def test_pcd_precond(kw):
     Rg = [g.zeros(k) for k in "uvp"]
     ctx.pcd(Z, Rg, pr, ua, va, 1e-3)
     for k in range(3):
-        close(Rg[k], Ro[k], 1e-10, f"R[{k}]")
+        match(Rg[k], Ro[k], sum_order, 1e-10, f"R[{k}]")
 
 
 @pytest.mark.parametrize("kw", GRIDS[:2], ids=IDS[:2])
 @pytest.mark.parametrize("order,consistent", [(1, False), (2, False), (2, True)],
                          ids=["bicgstab", "bicgstab2_ref", "bicgstab2_bc"])
-def test_krylov_solve(kw, order, consistent):
-    """bicgstab_solve / bicgstab2_solve: same iteration count and the solution within 1e-10
-    relative (u, v, p) of the oracle's (both reduce with correctly rounded sums)."""
+def test_krylov_solve(kw, order, consistent, sum_order):
+    """bicgstab_solve / bicgstab2_solve: same iteration count; the solution (u, v, p, ghosts
+    included) and the residual bit-identical in the reference summation order, within 1e-10
+    relative in the tree order."""
     g = grid(kw)
-    ctx = ctx_for(g)
+    ctx = ctx_for(g, sum_order=sum_order)
     rng = np.random.default_rng(107)
     c, pr, ua, va = two_phase_case(g, rng, rho=(10.0, 1.0), mu=(1.0, 0.1))
     B = [rand(g, "u", rng), rand(g, "v", rng), g.zeros("p")]
@@ -207,8 +240,10 @@ def test_krylov_solve(kw, order, consistent):
         it_g, res_g = ctx.krylov(Xg, B, pr, ua, va, 5e-3, tol=tol, max_iter=40, order=order, u_lid=lid,
                                  flags=abi.B2_CONSISTENT_BCS if consistent else 0)
         assert it_g == it_o, (it_g, it_o)
+        if sum_order == abi.SUM_REFERENCE:
+            assert res_g == res_o, (res_g, res_o)
         for k in range(3):
-            close(Xg[k], Xo[k], 1e-10, f"X[{k}]")
+            match(Xg[k], Xo[k], sum_order, 1e-10, f"X[{k}]")
 
 
 @pytest.mark.parametrize("kw", GRIDS, ids=IDS)
