@@ -5,6 +5,7 @@
 // 12-byte copy per Krylov iteration) and the scalars a caller asks for.
 #include "fugu2d.h"
 #include "fugu2d_kernels.hpp"
+#include "fugu2d_seqsum.hpp"
 #include "fugu2d_mg.hpp"
 #include "fugu2d_adjoint.hpp"
 
@@ -116,10 +117,14 @@ struct fugu_ctx {
   // means): true (default) = the reference's sequential order, bit for bit; false = the
   // double-double tree (fast).  The chosen fast forms and the adjoint always use the tree
   // (tree_depth > 0 inside them).  terms: the per-term buffer of the reference order,
-  // 5 chains x (owned u + v + p points), allocated on first use.
+  // 5 chains x (owned u + v + p points), with the chunk totals, value counts and chain sums of
+  // the sequential-sum pipeline; allocated on first use.
   bool sum_ref = true;
   int tree_depth = 0;
   double* terms = nullptr;
+  double* seq_ctot = nullptr;
+  int* seq_cnt = nullptr;
+  double* seq_tot = nullptr;
   double* h_sc = nullptr;
   int* h_fl = nullptr;
   size_t ntrip = 0;
@@ -276,13 +281,30 @@ bool seq_sums(const fugu_ctx* c) { return c->sum_ref && c->tree_depth == 0; }
 // Sink of a reduction with `nterms` terms per chain: the term buffer (reference order) or the
 // per-workgroup partials (tree).
 Sink sink(fugu_ctx* c, long nterms) {
-  if (!seq_sums(c)) return Sink{c->partials, 0};
+  if (!seq_sums(c)) return Sink{c->partials, 0, nullptr};
   if (!c->terms) {
     // at most 5 chains (the fused GCR(2) dots) x every owned point of a triplet
     const Grid& g = c->g;
-    c->terms = dalloc<double>(c, (size_t)5 * 3 * (size_t)g.nx * (size_t)g.ny);
+    const size_t nmax = (size_t)3 * g.nx * g.ny;
+    const size_t nch = (nmax + SQ_CH - 1) / SQ_CH;
+    c->terms = dalloc<double>(c, 5 * nmax);
+    c->seq_ctot = dalloc<double>(c, 5 * 3 * nch);
+    c->seq_cnt = dalloc<int>(c, 5 * nch);
+    c->seq_tot = dalloc<double>(c, 8);
   }
-  return Sink{c->terms, nterms};
+  return Sink{c->terms, nterms, c->seq_tot};
+}
+// The sequential-sum pipeline of nv chains after a pass-1 kernel wrote them into `sk`
+// (reference order only; nothing in the tree order).  skip: the pass-1 kernel's skip flag.
+void seq_reduce(fugu_ctx* c, const Sink& sk, int nv, Skip skip) {
+  if (!sk.n) return;
+  SeqArgs a;
+  a.t = sk.p; a.n = sk.n; a.nch = (int)((sk.n + SQ_CH - 1) / SQ_CH);
+  a.ctot = c->seq_ctot; a.cnt = c->seq_cnt; a.tot = c->seq_tot;
+  const dim3 gc((unsigned)a.nch, (unsigned)nv);
+  LAUNCH(k_seq_chunks, gc, dim3(SQ_NT), c->s, a, skip, (const int*)c->fl);
+  LAUNCH(k_seq_compact, gc, dim3(SQ_NT), c->s, a, skip, (const int*)c->fl);
+  LAUNCH(k_seq_fold, dim3((unsigned)nv), dim3(SQ_NT), c->s, a, skip, (const int*)c->fl);
 }
 long cdot_terms(const Grid& g) {
   const long iu = g.per_x ? g.nx : g.nx - 1, jv = g.per_y ? g.ny : g.ny - 1;
@@ -396,14 +418,17 @@ void cdot(fugu_ctx* c, int np, const T* const* a, const T* const* b, int phase, 
   switch (np) {
     case 1:
       LAUNCH((k_cdot<T, 1>), gr, bl, c->s, c->g, d, sk, fl);
+      seq_reduce(c, sk, 1, Skip{F_BRK, 1});
       LAUNCH((k_fold_krylov<1>), dim3(1), dim3(NT), c->s, nb, phase, k, max_iter, sk, c->sc, c->fl);
       break;
     case 2:
       LAUNCH((k_cdot<T, 2>), gr, bl, c->s, c->g, d, sk, fl);
+      seq_reduce(c, sk, 2, Skip{F_BRK, 1});
       LAUNCH((k_fold_krylov<2>), dim3(1), dim3(NT), c->s, nb, phase, k, max_iter, sk, c->sc, c->fl);
       break;
     case 5:
       LAUNCH((k_cdot<T, 5>), gr, bl, c->s, c->g, d, sk, fl);
+      seq_reduce(c, sk, 5, Skip{F_BRK, 1});
       LAUNCH((k_fold_krylov<5>), dim3(1), dim3(NT), c->s, nb, phase, k, max_iter, sk, c->sc, c->fl);
       break;
     default: throw Error("cdot: unsupported pair count");
@@ -416,6 +441,7 @@ void cell_mean(fugu_ctx* c, const T* a, int slot, int skip_mode) {
   const dim3 gr = grid_cells_red(c->g);
   const Sink sk = sink(c, cell_terms(c->g));
   LAUNCH((k_pcg_sum<T>), gr, dim3(TX, TY), c->s, c->g, a, sk, skip_mode, (const int*)c->fl);
+  seq_reduce(c, sk, 1, skip_mode == 1 ? Skip{F_BRK, 1} : skip_mode == 2 ? Skip{F_P_DONE, 2} : Skip{-1, 0});
   LAUNCH(k_fold_sum, dim3(1), dim3(NT), c->s, c->g, (int)nblocks(gr), slot, skip_mode, sk, c->sc,
          (const int*)c->fl);
 }
@@ -514,16 +540,19 @@ void pcg(fugu_ctx* c, Work<T>& w, T* phi, const T* rhs, bool phi_zero, double to
   if (walls_both) cell_mean<T>(c, rhs, (int)S_P_MEAN, 1);
   LAUNCH((k_pcg_init<T>), gc, bl, c->s, g, rhs, phi, w.cx, w.cy, w.dinv, w.pr, w.pz, w.pp0, walls_both,
          phi_zero ? 1 : 0, sk, sc, fl);
+  seq_reduce(c, sk, 2, Skip{F_BRK, 1});
   LAUNCH(k_fold_pcg_init, dim3(1), b1, c->s, nb, sk, c->sc, c->fl);
   T* P[2] = {w.pp0, w.pp1};
   for (int it = 1; it <= max_iter; ++it) {
     // z, p_old, c_x, c_y read; p_new, Ap written
     TLAUNCH(c, sizeof(T) == 8 ? "pcg_iter1_f64" : "pcg_iter1_f32", 6.0 * sizeof(T) * g.nx * g.ny, (k_pcg_iter1<T>), gc, bl, c->s, g, it,
             w.pz, P[(it - 1) & 1], P[it & 1], w.Ap, w.cx, w.cy, sk, sc, fl);
+    seq_reduce(c, sk, 1, Skip{F_P_DONE, 1});
     LAUNCH(k_fold_pcg_iter1, dim3(1), b1, c->s, nb, it, sk, c->sc, c->fl);
     // phi, p, r, Ap, D^-1 read; phi, r, z written
     TLAUNCH(c, sizeof(T) == 8 ? "pcg_iter2_f64" : "pcg_iter2_f32", 8.0 * sizeof(T) * g.nx * g.ny, (k_pcg_iter2<T>), gc, bl, c->s, g, phi,
             P[it & 1], w.pr, w.Ap, w.pz, w.dinv, sk, sc, fl);
+    seq_reduce(c, sk, 2, Skip{F_P_DONE, 1});
     LAUNCH(k_fold_pcg_iter2, dim3(1), b1, c->s, nb, it, max_iter, sk, c->sc, c->fl);
   }
   (void)tol;
@@ -909,7 +938,7 @@ const T* mg_pressure_compact(fugu_ctx* c, Work<T>& w, const T* Zp) {
       nb = (int)std::min<unsigned>(nblk((size_t)L0.d.nx * L0.d.ny), (unsigned)RED_MAXB);
       LAUNCH((k_mg_sum<T>), dim3((unsigned)nb), dim3(256), c->s, L0.d, res, c->partials);
     }
-    LAUNCH(k_fold_sum, dim3(1), dim3(NT), c->s, g, nb, (int)S_P_MEAN, 0, Sink{c->partials, 0}, c->sc,
+    LAUNCH(k_fold_sum, dim3(1), dim3(NT), c->s, g, nb, (int)S_P_MEAN, 0, Sink{c->partials, 0, nullptr}, c->sc,
            (const int*)c->fl);
   }
   return res;

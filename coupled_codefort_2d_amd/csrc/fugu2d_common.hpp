@@ -158,7 +158,7 @@ __device__ inline void block_max(double (&v)[NV], double (*lds)[NV]) {
 // to the same double whatever the summation order.  It is not a correctly rounded sum: under
 // heavy cancellation, or when the exact sum lies within that error of a rounding boundary,
 // two orders can give neighbouring doubles.  Needs -ffp-contract=off (no FMA inside TwoSum).
-// The reference's own order (sequential, j outer, i inner) is the Sink / fold_seq path below.
+// The reference's own order (sequential, j outer, i inner) is the Sink path below.
 struct DD { double hi, lo; };
 
 __host__ __device__ inline DD dd_add(DD a, DD b) {
@@ -284,12 +284,14 @@ __device__ inline void fold_dd(const double* partials, int nb, double (&tot)[NV]
 //
 // Sink: where a pass-1 kernel sends its terms.  n == 0: tree order -- per-workgroup
 // double-double partials at p (publish_dd / fold_dd).  n > 0: reference order -- term k of chain
-// q (k = its position in the reference's loop) is stored at p[q * n + k], and the fold adds the
-// n terms of each chain one by one (fold_seq).  The terms themselves are computed in parallel
-// with the reference's operation order, so the fold is the only serial work.
+// q (k = its position in the reference's loop) is stored at p[q * n + k]; the sequential-sum
+// pipeline (fugu2d_seqsum.hpp) then leaves chain q's sum, bit-identical to the reference's
+// one-by-one loop, at tot[q].  The terms are computed in parallel with the reference's operation
+// order.
 struct Sink {
   double* p;
   long n;
+  double* tot;
 };
 
 template <int NV>
@@ -302,55 +304,15 @@ __device__ inline void sink_publish(const Sink& s, DD (&v)[NV]) {
   if (!s.n) publish_dd<NV>(v, s.p);   // s.n is uniform: every thread takes the same branch
 }
 
-// Reference-order fold of NV chains (one workgroup of >= 2 wavefronts): the second and later
-// wavefronts stage the next chunk of every chain in LDS while lane q < NV of the first
-// wavefront adds chunk c of chain q into its running sum -- a dependent chain of FP64 adds, one
-// per term, in the reference's order.  Independent chains share the same instructions.
-template <int NV>
-__device__ inline void fold_seq(const double* __restrict__ p, long n, double (&tot)[NV]) {
-  constexpr int CH = (2048 / NV) & ~63;   // terms per chain per chunk
-  constexpr int LD = CH + 1;              // row stride: chains on different LDS banks
-  __shared__ double buf[2][NV * LD];
-  __shared__ double out[NV];
-  const int tid = threadIdx.x + threadIdx.y * blockDim.x;
-  const int nthr = blockDim.x * blockDim.y;
-  const long nch = (n + CH - 1) / CH;
-  auto stage = [&](long ch, int b) {
-    const long base = ch * CH;
-    for (int k = tid - 64; k < NV * CH; k += nthr - 64) {
-      const int q = k / CH, m = k - q * CH;
-      const long idx = base + m;
-      buf[b][q * LD + m] = idx < n ? p[(long)q * n + idx] : 0.0;
-    }
-  };
-  double s = 0.0;
-  if (tid >= 64 && nch > 0) stage(0, 0);
-  __syncthreads();
-  for (long ch = 0; ch < nch; ++ch) {
-    if (tid >= 64 && ch + 1 < nch) stage(ch + 1, (int)((ch + 1) & 1));
-    if (tid < NV) {
-      const double* b = &buf[ch & 1][tid * LD];
-      const long rem = n - ch * CH;
-      if (rem >= CH) {
-#pragma unroll 32
-        for (int m = 0; m < CH; ++m) s = s + b[m];
-      } else {
-        for (int m = 0; m < (int)rem; ++m) s = s + b[m];
-      }
-    }
-    __syncthreads();
-  }
-  if (tid < NV) out[tid] = s;
-  __syncthreads();
-#pragma unroll
-  for (int q = 0; q < NV; ++q) tot[q] = out[q];
-}
-
-// Fold of a Sink: nb per-workgroup partials (tree) or the reference's sequential sums.
+// Fold of a Sink: nb per-workgroup partials (tree) or the reference-order sums.
 template <int NV>
 __device__ inline void fold_sink(const Sink& s, int nb, double (&tot)[NV]) {
-  if (s.n) fold_seq<NV>(s.p, s.n, tot);
-  else fold_dd<NV>(s.p, nb, tot);
+  if (s.n) {
+#pragma unroll
+    for (int q = 0; q < NV; ++q) tot[q] = s.tot[q];
+  } else {
+    fold_dd<NV>(s.p, nb, tot);
+  }
 }
 
 template <int NV>
