@@ -311,6 +311,19 @@ long cdot_terms(const Grid& g) {
   return (long)g.ny * iu + jv * (long)g.nx + (long)g.nx * g.ny;
 }
 long cell_terms(const Grid& g) { return (long)g.nx * g.ny; }
+// Preconditioner switches of one solve (multigrid pressure / velocity blocks, the viscous Schur
+// term, V-cycles per application): set for the solve, restored to the reference PCD when the
+// scope ends, also when the solve throws, so later drop-in calls never inherit them.
+struct PrecondScope {
+  fugu_ctx* c;
+  explicit PrecondScope(fugu_ctx* ctx) : c(ctx) { reset(c); }
+  ~PrecondScope() { reset(c); }
+  static void reset(fugu_ctx* c) {
+    c->use_mg = false; c->use_vmg = false; c->pcd_visc = 0.0; c->mg_cycles = 1;
+  }
+  PrecondScope(const PrecondScope&) = delete;
+  PrecondScope& operator=(const PrecondScope&) = delete;
+};
 // The chosen fast forms and the adjoint sum in the tree order (not parity-bound).
 struct TreeSums {
   fugu_ctx* c;
@@ -1734,6 +1747,7 @@ int step_impl(fugu_ctx* c, double dt, int step_idx, const fugu_step_opts* o, fug
   int iters = 0, outer = 0;
   double res = 0.0;
   c->pcg_calls = 0;
+  PrecondScope pscope(c);
   c->use_mg = (o->solver_flags & FUGU_PRECOND_MG) != 0;
   c->use_vmg = (o->solver_flags & FUGU_PRECOND_VMG) != 0;
   c->pcd_visc = (o->solver_flags & FUGU_PRECOND_VISC) ? c->pcd_visc_coef : 0.0;
@@ -1772,8 +1786,6 @@ int step_impl(fugu_ctx* c, double dt, int step_idx, const fugu_step_opts* o, fug
   HIPCHK(hipEventRecord(c->ev[3], c->s));
   sync(c);
   kt_resolve(c);
-  c->use_mg = false; c->use_vmg = false;
-  c->pcd_visc = 0.0;
   if (st) {
     float ms_solve = 0.f, ms_step = 0.f;
     HIPCHK(hipEventElapsedTime(&ms_solve, c->ev[1], c->ev[2]));
@@ -1788,7 +1800,11 @@ int step_impl(fugu_ctx* c, double dt, int step_idx, const fugu_step_opts* o, fug
   }
   int rc = 0;
   if (!(res == res)) rc |= FUGU_W_NONFINITE;
-  if (!(o->solver_flags & FUGU_SOLVE_REFINED) && iters >= o->max_iter) rc |= FUGU_W_MAX_ITER;
+  if (o->solver_flags & FUGU_SOLVE_REFINED) {
+    if (res > o->tol) rc |= FUGU_W_MAX_ITER;   // stopped at max_iter FGMRES steps short of tol
+  } else if (iters >= o->max_iter) {
+    rc |= FUGU_W_MAX_ITER;
+  }
   return rc;
 }
 
@@ -2325,6 +2341,7 @@ int fugu_solve_poisson_pressure(fugu_ctx* c, double* phi, const double* rhs, con
                                 const double* rho_v, double tol, int max_iter, int* iter_out, double* res_out) {
   NEED(c);
   return guarded([&]() {
+    PrecondScope pscope(c);   // the reference solvers
     const Grid& g = c->g;
     Work<double>& w = c->wd;
     if (tol <= 0) tol = 1.0e-5;
@@ -2433,6 +2450,7 @@ int fugu_apply_pcd_precond(fugu_ctx* c, const double* zu, const double* zv, cons
                            double dt, const double* pu, const double* pv) {
   NEED(c);
   return guarded([&]() {
+    PrecondScope pscope(c);   // the reference PCD (Jacobi-PCG pressure, Jacobi velocity blocks)
     const Grid& g = c->g;
     Work<double>& w = c->wd;
     upload_props(c, uadv, vadv, nullptr, rho_u, rho_v, kap, zeta, eta, pu, pv, nullptr, nullptr);
@@ -2455,6 +2473,7 @@ static int krylov_entry(fugu_ctx* c, int order, double* Xu, double* Xv, double* 
                         const double* mu, const double* mv, unsigned flags) {
   NEED(c);
   return guarded([&]() {
+    PrecondScope pscope(c);   // the reference solvers
     const Grid& g = c->g;
     Work<double>& w = c->wd;
     if (tol <= 0) tol = 1.0e-5;
@@ -2475,7 +2494,8 @@ static int krylov_entry(fugu_ctx* c, int order, double* Xu, double* Xv, double* 
     sync(c);
     if (iter_out) *iter_out = it;
     if (res_out) *res_out = res;
-    return (res == res) ? 0 : FUGU_W_NONFINITE;
+    if (!(res == res)) return (int)FUGU_W_NONFINITE;
+    return it >= max_iter ? (int)FUGU_W_MAX_ITER : 0;
   });
 }
 
@@ -2742,6 +2762,7 @@ int fugu_apply_pcd_transpose(fugu_ctx* c, const double* yu, const double* yv, co
   NEED(c);
   return guarded([&]() {
     TreeSums tree(c);
+    PrecondScope pscope(c);
     const Grid& g = c->g;
     Work<double>& w = c->wd;
     upload_props(c, uadv, vadv, nullptr, rho_u, rho_v, kap, zeta, eta, nullptr, nullptr, nullptr, nullptr);
