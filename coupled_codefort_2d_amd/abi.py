@@ -81,7 +81,7 @@ EXPORTS = [
     "fugu_advance_coupled_step", "fugu_state_upload", "fugu_state_download", "fugu_field_download",
     "fugu_state_init_properties", "fugu_step", "fugu_kernel_timing", "fugu_kernel_time",
     "fugu_solve_pressure_mg", "fugu_matvec_transpose", "fugu_apply_pcd_transpose", "fugu_adjoint_solve",
-    "fugu_cell_properties", "fugu_adjoint_gradient",
+    "fugu_cell_properties", "fugu_adjoint_gradient", "fugu_coupled_dot_noalloc",
 ]
 
 _lib = None
@@ -118,6 +118,8 @@ def _declare(L):
     s("fugu_apply_velocity_bcs", [P, _D, _D, _D])
     s("fugu_matvec_coupled", [P] + [_D] * 14 + [C.c_double, C.c_int] + [_D] * 4)
     s("fugu_coupled_dot", [P] + [_D] * 7)
+    # device pointers (integers) for the allocation-free variant
+    s("fugu_coupled_dot_noalloc", [C.POINTER(FuguGrid)] + [C.c_void_p] * 7 + [C.c_int, C.c_void_p])
     s("fugu_solve_poisson_pressure", [P, _D, _D, _D, _D, C.c_double, C.c_int, _I, _D])
     s("fugu_solve_helmholtz_u", [P] + [_D] * 6 + [C.c_double, C.c_int, _D])
     s("fugu_solve_helmholtz_v", [P] + [_D] * 6 + [C.c_double, C.c_int, _D])
@@ -147,6 +149,15 @@ def _declare(L):
     s("fugu_cell_properties", [P] + [_D] * 8)
     s("fugu_adjoint_gradient", [P, C.POINTER(FuguAdjointOpts), _D, _D, C.c_double, _D, _D, _D, _D, _D, _D,
                                 C.POINTER(FuguAdjointStats)])
+
+
+def coupled_dot_noalloc(grid: FuguGrid, a_dev, b_dev, out_dev, sum_order=SUM_REFERENCE, stream=None):
+    """fugu_coupled_dot_noalloc on device addresses (ints): a_dev, b_dev are (u, v, p) triples,
+    out_dev one double; asynchronous on `stream` (an int hipStream_t or None)."""
+    rc = load().fugu_coupled_dot_noalloc(C.byref(grid), *[C.c_void_p(x) for x in a_dev],
+                                         *[C.c_void_p(x) for x in b_dev], C.c_void_p(out_dev), int(sum_order),
+                                         C.c_void_p(stream) if stream else None)
+    return check(rc, "fugu_coupled_dot_noalloc")
 
 
 def ptr(a):

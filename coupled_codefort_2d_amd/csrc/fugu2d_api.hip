@@ -16,6 +16,7 @@
 #include <cstring>
 #include <type_traits>
 #include <map>
+#include <mutex>
 #include <stdexcept>
 #include <string>
 #include <utility>
@@ -2100,6 +2101,43 @@ int guarded(F&& f) {
 }  // namespace
 
 // =====================================================================================
+// Allocation-free coupled dot (fugu_coupled_dot_noalloc): its scratch is this file-scope device
+// array -- the reference order's terms, chunk totals, value counts and sum, or the tree order's
+// per-workgroup partials -- held by every GPU the library runs on until the process ends.  Host
+// code never uses the array's name as an address: each device's address comes from
+// hipGetSymbolAddress on the first call on that device (that device current) and is kept per
+// device.  Calls that overlap on one GPU share the array and race.
+constexpr long NOALLOC_MAX_TERMS = 1L << 20;   // owned u + v + p points (reference order)
+constexpr long NOALLOC_NCH = NOALLOC_MAX_TERMS / SQ_CH;
+constexpr long NOALLOC_PARTIALS = 2L * RED_MAXB;       // one double-double per workgroup
+constexpr long NOALLOC_CTOT = 3 * NOALLOC_NCH;
+constexpr long NOALLOC_CNT = NOALLOC_NCH / 2;          // NOALLOC_NCH ints
+constexpr long NOALLOC_TOT = 8;
+constexpr long NOALLOC_DOUBLES =
+    NOALLOC_PARTIALS + NOALLOC_CTOT + NOALLOC_CNT + NOALLOC_TOT + NOALLOC_MAX_TERMS;   // 8.4 MB
+static __device__ __align__(256) double scratch[NOALLOC_DOUBLES];
+
+namespace {
+constexpr int NOALLOC_MAX_DEV = 64;
+std::mutex g_scratch_mu;
+double* g_scratch_ptr[NOALLOC_MAX_DEV] = {};
+
+double* scratch_on_current_device() {
+  int dev = -1;
+  HIPCHK(hipGetDevice(&dev));
+  if (dev < 0 || dev >= NOALLOC_MAX_DEV) throw Error("fugu_coupled_dot_noalloc: device index out of range");
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* ptr = nullptr;
+    HIPCHK(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+    if (!ptr) throw Error("fugu_coupled_dot_noalloc: no device address for the scratch array");
+    g_scratch_ptr[dev] = static_cast<double*>(ptr);
+  }
+  return g_scratch_ptr[dev];
+}
+}  // namespace
+
+// =====================================================================================
 // C ABI
 extern "C" {
 
@@ -2333,6 +2371,55 @@ int fugu_coupled_dot(fugu_ctx* c, const double* u1, const double* v1, const doub
     cdot1<double>(c, w.z, w.q, PH_STORE);
     read_slots(c);
     if (dot_out) *dot_out = c->h_sc[S_DOT0];
+    return 0;
+  });
+}
+
+int fugu_coupled_dot_noalloc(const fugu_grid* grid, const double* u1, const double* v1, const double* p1,
+                             const double* u2, const double* v2, const double* p2, double* dot_out,
+                             int sum_order, void* stream) {
+  if (!grid || !u1 || !v1 || !p1 || !u2 || !v2 || !p2 || !dot_out) { g_err = "null argument"; return -2; }
+  if (grid->nh != 2) { g_err = "nh must be 2 (src/main.f90:22)"; return -2; }
+  if (grid->nx < 4 || grid->ny < 4) { g_err = "grid too small (nx, ny >= 4)"; return -2; }
+  if (sum_order != FUGU_SUM_REFERENCE && sum_order != FUGU_SUM_TREE) {
+    g_err = "fugu_coupled_dot_noalloc: sum_order must be FUGU_SUM_REFERENCE (0) or FUGU_SUM_TREE (1)";
+    return -3;
+  }
+  return guarded([&]() {
+    const Grid g = make_grid(grid->nx, grid->ny, grid->nh, grid->lx / grid->nx, grid->ly / grid->ny,
+                             grid->periodic_x ? 1 : 0, grid->periodic_y ? 1 : 0);
+    const long nterms = cdot_terms(g);
+    if (sum_order == FUGU_SUM_REFERENCE && nterms > NOALLOC_MAX_TERMS)
+      throw Error("fugu_coupled_dot_noalloc: " + std::to_string(nterms) + " owned points exceed the reference-order "
+                  "scratch limit of " + std::to_string(NOALLOC_MAX_TERMS) + " (use fugu_coupled_dot)");
+    double* base = scratch_on_current_device();
+    double* partials = base;
+    double* ctot = partials + NOALLOC_PARTIALS;
+    int* cnt = reinterpret_cast<int*>(ctot + NOALLOC_CTOT);
+    double* tot = ctot + NOALLOC_CTOT + NOALLOC_CNT;
+    double* terms = tot + NOALLOC_TOT;
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    DotSplit d;
+    d.a[0] = u1; d.a[1] = v1; d.a[2] = p1;
+    d.b[0] = u2; d.b[1] = v2; d.b[2] = p2;
+    const dim3 gr = grid_union_red(g);
+    // every value the launches read is written by an earlier launch of this call: nothing to clear
+    if (sum_order == FUGU_SUM_REFERENCE) {
+      const Sink sk{terms, nterms, tot};
+      LAUNCH(k_cdot_split, gr, dim3(TX, TY), st, g, d, sk);
+      SeqArgs a;
+      a.t = terms; a.n = nterms; a.nch = (int)((nterms + SQ_CH - 1) / SQ_CH);
+      a.ctot = ctot; a.cnt = cnt; a.tot = tot;
+      const dim3 gc((unsigned)a.nch, 1u);
+      LAUNCH(k_seq_chunks, gc, dim3(SQ_NT), st, a, Skip{-1, 0}, (const int*)nullptr);
+      LAUNCH(k_seq_compact, gc, dim3(SQ_NT), st, a, Skip{-1, 0}, (const int*)nullptr);
+      LAUNCH(k_seq_fold, dim3(1), dim3(SQ_NT), st, a, Skip{-1, 0}, (const int*)nullptr);
+      LAUNCH(k_fold_dot_out, dim3(1), dim3(NT), st, 0, sk, dot_out);
+    } else {
+      const Sink sk{partials, 0, nullptr};
+      LAUNCH(k_cdot_split, gr, dim3(TX, TY), st, g, d, sk);
+      LAUNCH(k_fold_dot_out, dim3(1), dim3(NT), st, (int)nblocks(gr), sk, dot_out);
+    }
     return 0;
   });
 }

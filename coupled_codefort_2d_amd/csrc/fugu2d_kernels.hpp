@@ -591,6 +591,45 @@ __global__ void __launch_bounds__(NT) k_fold_krylov(int nb, int phase, int k, in
   }
 }
 
+// One coupled dot over separate padded arrays (a: u1, v1, p1; b: u2, v2, p2) -- the same owned
+// sets, term positions and arithmetic as k_cdot for arrays that are not one triplet allocation.
+struct DotSplit {
+  const double* a[3];
+  const double* b[3];
+};
+__global__ void __launch_bounds__(NT) k_cdot_split(Grid g, DotSplit d, Sink out) {
+  const int nx = g.nx, ny = g.ny, nh = g.nh;
+  const int i_u_max = g.per_x ? nx : nx - 1;
+  const int j_v_max = g.per_y ? ny : ny - 1;
+  const double vol = g.vol;
+  const long nU = (long)ny * i_u_max, nV = (long)j_v_max * nx;
+  DD v[1] = {dd_from(0.0)};
+  const int i = blockIdx.x * TX + (int)threadIdx.x - nh;
+  const int tiles = (g.rw + TY - 1) / TY;
+  for (int t = blockIdx.y; t < tiles; t += gridDim.y) {
+    const int j = t * TY + (int)threadIdx.y - nh;
+    if (j >= 1 && j <= ny && i >= 1 && i <= i_u_max) {
+      const size_t o = IU(g, i, j);
+      sink_add<1>(out, v, 0, (long)(j - 1) * i_u_max + (i - 1), d.a[0][o] * d.b[0][o] * vol);
+    }
+    if (j >= 1 && j <= j_v_max && i >= 1 && i <= nx) {
+      const size_t o = IV(g, i, j);
+      sink_add<1>(out, v, 0, nU + (long)(j - 1) * nx + (i - 1), d.a[1][o] * d.b[1][o] * vol);
+    }
+    if (j >= 1 && j <= ny && i >= 1 && i <= nx) {
+      const size_t o = IP(g, i, j);
+      sink_add<1>(out, v, 0, nU + nV + (long)(j - 1) * nx + (i - 1), d.a[2][o] * d.b[2][o] * vol);
+    }
+  }
+  sink_publish<1>(out, v);
+}
+// Fold of k_cdot_split into *out (device memory).
+__global__ void __launch_bounds__(NT) k_fold_dot_out(int nb, Sink in, double* __restrict__ out) {
+  double tot[1];
+  fold_sink<1>(in, nb, tot);
+  if (threadIdx.x == 0 && threadIdx.y == 0) out[0] = tot[0];
+}
+
 // Whole-array triplet updates (Fortran whole-array expressions, ghosts included), each
 // with the reference's association order.  Coefficients are read from device slots;
 // `sgn` flips a coefficient (for x - c*y written as x + (-c)*y we keep x - c*y).

@@ -131,6 +131,18 @@ int fugu_matvec_coupled(fugu_ctx* ctx, double* X_u, double* X_v, double* X_p,
 /* src/mod_bicgstab2.f90:32-72 */
 int fugu_coupled_dot(fugu_ctx* ctx, const double* u1, const double* v1, const double* p1,
                      const double* u2, const double* v2, const double* p2, double* dot_out);
+/* coupled_dot without a context and without allocating: no counterpart in the reference; for
+ * callers that cannot allocate device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc).  The six arrays are device memory of the current device in the
+ * padded layout of `grid`; the sum (the same double as fugu_coupled_dot in the same order) is
+ * written to the device pointer dot_out, asynchronously on `stream` (a hipStream_t, NULL = the
+ * null stream): the caller synchronises.  The scratch is a fixed device array of the library
+ * (8.4 MB per GPU, held until the process ends), shared by every call on that GPU: calls that
+ * overlap on one GPU race.  sum_order FUGU_SUM_REFERENCE: at most 2^20 owned u + v + p points
+ * (e.g. 590 x 590 cells), larger grids return an error; FUGU_SUM_TREE: any size. */
+int fugu_coupled_dot_noalloc(const fugu_grid* grid, const double* u1, const double* v1, const double* p1,
+                             const double* u2, const double* v2, const double* p2, double* dot_out,
+                             int sum_order, void* stream);
 /* src/mod_subsolvers.f90:32-246 (tol <= 0 / max_iter <= 0 select the defaults 1e-5 / 150) */
 int fugu_solve_poisson_pressure(fugu_ctx* ctx, double* phi, const double* rhs,
                                 const double* rho_u, const double* rho_v,
