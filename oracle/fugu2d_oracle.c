@@ -1185,9 +1185,9 @@ void og_advance_step(const og_grid* g, const og_phys* ph, const og_step_opts* o,
   }
 
   if (o->use_bicgstab2)
-    og_bicgstab2(g, u, v, p, Bu, Bv, Bp, uadv, vadv, rho_c, rho_u, rho_v, kappa, zeta, eta, dt_eff,
-                 o->tol, o->max_iter, &it, &res, 1, u_lid, penu, penv,
-                 o->use_amp ? mu_ : NULL, o->use_amp ? mv_ : NULL);
+    og_bicgstab2_ex(g, u, v, p, Bu, Bv, Bp, uadv, vadv, rho_c, rho_u, rho_v, kappa, zeta, eta, dt_eff,
+                    o->tol, o->max_iter, &it, &res, 1, u_lid, penu, penv,
+                    o->use_amp ? mu_ : NULL, o->use_amp ? mv_ : NULL, o->consistent);
   else
     og_bicgstab(g, u, v, p, Bu, Bv, Bp, uadv, vadv, rho_c, rho_u, rho_v, kappa, zeta, eta, dt_eff,
                 o->tol, o->max_iter, &it, &res, 1, u_lid, penu, penv,

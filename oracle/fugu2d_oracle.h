@@ -160,6 +160,7 @@ typedef struct {
   int max_iter;
   int use_amp;
   double u_solid, v_solid;
+  int consistent;   /* BiCGStab(2) with BCs on r and s (not in the reference; 0 = the reference) */
 } og_step_opts;
 
 void og_advance_step(const og_grid* g, const og_phys* ph, const og_step_opts* o,

@@ -39,7 +39,7 @@ class OgPhys(C.Structure):
 class OgStepOpts(C.Structure):
     _fields_ = [("use_bicgstab2", C.c_int), ("has_lid", C.c_int), ("u_lid", C.c_double),
                 ("tol", C.c_double), ("max_iter", C.c_int), ("use_amp", C.c_int),
-                ("u_solid", C.c_double), ("v_solid", C.c_double)]
+                ("u_solid", C.c_double), ("v_solid", C.c_double), ("consistent", C.c_int)]
 
 
 def build(force: bool = False) -> str:
@@ -333,9 +333,9 @@ class State:
 
 
 def advance_step(st: State, phys: OgPhys, dt, step_idx, use_bicgstab2=True, u_lid=None,
-                 tol=1e-6, max_iter=500, use_amp=False, phi_s=None, u_solid=0.0, v_solid=0.0):
+                 tol=1e-6, max_iter=500, use_amp=False, phi_s=None, u_solid=0.0, v_solid=0.0, consistent=False):
     o = OgStepOpts(int(use_bicgstab2), int(u_lid is not None), float(u_lid or 0.0), tol, max_iter,
-                   int(use_amp), u_solid, v_solid)
+                   int(use_amp), u_solid, v_solid, int(consistent))
     it = C.c_int(0)
     res = C.c_double(0.0)
     psu, psv = phi_s if phi_s is not None else (None, None)

@@ -19,9 +19,11 @@ sys.path.insert(0, os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
 import bench  # noqa: E402  (workload initialiser)
 
 
-def oracle_true_residual(g, ph, st_before, u, v, p, dt, step_idx):
+def oracle_true_residual(g, ph, st_before, u, v, p, dt, step_idx, u_lid=None):
     """||B - A X|| / ||B|| with the oracle's operator for the system of step `step_idx`,
-    rebuilt from the state before the step (VOF, mixture, CSF as the reference does)."""
+    rebuilt from the state before the step (VOF, mixture, CSF as the reference does); u_lid:
+    the lid's right-hand-side term 2 (zeta - eta) u_lid / dy^2 on the top row of u
+    (src/mod_solver_coupled.f90:282-287), X with homogeneous BCs."""
     s = st_before.copy()
     # run the oracle's step machinery up to the solve: reuse advance_step with max_iter=0 is
     # not available, so rebuild B and the operator pieces explicitly
@@ -45,6 +47,10 @@ def oracle_true_residual(g, ph, st_before, u, v, p, dt, step_idx):
     else:
         B = [pr["rho_u"] / dt * un + pr["rho_u"] * ph.gx + fu, pr["rho_v"] / dt * vn + pr["rho_v"] * ph.gy + fv,
              g.zeros("p")]
+    if u_lid is not None and not g.per_y:
+        nh, ny = g.nh, g.ny
+        i_u = slice(nh + 1, nh + g.nx + (1 if g.per_x else 0))       # i = 1 .. i_u_max
+        B[0][ny - 1 + nh, i_u] += (2.0 * (pr["zeta"][ny + nh, i_u] - pr["eta"][ny + nh, i_u]) / (g.dy * g.dy)) * u_lid
     O.mask_owned(g, B)
     X = [u.copy(), v.copy(), p.copy()]
     O.velocity_bcs(g, X[0], X[1])
