@@ -556,6 +556,13 @@ void og_helmholtz_v(const og_grid* g, double* v_sol, const double* rhs_v,
 }
 
 /* src/mod_pcd_precond.f90:14-104 */
+/* The PCD's inner PCG stopping rule: the reference's 1e-5 / 150 (mod_pcd_precond.f90:50).  A
+ * pin may tighten it so that the preconditioner becomes a fixed linear map (to rounding) and the
+ * Krylov recurrences around it can be checked against closed forms and dense solves. */
+static double g_pcd_pcg_tol = 1.0e-5;
+static int g_pcd_pcg_max = 150;
+void og_set_pcd_pcg(double tol, int max_iter) { g_pcd_pcg_tol = tol; g_pcd_pcg_max = max_iter; }
+
 void og_pcd(const og_grid* g, const double* zu, const double* zv, const double* zp,
             double* ru, double* rv, double* rp,
             const double* u_adv, const double* v_adv,
@@ -572,7 +579,7 @@ void og_pcd(const og_grid* g, const double* zu, const double* zv, const double* 
   memset(rv, 0, sv * sizeof(double));
   memset(rp, 0, sp * sizeof(double));
 
-  og_poisson_pcg(g, phi, zp, rho_u, rho_v, 1.0e-5, 150, NULL, NULL);
+  og_poisson_pcg(g, phi, zp, rho_u, rho_v, g_pcd_pcg_tol, g_pcd_pcg_max, NULL, NULL);
   og_halo_p(g, phi);
 
   for (int j = 1; j <= ny; ++j)

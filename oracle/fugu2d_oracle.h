@@ -41,6 +41,8 @@ typedef struct {
 /* Reduction summation: 1 = compensated (correctly rounded, default), 0 = plain sequential
  * in the reference's loop order. */
 void og_set_sum_mode(int mode);
+/* The PCD's inner PCG tolerance and cap (default 1e-5 / 150, the reference's). */
+void og_set_pcd_pcg(double tol, int max_iter);
 int og_get_sum_mode(void);
 
 size_t og_size_u(const og_grid* g);

@@ -73,6 +73,7 @@ def _declare(L):
         f.argtypes = args
         f.restype = res
     s("og_set_sum_mode", [C.c_int]); s("og_get_sum_mode", [], C.c_int)
+    s("og_set_pcd_pcg", [C.c_double, C.c_int], None)
     s("og_halo_p", [_G, _D]); s("og_halo_u", [_G, _D]); s("og_halo_v", [_G, _D])
     s("og_divergence", [_G, _D, _D, _D])
     s("og_grad_pressure", [_G, _D, _D, _D])
@@ -182,6 +183,11 @@ def set_sum_mode(mode: int):
     """0 (default): the reference's plain sequential sums; 1: double-double accumulation (the
     restatement of the GPU library's fast tree order, FUGU_SUM_TREE)."""
     lib().og_set_sum_mode(int(mode))
+
+
+def set_pcd_pcg(tol=1e-5, max_iter=150):
+    """The PCD's inner PCG stopping rule (default: the reference's 1e-5 / 150)."""
+    lib().og_set_pcd_pcg(float(tol), int(max_iter))
 
 
 def get_sum_mode() -> int:

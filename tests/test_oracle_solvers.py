@@ -193,10 +193,10 @@ def test_krylov_converges_to_dense_solution(order, consistent):
     x = np.linalg.lstsq(A, b, rcond=None)[0]
     got = np.array([X["uvp".index(k)][ix[k](i, j)] for (k, i, j) in dofs])
     nvel = sum(1 for d in dofs if d[0] != "p")
-    # The stopping test is on the left-preconditioned recurrence residual and some search
-    # vectors enter the matvec without velocity BCs (mod_bicgstab2.f90:430,489), so the
-    # recurrence drifts from the true residual; agreement to 1e-4 still fails for any wrong
-    # coefficient or update order, which do not converge at all.
+    # With the reference PCD (inner PCG stopped at 1e-5) the preconditioner is not a fixed linear
+    # map, so the recurrence residual drifts from the true one and 1e-4 is what this form allows.
+    # The tight pins (1e-8 against dense solves, closed-form first iterates, textbook iterates)
+    # run the PCD's PCG to 1e-13 instead: tests/test_oracle_krylov_pins.py.
     assert np.abs(got[:nvel] - x[:nvel]).max() < 1e-4 * np.abs(x[:nvel]).max()
     pg, px = got[nvel:] - got[nvel:].mean(), x[nvel:] - x[nvel:].mean()
     assert np.abs(pg - px).max() < 1e-4 * np.abs(px).max()
