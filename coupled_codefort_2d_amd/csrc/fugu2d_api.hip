@@ -2775,7 +2775,7 @@ int fugu_kernel_timing(fugu_ctx* c, int enable) {
     kt_resolve(c);
     c->kt_on = enable != 0;
     if (enable)
-      for (auto& kv : c->kt) { kv.second.total_ms = 0.0; kv.second.count = 0; }
+      for (auto& kv : c->kt) { kv.second.total_ms = 0.0; kv.second.count = 0; kv.second.total_bytes = 0.0; }
     return 0;
   });
 }
