@@ -39,7 +39,8 @@ class FuguStepOpts(C.Structure):
 
 class FuguStepStats(C.Structure):
     _fields_ = [("iters", C.c_int), ("outer", C.c_int), ("res", C.c_double), ("true_res", C.c_double),
-                ("solve_seconds", C.c_double), ("step_seconds", C.c_double), ("pcg_calls", C.c_int)]
+                ("solve_seconds", C.c_double), ("step_seconds", C.c_double), ("pcg_calls", C.c_int),
+                ("vol_before", C.c_double), ("vol_change", C.c_double), ("vol_clamp", C.c_double)]
 
 
 class FuguAdjointOpts(C.Structure):
@@ -69,10 +70,12 @@ SOLVE_EXTRAPOLATE = 64
 
 SUM_REFERENCE = 0   # the reference's sequential summation order (default)
 SUM_TREE = 1        # double-double tree (fast)
+LIMITER_VAN_LEER = 0   # the reference's VOF limiter (default)
+LIMITER_SUPERBEE = 1   # opt-in
 
 EXPORTS = [
     "fugu_ctx_create", "fugu_ctx_destroy", "fugu_set_params", "fugu_last_error", "fugu_field_size",
-    "fugu_set_sum_order", "fugu_get_sum_order",
+    "fugu_set_sum_order", "fugu_get_sum_order", "fugu_set_vof_limiter",
     "fugu_exchange_halos", "fugu_op_divergence", "fugu_apply_velocity_bcs", "fugu_matvec_coupled",
     "fugu_coupled_dot", "fugu_solve_poisson_pressure", "fugu_solve_helmholtz_u", "fugu_solve_helmholtz_v",
     "fugu_apply_pcd_precond", "fugu_bicgstab_solve", "fugu_bicgstab2_solve", "fugu_vof_advection_tvd",
@@ -111,6 +114,7 @@ def _declare(L):
     s("fugu_set_params", [P, C.POINTER(FuguParams)])
     s("fugu_set_sum_order", [P, C.c_int])
     s("fugu_get_sum_order", [P])
+    s("fugu_set_vof_limiter", [P, C.c_int])
     s("fugu_last_error", [], C.c_char_p)
     s("fugu_field_size", [P, C.c_int], C.c_longlong)
     s("fugu_exchange_halos", [P, _D, C.c_int])
@@ -207,6 +211,9 @@ class Context:
     def set_sum_order(self, order):
         """SUM_REFERENCE (the reference's sequential sums, bit-identical) or SUM_TREE (fast)."""
         check(load().fugu_set_sum_order(self.h, int(order)), "fugu_set_sum_order")
+
+    def set_vof_limiter(self, limiter):
+        check(load().fugu_set_vof_limiter(self.h, int(limiter)), "fugu_set_vof_limiter")
 
     def sum_order(self):
         return check(load().fugu_get_sum_order(self.h), "fugu_get_sum_order")

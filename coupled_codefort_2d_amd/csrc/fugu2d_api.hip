@@ -126,6 +126,8 @@ struct fugu_ctx {
   double* seq_ctot = nullptr;
   int* seq_cnt = nullptr;
   double* seq_tot = nullptr;
+  int vof_limiter = FUGU_LIMITER_VAN_LEER;   // the VOF face limiter (van Leer: the reference's)
+  double* dclamp = nullptr;                  // per-cell clamp correction of the step's VOF update
   double* h_sc = nullptr;
   int* h_fl = nullptr;
   size_t ntrip = 0;
@@ -1664,8 +1666,8 @@ void mixture(fugu_ctx* c, const double* cc) {
 }
 
 void vof(fugu_ctx* c, const double* cin, double* cout, const double* u, const double* v, double dt,
-         double* fx, double* fy) {
-  LAUNCH(k_vof, grid_union(c->g), dim3(TX, TY), c->s, c->g, cin, cout, u, v, dt, fx, fy);
+         double* fx, double* fy, double* dclamp = nullptr) {
+  LAUNCH(k_vof, grid_union(c->g), dim3(TX, TY), c->s, c->g, cin, cout, u, v, dt, fx, fy, c->vof_limiter, dclamp);
   if (fx) halo(c, fx, 0);
   if (fy) halo(c, fy, 1);
   halo(c, cout, 2);
@@ -1690,7 +1692,13 @@ int step_impl(fugu_ctx* c, double dt, int step_idx, const fugu_step_opts* o, fug
   LAUNCH(k_ab2, dim3(nblk(g.nu)), dim3(256), c->s, c->uadv, c->u_prev, (const double*)c->u, g.nu, step_idx <= 1 ? 1 : 0);
   LAUNCH(k_ab2, dim3(nblk(g.nv)), dim3(256), c->s, c->vadv, c->v_prev, (const double*)c->v, g.nv, step_idx <= 1 ? 1 : 0);
   // 2. VOF (:224-235) with u^n
-  vof(c, c->c, c->cnew, c->u, c->v, dt, amp ? c->fx : nullptr, amp ? c->fy : nullptr);
+  vof(c, c->c, c->cnew, c->u, c->v, dt, amp ? c->fx : nullptr, amp ? c->fy : nullptr, c->dclamp);
+  {   // volume budget of the update: sum C before and after, and the clamp's part (S_VOL_*)
+    const dim3 gr = grid_cells_red(g);
+    LAUNCH(k_vol_sums, gr, dim3(TX, TY), c->s, g, (const double*)c->c, (const double*)c->cnew,
+           (const double*)c->dclamp, c->partials);
+    LAUNCH(k_fold_vol, dim3(1), dim3(NT), c->s, (int)nblocks(gr), g.vol, (const double*)c->partials, c->sc);
+  }
   if (amp) {
     LAUNCH(k_amp, dim3(nblk(g.nu)), dim3(256), c->s, c->mdu, (const double*)c->fx, (const double*)c->uadv, g.nu,
            P.rho1 - P.rho2, P.rho2);
@@ -1798,6 +1806,10 @@ int step_impl(fugu_ctx* c, double dt, int step_idx, const fugu_step_opts* o, fug
     st->solve_seconds = ms_solve * 1e-3;
     st->step_seconds = ms_step * 1e-3;
     st->pcg_calls = c->pcg_calls;
+    read_slots(c);
+    st->vol_before = c->h_sc[S_VOL_OLD];
+    st->vol_change = c->h_sc[S_VOL_NEW] - c->h_sc[S_VOL_OLD];
+    st->vol_clamp = c->h_sc[S_VOL_CLAMP];
   }
   int rc = 0;
   if (!(res == res)) rc |= FUGU_W_NONFINITE;
@@ -2206,7 +2218,7 @@ int fugu_ctx_create(const fugu_grid* grid, const fugu_params* prm, fugu_ctx** ou
     HIPCHK(hipHostMalloc((void**)&c->h_fl, (F_COUNT + 2) * sizeof(int), hipHostMallocDefault));
     c->X = dalloc<double>(c, c->ntrip);
     c->u = c->X; c->v = c->X + g.nu; c->p = c->X + g.nu + g.nv;
-    c->c = dalloc<double>(c, g.np); c->cnew = dalloc<double>(c, g.np);
+    c->c = dalloc<double>(c, g.np); c->cnew = dalloc<double>(c, g.np); c->dclamp = dalloc<double>(c, g.np);
     c->u_prev = dalloc<double>(c, g.nu); c->v_prev = dalloc<double>(c, g.nv);
     c->unm1 = dalloc<double>(c, g.nu); c->vnm1 = dalloc<double>(c, g.nv);
     c->rho_c = dalloc<double>(c, g.np); c->rho_u = dalloc<double>(c, g.nu); c->rho_v = dalloc<double>(c, g.nv);
@@ -2269,6 +2281,16 @@ int fugu_set_sum_order(fugu_ctx* c, int order) {
     return -3;
   }
   c->sum_ref = order == FUGU_SUM_REFERENCE;
+  return 0;
+}
+
+int fugu_set_vof_limiter(fugu_ctx* c, int limiter) {
+  NEED(c);
+  if (limiter != FUGU_LIMITER_VAN_LEER && limiter != FUGU_LIMITER_SUPERBEE) {
+    g_err = "fugu_set_vof_limiter: limiter must be FUGU_LIMITER_VAN_LEER (0) or FUGU_LIMITER_SUPERBEE (1)";
+    return -3;
+  }
+  c->vof_limiter = limiter;
   return 0;
 }
 

@@ -24,6 +24,7 @@ enum Slot : int {
   S_GSCALE,      // scale of the breakdown guards (1 = the reference's absolute 1e-30 tests)
   S_INVH,        // 1 / h_{k+1,k} of the outer FGMRES
   S_FG_TOL,      // the outer FGMRES cycle's absolute stopping level for its residual estimate
+  S_VOL_OLD, S_VOL_NEW, S_VOL_CLAMP,   // the step's volume budget: sum C vol before, after, clamp part
   S_HBUF,        // 256 slots: dot products / combination coefficients of the outer FGMRES
   S_COUNT = S_HBUF + 256
 };
@@ -1529,8 +1530,19 @@ __global__ void k_mixture(Grid g, const double* __restrict__ c, double rho1, dou
 // VOF-TVD advection (src/mod_vof.f90:25-140).  Face fluxes are recomputed per use
 // (each face by its two cells) with the same arithmetic as the reference's Flux arrays.
 __device__ inline double van_leer(double r) { return (r <= 0.0) ? 0.0 : (2.0 * r) / (1.0 + r); }
+// Superbee, psi(r) = max(0, min(2r, 1), min(r, 2)) (Roe 1986; the upper edge of Sweby's TVD region):
+// an opt-in limiter, not the reference's.  Fortran min/max order, so the oracle's value bit for bit.
+__device__ inline double superbee(double r) {
+  const double a = (1.0 < 2.0 * r) ? 1.0 : 2.0 * r;
+  const double b = (2.0 < r) ? 2.0 : r;
+  const double m = (b > a) ? b : a;
+  return (m > 0.0) ? m : 0.0;
+}
+// the face limiter: 0 van Leer (the reference), 1 Superbee
+__device__ inline double vof_psi(double r, int lim) { return lim == 1 ? superbee(r) : van_leer(r); }
 
-__device__ inline double vof_flux_x(const Grid& g, const double* c, const double* u, double dt, int i, int j) {
+__device__ inline double vof_flux_x(const Grid& g, const double* c, const double* u, double dt, int i, int j,
+                                    int lim) {
   const double uf = u[IU(g, i, j)];
   double num, den, rr, cfl, cf;
   if (uf >= 0.0) {
@@ -1538,17 +1550,18 @@ __device__ inline double vof_flux_x(const Grid& g, const double* c, const double
     den = c[IP(g, i + 1, j)] - c[IP(g, i, j)];
     rr = (fabs(den) > 1.0e-12) ? num / den : 0.0;
     cfl = fabs(uf) * dt * g.inv_dx;
-    cf = c[IP(g, i, j)] + 0.5 * (1.0 - cfl) * van_leer(rr) * den;
+    cf = c[IP(g, i, j)] + 0.5 * (1.0 - cfl) * vof_psi(rr, lim) * den;
   } else {
     num = c[IP(g, i + 2, j)] - c[IP(g, i + 1, j)];
     den = c[IP(g, i + 1, j)] - c[IP(g, i, j)];
     rr = (fabs(den) > 1.0e-12) ? num / den : 0.0;
     cfl = fabs(uf) * dt * g.inv_dx;
-    cf = c[IP(g, i + 1, j)] - 0.5 * (1.0 - cfl) * van_leer(rr) * den;
+    cf = c[IP(g, i + 1, j)] - 0.5 * (1.0 - cfl) * vof_psi(rr, lim) * den;
   }
   return uf * clamp01(cf);
 }
-__device__ inline double vof_flux_y(const Grid& g, const double* c, const double* v, double dt, int i, int j) {
+__device__ inline double vof_flux_y(const Grid& g, const double* c, const double* v, double dt, int i, int j,
+                                    int lim) {
   const double vf = v[IV(g, i, j)];
   double num, den, rr, cfl, cf;
   if (vf >= 0.0) {
@@ -1556,37 +1569,70 @@ __device__ inline double vof_flux_y(const Grid& g, const double* c, const double
     den = c[IP(g, i, j + 1)] - c[IP(g, i, j)];
     rr = (fabs(den) > 1.0e-12) ? num / den : 0.0;
     cfl = fabs(vf) * dt * g.inv_dy;
-    cf = c[IP(g, i, j)] + 0.5 * (1.0 - cfl) * van_leer(rr) * den;
+    cf = c[IP(g, i, j)] + 0.5 * (1.0 - cfl) * vof_psi(rr, lim) * den;
   } else {
     num = c[IP(g, i, j + 2)] - c[IP(g, i, j + 1)];
     den = c[IP(g, i, j + 1)] - c[IP(g, i, j)];
     rr = (fabs(den) > 1.0e-12) ? num / den : 0.0;
     cfl = fabs(vf) * dt * g.inv_dy;
-    cf = c[IP(g, i, j + 1)] - 0.5 * (1.0 - cfl) * van_leer(rr) * den;
+    cf = c[IP(g, i, j + 1)] - 0.5 * (1.0 - cfl) * vof_psi(rr, lim) * den;
   }
   return vf * clamp01(cf);
 }
 
+// dclamp (optional, p-kind): the volume fraction the final clamp to [0, 1] adds to each interior
+// cell (clamped - unclamped update), for the step's volume budget.
 __global__ void __launch_bounds__(NT) k_vof(Grid g, const double* __restrict__ cin, double* __restrict__ cout,
                                             const double* __restrict__ u, const double* __restrict__ v,
-                                            double dt, double* __restrict__ fx, double* __restrict__ fy) {
+                                            double dt, double* __restrict__ fx, double* __restrict__ fy, int lim,
+                                            double* __restrict__ dclamp) {
   int i, j;
   union_ij(g, i, j);
   const int nx = g.nx, ny = g.ny, nh = g.nh;
   if (i >= 1 - nh && i <= nx + nh && j >= 1 - nh && j <= ny + nh) {
     double o = cin[IP(g, i, j)];
     if (i >= 1 && i <= nx && j >= 1 && j <= ny) {
-      const double fe = vof_flux_x(g, cin, u, dt, i, j), fw = vof_flux_x(g, cin, u, dt, i - 1, j);
-      const double fn = vof_flux_y(g, cin, v, dt, i, j), fs = vof_flux_y(g, cin, v, dt, i, j - 1);
+      const double fe = vof_flux_x(g, cin, u, dt, i, j, lim), fw = vof_flux_x(g, cin, u, dt, i - 1, j, lim);
+      const double fn = vof_flux_y(g, cin, v, dt, i, j, lim), fs = vof_flux_y(g, cin, v, dt, i, j - 1, lim);
       o = cin[IP(g, i, j)] - dt * ((fe - fw) * g.inv_dx + (fn - fs) * g.inv_dy);
-      o = clamp01(o);
+      const double oc = clamp01(o);
+      if (dclamp) dclamp[IP(g, i, j)] = oc - o;
+      o = oc;
     }
     cout[IP(g, i, j)] = o;
   }
   if (fx && i >= -nh && i <= nx + nh && j >= 1 - nh && j <= ny + nh)
-    fx[IU(g, i, j)] = (j >= 1 && j <= ny && i >= 0 && i <= nx) ? vof_flux_x(g, cin, u, dt, i, j) : 0.0;
+    fx[IU(g, i, j)] = (j >= 1 && j <= ny && i >= 0 && i <= nx) ? vof_flux_x(g, cin, u, dt, i, j, lim) : 0.0;
   if (fy && i >= 1 - nh && i <= nx + nh && j >= -nh && j <= ny + nh)
-    fy[IV(g, i, j)] = (i >= 1 && i <= nx && j >= 0 && j <= ny) ? vof_flux_y(g, cin, v, dt, i, j) : 0.0;
+    fy[IV(g, i, j)] = (i >= 1 && i <= nx && j >= 0 && j <= ny) ? vof_flux_y(g, cin, v, dt, i, j, lim) : 0.0;
+}
+
+// Volume budget of one VOF update over the interior cells: sum C_old, sum C_new, sum of the clamp
+// corrections (double-double, per-workgroup partials; k_fold_vol writes S_VOL_*).
+__global__ void __launch_bounds__(NT) k_vol_sums(Grid g, const double* __restrict__ cold,
+                                                 const double* __restrict__ cnew, const double* __restrict__ dclamp,
+                                                 double* __restrict__ partials) {
+  DD v[3] = {dd_from(0.0), dd_from(0.0), dd_from(0.0)};
+  const int i = blockIdx.x * TX + (int)threadIdx.x + 1;
+  const int tiles = (g.ny + TY - 1) / TY;
+  for (int t = blockIdx.y; t < tiles; t += gridDim.y) {
+    const int j = t * TY + (int)threadIdx.y + 1;
+    if (i <= g.nx && j <= g.ny) {
+      const size_t o = IP(g, i, j);
+      v[0] = dd_add(v[0], dd_from(cold[o]));
+      v[1] = dd_add(v[1], dd_from(cnew[o]));
+      v[2] = dd_add(v[2], dd_from(dclamp[o]));
+    }
+  }
+  publish_dd<3>(v, partials);
+}
+__global__ void __launch_bounds__(NT) k_fold_vol(int nb, double vol, const double* __restrict__ partials,
+                                                 double* __restrict__ sc) {
+  double tot[3];
+  fold_dd<3>(partials, nb, tot);
+  if (threadIdx.x == 0 && threadIdx.y == 0) {
+    sc[S_VOL_OLD] = tot[0] * vol; sc[S_VOL_NEW] = tot[1] * vol; sc[S_VOL_CLAMP] = tot[2] * vol;
+  }
 }
 
 // one SVOF Jacobi sweep (src/mod_vof.f90:168-190) with mirror/wrap neighbour reads

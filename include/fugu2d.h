@@ -110,6 +110,12 @@ long long fugu_field_size(const fugu_ctx* ctx, int kind);
  * No counterpart in the reference (it has one order). */
 enum { FUGU_SUM_REFERENCE = 0, FUGU_SUM_TREE = 1 };
 int fugu_set_sum_order(fugu_ctx* ctx, int order);
+/* Face limiter of the VOF-TVD advection (fugu_vof_advection_tvd and the step).
+ * FUGU_LIMITER_VAN_LEER (default): the reference's psi(r) = 2r/(1+r), r > 0 (src/mod_vof.f90:25-33).
+ * FUGU_LIMITER_SUPERBEE: psi(r) = max(0, min(2r, 1), min(r, 2)), an opt-in sharper limiter (the
+ * reference's README names Superbee; its code uses van Leer).  No counterpart in the reference. */
+enum { FUGU_LIMITER_VAN_LEER = 0, FUGU_LIMITER_SUPERBEE = 1 };
+int fugu_set_vof_limiter(fugu_ctx* ctx, int limiter);
 int fugu_get_sum_order(const fugu_ctx* ctx);
 
 /* ---- fine API: one entry per reference public procedure ----------------------------- */
@@ -234,6 +240,11 @@ typedef struct {
   double solve_seconds;   /* device time of the coupled solve (HIP events) */
   double step_seconds;    /* device time of the whole step */
   int pcg_calls;          /* preconditioner applications */
+  /* Volume budget of the step's VOF update (src/mod_vof.f90:120-124), sums of C dx dy over the
+   * interior cells: before the update, its change, and the part of the change the clamp of C to
+   * [0, 1] made (mod_vof.f90:124).  Walls carry no flux and periodic faces cancel, so
+   * vol_change - vol_clamp is the flux form's conservation error (rounding only). */
+  double vol_before, vol_change, vol_clamp;
 } fugu_step_stats;
 
 /* Per-kernel device timing: when enabled, launches of the instrumented kernels ("pcg_iter1_f64",

@@ -861,6 +861,18 @@ void og_bicgstab2(const og_grid* g, double* Xu, double* Xv, double* Xp,
 /* src/mod_vof.f90:25-33 */
 double og_van_leer(double r) { return (r <= 0.0) ? 0.0 : (2.0 * r) / (1.0 + r); }
 
+/* Superbee (P. L. Roe, Annu. Rev. Fluid Mech. 18 (1986) 337-365; P. K. Sweby, SIAM J. Numer. Anal.
+ * 21 (1984) 995-1011): psi(r) = max(0, min(2r, 1), min(r, 2)), the upper edge of Sweby's TVD region.
+ * Not in the reference (whose code is van Leer, src/mod_vof.f90:25-33); an opt-in limiter. */
+double og_superbee(double r) {
+  const double a = fmin_f(2.0 * r, 1.0), b = fmin_f(r, 2.0);
+  return fmax_f(0.0, fmax_f(a, b));
+}
+/* The face limiter of og_vof_advection: 0 van Leer (the reference), 1 Superbee. */
+static int g_vof_limiter = 0;
+void og_set_vof_limiter(int lim) { g_vof_limiter = lim; }
+static double vof_psi(double r) { return g_vof_limiter == 1 ? og_superbee(r) : og_van_leer(r); }
+
 /* src/mod_vof.f90:38-140 */
 void og_vof_advection(const og_grid* g, const double* c_in, double* c_out,
                       const double* u, const double* v, double dt,
@@ -881,13 +893,13 @@ void og_vof_advection(const og_grid* g, const double* c_in, double* c_out,
         den = P(c_in, i + 1, j) - P(c_in, i, j);
         rr = (fabs(den) > 1.0e-12) ? num / den : 0.0;
         cfl = fabs(uf) * dt * inv_dx;
-        cf = P(c_in, i, j) + half * (1.0 - cfl) * og_van_leer(rr) * den;
+        cf = P(c_in, i, j) + half * (1.0 - cfl) * vof_psi(rr) * den;
       } else {
         num = P(c_in, i + 2, j) - P(c_in, i + 1, j);
         den = P(c_in, i + 1, j) - P(c_in, i, j);
         rr = (fabs(den) > 1.0e-12) ? num / den : 0.0;
         cfl = fabs(uf) * dt * inv_dx;
-        cf = P(c_in, i + 1, j) - half * (1.0 - cfl) * og_van_leer(rr) * den;
+        cf = P(c_in, i + 1, j) - half * (1.0 - cfl) * vof_psi(rr) * den;
       }
       FX(i, j) = uf * clamp01(cf);
     }
@@ -900,13 +912,13 @@ void og_vof_advection(const og_grid* g, const double* c_in, double* c_out,
         den = P(c_in, i, j + 1) - P(c_in, i, j);
         rr = (fabs(den) > 1.0e-12) ? num / den : 0.0;
         cfl = fabs(vf) * dt * inv_dy;
-        cf = P(c_in, i, j) + half * (1.0 - cfl) * og_van_leer(rr) * den;
+        cf = P(c_in, i, j) + half * (1.0 - cfl) * vof_psi(rr) * den;
       } else {
         num = P(c_in, i, j + 2) - P(c_in, i, j + 1);
         den = P(c_in, i, j + 1) - P(c_in, i, j);
         rr = (fabs(den) > 1.0e-12) ? num / den : 0.0;
         cfl = fabs(vf) * dt * inv_dy;
-        cf = P(c_in, i, j + 1) - half * (1.0 - cfl) * og_van_leer(rr) * den;
+        cf = P(c_in, i, j + 1) - half * (1.0 - cfl) * vof_psi(rr) * den;
       }
       FY(i, j) = vf * clamp01(cf);
     }

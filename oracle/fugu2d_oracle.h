@@ -43,6 +43,9 @@ typedef struct {
 void og_set_sum_mode(int mode);
 /* The PCD's inner PCG tolerance and cap (default 1e-5 / 150, the reference's). */
 void og_set_pcd_pcg(double tol, int max_iter);
+/* VOF face limiter: 0 van Leer (the reference, default), 1 Superbee (opt-in). */
+void og_set_vof_limiter(int lim);
+double og_superbee(double r);
 int og_get_sum_mode(void);
 
 size_t og_size_u(const og_grid* g);

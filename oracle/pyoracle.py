@@ -74,6 +74,8 @@ def _declare(L):
         f.restype = res
     s("og_set_sum_mode", [C.c_int]); s("og_get_sum_mode", [], C.c_int)
     s("og_set_pcd_pcg", [C.c_double, C.c_int], None)
+    s("og_set_vof_limiter", [C.c_int], None)
+    s("og_superbee", [C.c_double], C.c_double)
     s("og_halo_p", [_G, _D]); s("og_halo_u", [_G, _D]); s("og_halo_v", [_G, _D])
     s("og_divergence", [_G, _D, _D, _D])
     s("og_grad_pressure", [_G, _D, _D, _D])
@@ -188,6 +190,11 @@ def set_sum_mode(mode: int):
 def set_pcd_pcg(tol=1e-5, max_iter=150):
     """The PCD's inner PCG stopping rule (default: the reference's 1e-5 / 150)."""
     lib().og_set_pcd_pcg(float(tol), int(max_iter))
+
+
+def set_vof_limiter(lim: int):
+    """0: van Leer (the reference's, default); 1: Superbee."""
+    lib().og_set_vof_limiter(int(lim))
 
 
 def get_sum_mode() -> int:

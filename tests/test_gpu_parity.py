@@ -247,16 +247,22 @@ def test_krylov_solve(kw, order, consistent, sum_order):
 
 
 @pytest.mark.parametrize("kw", GRIDS, ids=IDS)
-def test_vof_bitwise(kw):
+@pytest.mark.parametrize("limiter", [0, 1], ids=["van_leer", "superbee"])
+def test_vof_bitwise(kw, limiter):
     g = grid(kw)
     ctx = ctx_for(g)
-    rng = np.random.default_rng(108)
-    c = np.clip(droplet_c(g, 0.3) + 0.2 * rng.standard_normal(g.shape("p")), -0.05, 1.05)
-    u, v = rand(g, "u", rng), rand(g, "v", rng)
-    co, fxo, fyo = O.vof_advection(g, c, u, v, 0.01, want_flux=True)
-    cg, fxg, fyg = g.zeros("p"), g.zeros("u"), g.zeros("v")
-    ctx.vof_advection(c, cg, u, v, 0.01, fxg, fyg)
-    same(cg, co, "C_out"); same(fxg, fxo, "flux_x"); same(fyg, fyo, "flux_y")
+    ctx.set_vof_limiter(limiter)
+    O.set_vof_limiter(limiter)
+    try:
+        rng = np.random.default_rng(108)
+        c = np.clip(droplet_c(g, 0.3) + 0.2 * rng.standard_normal(g.shape("p")), -0.05, 1.05)
+        u, v = rand(g, "u", rng), rand(g, "v", rng)
+        co, fxo, fyo = O.vof_advection(g, c, u, v, 0.01, want_flux=True)
+        cg, fxg, fyg = g.zeros("p"), g.zeros("u"), g.zeros("v")
+        ctx.vof_advection(c, cg, u, v, 0.01, fxg, fyg)
+        same(cg, co, "C_out"); same(fxg, fxo, "flux_x"); same(fyg, fyo, "flux_y")
+    finally:
+        O.set_vof_limiter(0)
 
 
 @pytest.mark.parametrize("kw", GRIDS, ids=IDS)

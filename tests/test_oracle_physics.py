@@ -39,6 +39,63 @@ def test_van_leer_values_and_symmetry():
     assert vl(1e12) < 2.0
 
 
+def test_superbee_values_and_tvd_region():
+    """Superbee psi(r) = max(0, min(2r, 1), min(r, 2)) (Roe 1986): closed-form values on each
+    branch, the symmetry psi(r)/r = psi(1/r), the edges of Sweby's second-order TVD region
+    (psi(r) = min(2r, 2) is its upper bound; psi = max(r, 1) between r = 1/2 and 2) and psi(1) = 1."""
+    sb = O.lib().og_superbee
+    for r in (-5.0, -1.0, -1e-3, 0.0):
+        assert sb(r) == 0.0
+    for r, want in ((0.1, 0.2), (0.25, 0.5), (0.5, 1.0), (0.75, 1.0), (1.0, 1.0), (1.5, 1.5), (2.0, 2.0),
+                    (3.0, 2.0), (1e12, 2.0)):
+        assert sb(r) == want, (r, sb(r))
+    rs = np.concatenate([np.linspace(1e-3, 10, 997), [0.3, 0.5, 1.0, 2.0, 4.0]])
+    for r in rs:
+        v = sb(r)
+        assert 0.0 <= v <= min(2 * r, 2.0) + 1e-15
+        assert v >= min(r, 1.0) - 1e-15          # at least minmod: inside the second-order region
+        assert abs(v / r - sb(1.0 / r)) < 1e-14 * max(1.0, v / r)
+
+
+@pytest.fixture
+def superbee():
+    O.set_vof_limiter(1)
+    yield
+    O.set_vof_limiter(0)
+
+
+def test_superbee_vof_shift_conservation_and_sharpness(superbee):
+    """With Superbee the VOF update is still an exact shift at CFL 1 and conservative and bounded at
+    CFL 0.4; advecting a step 40 times at CFL 0.4 it keeps a narrower transition than van Leer."""
+    g = O.Grid(64, 8, 1.0, 0.125, per_x=True, per_y=True)
+    rng = np.random.default_rng(3)
+    prof = rng.uniform(0.0, 1.0, g.nx)
+    c = g.zeros("p")
+    c[g.nh:g.nh + g.ny, g.nh:g.nh + g.nx] = prof[None, :]
+    O.halo(g, c, "p")
+    dt = 1.0 / 256.0
+    u = _uniform(g, "u", g.dx / dt)
+    out = O.vof_advection(g, c, u, g.zeros("v"), dt)
+    inner = out[g.nh:g.nh + g.ny, g.nh:g.nh + g.nx]
+    assert np.array_equal(inner, np.broadcast_to(np.roll(prof, 1)[None, :], inner.shape))
+    step = g.zeros("p")
+    X, _ = g.coords("p")
+    step[...] = ((X > 0.25) & (X < 0.6)).astype(float)
+    s = (slice(g.nh, g.nh + g.ny), slice(g.nh, g.nh + g.nx))
+    width = {}
+    for lim in (0, 1):
+        O.set_vof_limiter(lim)
+        cc = step.copy()
+        for _ in range(40):
+            nc = O.vof_advection(g, cc, _uniform(g, "u", 0.4 * g.dx / dt), g.zeros("v"), dt)
+            assert abs(nc[s].sum() - cc[s].sum()) <= 1e-13 * cc[s].sum()
+            assert nc[s].min() >= -1e-15 and nc[s].max() <= 1 + 1e-15
+            cc = nc
+        row = cc[g.nh, g.nh:g.nh + g.nx]
+        width[lim] = int(np.sum((row > 0.01) & (row < 0.99)))
+    assert width[1] < width[0], width
+
+
 def _uniform(g, kind, val):
     a = g.zeros(kind)
     a[...] = val
