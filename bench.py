@@ -289,6 +289,66 @@ def run_dropin(abi, no_cpu=False):
     return out
 
 
+def run_physics(abi):
+    """Configs 1 and 2 at full length by the FP64 fast form, with time steps/s and the physics the
+    reference's validation programs print (coupled_codefort_2d_amd.diagnostics):
+    config 1: lid cavity 128^2, rho 1, mu 1e-3 (Re 1000), BDF1 dt 1e-3, 100 steps, solves to 1e-10;
+    config 2: static droplet 256^2, rho 1000/1, mu 1/0.01, sigma 1, BDF2 dt 1e-4, 50 steps, 1e-8."""
+    from coupled_codefort_2d_amd import diagnostics as D
+    from coupled_codefort_2d_amd import workloads
+    base = abi.SOLVE_REFINED | abi.PRECOND_MG | abi.PRECOND_VISC | abi.PRECOND_VMG
+    out = {}
+    # config 1
+    n = 128
+    ctx = abi.Context(n, n, params=abi.FuguParams(1.0, 1.0, 1e-3, 1e-3, 0.0, 0.0, 0.0, 1, 1.0e12, 0.5))
+    u = np.zeros((n + 4, n + 5)); v = np.zeros((n + 5, n + 4)); p = np.zeros((n + 4, n + 4))
+    ctx.upload(u, v, u.copy(), v.copy(), p, np.ones((n + 4, n + 4)))
+    ctx.init_properties()
+    opts = abi.step_opts(tol=1e-10, max_iter=600, solver_flags=base, inner_max_iter=-1, u_lid=1.0)
+    res, t0 = [], time.perf_counter()
+    for s_ in range(1, 101):
+        st, rc = ctx.step(1e-3, s_, opts)
+        res.append(st.res)
+    wall = time.perf_counter() - t0
+    u, v = ctx.field("u", u), ctx.field("v", v)
+    ctx.close()
+    out["config1"] = {"workload": "lid cavity 128x128, rho 1, mu 1e-3 (Re 1000), BDF1 dt 1e-3, 100 steps, FP64 "
+                                  "fast form to true residual 1e-10",
+                      "time_steps_per_s": round(100 / wall, 2), "max_res": float(f"{max(res):.3e}"),
+                      "centre_line_extrema_t0.1": D.cavity_extrema(u, v, n, n),
+                      "div_max": D.div_max(u, v, n, n, 1.0, 1.0)}
+    # config 2
+    n, R, sigma = 256, 0.25, 1.0
+    c0 = droplet_c(n)
+    ctx = abi.Context(n, n, params=abi.FuguParams(1000.0, 1.0, 1.0, 0.01, sigma, 0.0, 0.0, 2, 1.0e12, 0.5))
+    u = np.zeros((n + 4, n + 5)); v = np.zeros((n + 5, n + 4)); p = np.zeros((n + 4, n + 4))
+    ctx.upload(u, v, u.copy(), v.copy(), p, c0)
+    ctx.init_properties()
+    opts = abi.step_opts(tol=TOL, max_iter=400, solver_flags=base, inner_max_iter=-1)
+    res, worst, clamp, t0 = [], 0.0, 0.0, time.perf_counter()
+    for s_ in range(1, 51):
+        st, rc = ctx.step(DROP_DT, s_, opts)
+        res.append(st.res)
+        worst = max(worst, abs(st.vol_change - st.vol_clamp) / st.vol_before)
+        clamp = max(clamp, abs(st.vol_clamp) / st.vol_before)
+    wall = time.perf_counter() - t0
+    u, v, p = ctx.field("u", u), ctx.field("v", v), ctx.field("p", p)
+    c, kc = ctx.field("c", np.zeros_like(c0)), ctx.field("kappa_curv", np.zeros_like(c0))
+    ctx.close()
+    out["config2"] = {"workload": "static droplet 256x256, R 0.25, rho 1000/1, mu 1/0.01, sigma 1, BDF2 dt 1e-4, "
+                                  "50 steps, FP64 fast form to true residual 1e-8",
+                      "time_steps_per_s": round(50 / wall, 2), "max_res": float(f"{max(res):.3e}"),
+                      "laplace_jump": D.laplace_jump(p, n, n, 1.0, 1.0, R, sigma),
+                      "curvature": D.curvature_mean(kc, n, n, 1.0, 1.0, R),
+                      "spurious_velocity_max": D.spurious_velocity(u, v, n, n),
+                      "mass_error": D.mass_error(c, c0, n, n), "div_max": D.div_max(u, v, n, n, 1.0, 1.0),
+                      "max_step_conservation_error_rel": worst, "max_step_clamp_volume_rel": clamp}
+    for k in ("config1", "config2"):
+        print(f"# physics {k}: " + json.dumps({a: b for a, b in out[k].items() if a != "workload"}),
+              file=sys.stderr, flush=True)
+    return out
+
+
 def run_pcg2048(abi, n=2048, iters=150):
     """The drop-in's PCG iteration kernels on 2048^2 (the 256-inclusion density field, rho 1000/1),
     150 iterations from zero, in both summation orders, timed with HIP events (warm-up call first)."""
@@ -439,11 +499,12 @@ def main():
     ap.add_argument("--inner-max", type=int, default=-1, help="inner BiCGStab(2) cap; -1: PCD only")
     ap.add_argument("--restart", type=int, default=0,
                     help="FGMRES restart length; 0: the library's default (20 up to 2^21 cells, 30 above)")
-    ap.add_argument("--sections", default="headline,m1,adjoint,dropin,pcg2048",
+    ap.add_argument("--sections", default="headline,m1,adjoint,dropin,pcg2048,physics",
                     help="headline (rising bubble, value), m1 (2048 x 2048 FP64 vs mixed), adjoint (s/grad, "
                          "1024^2, 16 steps), adjoint2048 (2048^2, 256 inclusions, 32 steps, mixed forward), "
                          "dropin (reference algorithm on the 256^2 droplet, GPU vs the oracle on one core: "
-                         "cpu_baseline), pcg2048 (the drop-in's PCG kernels on 2048^2)")
+                         "cpu_baseline), pcg2048 (the drop-in's PCG kernels on 2048^2), physics (configs 1 and 2 "
+                         "at full length: time steps/s and the reference's validation outputs)")
     ap.add_argument("--no-m1", action="store_true", help="skip the 2048 x 2048 FP64-vs-mixed section")
     ap.add_argument("--m1-steps", type=int, default=3)
     ap.add_argument("--m1-warmup", type=int, default=1)
@@ -488,6 +549,7 @@ def main():
                               sigma=1.0, mixed=True, mg_cycles=6)
     dropin = run_dropin(abi, no_cpu=args.no_cpu) if "dropin" in sections else None
     pcg = run_pcg2048(abi) if "pcg2048" in sections else None
+    physics = run_physics(abi) if "physics" in sections else None
 
     def summary(f):
         sts = f["stats"]
@@ -565,6 +627,8 @@ def main():
                 "workload_differs_from_value": "the rising bubble's reference solver does not converge (the "
                                                "headline uses the fast forms), so the baseline is a workload where "
                                                "the reference path converges"}
+    if physics:
+        line["physics"] = physics
     if pcg:
         line["pcg2048"] = pcg
         line["dropin_roofline"] = pcg_roofline(pcg)
