@@ -85,6 +85,7 @@ EXPORTS = [
     "fugu_state_init_properties", "fugu_step", "fugu_kernel_timing", "fugu_kernel_time",
     "fugu_solve_pressure_mg", "fugu_matvec_transpose", "fugu_apply_pcd_transpose", "fugu_adjoint_solve",
     "fugu_cell_properties", "fugu_adjoint_gradient", "fugu_coupled_dot_noalloc",
+    "fugu_checkpoint_write", "fugu_checkpoint_read", "fugu_write_vtk",
 ]
 
 _lib = None
@@ -143,6 +144,9 @@ def _declare(L):
     s("fugu_state_download", [P] + [_D] * 6)
     s("fugu_field_download", [P, C.c_char_p, _D])
     s("fugu_state_init_properties", [P])
+    s("fugu_checkpoint_write", [P, C.c_char_p, C.c_int, C.c_double])
+    s("fugu_checkpoint_read", [P, C.c_char_p, _I, _D, _I])
+    s("fugu_write_vtk", [P, C.c_char_p])
     s("fugu_step", [P, C.c_double, C.c_int, C.POINTER(FuguStepOpts), C.POINTER(FuguStepStats)])
     s("fugu_kernel_timing", [P, C.c_int])
     s("fugu_solve_pressure_mg", [P, _D, _D, _D, _D, C.c_int])
@@ -372,6 +376,19 @@ class Context:
     def field(self, name, out):
         check(load().fugu_field_download(self.h, name.encode(), ptr(out)), "field_download")
         return out
+
+    def checkpoint_write(self, prefix, step, t):
+        check(load().fugu_checkpoint_write(self.h, prefix.encode(), int(step), float(t)), "checkpoint_write")
+
+    def checkpoint_read(self, prefix):
+        """(ok, step, t)."""
+        st, t, ok = C.c_int(0), C.c_double(0), C.c_int(0)
+        check(load().fugu_checkpoint_read(self.h, prefix.encode(), C.byref(st), C.byref(t), C.byref(ok)),
+              "checkpoint_read")
+        return bool(ok.value), st.value, t.value
+
+    def write_vtk(self, filename):
+        check(load().fugu_write_vtk(self.h, filename.encode()), "write_vtk")
 
     def init_properties(self):
         check(load().fugu_state_init_properties(self.h), "init_properties")

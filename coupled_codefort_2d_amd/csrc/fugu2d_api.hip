@@ -6,6 +6,7 @@
 #include "fugu2d.h"
 #include "fugu2d_kernels.hpp"
 #include "fugu2d_seqsum.hpp"
+#include "fugu2d_io.hpp"
 #include "fugu2d_mg.hpp"
 #include "fugu2d_adjoint.hpp"
 
@@ -2778,6 +2779,110 @@ int fugu_field_download(fugu_ctx* c, const char* name, double* o) {
         return 0;
       }
     throw Error(std::string("unknown field ") + name);
+  });
+}
+
+int fugu_checkpoint_write(fugu_ctx* c, const char* prefix, int step, double t) {
+  NEED(c);
+  if (!prefix) { g_err = "fugu_checkpoint_write: null prefix"; return -2; }
+  return guarded([&]() {
+    const Grid& g = c->g;
+    // the six padded arrays, in the reference's write order (src/mod_io.f90:185)
+    const double* src[6] = {c->u, c->v, c->u_prev, c->v_prev, c->p, c->c};
+    const size_t cnt[6] = {g.nu, g.nv, g.nu, g.nv, g.np, g.np};
+    size_t tot = 0;
+    for (size_t k : cnt) tot += k;
+    std::vector<double> host(tot);
+    size_t off = 0;
+    for (int k = 0; k < 6; ++k) {
+      HIPCHK(hipMemcpyAsync(host.data() + off, src[k], cnt[k] * sizeof(double), hipMemcpyDeviceToHost, c->s));
+      off += cnt[k];
+    }
+    sync(c);
+    const std::string name = checkpoint_name(prefix);
+    FILE* f = std::fopen(name.c_str(), "wb");
+    if (!f) throw Error("fugu_checkpoint_write: cannot open " + name);
+    unsigned char hb[24];
+    header_bytes(CkptHeader{step, t, g.nx, g.ny, g.nh}, hb);
+    const bool ok = std::fwrite(hb, 1, 24, f) == 24 && std::fwrite(host.data(), sizeof(double), tot, f) == tot;
+    if (std::fclose(f) != 0 || !ok) throw Error("fugu_checkpoint_write: write failed on " + name);
+    return 0;
+  });
+}
+
+int fugu_checkpoint_read(fugu_ctx* c, const char* prefix, int* step, double* t, int* ok) {
+  NEED(c);
+  if (!prefix || !ok) { g_err = "fugu_checkpoint_read: null argument"; return -2; }
+  *ok = 0;
+  return guarded([&]() {
+    const Grid& g = c->g;
+    const std::string name = checkpoint_name(prefix);
+    FILE* f = std::fopen(name.c_str(), "rb");
+    if (!f) return 0;   // no checkpoint: ok = 0 (src/mod_io.f90:206-210)
+    unsigned char hb[24];
+    if (std::fread(hb, 1, 24, f) != 24) { std::fclose(f); return 0; }
+    const CkptHeader h = header_from(hb);
+    if (step) *step = h.step;
+    if (t) *t = h.t;
+    if (h.nx != g.nx || h.ny != g.ny || h.nh != g.nh) { std::fclose(f); return 0; }   // :214-218
+    double* dst[6] = {c->u, c->v, c->u_prev, c->v_prev, c->p, c->c};
+    const size_t cnt[6] = {g.nu, g.nv, g.nu, g.nv, g.np, g.np};
+    size_t tot = 0;
+    for (size_t k : cnt) tot += k;
+    std::vector<double> host(tot);
+    const size_t got = std::fread(host.data(), sizeof(double), tot, f);
+    std::fclose(f);
+    if (got != tot) throw Error("fugu_checkpoint_read: " + name + " is shorter than its header says");
+    size_t off = 0;
+    for (int k = 0; k < 6; ++k) {
+      HIPCHK(hipMemcpyAsync(dst[k], host.data() + off, cnt[k] * sizeof(double), hipMemcpyHostToDevice, c->s));
+      off += cnt[k];
+    }
+    c->have_pprev = false;
+    mixture(c, c->c);   // the driver recomputes the mixture after a restart (src/main.f90:185-186)
+    sync(c);
+    *ok = 1;
+    return 0;
+  });
+}
+
+int fugu_write_vtk(fugu_ctx* c, const char* filename) {
+  NEED(c);
+  if (!filename) { g_err = "fugu_write_vtk: null filename"; return -2; }
+  return guarded([&]() {
+    const Grid& g = c->g;
+    const int nx = g.nx, ny = g.ny;
+    std::vector<double> p(g.np), cc(g.np), rc(g.np), kc(g.np), u(g.nu), v(g.nv);
+    out(c, p.data(), c->p, g.np); out(c, cc.data(), c->c, g.np); out(c, rc.data(), c->rho_c, g.np);
+    out(c, kc.data(), c->kcurv, g.np); out(c, u.data(), c->u, g.nu); out(c, v.data(), c->v, g.nv);
+    sync(c);
+    std::string s;
+    s.reserve((size_t)nx * ny * 16 * 7 + 1024);
+    s += "# vtk DataFile Version 3.0\nFUGU-2D Coupled Two-Phase Solver Output\nASCII\nDATASET STRUCTURED_POINTS\n";
+    s += "DIMENSIONS "; fortran_i(s, nx, 8); fortran_i(s, ny, 8); fortran_i(s, 1, 8); s += "\n";
+    s += "ORIGIN "; fortran_es(s, 0.5 * g.dx); fortran_es(s, 0.5 * g.dy); fortran_es(s, 0.0); s += "\n";
+    s += "SPACING "; fortran_es(s, g.dx); fortran_es(s, g.dy); fortran_es(s, g.dx); s += "\n";
+    s += "POINT_DATA "; fortran_i(s, (long long)nx * ny, 10); s += "\n";
+    const char* names[4] = {"VOF_C", "Pressure", "Density", "Curvature"};
+    const std::vector<double>* fields[4] = {&cc, &p, &rc, &kc};
+    for (int k = 0; k < 4; ++k) {
+      s += "SCALARS "; s += names[k]; s += " double 1\nLOOKUP_TABLE default\n";
+      for (int j = 1; j <= ny; ++j)
+        for (int i = 1; i <= nx; ++i) { fortran_es(s, (*fields[k])[IP(g, i, j)]); s += "\n"; }
+    }
+    s += "VECTORS Velocity double\n";
+    for (int j = 1; j <= ny; ++j)
+      for (int i = 1; i <= nx; ++i) {
+        // cell-centred velocity, half * (u(i-1) + u(i)) (src/mod_io.f90:48-53)
+        const double uc = 0.5 * (u[IU(g, i - 1, j)] + u[IU(g, i, j)]);
+        const double vc = 0.5 * (v[IV(g, i, j - 1)] + v[IV(g, i, j)]);
+        fortran_es(s, uc); fortran_es(s, vc); fortran_es(s, 0.0); s += "\n";
+      }
+    FILE* f = std::fopen(filename, "w");
+    if (!f) throw Error(std::string("fugu_write_vtk: cannot open ") + filename);
+    const bool ok = std::fwrite(s.data(), 1, s.size(), f) == s.size();
+    if (std::fclose(f) != 0 || !ok) throw Error(std::string("fugu_write_vtk: write failed on ") + filename);
+    return 0;
   });
 }
 

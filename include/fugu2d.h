@@ -232,6 +232,22 @@ int fugu_field_download(fugu_ctx* ctx, const char* name, double* out);
 /* Mixture properties from the resident C (src/main.f90:185-186). */
 int fugu_state_init_properties(fugu_ctx* ctx);
 
+/* ---- checkpoint / restart and VTK (src/mod_io.f90) at one rank, from the resident state -------- */
+/* src/mod_io.f90:169-187 write_checkpoint: "<prefix>_r0000.bin", stream bytes: int32 step, real64 t,
+ * int32 nx, ny, nh (24 bytes), then u, v, u_prev, v_prev, p, C as whole padded arrays, native byte
+ * order -- the file the reference's read_checkpoint reads. */
+int fugu_checkpoint_write(fugu_ctx* ctx, const char* prefix, int step, double t);
+/* src/mod_io.f90:192-225 read_checkpoint: *ok = 0 when the file is missing or its nx, ny, nh differ
+ * (the state is then untouched; step and t are set when the header was read); on success the six
+ * arrays become the resident state and the mixture properties are recomputed from C, as the
+ * reference driver does after a restart (src/main.f90:122-131, 185-186).  A restarted run continues
+ * with step index step + 1. */
+int fugu_checkpoint_read(fugu_ctx* ctx, const char* prefix, int* step, double* t, int* ok);
+/* src/mod_io.f90:29-164 write_vtk_fields: legacy ASCII STRUCTURED_POINTS with VOF_C, Pressure,
+ * Density, Curvature and the cell-centred velocity, every value in the reference's '(ES15.7)' /
+ * '(3ES15.7)' editing. */
+int fugu_write_vtk(fugu_ctx* ctx, const char* filename);
+
 typedef struct {
   int iters;              /* Krylov iterations (inner ones summed for the refined solve) */
   int outer;              /* refinement steps (0 for the reference solver) */
