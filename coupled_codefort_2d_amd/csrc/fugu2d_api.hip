@@ -127,6 +127,7 @@ struct fugu_ctx {
   double* seq_ctot = nullptr;
   int* seq_cnt = nullptr;
   double* seq_tot = nullptr;
+  int* seq_ticket = nullptr;
   int vof_limiter = FUGU_LIMITER_VAN_LEER;   // the VOF face limiter (van Leer: the reference's)
   double* dclamp = nullptr;                  // per-cell clamp correction of the step's VOF update
   double* h_sc = nullptr;
@@ -220,6 +221,27 @@ struct fugu_ctx {
     }
   } fg_key;
   std::vector<hipGraphExec_t> fg_graph;
+  // The reference Krylov solvers' iteration as a HIP graph (krylov_graphs; off unless
+  // FUGU_KRYLOV_GRAPHS=1): the body of one BiCGStab / BiCGStab(2) iteration -- its matvecs, PCDs with
+  // their inner PCG iterations, dots and updates, every launch predicated on device flags and reading
+  // its iteration number from F_KITER -- is captured once per key and replayed, so the thousands of
+  // dependent launches of an iteration cost one graph launch on the host.  Bit-identical to the
+  // eager launches (the same kernels with the same arguments).  Off by default: the solve is
+  // bound by the GPU's dependent-launch chain, not by host launch cost (1-7 % at 256^2), and a
+  // process under rocprofv3 kernel tracing crashed inside the runtime at the first re-capture.
+  bool krylov_graphs = false;
+  struct KryKey {
+    int order = 0, consistent = 0, mvflags = 0, max_iter = 0, seq = 0, tsize = 0, mv_tile = 0;
+    double dt = 0.0, pcd_visc = 0.0;
+    const void* work = nullptr;
+    bool operator==(const KryKey& o) const {
+      return order == o.order && consistent == o.consistent && mvflags == o.mvflags && max_iter == o.max_iter &&
+             seq == o.seq && tsize == o.tsize && mv_tile == o.mv_tile && dt == o.dt && pcd_visc == o.pcd_visc &&
+             work == o.work;
+    }
+  } kry_key;
+  hipGraphExec_t kry_graph = nullptr;
+  int kry_graph_pcg = 0;   // PCG solves one replay stands for
   std::vector<int> fg_graph_pcg;   // pressure-block calls one replay stands for
   // adjoint: solve_fgmres applies A_eff^T and the transposed PCD instead of A and the PCD
   bool adjoint = false;
@@ -295,6 +317,7 @@ Sink sink(fugu_ctx* c, long nterms) {
     c->seq_ctot = dalloc<double>(c, 5 * 3 * nch);
     c->seq_cnt = dalloc<int>(c, 5 * nch);
     c->seq_tot = dalloc<double>(c, 8);
+    c->seq_ticket = dalloc<int>(c, SQ_TICKETS);
   }
   return Sink{c->terms, nterms, c->seq_tot};
 }
@@ -304,11 +327,10 @@ void seq_reduce(fugu_ctx* c, const Sink& sk, int nv, Skip skip) {
   if (!sk.n) return;
   SeqArgs a;
   a.t = sk.p; a.n = sk.n; a.nch = (int)((sk.n + SQ_CH - 1) / SQ_CH);
-  a.ctot = c->seq_ctot; a.cnt = c->seq_cnt; a.tot = c->seq_tot;
+  a.ctot = c->seq_ctot; a.cnt = c->seq_cnt; a.tot = c->seq_tot; a.ticket = c->seq_ticket;
   const dim3 gc((unsigned)a.nch, (unsigned)nv);
   LAUNCH(k_seq_chunks, gc, dim3(SQ_NT), c->s, a, skip, (const int*)c->fl);
   LAUNCH(k_seq_compact, gc, dim3(SQ_NT), c->s, a, skip, (const int*)c->fl);
-  LAUNCH(k_seq_fold, dim3((unsigned)nv), dim3(SQ_NT), c->s, a, skip, (const int*)c->fl);
 }
 long cdot_terms(const Grid& g) {
   const long iu = g.per_x ? g.nx : g.nx - 1, jv = g.per_y ? g.ny : g.ny - 1;
@@ -1033,6 +1055,52 @@ bool pcd(fugu_ctx* c, Work<T>& w, const T* Z, T* R, T dt, bool bcs_after = false
 }
 
 // ---------------------------------------------------------------------------------
+// One iteration of the reference Krylov solvers, from its HIP graph or eagerly.  The graph holds
+// the iteration's launches exactly as `emit` enqueues them; it is captured on the first iteration
+// of a key and replayed after that.  Only the reference PCD qualifies (the multigrid forms read
+// set-up results on the host), and neither kernel timing nor the launch log may be on.
+void kry_graph_clear(fugu_ctx* c) {
+  if (c->kry_graph) (void)hipGraphExecDestroy(c->kry_graph);
+  c->kry_graph = nullptr;
+  c->kry_graph_pcg = 0;
+}
+template <typename T, typename F>
+void kry_iter(fugu_ctx* c, Work<T>& w, int order, bool consistent, int mvflags, int max_iter, double dt, F&& emit) {
+  const bool graphed = c->krylov_graphs && !c->use_mg && !c->use_vmg && !c->kt_on && !c->launch_log;
+  if (!graphed) {
+    emit();
+    return;
+  }
+  fugu_ctx::KryKey key;
+  key.order = order; key.consistent = consistent ? 1 : 0; key.mvflags = mvflags; key.max_iter = max_iter;
+  key.seq = seq_sums(c) ? 1 : 0; key.tsize = (int)sizeof(T); key.mv_tile = c->mv_tile;
+  key.dt = dt; key.pcd_visc = c->pcd_visc; key.work = (const void*)&w;
+  if (!c->kry_graph || !(key == c->kry_key)) {
+    kry_graph_clear(c);
+    const int pc0 = c->pcg_calls;
+    hipGraph_t gr = nullptr;
+    HIPCHK(hipStreamBeginCapture(c->s, hipStreamCaptureModeThreadLocal));
+    try {
+      emit();
+    } catch (...) {
+      (void)hipStreamEndCapture(c->s, &gr);
+      if (gr) (void)hipGraphDestroy(gr);
+      throw;
+    }
+    HIPCHK(hipStreamEndCapture(c->s, &gr));
+    hipGraphExec_t ge = nullptr;
+    const hipError_t e = hipGraphInstantiate(&ge, gr, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(gr);
+    HIPCHK(e);
+    c->kry_graph = ge;
+    c->kry_graph_pcg = c->pcg_calls - pc0;
+    c->kry_key = key;
+  } else {
+    c->pcg_calls += c->kry_graph_pcg;
+  }
+  HIPCHK(hipGraphLaunch(c->kry_graph, c->s));
+}
+
 // BiCGStab (src/mod_bicgstab2.f90:77-264), exact reference recurrences.
 template <typename T>
 void bicgstab1(fugu_ctx* c, Work<T>& w, double dt, double tol, int max_iter, int mvflags, bool has_lid,
@@ -1061,12 +1129,16 @@ void bicgstab1(fugu_ctx* c, Work<T>& w, double dt, double tol, int max_iter, int
   read_flags(c);
   int k = 1;
   if (!c->h_fl[F_DONE]) {
-    for (k = 1; k <= max_iter; ++k) {
+    // the iteration number k reaches the phases through the device counter F_KITER (kk = -1),
+    // so every iteration enqueues the same launches
+    const int kk = -1;
+    auto iter = [&]() {
+      LAUNCH(k_iter_begin, dim3(1), dim3(64), c->s, c->fl);
       bcs_trip(c, p, 0.0, brk);
       matvec(c, w, p, z, dt, mvflags, brk);
       pcd(c, w, z, v, (T)dt);
       bcs_trip(c, v, 0.0, brk);
-      cdot1<T>(c, r0, v, PH_B1_RV, k, max_iter);
+      cdot1<T>(c, r0, v, PH_B1_RV, kk, max_iter);
       lin<T>(c, L_SUB1, s, r, v, nullptr, nullptr, S_ALPHA);
       bcs_trip(c, s, 0.0, brk);
       matvec(c, w, s, z, dt, mvflags, brk);
@@ -1075,17 +1147,20 @@ void bicgstab1(fugu_ctx* c, Work<T>& w, double dt, double tol, int max_iter, int
       {
         const T* A[2] = {t, t};
         const T* Bq[2] = {t, s};
-        cdot<T>(c, 2, A, Bq, PH_B1_TT_TS, k, max_iter);
+        cdot<T>(c, 2, A, Bq, PH_B1_TT_TS, kk, max_iter);
       }
       lin<T>(c, L_ADD2, X, X, p, s, nullptr, S_ALPHA, S_OMEGA);
       lin<T>(c, L_SUB1, r, s, t, nullptr, nullptr, S_OMEGA);
       {
         const T* A[2] = {r, r0};
         const T* Bq[2] = {r, r};
-        cdot<T>(c, 2, A, Bq, PH_B1_RES, k, max_iter);
+        cdot<T>(c, 2, A, Bq, PH_B1_RES, kk, max_iter);
       }
       lin<T>(c, L_PUPD, p, r, p, v, nullptr, S_BETA, S_OMEGA);
       bcs_trip(c, X, 0.0, brk);
+    };
+    for (k = 1; k <= max_iter; ++k) {
+      kry_iter(c, w, 1, false, mvflags, max_iter, dt, iter);
       read_flags(c);
       if (c->h_fl[F_DONE]) break;
     }
@@ -1122,15 +1197,17 @@ void bicgstab2(fugu_ctx* c, Work<T>& w, double dt, double tol, int max_iter, int
   cdot1<T>(c, r, r, PH_B2_INIT, 0, max_iter);
   read_flags(c);
   if (!c->h_fl[F_DONE]) {
-    for (int k = 1; k <= max_iter; ++k) {
+    const int kk = -1;   // the iteration number comes from F_KITER, as in bicgstab1
+    auto iter = [&]() {
+      LAUNCH(k_iter_begin, dim3(1), dim3(64), c->s, c->fl);
       // even step
-      cdot1<T>(c, rt, r, PH_B2_XI_EVEN, k, max_iter);
+      cdot1<T>(c, rt, r, PH_B2_XI_EVEN, kk, max_iter);
       lin<T>(c, L_SUB1, uu, r, uu, nullptr, nullptr, S_BETA);
       bcs_trip(c, uu, 0.0, brk);
       matvec(c, w, uu, z, dt, mvflags, brk);
       pcd(c, w, z, vv, (T)dt);
       bcs_trip(c, vv, 0.0, brk);
-      cdot1<T>(c, rt, vv, PH_B2_GAMMA, k, max_iter);
+      cdot1<T>(c, rt, vv, PH_B2_GAMMA, kk, max_iter);
       lin<T>(c, L_SUB1, r, r, vv, nullptr, nullptr, S_ALPHA);
       if (consistent) bcs_trip(c, r, 0.0, brk);
       matvec(c, w, r, z, dt, mvflags, brk);
@@ -1138,13 +1215,13 @@ void bicgstab2(fugu_ctx* c, Work<T>& w, double dt, double tol, int max_iter, int
       bcs_trip(c, ss, 0.0, brk);
       lin<T>(c, L_ADD1, X, X, uu, nullptr, nullptr, S_ALPHA);
       // odd step
-      cdot1<T>(c, rt, ss, PH_B2_XI_ODD, k, max_iter);
+      cdot1<T>(c, rt, ss, PH_B2_XI_ODD, kk, max_iter);
       lin<T>(c, L_SUB1, vv, ss, vv, nullptr, nullptr, S_BETA);
       bcs_trip(c, vv, 0.0, brk);
       matvec(c, w, vv, z, dt, mvflags, brk);
       pcd(c, w, z, ww, (T)dt);
       bcs_trip(c, ww, 0.0, brk);
-      cdot1<T>(c, rt, ww, PH_B2_GAMMA, k, max_iter);
+      cdot1<T>(c, rt, ww, PH_B2_GAMMA, kk, max_iter);
       lin<T>(c, L_SUB1, uu, r, uu, nullptr, nullptr, S_BETA);
       lin<T>(c, L_SUB1, r, r, vv, nullptr, nullptr, S_ALPHA);
       lin<T>(c, L_SUB1, ss, ss, ww, nullptr, nullptr, S_ALPHA);
@@ -1155,13 +1232,16 @@ void bicgstab2(fugu_ctx* c, Work<T>& w, double dt, double tol, int max_iter, int
       {
         const T* A[5] = {r, ss, ss, tt, r};
         const T* Bq[5] = {ss, ss, tt, tt, tt};
-        cdot<T>(c, 5, A, Bq, PH_B2_GCR, k, max_iter);
+        cdot<T>(c, 5, A, Bq, PH_B2_GCR, kk, max_iter);
       }
       lin<T>(c, L_ADD3, X, X, r, ss, uu, S_OM1, S_OM2, S_ALPHA);
       lin<T>(c, L_SUB2, r, r, ss, tt, nullptr, S_OM1, S_OM2);
       lin<T>(c, L_SUB2, uu, uu, vv, ww, nullptr, S_OM1, S_OM2);
-      cdot1<T>(c, r, r, PH_B2_RES, k, max_iter);
+      cdot1<T>(c, r, r, PH_B2_RES, kk, max_iter);
       bcs_trip(c, X, 0.0, brk);
+    };
+    for (int k = 1; k <= max_iter; ++k) {
+      kry_iter(c, w, 2, consistent, mvflags, max_iter, dt, iter);
       read_flags(c);
       if (c->h_fl[F_DONE]) break;
     }
@@ -2115,7 +2195,7 @@ int guarded(F&& f) {
 
 // =====================================================================================
 // Allocation-free coupled dot (fugu_coupled_dot_noalloc): its scratch is this file-scope device
-// array -- the reference order's terms, chunk totals, value counts and sum, or the tree order's
+// array -- the reference order's terms, chunk totals, value counts, ticket counters and sum, or the tree order's
 // per-workgroup partials -- held by every GPU the library runs on until the process ends.  Host
 // code never uses the array's name as an address: each device's address comes from
 // hipGetSymbolAddress on the first call on that device (that device current) and is kept per
@@ -2126,8 +2206,9 @@ constexpr long NOALLOC_PARTIALS = 2L * RED_MAXB;       // one double-double per 
 constexpr long NOALLOC_CTOT = 3 * NOALLOC_NCH;
 constexpr long NOALLOC_CNT = NOALLOC_NCH / 2;          // NOALLOC_NCH ints
 constexpr long NOALLOC_TOT = 8;
+constexpr long NOALLOC_TICKET = SQ_TICKETS / 2;       // SQ_TICKETS ints
 constexpr long NOALLOC_DOUBLES =
-    NOALLOC_PARTIALS + NOALLOC_CTOT + NOALLOC_CNT + NOALLOC_TOT + NOALLOC_MAX_TERMS;   // 8.4 MB
+    NOALLOC_PARTIALS + NOALLOC_CTOT + NOALLOC_CNT + NOALLOC_TOT + NOALLOC_TICKET + NOALLOC_MAX_TERMS;   // 8.4 MB
 static __device__ __align__(256) double scratch[NOALLOC_DOUBLES];
 
 namespace {
@@ -2200,6 +2281,7 @@ int fugu_ctx_create(const fugu_grid* grid, const fugu_params* prm, fugu_ctx** ou
     if (const char* vr = std::getenv("FUGU_VMG_CYCLE_RATIO")) c->vmg_cycle_ratio = std::atof(vr);
     if (const char* rf = std::getenv("FUGU_REORTH_FRAC")) c->reorth_frac = std::atof(rf);
     if (const char* gr = std::getenv("FUGU_GRAPHS")) c->graphs = std::atoi(gr) != 0;
+    if (const char* kg = std::getenv("FUGU_KRYLOV_GRAPHS")) c->krylov_graphs = std::atoi(kg) != 0;
     if (const char* sb = std::getenv("FUGU_SVOF_BLOCKED")) c->svof_blocked = std::atoi(sb) != 0;
     HIPCHK(hipStreamCreateWithFlags(&c->s, hipStreamNonBlocking));
     for (auto& e : c->ev) HIPCHK(hipEventCreate(&e));
@@ -2253,6 +2335,7 @@ int fugu_ctx_destroy(fugu_ctx* c) {
   if (!c) return 0;
   if (c->s) (void)hipStreamSynchronize(c->s);
   fg_graphs_clear(c);
+  kry_graph_clear(c);
   for (void* p : c->allocs) (void)hipFree(p);
   for (auto& kv : c->kt)
     for (auto& e : kv.second.ev) (void)hipEventDestroy(e);
@@ -2420,23 +2503,24 @@ int fugu_coupled_dot_noalloc(const fugu_grid* grid, const double* u1, const doub
     double* ctot = partials + NOALLOC_PARTIALS;
     int* cnt = reinterpret_cast<int*>(ctot + NOALLOC_CTOT);
     double* tot = ctot + NOALLOC_CTOT + NOALLOC_CNT;
-    double* terms = tot + NOALLOC_TOT;
+    int* ticket = reinterpret_cast<int*>(tot + NOALLOC_TOT);
+    double* terms = tot + NOALLOC_TOT + NOALLOC_TICKET;
     hipStream_t st = static_cast<hipStream_t>(stream);
     DotSplit d;
     d.a[0] = u1; d.a[1] = v1; d.a[2] = p1;
     d.b[0] = u2; d.b[1] = v2; d.b[2] = p2;
     const dim3 gr = grid_union_red(g);
-    // every value the launches read is written by an earlier launch of this call: nothing to clear
+    // every value the launches read is written by an earlier launch of this call (the ticket
+    // counters too, zeroed by k_seq_chunks): nothing to clear
     if (sum_order == FUGU_SUM_REFERENCE) {
       const Sink sk{terms, nterms, tot};
       LAUNCH(k_cdot_split, gr, dim3(TX, TY), st, g, d, sk);
       SeqArgs a;
       a.t = terms; a.n = nterms; a.nch = (int)((nterms + SQ_CH - 1) / SQ_CH);
-      a.ctot = ctot; a.cnt = cnt; a.tot = tot;
+      a.ctot = ctot; a.cnt = cnt; a.tot = tot; a.ticket = ticket;
       const dim3 gc((unsigned)a.nch, 1u);
-      LAUNCH(k_seq_chunks, gc, dim3(SQ_NT), st, a, Skip{-1, 0}, (const int*)nullptr);
+      LAUNCH(k_seq_chunks, gc, dim3(SQ_NT), st, a, Skip{-1, 0}, (const int*)nullptr);   // zeroes the ticket
       LAUNCH(k_seq_compact, gc, dim3(SQ_NT), st, a, Skip{-1, 0}, (const int*)nullptr);
-      LAUNCH(k_seq_fold, dim3(1), dim3(SQ_NT), st, a, Skip{-1, 0}, (const int*)nullptr);
       LAUNCH(k_fold_dot_out, dim3(1), dim3(NT), st, 0, sk, dot_out);
     } else {
       const Sink sk{partials, 0, nullptr};

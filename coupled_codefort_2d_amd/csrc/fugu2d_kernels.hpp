@@ -41,6 +41,7 @@ enum Flag : int {
   F_REORTH,      // outer FGMRES: second Gram-Schmidt pass needed
   F_S_RBLK,      // SVOF in blocks: the block whose iterations are redone up to the converged one
   F_S_RCNT,      // ... and how many of them
+  F_KITER,       // the reference Krylov solvers' iteration number (bumped by k_iter_begin)
   F_COUNT
 };
 
@@ -65,6 +66,7 @@ enum Phase : int {
 
 __device__ inline void krylov_phase(int phase, int k, int max_iter, double* sc, int* fl,
                                     const double* tot) {
+  if (k < 0) k = fl[F_KITER];   // the iteration number lives on the device (replayable launches)
   // guard thresholds for dot-product-valued scalars: the reference's absolute 1e-30, or
   // 1e-30 * ||P^-1 b||^2 when relative guards are on
   const double G = sc[S_GSCALE];
@@ -629,6 +631,11 @@ __global__ void __launch_bounds__(NT) k_fold_dot_out(int nb, Sink in, double* __
   double tot[1];
   fold_sink<1>(in, nb, tot);
   if (threadIdx.x == 0 && threadIdx.y == 0) out[0] = tot[0];
+}
+
+// Start of a reference Krylov iteration: the device-side iteration number (F_KITER) the phases read.
+__global__ void k_iter_begin(int* __restrict__ fl) {
+  if (threadIdx.x == 0) fl[F_KITER] += 1;
 }
 
 // Whole-array triplet updates (Fortran whole-array expressions, ghosts included), each

@@ -27,9 +27,10 @@
 //                  G = Q * sum_{a<k<=b} rint(t_k / Q) (exact: |G| < 2^(e+1) is a multiple of Q);
 //                  an uncertain term or a tie becomes one value, itself.  The values overwrite
 //                  the chunk's terms in place, in order, with their count;
-//   k_seq_fold     one wavefront lane per chain adds every chunk's values one by one from zero
-//                  -- the same sequence of FP additions as the reference, with each run's
-//                  middle collapsed into one exact addition.
+//   then the last chunk workgroup of a chain to finish (a ticket counter, device-scope fences
+//                  on both sides) folds: one wavefront lane adds every chunk's values one by one
+//                  from zero -- the same sequence of FP additions as the reference, with each
+//                  run's middle collapsed into one exact addition.
 // Adding G is exact, and adding +0.0 changes nothing (s is never -0.0: from +0.0, an FP sum is
 // -0.0 only when both operands are).  Terms that are NaN or infinite fail certification and are
 // added one by one, so they propagate as in the reference.
@@ -40,12 +41,16 @@ namespace fugu {
 
 constexpr int SQ_CH = 4096;             // terms per chunk
 constexpr int SQ_NT = 256;              // threads per chunk workgroup
+constexpr int SQ_NW = SQ_NT / 64;       // its wavefronts
 constexpr int SQ_PT = SQ_CH / SQ_NT;    // consecutive terms per thread
+constexpr int SQ_LDP = SQ_PT + 1;       // padded LDS row of one thread's terms (odd: no bank conflicts)
 constexpr int SQ_WIN = 4096;            // chunks per window of the serial fold
 constexpr int SQ_BUF = 2048;            // values per staged piece of the serial fold
+constexpr int SQ_TICKETS = 8;           // ticket counters (one per chain)
 
 // A reference-order reduction: nv chains of n terms (chain q at t + q n), chunk totals ctot
-// (3 doubles per chunk: hi, lo, sum |t|), value counts cnt and the chains' sums tot.
+// (3 doubles per chunk: hi, lo, sum |t|), value counts cnt, the chains' sums tot and the
+// chains' ticket counters (zeroed by k_seq_chunks, counted up by k_seq_compact's workgroups).
 struct SeqArgs {
   double* t;
   long n;
@@ -53,6 +58,7 @@ struct SeqArgs {
   double* ctot;
   int* cnt;
   double* tot;
+  int* ticket;
 };
 // Skip a launch whose pass-1 kernel skipped (fl[fi] >= fv); fi < 0: never.
 struct Skip { int fi, fv; };
@@ -63,125 +69,215 @@ __device__ inline int seq_expo(double x) {   // unbiased binary exponent of a fi
 }
 __device__ inline double seq_pow2(int e) { return __longlong_as_double((long long)(e + 1023) << 52); }
 
-// Workgroup (SQ_NT threads) exclusive scans; every thread gets its own prefix and the total.
-__device__ inline DD seq_scan_dd(DD v, DD* lds, DD* total) {
-  const int tid = threadIdx.x;
-  lds[tid] = v;
-  __syncthreads();
-  for (int o = 1; o < SQ_NT; o <<= 1) {
-    DD a = lds[tid];
-    if (tid >= o) a = dd_add(lds[tid - o], a);
-    __syncthreads();
-    lds[tid] = a;
-    __syncthreads();
-  }
-  const DD incl = lds[tid];
-  *total = lds[SQ_NT - 1];
-  const DD ex = tid ? lds[tid - 1] : dd_from(0.0);
-  __syncthreads();
-  (void)incl;
-  return ex;
+// A prefix: the double-double sum of the terms and the sum of their magnitudes.
+struct PA {
+  DD p;
+  double a;
+};
+__device__ inline PA pa_zero() { PA r; r.p = dd_from(0.0); r.a = 0.0; return r; }
+__device__ inline PA pa_add(PA x, PA y) { PA r; r.p = dd_add(x.p, y.p); r.a = x.a + y.a; return r; }
+__device__ inline PA pa_shfl_up(PA v, int o) {
+  PA r;
+  r.p.hi = __shfl_up(v.p.hi, o, 64); r.p.lo = __shfl_up(v.p.lo, o, 64); r.a = __shfl_up(v.a, o, 64);
+  return r;
 }
-template <typename V>
-__device__ inline V seq_scan(V v, V* lds, V* total) {
-  const int tid = threadIdx.x;
-  lds[tid] = v;
-  __syncthreads();
-  for (int o = 1; o < SQ_NT; o <<= 1) {
-    V a = lds[tid];
-    if (tid >= o) a = lds[tid - o] + a;
-    __syncthreads();
-    lds[tid] = a;
-    __syncthreads();
+__device__ inline PA pa_shfl_xor(PA v, int o) {
+  PA r;
+  r.p.hi = __shfl_xor(v.p.hi, o, 64); r.p.lo = __shfl_xor(v.p.lo, o, 64); r.a = __shfl_xor(v.a, o, 64);
+  return r;
+}
+// Sum over the wavefront (every lane gets it; the grouping does not matter: only an
+// approximation of the prefix is needed, with the error bound below).
+__device__ inline PA pa_wave_sum(PA v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v = pa_add(v, pa_shfl_xor(v, o));
+  return v;
+}
+
+// Workgroup (SQ_NT threads) exclusive scans: wavefront scans by lane shuffles, then one LDS
+// exchange of the wavefront totals (`lds`: SQ_NW entries, not reused before a later barrier).
+__device__ inline PA sq_scan_pa(PA v, PA* lds) {
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  PA inc = v;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const PA y = pa_shfl_up(inc, o);
+    if (lane >= o) inc = pa_add(y, inc);
   }
-  *total = lds[SQ_NT - 1];
-  const V ex = tid ? lds[tid - 1] : V(0);
+  PA ex = pa_shfl_up(inc, 1);
+  if (lane == 0) ex = pa_zero();
+  if (lane == 63) lds[wid] = inc;
   __syncthreads();
+  PA pre = pa_zero();
+  for (int w = 0; w < wid; ++w) pre = pa_add(pre, lds[w]);
+  return pa_add(pre, ex);
+}
+struct I2 { int a, b; };
+__device__ inline I2 sq_scan_i2(I2 v, I2* lds, I2* total) {
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  I2 inc = v;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const int ya = __shfl_up(inc.a, o, 64), yb = __shfl_up(inc.b, o, 64);
+    if (lane >= o) { inc.a += ya; inc.b += yb; }
+  }
+  I2 ex;
+  ex.a = __shfl_up(inc.a, 1, 64); ex.b = __shfl_up(inc.b, 1, 64);
+  if (lane == 0) { ex.a = 0; ex.b = 0; }
+  if (lane == 63) lds[wid] = inc;
+  __syncthreads();
+  I2 pre{0, 0}, tot{0, 0};
+  for (int w = 0; w < SQ_NW; ++w) {
+    if (w < wid) { pre.a += lds[w].a; pre.b += lds[w].b; }
+    tot.a += lds[w].a; tot.b += lds[w].b;
+  }
+  *total = tot;
+  ex.a += pre.a; ex.b += pre.b;
   return ex;
 }
 
-// Chunk totals (grid: nch x nv).
+// Chunk totals (grid: nch x nv).  Workgroup (0, 0) also zeroes the ticket counters.
 __global__ void __launch_bounds__(SQ_NT) k_seq_chunks(SeqArgs a, Skip sk, const int* __restrict__ fl) {
   if (seq_skip(sk, fl)) return;
   const int c = blockIdx.x, q = blockIdx.y, tid = threadIdx.x;
+  if (c == 0 && q == 0 && tid < SQ_TICKETS) a.ticket[tid] = 0;
   const double* t = a.t + (long)q * a.n;
-  DD s = dd_from(0.0);
-  double A = 0.0;
+  PA v = pa_zero();
 #pragma unroll
   for (int m = 0; m < SQ_PT; ++m) {
     const long idx = (long)c * SQ_CH + tid + m * SQ_NT;
     if (idx < a.n) {
       const double x = t[idx];
-      s = dd_add(s, dd_from(x));
-      A = A + fabs(x);
+      v.p = dd_add(v.p, dd_from(x));
+      v.a = v.a + fabs(x);
     }
   }
-  __shared__ DD ls[SQ_NT];
-  __shared__ double la[SQ_NT];
-  DD tot;
-  double atot;
-  (void)seq_scan_dd(s, ls, &tot);
-  (void)seq_scan<double>(A, la, &atot);
+  v = pa_wave_sum(v);
+  __shared__ PA lw[SQ_NW];
+  if ((tid & 63) == 0) lw[tid >> 6] = v;
+  __syncthreads();
   if (tid == 0) {
+    PA s = lw[0];
+    for (int w = 1; w < SQ_NW; ++w) s = pa_add(s, lw[w]);
     double* o = a.ctot + ((long)q * a.nch + c) * 3;
-    o[0] = tot.hi; o[1] = tot.lo; o[2] = atot;
+    o[0] = s.p.hi; o[1] = s.p.lo; o[2] = s.a;
   }
 }
 
-// Certify, split into runs and compact one chunk in place (grid: nch x nv).
+// The serial fold of chain q (one workgroup): lane 0 adds every chunk's values in order while
+// wavefronts 1.. stage the next piece in LDS.  off: SQ_WIN + 1 ints; buf: 2 x SQ_BUF doubles.
+__device__ inline void sq_fold(const SeqArgs& a, int q, int* off, double* buf, I2* lsi) {
+  const int tid = threadIdx.x;
+  const double* t = a.t + (long)q * a.n;
+  const int* cnt = a.cnt + (long)q * a.nch;
+  constexpr int PW = SQ_WIN / SQ_NT;   // chunk counts per thread
+  double s = 0.0;
+  for (int w0 = 0; w0 < a.nch; w0 += SQ_WIN) {
+    const int wn = min(SQ_WIN, a.nch - w0);
+    // offsets of the window's chunks in the virtual concatenation of their values
+    int my[PW];
+    int loc = 0;
+#pragma unroll
+    for (int m = 0; m < PW; ++m) {
+      const int cc = tid * PW + m;
+      my[m] = cc < wn ? cnt[w0 + cc] : 0;
+      loc += my[m];
+    }
+    I2 T2;
+    const I2 e2 = sq_scan_i2(I2{loc, 0}, lsi, &T2);
+    int ex = e2.a;
+    const int T = T2.a;
+#pragma unroll
+    for (int m = 0; m < PW; ++m) {
+      off[tid * PW + m] = ex;
+      ex += my[m];
+    }
+    if (tid == 0) off[SQ_WIN] = T;
+    __syncthreads();
+    const int npc = (T + SQ_BUF - 1) / SQ_BUF;
+    auto stage = [&](int pc, int b) {
+      for (int j = pc * SQ_BUF + tid - 64; j < min(T, (pc + 1) * SQ_BUF); j += SQ_NT - 64) {
+        int lo = 0, hi = wn;   // last chunk with off[cc] <= j (off is non-decreasing)
+        while (hi - lo > 1) {
+          const int mid = (lo + hi) >> 1;
+          if (off[mid] <= j) lo = mid; else hi = mid;
+        }
+        buf[b * SQ_BUF + (j - pc * SQ_BUF)] = t[(long)(w0 + lo) * SQ_CH + (j - off[lo])];
+      }
+    };
+    if (tid >= 64 && npc > 0) stage(0, 0);
+    __syncthreads();
+    for (int pc = 0; pc < npc; ++pc) {
+      if (tid >= 64 && pc + 1 < npc) stage(pc + 1, (pc + 1) & 1);
+      if (tid == 0) {
+        const double* bb = buf + (pc & 1) * SQ_BUF;
+        const int nn = min(SQ_BUF, T - pc * SQ_BUF);
+        if (nn == SQ_BUF) {
+#pragma unroll 32
+          for (int m = 0; m < SQ_BUF; ++m) s = s + bb[m];
+        } else {
+          for (int m = 0; m < nn; ++m) s = s + bb[m];
+        }
+      }
+      __syncthreads();
+    }
+  }
+  if (tid == 0) a.tot[q] = s;
+}
+
+// Certify, split into runs and compact one chunk in place (grid: nch x nv); the chain's last
+// workgroup to finish then folds the chain.
 __global__ void __launch_bounds__(SQ_NT) k_seq_compact(SeqArgs a, Skip sk, const int* __restrict__ fl) {
   if (seq_skip(sk, fl)) return;
   const int c = blockIdx.x, q = blockIdx.y, tid = threadIdx.x;
+  const int lane = tid & 63, wid = tid >> 6;
   double* t = a.t + (long)q * a.n;
   const long base = (long)c * SQ_CH;
   const int len = (int)min((long)SQ_CH, a.n - base);
   const double u = 0x1p-53;
 
-  __shared__ double lt[SQ_CH];           // the chunk's terms, then the runs' integer sums
-  __shared__ DD lsd[SQ_NT];
-  __shared__ double lsa[SQ_NT];
-  __shared__ int lsi[SQ_NT];
-  __shared__ int lkey[SQ_NT + 1];
-  __shared__ int lstart[SQ_NT + 1];
-  __shared__ DD p0s;
-  __shared__ double a0s;
+  // the chunk's terms, one padded row per thread; then the runs' integer sums; then the
+  // fold's staging buffer
+  __shared__ double lt[SQ_NT * SQ_LDP];
+  __shared__ int loff[SQ_WIN + 1];
+  __shared__ PA lpre[SQ_NW], lthr[SQ_NW];
+  __shared__ int lkf[SQ_NW], lkl[SQ_NW];
+  __shared__ I2 lcnt[SQ_NW], lfold[SQ_NW];
+  __shared__ int llast;
+  static_assert(SQ_NT * SQ_LDP >= SQ_CH && SQ_NT * SQ_LDP >= 2 * SQ_BUF, "LDS reuse");
 
+  // coalesced load of the chunk: term L -> row L / SQ_PT, column L % SQ_PT
+  for (int m = tid; m < SQ_CH; m += SQ_NT) lt[(m / SQ_PT) * SQ_LDP + (m % SQ_PT)] = m < len ? t[base + m] : 0.0;
   // prefix of the earlier chunks (any order: only an approximation of P is needed)
   {
-    DD s = dd_from(0.0);
-    double A = 0.0;
+    PA e = pa_zero();
     for (int cc = tid; cc < c; cc += SQ_NT) {
       const double* o = a.ctot + ((long)q * a.nch + cc) * 3;
       DD v; v.hi = o[0]; v.lo = o[1];
-      s = dd_add(s, v);
-      A = A + o[2];
+      e.p = dd_add(e.p, v);
+      e.a = e.a + o[2];
     }
-    DD tot;
-    double atot;
-    (void)seq_scan_dd(s, lsd, &tot);
-    (void)seq_scan<double>(A, lsa, &atot);
-    if (tid == 0) { p0s = tot; a0s = atot; }
+    e = pa_wave_sum(e);
+    if (lane == 0) lpre[wid] = e;
   }
-  // coalesced load of the chunk into LDS, then SQ_PT consecutive terms per thread
-  for (int m = tid; m < SQ_CH; m += SQ_NT) lt[m] = m < len ? t[base + m] : 0.0;
   __syncthreads();
   double x[SQ_PT];
 #pragma unroll
-  for (int m = 0; m < SQ_PT; ++m) x[m] = lt[tid * SQ_PT + m];
+  for (int m = 0; m < SQ_PT; ++m) x[m] = lt[tid * SQ_LDP + m];
   const int nmine = max(0, min(SQ_PT, len - tid * SQ_PT));   // my valid terms
 
-  // thread totals -> exclusive prefix over the chunk
-  DD ts = dd_from(0.0);
-  double ta = 0.0;
-  for (int m = 0; m < nmine; ++m) { ts = dd_add(ts, dd_from(x[m])); ta = ta + fabs(x[m]); }
-  DD dummy;
-  double adummy;
-  DD P = seq_scan_dd(ts, lsd, &dummy);
-  double A = seq_scan<double>(ta, lsa, &adummy);
-  P = dd_add(p0s, P);
+  // thread totals -> exclusive prefix over the chunk, plus the earlier chunks
+  PA ts = pa_zero();
+  for (int m = 0; m < nmine; ++m) { ts.p = dd_add(ts.p, dd_from(x[m])); ts.a = ts.a + fabs(x[m]); }
+  const PA ex = sq_scan_pa(ts, lthr);   // (its barrier also ends every read of lt above)
+  PA p0 = lpre[0];
+  for (int w = 1; w < SQ_NW; ++w) p0 = pa_add(p0, lpre[w]);
+  DD P = dd_add(p0.p, ex.p);
   // an upper bound of the true prefix sum of |t|: the computed sums of positive terms are low by
   // at most a relative gamma_n, far inside the factor 2 of E below
-  A = a0s + A;
+  double A = p0.a + ex.a;
+  unsigned long long* R = reinterpret_cast<unsigned long long*>(lt);
+  for (int m = tid; m < SQ_CH; m += SQ_NT) R[m] = 0ull;
 
   // certification: key = sign * (exponent + 2048) of a certified binade, 0 = special term
   int key[SQ_PT];
@@ -210,13 +306,18 @@ __global__ void __launch_bounds__(SQ_NT) k_seq_compact(SeqArgs a, Skip sk, const
       }
     }
   }
-  // run starts: the chunk's first term, a special term or one after it, a change of binade/sign
-  lkey[tid + 1] = key[SQ_PT - 1];
-  if (tid == 0) lkey[0] = 0;
+  // the neighbouring threads' boundary keys: lane shuffles inside a wavefront, LDS across
+  int kprev = __shfl_up(key[SQ_PT - 1], 1, 64);
+  int knext = __shfl_down(key[0], 1, 64);
+  if (lane == 63) lkl[wid] = key[SQ_PT - 1];
+  if (lane == 0) lkf[wid] = key[0];
   __syncthreads();
+  if (lane == 0) kprev = wid > 0 ? lkl[wid - 1] : 0;
+  if (lane == 63) knext = wid + 1 < SQ_NW ? lkf[wid + 1] : 0;
+  // run starts: the chunk's first term, a special term or one after it, a change of binade/sign
   int start[SQ_PT];
   {
-    int prev = lkey[tid];
+    int prev = kprev;
 #pragma unroll
     for (int m = 0; m < SQ_PT; ++m) {
       const bool first = (tid == 0 && m == 0);
@@ -224,9 +325,9 @@ __global__ void __launch_bounds__(SQ_NT) k_seq_compact(SeqArgs a, Skip sk, const
       prev = key[m];
     }
   }
-  lstart[tid] = start[0];
-  if (tid == 0) lstart[SQ_NT] = 1;
-  __syncthreads();
+  // does the next thread's first term start a run?  (it has terms only if all of mine are valid)
+  const int klast = key[SQ_PT - 1];
+  const int nstart0 = (knext == 0 || klast == 0 || knext != klast) ? 1 : 0;
   // run ends (runs of two or more terms emit their G there)
   int endr[SQ_PT];
 #pragma unroll
@@ -235,26 +336,35 @@ __global__ void __launch_bounds__(SQ_NT) k_seq_compact(SeqArgs a, Skip sk, const
     int ns;                          // does the next term start a run (or the chunk end)?
     if (L + 1 >= len) ns = 1;
     else if (m + 1 < SQ_PT) ns = start[m + 1];
-    else ns = lstart[tid + 1];
+    else ns = nstart0;
     endr[m] = (L < len) && !start[m] && ns;
   }
   int nst = 0, nval = 0;
 #pragma unroll
   for (int m = 0; m < SQ_PT; ++m) { nst += start[m]; nval += start[m] + endr[m]; }
-  int tot_runs, tot_vals;
-  int run = seq_scan<int>(nst, lsi, &tot_runs) - 1;
-  int slot = seq_scan<int>(nval, lsi, &tot_vals);
-  // integer sums of the runs (exact; wrap-around is harmless, the totals fit)
-  unsigned long long* R = reinterpret_cast<unsigned long long*>(lt);
-  for (int m = tid; m < SQ_CH; m += SQ_NT) R[m] = 0ull;
-  __syncthreads();
+  I2 tot2;
+  const I2 ex2 = sq_scan_i2(I2{nst, nval}, lcnt, &tot2);   // (its barrier also orders R's zeroing)
+  const int run = ex2.a - 1;
+  int slot = ex2.b;
+  // integer sums of the runs (exact; wrap-around is harmless, the totals fit): each thread adds
+  // its share of every run it touches with one LDS atomic
   {
     int rr = run;
+    long long acc = 0;
+    bool have = false;
 #pragma unroll
     for (int m = 0; m < SQ_PT; ++m) {
-      rr += start[m];
-      if (m < nmine && !start[m]) atomicAdd(&R[rr], (unsigned long long)(long long)r[m]);
+      if (start[m]) {
+        if (have) atomicAdd(&R[rr], (unsigned long long)acc);
+        ++rr;
+        acc = 0;
+        have = false;
+      } else if (m < nmine) {
+        acc += (long long)r[m];
+        have = true;
+      }
     }
+    if (have) atomicAdd(&R[rr], (unsigned long long)acc);
   }
   __syncthreads();
   // values in order: a run start writes its term, a run end writes G = Q * R
@@ -270,69 +380,16 @@ __global__ void __launch_bounds__(SQ_NT) k_seq_compact(SeqArgs a, Skip sk, const
       }
     }
   }
-  if (tid == 0) a.cnt[(long)q * a.nch + c] = tot_vals;
-}
-
-// Serial fold (grid: nv; one chain per workgroup): lane 0 adds every chunk's values in order
-// while the other wavefronts stage the next piece in LDS.
-__global__ void __launch_bounds__(SQ_NT) k_seq_fold(SeqArgs a, Skip sk, const int* __restrict__ fl) {
-  if (seq_skip(sk, fl)) return;
-  const int q = blockIdx.x, tid = threadIdx.x;
-  const double* t = a.t + (long)q * a.n;
-  const int* cnt = a.cnt + (long)q * a.nch;
-  __shared__ int off[SQ_WIN + 1];
-  __shared__ int lsi[SQ_NT];
-  __shared__ double buf[2][SQ_BUF];
-  double s = 0.0;
-  for (int w0 = 0; w0 < a.nch; w0 += SQ_WIN) {
-    const int wn = min(SQ_WIN, a.nch - w0);
-    // offsets of the window's chunks in the virtual concatenation of their values
-    int my[SQ_WIN / SQ_NT];
-    int loc = 0;
-#pragma unroll
-    for (int m = 0; m < SQ_WIN / SQ_NT; ++m) {
-      const int cc = tid * (SQ_WIN / SQ_NT) + m;
-      my[m] = cc < wn ? cnt[w0 + cc] : 0;
-      loc += my[m];
-    }
-    int T;
-    int ex = seq_scan<int>(loc, lsi, &T);
-#pragma unroll
-    for (int m = 0; m < SQ_WIN / SQ_NT; ++m) {
-      off[tid * (SQ_WIN / SQ_NT) + m] = ex;
-      ex += my[m];
-    }
-    if (tid == 0) off[SQ_WIN] = T;
-    __syncthreads();
-    const int npc = (T + SQ_BUF - 1) / SQ_BUF;
-    auto stage = [&](int pc, int b) {
-      for (int j = pc * SQ_BUF + tid - 64; j < min(T, (pc + 1) * SQ_BUF); j += SQ_NT - 64) {
-        int lo = 0, hi = wn;   // last chunk with off[cc] <= j (off is non-decreasing)
-        while (hi - lo > 1) {
-          const int mid = (lo + hi) >> 1;
-          if (off[mid] <= j) lo = mid; else hi = mid;
-        }
-        buf[b][j - pc * SQ_BUF] = t[(long)(w0 + lo) * SQ_CH + (j - off[lo])];
-      }
-    };
-    if (tid >= 64 && npc > 0) stage(0, 0);
-    __syncthreads();
-    for (int pc = 0; pc < npc; ++pc) {
-      if (tid >= 64 && pc + 1 < npc) stage(pc + 1, (pc + 1) & 1);
-      if (tid == 0) {
-        const double* bb = buf[pc & 1];
-        const int nn = min(SQ_BUF, T - pc * SQ_BUF);
-        if (nn == SQ_BUF) {
-#pragma unroll 32
-          for (int m = 0; m < SQ_BUF; ++m) s = s + bb[m];
-        } else {
-          for (int m = 0; m < nn; ++m) s = s + bb[m];
-        }
-      }
-      __syncthreads();
-    }
-  }
-  if (tid == 0) a.tot[q] = s;
+  if (tid == 0) a.cnt[(long)q * a.nch + c] = tot2.b;
+  // the chain's last workgroup folds: release this chunk's values, take a ticket, and (last)
+  // acquire every other chunk's before reading them
+  __threadfence();
+  __syncthreads();
+  if (tid == 0) llast = (atomicAdd(&a.ticket[q], 1) == a.nch - 1) ? 1 : 0;
+  __syncthreads();
+  if (!llast) return;
+  __threadfence();
+  sq_fold(a, q, loff, lt, lfold);
 }
 
 }  // namespace fugu
