@@ -878,20 +878,37 @@ T* mg_vcycle(fugu_ctx* c, std::vector<MgLevel<T>>& lv, size_t l, bool zero_guess
   }
   MgLevel<T>& C = lv[l + 1];
   const double ncell = (double)L.d.nx * L.d.ny;
-  const dim3 gt = mgt_grid(L.d, MGT_TH);
+  // FP64 levels of at least 2^21 cells: 64-row tiles (1024 threads) for the pre-smoothing with up
+  // to 3 sweeps and the post-smoothing with 3.  Their 170-VGPR 32-row form fits one workgroup per
+  // CU; the 64-row form holds 6 cells per thread instead of 8, fits 128 VGPRs (four waves per
+  // SIMD) and has 1.5x instead of 1.75x halo overhead: 15-18 % faster at 2048^2 (pre, 2-3 sweeps;
+  // post, 3), bit-identical (the tile shape changes no arithmetic).  Smaller levels keep more
+  // workgroups than CUs with the 32-row tiles.
+  const bool th64 = sizeof(T) == 8 && ncell >= (double)(1 << 21);
+  const bool pre64 = th64 && (nu == 2 || nu == 3);
+  const dim3 gt = mgt_grid(L.d, pre64 ? 64 : MGT_TH);
   T* P = (g_in == L.phi) ? L.phi2 : L.phi;   // the fused pre-smoothing's output
   if (c->mg_fused) {
     // b, c_x, c_y (, m) (, the initial iterate) read; phi written; coarse b and phi written (a
     // quarter each)
     const char* nm = mg_kname(false, nu, sizeof(T) == 8);
     const double ab = ((L.m ? 5.5 : 4.5) + (g_in ? 1.0 : 0.0)) * sizeof(T) * ncell;
-#define FUGU_MG_PRE(NUV)                                                                                     \
-  TLAUNCH(c, nm, ab, (k_mg_pre<T, NUV, MGT_TH>), gt, dim3(16 * MGT_TH), c->s, L.d, (const T*)L.cx,             \
+#define FUGU_MG_PRE(NUV, THV)                                                                                \
+  TLAUNCH(c, nm, ab, (k_mg_pre<T, NUV, THV>), gt, dim3(16 * THV), c->s, L.d, (const T*)L.cx,                   \
           (const T*)L.cy, (const T*)L.m, (const T*)L.b, P, C.b, C.phi, g_in)
-    if (nu == 1) FUGU_MG_PRE(1);
-    else if (nu == 3) FUGU_MG_PRE(3);
-    else if (nu == 4) FUGU_MG_PRE(4);
-    else FUGU_MG_PRE(2);
+    if constexpr (sizeof(T) == 8) {
+      if (pre64 && nu == 3) FUGU_MG_PRE(3, 64);
+      else if (pre64) FUGU_MG_PRE(2, 64);
+      else if (nu == 1) FUGU_MG_PRE(1, MGT_TH);
+      else if (nu == 3) FUGU_MG_PRE(3, MGT_TH);
+      else if (nu == 4) FUGU_MG_PRE(4, MGT_TH);
+      else FUGU_MG_PRE(2, MGT_TH);
+    } else {
+      if (nu == 1) FUGU_MG_PRE(1, MGT_TH);
+      else if (nu == 3) FUGU_MG_PRE(3, MGT_TH);
+      else if (nu == 4) FUGU_MG_PRE(4, MGT_TH);
+      else FUGU_MG_PRE(2, MGT_TH);
+    }
 #undef FUGU_MG_PRE
   } else {
     guess_to_phi();
@@ -908,16 +925,25 @@ T* mg_vcycle(fugu_ctx* c, std::vector<MgLevel<T>>& lv, size_t l, bool zero_guess
   if (c->mg_fused) {
     // phi, coarse phi (a quarter), b, c_x, c_y (, m) read; the other buffer written
     T* Q = (P == L.phi) ? L.phi2 : L.phi;
-    const dim3 gp = mgt_grid(L.d, MGT_TH);
+    const bool post64 = th64 && nu == 3;
+    const dim3 gp = mgt_grid(L.d, post64 ? 64 : MGT_TH);
     const char* pname = mg_kname(true, nu, sizeof(T) == 8);
     const double pbytes = (L.m ? 6.25 : 5.25) * sizeof(T) * ncell;
 #define FUGU_MG_POST(NUV, THV)                                                                                      \
   TLAUNCH(c, pname, pbytes, (k_mg_post<T, NUV, THV>), gp, dim3(16 * THV), c->s, L.d, (const T*)L.cx, (const T*)L.cy, \
           (const T*)L.m, (const T*)L.b, (const T*)P, cres, Q, psum ? psum : (double*)nullptr)
-    if (nu == 1) FUGU_MG_POST(1, MGT_TH);
-    else if (nu == 3) FUGU_MG_POST(3, MGT_TH);
-    else if (nu == 4) FUGU_MG_POST(4, MGT_TH);
-    else FUGU_MG_POST(2, MGT_TH);
+    if constexpr (sizeof(T) == 8) {
+      if (post64) FUGU_MG_POST(3, 64);
+      else if (nu == 1) FUGU_MG_POST(1, MGT_TH);
+      else if (nu == 3) FUGU_MG_POST(3, MGT_TH);
+      else if (nu == 4) FUGU_MG_POST(4, MGT_TH);
+      else FUGU_MG_POST(2, MGT_TH);
+    } else {
+      if (nu == 1) FUGU_MG_POST(1, MGT_TH);
+      else if (nu == 3) FUGU_MG_POST(3, MGT_TH);
+      else if (nu == 4) FUGU_MG_POST(4, MGT_TH);
+      else FUGU_MG_POST(2, MGT_TH);
+    }
 #undef FUGU_MG_POST
     if (psum && psum_nb) *psum_nb = (int)(gp.x * gp.y);
     return Q;

@@ -160,6 +160,36 @@ def test_sweep_counts_fused_is_bitwise_the_multilaunch_cycle(monkeypatch, nu_p, 
         assert np.array_equal(a, b), (name, np.abs(a - b).max())
 
 
+def test_big_fp64_levels_fused_is_bitwise_the_multilaunch_cycle(monkeypatch):
+    """FP64 levels of 2^21 cells and more take the fused level kernels with 64-row tiles (pre- and
+    post-smoothing): one refined FP64 step solve on 1024 x 2048 cells, bit for bit the multi-launch
+    cycle's."""
+    import bench
+    nx, ny = 1024, 2048
+    outs = []
+    for cells, fused in (("0", "0"), ("-1", "1")):
+        monkeypatch.setenv("FUGU_MG_TAIL_CELLS", cells)
+        monkeypatch.setenv("FUGU_MG_FUSED", fused)
+        p = abi.FuguParams(1000.0, 1.0, 10.0, 0.1, 2.0, 0.0, -1.0, 2, 1e12, 0.5)
+        ctx = abi.Context(nx, ny, 1.0, 2.0, False, False, params=p)
+        nh = 2
+        u = np.zeros((ny + 2 * nh, nx + 1 + 2 * nh)); v = np.zeros((ny + 1 + 2 * nh, nx + 2 * nh))
+        pp = np.zeros((ny + 2 * nh, nx + 2 * nh))
+        ctx.upload(u, v, u.copy(), v.copy(), pp, bench.bubble_c(nx, ny, 1.0, 2.0))
+        ctx.init_properties()
+        flags = abi.SOLVE_REFINED | abi.PRECOND_MG | abi.PRECOND_VISC | abi.PRECOND_VMG
+        opts = abi.step_opts(tol=1e-8, max_iter=300, solver_flags=flags, inner_max_iter=-1, restart=30)
+        st, rc = ctx.step(2e-4, 1, opts)
+        f = [ctx.field(k, np.zeros(shape)) for k, shape in (("u", u.shape), ("v", v.shape), ("p", pp.shape))]
+        ctx.close()
+        outs.append(((rc, st.outer, st.res), f))
+        print(f"fused={fused}: rc {rc} outer {st.outer} res {st.res:.3e}", flush=True)
+    assert outs[0][0][0] == 0 and outs[0][0][2] <= 1e-8, outs[0][0]
+    assert outs[0][0] == outs[1][0]
+    for a, b, name in zip(outs[0][1], outs[1][1], "uvp"):
+        assert np.array_equal(a, b), (name, np.abs(a - b).max())
+
+
 def test_odd_periodic_extent_is_refused():
     """Red-black smoothing needs even periodic extents: the multigrid block says so instead of
     running a colouring whose seam neighbours share a colour."""
