@@ -728,3 +728,247 @@ void og_extend_T(const og_grid* g, double* Xu, double* Xv, double* Xp) {
   og_velocity_bcs_T(g, Xu, Xv);
   og_mask_owned(g, Xu, Xv, Xp);
 }
+
+/* ------------------------------------------------------------------------ */
+/* Transposes of the interface chain.  Each is the reverse mode of the forward routine at the
+ * given state: the forward statements are revisited in reverse order and every
+ * "dst = f(src)" becomes "src_bar += f'(src) dst_bar".  Kinks take one side: the clamp to [0, 1]
+ * passes the derivative only strictly inside (0, 1), the limiters their one-sided slope at
+ * r (van Leer: 2 / (1 + r)^2 for r > 0, else 0; Superbee: the slope of the branch the forward
+ * select takes), the upwind switch the branch the forward takes (d|u|/du = 1 at u = 0), and a
+ * face with |den| <= 1e-12 (r set to 0, psi(0) den = 0) contributes only its upwind cell. */
+
+static double clamp01_d(double x) { return (x > 0.0 && x < 1.0) ? 1.0 : 0.0; }
+static double clamp01_v(double x) {   /* Fortran max(0, min(1, x)), as og_vof_advection's clamp */
+  const double m = (x < 1.0) ? x : 1.0;
+  return (m > 0.0) ? m : 0.0;
+}
+static double lim_psi(double r, int lim) {
+  if (lim == 1) {
+    const double a = (1.0 < 2.0 * r) ? 1.0 : 2.0 * r, b = (2.0 < r) ? 2.0 : r, m = (b > a) ? b : a;
+    return (m > 0.0) ? m : 0.0;
+  }
+  return (r <= 0.0) ? 0.0 : (2.0 * r) / (1.0 + r);
+}
+static double lim_dpsi(double r, int lim) {
+  if (lim == 1) {
+    const double a = (1.0 < 2.0 * r) ? 1.0 : 2.0 * r, b = (2.0 < r) ? 2.0 : r, m = (b > a) ? b : a;
+    if (!(m > 0.0)) return 0.0;
+    if (b > a) return (2.0 < r) ? 0.0 : 1.0;
+    return (1.0 < 2.0 * r) ? 0.0 : 2.0;
+  }
+  return (r <= 0.0) ? 0.0 : 2.0 / ((1.0 + r) * (1.0 + r));
+}
+
+/* One face of og_vof_advection: c0, c1, c2 are the stencil (uf >= 0: c_{i-1}, c_i, c_{i+1};
+ * uf < 0: c_i, c_{i+1}, c_{i+2}), k = dt / h.  Returns clamp01(cf); dF/duf in *dfdu and the
+ * flux's derivatives along the stencil, times fbar, added to d[0..2]. */
+static double vof_face_T(double uf, double c0, double c1, double c2, double k, int lim, double fbar,
+                         double* ubar, double d[3]) {
+  double num, den, up, sgn;
+  if (uf >= 0.0) { num = c1 - c0; den = c2 - c1; up = c1; sgn = 1.0; }
+  else { num = c2 - c1; den = c1 - c0; up = c1; sgn = -1.0; }
+  const int big = fabs(den) > 1.0e-12;
+  const double rr = big ? num / den : 0.0;
+  const double cfl = fabs(uf) * k;
+  const double A = 0.5 * (1.0 - cfl);
+  const double psi = lim_psi(rr, lim);
+  const double cf = up + sgn * A * psi * den;
+  const double cl = clamp01_v(cf);
+  const double dcl = clamp01_d(cf);
+  /* F = uf clamp(cf); dcf/duf = -0.5 k psi den (both branches) */
+  *ubar += fbar * (cl + uf * dcl * (-0.5 * k * psi * den));
+  const double cfb = fbar * uf * dcl;
+  const double pn = big ? A * lim_dpsi(rr, lim) : 0.0;              /* d(psi den)/d num */
+  const double pd = big ? A * (psi - rr * lim_dpsi(rr, lim)) : 0.0; /* d(psi den)/d den */
+  if (uf >= 0.0) {  /* cf = c1 + A psi den, num = c1 - c0, den = c2 - c1 */
+    d[0] += cfb * (-pn);
+    d[1] += cfb * (1.0 + pn - pd);
+    d[2] += cfb * pd;
+  } else {          /* cf = c1 - A psi den, num = c2 - c1, den = c1 - c0 */
+    d[0] += cfb * pd;
+    d[1] += cfb * (1.0 + pn - pd);
+    d[2] += cfb * (-pn);
+  }
+  return cl;
+}
+
+/* Reverse mode of og_vof_advection (src/mod_vof.f90:38-140) at (c_in, u, v). */
+void og_vof_advection_T(const og_grid* g, const double* c_in, const double* u, const double* v, double dt,
+                        const double* cbar_out, double* cbar_in, double* ubar, double* vbar) {
+  DECL;
+  const int lim = og_get_vof_limiter();
+  const double inv_dx = 1.0 / g->dx, inv_dy = 1.0 / g->dy, kx = dt * inv_dx, ky = dt * inv_dy;
+  const size_t sp = og_size_p(g);
+  double* cb = zalloc(sp);
+  double* tb = zalloc(sp);
+  memcpy(cb, cbar_out, sp * sizeof(double));
+  og_halo_p_T(g, cb);                              /* og_halo_p(c_out) */
+  /* forward fluxes (for the unclamped updates) */
+  const int lfx = nx + 1;
+  double* Fx = zalloc((size_t)(nx + 1) * ny);
+  double* Fy = zalloc((size_t)nx * (ny + 1));
+#define FX(i, j) Fx[(size_t)(i) + (size_t)((j) - 1) * lfx]
+#define FY(i, j) Fy[(size_t)((i) - 1) + (size_t)(j) * nx]
+  double dummy, dd[3];
+  for (int j = 1; j <= ny; ++j)
+    for (int i = 0; i <= nx; ++i) {
+      const double uf = U(u, i, j);
+      const double cl = (uf >= 0.0) ? vof_face_T(uf, P(c_in, i - 1, j), P(c_in, i, j), P(c_in, i + 1, j), kx, lim, 0.0, &dummy, dd)
+                                    : vof_face_T(uf, P(c_in, i, j), P(c_in, i + 1, j), P(c_in, i + 2, j), kx, lim, 0.0, &dummy, dd);
+      FX(i, j) = uf * cl;
+    }
+  for (int j = 0; j <= ny; ++j)
+    for (int i = 1; i <= nx; ++i) {
+      const double vf = V(v, i, j);
+      const double cl = (vf >= 0.0) ? vof_face_T(vf, P(c_in, i, j - 1), P(c_in, i, j), P(c_in, i, j + 1), ky, lim, 0.0, &dummy, dd)
+                                    : vof_face_T(vf, P(c_in, i, j), P(c_in, i, j + 1), P(c_in, i, j + 2), ky, lim, 0.0, &dummy, dd);
+      FY(i, j) = vf * cl;
+    }
+  /* c_out = c_in outside the interior (the copy); interior: clamp01(c_in - dt div F) */
+  for (int j = 1 - nh; j <= ny + nh; ++j)
+    for (int i = 1 - nh; i <= nx + nh; ++i) {
+      if (i >= 1 && i <= nx && j >= 1 && j <= ny) {
+        const double pre = P(c_in, i, j) - dt * ((FX(i, j) - FX(i - 1, j)) * inv_dx + (FY(i, j) - FY(i, j - 1)) * inv_dy);
+        P(tb, i, j) = P(cb, i, j) * clamp01_d(pre);
+        P(cbar_in, i, j) += P(tb, i, j);
+      } else {
+        P(cbar_in, i, j) += P(cb, i, j);
+      }
+    }
+  /* fluxes: Fbar_x(i, j) = kx (tb(i+1, j) - tb(i, j)); then each face's stencil */
+  for (int j = 1; j <= ny; ++j)
+    for (int i = 0; i <= nx; ++i) {
+      const double fb = kx * (((i + 1 <= nx) ? P(tb, i + 1, j) : 0.0) - ((i >= 1) ? P(tb, i, j) : 0.0));
+      const double uf = U(u, i, j);
+      double d[3] = {0.0, 0.0, 0.0};
+      double ub = 0.0;
+      const int b = (uf >= 0.0) ? i - 1 : i;
+      vof_face_T(uf, P(c_in, b, j), P(c_in, b + 1, j), P(c_in, b + 2, j), kx, lim, fb, &ub, d);
+      U(ubar, i, j) += ub;
+      for (int q = 0; q < 3; ++q) P(cbar_in, b + q, j) += d[q];
+    }
+  for (int j = 0; j <= ny; ++j)
+    for (int i = 1; i <= nx; ++i) {
+      const double fb = ky * (((j + 1 <= ny) ? P(tb, i, j + 1) : 0.0) - ((j >= 1) ? P(tb, i, j) : 0.0));
+      const double vf = V(v, i, j);
+      double d[3] = {0.0, 0.0, 0.0};
+      double vb = 0.0;
+      const int b = (vf >= 0.0) ? j - 1 : j;
+      vof_face_T(vf, P(c_in, i, b), P(c_in, i, b + 1), P(c_in, i, b + 2), ky, lim, fb, &vb, d);
+      V(vbar, i, j) += vb;
+      for (int q = 0; q < 3; ++q) P(cbar_in, i, b + q) += d[q];
+    }
+#undef FX
+#undef FY
+  free(Fx); free(Fy); free(cb); free(tb);
+}
+
+/* Transpose of the Neumann mirror of og_svof (the copies reversed: y pass, then x pass). */
+static void mirror_p_T(const og_grid* g, double* a) {
+  DECL;
+  if (!g->per_y) {
+    for (int i = nx + nh; i >= 1 - nh; --i)
+      for (int j = nh; j >= 1; --j) { P(a, i, ny - j + 1) += P(a, i, ny + j); P(a, i, ny + j) = 0.0; }
+    for (int i = nx + nh; i >= 1 - nh; --i)
+      for (int j = nh; j >= 1; --j) { P(a, i, j) += P(a, i, 1 - j); P(a, i, 1 - j) = 0.0; }
+  }
+  if (!g->per_x) {
+    for (int j = ny + nh; j >= 1 - nh; --j)
+      for (int i = nh; i >= 1; --i) { P(a, nx - i + 1, j) += P(a, nx + i, j); P(a, nx + i, j) = 0.0; }
+    for (int j = ny + nh; j >= 1 - nh; --j)
+      for (int i = nh; i >= 1; --i) { P(a, i, j) += P(a, 1 - i, j); P(a, 1 - i, j) = 0.0; }
+  }
+}
+
+/* Reverse mode of og_svof (src/mod_vof.f90:145-194) with its sweep count fixed at `iters` (the
+ * count the forward call returned): the sweeps are linear in c_raw.  csbar: adjoint of the whole
+ * padded output; crawbar accumulates on the interior (the forward overwrites every ghost). */
+void og_svof_T(const og_grid* g, const double* csbar, double delta, int iters, double* crawbar) {
+  DECL;
+  const double cx = (delta / g->dx) * (delta / g->dx);
+  const double cy = (delta / g->dy) * (delta / g->dy);
+  const double diag = 1.0 + 2.0 * (cx + cy);
+  const size_t sp = og_size_p(g);
+  double* sb = zalloc(sp);
+  double* ob = zalloc(sp);
+  memcpy(sb, csbar, sp * sizeof(double));
+  mirror_p_T(g, sb);   /* the final halo + mirror */
+  og_halo_p_T(g, sb);
+  for (int it = iters; it >= 1; --it) {
+    /* c_smooth(int) = (c_raw + cx (old E + old W) + cy (old N + old S)) / diag */
+    memset(ob, 0, sp * sizeof(double));
+    for (int j = 1; j <= ny; ++j)
+      for (int i = 1; i <= nx; ++i) {
+        const double t = P(sb, i, j) / diag;
+        P(crawbar, i, j) += t;
+        P(ob, i + 1, j) += cx * t; P(ob, i - 1, j) += cx * t;
+        P(ob, i, j + 1) += cy * t; P(ob, i, j - 1) += cy * t;
+        P(sb, i, j) = 0.0;
+      }
+    /* c_old = c_smooth (whole array), after the halo exchange and the mirror */
+    for (size_t q = 0; q < sp; ++q) sb[q] += ob[q];
+    mirror_p_T(g, sb);
+    og_halo_p_T(g, sb);
+  }
+  /* c_smooth = c_raw (whole array): its ghosts are overwritten before any use */
+  for (int j = 1; j <= ny; ++j)
+    for (int i = 1; i <= nx; ++i) P(crawbar, i, j) += P(sb, i, j);
+  free(sb); free(ob);
+}
+
+/* Reverse mode of og_curvature (src/mod_surface_tension.f90:25-72) at cs. */
+void og_curvature_T(const og_grid* g, const double* cs, const double* kbar, double* csbar) {
+  DECL;
+  const double inv_dx = 1.0 / g->dx, inv_dy = 1.0 / g->dy, eps = 1.0e-12;
+  const size_t sp = og_size_p(g);
+  double* kb = zalloc(sp);
+  memcpy(kb, kbar, sp * sizeof(double));
+  og_halo_p_T(g, kb);   /* og_halo_p(kc); other ghosts stay zero: no dependence */
+  for (int j = 1; j <= ny; ++j)
+    for (int i = 0; i <= nx; ++i) {
+      const double nb = ((i + 1 <= nx) ? P(kb, i + 1, j) : 0.0) * inv_dx - ((i >= 1) ? P(kb, i, j) : 0.0) * inv_dx;
+      const double gx = (P(cs, i + 1, j) - P(cs, i, j)) * inv_dx;
+      const double gy = 0.25 * (P(cs, i, j + 1) + P(cs, i + 1, j + 1) - P(cs, i, j - 1) - P(cs, i + 1, j - 1)) * inv_dy;
+      const double n2 = gx * gx + gy * gy + eps, ng = sqrt(n2), n3 = n2 * ng;
+      const double gxb = nb * (gy * gy + eps) / n3, gyb = -nb * gx * gy / n3;
+      P(csbar, i + 1, j) += gxb * inv_dx; P(csbar, i, j) -= gxb * inv_dx;
+      const double t = 0.25 * gyb * inv_dy;
+      P(csbar, i, j + 1) += t; P(csbar, i + 1, j + 1) += t; P(csbar, i, j - 1) -= t; P(csbar, i + 1, j - 1) -= t;
+    }
+  for (int j = 0; j <= ny; ++j)
+    for (int i = 1; i <= nx; ++i) {
+      const double nb = ((j + 1 <= ny) ? P(kb, i, j + 1) : 0.0) * inv_dy - ((j >= 1) ? P(kb, i, j) : 0.0) * inv_dy;
+      const double gx = 0.25 * (P(cs, i + 1, j) + P(cs, i + 1, j + 1) - P(cs, i - 1, j) - P(cs, i - 1, j + 1)) * inv_dx;
+      const double gy = (P(cs, i, j + 1) - P(cs, i, j)) * inv_dy;
+      const double n2 = gx * gx + gy * gy + eps, ng = sqrt(n2), n3 = n2 * ng;
+      const double gyb = nb * (gx * gx + eps) / n3, gxb = -nb * gx * gy / n3;
+      P(csbar, i, j + 1) += gyb * inv_dy; P(csbar, i, j) -= gyb * inv_dy;
+      const double t = 0.25 * gxb * inv_dx;
+      P(csbar, i + 1, j) += t; P(csbar, i + 1, j + 1) += t; P(csbar, i - 1, j) -= t; P(csbar, i - 1, j + 1) -= t;
+    }
+  free(kb);
+}
+
+/* Reverse mode of og_csf (src/mod_surface_tension.f90:78-109) at (c_raw, kc). */
+void og_csf_T(const og_grid* g, const double* c_raw, const double* kc, double sigma, const double* fubar,
+              const double* fvbar, double* cbar, double* kbar) {
+  DECL;
+  const double inv_dx = 1.0 / g->dx, inv_dy = 1.0 / g->dy;
+  for (int j = 1; j <= ny; ++j)
+    for (int i = 0; i <= nx; ++i) {
+      const double fb = U(fubar, i, j);
+      const double kf = 0.5 * (P(kc, i, j) + P(kc, i + 1, j));
+      const double d = P(c_raw, i + 1, j) - P(c_raw, i, j);
+      P(cbar, i + 1, j) += sigma * kf * inv_dx * fb; P(cbar, i, j) -= sigma * kf * inv_dx * fb;
+      P(kbar, i, j) += 0.5 * sigma * d * inv_dx * fb; P(kbar, i + 1, j) += 0.5 * sigma * d * inv_dx * fb;
+    }
+  for (int j = 0; j <= ny; ++j)
+    for (int i = 1; i <= nx; ++i) {
+      const double fb = V(fvbar, i, j);
+      const double kf = 0.5 * (P(kc, i, j) + P(kc, i, j + 1));
+      const double d = P(c_raw, i, j + 1) - P(c_raw, i, j);
+      P(cbar, i, j + 1) += sigma * kf * inv_dy * fb; P(cbar, i, j) -= sigma * kf * inv_dy * fb;
+      P(kbar, i, j) += 0.5 * sigma * d * inv_dy * fb; P(kbar, i, j + 1) += 0.5 * sigma * d * inv_dy * fb;
+    }
+}

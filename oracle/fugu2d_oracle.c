@@ -871,6 +871,7 @@ double og_superbee(double r) {
 /* The face limiter of og_vof_advection: 0 van Leer (the reference), 1 Superbee. */
 static int g_vof_limiter = 0;
 void og_set_vof_limiter(int lim) { g_vof_limiter = lim; }
+int og_get_vof_limiter(void) { return g_vof_limiter; }
 static double vof_psi(double r) { return g_vof_limiter == 1 ? og_superbee(r) : og_van_leer(r); }
 
 /* src/mod_vof.f90:38-140 */

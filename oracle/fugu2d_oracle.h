@@ -214,6 +214,15 @@ void og_cell_properties(const og_grid* g, const double* mu_cell, const double* r
 void og_cell_properties_T(const og_grid* g, const double* mu_cell, double* g_rho_c, double* g_rho_u,
                           double* g_rho_v, double* g_kappa, double* g_zeta, double* g_eta,
                           double* g_mu, double* g_rho);
+/* Transposes of the interface chain (reverse mode of og_vof_advection, og_svof, og_curvature and
+ * og_csf at a given state; every output accumulates: callers zero it first). */
+int og_get_vof_limiter(void);
+void og_vof_advection_T(const og_grid* g, const double* c_in, const double* u, const double* v, double dt,
+                        const double* cbar_out, double* cbar_in, double* ubar, double* vbar);
+void og_svof_T(const og_grid* g, const double* csbar, double delta, int iters, double* crawbar);
+void og_curvature_T(const og_grid* g, const double* cs, const double* kbar, double* csbar);
+void og_csf_T(const og_grid* g, const double* c_raw, const double* kc, double sigma, const double* fubar,
+              const double* fvbar, double* cbar, double* kbar);
 
 #ifdef __cplusplus
 }

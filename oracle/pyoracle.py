@@ -116,6 +116,11 @@ def _declare(L):
     s("og_operator_coef_grad", [_G] + [_D] * 12 + [C.c_double, C.c_int, C.c_double] + [_D] * 8)
     s("og_cell_properties", [_G] + [_D] * 8)
     s("og_cell_properties_T", [_G] + [_D] * 9)
+    s("og_get_vof_limiter", [], C.c_int)
+    s("og_vof_advection_T", [_G, _D, _D, _D, C.c_double, _D, _D, _D, _D])
+    s("og_svof_T", [_G, _D, C.c_double, C.c_int, _D])
+    s("og_curvature_T", [_G, _D, _D, _D])
+    s("og_csf_T", [_G, _D, _D, C.c_double, _D, _D, _D, _D])
 
 
 def ptr(a):
@@ -308,6 +313,31 @@ def csf(g, c_raw, kc, sigma):
     fu, fv = g.zeros("u"), g.zeros("v")
     lib().og_csf(C.byref(g.c), ptr(c_raw), ptr(kc), sigma, ptr(fu), ptr(fv))
     return fu, fv
+
+
+# Transposes of the interface chain (reverse mode at the given state; fresh outputs).
+def vof_advection_T(g, c_in, u, v, dt, cbar_out):
+    cb, ub, vb = g.zeros("p"), g.zeros("u"), g.zeros("v")
+    lib().og_vof_advection_T(C.byref(g.c), ptr(c_in), ptr(u), ptr(v), dt, ptr(cbar_out), ptr(cb), ptr(ub), ptr(vb))
+    return cb, ub, vb
+
+
+def svof_T(g, csbar, delta, iters):
+    cb = g.zeros("p")
+    lib().og_svof_T(C.byref(g.c), ptr(csbar), delta, int(iters), ptr(cb))
+    return cb
+
+
+def curvature_T(g, cs, kbar):
+    sb = g.zeros("p")
+    lib().og_curvature_T(C.byref(g.c), ptr(cs), ptr(kbar), ptr(sb))
+    return sb
+
+
+def csf_T(g, c_raw, kc, sigma, fubar, fvbar):
+    cb, kb = g.zeros("p"), g.zeros("p")
+    lib().og_csf_T(C.byref(g.c), ptr(c_raw), ptr(kc), sigma, ptr(fubar), ptr(fvbar), ptr(cb), ptr(kb))
+    return cb, kb
 
 
 class State:
