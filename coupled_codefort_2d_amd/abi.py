@@ -59,6 +59,7 @@ ADJ_FORWARD_ONLY = 1
 ADJ_REUSE_INTERFACE = 2
 ADJ_PRECOND_VMG = 4
 ADJ_MIXED_FORWARD = 8
+ADJ_MOVING_INTERFACE = 16
 
 B2_CONSISTENT_BCS = 1
 SOLVE_REFINED = 2
@@ -86,6 +87,7 @@ EXPORTS = [
     "fugu_solve_pressure_mg", "fugu_matvec_transpose", "fugu_apply_pcd_transpose", "fugu_adjoint_solve",
     "fugu_cell_properties", "fugu_adjoint_gradient", "fugu_coupled_dot_noalloc",
     "fugu_checkpoint_write", "fugu_checkpoint_read", "fugu_write_vtk",
+    "fugu_vof_advection_transpose", "fugu_svof_transpose", "fugu_curvature_transpose", "fugu_csf_transpose",
 ]
 
 _lib = None
@@ -152,6 +154,10 @@ def _declare(L):
     s("fugu_solve_pressure_mg", [P, _D, _D, _D, _D, C.c_int])
     s("fugu_kernel_time", [P, C.c_char_p, _D, C.POINTER(C.c_longlong), _D])
     s("fugu_matvec_transpose", [P] + [_D] * 14 + [C.c_double, C.c_int])
+    s("fugu_vof_advection_transpose", [P, _D, _D, _D, C.c_double, _D, _D, _D, _D])
+    s("fugu_svof_transpose", [P, _D, C.c_double, C.c_int, C.c_double, _D, _D, _I])
+    s("fugu_curvature_transpose", [P, _D, _D, _D])
+    s("fugu_csf_transpose", [P, _D, _D, C.c_double, _D, _D, _D, _D])
     s("fugu_apply_pcd_transpose", [P] + [_D] * 13 + [C.c_double, C.c_double, C.c_int, C.c_double])
     s("fugu_adjoint_solve", [P] + [_D] * 14 + [C.c_double, C.c_double, C.c_int, _I, _D])
     s("fugu_cell_properties", [P] + [_D] * 8)
@@ -343,6 +349,24 @@ class Context:
 
     def csf(self, c_raw, kc, fu, fv):
         check(load().fugu_compute_csf_force(self.h, ptr(c_raw), ptr(kc), ptr(fu), ptr(fv)), "csf")
+
+    # transposes of the interface chain (outputs overwritten)
+    def vof_advection_T(self, c_in, u, v, dt, cbar_out, cbar_in, ubar, vbar):
+        check(load().fugu_vof_advection_transpose(self.h, ptr(c_in), ptr(u), ptr(v), dt, ptr(cbar_out), ptr(cbar_in),
+                                                  ptr(ubar), ptr(vbar)), "vof_T")
+
+    def svof_T(self, c_raw, delta, csbar, crawbar, max_iter=100, tol=1e-6):
+        it = C.c_int(0)
+        check(load().fugu_svof_transpose(self.h, ptr(c_raw), delta, max_iter, tol, ptr(csbar), ptr(crawbar),
+                                         C.byref(it)), "svof_T")
+        return it.value
+
+    def curvature_T(self, cs, kbar, csbar):
+        check(load().fugu_curvature_transpose(self.h, ptr(cs), ptr(kbar), ptr(csbar)), "curvature_T")
+
+    def csf_T(self, c_raw, kc, sigma, fubar, fvbar, cbar, kbar):
+        check(load().fugu_csf_transpose(self.h, ptr(c_raw), ptr(kc), sigma, ptr(fubar), ptr(fvbar), ptr(cbar),
+                                        ptr(kbar)), "csf_T")
 
     def mixture(self, c, rho_c, rho_u, rho_v, kappa, zeta, eta):
         check(load().fugu_compute_mixture_densities(self.h, ptr(c), ptr(rho_c), ptr(rho_u), ptr(rho_v)), "dens")

@@ -9,6 +9,7 @@
 #include "fugu2d_io.hpp"
 #include "fugu2d_mg.hpp"
 #include "fugu2d_adjoint.hpp"
+#include "fugu2d_iface_adj.hpp"
 
 #include <cmath>
 #include <cstdlib>
@@ -254,7 +255,16 @@ struct fugu_ctx {
     double *gmu = nullptr, *grho = nullptr;
     bool ready = false;
   } aw;
+  // scratch of the interface-chain transposes (VOF, SVOF, curvature, CSF)
+  struct IfaceWork {
+    double *cb = nullptr, *tb = nullptr, *kb = nullptr, *sb0 = nullptr, *sb1 = nullptr, *acc = nullptr;
+    double *csb = nullptr, *cbar = nullptr, *cbar2 = nullptr, *kbar = nullptr;
+    double *gux = nullptr, *guy = nullptr, *ubar = nullptr, *fub = nullptr, *dx3[3] = {nullptr, nullptr, nullptr};
+    double *gvx = nullptr, *gvy = nullptr, *vbar = nullptr, *fvb = nullptr, *dy3[3] = {nullptr, nullptr, nullptr};
+    bool ready = false;
+  } iw;
   std::vector<double*> traj_u, traj_v, traj_fu, traj_fv;   // forward trajectory (or window) of the adjoint horizon
+  std::vector<double*> traj_c;                             // ... and its volume fractions (moving interface)
   struct Checkpoint {
     double *u1 = nullptr, *u2 = nullptr, *v1 = nullptr, *v2 = nullptr, *p = nullptr, *c = nullptr;
   };
@@ -1352,6 +1362,73 @@ void extend_T(fugu_ctx* c, double* X) {
   if (!g.per_x) LAUNCH(k_bcs_T_x, dim3(nblk((size_t)g.ny + 1 + 2 * g.nh)), dim3(256), c->s, g, u, v);
   LAUNCH(k_mask_trip, grid_union(g), dim3(TX, TY), c->s, g, u, v, p);
 }
+// ---------------------------------------------------------------------------------
+// Transposes of the interface chain (no counterpart in the reference): the reverse mode of
+// vof(), svof() at a given sweep count, curvature() and csf() at a forward state.
+void iw_alloc(fugu_ctx* c) {
+  auto& w = c->iw;
+  if (w.ready) return;
+  const Grid& g = c->g;
+  for (double** a : {&w.cb, &w.tb, &w.kb, &w.sb0, &w.sb1, &w.acc, &w.csb, &w.cbar, &w.cbar2, &w.kbar})
+    *a = dalloc<double>(c, g.np);
+  for (double** a : {&w.gux, &w.guy, &w.ubar, &w.fub, &w.dx3[0], &w.dx3[1], &w.dx3[2]}) *a = dalloc<double>(c, g.nu);
+  for (double** a : {&w.gvx, &w.gvy, &w.vbar, &w.fvb, &w.dy3[0], &w.dy3[1], &w.dy3[2]}) *a = dalloc<double>(c, g.nv);
+  w.ready = true;
+}
+// cbar_in = (dVOF/dc_in)^T cbar_out; ubar, vbar = (dVOF/du)^T cbar_out, (dVOF/dv)^T cbar_out
+// (all three overwritten; cbar_out is not modified).
+void vof_T(fugu_ctx* c, const double* cin, const double* u, const double* v, double dt, const double* cbar_out,
+           double* cbar_in, double* ubar, double* vbar) {
+  iw_alloc(c);
+  auto& w = c->iw;
+  const Grid& g = c->g;
+  const dim3 gu = grid_union(g), bl(TX, TY);
+  HIPCHK(hipMemcpyAsync(w.cb, cbar_out, g.np * sizeof(double), hipMemcpyDeviceToDevice, c->s));
+  halo_T(c, w.cb, 2);
+  LAUNCH(k_vof_T_cells, gu, bl, c->s, g, cin, u, v, dt, c->vof_limiter, (const double*)w.cb, w.tb, cbar_in);
+  VofTFaces f;
+  f.ubar = ubar; f.vbar = vbar;
+  for (int q = 0; q < 3; ++q) { f.dx3[q] = w.dx3[q]; f.dy3[q] = w.dy3[q]; }
+  LAUNCH(k_vof_T_faces, gu, bl, c->s, g, cin, u, v, dt, c->vof_limiter, (const double*)w.tb, f);
+  LAUNCH(k_vof_T_gather, gu, bl, c->s, g, u, v, f, cbar_in);
+}
+// crawbar (interior) += (dSVOF/dc_raw)^T csbar at `iters` sweeps.
+void svof_T(fugu_ctx* c, const double* csbar, double delta, int iters, double* crawbar) {
+  iw_alloc(c);
+  auto& w = c->iw;
+  const Grid& g = c->g;
+  const dim3 gu = grid_union(g), bl(TX, TY);
+  const double cx = (delta / g.dx) * (delta / g.dx);
+  const double cy = (delta / g.dy) * (delta / g.dy);
+  const double diag = 1.0 + 2.0 * (cx + cy);
+  LAUNCH(k_fill_T_p, gu, bl, c->s, g, csbar, w.sb0);
+  LAUNCH(k_axpby, dim3(nblk(g.np)), dim3(256), c->s, w.acc, (const double*)nullptr, 0.0, (const double*)nullptr, 0.0,
+         (size_t)g.np);
+  double *a = w.sb0, *b = w.sb1;
+  for (int it = iters; it >= 1; --it) {
+    LAUNCH(k_svof_T_sweep, gu, bl, c->s, g, (const double*)a, b, w.acc, cx, cy, diag);
+    std::swap(a, b);
+  }
+  LAUNCH(k_svof_T_final, gu, bl, c->s, g, (const double*)w.acc, (const double*)a, crawbar);
+}
+// csbar += (dcurvature/dcs)^T kbar.
+void curv_T(fugu_ctx* c, const double* cs, const double* kbar, double* csbar) {
+  iw_alloc(c);
+  auto& w = c->iw;
+  const Grid& g = c->g;
+  const dim3 gu = grid_union(g), bl(TX, TY);
+  HIPCHK(hipMemcpyAsync(w.kb, kbar, g.np * sizeof(double), hipMemcpyDeviceToDevice, c->s));
+  halo_T(c, w.kb, 2);
+  LAUNCH(k_curv_T_faces, gu, bl, c->s, g, cs, (const double*)w.kb, w.gux, w.guy, w.gvx, w.gvy);
+  LAUNCH(k_curv_T_cells, gu, bl, c->s, g, (const double*)w.gux, (const double*)w.guy, (const double*)w.gvx,
+         (const double*)w.gvy, csbar);
+}
+// cbar += (dCSF/dc_raw)^T fbar; kbar = (dCSF/dk)^T fbar.
+void csf_T(fugu_ctx* c, const double* craw, const double* kc, double sigma, const double* fub, const double* fvb,
+           double* cbar, double* kbar) {
+  LAUNCH(k_csf_T, grid_union(c->g), dim3(TX, TY), c->s, c->g, craw, kc, sigma, fub, fvb, cbar, kbar);
+}
+
 void mask_trip(fugu_ctx* c, double* X) {
   const Grid& g = c->g;
   LAUNCH(k_mask_trip, grid_union(g), dim3(TX, TY), c->s, g, X, X + g.nu, X + g.nu + g.nv);
@@ -1950,10 +2027,15 @@ int adjoint_run(fugu_ctx* c, const fugu_adjoint_opts* o, const double* mu_in, co
   const bool reuse = (o->flags & FUGU_ADJ_REUSE_INTERFACE) != 0;
   const bool fwd_only = (o->flags & FUGU_ADJ_FORWARD_ONLY) != 0;
   const bool mixed = (o->flags & FUGU_ADJ_MIXED_FORWARD) != 0;
+  // the interface motion differentiated too (VOF, SVOF, curvature and CSF transposed in the reverse
+  // sweep); nothing to differentiate when sigma is zero (the interface exerts no force)
+  const bool moving = (o->flags & FUGU_ADJ_MOVING_INTERFACE) != 0 && sigma > 1.0e-14;
   // checkpointing: every K-th step's restart state kept, a window of W states recomputed per
   // segment in the reverse sweep (K <= 1: the whole horizon stays resident)
   const int K = (o->checkpoint_every > 1 && o->checkpoint_every < N) ? o->checkpoint_every : 0;
   if (K && reuse) throw Error("adjoint: FUGU_ADJ_REUSE_INTERFACE needs the whole horizon resident (no checkpointing)");
+  if ((o->flags & FUGU_ADJ_MOVING_INTERFACE) && reuse)
+    throw Error("adjoint: FUGU_ADJ_MOVING_INTERFACE differentiates the interface motion; it cannot reuse a frozen one");
   const int W = K ? K + 5 : N + 1;
   auto slot = [&](int n) { return K ? n % W : n; };
   if (reuse && c->traj_n != N) throw Error("adjoint: no recorded interface of this horizon to reuse");
@@ -1973,6 +2055,11 @@ int adjoint_run(fugu_ctx* c, const fugu_adjoint_opts* o, const double* mu_in, co
     q.v2 = dalloc<double>(c, nv); q.p = dalloc<double>(c, g.np); q.c = dalloc<double>(c, g.np);
     c->cp.push_back(q);
   }
+  if (moving) {
+    while ((int)c->traj_c.size() < W) c->traj_c.push_back(dalloc<double>(c, g.np));
+    iw_alloc(c);
+  }
+  auto CT = [&](int n) { return c->traj_c[slot(n)]; };   // C^n (moving interface)
   auto Us = [&](int n) { return c->traj_u[slot(n)]; };
   auto Vs = [&](int n) { return c->traj_v[slot(n)]; };
   auto FUs = [&](int n) { return c->traj_fu[slot(n)]; };
@@ -1995,6 +2082,7 @@ int adjoint_run(fugu_ctx* c, const fugu_adjoint_opts* o, const double* mu_in, co
   in(c, Us(0), u0, nu); in(c, Vs(0), v0, nv);
   bcs(c, Us(0), Vs(0), 0.0); halo(c, Us(0), 0); halo(c, Vs(0), 1);
   if (!reuse) in(c, c->c, c0, g.np);
+  if (moving) HIPCHK(hipMemcpyAsync(CT(0), c->c, g.np * sizeof(double), hipMemcpyDeviceToDevice, c->s));
   HIPCHK(hipMemcpyAsync(c->u, Us(0), nu * sizeof(double), hipMemcpyDeviceToDevice, c->s));
   HIPCHK(hipMemcpyAsync(c->v, Vs(0), nv * sizeof(double), hipMemcpyDeviceToDevice, c->s));
   zero(c, c->p, g.np);
@@ -2021,6 +2109,7 @@ int adjoint_run(fugu_ctx* c, const fugu_adjoint_opts* o, const double* mu_in, co
     if (!reuse) {
       vof(c, c->c, c->cnew, Us(n - 1), Vs(n - 1), dt, nullptr, nullptr);
       std::swap(c->c, c->cnew);
+      if (moving) cpy(CT(n), c->c, g.np);
       if (sigma > 1.0e-14) {
         const double delta = 1.5 * (g.dx < g.dy ? g.dx : g.dy);
         svof(c, c->c, c->cs, delta, 150, 1.0e-7, nullptr);
@@ -2071,6 +2160,7 @@ int adjoint_run(fugu_ctx* c, const fugu_adjoint_opts* o, const double* mu_in, co
     cpy(Us(n - 1), q.u1, nu); cpy(Vs(n - 1), q.v1, nv);
     if (n >= 2) { cpy(Us(n - 2), q.u2, nu); cpy(Vs(n - 2), q.v2, nv); }
     cpy(c->u, q.u1, nu); cpy(c->v, q.v1, nv); cpy(c->p, q.p, g.np); cpy(c->c, q.c, g.np);
+    if (moving) cpy(CT(n - 1), q.c, g.np);
   };
   HIPCHK(hipEventRecord(c->ev[0], c->s));
   for (int n = 1; n <= N; ++n) {
@@ -2139,6 +2229,16 @@ int adjoint_run(fugu_ctx* c, const fugu_adjoint_opts* o, const double* mu_in, co
         acc_prod(c, G, -cadv, nullptr, a.ga, nullptr, nu);
         acc_prod(c, G + nu, -cadv, nullptr, a.ga + nu, nullptr, nv);
       }
+      // moving interface: the velocity u^n advected C^n into C^(n+1) (VOF of step n + 1, transposed
+      // at the previous iteration); u^n = E X^n, so E^T of that velocity adjoint joins G
+      if (moving && n < N) {
+        zero(c, a.ga, n3);
+        cpy(a.ga, c->iw.ubar, nu);
+        cpy(a.ga + nu, c->iw.vbar, nv);
+        extend_T(c, a.ga);
+        acc_prod(c, G, 1.0, nullptr, a.ga, nullptr, nu);
+        acc_prod(c, G + nu, 1.0, nullptr, a.ga + nu, nullptr, nv);
+      }
       mask_trip(c, G);
       // A_n^T lam_n = G
       set_uadv(n);
@@ -2171,6 +2271,26 @@ int adjoint_run(fugu_ctx* c, const fugu_adjoint_opts* o, const double* mu_in, co
       cdot1<double>(c, Ln, a.t2, PH_STORE);
       read_slots(c);
       gs += c->h_sc[S_DOT0] / vol;
+      if (moving) {
+        // dJ/dfhat^n = sigma lam_n: through CSF (C^n itself and the curvature), the curvature
+        // (of the filtered C^n) and the SVOF filter into Cbar^n, which already holds what step n + 1's
+        // advection sent back; then the advection of step n (from C^(n-1) with u^(n-1)) transposed:
+        // Cbar^(n-1) and the velocity adjoint for the next iteration's G
+        auto& w = c->iw;
+        if (n == N) zero(c, w.cbar, g.np);
+        const double delta = 1.5 * (g.dx < g.dy ? g.dx : g.dy);
+        int its = 0;
+        svof(c, CT(n), c->cs, delta, 150, 1.0e-7, &its);
+        curvature(c, c->cs, c->kcurv);
+        axpby(c, w.fub, Ln, sigma, nullptr, 0.0, nu);
+        axpby(c, w.fvb, Ln + nu, sigma, nullptr, 0.0, nv);
+        csf_T(c, CT(n), c->kcurv, 1.0, w.fub, w.fvb, w.cbar, w.kbar);
+        zero(c, w.csb, g.np);
+        curv_T(c, c->cs, w.kbar, w.csb);
+        svof_T(c, w.csb, delta, its, w.cbar);
+        vof_T(c, CT(n - 1), Us(n - 1), Vs(n - 1), dt, w.cbar, w.cbar2, w.ubar, w.vbar);
+        std::swap(w.cbar, w.cbar2);
+      }
     }
     c->adjoint = false;
     // the property map
@@ -3062,6 +3182,69 @@ int fugu_advance_coupled_step(fugu_ctx* c, double* u, double* v, double* p, doub
 }
 
 // ---- adjoint (no counterpart in the reference) --------------------------------------
+int fugu_vof_advection_transpose(fugu_ctx* c, const double* cin, const double* u, const double* v, double dt,
+                                 const double* cbar_out, double* cbar_in, double* ubar, double* vbar) {
+  NEED(c);
+  if (!cin || !u || !v || !cbar_out) { g_err = "null input"; return -2; }
+  return guarded([&]() {
+    const Grid& g = c->g;
+    in(c, c->sp[0], cin, g.np); in(c, c->su[0], u, g.nu); in(c, c->sv[0], v, g.nv); in(c, c->sp[1], cbar_out, g.np);
+    vof_T(c, c->sp[0], c->su[0], c->sv[0], dt, c->sp[1], c->sp[2], c->su[1], c->sv[1]);
+    out(c, cbar_in, c->sp[2], g.np); out(c, ubar, c->su[1], g.nu); out(c, vbar, c->sv[1], g.nv);
+    sync(c);
+    return 0;
+  });
+}
+
+int fugu_svof_transpose(fugu_ctx* c, const double* craw, double delta, int max_iter, double tol,
+                        const double* csbar, double* crawbar, int* iter_out) {
+  NEED(c);
+  if (!craw || !csbar) { g_err = "null input"; return -2; }
+  return guarded([&]() {
+    const Grid& g = c->g;
+    if (max_iter <= 0) max_iter = 100;
+    if (tol <= 0) tol = 1.0e-6;
+    in(c, c->sp[0], craw, g.np); in(c, c->sp[2], csbar, g.np);
+    int it = 0;
+    svof(c, c->sp[0], c->sp[1], delta, max_iter, tol, &it);   // the forward's sweep count
+    zero(c, c->sp[3], g.np);
+    svof_T(c, c->sp[2], delta, it, c->sp[3]);
+    out(c, crawbar, c->sp[3], g.np);
+    sync(c);
+    if (iter_out) *iter_out = it;
+    return 0;
+  });
+}
+
+int fugu_curvature_transpose(fugu_ctx* c, const double* cs, const double* kbar, double* csbar) {
+  NEED(c);
+  if (!cs || !kbar) { g_err = "null input"; return -2; }
+  return guarded([&]() {
+    const Grid& g = c->g;
+    in(c, c->sp[0], cs, g.np); in(c, c->sp[1], kbar, g.np);
+    zero(c, c->sp[2], g.np);
+    curv_T(c, c->sp[0], c->sp[1], c->sp[2]);
+    out(c, csbar, c->sp[2], g.np);
+    sync(c);
+    return 0;
+  });
+}
+
+int fugu_csf_transpose(fugu_ctx* c, const double* craw, const double* kc, double sigma, const double* fubar,
+                       const double* fvbar, double* cbar, double* kbar) {
+  NEED(c);
+  if (!craw || !kc || !fubar || !fvbar) { g_err = "null input"; return -2; }
+  return guarded([&]() {
+    const Grid& g = c->g;
+    in(c, c->sp[0], craw, g.np); in(c, c->sp[1], kc, g.np); in(c, c->su[0], fubar, g.nu); in(c, c->sv[0], fvbar, g.nv);
+    zero(c, c->sp[2], g.np);
+    csf_T(c, c->sp[0], c->sp[1], sigma, c->su[0], c->sv[0], c->sp[2], c->sp[3]);
+    out(c, cbar, c->sp[2], g.np); out(c, kbar, c->sp[3], g.np);
+    sync(c);
+    return 0;
+  });
+}
+
 int fugu_matvec_transpose(fugu_ctx* c, const double* Lu, const double* Lv, const double* Lp, double* Ou, double* Ov,
                           double* Op, const double* uadv, const double* vadv, const double* rho_c,
                           const double* rho_u, const double* rho_v, const double* kap, const double* zeta,

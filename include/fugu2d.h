@@ -310,6 +310,26 @@ int fugu_adjoint_solve(fugu_ctx* ctx, double* L_u, double* L_v, double* L_p,
                        const double* rho_c, const double* rho_u, const double* rho_v,
                        const double* kappa, const double* zeta, const double* eta,
                        double dt_step, double tol, int max_iter, int* iter_out, double* res_out);
+/* Transposes of the interface chain (reverse mode at the given state; the kinks take one side:
+ * the clamp to [0, 1] passes the derivative only strictly inside, the limiters their one-sided
+ * slope, the upwind switch the branch the forward takes, a face with |den| <= 1e-12 only its
+ * upwind cell).  Outputs are whole padded arrays, overwritten.
+ * (dVOF/dC_in)^T, (dVOF/du)^T, (dVOF/dv)^T of fugu_vof_advection_tvd (src/mod_vof.f90:38-140,
+ * the context's limiter) applied to cbar_out. */
+int fugu_vof_advection_transpose(fugu_ctx* ctx, const double* c_in, const double* u, const double* v,
+                                 double dt_step, const double* cbar_out, double* cbar_in, double* ubar,
+                                 double* vbar);
+/* (dSVOF/dC_raw)^T csbar of fugu_svof_helmholtz_filter (src/mod_vof.f90:145-194) at the sweep count
+ * its forward run on c_raw stops at (returned in iter_out): the sweeps are linear in C_raw.
+ * max_iter <= 0: 100; tol <= 0: 1e-6. */
+int fugu_svof_transpose(fugu_ctx* ctx, const double* c_raw, double delta, int max_iter, double tol,
+                        const double* csbar, double* crawbar, int* iter_out);
+/* (dkappa/dC_smooth)^T kbar of fugu_compute_interface_curvature (src/mod_surface_tension.f90:25-72). */
+int fugu_curvature_transpose(fugu_ctx* ctx, const double* c_smooth, const double* kbar, double* csbar);
+/* (dF/dC_raw)^T and (dF/dkappa)^T of fugu_compute_csf_force (src/mod_surface_tension.f90:78-109)
+ * applied to (fubar, fvbar). */
+int fugu_csf_transpose(fugu_ctx* ctx, const double* c_raw, const double* kappa_curv, double sigma,
+                       const double* fubar, const double* fvbar, double* cbar, double* kbar);
 /* Per-cell property path (the configurations with per-inclusion properties): mu_cell, rho_cell
  * over the padded cells -> the operator coefficients with the edge conventions of
  * src/mod_viscosity.f90:54-151; vertex viscosity 4 / sum(1/mu_k), which equals
@@ -343,7 +363,13 @@ enum {
   FUGU_ADJ_PRECOND_VMG = 1u << 2,
   /* forward solves by the mixed form (FP32 Arnoldi cycles, FP64 true residual and update at
    * every restart: FUGU_SOLVE_MIXED); the adjoint solves stay FP64 */
-  FUGU_ADJ_MIXED_FORWARD = 1u << 3
+  FUGU_ADJ_MIXED_FORWARD = 1u << 3,
+  /* differentiate the interface motion too: the VOF advection (its limiter and clamps taken one
+   * sided at their kinks), the SVOF filter at each step's sweep count, the curvature and the CSF
+   * force are transposed in the reverse sweep, so the gradient sees how the parameters move the
+   * interface and, through the surface force, the flow (otherwise the interface is held frozen);
+   * not with FUGU_ADJ_REUSE_INTERFACE */
+  FUGU_ADJ_MOVING_INTERFACE = 1u << 4
 };
 typedef struct {
   double J;
@@ -354,7 +380,7 @@ typedef struct {
 /* Gradient of J with respect to the per-cell viscosity and density (padded p-kind arrays)
  * and the surface tension, over a BDF2 horizon from (u0, v0, C0), properties held at the
  * per-cell fields, interface motion frozen in the adjoint (VOF, SVOF and CSF of the forward
- * pass recorded and held fixed), all solves FP64 to opts->tol.  g_mu, g_rho, g_sigma may be
+ * pass recorded and held fixed) unless FUGU_ADJ_MOVING_INTERFACE, all solves FP64 to opts->tol.  g_mu, g_rho, g_sigma may be
  * NULL with FUGU_ADJ_FORWARD_ONLY; c0 may be NULL with FUGU_ADJ_REUSE_INTERFACE. */
 int fugu_adjoint_gradient(fugu_ctx* ctx, const fugu_adjoint_opts* opts, const double* mu_cell,
                           const double* rho_cell, double sigma, const double* c0, const double* u0,

@@ -761,16 +761,17 @@ static double lim_dpsi(double r, int lim) {
 }
 
 /* One face of og_vof_advection: c0, c1, c2 are the stencil (uf >= 0: c_{i-1}, c_i, c_{i+1};
- * uf < 0: c_i, c_{i+1}, c_{i+2}), k = dt / h.  Returns clamp01(cf); dF/duf in *dfdu and the
- * flux's derivatives along the stencil, times fbar, added to d[0..2]. */
-static double vof_face_T(double uf, double c0, double c1, double c2, double k, int lim, double fbar,
-                         double* ubar, double d[3]) {
+ * uf < 0: c_i, c_{i+1}, c_{i+2}), dt and 1 / h (the Courant number formed as the forward forms
+ * it).  Returns clamp01(cf); adds fbar dF/duf to *ubar and fbar dF/dstencil to d[0..2]. */
+static double vof_face_T(double uf, double c0, double c1, double c2, double dt, double inv_h, int lim,
+                         double fbar, double* ubar, double d[3]) {
+  const double k = dt * inv_h;
   double num, den, up, sgn;
   if (uf >= 0.0) { num = c1 - c0; den = c2 - c1; up = c1; sgn = 1.0; }
   else { num = c2 - c1; den = c1 - c0; up = c1; sgn = -1.0; }
   const int big = fabs(den) > 1.0e-12;
   const double rr = big ? num / den : 0.0;
-  const double cfl = fabs(uf) * k;
+  const double cfl = fabs(uf) * dt * inv_h;
   const double A = 0.5 * (1.0 - cfl);
   const double psi = lim_psi(rr, lim);
   const double cf = up + sgn * A * psi * den;
@@ -814,15 +815,15 @@ void og_vof_advection_T(const og_grid* g, const double* c_in, const double* u, c
   for (int j = 1; j <= ny; ++j)
     for (int i = 0; i <= nx; ++i) {
       const double uf = U(u, i, j);
-      const double cl = (uf >= 0.0) ? vof_face_T(uf, P(c_in, i - 1, j), P(c_in, i, j), P(c_in, i + 1, j), kx, lim, 0.0, &dummy, dd)
-                                    : vof_face_T(uf, P(c_in, i, j), P(c_in, i + 1, j), P(c_in, i + 2, j), kx, lim, 0.0, &dummy, dd);
+      const double cl = (uf >= 0.0) ? vof_face_T(uf, P(c_in, i - 1, j), P(c_in, i, j), P(c_in, i + 1, j), dt, inv_dx, lim, 0.0, &dummy, dd)
+                                    : vof_face_T(uf, P(c_in, i, j), P(c_in, i + 1, j), P(c_in, i + 2, j), dt, inv_dx, lim, 0.0, &dummy, dd);
       FX(i, j) = uf * cl;
     }
   for (int j = 0; j <= ny; ++j)
     for (int i = 1; i <= nx; ++i) {
       const double vf = V(v, i, j);
-      const double cl = (vf >= 0.0) ? vof_face_T(vf, P(c_in, i, j - 1), P(c_in, i, j), P(c_in, i, j + 1), ky, lim, 0.0, &dummy, dd)
-                                    : vof_face_T(vf, P(c_in, i, j), P(c_in, i, j + 1), P(c_in, i, j + 2), ky, lim, 0.0, &dummy, dd);
+      const double cl = (vf >= 0.0) ? vof_face_T(vf, P(c_in, i, j - 1), P(c_in, i, j), P(c_in, i, j + 1), dt, inv_dy, lim, 0.0, &dummy, dd)
+                                    : vof_face_T(vf, P(c_in, i, j), P(c_in, i, j + 1), P(c_in, i, j + 2), dt, inv_dy, lim, 0.0, &dummy, dd);
       FY(i, j) = vf * cl;
     }
   /* c_out = c_in outside the interior (the copy); interior: clamp01(c_in - dt div F) */
@@ -844,7 +845,7 @@ void og_vof_advection_T(const og_grid* g, const double* c_in, const double* u, c
       double d[3] = {0.0, 0.0, 0.0};
       double ub = 0.0;
       const int b = (uf >= 0.0) ? i - 1 : i;
-      vof_face_T(uf, P(c_in, b, j), P(c_in, b + 1, j), P(c_in, b + 2, j), kx, lim, fb, &ub, d);
+      vof_face_T(uf, P(c_in, b, j), P(c_in, b + 1, j), P(c_in, b + 2, j), dt, inv_dx, lim, fb, &ub, d);
       U(ubar, i, j) += ub;
       for (int q = 0; q < 3; ++q) P(cbar_in, b + q, j) += d[q];
     }
@@ -855,7 +856,7 @@ void og_vof_advection_T(const og_grid* g, const double* c_in, const double* u, c
       double d[3] = {0.0, 0.0, 0.0};
       double vb = 0.0;
       const int b = (vf >= 0.0) ? j - 1 : j;
-      vof_face_T(vf, P(c_in, i, b), P(c_in, i, b + 1), P(c_in, i, b + 2), ky, lim, fb, &vb, d);
+      vof_face_T(vf, P(c_in, i, b), P(c_in, i, b + 1), P(c_in, i, b + 2), dt, inv_dy, lim, fb, &vb, d);
       V(vbar, i, j) += vb;
       for (int q = 0; q < 3; ++q) P(cbar_in, i, b + q) += d[q];
     }
