@@ -415,12 +415,14 @@ def timed_rate(ktimes):
 
 
 def run_adjoint(abi, n=1024, nsteps=16, dt=5e-5, count=64, seed=4, rmin=0.01, rmax=0.04, rho_c=100.0,
-                mu_ratio=1e3, sigma=0.5, checkpoint_every=4, mixed=False, also_resident=False, mg_cycles=1):
+                mu_ratio=1e3, sigma=0.5, checkpoint_every=4, mixed=False, also_resident=False, mg_cycles=1,
+                moving=False):
     """Seconds per gradient on an adjoint workload: the forward horizon with checkpoints (restart
     states every `checkpoint_every` steps), the reverse sweep with each segment's forward states
     recomputed, and the property-path gradients; after a 2-step warm-up call on the same
     context.  also_resident: the same gradient with every step's state kept in HBM (no
-    recomputation) is timed too, and reported as seconds_per_gradient when it agrees."""
+    recomputation) is timed too, and reported as seconds_per_gradient when it agrees.  moving: the
+    interface motion differentiated too (FUGU_ADJ_MOVING_INTERFACE) instead of held frozen."""
     from coupled_codefort_2d_amd import workloads
     c, mu, rho, _ = workloads.inclusions(n, count, rmin, rmax, seed, rho_c, 1e-2, 1.0, mu_ratio)
     ctx = abi.Context(n, n, 1.0, 1.0, False, False,
@@ -428,7 +430,7 @@ def run_adjoint(abi, n=1024, nsteps=16, dt=5e-5, count=64, seed=4, rmin=0.01, rm
     u0 = np.zeros((n + 4, n + 5))
     v0 = np.zeros((n + 5, n + 4))
     gm, gr = np.zeros_like(mu), np.zeros_like(rho)
-    flags = abi.ADJ_PRECOND_VMG | (abi.ADJ_MIXED_FORWARD if mixed else 0)
+    flags = abi.ADJ_PRECOND_VMG | (abi.ADJ_MIXED_FORWARD if mixed else 0) | (abi.ADJ_MOVING_INTERFACE if moving else 0)
     ctx.adjoint_gradient(abi.FuguAdjointOpts(2, dt, 0.9, 1e-10, 3000, 30, mg_cycles, flags), mu, rho, sigma, c, u0, v0,
                          gm, gr)
     opts = abi.FuguAdjointOpts(nsteps, dt, 0.9, 1e-10, 3000, 30, mg_cycles, flags, checkpoint_every)
@@ -452,7 +454,8 @@ def run_adjoint(abi, n=1024, nsteps=16, dt=5e-5, count=64, seed=4, rmin=0.01, rm
         print(f"# adjoint resident horizon: {w2:.2f}s, agrees with the checkpointed run (J bitwise, gradients "
               f"to 1e-12): {same}", file=sys.stderr, flush=True)
     ctx.close()
-    print(f"# adjoint {n}x{n}, {nsteps} steps: J {J:.6e} dJ/dsigma {gs:.6e} forward {st.forward_seconds:.2f}s "
+    print(f"# adjoint {n}x{n}, {nsteps} steps{' (moving interface)' if moving else ''}: J {J:.6e} dJ/dsigma {gs:.6e} "
+          f"forward {st.forward_seconds:.2f}s "
           f"reverse {st.reverse_seconds:.2f}s iters {st.forward_iters}/{st.adjoint_iters}", file=sys.stderr, flush=True)
     return {"workload": f"{n}x{n} unit square, {count} seeded inclusions r~U[{rmin},{rmax}] (seed {seed}), carrier rho "
                         f"{rho_c:g} mu 1e-2, inclusions rho 1 mu 1e-2*10^U[0,{np.log10(mu_ratio):g}], sigma {sigma}, "
@@ -462,6 +465,7 @@ def run_adjoint(abi, n=1024, nsteps=16, dt=5e-5, count=64, seed=4, rmin=0.01, rm
             "reverse_seconds_incl_recompute": round(st.reverse_seconds, 3), "forward_iters": st.forward_iters,
             "adjoint_iters": st.adjoint_iters, "max_forward_res": float(f"{st.max_forward_res:.3e}"),
             "max_adjoint_res": float(f"{st.max_adjoint_res:.3e}"), "rc": rc, "J": J, "dJ_dsigma": gs,
+            "interface": "moving (VOF, SVOF, curvature and CSF transposed)" if moving else "frozen",
             "solves": ("mixed (FP32 cycles, FP64 restarts)" if mixed else "FP64") + " forward and FP64 adjoint "
                       f"FGMRES(30) to 1e-10, PCD with multigrid pressure ({mg_cycles} V-cycle(s)) and velocity blocks",
             **({"resident_horizon": res, "checkpointed_seconds_per_gradient": round(wall, 3),
@@ -541,6 +545,11 @@ def main():
     # the 1024^2 horizon (16 states, < 1 GB) is held in HBM; its checkpointed run is timed beside it.
     # Two pressure V-cycles per PCD: measured 3.2 s/grad against 3.7 s with one; four gain < 1 %.
     adj = run_adjoint(abi, also_resident=True, mg_cycles=2) if "adjoint" in sections else None
+    if adj is not None:
+        mv = run_adjoint(abi, mg_cycles=2, moving=True)
+        adj["moving_interface"] = {k: mv[k] for k in ("seconds_per_gradient", "forward_seconds",
+                                                      "reverse_seconds_incl_recompute", "adjoint_iters", "rc",
+                                                      "dJ_dsigma", "interface")}
     adj2048 = None
     if "adjoint2048" in sections:
         # pressure V-cycles per PCD: measured 61 s/grad with 2, 55 with 3, 51 with 4, 48.6 with 6, 48.5
