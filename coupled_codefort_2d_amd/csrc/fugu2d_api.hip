@@ -128,7 +128,6 @@ struct fugu_ctx {
   double* seq_ctot = nullptr;
   int* seq_cnt = nullptr;
   double* seq_tot = nullptr;
-  int* seq_ticket = nullptr;
   int vof_limiter = FUGU_LIMITER_VAN_LEER;   // the VOF face limiter (van Leer: the reference's)
   double* dclamp = nullptr;                  // per-cell clamp correction of the step's VOF update
   double* h_sc = nullptr;
@@ -327,7 +326,6 @@ Sink sink(fugu_ctx* c, long nterms) {
     c->seq_ctot = dalloc<double>(c, 5 * 3 * nch);
     c->seq_cnt = dalloc<int>(c, 5 * nch);
     c->seq_tot = dalloc<double>(c, 8);
-    c->seq_ticket = dalloc<int>(c, SQ_TICKETS);
   }
   return Sink{c->terms, nterms, c->seq_tot};
 }
@@ -337,10 +335,11 @@ void seq_reduce(fugu_ctx* c, const Sink& sk, int nv, Skip skip) {
   if (!sk.n) return;
   SeqArgs a;
   a.t = sk.p; a.n = sk.n; a.nch = (int)((sk.n + SQ_CH - 1) / SQ_CH);
-  a.ctot = c->seq_ctot; a.cnt = c->seq_cnt; a.tot = c->seq_tot; a.ticket = c->seq_ticket;
+  a.ctot = c->seq_ctot; a.cnt = c->seq_cnt; a.tot = c->seq_tot;
   const dim3 gc((unsigned)a.nch, (unsigned)nv);
   LAUNCH(k_seq_chunks, gc, dim3(SQ_NT), c->s, a, skip, (const int*)c->fl);
   LAUNCH(k_seq_compact, gc, dim3(SQ_NT), c->s, a, skip, (const int*)c->fl);
+  LAUNCH(k_seq_fold, dim3((unsigned)nv), dim3(SQ_NT), c->s, a, skip, (const int*)c->fl);
 }
 long cdot_terms(const Grid& g) {
   const long iu = g.per_x ? g.nx : g.nx - 1, jv = g.per_y ? g.ny : g.ny - 1;
@@ -888,12 +887,13 @@ T* mg_vcycle(fugu_ctx* c, std::vector<MgLevel<T>>& lv, size_t l, bool zero_guess
   }
   MgLevel<T>& C = lv[l + 1];
   const double ncell = (double)L.d.nx * L.d.ny;
-  // FP64 levels of at least 2^21 cells: 64-row tiles (1024 threads) for the pre-smoothing with up
-  // to 3 sweeps and the post-smoothing with 3.  Their 170-VGPR 32-row form fits one workgroup per
-  // CU; the 64-row form holds 6 cells per thread instead of 8, fits 128 VGPRs (four waves per
-  // SIMD) and has 1.5x instead of 1.75x halo overhead: 15-18 % faster at 2048^2 (pre, 2-3 sweeps;
-  // post, 3), bit-identical (the tile shape changes no arithmetic).  Smaller levels keep more
-  // workgroups than CUs with the 32-row tiles.
+  // FP64 levels of at least 2^21 cells: 64-row tiles (1024 threads) for the pre-smoothing with 2
+  // or 3 sweeps.  Its 170-VGPR 32-row form fits one workgroup per CU; the 64-row form holds 6
+  // cells per thread instead of 8, fits 128 VGPRs (four waves per SIMD) and has 1.5x instead of
+  // 1.75x halo overhead: 15-18 % faster at 2048^2, bit-identical (the tile shape changes no
+  // arithmetic), at 1.19x its algorithmic HBM bytes instead of 1.07x.  The post-smoothing keeps
+  // the 32-row tiles: as fast, and 1.06x its bytes instead of 1.66x (its larger grown tiles no
+  // longer share their halos in an XCD's L2).  Smaller levels keep more workgroups than CUs.
   const bool th64 = sizeof(T) == 8 && ncell >= (double)(1 << 21);
   const bool pre64 = th64 && (nu == 2 || nu == 3);
   const dim3 gt = mgt_grid(L.d, pre64 ? 64 : MGT_TH);
@@ -935,25 +935,16 @@ T* mg_vcycle(fugu_ctx* c, std::vector<MgLevel<T>>& lv, size_t l, bool zero_guess
   if (c->mg_fused) {
     // phi, coarse phi (a quarter), b, c_x, c_y (, m) read; the other buffer written
     T* Q = (P == L.phi) ? L.phi2 : L.phi;
-    const bool post64 = th64 && nu == 3;
-    const dim3 gp = mgt_grid(L.d, post64 ? 64 : MGT_TH);
+    const dim3 gp = mgt_grid(L.d, MGT_TH);
     const char* pname = mg_kname(true, nu, sizeof(T) == 8);
     const double pbytes = (L.m ? 6.25 : 5.25) * sizeof(T) * ncell;
 #define FUGU_MG_POST(NUV, THV)                                                                                      \
   TLAUNCH(c, pname, pbytes, (k_mg_post<T, NUV, THV>), gp, dim3(16 * THV), c->s, L.d, (const T*)L.cx, (const T*)L.cy, \
           (const T*)L.m, (const T*)L.b, (const T*)P, cres, Q, psum ? psum : (double*)nullptr)
-    if constexpr (sizeof(T) == 8) {
-      if (post64) FUGU_MG_POST(3, 64);
-      else if (nu == 1) FUGU_MG_POST(1, MGT_TH);
-      else if (nu == 3) FUGU_MG_POST(3, MGT_TH);
-      else if (nu == 4) FUGU_MG_POST(4, MGT_TH);
-      else FUGU_MG_POST(2, MGT_TH);
-    } else {
-      if (nu == 1) FUGU_MG_POST(1, MGT_TH);
-      else if (nu == 3) FUGU_MG_POST(3, MGT_TH);
-      else if (nu == 4) FUGU_MG_POST(4, MGT_TH);
-      else FUGU_MG_POST(2, MGT_TH);
-    }
+    if (nu == 1) FUGU_MG_POST(1, MGT_TH);
+    else if (nu == 3) FUGU_MG_POST(3, MGT_TH);
+    else if (nu == 4) FUGU_MG_POST(4, MGT_TH);
+    else FUGU_MG_POST(2, MGT_TH);
 #undef FUGU_MG_POST
     if (psum && psum_nb) *psum_nb = (int)(gp.x * gp.y);
     return Q;
@@ -2341,7 +2332,7 @@ int guarded(F&& f) {
 
 // =====================================================================================
 // Allocation-free coupled dot (fugu_coupled_dot_noalloc): its scratch is this file-scope device
-// array -- the reference order's terms, chunk totals, value counts, ticket counters and sum, or the tree order's
+// array -- the reference order's terms, chunk totals, value counts and sum, or the tree order's
 // per-workgroup partials -- held by every GPU the library runs on until the process ends.  Host
 // code never uses the array's name as an address: each device's address comes from
 // hipGetSymbolAddress on the first call on that device (that device current) and is kept per
@@ -2352,9 +2343,8 @@ constexpr long NOALLOC_PARTIALS = 2L * RED_MAXB;       // one double-double per 
 constexpr long NOALLOC_CTOT = 3 * NOALLOC_NCH;
 constexpr long NOALLOC_CNT = NOALLOC_NCH / 2;          // NOALLOC_NCH ints
 constexpr long NOALLOC_TOT = 8;
-constexpr long NOALLOC_TICKET = SQ_TICKETS / 2;       // SQ_TICKETS ints
 constexpr long NOALLOC_DOUBLES =
-    NOALLOC_PARTIALS + NOALLOC_CTOT + NOALLOC_CNT + NOALLOC_TOT + NOALLOC_TICKET + NOALLOC_MAX_TERMS;   // 8.4 MB
+    NOALLOC_PARTIALS + NOALLOC_CTOT + NOALLOC_CNT + NOALLOC_TOT + NOALLOC_MAX_TERMS;   // 8.4 MB
 static __device__ __align__(256) double scratch[NOALLOC_DOUBLES];
 
 namespace {
@@ -2649,24 +2639,23 @@ int fugu_coupled_dot_noalloc(const fugu_grid* grid, const double* u1, const doub
     double* ctot = partials + NOALLOC_PARTIALS;
     int* cnt = reinterpret_cast<int*>(ctot + NOALLOC_CTOT);
     double* tot = ctot + NOALLOC_CTOT + NOALLOC_CNT;
-    int* ticket = reinterpret_cast<int*>(tot + NOALLOC_TOT);
-    double* terms = tot + NOALLOC_TOT + NOALLOC_TICKET;
+    double* terms = tot + NOALLOC_TOT;
     hipStream_t st = static_cast<hipStream_t>(stream);
     DotSplit d;
     d.a[0] = u1; d.a[1] = v1; d.a[2] = p1;
     d.b[0] = u2; d.b[1] = v2; d.b[2] = p2;
     const dim3 gr = grid_union_red(g);
-    // every value the launches read is written by an earlier launch of this call (the ticket
-    // counters too, zeroed by k_seq_chunks): nothing to clear
+    // every value the launches read is written by an earlier launch of this call: nothing to clear
     if (sum_order == FUGU_SUM_REFERENCE) {
       const Sink sk{terms, nterms, tot};
       LAUNCH(k_cdot_split, gr, dim3(TX, TY), st, g, d, sk);
       SeqArgs a;
       a.t = terms; a.n = nterms; a.nch = (int)((nterms + SQ_CH - 1) / SQ_CH);
-      a.ctot = ctot; a.cnt = cnt; a.tot = tot; a.ticket = ticket;
+      a.ctot = ctot; a.cnt = cnt; a.tot = tot;
       const dim3 gc((unsigned)a.nch, 1u);
-      LAUNCH(k_seq_chunks, gc, dim3(SQ_NT), st, a, Skip{-1, 0}, (const int*)nullptr);   // zeroes the ticket
+      LAUNCH(k_seq_chunks, gc, dim3(SQ_NT), st, a, Skip{-1, 0}, (const int*)nullptr);
       LAUNCH(k_seq_compact, gc, dim3(SQ_NT), st, a, Skip{-1, 0}, (const int*)nullptr);
+      LAUNCH(k_seq_fold, dim3(1), dim3(SQ_NT), st, a, Skip{-1, 0}, (const int*)nullptr);
       LAUNCH(k_fold_dot_out, dim3(1), dim3(NT), st, 0, sk, dot_out);
     } else {
       const Sink sk{partials, 0, nullptr};

@@ -27,10 +27,11 @@
 //                  G = Q * sum_{a<k<=b} rint(t_k / Q) (exact: |G| < 2^(e+1) is a multiple of Q);
 //                  an uncertain term or a tie becomes one value, itself.  The values overwrite
 //                  the chunk's terms in place, in order, with their count;
-//   then the last chunk workgroup of a chain to finish (a ticket counter, device-scope fences
-//                  on both sides) folds: one wavefront lane adds every chunk's values one by one
-//                  from zero -- the same sequence of FP additions as the reference, with each
-//                  run's middle collapsed into one exact addition.
+//   k_seq_fold     one wavefront lane per chain adds every chunk's values one by one from zero
+//                  -- the same sequence of FP additions as the reference, with each run's
+//                  middle collapsed into one exact addition.  (Folding in the last chunk
+//                  workgroup to finish instead, behind a ticket and device-scope fences, measured
+//                  slower at every size: 1.05x at 2^16 terms, 2.2x at 2^22.)
 // Adding G is exact, and adding +0.0 changes nothing (s is never -0.0: from +0.0, an FP sum is
 // -0.0 only when both operands are).  Terms that are NaN or infinite fail certification and are
 // added one by one, so they propagate as in the reference.
@@ -46,11 +47,9 @@ constexpr int SQ_PT = SQ_CH / SQ_NT;    // consecutive terms per thread
 constexpr int SQ_LDP = SQ_PT + 1;       // padded LDS row of one thread's terms (odd: no bank conflicts)
 constexpr int SQ_WIN = 4096;            // chunks per window of the serial fold
 constexpr int SQ_BUF = 2048;            // values per staged piece of the serial fold
-constexpr int SQ_TICKETS = 8;           // ticket counters (one per chain)
 
 // A reference-order reduction: nv chains of n terms (chain q at t + q n), chunk totals ctot
-// (3 doubles per chunk: hi, lo, sum |t|), value counts cnt, the chains' sums tot and the
-// chains' ticket counters (zeroed by k_seq_chunks, counted up by k_seq_compact's workgroups).
+// (3 doubles per chunk: hi, lo, sum |t|), value counts cnt and the chains' sums tot.
 struct SeqArgs {
   double* t;
   long n;
@@ -58,7 +57,6 @@ struct SeqArgs {
   double* ctot;
   int* cnt;
   double* tot;
-  int* ticket;
 };
 // Skip a launch whose pass-1 kernel skipped (fl[fi] >= fv); fi < 0: never.
 struct Skip { int fi, fv; };
@@ -136,11 +134,10 @@ __device__ inline I2 sq_scan_i2(I2 v, I2* lds, I2* total) {
   return ex;
 }
 
-// Chunk totals (grid: nch x nv).  Workgroup (0, 0) also zeroes the ticket counters.
+// Chunk totals (grid: nch x nv).
 __global__ void __launch_bounds__(SQ_NT) k_seq_chunks(SeqArgs a, Skip sk, const int* __restrict__ fl) {
   if (seq_skip(sk, fl)) return;
   const int c = blockIdx.x, q = blockIdx.y, tid = threadIdx.x;
-  if (c == 0 && q == 0 && tid < SQ_TICKETS) a.ticket[tid] = 0;
   const double* t = a.t + (long)q * a.n;
   PA v = pa_zero();
 #pragma unroll
@@ -165,7 +162,8 @@ __global__ void __launch_bounds__(SQ_NT) k_seq_chunks(SeqArgs a, Skip sk, const 
 }
 
 // The serial fold of chain q (one workgroup): lane 0 adds every chunk's values in order while
-// wavefronts 1.. stage the next piece in LDS.  off: SQ_WIN + 1 ints; buf: 2 x SQ_BUF doubles.
+// wavefronts 1.. stage the next piece in LDS (all of them stage the first).  off: SQ_WIN + 1 ints;
+// buf: 2 x SQ_BUF doubles.
 __device__ inline void sq_fold(const SeqArgs& a, int q, int* off, double* buf, I2* lsi) {
   const int tid = threadIdx.x;
   const double* t = a.t + (long)q * a.n;
@@ -195,29 +193,51 @@ __device__ inline void sq_fold(const SeqArgs& a, int q, int* off, double* buf, I
     if (tid == 0) off[SQ_WIN] = T;
     __syncthreads();
     const int npc = (T + SQ_BUF - 1) / SQ_BUF;
-    auto stage = [&](int pc, int b) {
-      for (int j = pc * SQ_BUF + tid - 64; j < min(T, (pc + 1) * SQ_BUF); j += SQ_NT - 64) {
-        int lo = 0, hi = wn;   // last chunk with off[cc] <= j (off is non-decreasing)
-        while (hi - lo > 1) {
-          const int mid = (lo + hi) >> 1;
-          if (off[mid] <= j) lo = mid; else hi = mid;
+    // the last chunk with off[cc] <= j (off is non-decreasing): a fixed-length search, so that a
+    // thread's four searches and then its four loads are in flight together
+    auto chunk_of = [&](int j) {
+      int lo = 0;
+#pragma unroll
+      for (int s = SQ_WIN >> 1; s > 0; s >>= 1)
+        if (lo + s < wn && off[lo + s] <= j) lo += s;
+      return lo;
+    };
+    // stage piece pc into buffer b with threads t0.. (nt of them), four values per thread at a time
+    auto stage = [&](int pc, int b, int t0, int nt) {
+      const int j0 = pc * SQ_BUF, j1 = min(T, j0 + SQ_BUF);
+      for (int j = j0 + (tid - t0); j < j1; j += 4 * nt) {
+        int cc[4];
+        double v[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) cc[q] = (j + q * nt < j1) ? chunk_of(j + q * nt) : 0;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          const int jj = j + q * nt;
+          v[q] = (jj < j1) ? t[(long)(w0 + cc[q]) * SQ_CH + (jj - off[cc[q]])] : 0.0;
         }
-        buf[b * SQ_BUF + (j - pc * SQ_BUF)] = t[(long)(w0 + lo) * SQ_CH + (j - off[lo])];
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+          if (j + q * nt < j1) buf[b * SQ_BUF + (j + q * nt - j0)] = v[q];
       }
     };
-    if (tid >= 64 && npc > 0) stage(0, 0);
+    if (npc > 0) stage(0, 0, 0, SQ_NT);   // every wavefront: the fold waits for this piece anyway
     __syncthreads();
     for (int pc = 0; pc < npc; ++pc) {
-      if (tid >= 64 && pc + 1 < npc) stage(pc + 1, (pc + 1) & 1);
+      if (tid >= 64 && pc + 1 < npc) stage(pc + 1, (pc + 1) & 1, 64, SQ_NT - 64);
       if (tid == 0) {
+        // the additions in order, 32 values at a time read into registers first: the LDS latency
+        // is paid once per batch, not once per dependent addition
         const double* bb = buf + (pc & 1) * SQ_BUF;
         const int nn = min(SQ_BUF, T - pc * SQ_BUF);
-        if (nn == SQ_BUF) {
-#pragma unroll 32
-          for (int m = 0; m < SQ_BUF; ++m) s = s + bb[m];
-        } else {
-          for (int m = 0; m < nn; ++m) s = s + bb[m];
+        int m = 0;
+        for (; m + 32 <= nn; m += 32) {
+          double r[32];
+#pragma unroll
+          for (int q = 0; q < 32; ++q) r[q] = bb[m + q];
+#pragma unroll
+          for (int q = 0; q < 32; ++q) s = s + r[q];
         }
+        for (; m < nn; ++m) s = s + bb[m];
       }
       __syncthreads();
     }
@@ -225,8 +245,7 @@ __device__ inline void sq_fold(const SeqArgs& a, int q, int* off, double* buf, I
   if (tid == 0) a.tot[q] = s;
 }
 
-// Certify, split into runs and compact one chunk in place (grid: nch x nv); the chain's last
-// workgroup to finish then folds the chain.
+// Certify, split into runs and compact one chunk in place (grid: nch x nv).
 __global__ void __launch_bounds__(SQ_NT) k_seq_compact(SeqArgs a, Skip sk, const int* __restrict__ fl) {
   if (seq_skip(sk, fl)) return;
   const int c = blockIdx.x, q = blockIdx.y, tid = threadIdx.x;
@@ -236,15 +255,12 @@ __global__ void __launch_bounds__(SQ_NT) k_seq_compact(SeqArgs a, Skip sk, const
   const int len = (int)min((long)SQ_CH, a.n - base);
   const double u = 0x1p-53;
 
-  // the chunk's terms, one padded row per thread; then the runs' integer sums; then the
-  // fold's staging buffer
+  // the chunk's terms, one padded row per thread; then the runs' integer sums
   __shared__ double lt[SQ_NT * SQ_LDP];
-  __shared__ int loff[SQ_WIN + 1];
   __shared__ PA lpre[SQ_NW], lthr[SQ_NW];
   __shared__ int lkf[SQ_NW], lkl[SQ_NW];
-  __shared__ I2 lcnt[SQ_NW], lfold[SQ_NW];
-  __shared__ int llast;
-  static_assert(SQ_NT * SQ_LDP >= SQ_CH && SQ_NT * SQ_LDP >= 2 * SQ_BUF, "LDS reuse");
+  __shared__ I2 lcnt[SQ_NW];
+  static_assert(SQ_NT * SQ_LDP >= SQ_CH, "LDS reuse");
 
   // coalesced load of the chunk: term L -> row L / SQ_PT, column L % SQ_PT
   for (int m = tid; m < SQ_CH; m += SQ_NT) lt[(m / SQ_PT) * SQ_LDP + (m % SQ_PT)] = m < len ? t[base + m] : 0.0;
@@ -381,15 +397,15 @@ __global__ void __launch_bounds__(SQ_NT) k_seq_compact(SeqArgs a, Skip sk, const
     }
   }
   if (tid == 0) a.cnt[(long)q * a.nch + c] = tot2.b;
-  // the chain's last workgroup folds: release this chunk's values, take a ticket, and (last)
-  // acquire every other chunk's before reading them
-  __threadfence();
-  __syncthreads();
-  if (tid == 0) llast = (atomicAdd(&a.ticket[q], 1) == a.nch - 1) ? 1 : 0;
-  __syncthreads();
-  if (!llast) return;
-  __threadfence();
-  sq_fold(a, q, loff, lt, lfold);
+}
+
+// The serial fold (grid: nv, one chain per workgroup).
+__global__ void __launch_bounds__(SQ_NT) k_seq_fold(SeqArgs a, Skip sk, const int* __restrict__ fl) {
+  if (seq_skip(sk, fl)) return;
+  __shared__ int loff[SQ_WIN + 1];
+  __shared__ double lbuf[2 * SQ_BUF];
+  __shared__ I2 lfold[SQ_NW];
+  sq_fold(a, blockIdx.x, loff, lbuf, lfold);
 }
 
 }  // namespace fugu
