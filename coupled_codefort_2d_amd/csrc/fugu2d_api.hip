@@ -622,6 +622,10 @@ inline dim3 mgt_grid(const MgDims& d, int th) {
 constexpr size_t MG_TAIL_BLOCK_CELLS = 8192;
 template <typename T>
 std::vector<MgLevel<T>> mg_levels(fugu_ctx* c, MgDims d, bool mass, int max_levels) {
+  // the fused level kernels address a level's arrays with 32-bit byte offsets
+  if ((size_t)(d.nx + 1) * (size_t)(d.ny + 1) * sizeof(T) >= ((size_t)1 << 32))
+    throw Error("multigrid: a level's arrays must stay below 4 GiB (" + std::to_string(d.nx) + " x " +
+                std::to_string(d.ny) + " cells)");
   std::vector<MgDims> ds;
   for (;;) {
     ds.push_back(d);
@@ -887,15 +891,17 @@ T* mg_vcycle(fugu_ctx* c, std::vector<MgLevel<T>>& lv, size_t l, bool zero_guess
   }
   MgLevel<T>& C = lv[l + 1];
   const double ncell = (double)L.d.nx * L.d.ny;
-  // FP64 levels of at least 2^21 cells: 64-row tiles (1024 threads) for the pre-smoothing with 2
-  // or 3 sweeps.  Its 170-VGPR 32-row form fits one workgroup per CU; the 64-row form holds 6
-  // cells per thread instead of 8, fits 128 VGPRs (four waves per SIMD) and has 1.5x instead of
-  // 1.75x halo overhead: 15-18 % faster at 2048^2, bit-identical (the tile shape changes no
-  // arithmetic), at 1.19x its algorithmic HBM bytes instead of 1.07x.  The post-smoothing keeps
-  // the 32-row tiles: as fast, and 1.06x its bytes instead of 1.66x (its larger grown tiles no
-  // longer share their halos in an XCD's L2).  Smaller levels keep more workgroups than CUs.
+  // FP64 levels of at least 2^21 cells: 64-row tiles (1024 threads) for the pre-smoothing with 3
+  // sweeps.  Its 162-VGPR 32-row form fits one workgroup per CU; the 64-row form holds 6 cells
+  // per thread instead of 8, fits 128 VGPRs (four waves per SIMD) and has 1.5x instead of 1.75x
+  // halo overhead: 18 % faster at 2048^2, bit-identical (the tile shape changes no arithmetic),
+  // at 1.19x its algorithmic HBM bytes instead of 1.07x.  With 2 sweeps the 32-row form fits
+  // 124 VGPRs, two workgroups per CU, and is the faster one (44 against 57 us at 2048^2).  The
+  // post-smoothing keeps the 32-row tiles: as fast, and 1.06x its bytes instead of 1.66x (its
+  // larger grown tiles no longer share their halos in an XCD's L2).  Smaller levels keep more
+  // workgroups than CUs.
   const bool th64 = sizeof(T) == 8 && ncell >= (double)(1 << 21);
-  const bool pre64 = th64 && (nu == 2 || nu == 3);
+  const bool pre64 = th64 && nu == 3;
   const dim3 gt = mgt_grid(L.d, pre64 ? 64 : MGT_TH);
   T* P = (g_in == L.phi) ? L.phi2 : L.phi;   // the fused pre-smoothing's output
   if (c->mg_fused) {
@@ -907,8 +913,7 @@ T* mg_vcycle(fugu_ctx* c, std::vector<MgLevel<T>>& lv, size_t l, bool zero_guess
   TLAUNCH(c, nm, ab, (k_mg_pre<T, NUV, THV>), gt, dim3(16 * THV), c->s, L.d, (const T*)L.cx,                   \
           (const T*)L.cy, (const T*)L.m, (const T*)L.b, P, C.b, C.phi, g_in)
     if constexpr (sizeof(T) == 8) {
-      if (pre64 && nu == 3) FUGU_MG_PRE(3, 64);
-      else if (pre64) FUGU_MG_PRE(2, 64);
+      if (pre64) FUGU_MG_PRE(3, 64);
       else if (nu == 1) FUGU_MG_PRE(1, MGT_TH);
       else if (nu == 3) FUGU_MG_PRE(3, MGT_TH);
       else if (nu == 4) FUGU_MG_PRE(4, MGT_TH);

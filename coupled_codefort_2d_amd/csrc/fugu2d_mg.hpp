@@ -303,8 +303,8 @@ __global__ void __launch_bounds__(1024) k_mg_coarse_inv(MgDims d, const T* __res
 // and the residual restriction, another the prolongation and NU post-smoothing sweeps, instead
 // of 2 NU + 1 launches each.  A workgroup owns a tile (shape below) plus a halo of
 // H cells; each thread owns fixed cells of that extended tile (slot k: cell tid + NT k, row
-// major) and loads their b, D^-1 and four face conductances into registers once, in one round
-// of independent loads; phi of the extended tile lives in LDS.  Half-sweep h updates its colour
+// major) and loads their b and four face conductances into registers once (D^-1 formed from
+// them), in one round of independent loads; phi of the extended tile lives in LDS.  Half-sweep h updates its colour
 // on the tile grown by H - 1 - h rings, so every value the owner needs is final when read, and
 // the owner writes its tile only.  Tiles are numbered so that each XCD gets a contiguous band
 // of them (its L2 then holds the shared halos).  No launch reads what it writes: the
@@ -320,10 +320,22 @@ __global__ void __launch_bounds__(1024) k_mg_coarse_inv(MgDims d, const T* __res
 constexpr int MGT_W = 64;
 constexpr int MGT_TH = 32;
 
-__device__ inline int mgt_wrap(int g, int n, int per) {
-  if (g >= 0 && g < n) return g;
-  if (!per) return -1;
-  return ((g % n) + n) % n;
+// A tile's global index g of a level extent n: wrapped (periodic), else clamped into the level
+// (a cell outside a wall loads an in-level value it never uses).  A small periodic level can
+// see several wraps (a tile wider than the level holds copies of its cells).
+__device__ inline int mgt_index(int g, int n, int per) {
+  if (!per) return min(max(g, 0), n - 1);
+  int r = g < 0 ? g + n : g;
+  r = r >= n ? r - n : r;
+  if ((unsigned)r >= (unsigned)n) r = ((g % n) + n) % n;   // more than one wrap
+  return r;
+}
+
+// Element e of a level array from a 32-bit byte offset (a uniform base and one offset register
+// per load; level arrays stay below 4 GiB, which the level set-up checks).
+template <typename T>
+__device__ inline T mgt_ld(const T* base, unsigned e) {
+  return *reinterpret_cast<const T*>(reinterpret_cast<const char*>(base) + e * (unsigned)sizeof(T));
 }
 
 // Tile of this workgroup: dispatch order places workgroup w on XCD w mod 8; renumber so that
@@ -343,6 +355,9 @@ __device__ inline void mgt_tile(int& i0, int& j0) {
 // every lane busy.  The grown tile is EW (even) wide, so each of its rows holds EW/2 cells of
 // either colour, and the colour of a cell, (i + j) & 1 of its wrapped global index, is (li + lj)
 // & 1: i0, j0 and 2H are even, and a periodic wrap moves by an even extent.
+// The conductances are held unmasked: at a wall the neighbour across the boundary face is a cell
+// outside the level, whose LDS value is +0 and never updated, so c * (+0) = +0 is exactly the
+// masked face's 0 * (+0) (conductances are finite and >= 0).
 template <typename T, int H, int TH>
 struct MgtCells {
   static constexpr int TILE_H = TH, NT = 16 * TH;
@@ -350,98 +365,100 @@ struct MgtCells {
   static constexpr int NSC = ((EW / 2) * EH + NT - 1) / NT, NS = 2 * NSC;
   static_assert(EW % 2 == 0, "the grown tile must be even wide");
   T b[NS], dinv[NS], cW[NS], cE[NS], cS[NS], cN[NS];
-  T dg[NS];                       // unmasked diagonal (+ m): the residual of the restriction
-  int li[NS], lj[NS], ring[NS];   // ring: distance outside the tile (0 inside); -1: no cell
+  T dg[NS];                 // diagonal (+ m): the residual of the restriction
+  int off[NS], ring[NS];    // off: lj * EW + li in the LDS tile; ring: distance outside the tile
+                            // (0 inside), -1: no cell or beyond the updated rings
 };
 
 // Load the thread's cells (registers) and phi (+ the parent's coarse correction) into LDS.
-// Cells up to ring rc get b and their four conductances (and m), from which D^-1 and the
-// diagonal are formed exactly as k_mg_dinv forms them (no D^-1 array is read); cells beyond
-// ring rc are only ever read as neighbours (their phi) or not at all, and load nothing else.
+// All slots' loads are in flight together: nothing in the load loop uses a loaded value (a use
+// there waits for every load issued before it).  Only cells up to ring rc are ever updated and
+// load b and their conductances; cells beyond it are read as neighbours (their phi) or not at
+// all.  D^-1 and the diagonal are formed exactly as k_mg_dinv forms them (no D^-1 array is
+// read).  Offsets are 32-bit: one offset register per load against a uniform base.
 template <typename T, int H, int TH>
 __device__ inline void mgt_init(const MgDims& f, int i0, int j0, const T* cx, const T* cy, const T* mass,
-                                const T* b, const T* phi, const T* phic, int rc, MgtCells<T, H, TH>& m,
-                                T (*s)[MgtCells<T, H, TH>::EW]) {
+                                const T* b, const T* phi, const T* phic, int rc, MgtCells<T, H, TH>& m, T* s) {
   using M = MgtCells<T, H, TH>;
   const int tid = threadIdx.x;
-  const int nxc = (f.nx + 1) / 2;
-  const size_t rx = (size_t)(f.nx + 1);
-  T pv[M::NS];
+  const int nx = f.nx, rx = f.nx + 1, nxc = (f.nx + 1) / 2;
+  T pv[M::NS], pc[M::NS];
+  bool cell[M::NS], slot[M::NS];
 #pragma unroll
   for (int k = 0; k < M::NS; ++k) {
     const int colour = k / M::NSC;
     const int q = tid + M::NT * (k - colour * M::NSC);   // the colour's q-th cell of the grown tile
     const int lj = q / (M::EW / 2), li = 2 * (q - lj * (M::EW / 2)) + ((lj + colour) & 1);
-    m.li[k] = li; m.lj[k] = lj;
-    const int i = mgt_wrap(i0 - H + li, f.nx, f.per_x), j = mgt_wrap(j0 - H + lj, f.ny, f.per_y);
-    int dist = 0;
-    dist = max(dist, H - li);
-    dist = max(dist, li - (H + MGT_W - 1));
-    dist = max(dist, H - lj);
-    dist = max(dist, lj - (H + M::TILE_H - 1));
-    const bool slot = lj < M::EH;
-    const bool cell = slot && i >= 0 && j >= 0;
-    m.ring[k] = (cell && dist <= rc) ? dist : -1;
-    // raw loads only here (all slots' loads in flight together); D^-1 and masks below
-    T bv = T(0), rw = T(0), re = T(0), rs = T(0), rn = T(0), mm = T(0), p = T(0);
-    if (cell) {
-      const size_t o = i + (size_t)j * f.nx;
-      if (dist <= rc) {
-        bv = b[o];
-        rw = cx[i + j * rx]; re = cx[i + 1 + j * rx]; rs = cy[o]; rn = cy[o + f.nx];
-        if (mass) mm = mass[o];
-      }
-      if (phi) {
-        p = phi[o];
-        if (phic) p = p + phic[(i >> 1) + (size_t)(j >> 1) * nxc];
-      }
+    const int gi = i0 - H + li, gj = j0 - H + lj;
+    const int i = mgt_index(gi, nx, f.per_x), j = mgt_index(gj, f.ny, f.per_y);
+    const int dist = max(max(max(H - li, li - (H + MGT_W - 1)), max(H - lj, lj - (H + M::TILE_H - 1))), 0);
+    slot[k] = lj < M::EH;   // the last slots of the last threads can lie past the grown tile
+    cell[k] = slot[k] && (f.per_x || (unsigned)gi < (unsigned)nx) && (f.per_y || (unsigned)gj < (unsigned)f.ny);
+    m.off[k] = slot[k] ? lj * M::EW + li : 0;
+    m.ring[k] = (cell[k] && dist <= rc) ? dist : -1;
+    const unsigned o = (unsigned)(i + j * nx), ox = (unsigned)(i + j * rx);
+    // b and the conductances only where the cell is updated; phi (+ phi_c) wherever it exists
+    T bv = T(0), rw = T(0), re = T(0), rs = T(0), rn = T(0), mm = T(0), p = T(0), pp = T(0);
+    if (m.ring[k] >= 0) {
+      bv = mgt_ld(b, o);
+      rw = mgt_ld(cx, ox);
+      re = mgt_ld(cx, ox + 1u);
+      rs = mgt_ld(cy, o);
+      rn = mgt_ld(cy, o + (unsigned)nx);
+      if (mass) mm = mgt_ld(mass, o);
+    }
+    if (slot[k]) {
+      if (phi) p = mgt_ld(phi, o);
+      if (phic) pp = mgt_ld(phic, (unsigned)((i >> 1) + (j >> 1) * nxc));
     }
     m.b[k] = bv; m.cW[k] = rw; m.cE[k] = re; m.cS[k] = rs; m.cN[k] = rn; m.dg[k] = mm;
     pv[k] = p;
+    pc[k] = pp;
+  }
+  // the prolonged iterate phi + phi_c; +0 outside the level
+#pragma unroll
+  for (int k = 0; k < M::NS; ++k) {
+    const T p = phic ? pv[k] + pc[k] : pv[k];
+    pv[k] = cell[k] ? p : T(0);
   }
 #pragma unroll
   for (int k = 0; k < M::NS; ++k) {
-    // diagonal and D^-1 exactly as k_mg_dinv forms them, then the boundary-face masks
     T dg = m.cW[k] + m.cE[k] + m.cS[k] + m.cN[k];
     if (mass) dg += m.dg[k];
     m.dg[k] = dg;
     m.dinv[k] = (dg > T(1.0e-14)) ? T(1) / dg : T(0);
-    const int i = mgt_wrap(i0 - H + m.li[k], f.nx, f.per_x), j = mgt_wrap(j0 - H + m.lj[k], f.ny, f.per_y);
-    if (!(i > 0 || f.per_x)) m.cW[k] = T(0);
-    if (!(i < f.nx - 1 || f.per_x)) m.cE[k] = T(0);
-    if (!(j > 0 || f.per_y)) m.cS[k] = T(0);
-    if (!(j < f.ny - 1 || f.per_y)) m.cN[k] = T(0);
   }
 #pragma unroll
   for (int k = 0; k < M::NS; ++k)
-    if (m.lj[k] < M::EH) s[m.lj[k]][m.li[k]] = pv[k];
+    if (slot[k]) s[m.off[k]] = pv[k];
   __syncthreads();
 }
 
 // One half-sweep of colour COL over the tile grown by r rings (the colour's slots only).
 template <typename T, int H, int TH, int COL>
-__device__ inline void mgt_half(int r, int zero_other, const MgtCells<T, H, TH>& m, T (*s)[MgtCells<T, H, TH>::EW]) {
+__device__ inline void mgt_half(int r, int zero_other, const MgtCells<T, H, TH>& m, T* s) {
   using M = MgtCells<T, H, TH>;
 #pragma unroll
   for (int k = COL * M::NSC; k < (COL + 1) * M::NSC; ++k) {
-    if (m.ring[k] < 0 || m.ring[k] > r) continue;
-    const int li = m.li[k], lj = m.lj[k];
+    if ((unsigned)m.ring[k] > (unsigned)r) continue;   // ring -1: never
+    const int o = m.off[k];
     T sum = T(0);
-    if (!zero_other) sum = m.cW[k] * s[lj][li - 1] + m.cE[k] * s[lj][li + 1] + m.cS[k] * s[lj - 1][li] + m.cN[k] * s[lj + 1][li];
-    s[lj][li] = (m.b[k] + sum) * m.dinv[k];
+    if (!zero_other) sum = m.cW[k] * s[o - 1] + m.cE[k] * s[o + 1] + m.cS[k] * s[o - M::EW] + m.cN[k] * s[o + M::EW];
+    s[o] = (m.b[k] + sum) * m.dinv[k];
   }
   __syncthreads();
 }
 
 template <typename T, int H, int TH>
-__device__ inline void mgt_store(const MgDims& f, int i0, int j0, T* out, const T (*s)[MgtCells<T, H, TH>::EW]) {
+__device__ inline void mgt_store(const MgDims& f, int i0, int j0, T* out, const T* s) {
+  using M = MgtCells<T, H, TH>;
   const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
   const int gi = i0 + tx;
   if (gi >= f.nx) return;
 #pragma unroll
   for (int qj = ty; qj < TH; qj += TH / 4) {
     const int gj = j0 + qj;
-    if (gj < f.ny) out[gi + (size_t)gj * f.nx] = s[H + qj][H + tx];
+    if (gj < f.ny) out[gi + (size_t)gj * f.nx] = s[(H + qj) * M::EW + H + tx];
   }
 }
 
@@ -457,7 +474,7 @@ __global__ void __launch_bounds__(16 * TH) k_mg_pre(MgDims f, const T* __restric
                                                 const T* __restrict__ phi_in) {
   constexpr int H = 2 * NU + 1;
   using M = MgtCells<T, H, TH>;
-  __shared__ T s[M::EH][M::EW];
+  __shared__ T s[M::EH * M::EW];
   __shared__ T sres[TH][MGT_W];
   int i0, j0;
   mgt_tile<TH>(i0, j0);
@@ -475,10 +492,10 @@ __global__ void __launch_bounds__(16 * TH) k_mg_pre(MgDims f, const T* __restric
 #pragma unroll
   for (int k = 0; k < M::NS; ++k) {
     if (cells.ring[k] != 0) continue;
-    const int li = cells.li[k], lj = cells.lj[k];
-    const T nb = cells.cW[k] * s[lj][li - 1] + cells.cE[k] * s[lj][li + 1] + cells.cS[k] * s[lj - 1][li] +
-                 cells.cN[k] * s[lj + 1][li];
-    sres[lj - H][li - H] = cells.b[k] - (cells.dg[k] * s[lj][li] - nb);
+    const int o = cells.off[k];
+    const T nb = cells.cW[k] * s[o - 1] + cells.cE[k] * s[o + 1] + cells.cS[k] * s[o - M::EW] + cells.cN[k] * s[o + M::EW];
+    const int lj = o / M::EW, li = o - lj * M::EW;
+    sres[lj - H][li - H] = cells.b[k] - (cells.dg[k] * s[o] - nb);
   }
   __syncthreads();
   const int nxc = (f.nx + 1) / 2, nyc = (f.ny + 1) / 2;
@@ -506,7 +523,7 @@ __global__ void __launch_bounds__(16 * TH) k_mg_post(MgDims f, const T* __restri
                                                  T* __restrict__ phi_out, double* __restrict__ psum) {
   constexpr int H = 2 * NU;
   using M = MgtCells<T, H, TH>;
-  __shared__ T s[M::EH][M::EW];
+  __shared__ T s[M::EH * M::EW];
   int i0, j0;
   mgt_tile<TH>(i0, j0);
   M cells;
@@ -523,7 +540,7 @@ __global__ void __launch_bounds__(16 * TH) k_mg_post(MgDims f, const T* __restri
     if (i0 + tx < f.nx) {
 #pragma unroll
       for (int qj = ty; qj < TH; qj += TH / 4)
-        if (j0 + qj < f.ny) v[0] = dd_add(v[0], dd_from((double)s[H + qj][H + tx]));
+        if (j0 + qj < f.ny) v[0] = dd_add(v[0], dd_from((double)s[(H + qj) * M::EW + H + tx]));
     }
     publish_dd<1>(v, psum);
   }
