@@ -211,6 +211,7 @@ struct fugu_ctx {
   // per-kernel cost (a few microseconds between dependent kernels) is on the device side.
   bool graphs = false;
   bool svof_blocked = true;   // SVOF sweeps temporally blocked (FUGU_SVOF_BLOCKED=0: one per launch)
+  int svof_pred = 0;          // the last blocked SVOF's iteration count (its next call's first batch)
   struct FgKey {
     int prec = 0, mvflags = 0, nl0 = 0, nl1 = 0, flags = 0, mg_cycles = 0, cyc0 = 0, cyc1 = 0;
     double dt = 0.0, pcd_visc = 0.0;
@@ -622,8 +623,10 @@ inline dim3 mgt_grid(const MgDims& d, int th) {
 constexpr size_t MG_TAIL_BLOCK_CELLS = 8192;
 template <typename T>
 std::vector<MgLevel<T>> mg_levels(fugu_ctx* c, MgDims d, bool mass, int max_levels) {
-  // the fused level kernels address a level's arrays with 32-bit byte offsets
-  if ((size_t)(d.nx + 1) * (size_t)(d.ny + 1) * sizeof(T) >= ((size_t)1 << 32))
+  // the fused level kernels address a level's arrays with 32-bit byte offsets and 24-bit
+  // index products
+  if ((size_t)(d.nx + 1) * (size_t)(d.ny + 1) * sizeof(T) >= ((size_t)1 << 32) || d.nx + 1 >= (1 << 24) ||
+      d.ny + 1 >= (1 << 24))
     throw Error("multigrid: a level's arrays must stay below 4 GiB (" + std::to_string(d.nx) + " x " +
                 std::to_string(d.ny) + " cells)");
   std::vector<MgDims> ds;
@@ -1789,16 +1792,31 @@ void svof(fugu_ctx* c, const double* craw, double* cs_out, double delta, int max
     // SVOF_S iterations per launch; block b reads B[b & 1] and writes B[(b + 1) & 1]; the
     // launch after the last block only recomputes a block that converged before its end
     const int nbk = (max_iter + SVOF_S - 1) / SVOF_S;
-    for (int b = 0; b <= nbk; ++b) {
+    auto block = [&](int b) {
       const int it0 = b * SVOF_S + 1, cnt = b < nbk ? std::min(SVOF_S, max_iter - b * SVOF_S) : 0;
       LAUNCH(k_svof_block, tb, dim3(SVOF_NT), c->s, g, craw, (const double*)B[b & 1], B[(b + 1) & 1], cx, cy, diag, b,
              cnt, c->partials, (const int*)c->fl);
       if (cnt > 0)
         LAUNCH(k_fold_svof_blk, dim3(1), dim3(NT), c->s, (int)ntb, b, it0, cnt, tol, (const double*)c->partials,
                c->sc, c->fl);
-    }
+    };
+    // The blocks after convergence launch and exit at once.  A first batch sized from the last
+    // call's iteration count (one block of margin) leaves them out; the flags read after it
+    // decide whether the rest is needed, or only the launch that recomputes a block that
+    // converged before its end (the same launches as the full sequence up to that point).
+    const int first = c->svof_pred > 0 ? std::min(nbk, (c->svof_pred + SVOF_S - 1) / SVOF_S + 1) : nbk;
+    for (int b = 0; b <= first; ++b) block(b);
     read_flags(c);
+    if (first < nbk) {
+      if (!c->h_fl[F_S_DONE]) {
+        for (int b = first + 1; b <= nbk; ++b) block(b);
+        read_flags(c);
+      } else if (c->h_fl[F_S_RBLK] == first) {
+        block(first + 1);
+      }
+    }
     const int itf = c->h_fl[F_S_ITER];
+    c->svof_pred = c->h_fl[F_S_DONE] ? itf : 0;
     if (it_out) *it_out = itf;
     if (max_iter >= 1) last = B[((itf - 1) / SVOF_S + 1) & 1];   // the output of the block holding itf
   } else {

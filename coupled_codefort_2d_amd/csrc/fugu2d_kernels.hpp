@@ -1322,21 +1322,28 @@ __global__ void __launch_bounds__(NT) k_pcd_rp(Grid g, const T* __restrict__ phi
   if (i < 1 - nh || i > nx + nh || j < 1 - nh || j > ny + nh) return;
   T out = T(0);
   if (i >= 1 && i <= nx && j >= 1 && j <= ny) {
+    // the five phi values: neighbours wrapped (periodic) or outside (a wall: value 0); every
+    // load is issued before any value is used (a use between loads waits for all of them)
     const bool early = (fl[F_P_DONE] == 2);
     const T mean = walls_both ? (T)sc[S_P_MEAN] : T(0);
-    auto ph = [&](int ii, int jj) -> T {
-      if (early) return T(0);
-      if (ii < 1 || ii > nx) { if (!g.per_x) return T(0); ii = (ii < 1) ? ii + nx : ii - nx; }
-      if (jj < 1 || jj > ny) { if (!g.per_y) return T(0); jj = (jj < 1) ? jj + ny : jj - ny; }
-      const T x = phic ? phic[(ii - 1) + (size_t)(jj - 1) * nx] : phi[IP(g, ii, jj)];
+    const bool okE = i < nx || g.per_x, okW = i > 1 || g.per_x;
+    const bool okN = j < ny || g.per_y, okS = j > 1 || g.per_y;
+    const int iE = i < nx ? i + 1 : 1, iW = i > 1 ? i - 1 : nx;
+    const int jN = j < ny ? j + 1 : 1, jS = j > 1 ? j - 1 : ny;
+    const T* src = phic ? phic : phi;
+    auto at = [&](int ii, int jj) -> size_t { return phic ? (size_t)(ii - 1) + (size_t)(jj - 1) * nx : IP(g, ii, jj); };
+    const T pC = src[at(i, j)], pE = src[at(iE, j)], pW = src[at(iW, j)], pN = src[at(i, jN)], pS = src[at(i, jS)];
+    const T ua = uadv[IU(g, i, j)], ub = uadv[IU(g, i - 1, j)], va = vadv[IV(g, i, j)], vb = vadv[IV(g, i, j - 1)];
+    auto ph = [&](T x, bool ok) -> T {
+      if (early || !ok) return T(0);
       return walls_both ? x - mean : x;
     };
     const T half = T(0.5);
-    const T u_avg = half * (uadv[IU(g, i, j)] + uadv[IU(g, i - 1, j)]);
-    const T v_avg = half * (vadv[IV(g, i, j)] + vadv[IV(g, i, j - 1)]);
-    const T conv = u_avg * (ph(i + 1, j) - ph(i - 1, j)) * (half * (T)g.inv_dx) +
-                   v_avg * (ph(i, j + 1) - ph(i, j - 1)) * (half * (T)g.inv_dy);
-    out = -((T(1) / dt) * ph(i, j) + conv);
+    const T u_avg = half * (ua + ub);
+    const T v_avg = half * (va + vb);
+    const T conv = u_avg * (ph(pE, okE) - ph(pW, okW)) * (half * (T)g.inv_dx) +
+                   v_avg * (ph(pN, okN) - ph(pS, okS)) * (half * (T)g.inv_dy);
+    out = -((T(1) / dt) * ph(pC, true) + conv);
     if (visc > T(0)) out = out - visc * kap[IP(g, i, j)] * zp[IP(g, i, j)];
   }
   rp[IP(g, i, j)] = out;
@@ -1352,22 +1359,22 @@ __global__ void __launch_bounds__(NT) k_pcd_zwork(Grid g, const T* __restrict__ 
   int i, j;
   union_ij(g, i, j);
   const int nx = g.nx, ny = g.ny, nh = g.nh;
-  if (i >= -nh && i <= nx + nh && j >= 1 - nh && j <= ny + nh) {
-    T o = T(0);
-    if (j >= 1 && j <= ny) {
-      if (i >= 1 && i <= nx - 1) o = zu[IU(g, i, j)] - (rp[IP(g, i + 1, j)] - rp[IP(g, i, j)]) * (T)g.inv_dx;
-      else if (i == 0 || i == nx) o = zu[IU(g, i, j)];
-    }
-    zuw[IU(g, i, j)] = o;
-  }
+  // every operand loaded first, from indices clamped into the arrays (a thread uses only those
+  // its point needs), so that no load waits for another
+  const int iu = min(max(i, 0), nx), ju = min(max(j, 1), ny), ir = min(max(i, 1), nx - 1);
+  const int iv = min(max(i, 1), nx), jv = min(max(j, 0), ny), jr = min(max(j, 1), ny - 1);
+  const T zu0 = zu[IU(g, iu, ju)], ru1 = rp[IP(g, ir + 1, ju)], ru0 = rp[IP(g, ir, ju)];
+  const T zv0 = zv[IV(g, iv, jv)], rv1 = rp[IP(g, iv, jr + 1)], rv0 = rp[IP(g, iv, jr)];
+  // both results formed before either store
+  const bool urow = j >= 1 && j <= ny, vcol = i >= 1 && i <= nx;
+  const T du = zu0 - (ru1 - ru0) * (T)g.inv_dx;
+  const T dv = zv0 - (rv1 - rv0) * (T)g.inv_dy;
+  const T ou = (urow && i >= 1 && i <= nx - 1) ? du : ((urow && (i == 0 || i == nx)) ? zu0 : T(0));
+  const T ov = (vcol && j >= 1 && j <= ny - 1) ? dv : ((vcol && (j == 0 || j == ny)) ? zv0 : T(0));
+  if (i >= -nh && i <= nx + nh && j >= 1 - nh && j <= ny + nh) zuw[IU(g, i, j)] = ou;
   if (i >= 1 - nh && i <= nx + nh && j >= -nh && j <= ny + nh) {
-    T o = T(0);
-    if (i >= 1 && i <= nx) {
-      if (j >= 1 && j <= ny - 1) o = zv[IV(g, i, j)] - (rp[IP(g, i, j + 1)] - rp[IP(g, i, j)]) * (T)g.inv_dy;
-      else if (j == 0 || j == ny) o = zv[IV(g, i, j)];
-    }
-    zvw[IV(g, i, j)] = o;
-    if (bv && i >= 1 && i <= nx && j >= 1 && j <= (g.per_y ? ny : ny - 1)) bv[(i - 1) + (size_t)(j - 1) * nx] = o;
+    zvw[IV(g, i, j)] = ov;
+    if (bv && vcol && j >= 1 && j <= (g.per_y ? ny : ny - 1)) bv[(i - 1) + (size_t)(j - 1) * nx] = ov;
   }
 }
 

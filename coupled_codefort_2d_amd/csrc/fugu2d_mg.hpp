@@ -323,12 +323,14 @@ constexpr int MGT_TH = 32;
 // A tile's global index g of a level extent n: wrapped (periodic), else clamped into the level
 // (a cell outside a wall loads an in-level value it never uses).  A small periodic level can
 // see several wraps (a tile wider than the level holds copies of its cells).
+// Branch-free (a divergent branch between a slot's loads and the next's makes the next wait for
+// them): the wrap is g - n q with q = floor(g / n) from the hardware reciprocal, off by at most
+// one for |g|, n < 2^16 (relative error ~2^-22), which the two corrections absorb.
 __device__ inline int mgt_index(int g, int n, int per) {
-  if (!per) return min(max(g, 0), n - 1);
-  int r = g < 0 ? g + n : g;
+  int r = g - n * (int)floorf((float)g * __builtin_amdgcn_rcpf((float)n));
+  r = r < 0 ? r + n : r;
   r = r >= n ? r - n : r;
-  if ((unsigned)r >= (unsigned)n) r = ((g % n) + n) % n;   // more than one wrap
-  return r;
+  return per ? r : min(max(g, 0), n - 1);
 }
 
 // Element e of a level array from a 32-bit byte offset (a uniform base and one offset register
@@ -396,7 +398,11 @@ __device__ inline void mgt_init(const MgDims& f, int i0, int j0, const T* cx, co
     cell[k] = slot[k] && (f.per_x || (unsigned)gi < (unsigned)nx) && (f.per_y || (unsigned)gj < (unsigned)f.ny);
     m.off[k] = slot[k] ? lj * M::EW + li : 0;
     m.ring[k] = (cell[k] && dist <= rc) ? dist : -1;
-    const unsigned o = (unsigned)(i + j * nx), ox = (unsigned)(i + j * rx);
+    // 24-bit products (level extents stay below 2^24): a full 32-bit multiply-add is a 64-bit
+    // instruction whose unused high addend can be a register a load in flight writes, and then
+    // waits for every load issued so far
+    const unsigned o = (unsigned)i + __umul24((unsigned)j, (unsigned)nx);
+    const unsigned ox = (unsigned)i + __umul24((unsigned)j, (unsigned)rx);
     // b and the conductances only where the cell is updated; phi (+ phi_c) wherever it exists
     T bv = T(0), rw = T(0), re = T(0), rs = T(0), rn = T(0), mm = T(0), p = T(0), pp = T(0);
     if (m.ring[k] >= 0) {
@@ -409,7 +415,7 @@ __device__ inline void mgt_init(const MgDims& f, int i0, int j0, const T* cx, co
     }
     if (slot[k]) {
       if (phi) p = mgt_ld(phi, o);
-      if (phic) pp = mgt_ld(phic, (unsigned)((i >> 1) + (j >> 1) * nxc));
+      if (phic) pp = mgt_ld(phic, (unsigned)(i >> 1) + __umul24((unsigned)(j >> 1), (unsigned)nxc));
     }
     m.b[k] = bv; m.cW[k] = rw; m.cE[k] = re; m.cS[k] = rs; m.cN[k] = rn; m.dg[k] = mm;
     pv[k] = p;
@@ -895,7 +901,9 @@ __global__ void __launch_bounds__(NT) k_vmg_store_cross(Grid g, const T* __restr
   union_ij(g, i, j);
   const int nx = g.nx, ny = g.ny, nh = g.nh;
   const int j_v_max = g.per_y ? ny : ny - 1;
-  auto vs = [&](int ii, int jj) -> T {   // the stored v at (ii, jj), periodic ghosts from their images
+  // Where the stored v at (ii, jj) comes from (periodic ghosts from their images): the compact
+  // result (kind 0, index q), a wall row of rhs (kind 1, padded index q) or zero (kind 2).
+  auto where = [&](int ii, int jj, int& kind, size_t& q) {
     if (g.per_x) {
       if (ii >= 1 - nh && ii <= 0) ii += nx;
       else if (ii >= nx + 1 && ii <= nx + nh) ii -= nx;
@@ -904,18 +912,45 @@ __global__ void __launch_bounds__(NT) k_vmg_store_cross(Grid g, const T* __restr
       if (jj >= 1 - nh && jj <= 0) jj += ny;
       else if (jj >= ny + 1 && jj <= ny + nh) jj -= ny;
     }
-    if (ii >= 1 && ii <= nx && jj >= 1 && jj <= j_v_max) return phi[(ii - 1) + (size_t)(jj - 1) * nx];
-    if (!g.per_y && ii >= 1 && ii <= nx && (jj == 0 || jj == ny)) return rhs[IV(g, ii, jj)];
-    return T(0);
+    kind = 2;
+    q = 0;
+    if (ii >= 1 && ii <= nx && jj >= 1 && jj <= j_v_max) {
+      kind = 0;
+      q = (ii - 1) + (size_t)(jj - 1) * nx;
+    } else if (!g.per_y && ii >= 1 && ii <= nx && (jj == 0 || jj == ny)) {
+      kind = 1;
+      q = IV(g, ii, jj);
+    }
   };
-  if (i >= 1 - nh && i <= nx + nh && j >= -nh && j <= ny + nh) rv[IV(g, i, j)] = vs(i, j);
+  // the four stored values this point needs, (i, j), (i+1, j), (i, j-1), (i+1, j-1): every load
+  // issued before any is used
+  int kd[4];
+  size_t qq[4];
+  where(i, j, kd[0], qq[0]);
+  where(i + 1, j, kd[1], qq[1]);
+  where(i, j - 1, kd[2], qq[2]);
+  where(i + 1, j - 1, kd[3], qq[3]);
+  T v[4];
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    T x = T(0);
+    if (kd[k] == 0) x = phi[qq[k]];
+    else if (kd[k] == 1) x = rhs[qq[k]];
+    v[k] = x;
+  }
   const int i_u_max = g.per_x ? nx : nx - 1;
-  if (i >= 1 && i <= i_u_max && j >= 1 && j <= ny) {
-    T val = zuw[IU(g, i, j)];
+  const bool urow = i >= 1 && i <= i_u_max && j >= 1 && j <= ny;
+  T val = T(0), e0 = T(0), e1 = T(0);
+  if (urow) {
+    val = zuw[IU(g, i, j)];
+    e0 = eta[IW(g, i, j)];
+    e1 = eta[IW(g, i, j - 1)];
+  }
+  if (i >= 1 - nh && i <= nx + nh && j >= -nh && j <= ny + nh) rv[IV(g, i, j)] = v[0];
+  if (urow) {
     if (i <= nx - 1) {
       const T inv_dx = (T)g.inv_dx, inv_dy = (T)g.inv_dy;
-      const T cross = -inv_dy * (eta[IW(g, i, j)] * (vs(i + 1, j) - vs(i, j)) * inv_dx -
-                                 eta[IW(g, i, j - 1)] * (vs(i + 1, j - 1) - vs(i, j - 1)) * inv_dx);
+      const T cross = -inv_dy * (e0 * (v[1] - v[0]) * inv_dx - e1 * (v[3] - v[2]) * inv_dx);
       val = val - cross;
     }
     bu[(i - 1) + (size_t)(j - 1) * i_u_max] = val;
@@ -934,47 +969,75 @@ __global__ void __launch_bounds__(NT) k_vmg_store_bcs(Grid g, const T* __restric
   union_ij(g, i, j);
   const int nx = g.nx, ny = g.ny, nh = g.nh;
   const int i_u_max = g.per_x ? nx : nx - 1;
-  auto us = [&](int ii, int jj) -> T {   // the stored u at (ii, jj), periodic ghosts from their images
-    if (g.per_x) {
-      if (ii >= 1 - nh && ii <= 0) ii += nx;
-      else if (ii >= nx + 1 && ii <= nx + nh) ii -= nx;
-    }
-    if (g.per_y) {
-      if (jj >= 1 - nh && jj <= 0) jj += ny;
-      else if (jj >= ny + 1 && jj <= ny + nh) jj -= ny;
-    }
-    if (jj >= 1 && jj <= ny && ii >= 1 && ii <= i_u_max) return phi[(ii - 1) + (size_t)(jj - 1) * i_u_max];
-    if (!g.per_x && jj >= 1 && jj <= ny && (ii == 0 || ii == nx)) return rhs[IU(g, ii, jj)];
-    return T(0);
-  };
-  auto uax = [&](int ii, int jj) -> T {   // u_after_x on the stored values
-    if (g.per_x) return us(ii, jj);
-    if (ii == 0 || ii == nx) return T(0);
-    if (ii < 0 && ii >= -nh) return -us(-ii, jj);
-    if (ii > nx && ii <= nx + nh) return -us(2 * nx - ii, jj);
-    return us(ii, jj);
-  };
-  if (i >= -nh && i <= nx + nh && j >= 1 - nh && j <= ny + nh) {
+  // Each output is one stored value with signs applied: its source (the compact result, an rhs
+  // wall face, or zero) is resolved first and both outputs' loads are issued before either value
+  // is used (a use between loads waits for all of them).  Signs and zeros are applied with the
+  // same operations as the store / halo / BC sequence: -x for a mirrored value, 0 - x at the top
+  // wall (the zero lid), a literal zero on wall faces.
+  // ru at (i, j)
+  const bool u_here = i >= -nh && i <= nx + nh && j >= 1 - nh && j <= ny + nh;
+  int ui = i, uj = j;
+  int outer = 0;         // 0: x, 1: -x, 2: 0 - x
+  bool inner_neg = false, inner_zero = false;
+  {
     const bool xt = !g.per_x && (i <= 0 || i >= nx);
     const bool yt = !g.per_y && (j <= 0 || j >= ny + 1);
-    T val;
-    if (yt) val = (j <= 0) ? -uax(i, 1 - j) : (T)(2.0 * 0.0) - uax(i, 2 * ny + 1 - j);
-    else if (xt) val = uax(i, j);
-    else val = us(i, j);
-    ru[IU(g, i, j)] = val;
-  }
-  if (i >= 1 - nh && i <= nx + nh && j >= -nh && j <= ny + nh) {
-    const bool xt = !g.per_x && (i <= 0 || i >= nx + 1);
-    const bool yt = !g.per_y && (j <= 0 || j >= ny);
+    bool ax = false;     // through u_after_x
     if (yt) {
-      T val;
-      if (j == 0 || j == ny) val = T(0);
-      else if (j < 0) val = -v_after_x(g, (const T*)rv, i, -j);
-      else val = -v_after_x(g, (const T*)rv, i, 2 * ny - j);
-      rv[IV(g, i, j)] = val;
+      ax = true;
+      if (j <= 0) { outer = 1; uj = 1 - j; }
+      else { outer = 2; uj = 2 * ny + 1 - j; }
     } else if (xt) {
-      rv[IV(g, i, j)] = v_after_x(g, (const T*)rv, i, j);
+      ax = true;
     }
+    if (ax && !g.per_x) {
+      if (ui == 0 || ui == nx) inner_zero = true;
+      else if (ui < 0 && ui >= -nh) { inner_neg = true; ui = -ui; }
+      else if (ui > nx && ui <= nx + nh) { inner_neg = true; ui = 2 * nx - ui; }
+    }
+  }
+  // the stored u at (ui, uj): periodic ghosts from their images, then the compact result, a
+  // wall face of rhs, or zero
+  if (g.per_x) {
+    if (ui >= 1 - nh && ui <= 0) ui += nx;
+    else if (ui >= nx + 1 && ui <= nx + nh) ui -= nx;
+  }
+  if (g.per_y) {
+    if (uj >= 1 - nh && uj <= 0) uj += ny;
+    else if (uj >= ny + 1 && uj <= ny + nh) uj -= ny;
+  }
+  const bool u_phi = uj >= 1 && uj <= ny && ui >= 1 && ui <= i_u_max;
+  const bool u_rhs = !u_phi && !g.per_x && uj >= 1 && uj <= ny && (ui == 0 || ui == nx);
+  // rv at (i, j): BC positions only, from rv itself (positions the BCs do not write)
+  const bool v_here = i >= 1 - nh && i <= nx + nh && j >= -nh && j <= ny + nh;
+  const bool vxt = !g.per_x && (i <= 0 || i >= nx + 1);
+  const bool vyt = !g.per_y && (j <= 0 || j >= ny);
+  const bool v_store = v_here && (vyt || vxt);
+  bool v_zero = false, v_outer_neg = false, v_inner_neg = false;
+  int vi = i, vj = j;
+  if (vyt) {
+    if (j == 0 || j == ny) v_zero = true;
+    else { v_outer_neg = true; vj = (j < 0) ? -j : 2 * ny - j; }
+  }
+  if (!g.per_x) {
+    if (vi <= 0 && vi >= 1 - nh) { v_inner_neg = true; vi = 1 - vi; }
+    else if (vi > nx && vi <= nx + nh) { v_inner_neg = true; vi = 2 * nx + 1 - vi; }
+  }
+  // one unconditional load each, from element 0 where nothing is needed (its value is unused)
+  const bool u_load = u_here && !inner_zero && (u_phi || u_rhs);
+  const bool v_load = v_store && !v_zero;
+  const T* usrc = u_phi ? phi : rhs;
+  const size_t uq = !u_load ? 0 : (u_phi ? (ui - 1) + (size_t)(uj - 1) * i_u_max : IU(g, ui, uj));
+  const T xu0 = usrc[uq];
+  const T xv0 = rv[v_load ? IV(g, vi, vj) : 0];
+  const T xu = u_load ? xu0 : T(0), xv = v_load ? xv0 : T(0);
+  if (u_here) {
+    const T y = inner_zero ? T(0) : (inner_neg ? -xu : xu);
+    ru[IU(g, i, j)] = outer == 1 ? -y : (outer == 2 ? (T)(2.0 * 0.0) - y : y);
+  }
+  if (v_store) {
+    const T y = v_inner_neg ? -xv : xv;
+    rv[IV(g, i, j)] = v_zero ? T(0) : (v_outer_neg ? -y : y);
   }
 }
 
