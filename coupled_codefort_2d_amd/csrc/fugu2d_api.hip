@@ -8,6 +8,7 @@
 #include "fugu2d_seqsum.hpp"
 #include "fugu2d_io.hpp"
 #include "fugu2d_mg.hpp"
+#include "fugu2d_mgs.hpp"
 #include "fugu2d_adjoint.hpp"
 #include "fugu2d_iface_adj.hpp"
 
@@ -161,6 +162,9 @@ struct fugu_ctx {
   int mg_tail_cells = -1;    // coarse-tail start: < 0 the LDS-pool fit, > 0 a cell count, 0 never
   bool mg_fused = true;       // fused (temporal-blocking) level kernels instead of one launch per half-sweep
   bool mg_tail_stage = true; // stage the coarse tail in LDS when it fits
+  // register-strip level kernels (fugu2d_mgs.hpp): 1 on the large levels (mg_use_strip), 0 never,
+  // 2 on every fused level (bit-identical either way)
+  int mg_strip = 1;
   // red-black sweeps before and after each coarse correction: pressure and velocity hierarchies
   // (three on the velocity levels: fewer FGMRES steps, -5 % solve time on viscous-dominated blocks)
   int mg_nu_p = 2, mg_nu_v = 3;
@@ -797,6 +801,72 @@ inline const char* mg_kname(bool post, int nu, bool f64) {
   return names[post ? 1 : 0][f64 ? 1 : 0][(nu >= 1 && nu <= 4) ? nu : 2];
 }
 
+// Register-strip level kernels (fugu2d_mgs.hpp) on the large levels: at least 2^21 cells, or
+// 2^20 with three or more sweeps.  Measured against the LDS-tile kernels (same results bit for
+// bit): at 2048^2 the FP64 three-sweep velocity levels -25 % (pre 76 -> 57 us, post 72 -> 54 us),
+// FP32 -7 to -27 %; below 2^20 cells the tile kernels are faster (a strip tile holds three times
+// the cells, so a small level's few workgroups each take longer).
+inline bool mg_use_strip(const fugu_ctx* c, const MgDims& d, int nu) {
+  if (c->mg_strip == 0) return false;
+  if (c->mg_strip == 2) return true;
+  const double n = (double)d.nx * d.ny;
+  return n >= (double)(1 << 21) || (n >= (double)(1 << 20) && nu >= 3);
+}
+inline const char* mgs_kname(bool post, int nu, bool f64) {
+  static const char* const names[2][2][5] = {
+      {{"", "mgs_pre1_f32", "mgs_pre_f32", "mgs_pre3_f32", "mgs_pre4_f32"},
+       {"", "mgs_pre1_f64", "mgs_pre_f64", "mgs_pre3_f64", "mgs_pre4_f64"}},
+      {{"", "mgs_post1_f32", "mgs_post_f32", "mgs_post3_f32", "mgs_post4_f32"},
+       {"", "mgs_post1_f64", "mgs_post_f64", "mgs_post3_f64", "mgs_post4_f64"}}};
+  return names[post ? 1 : 0][f64 ? 1 : 0][(nu >= 1 && nu <= 4) ? nu : 2];
+}
+template <typename T, int NU, int R, int NW>
+void mgs_pre_launch(fugu_ctx* c, const char* nm, double ab, const MgLevel<T>& L, T* out, T* bc, T* phic,
+                    const T* g_in) {
+  const dim3 g = mgs_grid_pre<NU, R, NW>(L.d);
+  if (L.m)
+    TLAUNCH(c, nm, ab, (k_mgs_pre<T, NU, R, NW, true>), g, dim3(64 * NW), c->s, L.d, (const T*)L.cx, (const T*)L.cy,
+            (const T*)L.m, (const T*)L.b, out, bc, phic, g_in);
+  else
+    TLAUNCH(c, nm, ab, (k_mgs_pre<T, NU, R, NW, false>), g, dim3(64 * NW), c->s, L.d, (const T*)L.cx, (const T*)L.cy,
+            (const T*)nullptr, (const T*)L.b, out, bc, phic, g_in);
+}
+// Pre-smoothing (+ restriction when bc is set) by a strip kernel; shapes by measurement: FP64
+// with two sweeps 4 rows x 16 wavefronts, otherwise 8 x 8.
+template <typename T>
+void mgs_pre(fugu_ctx* c, const char* nm, double ab, const MgLevel<T>& L, int nu, T* out, T* bc, T* phic,
+             const T* g_in) {
+  if (nu == 1) mgs_pre_launch<T, 1, 8, 8>(c, nm, ab, L, out, bc, phic, g_in);
+  else if (nu == 3) mgs_pre_launch<T, 3, 8, 8>(c, nm, ab, L, out, bc, phic, g_in);
+  else if (nu == 4) mgs_pre_launch<T, 4, 8, 8>(c, nm, ab, L, out, bc, phic, g_in);
+  else if (sizeof(T) == 8) mgs_pre_launch<T, 2, 4, 16>(c, nm, ab, L, out, bc, phic, g_in);
+  else mgs_pre_launch<T, 2, 8, 8>(c, nm, ab, L, out, bc, phic, g_in);
+}
+template <typename T, int NU, int R, int NW>
+int mgs_post_launch(fugu_ctx* c, const char* nm, double ab, const MgLevel<T>& L, const T* phi, const T* phic,
+                    T* out, double* psum) {
+  const dim3 g = mgs_grid_post<NU, R, NW>(L.d);
+  if (L.m)
+    TLAUNCH(c, nm, ab, (k_mgs_post<T, NU, R, NW, true>), g, dim3(64 * NW), c->s, L.d, (const T*)L.cx,
+            (const T*)L.cy, (const T*)L.m, (const T*)L.b, phi, phic, out, psum);
+  else
+    TLAUNCH(c, nm, ab, (k_mgs_post<T, NU, R, NW, false>), g, dim3(64 * NW), c->s, L.d, (const T*)L.cx,
+            (const T*)L.cy, (const T*)nullptr, (const T*)L.b, phi, phic, out, psum);
+  return (int)(g.x * g.y);
+}
+// Prolongation + post-smoothing by a strip kernel (FP64 with two sweeps: 4 rows x 8 wavefronts,
+// which fits two workgroups per CU; otherwise 8 x 8); returns its workgroup count (the psum
+// partials).
+template <typename T>
+int mgs_post(fugu_ctx* c, const char* nm, double ab, const MgLevel<T>& L, int nu, const T* phi, const T* phic,
+             T* out, double* psum) {
+  if (nu == 1) return mgs_post_launch<T, 1, 8, 8>(c, nm, ab, L, phi, phic, out, psum);
+  if (nu == 3) return mgs_post_launch<T, 3, 8, 8>(c, nm, ab, L, phi, phic, out, psum);
+  if (nu == 4) return mgs_post_launch<T, 4, 8, 8>(c, nm, ab, L, phi, phic, out, psum);
+  if (sizeof(T) == 8) return mgs_post_launch<T, 2, 4, 8>(c, nm, ab, L, phi, phic, out, psum);
+  return mgs_post_launch<T, 2, 8, 8>(c, nm, ab, L, phi, phic, out, psum);
+}
+
 // V-cycle on level l; `zero_guess`: phi of level l is (treated as) zero on entry; `project`:
 // the operator is singular (no mass term, no Dirichlet face) and the coarsest rhs is centred.
 // nl: the levels in use (a velocity hierarchy may be cut short, its last one mass-dominated).
@@ -865,6 +935,11 @@ T* mg_vcycle(fugu_ctx* c, std::vector<MgLevel<T>>& lv, size_t l, bool zero_guess
         const double ab = (g_in ? 6.0 : 5.0) * sizeof(T) * L.d.nx * L.d.ny;
         const char* nm = sizeof(T) == 8 ? "mg_cut_f64" : "mg_cut_f32";
         T* out = (g_in == L.phi) ? L.phi2 : L.phi;
+        if (mg_use_strip(c, L.d, cnu)) {
+          mgs_pre<T>(c, sizeof(T) == 8 ? "mgs_cut_f64" : "mgs_cut_f32", ab, L, (cnu == 3 || cnu == 4) ? cnu : 2, out,
+                     (T*)nullptr, (T*)nullptr, g_in);
+          return out;
+        }
 #define FUGU_MG_CUT(NUV)                                                                                     \
   TLAUNCH(c, nm, ab, (k_mg_pre<T, NUV, MGT_TH>), gt, dim3(16 * MGT_TH), c->s, L.d, (const T*)L.cx,             \
           (const T*)L.cy, (const T*)L.m, (const T*)L.b, out, (T*)nullptr, (T*)nullptr, g_in)
@@ -907,7 +982,10 @@ T* mg_vcycle(fugu_ctx* c, std::vector<MgLevel<T>>& lv, size_t l, bool zero_guess
   const bool pre64 = th64 && nu == 3;
   const dim3 gt = mgt_grid(L.d, pre64 ? 64 : MGT_TH);
   T* P = (g_in == L.phi) ? L.phi2 : L.phi;   // the fused pre-smoothing's output
-  if (c->mg_fused) {
+  if (c->mg_fused && mg_use_strip(c, L.d, nu)) {
+    const double ab = ((L.m ? 5.5 : 4.5) + (g_in ? 1.0 : 0.0)) * sizeof(T) * ncell;
+    mgs_pre<T>(c, mgs_kname(false, nu, sizeof(T) == 8), ab, L, nu, P, C.b, C.phi, g_in);
+  } else if (c->mg_fused) {
     // b, c_x, c_y (, m) (, the initial iterate) read; phi written; coarse b and phi written (a
     // quarter each)
     const char* nm = mg_kname(false, nu, sizeof(T) == 8);
@@ -940,6 +1018,14 @@ T* mg_vcycle(fugu_ctx* c, std::vector<MgLevel<T>>& lv, size_t l, bool zero_guess
             (const T*)L.m, (const T*)L.b, (const T*)L.phi, C.b, C.phi);
   }
   const T* cres = mg_vcycle(c, lv, l + 1, true, project, nl);
+  if (c->mg_fused && mg_use_strip(c, L.d, nu)) {
+    T* Q = (P == L.phi) ? L.phi2 : L.phi;
+    const double pbytes = (L.m ? 6.25 : 5.25) * sizeof(T) * ncell;
+    const int nb = mgs_post<T>(c, mgs_kname(true, nu, sizeof(T) == 8), pbytes, L, nu, (const T*)P, cres, Q,
+                               psum ? psum : (double*)nullptr);
+    if (psum && psum_nb) *psum_nb = nb;
+    return Q;
+  }
   if (c->mg_fused) {
     // phi, coarse phi (a quarter), b, c_x, c_y (, m) read; the other buffer written
     T* Q = (P == L.phi) ? L.phi2 : L.phi;
@@ -2423,6 +2509,7 @@ int fugu_ctx_create(const fugu_grid* grid, const fugu_params* prm, fugu_ctx** ou
     if (const char* tc = std::getenv("FUGU_MG_TAIL_CELLS")) c->mg_tail_cells = std::atoi(tc);
     if (const char* fu = std::getenv("FUGU_MG_FUSED")) c->mg_fused = std::atoi(fu) != 0;
     if (const char* ts = std::getenv("FUGU_MG_TAIL_STAGE")) c->mg_tail_stage = std::atoi(ts) != 0;
+    if (const char* st = std::getenv("FUGU_MG_STRIP")) c->mg_strip = std::atoi(st);
     if (const char* s = std::getenv("FUGU_MG_NU_P")) c->mg_nu_p = std::min(4, std::max(1, std::atoi(s)));
     if (const char* s = std::getenv("FUGU_MV_TILE")) c->mv_tile = std::atoi(s);
     if (const char* s = std::getenv("FUGU_GS_VEC")) c->gs_vec = std::atoi(s) != 0;

@@ -58,16 +58,19 @@ def test_coarse_tail_is_bitwise_the_multilaunch_cycle(monkeypatch, nx, ny, per_x
     """The single-launch coarse tail (LDS-staged or global) and the fused (temporal-blocking)
     level kernels perform the one-launch-per-half-sweep V-cycle's arithmetic: results equal bit
     for bit with both off, at a 4096-cell threshold, with the default (LDS-fit) start and with the
-    whole hierarchy in the tail (odd extents included)."""
+    whole hierarchy in the tail (odd extents included), with the LDS-tile and the register-strip
+    level kernels."""
     g = O.Grid(nx, ny, 1.0, ny / nx, per_x=per_x, per_y=per_y)
     rng = np.random.default_rng(202)
     c, pr, _, _ = two_phase_case(g, rng, rho=(100.0, 1.0))
     O.halo(g, pr["rho_u"], "u"); O.halo(g, pr["rho_v"], "v")
     rhs = rand(g, "p", rng)
     out = {}
-    for cells, fused in (("0", "0"), ("0", "1"), ("4096", "1"), ("1000000", "1"), ("4096", "0"), ("-1", "1")):
+    for cells, fused, strip in (("0", "0", "0"), ("0", "1", "0"), ("4096", "1", "0"), ("1000000", "1", "0"),
+                                ("4096", "0", "0"), ("-1", "1", "0"), ("0", "1", "2"), ("-1", "1", "2")):
         monkeypatch.setenv("FUGU_MG_TAIL_CELLS", cells)
         monkeypatch.setenv("FUGU_MG_FUSED", fused)
+        monkeypatch.setenv("FUGU_MG_STRIP", strip)
         ctx = abi.Context(g.nx, g.ny, g.lx, g.ly, per_x=per_x, per_y=per_y)
         res = []
         for n in (1, 3):
@@ -75,9 +78,9 @@ def test_coarse_tail_is_bitwise_the_multilaunch_cycle(monkeypatch, nx, ny, per_x
             ctx.pressure_mg(phi, rhs, pr["rho_u"], pr["rho_v"], n_cycles=n)
             res.append(phi)
         ctx.close()
-        out[(cells, fused)] = res
+        out[(cells, fused, strip)] = res
     for key in out:
-        for a, b in zip(out[key], out[("0", "0")]):
+        for a, b in zip(out[key], out[("0", "0", "0")]):
             assert np.array_equal(a, b), key
 
 
@@ -90,7 +93,8 @@ def test_coarse_tail_bitwise_in_the_step_solve(monkeypatch, mixed, theta, coarse
     velocity hierarchies at full depth (theta 0), cut by their diffusion-to-mass ratio (3, 10; the
     cut coarsest level smoothed by 2, 3 or 4 sweeps) and cut to the finest level alone (1e9,
     whose sweeps are then the whole velocity block), with one, two or the ratio-chosen number of
-    V-cycles per velocity block."""
+    V-cycles per velocity block; the fused levels by the LDS-tile or (on every level) the
+    register-strip kernels."""
     import bench
     nx, ny = 96, 192
     outs = []
@@ -98,9 +102,10 @@ def test_coarse_tail_bitwise_in_the_step_solve(monkeypatch, mixed, theta, coarse
     monkeypatch.setenv("FUGU_VMG_THETA2", theta)
     monkeypatch.setenv("FUGU_VMG_COARSE_NU", coarse_nu)
     monkeypatch.setenv("FUGU_VMG_CYCLES", cycles)
-    for cells, fused in (("0", "0"), ("-1", "1")):
+    for cells, fused, strip in (("0", "0", "0"), ("-1", "1", "0"), ("-1", "1", "2")):
         monkeypatch.setenv("FUGU_MG_TAIL_CELLS", cells)
         monkeypatch.setenv("FUGU_MG_FUSED", fused)
+        monkeypatch.setenv("FUGU_MG_STRIP", strip)
         p = abi.FuguParams(1000.0, 1.0, 10.0, 0.1, 2.0, 0.0, -1.0, 2, 1e12, 0.5)
         ctx = abi.Context(nx, ny, 1.0, 2.0, False, False, params=p)
         nh = 2
@@ -118,16 +123,17 @@ def test_coarse_tail_bitwise_in_the_step_solve(monkeypatch, mixed, theta, coarse
         f = [ctx.field(k, np.zeros(shape)) for k, shape in (("u", u.shape), ("v", v.shape), ("p", pp.shape))]
         ctx.close()
         outs.append((its, f))
-    assert outs[0][0] == outs[1][0]
-    for a, b in zip(outs[0][1], outs[1][1]):
-        assert np.array_equal(a, b)
+    for o in outs[1:]:
+        assert outs[0][0] == o[0]
+        for a, b in zip(outs[0][1], o[1]):
+            assert np.array_equal(a, b)
 
 
 @pytest.mark.parametrize("nu_p,nu_v", [("1", "2"), ("3", "4"), ("2", "3"), ("4", "1")])
 def test_sweep_counts_fused_is_bitwise_the_multilaunch_cycle(monkeypatch, nu_p, nu_v):
     """Other red-black sweep counts per V-cycle half (pressure and velocity hierarchies set apart):
-    the fused level kernels and the coarse tail still perform the multi-launch cycle's arithmetic,
-    bit for bit, in a refined step solve."""
+    the fused level kernels (LDS tiles, and register strips on every level) and the coarse tail
+    still perform the multi-launch cycle's arithmetic, bit for bit, in a refined step solve."""
     import bench
     nx, ny = 96, 192
     monkeypatch.setenv("FUGU_MG_NU_P", nu_p)
@@ -135,9 +141,10 @@ def test_sweep_counts_fused_is_bitwise_the_multilaunch_cycle(monkeypatch, nu_p, 
     monkeypatch.setenv("FUGU_VMG_THETA", "0")     # full-depth velocity hierarchies: their tail runs too
     monkeypatch.setenv("FUGU_VMG_THETA2", "0")
     outs = []
-    for cells, fused in (("0", "0"), ("-1", "1")):
+    for cells, fused, strip in (("0", "0", "0"), ("-1", "1", "0"), ("-1", "1", "2")):
         monkeypatch.setenv("FUGU_MG_TAIL_CELLS", cells)
         monkeypatch.setenv("FUGU_MG_FUSED", fused)
+        monkeypatch.setenv("FUGU_MG_STRIP", strip)
         p = abi.FuguParams(1000.0, 1.0, 10.0, 0.1, 2.0, 0.0, -1.0, 2, 1e12, 0.5)
         ctx = abi.Context(nx, ny, 1.0, 2.0, False, False, params=p)
         nh = 2
@@ -155,21 +162,26 @@ def test_sweep_counts_fused_is_bitwise_the_multilaunch_cycle(monkeypatch, nu_p, 
         ctx.close()
         outs.append((its, f))
     assert all(rc == 0 and res <= 1e-8 for rc, _, res in outs[0][0]), outs[0][0]
-    assert outs[0][0] == outs[1][0]
-    for a, b, name in zip(outs[0][1], outs[1][1], "uvp"):
-        assert np.array_equal(a, b), (name, np.abs(a - b).max())
+    for o in outs[1:]:
+        assert outs[0][0] == o[0]
+        for a, b, name in zip(outs[0][1], o[1], "uvp"):
+            assert np.array_equal(a, b), (name, np.abs(a - b).max())
 
 
-def test_big_fp64_levels_fused_is_bitwise_the_multilaunch_cycle(monkeypatch):
-    """FP64 levels of 2^21 cells and more take the fused level kernels with 64-row tiles (pre- and
-    post-smoothing): one refined FP64 step solve on 1024 x 2048 cells, bit for bit the multi-launch
-    cycle's."""
+@pytest.mark.parametrize("mixed", [False, True])
+def test_big_levels_fused_is_bitwise_the_multilaunch_cycle(monkeypatch, mixed):
+    """Levels of 2^21 cells and more take the register-strip level kernels (FP64: 4-row strips for
+    the two-sweep pressure levels, 8-row strips for the three-sweep velocity levels; FP32: 8-row
+    strips), and with the strips off the LDS-tile kernels (64-row tiles for the FP64 three-sweep
+    pre-smoothing): one refined step solve on 1024 x 2048 cells, FP64 or mixed, bit for bit the
+    multi-launch cycle's."""
     import bench
     nx, ny = 1024, 2048
     outs = []
-    for cells, fused in (("0", "0"), ("-1", "1")):
+    for cells, fused, strip in (("0", "0", "0"), ("-1", "1", "1"), ("-1", "1", "0")):
         monkeypatch.setenv("FUGU_MG_TAIL_CELLS", cells)
         monkeypatch.setenv("FUGU_MG_FUSED", fused)
+        monkeypatch.setenv("FUGU_MG_STRIP", strip)
         p = abi.FuguParams(1000.0, 1.0, 10.0, 0.1, 2.0, 0.0, -1.0, 2, 1e12, 0.5)
         ctx = abi.Context(nx, ny, 1.0, 2.0, False, False, params=p)
         nh = 2
@@ -177,17 +189,18 @@ def test_big_fp64_levels_fused_is_bitwise_the_multilaunch_cycle(monkeypatch):
         pp = np.zeros((ny + 2 * nh, nx + 2 * nh))
         ctx.upload(u, v, u.copy(), v.copy(), pp, bench.bubble_c(nx, ny, 1.0, 2.0))
         ctx.init_properties()
-        flags = abi.SOLVE_REFINED | abi.PRECOND_MG | abi.PRECOND_VISC | abi.PRECOND_VMG
+        flags = abi.SOLVE_REFINED | abi.PRECOND_MG | abi.PRECOND_VISC | abi.PRECOND_VMG | (abi.SOLVE_MIXED if mixed else 0)
         opts = abi.step_opts(tol=1e-8, max_iter=300, solver_flags=flags, inner_max_iter=-1, restart=30)
         st, rc = ctx.step(2e-4, 1, opts)
         f = [ctx.field(k, np.zeros(shape)) for k, shape in (("u", u.shape), ("v", v.shape), ("p", pp.shape))]
         ctx.close()
         outs.append(((rc, st.outer, st.res), f))
-        print(f"fused={fused}: rc {rc} outer {st.outer} res {st.res:.3e}", flush=True)
+        print(f"fused={fused} strip={strip}: rc {rc} outer {st.outer} res {st.res:.3e}", flush=True)
     assert outs[0][0][0] == 0 and outs[0][0][2] <= 1e-8, outs[0][0]
-    assert outs[0][0] == outs[1][0]
-    for a, b, name in zip(outs[0][1], outs[1][1], "uvp"):
-        assert np.array_equal(a, b), (name, np.abs(a - b).max())
+    for o in outs[1:]:
+        assert outs[0][0] == o[0]
+        for a, b, name in zip(outs[0][1], o[1], "uvp"):
+            assert np.array_equal(a, b), (name, np.abs(a - b).max())
 
 
 def test_odd_periodic_extent_is_refused():
@@ -207,12 +220,14 @@ def test_odd_periodic_extent_is_refused():
 def test_fused_pcd_glue_is_bitwise(monkeypatch, mixed, per_x, per_y):
     """The PCD's fused glue -- the v-block store, periodic halo and cross term in one launch, and
     the u-block store, periodic halo and the velocity BCs FGMRES applies to z_k in another -- gives
-    the fields and step counts of the separate launches, bit for bit (walls, periodic x, periodic y)."""
+    the fields and step counts of the separate launches, bit for bit (walls, periodic x, periodic y);
+    also with the register-strip level kernels on every level."""
     import bench
     nx, ny = 96, 192
     outs = []
-    for fused in ("0", "1"):
+    for fused, strip in (("0", "0"), ("1", "0"), ("1", "2")):
         monkeypatch.setenv("FUGU_PCD_FUSED_GLUE", fused)
+        monkeypatch.setenv("FUGU_MG_STRIP", strip)
         p = abi.FuguParams(1000.0, 1.0, 10.0, 0.1, 2.0, 0.0, -1.0, 2, 1e12, 0.5)
         ctx = abi.Context(nx, ny, 1.0, 2.0, per_x, per_y, params=p)
         nh = 2
@@ -230,6 +245,7 @@ def test_fused_pcd_glue_is_bitwise(monkeypatch, mixed, per_x, per_y):
         ctx.close()
         outs.append((its, f))
     assert all(rc == 0 and res <= 1e-8 for rc, _, res in outs[0][0]), outs[0][0]
-    assert outs[0][0] == outs[1][0]
-    for a, b, name in zip(outs[0][1], outs[1][1], "uvp"):
-        assert np.array_equal(a, b), (name, np.abs(a - b).max())
+    for o in outs[1:]:
+        assert outs[0][0] == o[0]
+        for a, b, name in zip(outs[0][1], o[1], "uvp"):
+            assert np.array_equal(a, b), (name, np.abs(a - b).max())
