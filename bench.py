@@ -28,6 +28,11 @@ driver's default) on the 256^2 static droplet (rho 1000/1, mu 1/0.01, sigma 1, B
 steps 3 and 4 from the state after step 2 (both converge to the reference stopping test 1e-8):
 the GPU in the reference summation order (bit-identical to the oracle, checked), the GPU in the
 tree order, the FP64 fast form, and the oracle on one host core -- the `cpu_baseline`.
+"adjoint2048": the combined 2048^2 workload in one call -- the mixed forward over 32 BDF2 steps
+(dt 5e-5; 256 seeded inclusions r ~ U[0.005, 0.02], carrier rho 1000 / mu 1e-2, inclusions
+rho 1 / mu 1e-2 * 10^U[0,3], sigma 1, g -1) with a restart state every 4 steps, then the full
+adjoint gradient of the same objective (FP64 adjoint solves, each segment's forward states
+recomputed from its checkpoint).
 "pcg2048": the drop-in's PCG kernels (k_pcg_iter1/k_pcg_iter2, 112 B per cell and iteration)
 on 2048^2, 150 iterations, timed with HIP events -- their roofline.
 
@@ -404,6 +409,16 @@ def pcg_roofline(pcg, tag="pcg_2048x2048"):
                 out["profile_avg_launch_us"] = pavg
                 out["frac_at_profile_avg"] = round(v["bytes_per_launch"] / (pavg * 1e-6) / 1e9 / HBM_PEAK_GBS, 4)
                 out["profile_source"] = os.path.relpath(trace, ROOT)
+    # measured HBM bytes / algorithmic bytes of the same kernel (two rocprofv3 --pmc runs)
+    prof = os.path.join(ROOT, "profiles", f"hbm_traffic_{tag}.txt")
+    if os.path.exists(prof):
+        for ln in open(prof):
+            f = ln.split()
+            if f and not ln.startswith("#") and f[0] == name and len(f) >= 6:
+                ratio = float(f[-3])
+                out["traffic"] = round(ratio * v["bytes_per_launch"])
+                out["traffic_ratio"] = ratio
+                out["traffic_source"] = os.path.relpath(prof, ROOT)
     return out
 
 
@@ -503,7 +518,7 @@ def main():
     ap.add_argument("--inner-max", type=int, default=-1, help="inner BiCGStab(2) cap; -1: PCD only")
     ap.add_argument("--restart", type=int, default=0,
                     help="FGMRES restart length; 0: the library's default (20 up to 2^21 cells, 30 above)")
-    ap.add_argument("--sections", default="headline,m1,adjoint,dropin,pcg2048,physics",
+    ap.add_argument("--sections", default="headline,m1,adjoint,adjoint2048,dropin,pcg2048,physics",
                     help="headline (rising bubble, value), m1 (2048 x 2048 FP64 vs mixed), adjoint (s/grad, "
                          "1024^2, 16 steps), adjoint2048 (2048^2, 256 inclusions, 32 steps, mixed forward), "
                          "dropin (reference algorithm on the 256^2 droplet, GPU vs the oracle on one core: "

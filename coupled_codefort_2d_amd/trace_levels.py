@@ -14,7 +14,7 @@ import csv
 import sys
 from collections import defaultdict
 
-PREFIXES = ("k_mg_", "k_vmg_", "k_matvec", "k_mdot", "k_maxpy", "k_pcd", "k_fold", "k_lin")
+PREFIXES = ("k_mg_", "k_mgs_", "k_vmg_", "k_matvec", "k_mdot", "k_maxpy", "k_pcd", "k_fold", "k_lin")
 
 
 def level_stats(src):
