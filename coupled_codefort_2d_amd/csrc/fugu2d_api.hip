@@ -162,8 +162,9 @@ struct fugu_ctx {
   int mg_tail_cells = -1;    // coarse-tail start: < 0 the LDS-pool fit, > 0 a cell count, 0 never
   bool mg_fused = true;       // fused (temporal-blocking) level kernels instead of one launch per half-sweep
   bool mg_tail_stage = true; // stage the coarse tail in LDS when it fits
-  // register-strip level kernels (fugu2d_mgs.hpp): 1 on the large levels (mg_use_strip), 0 never,
-  // 2 on every fused level (bit-identical either way)
+  // register-strip level kernels (fugu2d_mgs.hpp): 1 on the large and the small levels
+  // (mg_strip_shape), 0 never, 2 / 3 the large / small shapes on every fused level (bit-identical
+  // either way)
   int mg_strip = 1;
   // red-black sweeps before and after each coarse correction: pressure and velocity hierarchies
   // (three on the velocity levels: fewer FGMRES steps, -5 % solve time on viscous-dominated blocks)
@@ -801,16 +802,26 @@ inline const char* mg_kname(bool post, int nu, bool f64) {
   return names[post ? 1 : 0][f64 ? 1 : 0][(nu >= 1 && nu <= 4) ? nu : 2];
 }
 
-// Register-strip level kernels (fugu2d_mgs.hpp) on the large levels: at least 2^21 cells, or
-// 2^20 with three or more sweeps.  Measured against the LDS-tile kernels (same results bit for
-// bit): at 2048^2 the FP64 three-sweep velocity levels -25 % (pre 76 -> 57 us, post 72 -> 54 us),
-// FP32 -7 to -27 %; below 2^20 cells the tile kernels are faster (a strip tile holds three times
-// the cells, so a small level's few workgroups each take longer).
-inline bool mg_use_strip(const fugu_ctx* c, const MgDims& d, int nu) {
-  if (c->mg_strip == 0) return false;
-  if (c->mg_strip == 2) return true;
+// Register-strip level kernels (fugu2d_mgs.hpp) on the large and on the small levels, the
+// LDS-tile kernels (fugu2d_mg.hpp) in between.  Measured against the tile kernels (same results
+// bit for bit):
+//  - large (at least 2^21 cells, or 2^20 with three or more sweeps): 8-row strips (FP64 two-sweep
+//    levels: 4 rows); at 2048^2 the FP64 three-sweep velocity levels -25 % (pre 76 -> 57 us, post
+//    72 -> 54 us), FP32 -7 to -27 %;
+//  - small (at most 2^17 cells): 2-row strips (4 rows for the pre-smoothing with three or more
+//    sweeps, whose halo needs a taller tile), each wavefront's chain of instructions a quarter
+//    as long: -24 to -41 % for the two-sweep pressure levels, -15 to -35 % for the velocity
+//    levels (64 x 128 to 256 x 512);
+//  - in between (the bubble's 512 x 1024 levels), the tile kernels are faster.
+enum class MgsShape { none, large, small };
+inline MgsShape mg_strip_shape(const fugu_ctx* c, const MgDims& d, int nu) {
+  if (c->mg_strip == 0) return MgsShape::none;
+  if (c->mg_strip == 2) return MgsShape::large;   // every fused level (tests)
+  if (c->mg_strip == 3) return MgsShape::small;
   const double n = (double)d.nx * d.ny;
-  return n >= (double)(1 << 21) || (n >= (double)(1 << 20) && nu >= 3);
+  if (n >= (double)(1 << 21) || (n >= (double)(1 << 20) && nu >= 3)) return MgsShape::large;
+  if (n <= (double)(1 << 17)) return MgsShape::small;
+  return MgsShape::none;
 }
 inline const char* mgs_kname(bool post, int nu, bool f64) {
   static const char* const names[2][2][5] = {
@@ -819,6 +830,30 @@ inline const char* mgs_kname(bool post, int nu, bool f64) {
       {{"", "mgs_post1_f32", "mgs_post_f32", "mgs_post3_f32", "mgs_post4_f32"},
        {"", "mgs_post1_f64", "mgs_post_f64", "mgs_post3_f64", "mgs_post4_f64"}}};
   return names[post ? 1 : 0][f64 ? 1 : 0][(nu >= 1 && nu <= 4) ? nu : 2];
+}
+// Strip shape (rows per wavefront, wavefronts) of a launch: one table for the dispatch and for
+// the partials count.
+struct MgsCfg {
+  int R, NW;
+};
+inline MgsCfg mgs_cfg(bool post, int nu, MgsShape sh, bool f64) {
+  if (sh == MgsShape::small) {
+    if (post) return nu == 4 ? MgsCfg{4, 8} : MgsCfg{2, 8};
+    return nu >= 3 ? MgsCfg{4, 8} : MgsCfg{2, 8};
+  }
+  if (nu == 2 && f64) return post ? MgsCfg{4, 8} : MgsCfg{4, 16};
+  return MgsCfg{8, 8};
+}
+// Workgroups of a level's fused post-smoothing launch (one double-double partial each).
+inline size_t mg_post_groups(const fugu_ctx* c, const MgDims& d, int nu, bool f64) {
+  const MgsShape sh = c->mg_fused ? mg_strip_shape(c, d, nu) : MgsShape::none;
+  if (sh == MgsShape::none) {
+    const dim3 g = mgt_grid(d, MGT_TH);
+    return (size_t)g.x * g.y;
+  }
+  const MgsCfg k = mgs_cfg(true, nu, sh, f64);
+  const int hp = 2 * nu, tw = MGS_EW - 2 * hp, th = k.NW * k.R - 2 * hp;
+  return (size_t)((d.nx + tw - 1) / tw) * (size_t)((d.ny + th - 1) / th);
 }
 template <typename T, int NU, int R, int NW>
 void mgs_pre_launch(fugu_ctx* c, const char* nm, double ab, const MgLevel<T>& L, T* out, T* bc, T* phic,
@@ -831,16 +866,20 @@ void mgs_pre_launch(fugu_ctx* c, const char* nm, double ab, const MgLevel<T>& L,
     TLAUNCH(c, nm, ab, (k_mgs_pre<T, NU, R, NW, false>), g, dim3(64 * NW), c->s, L.d, (const T*)L.cx, (const T*)L.cy,
             (const T*)nullptr, (const T*)L.b, out, bc, phic, g_in);
 }
-// Pre-smoothing (+ restriction when bc is set) by a strip kernel; shapes by measurement: FP64
-// with two sweeps 4 rows x 16 wavefronts, otherwise 8 x 8.
+// Pre-smoothing (+ restriction when bc is set) by a strip kernel of the given shape.
 template <typename T>
-void mgs_pre(fugu_ctx* c, const char* nm, double ab, const MgLevel<T>& L, int nu, T* out, T* bc, T* phic,
-             const T* g_in) {
-  if (nu == 1) mgs_pre_launch<T, 1, 8, 8>(c, nm, ab, L, out, bc, phic, g_in);
-  else if (nu == 3) mgs_pre_launch<T, 3, 8, 8>(c, nm, ab, L, out, bc, phic, g_in);
-  else if (nu == 4) mgs_pre_launch<T, 4, 8, 8>(c, nm, ab, L, out, bc, phic, g_in);
-  else if (sizeof(T) == 8) mgs_pre_launch<T, 2, 4, 16>(c, nm, ab, L, out, bc, phic, g_in);
-  else mgs_pre_launch<T, 2, 8, 8>(c, nm, ab, L, out, bc, phic, g_in);
+void mgs_pre(fugu_ctx* c, const char* nm, double ab, const MgLevel<T>& L, int nu, MgsShape sh, T* out, T* bc,
+             T* phic, const T* g_in) {
+  const MgsCfg k = mgs_cfg(false, nu, sh, sizeof(T) == 8);
+#define FUGU_MGS_PRE(NUV, RV, NWV)                                            \
+  if (nu == NUV && k.R == RV && k.NW == NWV) {                                \
+    mgs_pre_launch<T, NUV, RV, NWV>(c, nm, ab, L, out, bc, phic, g_in);       \
+    return;                                                                   \
+  }
+  FUGU_MGS_PRE(1, 2, 8) FUGU_MGS_PRE(2, 2, 8) FUGU_MGS_PRE(3, 4, 8) FUGU_MGS_PRE(4, 4, 8)
+  FUGU_MGS_PRE(1, 8, 8) FUGU_MGS_PRE(2, 8, 8) FUGU_MGS_PRE(3, 8, 8) FUGU_MGS_PRE(4, 8, 8) FUGU_MGS_PRE(2, 4, 16)
+#undef FUGU_MGS_PRE
+  throw Error("multigrid: no strip kernel for this shape");
 }
 template <typename T, int NU, int R, int NW>
 int mgs_post_launch(fugu_ctx* c, const char* nm, double ab, const MgLevel<T>& L, const T* phi, const T* phic,
@@ -854,17 +893,18 @@ int mgs_post_launch(fugu_ctx* c, const char* nm, double ab, const MgLevel<T>& L,
             (const T*)L.cy, (const T*)nullptr, (const T*)L.b, phi, phic, out, psum);
   return (int)(g.x * g.y);
 }
-// Prolongation + post-smoothing by a strip kernel (FP64 with two sweeps: 4 rows x 8 wavefronts,
-// which fits two workgroups per CU; otherwise 8 x 8); returns its workgroup count (the psum
-// partials).
+// Prolongation + post-smoothing by a strip kernel of the given shape; returns its workgroup
+// count (the psum partials).
 template <typename T>
-int mgs_post(fugu_ctx* c, const char* nm, double ab, const MgLevel<T>& L, int nu, const T* phi, const T* phic,
-             T* out, double* psum) {
-  if (nu == 1) return mgs_post_launch<T, 1, 8, 8>(c, nm, ab, L, phi, phic, out, psum);
-  if (nu == 3) return mgs_post_launch<T, 3, 8, 8>(c, nm, ab, L, phi, phic, out, psum);
-  if (nu == 4) return mgs_post_launch<T, 4, 8, 8>(c, nm, ab, L, phi, phic, out, psum);
-  if (sizeof(T) == 8) return mgs_post_launch<T, 2, 4, 8>(c, nm, ab, L, phi, phic, out, psum);
-  return mgs_post_launch<T, 2, 8, 8>(c, nm, ab, L, phi, phic, out, psum);
+int mgs_post(fugu_ctx* c, const char* nm, double ab, const MgLevel<T>& L, int nu, MgsShape sh, const T* phi,
+             const T* phic, T* out, double* psum) {
+  const MgsCfg k = mgs_cfg(true, nu, sh, sizeof(T) == 8);
+#define FUGU_MGS_POST(NUV, RV, NWV) \
+  if (nu == NUV && k.R == RV && k.NW == NWV) return mgs_post_launch<T, NUV, RV, NWV>(c, nm, ab, L, phi, phic, out, psum);
+  FUGU_MGS_POST(1, 2, 8) FUGU_MGS_POST(2, 2, 8) FUGU_MGS_POST(3, 2, 8) FUGU_MGS_POST(4, 4, 8)
+  FUGU_MGS_POST(1, 8, 8) FUGU_MGS_POST(2, 8, 8) FUGU_MGS_POST(3, 8, 8) FUGU_MGS_POST(4, 8, 8) FUGU_MGS_POST(2, 4, 8)
+#undef FUGU_MGS_POST
+  throw Error("multigrid: no strip kernel for this shape");
 }
 
 // V-cycle on level l; `zero_guess`: phi of level l is (treated as) zero on entry; `project`:
@@ -935,9 +975,11 @@ T* mg_vcycle(fugu_ctx* c, std::vector<MgLevel<T>>& lv, size_t l, bool zero_guess
         const double ab = (g_in ? 6.0 : 5.0) * sizeof(T) * L.d.nx * L.d.ny;
         const char* nm = sizeof(T) == 8 ? "mg_cut_f64" : "mg_cut_f32";
         T* out = (g_in == L.phi) ? L.phi2 : L.phi;
-        if (mg_use_strip(c, L.d, cnu)) {
-          mgs_pre<T>(c, sizeof(T) == 8 ? "mgs_cut_f64" : "mgs_cut_f32", ab, L, (cnu == 3 || cnu == 4) ? cnu : 2, out,
-                     (T*)nullptr, (T*)nullptr, g_in);
+        const int snu = (cnu == 3 || cnu == 4) ? cnu : 2;
+        const MgsShape sh = mg_strip_shape(c, L.d, snu);
+        if (sh != MgsShape::none) {
+          mgs_pre<T>(c, sizeof(T) == 8 ? "mgs_cut_f64" : "mgs_cut_f32", ab, L, snu, sh, out, (T*)nullptr,
+                     (T*)nullptr, g_in);
           return out;
         }
 #define FUGU_MG_CUT(NUV)                                                                                     \
@@ -982,9 +1024,10 @@ T* mg_vcycle(fugu_ctx* c, std::vector<MgLevel<T>>& lv, size_t l, bool zero_guess
   const bool pre64 = th64 && nu == 3;
   const dim3 gt = mgt_grid(L.d, pre64 ? 64 : MGT_TH);
   T* P = (g_in == L.phi) ? L.phi2 : L.phi;   // the fused pre-smoothing's output
-  if (c->mg_fused && mg_use_strip(c, L.d, nu)) {
+  const MgsShape shape = c->mg_fused ? mg_strip_shape(c, L.d, nu) : MgsShape::none;
+  if (shape != MgsShape::none) {
     const double ab = ((L.m ? 5.5 : 4.5) + (g_in ? 1.0 : 0.0)) * sizeof(T) * ncell;
-    mgs_pre<T>(c, mgs_kname(false, nu, sizeof(T) == 8), ab, L, nu, P, C.b, C.phi, g_in);
+    mgs_pre<T>(c, mgs_kname(false, nu, sizeof(T) == 8), ab, L, nu, shape, P, C.b, C.phi, g_in);
   } else if (c->mg_fused) {
     // b, c_x, c_y (, m) (, the initial iterate) read; phi written; coarse b and phi written (a
     // quarter each)
@@ -1018,10 +1061,10 @@ T* mg_vcycle(fugu_ctx* c, std::vector<MgLevel<T>>& lv, size_t l, bool zero_guess
             (const T*)L.m, (const T*)L.b, (const T*)L.phi, C.b, C.phi);
   }
   const T* cres = mg_vcycle(c, lv, l + 1, true, project, nl);
-  if (c->mg_fused && mg_use_strip(c, L.d, nu)) {
+  if (shape != MgsShape::none) {
     T* Q = (P == L.phi) ? L.phi2 : L.phi;
     const double pbytes = (L.m ? 6.25 : 5.25) * sizeof(T) * ncell;
-    const int nb = mgs_post<T>(c, mgs_kname(true, nu, sizeof(T) == 8), pbytes, L, nu, (const T*)P, cres, Q,
+    const int nb = mgs_post<T>(c, mgs_kname(true, nu, sizeof(T) == 8), pbytes, L, nu, shape, (const T*)P, cres, Q,
                                psum ? psum : (double*)nullptr);
     if (psum && psum_nb) *psum_nb = nb;
     return Q;
@@ -1084,8 +1127,8 @@ const T* mg_pressure_compact(fugu_ctx* c, Work<T>& w, const T* Zp) {
   MgLevel<T>& L0 = w.mg[0];
   LAUNCH((k_mg_load<T>), g2(g.nx, g.ny), dim3(64, 4), c->s, g, Zp, L0.b, centre, (const double*)c->sc,
          (int)S_P_MEAN, c->fl, (int)F_P_DONE);
-  const dim3 gp0 = mgt_grid(L0.d, MGT_TH);   // the level-0 post-smoothing's tiles
-  const size_t tiles = (size_t)gp0.x * gp0.y;
+  // the level-0 post-smoothing's workgroups (one partial each), when it is a fused launch
+  const size_t tiles = mg_post_groups(c, L0.d, c->mg_nu_p, sizeof(T) == 8);
   double* psum = (walls_both && 2 * tiles <= c->partials_n) ? c->partials : nullptr;
   const T* res = nullptr;
   int nb = 0;

@@ -59,7 +59,7 @@ def test_coarse_tail_is_bitwise_the_multilaunch_cycle(monkeypatch, nx, ny, per_x
     level kernels perform the one-launch-per-half-sweep V-cycle's arithmetic: results equal bit
     for bit with both off, at a 4096-cell threshold, with the default (LDS-fit) start and with the
     whole hierarchy in the tail (odd extents included), with the LDS-tile and the register-strip
-    level kernels."""
+    level kernels (large and small strip shapes)."""
     g = O.Grid(nx, ny, 1.0, ny / nx, per_x=per_x, per_y=per_y)
     rng = np.random.default_rng(202)
     c, pr, _, _ = two_phase_case(g, rng, rho=(100.0, 1.0))
@@ -67,7 +67,8 @@ def test_coarse_tail_is_bitwise_the_multilaunch_cycle(monkeypatch, nx, ny, per_x
     rhs = rand(g, "p", rng)
     out = {}
     for cells, fused, strip in (("0", "0", "0"), ("0", "1", "0"), ("4096", "1", "0"), ("1000000", "1", "0"),
-                                ("4096", "0", "0"), ("-1", "1", "0"), ("0", "1", "2"), ("-1", "1", "2")):
+                                ("4096", "0", "0"), ("-1", "1", "0"), ("0", "1", "2"), ("-1", "1", "2"),
+                                ("0", "1", "3"), ("-1", "1", "3")):
         monkeypatch.setenv("FUGU_MG_TAIL_CELLS", cells)
         monkeypatch.setenv("FUGU_MG_FUSED", fused)
         monkeypatch.setenv("FUGU_MG_STRIP", strip)
@@ -94,7 +95,7 @@ def test_coarse_tail_bitwise_in_the_step_solve(monkeypatch, mixed, theta, coarse
     cut coarsest level smoothed by 2, 3 or 4 sweeps) and cut to the finest level alone (1e9,
     whose sweeps are then the whole velocity block), with one, two or the ratio-chosen number of
     V-cycles per velocity block; the fused levels by the LDS-tile or (on every level) the
-    register-strip kernels."""
+    register-strip kernels of either shape."""
     import bench
     nx, ny = 96, 192
     outs = []
@@ -102,7 +103,7 @@ def test_coarse_tail_bitwise_in_the_step_solve(monkeypatch, mixed, theta, coarse
     monkeypatch.setenv("FUGU_VMG_THETA2", theta)
     monkeypatch.setenv("FUGU_VMG_COARSE_NU", coarse_nu)
     monkeypatch.setenv("FUGU_VMG_CYCLES", cycles)
-    for cells, fused, strip in (("0", "0", "0"), ("-1", "1", "0"), ("-1", "1", "2")):
+    for cells, fused, strip in (("0", "0", "0"), ("-1", "1", "0"), ("-1", "1", "2"), ("-1", "1", "3")):
         monkeypatch.setenv("FUGU_MG_TAIL_CELLS", cells)
         monkeypatch.setenv("FUGU_MG_FUSED", fused)
         monkeypatch.setenv("FUGU_MG_STRIP", strip)
@@ -132,7 +133,7 @@ def test_coarse_tail_bitwise_in_the_step_solve(monkeypatch, mixed, theta, coarse
 @pytest.mark.parametrize("nu_p,nu_v", [("1", "2"), ("3", "4"), ("2", "3"), ("4", "1")])
 def test_sweep_counts_fused_is_bitwise_the_multilaunch_cycle(monkeypatch, nu_p, nu_v):
     """Other red-black sweep counts per V-cycle half (pressure and velocity hierarchies set apart):
-    the fused level kernels (LDS tiles, and register strips on every level) and the coarse tail
+    the fused level kernels (LDS tiles, and register strips of either shape on every level) and the coarse tail
     still perform the multi-launch cycle's arithmetic, bit for bit, in a refined step solve."""
     import bench
     nx, ny = 96, 192
@@ -141,7 +142,7 @@ def test_sweep_counts_fused_is_bitwise_the_multilaunch_cycle(monkeypatch, nu_p, 
     monkeypatch.setenv("FUGU_VMG_THETA", "0")     # full-depth velocity hierarchies: their tail runs too
     monkeypatch.setenv("FUGU_VMG_THETA2", "0")
     outs = []
-    for cells, fused, strip in (("0", "0", "0"), ("-1", "1", "0"), ("-1", "1", "2")):
+    for cells, fused, strip in (("0", "0", "0"), ("-1", "1", "0"), ("-1", "1", "2"), ("-1", "1", "3")):
         monkeypatch.setenv("FUGU_MG_TAIL_CELLS", cells)
         monkeypatch.setenv("FUGU_MG_FUSED", fused)
         monkeypatch.setenv("FUGU_MG_STRIP", strip)
@@ -221,11 +222,11 @@ def test_fused_pcd_glue_is_bitwise(monkeypatch, mixed, per_x, per_y):
     """The PCD's fused glue -- the v-block store, periodic halo and cross term in one launch, and
     the u-block store, periodic halo and the velocity BCs FGMRES applies to z_k in another -- gives
     the fields and step counts of the separate launches, bit for bit (walls, periodic x, periodic y);
-    also with the register-strip level kernels on every level."""
+    also with the register-strip level kernels (either shape) on every level."""
     import bench
     nx, ny = 96, 192
     outs = []
-    for fused, strip in (("0", "0"), ("1", "0"), ("1", "2")):
+    for fused, strip in (("0", "0"), ("1", "0"), ("1", "2"), ("1", "3")):
         monkeypatch.setenv("FUGU_PCD_FUSED_GLUE", fused)
         monkeypatch.setenv("FUGU_MG_STRIP", strip)
         p = abi.FuguParams(1000.0, 1.0, 10.0, 0.1, 2.0, 0.0, -1.0, 2, 1e12, 0.5)
