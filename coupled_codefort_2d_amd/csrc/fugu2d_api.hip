@@ -133,6 +133,7 @@ struct fugu_ctx {
   double* dclamp = nullptr;                  // per-cell clamp correction of the step's VOF update
   double* h_sc = nullptr;
   int* h_fl = nullptr;
+  int* d_hfl = nullptr;   // h_fl as the device addresses it (mapped pinned memory)
   size_t ntrip = 0;
   // resident state: (u, v, p) is one contiguous triplet
   double *X = nullptr, *u = nullptr, *v = nullptr, *p = nullptr;
@@ -1759,7 +1760,7 @@ void solve_fgmres(fugu_ctx* c, Work<T>& w, double* X, double dt, double tol, dou
               (const int*)c->fl, -1);
     if (gs_mode)
       LAUNCH(k_fold_gs_norm, dim3(1), dim3(NT), c->s, (int)nb, gs_mode, c->g.vol, (const double*)c->partials,
-             c->sc, c->fl, fg_k, c->fg);
+             c->sc, c->fl, fg_k, c->fg, fg_k >= 0 ? c->d_hfl + F_COUNT + (fg_k & 1) : (int*)nullptr);
   };
   // W -= V h1 (coefficients at S_HBUF) fused with <W', V_i> -> S_HBUF + 64 + i when the
   // re-orthogonalisation test fires (zero otherwise), ||W'||^2 -> S_DOT0, the test -> F_REORTH;
@@ -1783,7 +1784,8 @@ void solve_fgmres(fugu_ctx* c, Work<T>& w, double* X, double dt, double tol, dou
     FUGU_MAXPY_DOT(8) FUGU_MAXPY_DOT(16) FUGU_MAXPY_DOT(32) FUGU_MAXPY_DOT(48)
 #undef FUGU_MAXPY_DOT
     LAUNCH(k_fold_maxpy_dot, dim3(1), dim3(1024), c->s, (int)nb, nq, stride, c->g.vol, c->reorth_frac,
-           (const double*)c->partials, c->sc, c->fl, fg_k, c->fg);
+           (const double*)c->partials, c->sc, c->fl, fg_k, c->fg,
+           fg_k >= 0 ? c->d_hfl + F_COUNT + (fg_k & 1) : (int*)nullptr);
   };
   // <W, V_i>, i = 0..k -> S_HBUF + off + i (and vol <W, W> -> S_DOT1); pass 2 predicated on F_REORTH
   auto dots_into = [&](const auto* W, auto* const* basis, int k, int off, bool second) {
@@ -1879,9 +1881,8 @@ void solve_fgmres(fugu_ctx* c, Work<T>& w, double* X, double dt, double tol, dou
       else step_k();
       total_inner += it;
       ++steps;
-      // convergence is read one step behind, so the device always has the next step queued
-      HIPCHK(hipMemcpyAsync(c->h_fl + F_COUNT + (k & 1), c->fl + F_FG_DONE, sizeof(int), hipMemcpyDeviceToHost,
-                            c->s));
+      // convergence is read one step behind, so the device always has the next step queued; the
+      // fold that ran step k wrote its flag into h_fl[F_COUNT + (k & 1)] (mapped pinned memory)
       HIPCHK(hipEventRecord(c->fg_ev[k & 1], c->s));
       if (k >= 1) {
         HIPCHK(hipEventSynchronize(c->fg_ev[(k - 1) & 1]));
@@ -2588,6 +2589,7 @@ int fugu_ctx_create(const fugu_grid* grid, const fugu_params* prm, fugu_ctx** ou
     // two landing slots past the flags for the FGMRES convergence flag (read_flags copies
     // only the first F_COUNT, so an inner solve cannot overwrite them)
     HIPCHK(hipHostMalloc((void**)&c->h_fl, (F_COUNT + 2) * sizeof(int), hipHostMallocDefault));
+    HIPCHK(hipHostGetDevicePointer((void**)&c->d_hfl, c->h_fl, 0));
     c->X = dalloc<double>(c, c->ntrip);
     c->u = c->X; c->v = c->X + g.nu; c->p = c->X + g.nu + g.nv;
     c->c = dalloc<double>(c, g.np); c->cnew = dalloc<double>(c, g.np); c->dclamp = dalloc<double>(c, g.np);

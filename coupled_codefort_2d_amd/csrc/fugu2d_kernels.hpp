@@ -710,9 +710,20 @@ constexpr int FG_SIZE = FG_KK + 8;
 // running entry carried in a register (unrolled, so the LDS reads of several steps are in flight
 // together; the dependent chain is arithmetic only) -- the same operations in the same order as
 // rotating the column in place.
-__device__ inline void fgmres_step_blk(int k, double* __restrict__ sc, double* __restrict__ fg, int* __restrict__ fl) {
+// hflag (optional): a host-mapped int that receives the cycle's convergence flag after this
+// step (the host reads it one step behind, after an event; written with a system-scope fence).
+__device__ inline void fgmres_publish(int* hflag, int done) {
+  if (!hflag) return;
+  *(volatile int*)hflag = done;
+  __threadfence_system();
+}
+__device__ inline void fgmres_step_blk(int k, double* __restrict__ sc, double* __restrict__ fg, int* __restrict__ fl,
+                                       int* hflag) {
   __shared__ double hcol[FG_M + 1], rc[FG_M], rs[FG_M];
-  if (fl[F_FG_DONE]) return;
+  if (fl[F_FG_DONE]) {
+    if (threadIdx.x == 0) fgmres_publish(hflag, 1);
+    return;
+  }
   for (int i = threadIdx.x; i <= k; i += blockDim.x) {
     hcol[i] = sc[S_HBUF + i] + sc[S_HBUF + 64 + i];
     if (i < k) {
@@ -744,7 +755,9 @@ __device__ inline void fgmres_step_blk(int k, double* __restrict__ sc, double* _
   fg[FG_G + k] = c * fg[FG_G + k];
   fg[FG_KK] = (double)(k + 1);
   sc[S_INVH] = hn > 0.0 ? 1.0 / hn : 0.0;
-  if (fabs(fg[FG_G + k + 1]) <= tol_abs || hn == 0.0) fl[F_FG_DONE] = 1;
+  const int done = (fabs(fg[FG_G + k + 1]) <= tol_abs || hn == 0.0) ? 1 : 0;
+  if (done) fl[F_FG_DONE] = 1;
+  fgmres_publish(hflag, done);
 }
 
 template <typename O, typename V>
@@ -827,7 +840,7 @@ __global__ void __launch_bounds__(256) k_maxpy_dot(MaxpyArgs<V, V> a, const doub
 __global__ void __launch_bounds__(1024) k_fold_maxpy_dot(int nb, int nq, int stride, double vol, double frac,
                                                          const double* __restrict__ partials,
                                                          double* __restrict__ sc, int* __restrict__ fl, int fg_k,
-                                                         double* __restrict__ fg) {
+                                                         double* __restrict__ fg, int* hflag) {
   __shared__ double red[MAXPY_MAX + 1];
   __shared__ int reorth;
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
@@ -854,7 +867,7 @@ __global__ void __launch_bounds__(1024) k_fold_maxpy_dot(int nb, int nq, int str
   if ((int)threadIdx.x < nq) sc[S_HBUF + 64 + threadIdx.x] = reorth ? red[threadIdx.x] : 0.0;
   if (fg_k >= 0 && !reorth) {
     __syncthreads();
-    fgmres_step_blk(fg_k, sc, fg, fl);
+    fgmres_step_blk(fg_k, sc, fg, fl, hflag);
   }
 }
 
@@ -974,7 +987,7 @@ __global__ void __launch_bounds__(1024) k_mdot_fold(int nv, int off, double vol,
 // then (fg_k >= 0) the FGMRES step k follows in the same launch.
 __global__ void __launch_bounds__(NT) k_fold_gs_norm(int nb, int mode, double vol, const double* __restrict__ partials,
                                                      double* __restrict__ sc, int* __restrict__ fl, int fg_k,
-                                                     double* __restrict__ fg) {
+                                                     double* __restrict__ fg, int* hflag) {
   if (mode == 2 && !fl[F_REORTH]) return;
   double v[1] = {0.0};
   for (int b = threadIdx.x; b < nb; b += blockDim.x) v[0] += partials[b];
@@ -986,7 +999,7 @@ __global__ void __launch_bounds__(NT) k_fold_gs_norm(int nb, int mode, double vo
   }
   if (mode == 2 && fg_k >= 0) {
     __syncthreads();
-    fgmres_step_blk(fg_k, sc, fg, fl);
+    fgmres_step_blk(fg_k, sc, fg, fl, hflag);
   }
 }
 
