@@ -167,6 +167,9 @@ struct fugu_ctx {
   // (mg_strip_shape), 0 never, 2 / 3 the large / small shapes on every fused level (bit-identical
   // either way)
   int mg_strip = 1;
+  // terms per chunk of the reference-order sums: 0 by the sum's length (sq_chunk), else 512, 1024
+  // or 4096 (the same sums bit for bit)
+  int seq_chunk = 0;
   // red-black sweeps before and after each coarse correction: pressure and velocity hierarchies
   // (three on the velocity levels: fewer FGMRES steps, -5 % solve time on viscous-dominated blocks)
   int mg_nu_p = 2, mg_nu_v = 3;
@@ -328,7 +331,7 @@ Sink sink(fugu_ctx* c, long nterms) {
     // at most 5 chains (the fused GCR(2) dots) x every owned point of a triplet
     const Grid& g = c->g;
     const size_t nmax = (size_t)3 * g.nx * g.ny;
-    const size_t nch = (nmax + SQ_CH - 1) / SQ_CH;
+    const size_t nch = (nmax + SQ_CH_MIN - 1) / SQ_CH_MIN;
     c->terms = dalloc<double>(c, 5 * nmax);
     c->seq_ctot = dalloc<double>(c, 5 * 3 * nch);
     c->seq_cnt = dalloc<int>(c, 5 * nch);
@@ -341,12 +344,13 @@ Sink sink(fugu_ctx* c, long nterms) {
 void seq_reduce(fugu_ctx* c, const Sink& sk, int nv, Skip skip) {
   if (!sk.n) return;
   SeqArgs a;
-  a.t = sk.p; a.n = sk.n; a.nch = (int)((sk.n + SQ_CH - 1) / SQ_CH);
+  a.t = sk.p; a.n = sk.n;
   a.ctot = c->seq_ctot; a.cnt = c->seq_cnt; a.tot = c->seq_tot;
-  const dim3 gc((unsigned)a.nch, (unsigned)nv);
-  LAUNCH(k_seq_chunks, gc, dim3(SQ_NT), c->s, a, skip, (const int*)c->fl);
-  LAUNCH(k_seq_compact, gc, dim3(SQ_NT), c->s, a, skip, (const int*)c->fl);
-  LAUNCH(k_seq_fold, dim3((unsigned)nv), dim3(SQ_NT), c->s, a, skip, (const int*)c->fl);
+  const int ch = (c->seq_chunk == SQ_CH_TINY || c->seq_chunk == SQ_CH_SHORT || c->seq_chunk == SQ_CH_LONG)
+                     ? c->seq_chunk
+                     : sq_chunk(sk.n);
+  seq_launch_chunks(a, ch, nv, skip, (const int*)c->fl, c->s);
+  HIPCHK(hipGetLastError());
 }
 long cdot_terms(const Grid& g) {
   const long iu = g.per_x ? g.nx : g.nx - 1, jv = g.per_y ? g.ny : g.ny - 1;
@@ -2491,7 +2495,7 @@ int guarded(F&& f) {
 // hipGetSymbolAddress on the first call on that device (that device current) and is kept per
 // device.  Calls that overlap on one GPU share the array and race.
 constexpr long NOALLOC_MAX_TERMS = 1L << 20;   // owned u + v + p points (reference order)
-constexpr long NOALLOC_NCH = NOALLOC_MAX_TERMS / SQ_CH;
+constexpr long NOALLOC_NCH = NOALLOC_MAX_TERMS / SQ_CH_MIN;   // chunks of the shortest length
 constexpr long NOALLOC_PARTIALS = 2L * RED_MAXB;       // one double-double per workgroup
 constexpr long NOALLOC_CTOT = 3 * NOALLOC_NCH;
 constexpr long NOALLOC_CNT = NOALLOC_NCH / 2;          // NOALLOC_NCH ints
@@ -2554,6 +2558,7 @@ int fugu_ctx_create(const fugu_grid* grid, const fugu_params* prm, fugu_ctx** ou
     if (const char* fu = std::getenv("FUGU_MG_FUSED")) c->mg_fused = std::atoi(fu) != 0;
     if (const char* ts = std::getenv("FUGU_MG_TAIL_STAGE")) c->mg_tail_stage = std::atoi(ts) != 0;
     if (const char* st = std::getenv("FUGU_MG_STRIP")) c->mg_strip = std::atoi(st);
+    if (const char* sq = std::getenv("FUGU_SEQ_CHUNK")) c->seq_chunk = std::atoi(sq);
     if (const char* s = std::getenv("FUGU_MG_NU_P")) c->mg_nu_p = std::min(4, std::max(1, std::atoi(s)));
     if (const char* s = std::getenv("FUGU_MV_TILE")) c->mv_tile = std::atoi(s);
     if (const char* s = std::getenv("FUGU_GS_VEC")) c->gs_vec = std::atoi(s) != 0;
@@ -2805,12 +2810,10 @@ int fugu_coupled_dot_noalloc(const fugu_grid* grid, const double* u1, const doub
       const Sink sk{terms, nterms, tot};
       LAUNCH(k_cdot_split, gr, dim3(TX, TY), st, g, d, sk);
       SeqArgs a;
-      a.t = terms; a.n = nterms; a.nch = (int)((nterms + SQ_CH - 1) / SQ_CH);
+      a.t = terms; a.n = nterms;
       a.ctot = ctot; a.cnt = cnt; a.tot = tot;
-      const dim3 gc((unsigned)a.nch, 1u);
-      LAUNCH(k_seq_chunks, gc, dim3(SQ_NT), st, a, Skip{-1, 0}, (const int*)nullptr);
-      LAUNCH(k_seq_compact, gc, dim3(SQ_NT), st, a, Skip{-1, 0}, (const int*)nullptr);
-      LAUNCH(k_seq_fold, dim3(1), dim3(SQ_NT), st, a, Skip{-1, 0}, (const int*)nullptr);
+      seq_launch_chunks(a, sq_chunk(nterms), 1, Skip{-1, 0}, (const int*)nullptr, st);
+      HIPCHK(hipGetLastError());
       LAUNCH(k_fold_dot_out, dim3(1), dim3(NT), st, 0, sk, dot_out);
     } else {
       const Sink sk{partials, 0, nullptr};

@@ -19,7 +19,7 @@
 // interval around x_k; when the whole interval lies in one binade of one sign the binade is certain.
 //
 // Pipeline per reduction (after a pass-1 kernel has written the terms, chain q's term k at
-// t[q n + k]), in chunks of SQ_CH terms:
+// t[q n + k]), in chunks of CH terms:
 //   k_seq_chunks   chunk totals: double-double sum of t and the sum of |t|;
 //   k_seq_compact  each chunk certifies its terms and splits them into runs (consecutive terms
 //                  of one certified binade, sign and no tie).  A run becomes two values, its first
@@ -40,11 +40,23 @@
 
 namespace fugu {
 
-constexpr int SQ_CH = 4096;             // terms per chunk
+// Terms per chunk: a template parameter CH of the pipeline's kernels.  Short sums use short
+// chunks (more workgroups, each thread's certification chain shorter), long sums long ones (fewer
+// run values for the serial fold).  Measured on the drop-in's sums (one launch triple each, the
+// same doubles whatever the chunk): the 256^2 droplet's reference-algorithm solve runs 0.95 /
+// 1.34 / 1.40 solves/s with 4096 / 1024 / 512-term chunks; the 2048^2 150-iteration PCG takes
+// 51 / 65 / 94 ms.
+constexpr int SQ_CH_TINY = 512;
+constexpr int SQ_CH_SHORT = 1024;
+constexpr int SQ_CH_LONG = 4096;
+constexpr int SQ_CH_MIN = SQ_CH_TINY;     // the smallest chunk (sizes the per-chunk buffers)
+inline int sq_chunk(long n) { return n <= (1L << 16) ? SQ_CH_TINY : (n <= (1L << 18) ? SQ_CH_SHORT : SQ_CH_LONG); }
 constexpr int SQ_NT = 256;              // threads per chunk workgroup
 constexpr int SQ_NW = SQ_NT / 64;       // its wavefronts
-constexpr int SQ_PT = SQ_CH / SQ_NT;    // consecutive terms per thread
-constexpr int SQ_LDP = SQ_PT + 1;       // padded LDS row of one thread's terms (odd: no bank conflicts)
+template <int CH> struct SqGeom {
+  static constexpr int PT = CH / SQ_NT;   // consecutive terms per thread
+  static constexpr int LDP = PT + 1;      // padded LDS row of one thread's terms (odd: no bank conflicts)
+};
 constexpr int SQ_WIN = 4096;            // chunks per window of the serial fold
 constexpr int SQ_BUF = 2048;            // values per staged piece of the serial fold
 
@@ -135,14 +147,15 @@ __device__ inline I2 sq_scan_i2(I2 v, I2* lds, I2* total) {
 }
 
 // Chunk totals (grid: nch x nv).
+template <int CH>
 __global__ void __launch_bounds__(SQ_NT) k_seq_chunks(SeqArgs a, Skip sk, const int* __restrict__ fl) {
   if (seq_skip(sk, fl)) return;
   const int c = blockIdx.x, q = blockIdx.y, tid = threadIdx.x;
   const double* t = a.t + (long)q * a.n;
   PA v = pa_zero();
 #pragma unroll
-  for (int m = 0; m < SQ_PT; ++m) {
-    const long idx = (long)c * SQ_CH + tid + m * SQ_NT;
+  for (int m = 0; m < SqGeom<CH>::PT; ++m) {
+    const long idx = (long)c * CH + tid + m * SQ_NT;
     if (idx < a.n) {
       const double x = t[idx];
       v.p = dd_add(v.p, dd_from(x));
@@ -164,6 +177,7 @@ __global__ void __launch_bounds__(SQ_NT) k_seq_chunks(SeqArgs a, Skip sk, const 
 // The serial fold of chain q (one workgroup): lane 0 adds every chunk's values in order while
 // wavefronts 1.. stage the next piece in LDS (all of them stage the first).  off: SQ_WIN + 1 ints;
 // buf: 2 x SQ_BUF doubles.
+template <int CH>
 __device__ inline void sq_fold(const SeqArgs& a, int q, int* off, double* buf, I2* lsi) {
   const int tid = threadIdx.x;
   const double* t = a.t + (long)q * a.n;
@@ -213,7 +227,7 @@ __device__ inline void sq_fold(const SeqArgs& a, int q, int* off, double* buf, I
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
           const int jj = j + q * nt;
-          v[q] = (jj < j1) ? t[(long)(w0 + cc[q]) * SQ_CH + (jj - off[cc[q]])] : 0.0;
+          v[q] = (jj < j1) ? t[(long)(w0 + cc[q]) * CH + (jj - off[cc[q]])] : 0.0;
         }
 #pragma unroll
         for (int q = 0; q < 4; ++q)
@@ -246,8 +260,10 @@ __device__ inline void sq_fold(const SeqArgs& a, int q, int* off, double* buf, I
 }
 
 // Certify, split into runs and compact one chunk in place (grid: nch x nv).
+template <int CH>
 __global__ void __launch_bounds__(SQ_NT) k_seq_compact(SeqArgs a, Skip sk, const int* __restrict__ fl) {
   if (seq_skip(sk, fl)) return;
+  constexpr int SQ_CH = CH, SQ_PT = SqGeom<CH>::PT, SQ_LDP = SqGeom<CH>::LDP;
   const int c = blockIdx.x, q = blockIdx.y, tid = threadIdx.x;
   const int lane = tid & 63, wid = tid >> 6;
   double* t = a.t + (long)q * a.n;
@@ -400,12 +416,29 @@ __global__ void __launch_bounds__(SQ_NT) k_seq_compact(SeqArgs a, Skip sk, const
 }
 
 // The serial fold (grid: nv, one chain per workgroup).
+template <int CH>
 __global__ void __launch_bounds__(SQ_NT) k_seq_fold(SeqArgs a, Skip sk, const int* __restrict__ fl) {
   if (seq_skip(sk, fl)) return;
   __shared__ int loff[SQ_WIN + 1];
   __shared__ double lbuf[2 * SQ_BUF];
   __shared__ I2 lfold[SQ_NW];
-  sq_fold(a, blockIdx.x, loff, lbuf, lfold);
+  sq_fold<CH>(a, blockIdx.x, loff, lbuf, lfold);
+}
+
+// The pipeline's three launches for chunks of CH terms (a.nch set for CH).
+template <int CH>
+inline void seq_launch(const SeqArgs& a, int nv, Skip skip, const int* fl, hipStream_t s) {
+  const dim3 gc((unsigned)a.nch, (unsigned)nv);
+  hipLaunchKernelGGL(k_seq_chunks<CH>, gc, dim3(SQ_NT), 0, s, a, skip, fl);
+  hipLaunchKernelGGL(k_seq_compact<CH>, gc, dim3(SQ_NT), 0, s, a, skip, fl);
+  hipLaunchKernelGGL(k_seq_fold<CH>, dim3((unsigned)nv), dim3(SQ_NT), 0, s, a, skip, fl);
+}
+// The pipeline for chunks of ch terms (512, 1024 or 4096); a.nch is set here.
+inline void seq_launch_chunks(SeqArgs a, int ch, int nv, Skip skip, const int* fl, hipStream_t s) {
+  a.nch = (int)((a.n + ch - 1) / ch);
+  if (ch == SQ_CH_TINY) seq_launch<SQ_CH_TINY>(a, nv, skip, fl, s);
+  else if (ch == SQ_CH_SHORT) seq_launch<SQ_CH_SHORT>(a, nv, skip, fl, s);
+  else seq_launch<SQ_CH_LONG>(a, nv, skip, fl, s);
 }
 
 }  // namespace fugu

@@ -6,7 +6,9 @@ inner).  The GPU's parallel reproduction of that sequential sum (chunked binade 
 exact grid sums of each run, a serial pass over the runs) must return the same double as a plain
 Python loop over the same terms, on sequences built to defeat it: random walks around zero, ties
 on the rounding grid, heavy cancellation, thirty decades of magnitude, exact powers of two,
-zeros, infinities and NaNs.  Sizes span one to ~70 chunks of 4096 terms."""
+zeros, infinities and NaNs.  Sizes span one to ~70 chunks of 4096 terms (up to ~530 chunks of
+512); every chunk length the library uses (512, 1024, 4096, and its choice by the sum's length)
+is run."""
 import numpy as np
 import pytest
 
@@ -69,7 +71,9 @@ KINDS = ["positive", "walk", "decades", "ties", "cancel", "zeros", "pow2", "prod
 @pytest.mark.parametrize("shape", [(30, 20, False, False), (120, 100, True, False), (160, 150, False, True)],
                          ids=["1chunk", "9chunks_px", "18chunks_py"])
 @pytest.mark.parametrize("kind", KINDS)
-def test_reference_order_sum_bitwise(shape, kind):
+@pytest.mark.parametrize("chunk", ["0", "512", "1024", "4096"])
+def test_reference_order_sum_bitwise(monkeypatch, shape, kind, chunk):
+    monkeypatch.setenv("FUGU_SEQ_CHUNK", chunk)
     nx, ny, px, py = shape
     g = O.Grid(nx, ny, float(nx), float(ny), per_x=px, per_y=py)   # dx = dy = 1
     ctx = abi.Context(nx, ny, float(nx), float(ny), px, py, sum_order=abi.SUM_REFERENCE)
@@ -82,8 +86,10 @@ def test_reference_order_sum_bitwise(shape, kind):
     ctx.close()
 
 
-def test_reference_order_sum_large_and_nonfinite():
-    """~70 chunks of terms (a 270k-term sum), then the same with an infinity and a NaN planted."""
+@pytest.mark.parametrize("chunk", ["0", "512", "1024", "4096"])
+def test_reference_order_sum_large_and_nonfinite(monkeypatch, chunk):
+    """~70 chunks of 4096 terms (a 270k-term sum), then the same with an infinity and a NaN planted."""
+    monkeypatch.setenv("FUGU_SEQ_CHUNK", chunk)
     nx = ny = 300
     g = O.Grid(nx, ny, float(nx), float(ny))
     ctx = abi.Context(nx, ny, float(nx), float(ny), sum_order=abi.SUM_REFERENCE)
