@@ -567,10 +567,10 @@ def main():
                                                       "dJ_dsigma", "interface")}
     adj2048 = None
     if "adjoint2048" in sections:
-        # pressure V-cycles per PCD: measured 61 s/grad with 2, 55 with 3, 51 with 4, 48.6 with 6, 48.5
-        # with 8 (the 1e-10 adjoint solves lean on the pressure block)
+        # pressure V-cycles per PCD (the 1e-10 adjoint solves lean on the pressure block): with the
+        # register-strip level kernels 40.1 s/grad with 4, 39.3 with 6, 36.7 with 8, 36.9 with 12
         adj2048 = run_adjoint(abi, n=2048, nsteps=32, count=256, seed=5, rmin=0.005, rmax=0.02, rho_c=1000.0,
-                              sigma=1.0, mixed=True, mg_cycles=6)
+                              sigma=1.0, mixed=True, mg_cycles=8)
     dropin = run_dropin(abi, no_cpu=args.no_cpu) if "dropin" in sections else None
     pcg = run_pcg2048(abi) if "pcg2048" in sections else None
     physics = run_physics(abi) if "physics" in sections else None
