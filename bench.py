@@ -140,7 +140,9 @@ def run_form(abi, flags, steps, warmup, max_outer, inner_tol, mg_cycles, inner_m
 KERNELS = ("pcg_iter1_f64", "pcg_iter2_f64", "pcg_iter1_f32", "pcg_iter2_f32", "matvec_f64", "matvec_f32",
            "mg_smooth_f64", "mg_smooth_f32", "mg_resid_f64", "mg_resid_f32", "mdot_f64", "mdot_f32", "maxpy_f64",
            "maxpy_f32", "maxpy_update", "maxpy_dot_f64", "maxpy_dot_f32", "mg_pre_f64", "mg_pre_f32", "mg_post_f64",
-           "mg_post_f32", "mg_pre3_f64", "mg_pre3_f32", "mg_post3_f64", "mg_post3_f32", "mg_cut_f64", "mg_cut_f32")
+           "mg_post_f32", "mg_pre3_f64", "mg_pre3_f32", "mg_post3_f64", "mg_post3_f32", "mg_cut_f64", "mg_cut_f32",
+           "mgs_pre_f64", "mgs_pre_f32", "mgs_post_f64", "mgs_post_f32", "mgs_pre3_f64", "mgs_pre3_f32",
+           "mgs_post3_f64", "mgs_post3_f32", "mgs_cut_f64", "mgs_cut_f32")
 DEVICE_NAMES = {"maxpy_f64": "k_maxpy<double, double>", "maxpy_f32": "k_maxpy<float, float>",
                 # the matvec from LDS tiles of 8 rows (the library's default)
                 "matvec_f64": "k_matvec_tile<double, 8>", "matvec_f32": "k_matvec_tile<float, 8>",
@@ -151,7 +153,13 @@ DEVICE_NAMES = {"maxpy_f64": "k_maxpy<double, double>", "maxpy_f32": "k_maxpy<fl
                 "mg_pre3_f64": "k_mg_pre<double, 3>", "mg_pre3_f32": "k_mg_pre<float, 3>",
                 "mg_post3_f64": "k_mg_post<double, 3>", "mg_post3_f32": "k_mg_post<float, 3>",
                 # the cut velocity hierarchy's coarsest level: k_mg_pre with NU sweeps, no restriction
-                "mg_cut_f64": "k_mg_pre<double, NU> (cut coarsest)", "mg_cut_f32": "k_mg_pre<float, NU> (cut coarsest)"}
+                "mg_cut_f64": "k_mg_pre<double, NU> (cut coarsest)", "mg_cut_f32": "k_mg_pre<float, NU> (cut coarsest)",
+                # the register-strip level kernels (the large and the small levels)
+                "mgs_pre_f64": "k_mgs_pre<double, 2>", "mgs_pre_f32": "k_mgs_pre<float, 2>",
+                "mgs_post_f64": "k_mgs_post<double, 2>", "mgs_post_f32": "k_mgs_post<float, 2>",
+                "mgs_pre3_f64": "k_mgs_pre<double, 3>", "mgs_pre3_f32": "k_mgs_pre<float, 3>",
+                "mgs_post3_f64": "k_mgs_post<double, 3>", "mgs_post3_f32": "k_mgs_post<float, 3>",
+                "mgs_cut_f64": "k_mgs_pre<double, NU> (cut coarsest)", "mgs_cut_f32": "k_mgs_pre<float, NU> (cut coarsest)"}
 
 
 def device_name(name):
