@@ -12,8 +12,8 @@ rate and the ratio are reported beside it.  Every timed step must reach the tole
 that does not is listed in `unconverged_steps` and the run exits with status 3.
 
 Both forms solve every step's system to ||B - A X|| / ||B|| <= 1e-8 (coupled_dot norm, FP64)
-with restarted flexible GMRES(30) whose every restart recomputes the true residual and updates
-X in FP64.  The preconditioner is the block PCD with geometric-multigrid pressure and velocity
+with restarted flexible GMRES(m) whose every restart recomputes the true residual and updates
+X in FP64 (m by the library's default rule).  The preconditioner is the block PCD with geometric-multigrid pressure and velocity
 blocks and the viscous Schur correction.  FP64 form: everything FP64.  Mixed form: the Arnoldi
 cycles (basis, directions, matvec operands, preconditioner) in FP32 storage, FP64 products,
 sums and Hessenberg.
@@ -525,7 +525,8 @@ def main():
     ap.add_argument("--mg-cycles", type=int, default=1)
     ap.add_argument("--inner-max", type=int, default=-1, help="inner BiCGStab(2) cap; -1: PCD only")
     ap.add_argument("--restart", type=int, default=0,
-                    help="FGMRES restart length; 0: the library's default (20 up to 2^21 cells, 30 above)")
+                    help="FGMRES restart length; 0: the library's default (10 after a solve of 16-30 steps, "
+                         "else 20 up to 2^21 cells, 30 above)")
     ap.add_argument("--sections", default="headline,m1,adjoint,adjoint2048,dropin,pcg2048,physics",
                     help="headline (rising bubble, value), m1 (2048 x 2048 FP64 vs mixed), adjoint (s/grad, "
                          "1024^2, 16 steps), adjoint2048 (2048^2, 256 inclusions, 32 steps, mixed forward), "
@@ -620,8 +621,9 @@ def main():
                        "mixed": "FGMRES(m) Arnoldi cycles in FP32 storage (FP64 dots and Hessenberg) + FP32 PCD; "
                                 "true residual and update in FP64 at every restart; true residual 1e-8",
                        "restart": (f"m = {args.restart}" if args.restart > 0 else
-                                   "m = 20 at 512x1024, 30 at 2048x2048 (the library's default: 20 up to 2^21 "
-                                   "cells, 30 above)")},
+                                   "the library's default: m = 10 after a solve of 16-30 FGMRES steps on the "
+                                   "context, else 20 up to 2^21 cells and 30 above (the bubble's solves take "
+                                   "about 20 steps: m = 10 from the second step on; 2048x2048: m = 30)")},
         "roofline": roofline(forms[head]["ktimes"]) if head in forms else None,   # the headline form's kernels
         "kernels": {k: {"avg_us": round(1e3 * v[0] / v[1], 2), "launches": v[1], "total_ms": round(v[0], 2),
                         "GBps": round(v[2] / (v[0] * 1e-3) / 1e9, 1)} for k, v in all_k.items()},

@@ -170,6 +170,7 @@ struct fugu_ctx {
   // terms per chunk of the reference-order sums: 0 by the sum's length (sq_chunk), else 512, 1024
   // or 4096 (the same sums bit for bit)
   int seq_chunk = 0;
+  int fg_last_steps = 0;   // FGMRES steps of the last refined step solve (the default restart length)
   // red-black sweeps before and after each coarse correction: pressure and velocity hierarchies
   // (three on the velocity levels: fewer FGMRES steps, -5 % solve time on viscous-dominated blocks)
   int mg_nu_p = 2, mg_nu_v = 3;
@@ -2095,10 +2096,18 @@ int step_impl(fugu_ctx* c, double dt, int step_idx, const fugu_step_opts* o, fug
   c->mg_cycles = o->mg_cycles > 0 ? o->mg_cycles : 1;
   const double itol = o->inner_tol > 0 ? o->inner_tol : 1.0e-3;
   const int imax = o->inner_max_iter > 0 ? o->inner_max_iter : (o->inner_max_iter < 0 ? 0 : 25);
-  // default restart: 20 up to 2^21 cells (about 20-30 steps per solve there, and each step's
-  // Gram-Schmidt sweep grows with the basis), 30 above (measured: -5 % solve time on the 512x1024
-  // and 1024x2048 bubbles with 20, +6 % at 2048^2)
-  const int rm = o->restart > 0 ? o->restart : ((long)c->g.nx * c->g.ny <= (1L << 21) ? 20 : 30);
+  // default restart (step k's Gram-Schmidt sweeps read k + 2 basis vectors twice; a restart costs
+  // one true residual and a host round trip): 10 after a refined solve of 16-30 FGMRES steps on
+  // this context, otherwise 20 up to 2^21 cells and 30 above.  Measured (whole steps): the
+  // 512x1024 bubble (20 steps per solve) 153 -> 163-165 FP64 solves/s with 10 (same step count;
+  // 8: 147, 12: 156, 15: 154); the 128^2 cavity (12-15 steps) 534 time steps/s with 20, 478 with
+  // 10; the 256^2 droplet (7-12) alike with either; 2048^2 (84-111 steps at 1e-8) 5.30 / 8.36
+  // solves/s (FP64 / mixed) with 30, 5.50 / 8.49 with 20, 4.24 / 6.39 with 10 -- but 20 stalls a
+  // 1e-13 solve there (2e-11 after 600 steps), so 30 stays.
+  const int rm = o->restart > 0 ? o->restart
+                                : ((c->fg_last_steps >= 16 && c->fg_last_steps <= 30)
+                                       ? 10
+                                       : ((long)c->g.nx * c->g.ny <= (1L << 21) ? 20 : 30));
   if (o->solver_flags & FUGU_SOLVE_REFINED) {
     if (o->solver_flags & FUGU_SOLVE_MIXED) {
       work_alloc<float>(c, c->wf, true);
@@ -2109,6 +2118,7 @@ int step_impl(fugu_ctx* c, double dt, int step_idx, const fugu_step_opts* o, fug
       solve_fgmres<double>(c, c->wd, c->X, dt_eff, o->tol, itol, o->max_iter, imax, rm, mvflags, true, lid,
                            &iters, &outer, &res);
     }
+    c->fg_last_steps = outer;
   } else {
     HIPCHK(hipMemcpyAsync(w.X, c->X, c->ntrip * sizeof(double), hipMemcpyDeviceToDevice, c->s));
     if (o->use_bicgstab2)

@@ -57,8 +57,9 @@ typedef struct {
   int inner_max_iter;       /* refined solve: inner Krylov iteration cap (0: 25; < 0: no inner Krylov,
                                the PCD alone preconditions the outer FGMRES) */
   int mg_cycles;            /* FUGU_PRECOND_MG: V-cycles per preconditioner application (<= 0: 1) */
-  int restart;              /* refined solve: outer FGMRES restart length (<= 0: 20 up to 2^21
-                               cells, 30 above; max 47) */
+  int restart;              /* refined solve: outer FGMRES restart length (<= 0: 10 after a refined
+                               solve of 16-30 FGMRES steps on the context, else 20 up to 2^21
+                               cells and 30 above; max 47) */
 } fugu_step_opts;
 
 enum {
