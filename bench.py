@@ -439,7 +439,7 @@ def timed_rate(ktimes):
 
 def run_adjoint(abi, n=1024, nsteps=16, dt=5e-5, count=64, seed=4, rmin=0.01, rmax=0.04, rho_c=100.0,
                 mu_ratio=1e3, sigma=0.5, checkpoint_every=4, mixed=False, also_resident=False, mg_cycles=1,
-                moving=False):
+                moving=False, restart=30):
     """Seconds per gradient on an adjoint workload: the forward horizon with checkpoints (restart
     states every `checkpoint_every` steps), the reverse sweep with each segment's forward states
     recomputed, and the property-path gradients; after a 2-step warm-up call on the same
@@ -454,9 +454,9 @@ def run_adjoint(abi, n=1024, nsteps=16, dt=5e-5, count=64, seed=4, rmin=0.01, rm
     v0 = np.zeros((n + 5, n + 4))
     gm, gr = np.zeros_like(mu), np.zeros_like(rho)
     flags = abi.ADJ_PRECOND_VMG | (abi.ADJ_MIXED_FORWARD if mixed else 0) | (abi.ADJ_MOVING_INTERFACE if moving else 0)
-    ctx.adjoint_gradient(abi.FuguAdjointOpts(2, dt, 0.9, 1e-10, 3000, 30, mg_cycles, flags), mu, rho, sigma, c, u0, v0,
+    ctx.adjoint_gradient(abi.FuguAdjointOpts(2, dt, 0.9, 1e-10, 3000, restart, mg_cycles, flags), mu, rho, sigma, c, u0, v0,
                          gm, gr)
-    opts = abi.FuguAdjointOpts(nsteps, dt, 0.9, 1e-10, 3000, 30, mg_cycles, flags, checkpoint_every)
+    opts = abi.FuguAdjointOpts(nsteps, dt, 0.9, 1e-10, 3000, restart, mg_cycles, flags, checkpoint_every)
     t0 = time.perf_counter()
     J, gs, st, rc = ctx.adjoint_gradient(opts, mu, rho, sigma, c, u0, v0, gm, gr)
     wall = time.perf_counter() - t0
@@ -464,7 +464,7 @@ def run_adjoint(abi, n=1024, nsteps=16, dt=5e-5, count=64, seed=4, rmin=0.01, rm
     if also_resident:
         gm2, gr2 = np.zeros_like(mu), np.zeros_like(rho)
         t0 = time.perf_counter()
-        J2, gs2, st2, rc2 = ctx.adjoint_gradient(abi.FuguAdjointOpts(nsteps, dt, 0.9, 1e-10, 3000, 30, mg_cycles, flags, 0),
+        J2, gs2, st2, rc2 = ctx.adjoint_gradient(abi.FuguAdjointOpts(nsteps, dt, 0.9, 1e-10, 3000, restart, mg_cycles, flags, 0),
                                                  mu, rho, sigma, c, u0, v0, gm2, gr2)
         w2 = time.perf_counter() - t0
         # J bit for bit; the coefficient gradients accumulate with atomics (order not fixed): to rounding
@@ -490,7 +490,7 @@ def run_adjoint(abi, n=1024, nsteps=16, dt=5e-5, count=64, seed=4, rmin=0.01, rm
             "max_adjoint_res": float(f"{st.max_adjoint_res:.3e}"), "rc": rc, "J": J, "dJ_dsigma": gs,
             "interface": "moving (VOF, SVOF, curvature and CSF transposed)" if moving else "frozen",
             "solves": ("mixed (FP32 cycles, FP64 restarts)" if mixed else "FP64") + " forward and FP64 adjoint "
-                      f"FGMRES(30) to 1e-10, PCD with multigrid pressure ({mg_cycles} V-cycle(s)) and velocity blocks",
+                      f"FGMRES({restart}) to 1e-10, PCD with multigrid pressure ({mg_cycles} V-cycle(s)) and velocity blocks",
             **({"resident_horizon": res, "checkpointed_seconds_per_gradient": round(wall, 3),
                 "seconds_per_gradient": res["seconds_per_gradient"] if res["agrees_with_checkpointed"]
                 else round(wall, 3)} if res else {})}
