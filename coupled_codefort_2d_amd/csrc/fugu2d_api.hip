@@ -163,6 +163,7 @@ struct fugu_ctx {
   int mg_tail_cells = -1;    // coarse-tail start: < 0 the LDS-pool fit, > 0 a cell count, 0 never
   bool mg_fused = true;       // fused (temporal-blocking) level kernels instead of one launch per half-sweep
   bool mg_tail_stage = true; // stage the coarse tail in LDS when it fits
+  bool mg_tail_quad = true;  // the coarse tail with its coefficients in registers where it applies
   // register-strip level kernels (fugu2d_mgs.hpp): 1 on the large and the small levels
   // (mg_strip_shape), 0 never, 2 / 3 the large / small shapes on every fused level (bit-identical
   // either way)
@@ -965,7 +966,24 @@ T* mg_vcycle(fugu_ctx* c, std::vector<MgLevel<T>>& lv, size_t l, bool zero_guess
     }
     ta.blk = lv[l].cx;
     ta.nblk = off;
-    if (c->mg_tail_stage && packed && (size_t)off * sizeof(T) <= (size_t)MG_TAIL_POOL)
+    // the register-resident form: even smoothing levels whose quads fit the workgroup, at most
+    // MG_TAILQ_LEVELS of them, b and phi of every level in its LDS pool
+    const int nsm = ta.nlev - 1;
+    bool quad = c->mg_tail_quad && nsm >= 1 && nsm <= MG_TAILQ_LEVELS;
+    size_t qpool = 0;
+    for (size_t q = l; q < nl; ++q) {
+      const MgDims& d = lv[q].d;
+      qpool += 2 * (size_t)d.nx * d.ny;
+      if (q + 1 < nl)
+        quad = quad && (d.nx % 2 == 0) && (d.ny % 2 == 0) && (d.nx / 2) * (d.ny / 2) <= MG_TAILQ_NTH;
+    }
+    quad = quad && qpool <= (size_t)MG_TAILQ_POOL &&
+           mg_tailq_coarse_elems(lv[nl - 1].d, lv[nl - 1].cinv != nullptr) <= (size_t)MG_TAILQ_COARSE;
+    if (quad) {
+      if (nsm == 1) LAUNCH((k_mg_tail_q<T, 1>), dim3(1), dim3(MG_TAILQ_NTH), c->s, ta);
+      else if (nsm == 2) LAUNCH((k_mg_tail_q<T, 2>), dim3(1), dim3(MG_TAILQ_NTH), c->s, ta);
+      else LAUNCH((k_mg_tail_q<T, 3>), dim3(1), dim3(MG_TAILQ_NTH), c->s, ta);
+    } else if (c->mg_tail_stage && packed && (size_t)off * sizeof(T) <= (size_t)MG_TAIL_POOL)
       LAUNCH((k_mg_tail<T, true, MG_TAIL_NTH>), dim3(1), dim3(MG_TAIL_NTH), c->s, ta);
     else
       LAUNCH((k_mg_tail<T, false, MG_TAIL_NTH>), dim3(1), dim3(MG_TAIL_NTH), c->s, ta);
@@ -1728,7 +1746,8 @@ void solve_fgmres(fugu_ctx* c, Work<T>& w, double* X, double dt, double tol, dou
     key.nl1 = c->use_vmg ? (int)w.vmg_nl[1] : 0;
     key.cyc0 = c->use_vmg ? w.vmg_cyc[0] : 0;
     key.cyc1 = c->use_vmg ? w.vmg_cyc[1] : 0;
-    key.flags = (c->use_vmg ? 1 : 0) | (c->pcd_cross ? 2 : 0) | (c->mg_fused ? 4 : 0) | (c->mg_tail_stage ? 8 : 0);
+    key.flags = (c->use_vmg ? 1 : 0) | (c->pcd_cross ? 2 : 0) | (c->mg_fused ? 4 : 0) | (c->mg_tail_stage ? 8 : 0) |
+                (c->mg_tail_quad ? 16 : 0);
     key.mg_cycles = c->mg_cycles;
     key.dt = dt;
     key.pcd_visc = c->pcd_visc;
@@ -2567,6 +2586,7 @@ int fugu_ctx_create(const fugu_grid* grid, const fugu_params* prm, fugu_ctx** ou
     if (const char* tc = std::getenv("FUGU_MG_TAIL_CELLS")) c->mg_tail_cells = std::atoi(tc);
     if (const char* fu = std::getenv("FUGU_MG_FUSED")) c->mg_fused = std::atoi(fu) != 0;
     if (const char* ts = std::getenv("FUGU_MG_TAIL_STAGE")) c->mg_tail_stage = std::atoi(ts) != 0;
+    if (const char* tq = std::getenv("FUGU_MG_TAIL_QUAD")) c->mg_tail_quad = std::atoi(tq) != 0;
     if (const char* st = std::getenv("FUGU_MG_STRIP")) c->mg_strip = std::atoi(st);
     if (const char* sq = std::getenv("FUGU_SEQ_CHUNK")) c->seq_chunk = std::atoi(sq);
     if (const char* s = std::getenv("FUGU_MG_NU_P")) c->mg_nu_p = std::min(4, std::max(1, std::atoi(s)));

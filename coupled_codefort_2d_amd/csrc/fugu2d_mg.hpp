@@ -750,6 +750,339 @@ __global__ void __launch_bounds__(NTH) k_mg_tail(MgTailArgs<T> a) {
   }
 }
 
+// ---------------------------------------------------------------------------------
+// Coarse tail with the coefficients in registers (levels of even extents, at most
+// 4 MG_TAILQ_NTH cells, at most MG_TAILQ_LEVELS of them above the coarsest).  The general tail
+// above re-derives every cell's neighbours, wraps and masks and re-reads its coefficients from
+// LDS in each half-sweep: about 70 instructions and 9 LDS reads per update, which on one
+// workgroup is the whole cost.  Here thread t owns quad t of every level -- the 2 x 2 cells
+// (2I + (k & 1), 2J + (k >> 1)), k = 0..3, that restrict onto coarse cell t = I + J nx/2 --
+// and holds their masked conductances, D^-1 and raw diagonals in registers with the LDS
+// indices of the neighbours outside the quad (those inside are at fixed offsets).  An update
+// is then 5 LDS reads, the stencil and a write.  Owning all four children of its coarse cell,
+// a thread forms their residuals and sums them onto the coarse cell in one phase (no residual
+// array, no second barrier).  Only b and phi of the levels live in LDS.
+// The arithmetic is the general tail's, operand for operand: the neighbour sum
+// cW pW + cE pE + cS pS + cN pN with clamped (wall) or wrapped (periodic) neighbour indices and
+// masked wall conductances, (b + s) D^-1, the residual b - (diag phi - s) summed over the
+// children in the order k = 0..3, the prolonged neighbour values phi + phi_c(parent) in the
+// first post-smoothing half-sweep, and the same coarsest solve: the results are bit-identical.
+constexpr int MG_TAILQ_NTH = 512;     // threads: one quad of the top level each
+constexpr int MG_TAILQ_LEVELS = 3;    // smoothing levels (the coarsest not counted)
+constexpr int MG_TAILQ_POOL = 8192;   // LDS elements for b and phi of every tail level
+// LDS elements for the coarsest level's operator: its dense inverse (transposed), or c_x, c_y
+// and D^-1 for the red-black sweeps
+constexpr int MG_TAILQ_COARSE = MG_DENSE_MAX * MG_DENSE_MAX + 128;
+inline size_t mg_tailq_coarse_elems(const MgDims& d, bool dense) {
+  const size_t n = (size_t)d.nx * d.ny;
+  return dense ? n * n : (size_t)(d.nx + 1) * d.ny + (size_t)d.nx * (d.ny + 1) + n;
+}
+
+template <typename T>
+struct TqQuad {
+  T cw[4], ce[4], cs[4], cn[4], dinv[4], diag[4];
+  // the two neighbours of cell k outside the quad, LDS element indices: horizontal | vertical
+  // << 16 (k = 0: W, S; 1: E, S; 2: W, N; 3: E, N)
+  unsigned out[4];
+  // cells 1 and 2 (the colour the post-smoothing updates first): the coarse parents of those two
+  // neighbours, packed the same way (the parent of a neighbour inside the quad is coarse cell t)
+  unsigned pout[2];
+};
+
+__device__ inline unsigned tq_lo(unsigned p) { return p & 0xffffu; }
+__device__ inline unsigned tq_hi(unsigned p) { return p >> 16; }
+
+// t = I + J w (a quad of a level w quads wide; an element of an n x n matrix): J = floor((t +
+// 1/2) / w) in single precision instead of an integer division (tens of instructions each, in a
+// set-up whose instruction count is its cost).  (t + 1/2) / w lies at least 1/(2w) from an
+// integer; the hardware reciprocal and the product err by less than 2^-22 relative, i.e. by
+// less than (t + 1/2) 2^-22 / w, which is below 1/(2w) for every t < 2^21.
+__device__ inline void tq_quad(int t, int w, int& I, int& J) {
+  J = (int)(((float)t + 0.5f) * __builtin_amdgcn_rcpf((float)w));
+  I = t - J * w;
+}
+
+// Loads of quad (I, J)'s coefficients (level arrays in global memory, 32-bit offsets against
+// the level's base); no loaded value is used here, so every load of every level is in flight
+// together.
+template <typename T>
+__device__ inline void tq_load(const MgTailLevel<T>& L, int I, int J, unsigned phi0, unsigned phic0, TqQuad<T>& q,
+                               T (&mm)[4]) {
+  const MgDims d = L.d;
+  const int hx = d.nx >> 1;
+  const unsigned rx = (unsigned)d.nx + 1u, nx = (unsigned)d.nx;
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    const int i = 2 * I + (k & 1), j = 2 * J + (k >> 1);
+    const unsigned o = (unsigned)i + (unsigned)j * nx, ox = (unsigned)i + (unsigned)j * rx;
+    q.cw[k] = mgt_ld(L.cx, ox);
+    q.ce[k] = mgt_ld(L.cx, ox + 1u);
+    q.cs[k] = mgt_ld(L.cy, o);
+    q.cn[k] = mgt_ld(L.cy, o + nx);
+    q.dinv[k] = mgt_ld(L.dinv, o);
+    mm[k] = L.m ? mgt_ld(L.m, o) : T(0);
+    // the outer neighbours (mg_nbr_sum's wrapped or clamped indices) and their parents
+    const int ix = (k & 1) ? mg_wrap(i + 1, d.nx, d.per_x) : mg_wrap(i - 1, d.nx, d.per_x);
+    const int jy = (k >> 1) ? mg_wrap(j + 1, d.ny, d.per_y) : mg_wrap(j - 1, d.ny, d.per_y);
+    q.out[k] = (phi0 + (unsigned)(ix + j * d.nx)) | ((phi0 + (unsigned)(i + jy * d.nx)) << 16);
+    if (k == 1 || k == 2)
+      q.pout[k - 1] = (phic0 + (unsigned)((ix >> 1) + (j >> 1) * hx)) |
+                      ((phic0 + (unsigned)((i >> 1) + (jy >> 1) * hx)) << 16);
+  }
+}
+
+// After the loads: raw diagonal (the residual's, k_mg_dinv's sum) and masked conductances.
+template <typename T>
+__device__ inline void tq_finish(const MgDims& d, int I, int J, TqQuad<T>& q, const T (&mm)[4], bool mass) {
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    const int i = 2 * I + (k & 1), j = 2 * J + (k >> 1);
+    T dg = q.cw[k] + q.ce[k] + q.cs[k] + q.cn[k];
+    if (mass) dg += mm[k];
+    q.diag[k] = dg;
+    if (!(i > 0 || d.per_x)) q.cw[k] = T(0);
+    if (!(i < d.nx - 1 || d.per_x)) q.ce[k] = T(0);
+    if (!(j > 0 || d.per_y)) q.cs[k] = T(0);
+    if (!(j < d.ny - 1 || d.per_y)) q.cn[k] = T(0);
+  }
+}
+
+// Neighbour values of cell k (own: its LDS index; nx: the level's row length).  PRO: plus the
+// coarse correction of each neighbour's parent (pc: coarse cell t's index, for neighbours in
+// the quad).
+template <typename T, int K, bool PRO>
+__device__ inline T tq_nbr(const TqQuad<T>& q, unsigned own, int nx, unsigned pc, const T* s) {
+  const unsigned ox = tq_lo(q.out[K]), oy = tq_hi(q.out[K]);
+  const unsigned w = (K & 1) ? own - 1u : ox, e = (K & 1) ? ox : own + 1u;
+  const unsigned so = (K >> 1) ? own - (unsigned)nx : oy, no = (K >> 1) ? oy : own + (unsigned)nx;
+  T pW = s[w], pE = s[e], pS = s[so], pN = s[no];
+  if constexpr (PRO) {
+    const unsigned px = tq_lo(q.pout[K - 1]), py = tq_hi(q.pout[K - 1]);
+    pW = pW + s[(K & 1) ? pc : px];
+    pE = pE + s[(K & 1) ? px : pc];
+    pS = pS + s[(K >> 1) ? pc : py];
+    pN = pN + s[(K >> 1) ? py : pc];
+  }
+  return q.cw[K] * pW + q.ce[K] * pE + q.cs[K] * pS + q.cn[K] * pN;
+}
+
+// Half-sweep of colour COL (cells 0 and 3: colour 0; 1 and 2: colour 1) on quad q (own0: the
+// LDS index of its cell 0; db: b's index minus phi's in the level).
+template <typename T, int COL, bool PRO>
+__device__ inline void tq_half(const TqQuad<T>& q, bool act, unsigned own0, int nx, int db, unsigned pc,
+                               bool zero_other, T* s) {
+  if (act) {
+    constexpr int KA = COL ? 1 : 0, KB = COL ? 2 : 3;
+    const unsigned oa = own0 + (KA & 1) + (KA >> 1) * (unsigned)nx;
+    const unsigned ob = own0 + (KB & 1) + (KB >> 1) * (unsigned)nx;
+    T sa = T(0), sb = T(0);
+    if (!zero_other) {
+      sa = tq_nbr<T, KA, PRO>(q, oa, nx, pc, s);
+      sb = tq_nbr<T, KB, PRO>(q, ob, nx, pc, s);
+    }
+    const T va = (s[oa + db] + sa) * q.dinv[KA];
+    const T vb = (s[ob + db] + sb) * q.dinv[KB];
+    s[oa] = va;
+    s[ob] = vb;
+  }
+  __syncthreads();
+}
+
+// Residuals of the quad's four cells summed onto its coarse cell (bc, phic: LDS indices).
+template <typename T>
+__device__ inline void tq_restrict(const TqQuad<T>& q, bool act, unsigned own0, int nx, int db, unsigned bc,
+                                   unsigned phic, T* s) {
+  if (act) {
+    T r[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      const unsigned o = own0 + (k & 1) + (k >> 1) * (unsigned)nx;
+      T sn;
+      if (k == 0) sn = tq_nbr<T, 0, false>(q, o, nx, 0u, s);
+      else if (k == 1) sn = tq_nbr<T, 1, false>(q, o, nx, 0u, s);
+      else if (k == 2) sn = tq_nbr<T, 2, false>(q, o, nx, 0u, s);
+      else sn = tq_nbr<T, 3, false>(q, o, nx, 0u, s);
+      r[k] = s[o + db] - (q.diag[k] * s[o] - sn);
+    }
+    T acc = T(0);
+#pragma unroll
+    for (int k = 0; k < 4; ++k) acc += r[k];
+    s[bc] = acc;
+    s[phic] = T(0);
+  }
+  __syncthreads();
+}
+
+// The coarsest level's solve, mg_coarse_solve's arithmetic with the operator in LDS: ci holds
+// the dense inverse transposed (element (t, q) at q n + t, so that the threads' reads of one q
+// are consecutive), or c_x, c_y and D^-1 back to back.
+template <typename T>
+__device__ inline void tq_coarse(const MgDims& d, bool dense, const T* ci, const T* b, T* phi, int sweeps,
+                                 int project, T* sphi, T* sb, double* red) {
+  const int n = d.nx * d.ny;
+  const int nt = (int)blockDim.x;
+  for (int t = threadIdx.x; t < n; t += nt) { sb[t] = b[t]; sphi[t] = T(0); }
+  __syncthreads();
+  if (project) {
+    if (threadIdx.x < 64) {
+      double my = 0.0;
+      for (int t = threadIdx.x; t < n; t += 64) my += (double)sb[t];
+      my = wave_sum(my);
+      if (threadIdx.x == 0) red[0] = my / (double)n;
+    }
+    __syncthreads();
+    const double m = red[0];
+    for (int t = threadIdx.x; t < n; t += nt) sb[t] = sb[t] - (T)m;
+    __syncthreads();
+  }
+  if (dense) {
+    for (int t = threadIdx.x; t < n; t += nt) {
+      double s = 0.0;
+      for (int q = 0; q < n; ++q) s += (double)ci[q * n + t] * (double)sb[q];
+      phi[t] = (T)s;
+    }
+    __syncthreads();
+    return;
+  }
+  const T* cx = ci;
+  const T* cy = ci + (d.nx + 1) * d.ny;
+  const T* dinv = cy + d.nx * (d.ny + 1);
+  for (int s = 0; s < sweeps; ++s) {
+    for (int colour = 0; colour < 2; ++colour) {
+      for (int t = threadIdx.x; t < n; t += nt) {
+        const int i = t % d.nx, j = t / d.nx;
+        if (((i + j) & 1) == colour) sphi[t] = (sb[t] + mg_nbr_sum(d, cx, cy, (const T*)sphi, i, j)) * dinv[t];
+      }
+      __syncthreads();
+    }
+  }
+  for (int t = threadIdx.x; t < n; t += nt) phi[t] = sphi[t];
+  __syncthreads();
+}
+
+// NL smoothing levels a.L[0..NL-1] and the coarsest a.L[NL]; LDS: level l's b at bo[l], its phi
+// at bo[l] + n_l; the coarsest level's operator in sc.
+template <typename T, int NL>
+__global__ void __launch_bounds__(MG_TAILQ_NTH) k_mg_tail_q(MgTailArgs<T> a) {
+  __shared__ T s[MG_TAILQ_POOL];
+  __shared__ T sc[MG_TAILQ_COARSE];
+  __shared__ T sphi[MG_COARSE_MAX];
+  __shared__ T sb[MG_COARSE_MAX];
+  __shared__ double red[16];
+  constexpr int NTH = MG_TAILQ_NTH;
+  constexpr int KC = (MG_TAILQ_COARSE + NTH - 1) / NTH;
+  const int t = (int)threadIdx.x;
+  int bo[NL + 1], nq[NL];
+  {
+    int off = 0;
+#pragma unroll
+    for (int l = 0; l <= NL; ++l) {
+      bo[l] = off;
+      off += 2 * a.L[l].d.nx * a.L[l].d.ny;
+    }
+#pragma unroll
+    for (int l = 0; l < NL; ++l) nq[l] = (a.L[l].d.nx >> 1) * (a.L[l].d.ny >> 1);
+  }
+  auto PHI = [&](int l) { return (unsigned)(bo[l] + a.L[l].d.nx * a.L[l].d.ny); };
+  // every load in flight: the top level's b and phi (zero for a zero guess; finite values are
+  // needed even then, the wall terms read them times a zero conductance) and the quads'
+  // coefficients
+  // (indices clamped into the arrays instead of branches around the loads: the set-up's cost is
+  // its instruction count; values loaded for no cell are never used)
+  const MgDims d0 = a.L[0].d;
+  const int n0 = d0.nx * d0.ny;
+  T vb[4], vp[4];
+#pragma unroll
+  for (int r = 0; r < 4; ++r) {
+    const unsigned e = (unsigned)min(t + NTH * r, n0 - 1);
+    vb[r] = mgt_ld(a.L[0].b, e);
+    vp[r] = a.zero_guess ? T(0) : mgt_ld((const T*)a.L[0].phi, e);
+  }
+  // the coarsest level's operator: the dense inverse, transposed (LDS element p = q n + r holds
+  // inverse element (r, q)), or c_x, c_y and D^-1 back to back
+  const MgDims dc = a.L[NL].d;
+  const int ncc = dc.nx * dc.ny;
+  const bool dense = a.L[NL].cinv != nullptr;
+  const int nco = dense ? ncc * ncc : (dc.nx + 1) * dc.ny + dc.nx * (dc.ny + 1) + ncc;
+  const int ncx = (dc.nx + 1) * dc.ny, ncy = dc.nx * (dc.ny + 1);
+  T vc[KC];
+  if (dense) {
+#pragma unroll
+    for (int k = 0; k < KC; ++k) {
+      int r, qq;
+      tq_quad(min(t + NTH * k, nco - 1), ncc, r, qq);   // p = r + qq n
+      vc[k] = mgt_ld(a.L[NL].cinv, (unsigned)(r * ncc + qq));
+    }
+  } else {
+#pragma unroll
+    for (int k = 0; k < KC; ++k) {
+      const int e = t + NTH * k;
+      vc[k] = T(0);
+      if (e < ncx) vc[k] = mgt_ld(a.L[NL].cx, (unsigned)e);
+      else if (e < ncx + ncy) vc[k] = mgt_ld(a.L[NL].cy, (unsigned)(e - ncx));
+      else if (e < nco) vc[k] = mgt_ld(a.L[NL].dinv, (unsigned)(e - ncx - ncy));
+    }
+  }
+  TqQuad<T> q[NL];
+  T mm[NL][4];
+  int qI[NL], qJ[NL];
+  // (the quads of the lower levels occupy the first wavefronts only: the others skip their loads)
+#pragma unroll
+  for (int l = 0; l < NL; ++l) {
+    tq_quad(t, a.L[l].d.nx >> 1, qI[l], qJ[l]);
+    if (t < nq[l]) tq_load<T>(a.L[l], qI[l], qJ[l], PHI(l), PHI(l + 1), q[l], mm[l]);
+  }
+#pragma unroll
+  for (int l = 0; l < NL; ++l)
+    if (t < nq[l]) tq_finish<T>(a.L[l].d, qI[l], qJ[l], q[l], mm[l], a.L[l].m != nullptr);
+#pragma unroll
+  for (int r = 0; r < 4; ++r) {
+    const int e = t + NTH * r;
+    if (e < n0) { s[bo[0] + e] = vb[r]; s[PHI(0) + e] = vp[r]; }
+  }
+#pragma unroll
+  for (int k = 0; k < KC; ++k) {
+    const int e = t + NTH * k;
+    if (e < nco) sc[e] = vc[k];
+  }
+  __syncthreads();
+  unsigned own0[NL];
+#pragma unroll
+  for (int l = 0; l < NL; ++l) own0[l] = PHI(l) + (unsigned)(2 * qI[l] + 2 * qJ[l] * a.L[l].d.nx);
+  const int nu = a.nu;
+  // descent: pre-smoothing, residuals summed onto the coarse cells
+#pragma unroll
+  for (int l = 0; l < NL; ++l) {
+    const int nx = a.L[l].d.nx, db = bo[l] - (int)PHI(l);
+    const bool act = t < nq[l];
+    const bool zg = (l == 0) ? a.zero_guess != 0 : true;
+    for (int sw = 0; sw < nu; ++sw) {
+      tq_half<T, 0, false>(q[l], act, own0[l], nx, db, 0u, sw == 0 && zg, s);
+      tq_half<T, 1, false>(q[l], act, own0[l], nx, db, 0u, false, s);
+    }
+    tq_restrict<T>(q[l], act, own0[l], nx, db, (unsigned)(bo[l + 1] + t), PHI(l + 1) + (unsigned)t, s);
+  }
+  tq_coarse(dc, dense, (const T*)sc, (const T*)(s + bo[NL]), s + PHI(NL), a.coarse_sweeps, a.project, sphi, sb, red);
+  // ascent: post-smoothing, the prolongation folded into its first half-sweep
+#pragma unroll
+  for (int l = NL - 1; l >= 0; --l) {
+    const int nx = a.L[l].d.nx, db = bo[l] - (int)PHI(l);
+    const bool act = t < nq[l];
+    const unsigned pc = PHI(l + 1) + (unsigned)t;
+    tq_half<T, 1, true>(q[l], act, own0[l], nx, db, pc, false, s);
+    tq_half<T, 0, false>(q[l], act, own0[l], nx, db, 0u, false, s);
+    for (int sw = 1; sw < nu; ++sw) {
+      tq_half<T, 1, false>(q[l], act, own0[l], nx, db, 0u, false, s);
+      tq_half<T, 0, false>(q[l], act, own0[l], nx, db, 0u, false, s);
+    }
+  }
+#pragma unroll
+  for (int r = 0; r < 4; ++r) {
+    const int e = t + NTH * r;
+    if (e < n0) a.L[0].phi[e] = s[PHI(0) + e];
+  }
+}
+
 // Level-0 load: b0 = z_p (- mean when both directions are walls), compact layout; also clears
 // fl[done_flag] (the pressure block's early-exit flag, which the PCD reads after it).
 template <typename T>

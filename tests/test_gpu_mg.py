@@ -53,25 +53,28 @@ def test_mg_vcycle_is_linear():
 
 
 @pytest.mark.parametrize("nx,ny,per_x,per_y", [(32, 24, False, False), (48, 40, True, False), (40, 64, False, True),
-                                               (96, 80, True, True), (130, 70, False, False)])
+                                               (96, 80, True, True), (130, 70, False, False), (64, 128, False, False)])
 def test_coarse_tail_is_bitwise_the_multilaunch_cycle(monkeypatch, nx, ny, per_x, per_y):
-    """The single-launch coarse tail (LDS-staged or global) and the fused (temporal-blocking)
-    level kernels perform the one-launch-per-half-sweep V-cycle's arithmetic: results equal bit
-    for bit with both off, at a 4096-cell threshold, with the default (LDS-fit) start and with the
-    whole hierarchy in the tail (odd extents included), with the LDS-tile and the register-strip
-    level kernels (large and small strip shapes)."""
+    """The single-launch coarse tail (register-resident quads, LDS-staged or global) and the fused
+    (temporal-blocking) level kernels perform the one-launch-per-half-sweep V-cycle's arithmetic:
+    results equal bit for bit (signs of zeros included) with both off, at a 4096-cell threshold,
+    with the default (LDS-fit) start and with the whole hierarchy in the tail (odd extents
+    included), with the quad form of the tail on and off, with the LDS-tile and the
+    register-strip level kernels (large and small strip shapes)."""
     g = O.Grid(nx, ny, 1.0, ny / nx, per_x=per_x, per_y=per_y)
     rng = np.random.default_rng(202)
     c, pr, _, _ = two_phase_case(g, rng, rho=(100.0, 1.0))
     O.halo(g, pr["rho_u"], "u"); O.halo(g, pr["rho_v"], "v")
     rhs = rand(g, "p", rng)
     out = {}
-    for cells, fused, strip in (("0", "0", "0"), ("0", "1", "0"), ("4096", "1", "0"), ("1000000", "1", "0"),
-                                ("4096", "0", "0"), ("-1", "1", "0"), ("0", "1", "2"), ("-1", "1", "2"),
-                                ("0", "1", "3"), ("-1", "1", "3")):
+    for cells, fused, strip, quad in (("0", "0", "0", "1"), ("0", "1", "0", "1"), ("4096", "1", "0", "1"),
+                                      ("1000000", "1", "0", "1"), ("4096", "0", "0", "1"), ("-1", "1", "0", "1"),
+                                      ("-1", "1", "0", "0"), ("-1", "0", "0", "0"), ("0", "1", "2", "1"),
+                                      ("-1", "1", "2", "1"), ("0", "1", "3", "1"), ("-1", "1", "3", "1")):
         monkeypatch.setenv("FUGU_MG_TAIL_CELLS", cells)
         monkeypatch.setenv("FUGU_MG_FUSED", fused)
         monkeypatch.setenv("FUGU_MG_STRIP", strip)
+        monkeypatch.setenv("FUGU_MG_TAIL_QUAD", quad)
         ctx = abi.Context(g.nx, g.ny, g.lx, g.ly, per_x=per_x, per_y=per_y)
         res = []
         for n in (1, 3):
@@ -79,10 +82,10 @@ def test_coarse_tail_is_bitwise_the_multilaunch_cycle(monkeypatch, nx, ny, per_x
             ctx.pressure_mg(phi, rhs, pr["rho_u"], pr["rho_v"], n_cycles=n)
             res.append(phi)
         ctx.close()
-        out[(cells, fused, strip)] = res
+        out[(cells, fused, strip, quad)] = res
     for key in out:
-        for a, b in zip(out[key], out[("0", "0", "0")]):
-            assert np.array_equal(a, b), key
+        for a, b in zip(out[key], out[("0", "0", "0", "1")]):
+            assert np.array_equal(a, b) and np.array_equal(np.signbit(a), np.signbit(b)), key
 
 
 @pytest.mark.parametrize("theta,coarse_nu,cycles", [("0", "2", "1"), ("3", "2", "1"), ("10", "4", "0"), ("10", "3", "2"),
@@ -103,10 +106,12 @@ def test_coarse_tail_bitwise_in_the_step_solve(monkeypatch, mixed, theta, coarse
     monkeypatch.setenv("FUGU_VMG_THETA2", theta)
     monkeypatch.setenv("FUGU_VMG_COARSE_NU", coarse_nu)
     monkeypatch.setenv("FUGU_VMG_CYCLES", cycles)
-    for cells, fused, strip in (("0", "0", "0"), ("-1", "1", "0"), ("-1", "1", "2"), ("-1", "1", "3")):
+    for cells, fused, strip, quad in (("0", "0", "0", "1"), ("-1", "1", "0", "1"), ("-1", "1", "0", "0"),
+                                      ("-1", "1", "2", "1"), ("-1", "1", "3", "1")):
         monkeypatch.setenv("FUGU_MG_TAIL_CELLS", cells)
         monkeypatch.setenv("FUGU_MG_FUSED", fused)
         monkeypatch.setenv("FUGU_MG_STRIP", strip)
+        monkeypatch.setenv("FUGU_MG_TAIL_QUAD", quad)
         p = abi.FuguParams(1000.0, 1.0, 10.0, 0.1, 2.0, 0.0, -1.0, 2, 1e12, 0.5)
         ctx = abi.Context(nx, ny, 1.0, 2.0, False, False, params=p)
         nh = 2
@@ -142,10 +147,12 @@ def test_sweep_counts_fused_is_bitwise_the_multilaunch_cycle(monkeypatch, nu_p, 
     monkeypatch.setenv("FUGU_VMG_THETA", "0")     # full-depth velocity hierarchies: their tail runs too
     monkeypatch.setenv("FUGU_VMG_THETA2", "0")
     outs = []
-    for cells, fused, strip in (("0", "0", "0"), ("-1", "1", "0"), ("-1", "1", "2"), ("-1", "1", "3")):
+    for cells, fused, strip, quad in (("0", "0", "0", "1"), ("-1", "1", "0", "1"), ("-1", "1", "0", "0"),
+                                      ("-1", "1", "2", "1"), ("-1", "1", "3", "1")):
         monkeypatch.setenv("FUGU_MG_TAIL_CELLS", cells)
         monkeypatch.setenv("FUGU_MG_FUSED", fused)
         monkeypatch.setenv("FUGU_MG_STRIP", strip)
+        monkeypatch.setenv("FUGU_MG_TAIL_QUAD", quad)
         p = abi.FuguParams(1000.0, 1.0, 10.0, 0.1, 2.0, 0.0, -1.0, 2, 1e12, 0.5)
         ctx = abi.Context(nx, ny, 1.0, 2.0, False, False, params=p)
         nh = 2
