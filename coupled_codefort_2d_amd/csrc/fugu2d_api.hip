@@ -168,6 +168,9 @@ struct fugu_ctx {
   // (mg_strip_shape), 0 never, 2 / 3 the large / small shapes on every fused level (bit-identical
   // either way)
   int mg_strip = 1;
+  // the shape rule's bounds (log2 of a level's cells): large strips from 2^mgs_large_log2 cells
+  // (half that with three or more sweeps), small strips up to 2^mgs_small_log2
+  int mgs_large_log2 = 21, mgs_small_log2 = 17;
   // terms per chunk of the reference-order sums: 0 by the sum's length (sq_chunk), else 512, 1024
   // or 4096 (the same sums bit for bit)
   int seq_chunk = 0;
@@ -826,8 +829,9 @@ inline MgsShape mg_strip_shape(const fugu_ctx* c, const MgDims& d, int nu) {
   if (c->mg_strip == 2) return MgsShape::large;   // every fused level (tests)
   if (c->mg_strip == 3) return MgsShape::small;
   const double n = (double)d.nx * d.ny;
-  if (n >= (double)(1 << 21) || (n >= (double)(1 << 20) && nu >= 3)) return MgsShape::large;
-  if (n <= (double)(1 << 17)) return MgsShape::small;
+  const int lg = c->mgs_large_log2;
+  if (n >= std::ldexp(1.0, lg) || (n >= std::ldexp(1.0, lg - 1) && nu >= 3)) return MgsShape::large;
+  if (n <= std::ldexp(1.0, c->mgs_small_log2)) return MgsShape::small;
   return MgsShape::none;
 }
 inline const char* mgs_kname(bool post, int nu, bool f64) {
@@ -2588,6 +2592,8 @@ int fugu_ctx_create(const fugu_grid* grid, const fugu_params* prm, fugu_ctx** ou
     if (const char* ts = std::getenv("FUGU_MG_TAIL_STAGE")) c->mg_tail_stage = std::atoi(ts) != 0;
     if (const char* tq = std::getenv("FUGU_MG_TAIL_QUAD")) c->mg_tail_quad = std::atoi(tq) != 0;
     if (const char* st = std::getenv("FUGU_MG_STRIP")) c->mg_strip = std::atoi(st);
+    if (const char* sl = std::getenv("FUGU_MGS_LARGE_LOG2")) c->mgs_large_log2 = std::atoi(sl);
+    if (const char* ss = std::getenv("FUGU_MGS_SMALL_LOG2")) c->mgs_small_log2 = std::atoi(ss);
     if (const char* sq = std::getenv("FUGU_SEQ_CHUNK")) c->seq_chunk = std::atoi(sq);
     if (const char* s = std::getenv("FUGU_MG_NU_P")) c->mg_nu_p = std::min(4, std::max(1, std::atoi(s)));
     if (const char* s = std::getenv("FUGU_MV_TILE")) c->mv_tile = std::atoi(s);
