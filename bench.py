@@ -31,7 +31,8 @@ tree order, the FP64 fast form, and the oracle on one host core -- the `cpu_base
 "adjoint2048": the combined 2048^2 workload in one call -- the mixed forward over 32 BDF2 steps
 (dt 5e-5; 256 seeded inclusions r ~ U[0.005, 0.02], carrier rho 1000 / mu 1e-2, inclusions
 rho 1 / mu 1e-2 * 10^U[0,3], sigma 1, g -1) with a restart state every 4 steps, then the full
-adjoint gradient of the same objective (FP64 adjoint solves, each segment's forward states
+adjoint gradient of the same objective (adjoint solves by the mixed form too: FP32 cycles of
+the transposed operator and PCD refined in FP64 to 1e-10; each segment's forward states
 recomputed from its checkpoint).
 "pcg2048": the drop-in's PCG kernels (k_pcg_iter1/k_pcg_iter2, 112 B per cell and iteration)
 on 2048^2, 150 iterations, timed with HIP events -- their roofline.
@@ -439,13 +440,15 @@ def timed_rate(ktimes):
 
 def run_adjoint(abi, n=1024, nsteps=16, dt=5e-5, count=64, seed=4, rmin=0.01, rmax=0.04, rho_c=100.0,
                 mu_ratio=1e3, sigma=0.5, checkpoint_every=4, mixed=False, also_resident=False, mg_cycles=1,
-                moving=False, restart=30):
+                moving=False, restart=30, mixed_adjoint=False):
     """Seconds per gradient on an adjoint workload: the forward horizon with checkpoints (restart
     states every `checkpoint_every` steps), the reverse sweep with each segment's forward states
     recomputed, and the property-path gradients; after a 2-step warm-up call on the same
     context.  also_resident: the same gradient with every step's state kept in HBM (no
     recomputation) is timed too, and reported as seconds_per_gradient when it agrees.  moving: the
-    interface motion differentiated too (FUGU_ADJ_MOVING_INTERFACE) instead of held frozen."""
+    interface motion differentiated too (FUGU_ADJ_MOVING_INTERFACE) instead of held frozen.
+    mixed / mixed_adjoint: the forward / adjoint solves by FP32 Arnoldi cycles refined in FP64 to
+    the same true residual."""
     from coupled_codefort_2d_amd import workloads
     c, mu, rho, _ = workloads.inclusions(n, count, rmin, rmax, seed, rho_c, 1e-2, 1.0, mu_ratio)
     ctx = abi.Context(n, n, 1.0, 1.0, False, False,
@@ -453,7 +456,8 @@ def run_adjoint(abi, n=1024, nsteps=16, dt=5e-5, count=64, seed=4, rmin=0.01, rm
     u0 = np.zeros((n + 4, n + 5))
     v0 = np.zeros((n + 5, n + 4))
     gm, gr = np.zeros_like(mu), np.zeros_like(rho)
-    flags = abi.ADJ_PRECOND_VMG | (abi.ADJ_MIXED_FORWARD if mixed else 0) | (abi.ADJ_MOVING_INTERFACE if moving else 0)
+    flags = (abi.ADJ_PRECOND_VMG | (abi.ADJ_MIXED_FORWARD if mixed else 0) | (abi.ADJ_MOVING_INTERFACE if moving else 0) |
+             (abi.ADJ_MIXED_ADJOINT if mixed_adjoint else 0))
     ctx.adjoint_gradient(abi.FuguAdjointOpts(2, dt, 0.9, 1e-10, 3000, restart, mg_cycles, flags), mu, rho, sigma, c, u0, v0,
                          gm, gr)
     opts = abi.FuguAdjointOpts(nsteps, dt, 0.9, 1e-10, 3000, restart, mg_cycles, flags, checkpoint_every)
@@ -489,8 +493,9 @@ def run_adjoint(abi, n=1024, nsteps=16, dt=5e-5, count=64, seed=4, rmin=0.01, rm
             "adjoint_iters": st.adjoint_iters, "max_forward_res": float(f"{st.max_forward_res:.3e}"),
             "max_adjoint_res": float(f"{st.max_adjoint_res:.3e}"), "rc": rc, "J": J, "dJ_dsigma": gs,
             "interface": "moving (VOF, SVOF, curvature and CSF transposed)" if moving else "frozen",
-            "solves": ("mixed (FP32 cycles, FP64 restarts)" if mixed else "FP64") + " forward and FP64 adjoint "
-                      f"FGMRES({restart}) to 1e-10, PCD with multigrid pressure ({mg_cycles} V-cycle(s)) and velocity blocks",
+            "solves": ("mixed (FP32 cycles, FP64 restarts)" if mixed else "FP64") + " forward and " +
+                      ("mixed" if mixed_adjoint else "FP64") + f" adjoint FGMRES({restart}) to 1e-10, PCD with multigrid "
+                      f"pressure ({mg_cycles} V-cycle(s)) and velocity blocks",
             **({"resident_horizon": res, "checkpointed_seconds_per_gradient": round(wall, 3),
                 "seconds_per_gradient": res["seconds_per_gradient"] if res["agrees_with_checkpointed"]
                 else round(wall, 3)} if res else {})}
@@ -529,7 +534,7 @@ def main():
                          "else 20 up to 2^21 cells, 30 above)")
     ap.add_argument("--sections", default="headline,m1,adjoint,adjoint2048,dropin,pcg2048,physics",
                     help="headline (rising bubble, value), m1 (2048 x 2048 FP64 vs mixed), adjoint (s/grad, "
-                         "1024^2, 16 steps), adjoint2048 (2048^2, 256 inclusions, 32 steps, mixed forward), "
+                         "1024^2, 16 steps), adjoint2048 (2048^2, 256 inclusions, 32 steps, mixed forward and adjoint), "
                          "dropin (reference algorithm on the 256^2 droplet, GPU vs the oracle on one core: "
                          "cpu_baseline), pcg2048 (the drop-in's PCG kernels on 2048^2), physics (configs 1 and 2 "
                          "at full length: time steps/s and the reference's validation outputs)")
@@ -578,8 +583,9 @@ def main():
     if "adjoint2048" in sections:
         # pressure V-cycles per PCD (the 1e-10 adjoint solves lean on the pressure block): with the
         # register-strip level kernels 40.1 s/grad with 4, 39.3 with 6, 36.7 with 8, 36.9 with 12
+        # (FP64 adjoint solves); the mixed adjoint solves: 35.5 -> 28.8 s/grad, the same FGMRES steps
         adj2048 = run_adjoint(abi, n=2048, nsteps=32, count=256, seed=5, rmin=0.005, rmax=0.02, rho_c=1000.0,
-                              sigma=1.0, mixed=True, mg_cycles=8)
+                              sigma=1.0, mixed=True, mg_cycles=8, mixed_adjoint=True)
     dropin = run_dropin(abi, no_cpu=args.no_cpu) if "dropin" in sections else None
     pcg = run_pcg2048(abi) if "pcg2048" in sections else None
     physics = run_physics(abi) if "physics" in sections else None

@@ -60,6 +60,7 @@ ADJ_REUSE_INTERFACE = 2
 ADJ_PRECOND_VMG = 4
 ADJ_MIXED_FORWARD = 8
 ADJ_MOVING_INTERFACE = 16
+ADJ_MIXED_ADJOINT = 32
 
 B2_CONSISTENT_BCS = 1
 SOLVE_REFINED = 2

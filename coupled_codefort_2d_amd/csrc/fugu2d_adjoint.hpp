@@ -13,6 +13,11 @@
 // neighbour wrap rules (periodic neighbours wrap; wall ghosts are constant zeros whose adjoints
 // are dropped).  The coefficient-gradient kernel accumulates lam^T dA/dcoef X with FP64 atomics
 // (one pass per step of the reverse sweep, not on the Krylov path).
+//
+// The Krylov-path kernels (the transposed matvec, halo and BC folds, the transposed PCD pieces)
+// take their arrays in a storage type T -- double, or float for the FP32 Arnoldi cycles of the
+// mixed adjoint solves -- and compute in FP64 registers; for T = double the arithmetic is the
+// FP64 kernels' operation for operation.
 #pragma once
 #include "fugu2d_common.hpp"
 
@@ -39,32 +44,34 @@ __device__ inline bool in_w(const Grid& g, int i, int j) {
 }
 
 // E0^T / R^T: zero everything outside the owned sets (any of the pointers may be null).
-__global__ void __launch_bounds__(NT) k_mask_trip(Grid g, double* __restrict__ u, double* __restrict__ v,
-                                                  double* __restrict__ p) {
+template <typename T>
+__global__ void __launch_bounds__(NT) k_mask_trip(Grid g, T* __restrict__ u, T* __restrict__ v, T* __restrict__ p) {
   int i, j;
   union_ij(g, i, j);
-  if (u && in_u(g, i, j) && !own_u(g, i, j)) u[IU(g, i, j)] = 0.0;
-  if (v && in_v(g, i, j) && !own_v(g, i, j)) v[IV(g, i, j)] = 0.0;
-  if (p && in_p(g, i, j) && !own_p(g, i, j)) p[IP(g, i, j)] = 0.0;
+  if (u && in_u(g, i, j) && !own_u(g, i, j)) u[IU(g, i, j)] = T(0);
+  if (v && in_v(g, i, j) && !own_v(g, i, j)) v[IV(g, i, j)] = T(0);
+  if (p && in_p(g, i, j) && !own_p(g, i, j)) p[IP(g, i, j)] = T(0);
 }
 
+template <typename T>
 struct MatvecTArgs {
-  const double *Lu, *Lv, *Lp;
-  double *Ou, *Ov, *Op;
-  const double *uadv, *vadv, *rho_c, *rho_u, *rho_v, *kap, *zeta, *eta;
+  const T *Lu, *Lv, *Lp;
+  T *Ou, *Ov, *Op;
+  const T *uadv, *vadv, *rho_c, *rho_u, *rho_v, *kap, *zeta, *eta;
   double dt;
   int adv;
 };
 
 // A_pad^T (R^T lam) at every padded position (gather form of the reverse-mode of the matvec).
-__global__ void __launch_bounds__(NT) k_matvec_T(Grid g, MatvecTArgs a) {
+template <typename T>
+__global__ void __launch_bounds__(NT) k_matvec_T(Grid g, MatvecTArgs<T> a) {
   int i, j;
   union_ij(g, i, j);
   const double ix = g.inv_dx, iy = g.inv_dy, half = 0.5, dt = a.dt;
-  auto LU = [&](int ii, int jj) -> double { return own_u(g, ii, jj) ? a.Lu[IU(g, ii, jj)] : 0.0; };
-  auto LV = [&](int ii, int jj) -> double { return own_v(g, ii, jj) ? a.Lv[IV(g, ii, jj)] : 0.0; };
-  auto LP = [&](int ii, int jj) -> double { return own_p(g, ii, jj) ? a.Lp[IP(g, ii, jj)] : 0.0; };
-  auto KAP = [&](int ii, int jj) -> double { return in_p(g, ii, jj) ? a.kap[IP(g, ii, jj)] : 0.0; };
+  auto LU = [&](int ii, int jj) -> double { return own_u(g, ii, jj) ? (double)a.Lu[IU(g, ii, jj)] : 0.0; };
+  auto LV = [&](int ii, int jj) -> double { return own_v(g, ii, jj) ? (double)a.Lv[IV(g, ii, jj)] : 0.0; };
+  auto LP = [&](int ii, int jj) -> double { return own_p(g, ii, jj) ? (double)a.Lp[IP(g, ii, jj)] : 0.0; };
+  auto KAP = [&](int ii, int jj) -> double { return in_p(g, ii, jj) ? (double)a.kap[IP(g, ii, jj)] : 0.0; };
   // adjoint stresses from lam (zero wherever no owned row references them)
   auto TXXb = [&](int ci, int cj) -> double { return (LU(ci, cj) - LU(ci - 1, cj)) * ix; };
   auto TYYb = [&](int ci, int cj) -> double { return (LV(ci, cj) - LV(ci, cj - 1)) * iy; };
@@ -72,41 +79,42 @@ __global__ void __launch_bounds__(NT) k_matvec_T(Grid g, MatvecTArgs a) {
   auto TYXb = [&](int vi, int vj) -> double { return (LV(vi + 1, vj) - LV(vi, vj)) * ix; };
   auto DUB = [&](int vi, int vj) -> double {     // d/d(du_dy) at vertex, times 1/dy
     if (!in_w(g, vi, vj)) return 0.0;
-    const double ze = a.zeta[IW(g, vi, vj)], et = a.eta[IW(g, vi, vj)];
+    const double ze = (double)a.zeta[IW(g, vi, vj)], et = (double)a.eta[IW(g, vi, vj)];
     return ((ze - et) * TXYb(vi, vj) + et * TYXb(vi, vj)) * iy;
   };
   auto DVB = [&](int vi, int vj) -> double {     // d/d(dv_dx) at vertex, times 1/dx
     if (!in_w(g, vi, vj)) return 0.0;
-    const double ze = a.zeta[IW(g, vi, vj)], et = a.eta[IW(g, vi, vj)];
+    const double ze = (double)a.zeta[IW(g, vi, vj)], et = (double)a.eta[IW(g, vi, vj)];
     return (et * TXYb(vi, vj) + (ze - et) * TYXb(vi, vj)) * ix;
   };
   // advective mass fluxes of the u row (ri, rj) and of the v row (ri, rj)
-  auto UA = [&](int ii, int jj) -> double { return a.uadv[IU(g, ii, jj)]; };
-  auto VA = [&](int ii, int jj) -> double { return a.vadv[IV(g, ii, jj)]; };
-  auto mEu = [&](int ri, int rj) { return a.rho_c[IP(g, ri + 1, rj)] * half * (UA(ri, rj) + UA(ri + 1, rj)); };
-  auto mWu = [&](int ri, int rj) { return a.rho_c[IP(g, ri, rj)] * half * (UA(ri - 1, rj) + UA(ri, rj)); };
+  auto UA = [&](int ii, int jj) -> double { return (double)a.uadv[IU(g, ii, jj)]; };
+  auto VA = [&](int ii, int jj) -> double { return (double)a.vadv[IV(g, ii, jj)]; };
+  auto RC = [&](int ii, int jj) -> double { return (double)a.rho_c[IP(g, ii, jj)]; };
+  auto RU = [&](int ii, int jj) -> double { return (double)a.rho_u[IU(g, ii, jj)]; };
+  auto RV = [&](int ii, int jj) -> double { return (double)a.rho_v[IV(g, ii, jj)]; };
+  auto mEu = [&](int ri, int rj) { return RC(ri + 1, rj) * half * (UA(ri, rj) + UA(ri + 1, rj)); };
+  auto mWu = [&](int ri, int rj) { return RC(ri, rj) * half * (UA(ri - 1, rj) + UA(ri, rj)); };
   auto mNu = [&](int ri, int rj) {
-    return half * (a.rho_v[IV(g, ri, rj)] + a.rho_v[IV(g, ri + 1, rj)]) * half * (VA(ri, rj) + VA(ri + 1, rj));
+    return half * (RV(ri, rj) + RV(ri + 1, rj)) * half * (VA(ri, rj) + VA(ri + 1, rj));
   };
   auto mSu = [&](int ri, int rj) {
-    return half * (a.rho_v[IV(g, ri, rj - 1)] + a.rho_v[IV(g, ri + 1, rj - 1)]) * half *
-           (VA(ri, rj - 1) + VA(ri + 1, rj - 1));
+    return half * (RV(ri, rj - 1) + RV(ri + 1, rj - 1)) * half * (VA(ri, rj - 1) + VA(ri + 1, rj - 1));
   };
   auto mEv = [&](int ri, int rj) {
-    return half * (a.rho_u[IU(g, ri, rj)] + a.rho_u[IU(g, ri, rj + 1)]) * half * (UA(ri, rj) + UA(ri, rj + 1));
+    return half * (RU(ri, rj) + RU(ri, rj + 1)) * half * (UA(ri, rj) + UA(ri, rj + 1));
   };
   auto mWv = [&](int ri, int rj) {
-    return half * (a.rho_u[IU(g, ri - 1, rj)] + a.rho_u[IU(g, ri - 1, rj + 1)]) * half *
-           (UA(ri - 1, rj) + UA(ri - 1, rj + 1));
+    return half * (RU(ri - 1, rj) + RU(ri - 1, rj + 1)) * half * (UA(ri - 1, rj) + UA(ri - 1, rj + 1));
   };
-  auto mNv = [&](int ri, int rj) { return a.rho_c[IP(g, ri, rj + 1)] * half * (VA(ri, rj) + VA(ri, rj + 1)); };
-  auto mSv = [&](int ri, int rj) { return a.rho_c[IP(g, ri, rj)] * half * (VA(ri, rj - 1) + VA(ri, rj)); };
+  auto mNv = [&](int ri, int rj) { return RC(ri, rj + 1) * half * (VA(ri, rj) + VA(ri, rj + 1)); };
+  auto mSv = [&](int ri, int rj) { return RC(ri, rj) * half * (VA(ri, rj - 1) + VA(ri, rj)); };
 
   // ---- u position (i, j)
   if (in_u(g, i, j)) {
     double o = 0.0;
     o += (LP(i, j) - LP(i + 1, j)) * ix;                                  // divergence rows
-    if (own_u(g, i, j)) o += (a.rho_u[IU(g, i, j)] / dt) * LU(i, j);      // mass
+    if (own_u(g, i, j)) o += (RU(i, j) / dt) * LU(i, j);                   // mass
     o += KAP(i, j) * TXXb(i, j) * ix - KAP(i + 1, j) * TXXb(i + 1, j) * ix;
     o += DUB(i, j - 1) - DUB(i, j);
     if (a.adv) {
@@ -119,13 +127,13 @@ __global__ void __launch_bounds__(NT) k_matvec_T(Grid g, MatvecTArgs a) {
       if (own_u(g, i, j - 1)) o += LU(i, j - 1) * half * mNu(i, j - 1) * iy;
       if (own_u(g, i, j + 1)) o -= LU(i, j + 1) * half * mSu(i, j + 1) * iy;
     }
-    a.Ou[IU(g, i, j)] = o;
+    a.Ou[IU(g, i, j)] = (T)o;
   }
   // ---- v position (i, j)
   if (in_v(g, i, j)) {
     double o = 0.0;
     o += (LP(i, j) - LP(i, j + 1)) * iy;
-    if (own_v(g, i, j)) o += (a.rho_v[IV(g, i, j)] / dt) * LV(i, j);
+    if (own_v(g, i, j)) o += (RV(i, j) / dt) * LV(i, j);
     o += KAP(i, j) * TYYb(i, j) * iy - KAP(i, j + 1) * TYYb(i, j + 1) * iy;
     o += DVB(i - 1, j) - DVB(i, j);
     if (a.adv) {
@@ -138,12 +146,12 @@ __global__ void __launch_bounds__(NT) k_matvec_T(Grid g, MatvecTArgs a) {
       if (own_v(g, i, j - 1)) o += LV(i, j - 1) * half * mNv(i, j - 1) * iy;
       if (own_v(g, i, j + 1)) o -= LV(i, j + 1) * half * mSv(i, j + 1) * iy;
     }
-    a.Ov[IV(g, i, j)] = o;
+    a.Ov[IV(g, i, j)] = (T)o;
   }
   // ---- p position (i, j): the pressure gradient of the u and v rows
   if (in_p(g, i, j)) {
     const double o = (LU(i - 1, j) - LU(i, j)) * ix + (LV(i, j - 1) - LV(i, j)) * iy;
-    a.Op[IP(g, i, j)] = o;
+    a.Op[IP(g, i, j)] = (T)o;
   }
 }
 
@@ -152,7 +160,14 @@ __global__ void __launch_bounds__(NT) k_matvec_T(Grid g, MatvecTArgs a) {
 __device__ inline size_t idx_kind(const Grid& g, int kind, int i, int j) {
   return kind == 0 ? IU(g, i, j) : kind == 1 ? IV(g, i, j) : IP(g, i, j);
 }
-__global__ void k_halo_T_y(Grid g, double* __restrict__ a, int kind) {
+// (a ghost folded onto its source: s += d in FP64, then d = 0)
+template <typename T>
+__device__ inline void fold_ghost(T* a, size_t s, size_t d) {
+  a[s] = (T)((double)a[s] + (double)a[d]);
+  a[d] = T(0);
+}
+template <typename T>
+__global__ void k_halo_T_y(Grid g, T* __restrict__ a, int kind) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   const int i0 = (kind == 0) ? -g.nh : 1 - g.nh, i1 = g.nx + g.nh;
   const int ni = i1 - i0 + 1;
@@ -161,56 +176,61 @@ __global__ void k_halo_T_y(Grid g, double* __restrict__ a, int kind) {
   const int ny = g.ny, nh = g.nh;
   // dst rows: ny+1+k (from 1+k) and 1-nh+k (from ny-nh+1+k); v rows are offset by the 0 face
   const int top = (kind == 1) ? ny + 1 + k : ny + 1 + k;
-  size_t s = idx_kind(g, kind, i, 1 + k), d = idx_kind(g, kind, i, top);
-  a[s] += a[d]; a[d] = 0.0;
-  s = idx_kind(g, kind, i, ny - nh + 1 + k); d = idx_kind(g, kind, i, 1 - nh + k);
-  a[s] += a[d]; a[d] = 0.0;
+  fold_ghost(a, idx_kind(g, kind, i, 1 + k), idx_kind(g, kind, i, top));
+  fold_ghost(a, idx_kind(g, kind, i, ny - nh + 1 + k), idx_kind(g, kind, i, 1 - nh + k));
 }
-__global__ void k_halo_T_x(Grid g, double* __restrict__ a, int kind) {
+template <typename T>
+__global__ void k_halo_T_x(Grid g, T* __restrict__ a, int kind) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   const int j0 = (kind == 1) ? -g.nh : 1 - g.nh, j1 = g.ny + g.nh;
   const int nj = j1 - j0 + 1;
   if (t >= nj * g.nh) return;
   const int j = j0 + t % nj, k = t / nj;
   const int nx = g.nx, nh = g.nh;
-  size_t s = idx_kind(g, kind, 1 + k, j), d = idx_kind(g, kind, nx + 1 + k, j);
-  a[s] += a[d]; a[d] = 0.0;
-  s = idx_kind(g, kind, nx - nh + 1 + k, j); d = idx_kind(g, kind, 1 - nh + k, j);
-  a[s] += a[d]; a[d] = 0.0;
+  fold_ghost(a, idx_kind(g, kind, 1 + k, j), idx_kind(g, kind, nx + 1 + k, j));
+  fold_ghost(a, idx_kind(g, kind, nx - nh + 1 + k, j), idx_kind(g, kind, 1 - nh + k, j));
 }
 
 // B^T: the velocity-BC statements in reverse order; y walls first (one thread per column),
 // then x walls (one thread per row).
-__global__ void k_bcs_T_y(Grid g, double* __restrict__ u, double* __restrict__ v) {
+// (a mirrored ghost folded onto its source with the mirror's sign: s -= d in FP64, then d = 0)
+template <typename T>
+__device__ inline void fold_mirror(T* a, size_t s, size_t d) {
+  a[s] = (T)((double)a[s] - (double)a[d]);
+  a[d] = T(0);
+}
+template <typename T>
+__global__ void k_bcs_T_y(Grid g, T* __restrict__ u, T* __restrict__ v) {
   const int i = -g.nh + (int)(blockIdx.x * blockDim.x + threadIdx.x);
   if (i > g.nx + g.nh) return;
   const int ny = g.ny, nh = g.nh;
   const bool iv = (i >= 1 - nh);
-  for (int j = 1; j <= nh; ++j) { u[IU(g, i, ny - j + 1)] -= u[IU(g, i, ny + j)]; u[IU(g, i, ny + j)] = 0.0; }
+  for (int j = 1; j <= nh; ++j) fold_mirror(u, IU(g, i, ny - j + 1), IU(g, i, ny + j));
   if (iv) {
-    for (int j = 1; j <= nh; ++j) { v[IV(g, i, ny - j)] -= v[IV(g, i, ny + j)]; v[IV(g, i, ny + j)] = 0.0; }
-    v[IV(g, i, ny)] = 0.0;
+    for (int j = 1; j <= nh; ++j) fold_mirror(v, IV(g, i, ny - j), IV(g, i, ny + j));
+    v[IV(g, i, ny)] = T(0);
   }
-  for (int j = 1; j <= nh; ++j) { u[IU(g, i, j)] -= u[IU(g, i, 1 - j)]; u[IU(g, i, 1 - j)] = 0.0; }
+  for (int j = 1; j <= nh; ++j) fold_mirror(u, IU(g, i, j), IU(g, i, 1 - j));
   if (iv) {
-    for (int j = 1; j <= nh; ++j) { v[IV(g, i, j)] -= v[IV(g, i, -j)]; v[IV(g, i, -j)] = 0.0; }
-    v[IV(g, i, 0)] = 0.0;
+    for (int j = 1; j <= nh; ++j) fold_mirror(v, IV(g, i, j), IV(g, i, -j));
+    v[IV(g, i, 0)] = T(0);
   }
 }
-__global__ void k_bcs_T_x(Grid g, double* __restrict__ u, double* __restrict__ v) {
+template <typename T>
+__global__ void k_bcs_T_x(Grid g, T* __restrict__ u, T* __restrict__ v) {
   const int j = -g.nh + (int)(blockIdx.x * blockDim.x + threadIdx.x);
   if (j > g.ny + g.nh) return;
   const int nx = g.nx, nh = g.nh;
   const bool ju = (j >= 1 - nh);
-  for (int i = 1; i <= nh; ++i) { v[IV(g, nx - i + 1, j)] -= v[IV(g, nx + i, j)]; v[IV(g, nx + i, j)] = 0.0; }
+  for (int i = 1; i <= nh; ++i) fold_mirror(v, IV(g, nx - i + 1, j), IV(g, nx + i, j));
   if (ju) {
-    for (int i = 1; i <= nh; ++i) { u[IU(g, nx - i, j)] -= u[IU(g, nx + i, j)]; u[IU(g, nx + i, j)] = 0.0; }
-    u[IU(g, nx, j)] = 0.0;
+    for (int i = 1; i <= nh; ++i) fold_mirror(u, IU(g, nx - i, j), IU(g, nx + i, j));
+    u[IU(g, nx, j)] = T(0);
   }
-  for (int i = 1; i <= nh; ++i) { v[IV(g, i, j)] -= v[IV(g, 1 - i, j)]; v[IV(g, 1 - i, j)] = 0.0; }
+  for (int i = 1; i <= nh; ++i) fold_mirror(v, IV(g, i, j), IV(g, 1 - i, j));
   if (ju) {
-    for (int i = 1; i <= nh; ++i) { u[IU(g, i, j)] -= u[IU(g, -i, j)]; u[IU(g, -i, j)] = 0.0; }
-    u[IU(g, 0, j)] = 0.0;
+    for (int i = 1; i <= nh; ++i) fold_mirror(u, IU(g, i, j), IU(g, -i, j));
+    u[IU(g, 0, j)] = T(0);
   }
 }
 
@@ -221,10 +241,11 @@ __device__ inline int wrapi(int i, int n, int per) { return per ? (i < 1 ? i + n
 
 // One transposed damped-Jacobi sweep of the u block: xb (adjoint of the sweep's output) ->
 // hb (adjoint of its input; rows and wall faces only) and rhs_b += the rhs adjoint.
-__global__ void __launch_bounds__(NT) k_jacobi_u_T(Grid g, const double* __restrict__ xb, double* __restrict__ hb,
-                                                   double* __restrict__ rb, const double* __restrict__ rho_u,
-                                                   const double* __restrict__ kap, const double* __restrict__ zeta,
-                                                   const double* __restrict__ eta, double dt) {
+template <typename T>
+__global__ void __launch_bounds__(NT) k_jacobi_u_T(Grid g, const T* __restrict__ xb, T* __restrict__ hb,
+                                                   T* __restrict__ rb, const T* __restrict__ rho_u,
+                                                   const T* __restrict__ kap, const T* __restrict__ zeta,
+                                                   const T* __restrict__ eta, double dt) {
   int i, j;
   union_ij(g, i, j);
   if (!in_u(g, i, j)) return;
@@ -233,12 +254,12 @@ __global__ void __launch_bounds__(NT) k_jacobi_u_T(Grid g, const double* __restr
   const double ix2 = g.inv_dx2, iy2 = g.inv_dy2, omega = 0.8;
   auto row = [&](int ii, int jj) { return jj >= 1 && jj <= ny && ii >= 1 && ii <= i_u_max; };
   auto s_of = [&](int ii, int jj, double& cE, double& cW, double& cN, double& cS) -> double {
-    cE = kap[IP(g, ii + 1, jj)] * ix2;
-    cW = kap[IP(g, ii, jj)] * ix2;
-    cN = (zeta[IW(g, ii, jj)] - eta[IW(g, ii, jj)]) * iy2;
-    cS = (zeta[IW(g, ii, jj - 1)] - eta[IW(g, ii, jj - 1)]) * iy2;
-    const double diag = (rho_u[IU(g, ii, jj)] / dt) + cE + cW + cN + cS;
-    return omega * xb[IU(g, ii, jj)] / diag;
+    cE = (double)kap[IP(g, ii + 1, jj)] * ix2;
+    cW = (double)kap[IP(g, ii, jj)] * ix2;
+    cN = ((double)zeta[IW(g, ii, jj)] - (double)eta[IW(g, ii, jj)]) * iy2;
+    cS = ((double)zeta[IW(g, ii, jj - 1)] - (double)eta[IW(g, ii, jj - 1)]) * iy2;
+    const double diag = ((double)rho_u[IU(g, ii, jj)] / dt) + cE + cW + cN + cS;
+    return omega * (double)xb[IU(g, ii, jj)] / diag;
   };
   const bool is_row = row(i, j);
   const bool wall = !g.per_x && j >= 1 && j <= ny && (i == 0 || i == nx);
@@ -246,10 +267,10 @@ __global__ void __launch_bounds__(NT) k_jacobi_u_T(Grid g, const double* __restr
   if (is_row) {
     double cE, cW, cN, cS;
     const double s = s_of(i, j, cE, cW, cN, cS);
-    rb[IU(g, i, j)] += s;
-    h = (1.0 - omega) * xb[IU(g, i, j)];
+    rb[IU(g, i, j)] = (T)((double)rb[IU(g, i, j)] + s);
+    h = (1.0 - omega) * (double)xb[IU(g, i, j)];
   } else if (wall) {
-    rb[IU(g, i, j)] += xb[IU(g, i, j)];
+    rb[IU(g, i, j)] = (T)((double)rb[IU(g, i, j)] + (double)xb[IU(g, i, j)]);
   }
   if (is_row || wall) {
     double cE, cW, cN, cS;
@@ -262,13 +283,14 @@ __global__ void __launch_bounds__(NT) k_jacobi_u_T(Grid g, const double* __restr
     rj = wrapi(j + 1, ny, g.per_y);
     if (row(ri, rj)) { const double s = s_of(ri, rj, cE, cW, cN, cS); h += cS * s; }
   }
-  hb[IU(g, i, j)] = h;
+  hb[IU(g, i, j)] = (T)h;
 }
 
-__global__ void __launch_bounds__(NT) k_jacobi_v_T(Grid g, const double* __restrict__ xb, double* __restrict__ hb,
-                                                   double* __restrict__ rb, const double* __restrict__ rho_v,
-                                                   const double* __restrict__ kap, const double* __restrict__ zeta,
-                                                   const double* __restrict__ eta, double dt) {
+template <typename T>
+__global__ void __launch_bounds__(NT) k_jacobi_v_T(Grid g, const T* __restrict__ xb, T* __restrict__ hb,
+                                                   T* __restrict__ rb, const T* __restrict__ rho_v,
+                                                   const T* __restrict__ kap, const T* __restrict__ zeta,
+                                                   const T* __restrict__ eta, double dt) {
   int i, j;
   union_ij(g, i, j);
   if (!in_v(g, i, j)) return;
@@ -277,12 +299,12 @@ __global__ void __launch_bounds__(NT) k_jacobi_v_T(Grid g, const double* __restr
   const double ix2 = g.inv_dx2, iy2 = g.inv_dy2, omega = 0.8;
   auto row = [&](int ii, int jj) { return jj >= 1 && jj <= j_v_max && ii >= 1 && ii <= nx; };
   auto s_of = [&](int ii, int jj, double& cE, double& cW, double& cN, double& cS) -> double {
-    cE = (zeta[IW(g, ii, jj)] - eta[IW(g, ii, jj)]) * ix2;
-    cW = (zeta[IW(g, ii - 1, jj)] - eta[IW(g, ii - 1, jj)]) * ix2;
-    cN = kap[IP(g, ii, jj + 1)] * iy2;
-    cS = kap[IP(g, ii, jj)] * iy2;
-    const double diag = (rho_v[IV(g, ii, jj)] / dt) + cE + cW + cN + cS;
-    return omega * xb[IV(g, ii, jj)] / diag;
+    cE = ((double)zeta[IW(g, ii, jj)] - (double)eta[IW(g, ii, jj)]) * ix2;
+    cW = ((double)zeta[IW(g, ii - 1, jj)] - (double)eta[IW(g, ii - 1, jj)]) * ix2;
+    cN = (double)kap[IP(g, ii, jj + 1)] * iy2;
+    cS = (double)kap[IP(g, ii, jj)] * iy2;
+    const double diag = ((double)rho_v[IV(g, ii, jj)] / dt) + cE + cW + cN + cS;
+    return omega * (double)xb[IV(g, ii, jj)] / diag;
   };
   const bool is_row = row(i, j);
   const bool wall = !g.per_y && i >= 1 && i <= nx && (j == 0 || j == ny);
@@ -290,10 +312,10 @@ __global__ void __launch_bounds__(NT) k_jacobi_v_T(Grid g, const double* __restr
   if (is_row) {
     double cE, cW, cN, cS;
     const double s = s_of(i, j, cE, cW, cN, cS);
-    rb[IV(g, i, j)] += s;
-    h = (1.0 - omega) * xb[IV(g, i, j)];
+    rb[IV(g, i, j)] = (T)((double)rb[IV(g, i, j)] + s);
+    h = (1.0 - omega) * (double)xb[IV(g, i, j)];
   } else if (wall) {
-    rb[IV(g, i, j)] += xb[IV(g, i, j)];
+    rb[IV(g, i, j)] = (T)((double)rb[IV(g, i, j)] + (double)xb[IV(g, i, j)]);
   }
   if (is_row || wall) {
     double cE, cW, cN, cS;
@@ -306,72 +328,76 @@ __global__ void __launch_bounds__(NT) k_jacobi_v_T(Grid g, const double* __restr
     rj = wrapi(j + 1, ny, g.per_y);
     if (row(ri, rj)) { const double s = s_of(ri, rj, cE, cW, cN, cS); h += cS * s; }
   }
-  hb[IV(g, i, j)] = h;
+  hb[IV(g, i, j)] = (T)h;
 }
 
 // rv_b += (d(-cross)/d rv)^T zuw_b: the u block's eta cross term (rows i=1..nx-1, j=1..ny;
 // rv(., 0) wraps to rv(., ny) when periodic in y).
-__global__ void __launch_bounds__(NT) k_cross_T(Grid g, const double* __restrict__ zuwb, const double* __restrict__ eta,
-                                                double* __restrict__ rvb) {
+template <typename T>
+__global__ void __launch_bounds__(NT) k_cross_T(Grid g, const T* __restrict__ zuwb, const T* __restrict__ eta,
+                                                T* __restrict__ rvb) {
   int a, b;
   union_ij(g, a, b);
   if (!in_v(g, a, b)) return;
   const int nx = g.nx, ny = g.ny;
   const double f = g.inv_dy * g.inv_dx;
   auto row = [&](int i, int j) { return i >= 1 && i <= nx - 1 && j >= 1 && j <= ny; };
-  auto ZB = [&](int i, int j) -> double { return row(i, j) ? zuwb[IU(g, i, j)] : 0.0; };
+  auto ZB = [&](int i, int j) -> double { return row(i, j) ? (double)zuwb[IU(g, i, j)] : 0.0; };
+  auto ET = [&](int i, int j) -> double { return (double)eta[IW(g, i, j)]; };
   double o = 0.0;
   // references of (a, b) as rv(i+1, j) and rv(i, j) of row (i, j) = (a-1, b), (a, b)
   if (b >= 1 && b <= ny) {
-    o += ZB(a - 1, b) * f * eta[IW(g, a - 1, b)];
-    o -= ZB(a, b) * f * eta[IW(g, a, b)];
+    o += ZB(a - 1, b) * f * ET(a - 1, b);
+    o -= ZB(a, b) * f * ET(a, b);
   }
   // references as rv(i+1, j-1) and rv(i, j-1) of row (i, j) with j-1 = b (or wrapped)
   int jr = b + 1;
   if (g.per_y && b == ny) jr = 1;   // row j = 1 reads rv(., 0) = rv(., ny)
   if (jr >= 1 && jr <= ny && (b >= 0)) {
     const int bb = jr - 1;           // the vertex row of the forward term
-    o -= ZB(a - 1, jr) * f * eta[IW(g, a - 1, bb)];
-    o += ZB(a, jr) * f * eta[IW(g, a, bb)];
+    o -= ZB(a - 1, jr) * f * ET(a - 1, bb);
+    o += ZB(a, jr) * f * ET(a, bb);
   }
   if (g.per_y && b == 0) o = 0.0;    // v(., 0) is the image of v(., ny): handled there
-  rvb[IV(g, a, b)] += o;
+  rvb[IV(g, a, b)] = (T)((double)rvb[IV(g, a, b)] + o);
 }
 
 // z_b (u, v parts) and rp_b from the adjoints of z_uw, z_vw (src/mod_pcd_precond.f90:68-84);
 // rp_b starts from yp_b, the adjoint of the output r_p itself.
-__global__ void __launch_bounds__(NT) k_zwork_T(Grid g, const double* __restrict__ zuwb, const double* __restrict__ zvwb,
-                                                const double* __restrict__ ypb, double* __restrict__ zub,
-                                                double* __restrict__ zvb, double* __restrict__ rpb) {
+template <typename T>
+__global__ void __launch_bounds__(NT) k_zwork_T(Grid g, const T* __restrict__ zuwb, const T* __restrict__ zvwb,
+                                                const T* __restrict__ ypb, T* __restrict__ zub,
+                                                T* __restrict__ zvb, T* __restrict__ rpb) {
   int i, j;
   union_ij(g, i, j);
   const int nx = g.nx, ny = g.ny;
-  if (in_u(g, i, j)) zub[IU(g, i, j)] = (j >= 1 && j <= ny && i >= 0 && i <= nx) ? zuwb[IU(g, i, j)] : 0.0;
-  if (in_v(g, i, j)) zvb[IV(g, i, j)] = (i >= 1 && i <= nx && j >= 0 && j <= ny) ? zvwb[IV(g, i, j)] : 0.0;
+  if (in_u(g, i, j)) zub[IU(g, i, j)] = (j >= 1 && j <= ny && i >= 0 && i <= nx) ? zuwb[IU(g, i, j)] : T(0);
+  if (in_v(g, i, j)) zvb[IV(g, i, j)] = (i >= 1 && i <= nx && j >= 0 && j <= ny) ? zvwb[IV(g, i, j)] : T(0);
   if (own_p(g, i, j)) {
-    double o = ypb[IP(g, i, j)];
-    if (i - 1 >= 1 && i - 1 <= nx - 1) o -= zuwb[IU(g, i - 1, j)] * g.inv_dx;
-    if (i >= 1 && i <= nx - 1) o += zuwb[IU(g, i, j)] * g.inv_dx;
-    if (j - 1 >= 1 && j - 1 <= ny - 1) o -= zvwb[IV(g, i, j - 1)] * g.inv_dy;
-    if (j >= 1 && j <= ny - 1) o += zvwb[IV(g, i, j)] * g.inv_dy;
-    rpb[IP(g, i, j)] = o;
+    double o = (double)ypb[IP(g, i, j)];
+    if (i - 1 >= 1 && i - 1 <= nx - 1) o -= (double)zuwb[IU(g, i - 1, j)] * g.inv_dx;
+    if (i >= 1 && i <= nx - 1) o += (double)zuwb[IU(g, i, j)] * g.inv_dx;
+    if (j - 1 >= 1 && j - 1 <= ny - 1) o -= (double)zvwb[IV(g, i, j - 1)] * g.inv_dy;
+    if (j >= 1 && j <= ny - 1) o += (double)zvwb[IV(g, i, j)] * g.inv_dy;
+    rpb[IP(g, i, j)] = (T)o;
   }
 }
 
 // phi_b from rp_b (the transposed convection + 1/dt term, neighbours wrap when periodic) and
 // z_p,b += -visc kappa rp_b.
-__global__ void __launch_bounds__(NT) k_rp_T(Grid g, const double* __restrict__ rpb, const double* __restrict__ uadv,
-                                             const double* __restrict__ vadv, double dt, const double* __restrict__ kap,
-                                             double visc, double* __restrict__ phib, double* __restrict__ zpb) {
+template <typename T>
+__global__ void __launch_bounds__(NT) k_rp_T(Grid g, const T* __restrict__ rpb, const T* __restrict__ uadv,
+                                             const T* __restrict__ vadv, double dt, const T* __restrict__ kap,
+                                             double visc, T* __restrict__ phib, T* __restrict__ zpb) {
   int a, b;
   union_ij(g, a, b);
   if (!in_p(g, a, b)) return;
-  if (!own_p(g, a, b)) { phib[IP(g, a, b)] = 0.0; return; }
+  if (!own_p(g, a, b)) { phib[IP(g, a, b)] = T(0); return; }
   const int nx = g.nx, ny = g.ny;
   const double half = 0.5;
-  auto RB = [&](int i, int j) -> double { return rpb[IP(g, i, j)]; };
-  auto uavg = [&](int i, int j) { return half * (uadv[IU(g, i, j)] + uadv[IU(g, i - 1, j)]); };
-  auto vavg = [&](int i, int j) { return half * (vadv[IV(g, i, j)] + vadv[IV(g, i, j - 1)]); };
+  auto RB = [&](int i, int j) -> double { return (double)rpb[IP(g, i, j)]; };
+  auto uavg = [&](int i, int j) { return half * ((double)uadv[IU(g, i, j)] + (double)uadv[IU(g, i - 1, j)]); };
+  auto vavg = [&](int i, int j) { return half * ((double)vadv[IV(g, i, j)] + (double)vadv[IV(g, i, j - 1)]); };
   double o = -RB(a, b) / dt;
   // rows whose east / west / north / south neighbour is (a, b)
   int r = a - 1;
@@ -386,8 +412,8 @@ __global__ void __launch_bounds__(NT) k_rp_T(Grid g, const double* __restrict__ 
   r = b + 1;
   if (r > ny && g.per_y) r -= ny;
   if (r <= ny) o += RB(a, r) * vavg(a, r) * (half * g.inv_dy);
-  phib[IP(g, a, b)] = o;
-  if (visc != 0.0) zpb[IP(g, a, b)] -= visc * kap[IP(g, a, b)] * RB(a, b);
+  phib[IP(g, a, b)] = (T)o;
+  if (visc != 0.0) zpb[IP(g, a, b)] = (T)((double)zpb[IP(g, a, b)] - visc * (double)kap[IP(g, a, b)] * RB(a, b));
 }
 
 // ---------------------------------------------------------------------------------
@@ -613,11 +639,12 @@ __global__ void k_acc_prod(double* __restrict__ o, double c, const double* __res
 }
 
 // zp(i, j) += phi(i, j) - sc[slot] over the interior (the projected pressure block).
-__global__ void __launch_bounds__(NT) k_add_centred(Grid g, double* __restrict__ zp, const double* __restrict__ phi,
+template <typename T>
+__global__ void __launch_bounds__(NT) k_add_centred(Grid g, T* __restrict__ zp, const T* __restrict__ phi,
                                                     const double* __restrict__ sc, int slot) {
   int i, j;
   union_ij(g, i, j);
-  if (own_p(g, i, j)) zp[IP(g, i, j)] += phi[IP(g, i, j)] - sc[slot];
+  if (own_p(g, i, j)) zp[IP(g, i, j)] = (T)((double)zp[IP(g, i, j)] + ((double)phi[IP(g, i, j)] - sc[slot]));
 }
 
 // Objective band masks: owned u faces with (j - 1/2) dy >= y0, owned v faces with j dy >= y0.

@@ -269,6 +269,14 @@ struct fugu_ctx {
     double *gmu = nullptr, *grho = nullptr;
     bool ready = false;
   } aw;
+  // the transposed PCD's scratch, per storage type of the Krylov vectors
+  template <typename T>
+  struct PcdTBuf {
+    T *y = nullptr, *hu0 = nullptr, *hu1 = nullptr, *zuwb = nullptr, *rvb = nullptr, *hv0 = nullptr;
+    T *hv1 = nullptr, *zvwb = nullptr, *rpb = nullptr, *phib = nullptr, *phi = nullptr;
+  };
+  PcdTBuf<double> ptd;
+  PcdTBuf<float> ptf;
   // scratch of the interface-chain transposes (VOF, SVOF, curvature, CSF)
   struct IfaceWork {
     double *cb = nullptr, *tb = nullptr, *kb = nullptr, *sb0 = nullptr, *sb1 = nullptr, *acc = nullptr;
@@ -1498,25 +1506,27 @@ void acc_prod(fugu_ctx* c, double* o, double s, const double* w, const double* x
 }
 
 // H^T of one padded field (y pass undone first, then x).
-void halo_T(fugu_ctx* c, double* a, int kind) {
+template <typename T>
+void halo_T(fugu_ctx* c, T* a, int kind) {
   const Grid& g = c->g;
   if (g.per_y) {
     const int ni = (kind == 0) ? g.nx + 1 + 2 * g.nh : g.nx + 2 * g.nh;
-    LAUNCH(k_halo_T_y, dim3(nblk((size_t)ni * g.nh)), dim3(256), c->s, g, a, kind);
+    LAUNCH((k_halo_T_y<T>), dim3(nblk((size_t)ni * g.nh)), dim3(256), c->s, g, a, kind);
   }
   if (g.per_x) {
     const int nj = (kind == 1) ? g.ny + 1 + 2 * g.nh : g.ny + 2 * g.nh;
-    LAUNCH(k_halo_T_x, dim3(nblk((size_t)nj * g.nh)), dim3(256), c->s, g, a, kind);
+    LAUNCH((k_halo_T_x<T>), dim3(nblk((size_t)nj * g.nh)), dim3(256), c->s, g, a, kind);
   }
 }
 // E^T of a triplet: H^T, then B^T (y walls, then x walls), then the owned-set mask.
-void extend_T(fugu_ctx* c, double* X) {
+template <typename T>
+void extend_T(fugu_ctx* c, T* X) {
   const Grid& g = c->g;
-  double *u = X, *v = X + g.nu, *p = X + g.nu + g.nv;
+  T *u = X, *v = X + g.nu, *p = X + g.nu + g.nv;
   halo_T(c, u, 0); halo_T(c, v, 1); halo_T(c, p, 2);
-  if (!g.per_y) LAUNCH(k_bcs_T_y, dim3(nblk((size_t)g.nx + 1 + 2 * g.nh)), dim3(256), c->s, g, u, v);
-  if (!g.per_x) LAUNCH(k_bcs_T_x, dim3(nblk((size_t)g.ny + 1 + 2 * g.nh)), dim3(256), c->s, g, u, v);
-  LAUNCH(k_mask_trip, grid_union(g), dim3(TX, TY), c->s, g, u, v, p);
+  if (!g.per_y) LAUNCH((k_bcs_T_y<T>), dim3(nblk((size_t)g.nx + 1 + 2 * g.nh)), dim3(256), c->s, g, u, v);
+  if (!g.per_x) LAUNCH((k_bcs_T_x<T>), dim3(nblk((size_t)g.ny + 1 + 2 * g.nh)), dim3(256), c->s, g, u, v);
+  LAUNCH((k_mask_trip<T>), grid_union(g), dim3(TX, TY), c->s, g, u, v, p);
 }
 // ---------------------------------------------------------------------------------
 // Transposes of the interface chain (no counterpart in the reference): the reverse mode of
@@ -1585,74 +1595,112 @@ void csf_T(fugu_ctx* c, const double* craw, const double* kc, double sigma, cons
   LAUNCH(k_csf_T, grid_union(c->g), dim3(TX, TY), c->s, c->g, craw, kc, sigma, fub, fvb, cbar, kbar);
 }
 
-void mask_trip(fugu_ctx* c, double* X) {
+template <typename T>
+void mask_trip(fugu_ctx* c, T* X) {
   const Grid& g = c->g;
-  LAUNCH(k_mask_trip, grid_union(g), dim3(TX, TY), c->s, g, X, X + g.nu, X + g.nu + g.nv);
+  LAUNCH((k_mask_trip<T>), grid_union(g), dim3(TX, TY), c->s, g, X, X + g.nu, X + g.nu + g.nv);
+}
+template <typename T>
+void zero_t(fugu_ctx* c, T* a, size_t n) {
+  HIPCHK(hipMemsetAsync(a, 0, n * sizeof(T), c->s));
 }
 
-// O = A_eff^T L with the context's FP64 operator fields (c->wd bindings).
-void matvec_T(fugu_ctx* c, const double* L, double* O, double dt, int adv) {
+// O = A_eff^T L with w's operator fields (storage T, FP64 arithmetic).
+template <typename T>
+void matvec_T_w(fugu_ctx* c, Work<T>& w, const T* L, T* O, double dt, int adv) {
   const Grid& g = c->g;
-  Work<double>& w = c->wd;
-  MatvecTArgs a;
+  MatvecTArgs<T> a;
   a.Lu = L; a.Lv = L + g.nu; a.Lp = L + g.nu + g.nv;
   a.Ou = O; a.Ov = O + g.nu; a.Op = O + g.nu + g.nv;
   a.uadv = w.uadv; a.vadv = w.vadv; a.rho_c = w.rho_c; a.rho_u = w.rho_u; a.rho_v = w.rho_v;
   a.kap = w.kap; a.zeta = w.zeta; a.eta = w.eta; a.dt = dt; a.adv = adv;
-  TLAUNCH(c, "matvec_T", 13.0 * sizeof(double) * g.nx * g.ny, k_matvec_T, grid_union(g), dim3(TX, TY), c->s, g, a);
+  TLAUNCH(c, sizeof(T) == 8 ? "matvec_T" : "matvec_T_f32", 13.0 * sizeof(T) * g.nx * g.ny, (k_matvec_T<T>),
+          grid_union(g), dim3(TX, TY), c->s, g, a);
   extend_T(c, O);
 }
+// O = A_eff^T L with the context's FP64 operator fields (c->wd bindings).
+void matvec_T(fugu_ctx* c, const double* L, double* O, double dt, int adv) { matvec_T_w<double>(c, c->wd, L, O, dt, adv); }
 
-// Z = P_L^T Y: the transposed linearised PCD.  Pressure block: ptol > 0 -> projected PCG to
-// ptol (pmax iterations at most); ptol <= 0 -> c->mg_cycles multigrid V-cycles (needs mg_setup).
-void pcd_T(fugu_ctx* c, const double* Y, double* Z, double dt, double ptol, int pmax, double visc) {
-  const Grid& g = c->g;
-  Work<double>& w = c->wd;
+// The transposed PCD's scratch in the storage type of its Krylov vectors (FP64: the adjoint
+// work arrays; FP32: allocated on first use).
+template <typename T>
+fugu_ctx::PcdTBuf<T>& pcd_t_buf(fugu_ctx* c);
+template <>
+fugu_ctx::PcdTBuf<double>& pcd_t_buf<double>(fugu_ctx* c) {
+  auto& b = c->ptd;
   auto& a = c->aw;
+  b.y = a.y; b.hu0 = a.hu0; b.hu1 = a.hu1; b.zuwb = a.zuwb; b.rvb = a.rvb; b.hv0 = a.hv0; b.hv1 = a.hv1;
+  b.zvwb = a.zvwb; b.rpb = a.rpb; b.phib = a.phib; b.phi = a.phi;
+  return b;
+}
+template <>
+fugu_ctx::PcdTBuf<float>& pcd_t_buf<float>(fugu_ctx* c) {
+  auto& b = c->ptf;
+  if (!b.y) {
+    const Grid& g = c->g;
+    b.y = dalloc<float>(c, c->ntrip);
+    b.hu0 = dalloc<float>(c, g.nu); b.hu1 = dalloc<float>(c, g.nu); b.zuwb = dalloc<float>(c, g.nu);
+    b.rvb = dalloc<float>(c, g.nv); b.hv0 = dalloc<float>(c, g.nv); b.hv1 = dalloc<float>(c, g.nv);
+    b.zvwb = dalloc<float>(c, g.nv); b.rpb = dalloc<float>(c, g.np); b.phib = dalloc<float>(c, g.np);
+    b.phi = dalloc<float>(c, g.np);
+  }
+  return b;
+}
+
+// Z = P_L^T Y: the transposed linearised PCD (w's operator fields and hierarchies, storage T).
+// Pressure block: ptol > 0 -> projected PCG to ptol (pmax iterations at most); ptol <= 0 ->
+// c->mg_cycles multigrid V-cycles (needs mg_setup).
+template <typename T>
+void pcd_T_w(fugu_ctx* c, Work<T>& w, const T* Y, T* Z, double dt, double ptol, int pmax, double visc) {
+  const Grid& g = c->g;
+  auto& a = pcd_t_buf<T>(c);
   const dim3 gu = grid_union(g), bl(TX, TY);
-  HIPCHK(hipMemcpyAsync(a.y, Y, c->ntrip * sizeof(double), hipMemcpyDeviceToDevice, c->s));
+  HIPCHK(hipMemcpyAsync(a.y, Y, c->ntrip * sizeof(T), hipMemcpyDeviceToDevice, c->s));
   mask_trip(c, a.y);
-  const double *yu = a.y, *yv = a.y + g.nu;
-  double *Zu = Z, *Zv = Z + g.nu, *Zp = Z + g.nu + g.nv;
+  const T *yu = a.y, *yv = a.y + g.nu;
+  T *Zu = Z, *Zv = Z + g.nu, *Zp = Z + g.nu + g.nv;
   // r_u = J_u^3(z_uw): three transposed sweeps, z_uw,b accumulated (velocity multigrid: the
   // symmetric V-cycle of the symmetric u block stands for its own transpose)
   if (c->use_vmg) {
-    vmg_solve<double>(c, w, 0, yu, a.zuwb);
+    vmg_solve<T>(c, w, 0, yu, a.zuwb);
   } else {
-    zero(c, a.zuwb, g.nu);
-    LAUNCH(k_jacobi_u_T, gu, bl, c->s, g, yu, a.hu0, a.zuwb, w.rho_u, w.kap, w.zeta, w.eta, dt);
-    LAUNCH(k_jacobi_u_T, gu, bl, c->s, g, (const double*)a.hu0, a.hu1, a.zuwb, w.rho_u, w.kap, w.zeta, w.eta, dt);
-    LAUNCH(k_jacobi_u_T, gu, bl, c->s, g, (const double*)a.hu1, a.hu0, a.zuwb, w.rho_u, w.kap, w.zeta, w.eta, dt);
+    zero_t(c, a.zuwb, g.nu);
+    LAUNCH((k_jacobi_u_T<T>), gu, bl, c->s, g, yu, a.hu0, a.zuwb, w.rho_u, w.kap, w.zeta, w.eta, dt);
+    LAUNCH((k_jacobi_u_T<T>), gu, bl, c->s, g, (const T*)a.hu0, a.hu1, a.zuwb, w.rho_u, w.kap, w.zeta, w.eta, dt);
+    LAUNCH((k_jacobi_u_T<T>), gu, bl, c->s, g, (const T*)a.hu1, a.hu0, a.zuwb, w.rho_u, w.kap, w.zeta, w.eta, dt);
   }
   // r_v from the output and from the u block's eta cross term
-  HIPCHK(hipMemcpyAsync(a.rvb, yv, g.nv * sizeof(double), hipMemcpyDeviceToDevice, c->s));
-  LAUNCH(k_cross_T, gu, bl, c->s, g, (const double*)a.zuwb, w.eta, a.rvb);
+  HIPCHK(hipMemcpyAsync(a.rvb, yv, g.nv * sizeof(T), hipMemcpyDeviceToDevice, c->s));
+  LAUNCH((k_cross_T<T>), gu, bl, c->s, g, (const T*)a.zuwb, w.eta, a.rvb);
   if (c->use_vmg) {
-    vmg_solve<double>(c, w, 1, (const double*)a.rvb, a.zvwb);
+    vmg_solve<T>(c, w, 1, (const T*)a.rvb, a.zvwb);
   } else {
-    zero(c, a.zvwb, g.nv);
-    LAUNCH(k_jacobi_v_T, gu, bl, c->s, g, (const double*)a.rvb, a.hv0, a.zvwb, w.rho_v, w.kap, w.zeta, w.eta, dt);
-    LAUNCH(k_jacobi_v_T, gu, bl, c->s, g, (const double*)a.hv0, a.hv1, a.zvwb, w.rho_v, w.kap, w.zeta, w.eta, dt);
-    LAUNCH(k_jacobi_v_T, gu, bl, c->s, g, (const double*)a.hv1, a.hv0, a.zvwb, w.rho_v, w.kap, w.zeta, w.eta, dt);
+    zero_t(c, a.zvwb, g.nv);
+    LAUNCH((k_jacobi_v_T<T>), gu, bl, c->s, g, (const T*)a.rvb, a.hv0, a.zvwb, w.rho_v, w.kap, w.zeta, w.eta, dt);
+    LAUNCH((k_jacobi_v_T<T>), gu, bl, c->s, g, (const T*)a.hv0, a.hv1, a.zvwb, w.rho_v, w.kap, w.zeta, w.eta, dt);
+    LAUNCH((k_jacobi_v_T<T>), gu, bl, c->s, g, (const T*)a.hv1, a.hv0, a.zvwb, w.rho_v, w.kap, w.zeta, w.eta, dt);
   }
   // z_u, z_v and r_p,b from z_uw,b and z_vw,b
-  LAUNCH(k_zwork_T, gu, bl, c->s, g, (const double*)a.zuwb, (const double*)a.zvwb,
-         (const double*)(a.y + g.nu + g.nv), Zu, Zv, a.rpb);
-  zero(c, Zp, g.np);
-  LAUNCH(k_rp_T, gu, bl, c->s, g, (const double*)a.rpb, w.uadv, w.vadv, dt, w.kap, visc, a.phib, Zp);
+  LAUNCH((k_zwork_T<T>), gu, bl, c->s, g, (const T*)a.zuwb, (const T*)a.zvwb, (const T*)(a.y + g.nu + g.nv), Zu,
+         Zv, a.rpb);
+  zero_t(c, Zp, g.np);
+  LAUNCH((k_rp_T<T>), gu, bl, c->s, g, (const T*)a.rpb, w.uadv, w.vadv, dt, w.kap, visc, a.phib, Zp);
   // z_p += Pi A_p^+ Pi phi_b (symmetric), centred
-  zero(c, a.phi, g.np);
+  zero_t(c, a.phi, g.np);
   if (ptol > 0.0) {
     reset_flags(c);
     set_slot(c, S_P_TOL, ptol);
     pcg_setup(c, w);
-    pcg<double>(c, w, a.phi, (const double*)a.phib, true, ptol, pmax, true);
+    pcg<T>(c, w, a.phi, (const T*)a.phib, true, ptol, pmax, true);
   } else {   // c->mg_cycles V-cycles from zero on the projected rhs (mg_setup done by the caller)
-    mg_pressure<double>(c, w, a.phi, (const double*)a.phib, true);
+    mg_pressure<T>(c, w, a.phi, (const T*)a.phib, true);
   }
-  cell_mean<double>(c, (const double*)a.phi, (int)S_P_MEAN, 0);
-  LAUNCH(k_add_centred, gu, bl, c->s, g, Zp, (const double*)a.phi, (const double*)c->sc, (int)S_P_MEAN);
+  cell_mean<T>(c, (const T*)a.phi, (int)S_P_MEAN, 0);
+  LAUNCH((k_add_centred<T>), gu, bl, c->s, g, Zp, (const T*)a.phi, (const double*)c->sc, (int)S_P_MEAN);
   mask_trip(c, Z);
+}
+void pcd_T(fugu_ctx* c, const double* Y, double* Z, double dt, double ptol, int pmax, double visc) {
+  pcd_T_w<double>(c, c->wd, Y, Z, dt, ptol, pmax, visc);
 }
 
 void fg_graphs_clear(fugu_ctx* c) {
@@ -1706,7 +1754,8 @@ void fg_graph_run(fugu_ctx* c, const fugu_ctx::FgKey& key, int k, F&& emit) {
 // directions, matvec operands and the preconditioner are FP32 triplets, the Gram-Schmidt
 // products, sums and the Hessenberg/Givens scalars FP64; the cycle's correction is added to
 // the FP64 X and the next restart's residual is FP64 again.  Flexibility absorbs the
-// preconditioner's inexactness (FP32 rounding, non-converged inner recurrences).
+// preconditioner's inexactness (FP32 rounding, non-converged inner recurrences).  The adjoint
+// (c->adjoint) runs the same cycles on the transposed operator and the transposed PCD.
 template <typename T>
 void solve_fgmres(fugu_ctx* c, Work<T>& w, double* X, double dt, double tol, double inner_tol, int max_steps,
                   int max_inner, int m, int mvflags, bool has_lid, double lid, int* iters, int* outer,
@@ -1714,7 +1763,7 @@ void solve_fgmres(fugu_ctx* c, Work<T>& w, double* X, double dt, double tol, dou
   TreeSums tree(c);
   Work<double>& wd = c->wd;
   const size_t n3 = c->ntrip;
-  const bool ir32 = sizeof(T) == sizeof(float) && max_inner == 0 && !c->adjoint;
+  const bool ir32 = sizeof(T) == sizeof(float) && max_inner == 0;
   if (m > FG_M - 1) m = FG_M - 1;
   if (m < 1) m = 1;
   while ((int)c->kv.size() < m + 1) c->kv.push_back(dalloc<double>(c, n3));
@@ -1863,10 +1912,18 @@ void solve_fgmres(fugu_ctx* c, Work<T>& w, double* X, double dt, double tol, dou
       auto step_k = [&]() {
       if (ir32) {
         float* z = c->kz32[k];
-        if (!pcd<float>(c, c->wf, (const float*)c->kv32[k], z, (float)dt, true)) bcs_trip(c, z);
-        // W = A z_k (FP32 operands; the matvec's halo exchange only rewrites z_k's periodic
-        // ghosts with the images the PCD already put there), then Gram-Schmidt on FP32 storage
-        matvec(c, c->wf, z, c->wf.z, dt, mvflags, nullptr);
+        if (c->adjoint) {
+          // the transposed PCD and operator on FP32 storage (FP64 arithmetic in the kernels)
+          pcd_T_w<float>(c, c->wf, (const float*)c->kv32[k], z, dt, -1.0, c->mg_cycles, c->pcd_visc);
+          bcs_trip(c, z);
+          HIPCHK(hipMemcpyAsync(c->wf.X, z, n3 * sizeof(float), hipMemcpyDeviceToDevice, c->s));
+          matvec_T_w<float>(c, c->wf, c->wf.X, c->wf.z, dt, (mvflags & MV_ADV) ? 1 : 0);
+        } else {
+          if (!pcd<float>(c, c->wf, (const float*)c->kv32[k], z, (float)dt, true)) bcs_trip(c, z);
+          // W = A z_k (FP32 operands; the matvec's halo exchange only rewrites z_k's periodic
+          // ghosts with the images the PCD already put there), then Gram-Schmidt on FP32 storage
+          matvec(c, c->wf, z, c->wf.z, dt, mvflags, nullptr);
+        }
         float* W = c->wf.z;
         dots_into(W, c->kv32.data(), k, 0, false);
         maxpy_dot(W, c->kv32.data(), k + 1, k);
@@ -2254,7 +2311,8 @@ int adjoint_run(fugu_ctx* c, const fugu_adjoint_opts* o, const double* mu_in, co
   LAUNCH(k_cell_props, grid_union(g), dim3(TX, TY), c->s, g, (const double*)a.mu, (const double*)a.rho, c->rho_c,
          c->rho_u, c->rho_v, c->kap, c->zeta, c->eta);
   bind_props_d(c, false, false);
-  if (mixed) work_alloc<float>(c, c->wf, true);
+  const bool mixed_adj = (o->flags & FUGU_ADJ_MIXED_ADJOINT) != 0;
+  if (mixed || mixed_adj) work_alloc<float>(c, c->wf, true);
   c->use_mg = true;
   c->use_vmg = (o->flags & FUGU_ADJ_PRECOND_VMG) != 0;
   c->pcd_visc = c->pcd_visc_coef;
@@ -2428,7 +2486,12 @@ int adjoint_run(fugu_ctx* c, const fugu_adjoint_opts* o, const double* mu_in, co
       zero(c, Ln, n3);
       int it = 0, outer = 0;
       double res = 0.0;
-      solve_fgmres<double>(c, wd, Ln, dte_of(n), tol, 0.0, mi, 0, rm, MV_ADV, false, 0.0, &it, &outer, &res);
+      if (mixed_adj) {
+        bind_props_f(c, false, false);
+        solve_fgmres<float>(c, c->wf, Ln, dte_of(n), tol, 0.0, mi, 0, rm, MV_ADV, false, 0.0, &it, &outer, &res);
+      } else {
+        solve_fgmres<double>(c, wd, Ln, dte_of(n), tol, 0.0, mi, 0, rm, MV_ADV, false, 0.0, &it, &outer, &res);
+      }
       adj_it += outer;
       adj_res = std::max(adj_res, res);
       mask_trip(c, Ln);

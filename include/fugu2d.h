@@ -363,14 +363,18 @@ enum {
    * (FUGU_PRECOND_VMG) */
   FUGU_ADJ_PRECOND_VMG = 1u << 2,
   /* forward solves by the mixed form (FP32 Arnoldi cycles, FP64 true residual and update at
-   * every restart: FUGU_SOLVE_MIXED); the adjoint solves stay FP64 */
+   * every restart: FUGU_SOLVE_MIXED); the adjoint solves stay FP64 unless FUGU_ADJ_MIXED_ADJOINT */
   FUGU_ADJ_MIXED_FORWARD = 1u << 3,
   /* differentiate the interface motion too: the VOF advection (its limiter and clamps taken one
    * sided at their kinks), the SVOF filter at each step's sweep count, the curvature and the CSF
    * force are transposed in the reverse sweep, so the gradient sees how the parameters move the
    * interface and, through the surface force, the flow (otherwise the interface is held frozen);
    * not with FUGU_ADJ_REUSE_INTERFACE */
-  FUGU_ADJ_MOVING_INTERFACE = 1u << 4
+  FUGU_ADJ_MOVING_INTERFACE = 1u << 4,
+  /* adjoint solves by the mixed form: FP32 Arnoldi cycles of the transposed operator and the
+   * transposed PCD (FP32 storage, FP64 arithmetic in the kernels), FP64 true residual and update
+   * at every restart, so every adjoint solve still meets opts->tol in FP64 */
+  FUGU_ADJ_MIXED_ADJOINT = 1u << 5
 };
 typedef struct {
   double J;
@@ -381,7 +385,8 @@ typedef struct {
 /* Gradient of J with respect to the per-cell viscosity and density (padded p-kind arrays)
  * and the surface tension, over a BDF2 horizon from (u0, v0, C0), properties held at the
  * per-cell fields, interface motion frozen in the adjoint (VOF, SVOF and CSF of the forward
- * pass recorded and held fixed) unless FUGU_ADJ_MOVING_INTERFACE, all solves FP64 to opts->tol.  g_mu, g_rho, g_sigma may be
+ * pass recorded and held fixed) unless FUGU_ADJ_MOVING_INTERFACE, every solve to opts->tol in FP64 (the mixed
+ * forms refine FP32 cycles to it).  g_mu, g_rho, g_sigma may be
  * NULL with FUGU_ADJ_FORWARD_ONLY; c0 may be NULL with FUGU_ADJ_REUSE_INTERFACE. */
 int fugu_adjoint_gradient(fugu_ctx* ctx, const fugu_adjoint_opts* opts, const double* mu_cell,
                           const double* rho_cell, double sigma, const double* c0, const double* u0,

@@ -158,10 +158,21 @@ def gpu_opts(case, flags=0, tol=1e-12):
     return abi.FuguAdjointOpts(case.nsteps, case.dt, case.y0, tol, 400, 30, 1, flags)
 
 
+MIXED_ADJ = abi.ADJ_MIXED_ADJOINT
+
+
 @pytest.mark.parametrize("per_x,time_order,flags", [(False, 2, 0), (True, 2, 0), (False, 1, 0),
-                                                    (False, 2, abi.ADJ_PRECOND_VMG), (True, 2, abi.ADJ_PRECOND_VMG)],
-                         ids=["walls_bdf2", "periodic_x_bdf2", "walls_bdf1", "walls_bdf2_vmg", "periodic_x_bdf2_vmg"])
+                                                    (False, 2, abi.ADJ_PRECOND_VMG), (True, 2, abi.ADJ_PRECOND_VMG),
+                                                    (False, 2, MIXED_ADJ), (True, 2, MIXED_ADJ),
+                                                    (True, 2, MIXED_ADJ | abi.ADJ_PRECOND_VMG),
+                                                    (False, 1, MIXED_ADJ | abi.ADJ_MIXED_FORWARD | abi.ADJ_PRECOND_VMG)],
+                         ids=["walls_bdf2", "periodic_x_bdf2", "walls_bdf1", "walls_bdf2_vmg", "periodic_x_bdf2_vmg",
+                              "walls_bdf2_mixed_adj", "periodic_x_bdf2_mixed_adj", "periodic_x_bdf2_vmg_mixed_adj",
+                              "walls_bdf1_vmg_mixed_both"])
 def test_adjoint_gradient_against_cpu_reference(per_x, time_order, flags):
+    """The GPU gradient against the CPU adjoint reference: FP64 solves, or (MIXED_ADJ) adjoint
+    solves by FP32 Arnoldi cycles of the transposed operator and PCD refined in FP64 to the same
+    1e-12 true residual -- with the Jacobi or the multigrid velocity blocks, walls or periodic."""
     case = make_case(per_x, time_order)
     J, g_mu, g_rho, g_sigma, _ = AR.gradient(case)
     ctx = gpu_ctx(case)
