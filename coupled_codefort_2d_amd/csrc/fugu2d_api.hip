@@ -70,6 +70,12 @@ struct MgLevel {
   T *cx = nullptr, *cy = nullptr, *m = nullptr, *dinv = nullptr, *b = nullptr, *phi = nullptr;
   T* cinv = nullptr;   // coarsest level of at most MG_DENSE_MAX cells: its dense inverse
   T* phi2 = nullptr;   // the fused post-smoothing's output (the level's result when it ran)
+  // a register-resident tail starting at this level: the quads' register images and the
+  // coarsest level's LDS image (k_mg_tail_q_pack), valid for tq_nl levels until the next set-up
+  TqQuad<T>* tq = nullptr;
+  T* tqc = nullptr;
+  bool tq_ok = false;
+  size_t tq_nl = 0;
 };
 
 // A multigrid hierarchy from the finest extents down (mass: allocate the mass terms).
@@ -691,10 +697,19 @@ std::vector<MgLevel<T>> mg_levels(fugu_ctx* c, MgDims d, bool mass, int max_leve
   return lv;
 }
 
-// Coarse operators and inverse diagonals below a level 0 whose c_x, c_y (and m) are set.
+template <typename T>
+size_t mg_tail_start(const fugu_ctx* c, const std::vector<MgLevel<T>>& lv, size_t nl);
+template <typename T>
+bool mg_tail_quad_ok(const fugu_ctx* c, const std::vector<MgLevel<T>>& lv, size_t l, size_t nl);
+template <typename T>
+void mg_tail_pack(fugu_ctx* c, std::vector<MgLevel<T>>& lv, size_t l, size_t nl);
+
+// Coarse operators and inverse diagonals below a level 0 whose c_x, c_y (and m) are set; then
+// the register images of a register-resident coarse tail.
 template <typename T>
 void mg_galerkin(fugu_ctx* c, std::vector<MgLevel<T>>& lv, size_t nl) {
   const dim3 bl(64, 4);
+  for (auto& L : lv) L.tq_ok = false;   // the tails' images are rebuilt from the new operators
   for (size_t l = 0; l + 1 < nl; ++l) {
     const MgDims& f = lv[l].d;
     const MgDims& C = lv[l + 1].d;
@@ -711,6 +726,8 @@ void mg_galerkin(fugu_ctx* c, std::vector<MgLevel<T>>& lv, size_t nl) {
   if (C.cinv)   // singular (pressure) block: no mass term
     LAUNCH((k_mg_coarse_inv<T>), dim3(1), dim3(1024), c->s, C.d, (const T*)C.cx, (const T*)C.cy, (const T*)C.m,
            C.m ? 0 : 1, C.cinv);
+  const size_t ts = mg_tail_start(c, lv, nl);
+  if (ts < nl && mg_tail_quad_ok(c, lv, ts, nl)) mg_tail_pack(c, lv, ts, nl);
 }
 
 template <typename T>
@@ -937,6 +954,62 @@ int mgs_post(fugu_ctx* c, const char* nm, double ab, const MgLevel<T>& L, int nu
 // guess (not zero_guess): the buffer holding the initial iterate (nullptr: phi).  The fused
 // kernels read it where it lies and write the other buffer; the in-place paths get it copied
 // into phi first.
+// The coarse tail's level arguments (levels l .. nl-1) with the LDS-pool offsets of the
+// staged general tail; nu, coarse_sweeps, project and zero_guess are the caller's.
+template <typename T>
+MgTailArgs<T> mg_tail_args(const fugu_ctx* c, const std::vector<MgLevel<T>>& lv, size_t l, size_t nl) {
+  MgTailArgs<T> ta{};
+  ta.nlev = (int)(nl - l);
+  int off = 0;
+  for (size_t q = l; q < nl; ++q) {
+    MgTailLevel<T> t{lv[q].d, lv[q].cx, lv[q].cy, lv[q].m, lv[q].dinv, lv[q].cinv, lv[q].b, lv[q].phi};
+    const int nx = lv[q].d.nx, ny = lv[q].d.ny, n = nx * ny;
+    t.ocx = off; off += (nx + 1) * ny;
+    t.ocy = off; off += nx * (ny + 1);
+    t.om = lv[q].m ? off : -1; off += lv[q].m ? n : 0;
+    t.odinv = off; off += n;
+    t.ob = off; off += n;
+    t.ophi = off; off += n;
+    ta.L[q - l] = t;
+  }
+  ta.blk = lv[l].cx;
+  ta.nblk = off;
+  return ta;
+}
+// The register-resident tail applies: even smoothing levels whose quads fit the workgroup, at
+// most MG_TAILQ_LEVELS of them, b and phi of every level in its LDS pool, the coarsest
+// operator's image in its LDS.
+template <typename T>
+bool mg_tail_quad_ok(const fugu_ctx* c, const std::vector<MgLevel<T>>& lv, size_t l, size_t nl) {
+  const int nsm = (int)(nl - l) - 1;
+  bool quad = c->mg_tail_quad && nsm >= 1 && nsm <= MG_TAILQ_LEVELS;
+  size_t qpool = 0;
+  for (size_t q = l; q < nl; ++q) {
+    const MgDims& d = lv[q].d;
+    qpool += 2 * (size_t)d.nx * d.ny;
+    if (q + 1 < nl) quad = quad && (d.nx % 2 == 0) && (d.ny % 2 == 0) && (d.nx / 2) * (d.ny / 2) <= MG_TAILQ_NTH;
+  }
+  return quad && qpool <= (size_t)MG_TAILQ_POOL &&
+         mg_tailq_coarse_elems(lv[nl - 1].d, lv[nl - 1].cinv != nullptr) <= (size_t)MG_TAILQ_COARSE;
+}
+// The register images of the tail starting at level l (k_mg_tail_q_pack), after a set-up of the
+// levels' operators.
+template <typename T>
+void mg_tail_pack(fugu_ctx* c, std::vector<MgLevel<T>>& lv, size_t l, size_t nl) {
+  MgLevel<T>& Lt = lv[l];
+  if (!Lt.tq) {
+    Lt.tq = dalloc<TqQuad<T>>(c, (size_t)MG_TAILQ_LEVELS * MG_TAILQ_NTH);
+    Lt.tqc = dalloc<T>(c, MG_TAILQ_COARSE);
+  }
+  const MgTailArgs<T> ta = mg_tail_args(c, lv, l, nl);
+  const int nsm = ta.nlev - 1;
+  if (nsm == 1) LAUNCH((k_mg_tail_q_pack<T, 1>), dim3(1), dim3(MG_TAILQ_NTH), c->s, ta, Lt.tq, Lt.tqc);
+  else if (nsm == 2) LAUNCH((k_mg_tail_q_pack<T, 2>), dim3(1), dim3(MG_TAILQ_NTH), c->s, ta, Lt.tq, Lt.tqc);
+  else LAUNCH((k_mg_tail_q_pack<T, 3>), dim3(1), dim3(MG_TAILQ_NTH), c->s, ta, Lt.tq, Lt.tqc);
+  Lt.tq_ok = true;
+  Lt.tq_nl = nl;
+}
+
 template <typename T>
 T* mg_vcycle(fugu_ctx* c, std::vector<MgLevel<T>>& lv, size_t l, bool zero_guess, int project, size_t nl,
              double* psum = nullptr, int* psum_nb = nullptr, const T* guess = nullptr) {
@@ -949,20 +1022,7 @@ T* mg_vcycle(fugu_ctx* c, std::vector<MgLevel<T>>& lv, size_t l, bool zero_guess
   };
   if (l == mg_tail_start(c, lv, nl)) {   // the coarse tail in one launch
     guess_to_phi();
-    MgTailArgs<T> ta{};
-    ta.nlev = (int)(nl - l);
-    int off = 0;
-    for (size_t q = l; q < nl; ++q) {
-      MgTailLevel<T> t{lv[q].d, lv[q].cx, lv[q].cy, lv[q].m, lv[q].dinv, lv[q].cinv, lv[q].b, lv[q].phi};
-      const int nx = lv[q].d.nx, ny = lv[q].d.ny, n = nx * ny;
-      t.ocx = off; off += (nx + 1) * ny;
-      t.ocy = off; off += nx * (ny + 1);
-      t.om = lv[q].m ? off : -1; off += lv[q].m ? n : 0;
-      t.odinv = off; off += n;
-      t.ob = off; off += n;
-      t.ophi = off; off += n;
-      ta.L[q - l] = t;
-    }
+    MgTailArgs<T> ta = mg_tail_args(c, lv, l, nl);
     // a cut velocity hierarchy's coarsest level gets the cut treatment's sweeps, whatever its size
     const int cnu = (c->vmg_coarse_nu == 3 || c->vmg_coarse_nu == 4) ? c->vmg_coarse_nu : 2;
     ta.nu = nu; ta.coarse_sweeps = (nl < lv.size() && lv[nl - 1].m) ? cnu : 60;
@@ -976,22 +1036,13 @@ T* mg_vcycle(fugu_ctx* c, std::vector<MgLevel<T>>& lv, size_t l, bool zero_guess
       packed = lv[q].cx == base + t.ocx && lv[q].cy == base + t.ocy && (t.om < 0 || lv[q].m == base + t.om) &&
                lv[q].dinv == base + t.odinv && lv[q].b == base + t.ob;
     }
-    ta.blk = lv[l].cx;
-    ta.nblk = off;
-    // the register-resident form: even smoothing levels whose quads fit the workgroup, at most
-    // MG_TAILQ_LEVELS of them, b and phi of every level in its LDS pool
+    const int off = ta.nblk;
     const int nsm = ta.nlev - 1;
-    bool quad = c->mg_tail_quad && nsm >= 1 && nsm <= MG_TAILQ_LEVELS;
-    size_t qpool = 0;
-    for (size_t q = l; q < nl; ++q) {
-      const MgDims& d = lv[q].d;
-      qpool += 2 * (size_t)d.nx * d.ny;
-      if (q + 1 < nl)
-        quad = quad && (d.nx % 2 == 0) && (d.ny % 2 == 0) && (d.nx / 2) * (d.ny / 2) <= MG_TAILQ_NTH;
-    }
-    quad = quad && qpool <= (size_t)MG_TAILQ_POOL &&
-           mg_tailq_coarse_elems(lv[nl - 1].d, lv[nl - 1].cinv != nullptr) <= (size_t)MG_TAILQ_COARSE;
-    if (quad) {
+    if (mg_tail_quad_ok(c, lv, l, nl)) {
+      MgLevel<T>& Lt = lv[l];
+      if (!Lt.tq_ok || Lt.tq_nl != nl) mg_tail_pack(c, lv, l, nl);   // (the set-up normally did it)
+      ta.tq = Lt.tq;
+      ta.tqc = Lt.tqc;
       if (nsm == 1) LAUNCH((k_mg_tail_q<T, 1>), dim3(1), dim3(MG_TAILQ_NTH), c->s, ta);
       else if (nsm == 2) LAUNCH((k_mg_tail_q<T, 2>), dim3(1), dim3(MG_TAILQ_NTH), c->s, ta);
       else LAUNCH((k_mg_tail_q<T, 3>), dim3(1), dim3(MG_TAILQ_NTH), c->s, ta);

@@ -589,6 +589,10 @@ struct MgTailArgs {
   const T* blk;   // STAGE: the pool's image in global memory (the top level's phi excepted)
   int nblk;       // its length in elements
   int nlev, nu, coarse_sweeps, project, zero_guess;
+  // register-resident tail: the quads' register images (TqQuad<T>, quad t of level l at
+  // l * MG_TAILQ_NTH + t) and the coarsest level's LDS image, written by k_mg_tail_q_pack
+  const void* tq;
+  const void* tqc;
 };
 
 // log2 of the lanes a tail phase puts along a row of w cells: the smallest power of two >= w, at
@@ -779,7 +783,7 @@ inline size_t mg_tailq_coarse_elems(const MgDims& d, bool dense) {
 }
 
 template <typename T>
-struct TqQuad {
+struct alignas(16) TqQuad {
   T cw[4], ce[4], cs[4], cn[4], dinv[4], diag[4];
   // the two neighbours of cell k outside the quad, LDS element indices: horizontal | vertical
   // << 16 (k = 0: W, S; 1: E, S; 2: W, N; 3: E, N)
@@ -787,6 +791,7 @@ struct TqQuad {
   // cells 1 and 2 (the colour the post-smoothing updates first): the coarse parents of those two
   // neighbours, packed the same way (the parent of a neighbour inside the quad is coarse cell t)
   unsigned pout[2];
+  unsigned own0;   // the LDS index of cell 0
 };
 
 __device__ inline unsigned tq_lo(unsigned p) { return p & 0xffffu; }
@@ -960,8 +965,62 @@ __device__ inline void tq_coarse(const MgDims& d, bool dense, const T* ci, const
   __syncthreads();
 }
 
+// The tail's LDS layout: level l's b at bo[l], its phi at bo[l] + n_l.
+template <int NL, typename T>
+__device__ inline void tq_layout(const MgTailArgs<T>& a, int (&bo)[NL + 1], int (&nq)[NL]) {
+  int off = 0;
+#pragma unroll
+  for (int l = 0; l <= NL; ++l) {
+    bo[l] = off;
+    off += 2 * a.L[l].d.nx * a.L[l].d.ny;
+  }
+#pragma unroll
+  for (int l = 0; l < NL; ++l) nq[l] = (a.L[l].d.nx >> 1) * (a.L[l].d.ny >> 1);
+}
+
+// The register images of every quad and the coarsest level's LDS image (the dense inverse
+// transposed: element p = q n + r holds inverse element (r, q); or c_x, c_y and D^-1 back to
+// back), from the level arrays: once per set-up of the levels, so that each V-cycle's tail
+// starts from plain 16-byte loads (its set-up's index arithmetic, wraps and packing were a
+// third of its time).  One workgroup of MG_TAILQ_NTH threads.
+template <typename T, int NL>
+__global__ void __launch_bounds__(MG_TAILQ_NTH) k_mg_tail_q_pack(MgTailArgs<T> a, TqQuad<T>* __restrict__ qout,
+                                                               T* __restrict__ cimg) {
+  constexpr int NTH = MG_TAILQ_NTH;
+  const int t = (int)threadIdx.x;
+  int bo[NL + 1], nq[NL];
+  tq_layout<NL>(a, bo, nq);
+  auto PHI = [&](int l) { return (unsigned)(bo[l] + a.L[l].d.nx * a.L[l].d.ny); };
+#pragma unroll
+  for (int l = 0; l < NL; ++l) {
+    if (t >= nq[l]) continue;
+    TqQuad<T> q;
+    T mm[4];
+    int I, J;
+    tq_quad(t, a.L[l].d.nx >> 1, I, J);
+    tq_load<T>(a.L[l], I, J, PHI(l), PHI(l + 1), q, mm);
+    tq_finish<T>(a.L[l].d, I, J, q, mm, a.L[l].m != nullptr);
+    q.own0 = PHI(l) + (unsigned)(2 * I + 2 * J * a.L[l].d.nx);
+    qout[l * NTH + t] = q;
+  }
+  const MgDims dc = a.L[NL].d;
+  const int ncc = dc.nx * dc.ny;
+  const int ncx = (dc.nx + 1) * dc.ny, ncy = dc.nx * (dc.ny + 1);
+  if (a.L[NL].cinv) {
+    for (int p = t; p < ncc * ncc; p += NTH) {
+      int r, qq;
+      tq_quad(p, ncc, r, qq);   // p = r + qq n
+      cimg[p] = a.L[NL].cinv[r * ncc + qq];
+    }
+  } else {
+    for (int e = t; e < ncx + ncy + ncc; e += NTH)
+      cimg[e] = e < ncx ? a.L[NL].cx[e] : (e < ncx + ncy ? a.L[NL].cy[e - ncx] : a.L[NL].dinv[e - ncx - ncy]);
+  }
+}
+
 // NL smoothing levels a.L[0..NL-1] and the coarsest a.L[NL]; LDS: level l's b at bo[l], its phi
-// at bo[l] + n_l; the coarsest level's operator in sc.
+// at bo[l] + n_l; the coarsest level's operator in sc.  The quads' registers and the coarsest
+// operator come from k_mg_tail_q_pack's images (a.tq, a.tqc).
 template <typename T, int NL>
 __global__ void __launch_bounds__(MG_TAILQ_NTH) k_mg_tail_q(MgTailArgs<T> a) {
   __shared__ T s[MG_TAILQ_POOL];
@@ -973,22 +1032,12 @@ __global__ void __launch_bounds__(MG_TAILQ_NTH) k_mg_tail_q(MgTailArgs<T> a) {
   constexpr int KC = (MG_TAILQ_COARSE + NTH - 1) / NTH;
   const int t = (int)threadIdx.x;
   int bo[NL + 1], nq[NL];
-  {
-    int off = 0;
-#pragma unroll
-    for (int l = 0; l <= NL; ++l) {
-      bo[l] = off;
-      off += 2 * a.L[l].d.nx * a.L[l].d.ny;
-    }
-#pragma unroll
-    for (int l = 0; l < NL; ++l) nq[l] = (a.L[l].d.nx >> 1) * (a.L[l].d.ny >> 1);
-  }
+  tq_layout<NL>(a, bo, nq);
   auto PHI = [&](int l) { return (unsigned)(bo[l] + a.L[l].d.nx * a.L[l].d.ny); };
   // every load in flight: the top level's b and phi (zero for a zero guess; finite values are
-  // needed even then, the wall terms read them times a zero conductance) and the quads'
-  // coefficients
-  // (indices clamped into the arrays instead of branches around the loads: the set-up's cost is
-  // its instruction count; values loaded for no cell are never used)
+  // needed even then, the wall terms read them times a zero conductance), the coarsest
+  // operator's image and the quads' registers (indices clamped into the arrays instead of
+  // branches around the loads; values loaded for no cell are never used)
   const MgDims d0 = a.L[0].d;
   const int n0 = d0.nx * d0.ny;
   T vb[4], vp[4];
@@ -998,43 +1047,19 @@ __global__ void __launch_bounds__(MG_TAILQ_NTH) k_mg_tail_q(MgTailArgs<T> a) {
     vb[r] = mgt_ld(a.L[0].b, e);
     vp[r] = a.zero_guess ? T(0) : mgt_ld((const T*)a.L[0].phi, e);
   }
-  // the coarsest level's operator: the dense inverse, transposed (LDS element p = q n + r holds
-  // inverse element (r, q)), or c_x, c_y and D^-1 back to back
   const MgDims dc = a.L[NL].d;
   const int ncc = dc.nx * dc.ny;
-  const bool dense = a.L[NL].cinv != nullptr;
-  const int nco = dense ? ncc * ncc : (dc.nx + 1) * dc.ny + dc.nx * (dc.ny + 1) + ncc;
-  const int ncx = (dc.nx + 1) * dc.ny, ncy = dc.nx * (dc.ny + 1);
+  const int nco = a.L[NL].cinv ? ncc * ncc : (dc.nx + 1) * dc.ny + dc.nx * (dc.ny + 1) + ncc;
+  const T* cimg = (const T*)a.tqc;
   T vc[KC];
-  if (dense) {
 #pragma unroll
-    for (int k = 0; k < KC; ++k) {
-      int r, qq;
-      tq_quad(min(t + NTH * k, nco - 1), ncc, r, qq);   // p = r + qq n
-      vc[k] = mgt_ld(a.L[NL].cinv, (unsigned)(r * ncc + qq));
-    }
-  } else {
-#pragma unroll
-    for (int k = 0; k < KC; ++k) {
-      const int e = t + NTH * k;
-      vc[k] = T(0);
-      if (e < ncx) vc[k] = mgt_ld(a.L[NL].cx, (unsigned)e);
-      else if (e < ncx + ncy) vc[k] = mgt_ld(a.L[NL].cy, (unsigned)(e - ncx));
-      else if (e < nco) vc[k] = mgt_ld(a.L[NL].dinv, (unsigned)(e - ncx - ncy));
-    }
-  }
-  TqQuad<T> q[NL];
-  T mm[NL][4];
-  int qI[NL], qJ[NL];
+  for (int k = 0; k < KC; ++k) vc[k] = mgt_ld(cimg, (unsigned)min(t + NTH * k, nco - 1));
   // (the quads of the lower levels occupy the first wavefronts only: the others skip their loads)
-#pragma unroll
-  for (int l = 0; l < NL; ++l) {
-    tq_quad(t, a.L[l].d.nx >> 1, qI[l], qJ[l]);
-    if (t < nq[l]) tq_load<T>(a.L[l], qI[l], qJ[l], PHI(l), PHI(l + 1), q[l], mm[l]);
-  }
+  const TqQuad<T>* qin = (const TqQuad<T>*)a.tq;
+  TqQuad<T> q[NL];
 #pragma unroll
   for (int l = 0; l < NL; ++l)
-    if (t < nq[l]) tq_finish<T>(a.L[l].d, qI[l], qJ[l], q[l], mm[l], a.L[l].m != nullptr);
+    if (t < nq[l]) q[l] = qin[l * NTH + t];
 #pragma unroll
   for (int r = 0; r < 4; ++r) {
     const int e = t + NTH * r;
@@ -1048,7 +1073,7 @@ __global__ void __launch_bounds__(MG_TAILQ_NTH) k_mg_tail_q(MgTailArgs<T> a) {
   __syncthreads();
   unsigned own0[NL];
 #pragma unroll
-  for (int l = 0; l < NL; ++l) own0[l] = PHI(l) + (unsigned)(2 * qI[l] + 2 * qJ[l] * a.L[l].d.nx);
+  for (int l = 0; l < NL; ++l) own0[l] = q[l].own0;
   const int nu = a.nu;
   // descent: pre-smoothing, residuals summed onto the coarse cells
 #pragma unroll
@@ -1062,7 +1087,7 @@ __global__ void __launch_bounds__(MG_TAILQ_NTH) k_mg_tail_q(MgTailArgs<T> a) {
     }
     tq_restrict<T>(q[l], act, own0[l], nx, db, (unsigned)(bo[l + 1] + t), PHI(l + 1) + (unsigned)t, s);
   }
-  tq_coarse(dc, dense, (const T*)sc, (const T*)(s + bo[NL]), s + PHI(NL), a.coarse_sweeps, a.project, sphi, sb, red);
+  tq_coarse(dc, a.L[NL].cinv != nullptr, (const T*)sc, (const T*)(s + bo[NL]), s + PHI(NL), a.coarse_sweeps, a.project, sphi, sb, red);
   // ascent: post-smoothing, the prolongation folded into its first half-sweep
 #pragma unroll
   for (int l = NL - 1; l >= 0; --l) {
