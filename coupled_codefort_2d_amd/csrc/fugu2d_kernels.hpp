@@ -1760,13 +1760,21 @@ __global__ void __launch_bounds__(SVOF_NT) k_svof_block(Grid g, const double* __
     if (present & (1u << r)) LB[0][threadIdx.x + SVOF_NT * r] = nw[r];
   __syncthreads();
   __shared__ double wmax[SVOF_S][SVOF_NT / 64];   // per iteration and wavefront: max |new - old| over owned cells
+  // rows of the extended tile a wavefront's 64 cells of slot r span (uniform per wavefront)
+  const int wq0 = (int)(threadIdx.x & ~63u);
   for (int t = 0; t < cnt; ++t) {
     const double* L = LB[t & 1];
     double* Lo = LB[(t + 1) & 1];
     double mt = 0.0;
+    // sweep t + 1 only matters on the rows within SVOF_S - t - 1 of the owned ones: rows
+    // t + 1 .. EH - t - 2 (the next sweep reads one row further in, the owned rows last); a
+    // wavefront whose cells all lie outside skips (its cells are never read again)
+    const int rlo = t + 1, rhi = SVOF_EH - t - 2;
 #pragma unroll
     for (int r = 0; r < SVOF_CPT; ++r) {
       const unsigned bit = 1u << r;
+      const int b0 = (wq0 + SVOF_NT * r) / SVOF_EW, b1 = (wq0 + SVOF_NT * r + 63) / SVOF_EW;
+      if (b1 < rlo || b0 > rhi) continue;
       if (!(present & bit)) continue;
       const int q = (int)threadIdx.x + SVOF_NT * r;
       const double e = L[(clamp_e & bit) ? q : q + 1], w = L[(clamp_w & bit) ? q : q - 1];
