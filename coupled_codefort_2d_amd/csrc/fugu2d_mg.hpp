@@ -765,7 +765,9 @@ __global__ void __launch_bounds__(NTH) k_mg_tail(MgTailArgs<T> a) {
 // indices of the neighbours outside the quad (those inside are at fixed offsets).  An update
 // is then 5 LDS reads, the stencil and a write.  Owning all four children of its coarse cell,
 // a thread forms their residuals and sums them onto the coarse cell in one phase (no residual
-// array, no second barrier).  Only b and phi of the levels live in LDS.
+// array, no second barrier).  Only b and phi of the levels and the coarsest level's operator
+// live in LDS.  The registers' images are built once per set-up of the levels
+// (k_mg_tail_q_pack) and read back with plain loads at each V-cycle.
 // The arithmetic is the general tail's, operand for operand: the neighbour sum
 // cW pW + cE pE + cS pS + cN pN with clamped (wall) or wrapped (periodic) neighbour indices and
 // masked wall conductances, (b + s) D^-1, the residual b - (diag phi - s) summed over the
