@@ -405,18 +405,10 @@ inline unsigned nblk(size_t n) {
   return (unsigned)b;
 }
 
-void set_slot(fugu_ctx* c, int slot, double v) {
-  HIPCHK(hipMemcpyAsync(c->sc + slot, &v, sizeof(double), hipMemcpyHostToDevice, c->s));
-  HIPCHK(hipStreamSynchronize(c->s));  // v lives on the host stack
-}
+// (one-thread kernels: no host-memory copy, no stream synchronisation)
+void set_slot(fugu_ctx* c, int slot, double v) { LAUNCH(k_set_slot, dim3(1), dim3(64), c->s, c->sc, slot, v); }
 void reset_flags(fugu_ctx* c, bool relative_guards = false) {
-  HIPCHK(hipMemsetAsync(c->fl, 0, F_COUNT * sizeof(int), c->s));
-  if (relative_guards) {
-    static const int one = 1;
-    HIPCHK(hipMemcpyAsync(c->fl + F_RELG, &one, sizeof(int), hipMemcpyHostToDevice, c->s));
-  }
-  static const double unit = 1.0;
-  HIPCHK(hipMemcpyAsync(c->sc + S_GSCALE, &unit, sizeof(double), hipMemcpyHostToDevice, c->s));
+  LAUNCH(k_reset_flags, dim3(1), dim3(64), c->s, c->fl, relative_guards ? 1 : 0, c->sc);
 }
 void read_flags(fugu_ctx* c) {
   HIPCHK(hipMemcpyAsync(c->h_fl, c->fl, F_COUNT * sizeof(int), hipMemcpyDeviceToHost, c->s));

@@ -1110,6 +1110,17 @@ __device__ inline Nbr cell_nbrs(const Grid& g, int i, int j) {
   return r;
 }
 
+// Host-set scalars without host memory or a stream synchronisation: the value travels as a
+// kernel argument.  sc[slot] = v.
+__global__ void k_set_slot(double* __restrict__ sc, int slot, double v) {
+  if (threadIdx.x == 0) sc[slot] = v;
+}
+// The flags cleared (F_RELG = relg) and sc[S_GSCALE] = 1, as before every Krylov solve.
+__global__ void k_reset_flags(int* __restrict__ fl, int relg, double* __restrict__ sc) {
+  for (int i = (int)threadIdx.x; i < (int)F_COUNT; i += (int)blockDim.x) fl[i] = (i == (int)F_RELG) ? relg : 0;
+  if (threadIdx.x == 0) sc[S_GSCALE] = 1.0;
+}
+
 // Sum of rhs over the interior (for the zero-mean projection, :91-99) -> S_P_MEAN.
 // Interior sum of a cell field (zero-mean projection, :91-99) -> sc[slot] = sum / (nx ny).
 // skip_mode: 0 never, 1 when a Krylov exit is pending (F_BRK), 2 when the PCG did not
