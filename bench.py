@@ -527,7 +527,8 @@ def main():
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="first run a plain FP64 copy of this size (counter calibration)")
     ap.add_argument("--inner-tol", type=float, default=1e-3)
-    ap.add_argument("--mg-cycles", type=int, default=1)
+    ap.add_argument("--mg-cycles", type=int, default=0,
+                    help="pressure V-cycles per PCD; 0: the library's default (2 from 2^22 cells, else 1)")
     ap.add_argument("--inner-max", type=int, default=-1, help="inner BiCGStab(2) cap; -1: PCD only")
     ap.add_argument("--restart", type=int, default=0,
                     help="FGMRES restart length; 0: the library's default (10 after a solve of 16-30 steps, "

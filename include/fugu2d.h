@@ -56,7 +56,8 @@ typedef struct {
   double inner_tol;         /* refined solve: inner Krylov tolerance (<= 0: 1e-3) */
   int inner_max_iter;       /* refined solve: inner Krylov iteration cap (0: 25; < 0: no inner Krylov,
                                the PCD alone preconditions the outer FGMRES) */
-  int mg_cycles;            /* FUGU_PRECOND_MG: V-cycles per preconditioner application (<= 0: 1) */
+  int mg_cycles;            /* FUGU_PRECOND_MG: V-cycles per preconditioner application (<= 0: 2 from
+                               2^22 cells, else 1) */
   int restart;              /* refined solve: outer FGMRES restart length (<= 0: 10 after a refined
                                solve of 16-30 FGMRES steps on the context, else 20 up to 2^21
                                cells and 30 above; max 47) */
